@@ -1,0 +1,521 @@
+// capi.hip -- extern "C" entry points of libxsmd.so (declared in include/xsmd.h).
+// Every entry point converts internal errors to an xs_status and records the
+// message on the context; nothing aborts.
+#include "xsmd_internal.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+
+namespace xsmd {
+
+Ctx::~Ctx()
+{
+  if (hscalar) (void)hipHostFree(hscalar);
+  if (stream) (void)hipStreamDestroy(stream);
+}
+
+namespace {
+
+template <class F>
+xs_status guarded(xs_ctx* h, F&& f)
+{
+  Ctx* c = reinterpret_cast<Ctx*>(h);
+  if (!c) return XS_E_ARG;
+  try {
+    XS_HIP(hipSetDevice(c->device));
+    f(*c);
+    c->err.clear();
+    return XS_OK;
+  } catch (const Error& e) {
+    c->err = e.what();
+    return e.code;
+  } catch (const std::exception& e) {
+    c->err = e.what();
+    return XS_E_STATE;
+  }
+}
+
+void require(bool ok, xs_status code, const char* msg)
+{
+  if (!ok) throw Error(code, msg);
+}
+
+void require_system(Ctx& c) { require(c.n > 0 || c.box_h.size() > 0, XS_E_STATE, "no system loaded"); }
+void require_list(Ctx& c) { require(c.list_valid, XS_E_STATE, "neighbour list not built (call xs_chunk_neighbors)"); }
+
+template <class T>
+void to_device(Ctx& c, DBuf<T>& d, const T* h, int64_t n)
+{
+  XS_HIP(hipMemcpyAsync(d.p, h, sizeof(T) * n, hipMemcpyHostToDevice, c.stream));
+}
+
+template <class T>
+std::vector<T> from_device(Ctx& c, const DBuf<T>& d, int64_t n)
+{
+  std::vector<T> h(n);
+  if (n) XS_HIP(hipMemcpyAsync(h.data(), d.p, sizeof(T) * n, hipMemcpyDeviceToHost, c.stream));
+  XS_HIP(hipStreamSynchronize(c.stream));
+  return h;
+}
+
+// gather a device array in cell order back to id order
+void to_id_order(Ctx& c, const DBuf<double>& d, double* out, int64_t stride = 0)
+{
+  if (!out) return;
+  const std::vector<int64_t> id = from_device(c, c.id, c.n);
+  if (stride == 0) {
+    const std::vector<double> v = from_device(c, d, c.n);
+    for (int64_t k = 0; k < c.n; k++) out[id[k]] = v[k];
+  } else {
+    // 9 components stored as vir[q*npad + k]
+    const std::vector<double> v = from_device(c, d, 9 * c.npad);
+    for (int64_t k = 0; k < c.n; k++)
+      for (int q = 0; q < 9; q++) out[9 * id[k] + q] = v[q * c.npad + k];
+  }
+}
+
+void allocate_atoms(Ctx& c, int64_t n)
+{
+  c.npad = std::max<int64_t>(64, (n + 63) / 64 * 64);
+  const int64_t np = c.npad;
+  for (DBuf<double>* b : {&c.rx, &c.ry, &c.rz, &c.vx, &c.vy, &c.vz, &c.ax, &c.ay, &c.az, &c.rx2, &c.ry2,
+                          &c.rz2, &c.vx2, &c.vy2, &c.vz2, &c.ax2, &c.ay2, &c.az2, &c.rrx, &c.rry, &c.rrz,
+                          &c.ep, &c.rho})
+    b->reserve(np);
+  c.vir.reserve(9 * np);
+  c.fpf.reserve(np);
+  c.type.reserve(np);
+  c.type2.reserve(np);
+  c.id.reserve(np);
+  c.id2.reserve(np);
+  c.keys.reserve(np);
+  c.keys2.reserve(np);
+  c.vals.reserve(np);
+  c.vals2.reserve(np);
+  if (c.has_rep) {
+    c.rep.reserve(np);
+    c.rep2.reserve(np);
+  }
+}
+
+void set_lj(Ctx& c, double eps, double sigma, double rcut)
+{
+  require(eps >= 0 && sigma > 0 && rcut > 0, XS_E_ARG, "lj: need epsilon >= 0, sigma > 0, rcut > 0");
+  c.has_lj = true;
+  c.lj_eps = eps;
+  c.lj_sigma = sigma;
+  c.lj_rcut = rcut;
+}
+
+void set_eam(Ctx& c, int nel, int nrho, double drho, int nr, double dr, double rcut, const double* frho,
+             const double* rhor, const double* z2r)
+{
+  require(nel >= 1 && nel <= 7, XS_E_ARG, "eam: 1..7 elements");
+  require(nrho >= 5 && nr >= 5 && drho > 0 && dr > 0 && rcut > 0, XS_E_ARG, "eam: bad table grid");
+  require(frho && rhor && z2r, XS_E_ARG, "eam: null table");
+  require(c.nspecies == 0 || c.nspecies <= nel, XS_E_ARG, "eam: more species than table elements");
+  EamHost& e = c.eam;
+  e.nel = nel;
+  e.nrho = nrho;
+  e.nr = nr;
+  e.drho = drho;
+  e.dr = dr;
+  e.rc = rcut;
+  e.rdr = 1.0 / dr;
+  e.rdrho = 1.0 / drho;
+  e.rhomax = (nrho - 1) * drho;
+  const int nz = nel * (nel + 1) / 2;
+  e.frho_knots.assign(frho, frho + (size_t)nel * nrho);
+  e.rhor_knots.assign(rhor, rhor + (size_t)nel * nr);
+  e.z2r_knots.assign(z2r, z2r + (size_t)nz * nr);
+  eam_upload(c);
+  c.has_eam = true;
+  c.rho_state = RhoState::None;
+}
+
+void compute_forces(Ctx& c, int potential, double rcut, unsigned flags, const ForceEpilogue& epi)
+{
+  if (potential == XS_POT_LJ) {
+    require(c.has_lj, XS_E_STATE, "lj parameters not set");
+    launch_lj(c, c.lj_eps, c.lj_sigma, rcut, flags, epi);
+  } else {
+    require(c.has_eam, XS_E_STATE, "eam tables not set");
+    launch_eam(c, rcut, true, true, true, flags, epi);
+  }
+}
+
+}  // namespace
+}  // namespace xsmd
+
+using namespace xsmd;
+
+extern "C" {
+
+int xs_version(void) { return 1; }
+
+xs_status xs_ctx_create(int device, xs_ctx** out)
+{
+  if (!out) return XS_E_ARG;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) return XS_E_HIP;
+  Ctx* c = new (std::nothrow) Ctx();
+  if (!c) return XS_E_ALLOC;
+  c->device = device;
+  try {
+    XS_HIP(hipSetDevice(device));
+    XS_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    XS_HIP(hipHostMalloc((void**)&c->hscalar, 4 * sizeof(unsigned long long), hipHostMallocDefault));
+    c->dscalar.reserve(4);
+    c->mass_d.reserve(8);
+    c->box_d.reserve(1);
+  } catch (const Error&) {
+    delete c;
+    return XS_E_HIP;
+  }
+  *out = reinterpret_cast<xs_ctx*>(c);
+  return XS_OK;
+}
+
+xs_status xs_ctx_destroy(xs_ctx* h)
+{
+  Ctx* c = reinterpret_cast<Ctx*>(h);
+  if (!c) return XS_E_ARG;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  delete c;
+  return XS_OK;
+}
+
+const char* xs_last_error(const xs_ctx* h)
+{
+  const Ctx* c = reinterpret_cast<const Ctx*>(h);
+  return c ? c->err.c_str() : "null context";
+}
+
+xs_status xs_synchronize(xs_ctx* h)
+{
+  return guarded(h, [&](Ctx& c) { XS_HIP(hipStreamSynchronize(c.stream)); });
+}
+
+xs_status xs_species_set(xs_ctx* h, int nspecies, const double* masses)
+{
+  return guarded(h, [&](Ctx& c) {
+    require(nspecies >= 1 && nspecies <= 7 && masses, XS_E_ARG, "species: 1..7 masses required");
+    double m[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int s = 0; s < nspecies; s++) {
+      require(masses[s] > 0, XS_E_ARG, "species: masses must be positive");
+      m[s] = masses[s];
+    }
+    for (int s = nspecies; s < 8; s++) m[s] = 1.0;  // unused slots stay finite
+    std::memcpy(c.mass, m, sizeof(m));
+    c.nspecies = nspecies;
+    XS_HIP(hipMemcpyAsync(c.mass_d.p, c.mass, sizeof(c.mass), hipMemcpyHostToDevice, c.stream));
+    XS_HIP(hipStreamSynchronize(c.stream));
+  });
+}
+
+xs_status xs_system_set(xs_ctx* h, int64_t n, const double* rx, const double* ry, const double* rz,
+                        const double* vx, const double* vy, const double* vz, const uint8_t* type,
+                        const uint16_t* rep, int nrep, const double* lo, const double* hi,
+                        const int periodic[3])
+{
+  return guarded(h, [&](Ctx& c) {
+    require(c.nspecies > 0, XS_E_STATE, "set species masses before the system");
+    require(n >= 0 && n < XSMD_NBR_MAX_ATOMS, XS_E_LIMIT, "atom count must be below 2^26");
+    require(nrep >= 1 && nrep <= 65535 && lo && hi && periodic, XS_E_ARG, "bad box arguments");
+    require(nrep == 1 || rep, XS_E_ARG, "rep array required when nrep > 1");
+    require(n == 0 || (rx && ry && rz && vx && vy && vz && type), XS_E_ARG, "null atom array");
+    for (int64_t i = 0; i < n; i++) {
+      require(type[i] < c.nspecies, XS_E_ARG, "atom type out of range of the species table");
+      if (rep) require(rep[i] < nrep, XS_E_ARG, "atom replica out of range");
+    }
+    c.box_h.assign(nrep, BoxDev{});
+    for (int r = 0; r < nrep; r++)
+      for (int a = 0; a < 3; a++) {
+        const double L = hi[3 * r + a] - lo[3 * r + a];
+        require(L > 0 && std::isfinite(L), XS_E_ARG, "box must have positive extent");
+        c.box_h[r].lo[a] = lo[3 * r + a];
+        c.box_h[r].L[a] = L;
+        c.box_h[r].inv_cs[a] = 1.0 / L;
+      }
+    for (int a = 0; a < 3; a++) c.periodic[a] = periodic[a] ? 1 : 0;
+    c.box_d.reserve(nrep);
+    XS_HIP(hipMemcpyAsync(c.box_d.p, c.box_h.data(), sizeof(BoxDev) * nrep, hipMemcpyHostToDevice, c.stream));
+    c.nrep = nrep;
+    c.has_rep = nrep > 1;
+    c.n = n;
+    allocate_atoms(c, n);
+    c.list_valid = false;
+    c.rho_state = RhoState::None;
+    if (n > 0) {
+      to_device(c, c.rx, rx, n);
+      to_device(c, c.ry, ry, n);
+      to_device(c, c.rz, rz, n);
+      to_device(c, c.vx, vx, n);
+      to_device(c, c.vy, vy, n);
+      to_device(c, c.vz, vz, n);
+      to_device(c, c.type, type, n);
+      std::vector<int64_t> id(n);
+      for (int64_t i = 0; i < n; i++) id[i] = i;
+      to_device(c, c.id, id.data(), n);
+      if (c.has_rep) to_device(c, c.rep, rep, n);
+      XS_HIP(hipMemcpyAsync(c.rrx.p, c.rx.p, n * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+      XS_HIP(hipMemcpyAsync(c.rry.p, c.ry.p, n * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+      XS_HIP(hipMemcpyAsync(c.rrz.p, c.rz.p, n * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+      launch_zero(c);
+    }
+    XS_HIP(hipStreamSynchronize(c.stream));
+  });
+}
+
+int64_t xs_system_size(const xs_ctx* h)
+{
+  const Ctx* c = reinterpret_cast<const Ctx*>(h);
+  return c ? c->n : -1;
+}
+
+xs_status xs_system_get(xs_ctx* h, double* rx, double* ry, double* rz, double* vx, double* vy, double* vz)
+{
+  return guarded(h, [&](Ctx& c) {
+    to_id_order(c, c.rx, rx);
+    to_id_order(c, c.ry, ry);
+    to_id_order(c, c.rz, rz);
+    to_id_order(c, c.vx, vx);
+    to_id_order(c, c.vy, vy);
+    to_id_order(c, c.vz, vz);
+  });
+}
+
+xs_status xs_velocity_set(xs_ctx* h, const double* vx, const double* vy, const double* vz)
+{
+  return guarded(h, [&](Ctx& c) {
+    require(vx && vy && vz, XS_E_ARG, "null velocity array");
+    const std::vector<int64_t> id = from_device(c, c.id, c.n);
+    std::vector<double> t(c.n);
+    for (auto [src, dst] : {std::pair<const double*, DBuf<double>*>{vx, &c.vx}, {vy, &c.vy}, {vz, &c.vz}}) {
+      for (int64_t k = 0; k < c.n; k++) t[k] = src[id[k]];
+      to_device(c, *dst, t.data(), c.n);
+      XS_HIP(hipStreamSynchronize(c.stream));
+    }
+  });
+}
+
+xs_status xs_field_get(xs_ctx* h, int field, double* out)
+{
+  return guarded(h, [&](Ctx& c) {
+    require(out != nullptr, XS_E_ARG, "null output");
+    switch (field) {
+      case XS_FIELD_EP: to_id_order(c, c.ep, out); break;
+      case XS_FIELD_VIRIAL: to_id_order(c, c.vir, out, 9); break;
+      case XS_FIELD_RHO_DEMB:
+        if (c.rho_state == RhoState::FpFloat) {
+          const std::vector<int64_t> id = from_device(c, c.id, c.n);
+          const std::vector<float> v = from_device(c, c.fpf, c.n);
+          for (int64_t k = 0; k < c.n; k++) out[id[k]] = v[k];
+        } else {
+          to_id_order(c, c.rho, out);
+        }
+        break;
+      case XS_FIELD_ACCEL:
+        to_id_order(c, c.ax, out);
+        to_id_order(c, c.ay, out + c.n);
+        to_id_order(c, c.az, out + 2 * c.n);
+        break;
+      default: throw Error(XS_E_ARG, "unknown field");
+    }
+  });
+}
+
+xs_status xs_chunk_neighbors(xs_ctx* h, double rcut_max, double rcut_inc, int chunk_size)
+{
+  return guarded(h, [&](Ctx& c) {
+    require(chunk_size >= 1 && (chunk_size & (chunk_size - 1)) == 0, XS_E_ARG,
+            "chunk_size must be a power of two");
+    require(rcut_max > 0 && rcut_inc >= 0, XS_E_ARG, "need rcut_max > 0 and rcut_inc >= 0");
+    require(!c.box_h.empty(), XS_E_STATE, "no system loaded (call xs_system_set)");
+    build_neighbor_list(c, rcut_max, rcut_inc);
+  });
+}
+
+xs_status xs_particle_displ_over(xs_ctx* h, double threshold, int* result)
+{
+  return guarded(h, [&](Ctx& c) {
+    require(result != nullptr, XS_E_ARG, "null result");
+    require_list(c);
+    *result = max_displacement2(c) > threshold * threshold ? 1 : 0;
+  });
+}
+
+xs_status xs_neighbors_get(xs_ctx* h, int64_t* counts, int64_t* total, int64_t* ids, int32_t* imgs)
+{
+  return guarded(h, [&](Ctx& c) {
+    require_list(c);
+    require(counts && total, XS_E_ARG, "null counts/total");
+    const std::vector<int64_t> id = from_device(c, c.id, c.n);
+    const std::vector<int> cnt = from_device(c, c.nnb, c.n);
+    int64_t tot = 0;
+    for (int64_t k = 0; k < c.n; k++) {
+      counts[id[k]] = cnt[k];
+      tot += cnt[k];
+    }
+    *total = tot;
+    if (!ids && !imgs) return;
+    const std::vector<uint32_t> nb = from_device(c, c.nbr, (int64_t)c.nmax * c.npad);
+    std::vector<int64_t> off(c.n + 1, 0);
+    for (int64_t i = 0; i < c.n; i++) off[i + 1] = off[i] + counts[i];
+    for (int64_t k = 0; k < c.n; k++) {
+      int64_t o = off[id[k]];
+      for (int q = 0; q < cnt[k]; q++, o++) {
+        const uint32_t e = nb[(size_t)q * c.npad + k];
+        if (ids) ids[o] = id[xsmd_nbr_index(e)];
+        if (imgs) {
+          imgs[3 * o] = xsmd_nbr_img_x(e);
+          imgs[3 * o + 1] = xsmd_nbr_img_y(e);
+          imgs[3 * o + 2] = xsmd_nbr_img_z(e);
+        }
+      }
+    }
+  });
+}
+
+xs_status xs_order_get(xs_ctx* h, int64_t* perm)
+{
+  return guarded(h, [&](Ctx& c) {
+    require(perm != nullptr, XS_E_ARG, "null perm");
+    const std::vector<int64_t> id = from_device(c, c.id, c.n);
+    std::copy(id.begin(), id.end(), perm);
+  });
+}
+
+xs_status xs_zero_force_energy(xs_ctx* h)
+{
+  return guarded(h, [&](Ctx& c) { launch_zero(c); });
+}
+
+xs_status xs_force_to_accel(xs_ctx* h)
+{
+  return guarded(h, [&](Ctx& c) { launch_force_to_accel(c); });
+}
+
+xs_status xs_lj_compute_force(xs_ctx* h, double epsilon, double sigma, double rcut, unsigned flags)
+{
+  return guarded(h, [&](Ctx& c) {
+    set_lj(c, epsilon, sigma, rcut);
+    require_list(c);
+    require(rcut <= c.list_rcut + c.list_skin, XS_E_ARG, "rcut exceeds the neighbour-list radius");
+    launch_lj(c, epsilon, sigma, rcut, flags, ForceEpilogue{});
+  });
+}
+
+xs_status xs_lj_parameters(xs_ctx* h, double epsilon, double sigma, double rcut)
+{
+  return guarded(h, [&](Ctx& c) { set_lj(c, epsilon, sigma, rcut); });
+}
+
+xs_status xs_eam_alloy_tables(xs_ctx* h, int nel, int nrho, double drho, int nr, double dr, double rcut,
+                              const double* frho, const double* rhor, const double* z2r)
+{
+  return guarded(h, [&](Ctx& c) { set_eam(c, nel, nrho, drho, nr, dr, rcut, frho, rhor, z2r); });
+}
+
+xs_status xs_eam_alloy_load(xs_ctx* h, const char* path, double* masses, int* nel_out)
+{
+  return guarded(h, [&](Ctx& c) {
+    require(path != nullptr, XS_E_ARG, "null path");
+    int nel, nrho, nr;
+    double drho, dr, rc;
+    std::vector<double> f, r, z, m;
+    read_setfl(path, nel, nrho, drho, nr, dr, rc, f, r, z, m);
+    set_eam(c, nel, nrho, drho, nr, dr, rc, f.data(), r.data(), z.data());
+    if (masses) std::copy(m.begin(), m.end(), masses);
+    if (nel_out) *nel_out = nel;
+  });
+}
+
+xs_status xs_eam_alloy_force(xs_ctx* h, double rcut, int eam_rho, int eam_rho2emb, int eam_force, unsigned flags)
+{
+  return guarded(h, [&](Ctx& c) {
+    require(c.has_eam, XS_E_STATE, "eam tables not set (xs_eam_alloy_tables / xs_eam_alloy_load)");
+    require(rcut > 0, XS_E_ARG, "rcut must be positive");
+    if (eam_rho || eam_force) {
+      require_list(c);
+      require(rcut <= c.list_rcut + c.list_skin, XS_E_ARG, "rcut exceeds the neighbour-list radius");
+    }
+    launch_eam(c, rcut, eam_rho != 0, eam_rho2emb != 0, eam_force != 0, flags, ForceEpilogue{});
+  });
+}
+
+xs_status xs_push_f_v_r(xs_ctx* h, double dt, double dt_scale)
+{
+  return guarded(h, [&](Ctx& c) { launch_push_f_v_r(c, dt * dt_scale); });
+}
+
+xs_status xs_push_f_v(xs_ctx* h, double dt, double dt_scale)
+{
+  return guarded(h, [&](Ctx& c) { launch_push_f_v(c, dt * dt_scale); });
+}
+
+xs_status xs_thermo(xs_ctx* h, double out27[27])
+{
+  return guarded(h, [&](Ctx& c) {
+    require(out27 != nullptr, XS_E_ARG, "null output");
+    launch_thermo(c, out27);
+  });
+}
+
+xs_status xs_md_run(xs_ctx* h, int potential, int nsteps, double dt, double rcut, double skin, unsigned flags,
+                    int thermo_every, double* thermo_out, xs_run_stats* stats)
+{
+  return guarded(h, [&](Ctx& c) {
+    require(potential == XS_POT_LJ || potential == XS_POT_EAM, XS_E_ARG, "unknown potential");
+    require(nsteps >= 0 && dt > 0 && rcut > 0 && skin >= 0, XS_E_ARG, "bad run parameters");
+    require(thermo_every <= 0 || thermo_out, XS_E_ARG, "thermo_out required when thermo_every > 0");
+    require(potential == XS_POT_LJ ? c.has_lj : c.has_eam, XS_E_STATE, "potential not set");
+    const unsigned prec = flags & XS_MIXED;
+    const auto t0 = std::chrono::steady_clock::now();
+    int64_t rebuilds = 0;
+
+    build_neighbor_list(c, rcut, skin);
+    rebuilds++;
+    auto is_thermo = [&](int step) { return thermo_every > 0 && step % thermo_every == 0; };
+    auto forces = [&](int step, double kick) {
+      const bool th = is_thermo(step);
+      const unsigned f = prec | (th ? (XS_ENERGY | XS_VIRIAL) : 0u);
+      if (th && potential == XS_POT_LJ) launch_zero(c);
+      ForceEpilogue epi;
+      epi.fused = true;
+      epi.kick_dt = kick;
+      compute_forces(c, potential, rcut, f, epi);
+      if (th) launch_thermo(c, thermo_out + 27 * (size_t)(step / thermo_every));
+    };
+    // first_iteration: forces at t = 0, no kick
+    forces(0, 0.0);
+    const double thr2 = (0.5 * skin) * (0.5 * skin);
+    for (int step = 1; step <= nsteps; step++) {
+      XS_HIP(hipMemsetAsync(c.dscalar.p + 1, 0, sizeof(unsigned long long), c.stream));
+      launch_drift_kick(c, dt);
+      XS_HIP(hipMemcpyAsync(c.hscalar + 1, c.dscalar.p + 1, sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                            c.stream));
+      XS_HIP(hipStreamSynchronize(c.stream));
+      double d2;
+      std::memcpy(&d2, &c.hscalar[1], sizeof(double));
+      if (d2 > thr2) {
+        build_neighbor_list(c, rcut, skin);
+        rebuilds++;
+      }
+      forces(step, 0.5 * dt);
+    }
+    XS_HIP(hipStreamSynchronize(c.stream));
+    const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (stats) {
+      stats->steps = nsteps;
+      stats->rebuilds = rebuilds;
+      stats->max_neighbors = c.nmax;
+      stats->seconds = sec;
+    }
+  });
+}
+
+}  // extern "C"

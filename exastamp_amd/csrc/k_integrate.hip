@@ -1,0 +1,216 @@
+// k_integrate.hip -- per-atom operators (zero, force -> acceleration, velocity-
+// Verlet pushes) and the deterministic FP64 thermodynamic reduction.
+#include "xsmd_internal.hpp"
+
+#include <algorithm>
+
+namespace xsmd {
+namespace {
+
+__global__ void k_zero(int64_t n, int64_t npad, double* ax, double* ay, double* az, double* ep,
+                       double* vir)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  ax[i] = 0.0;
+  ay[i] = 0.0;
+  az[i] = 0.0;
+  ep[i] = 0.0;
+#pragma unroll
+  for (int q = 0; q < 9; q++) vir[q * npad + i] = 0.0;
+}
+
+__global__ void k_force_to_accel(int64_t n, double* ax, double* ay, double* az,
+                                 const uint8_t* __restrict__ type, const double* __restrict__ mass)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double m = mass[type[i]];
+  ax[i] /= m;
+  ay[i] /= m;
+  az[i] /= m;
+}
+
+// x += v*dt + a*dt2 with dt2 = dt*dt/2, evaluated as x + fma(v, dt, a*dt2).
+__global__ void k_push_f_v_r(int64_t n, double dt, double dt2, double* rx, double* ry, double* rz,
+                             const double* __restrict__ vx, const double* __restrict__ vy,
+                             const double* __restrict__ vz, const double* __restrict__ ax,
+                             const double* __restrict__ ay, const double* __restrict__ az)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  rx[i] += fma(vx[i], dt, ax[i] * dt2);
+  ry[i] += fma(vy[i], dt, ay[i] * dt2);
+  rz[i] += fma(vz[i], dt, az[i] * dt2);
+}
+
+__global__ void k_push_f_v(int64_t n, double dt, double* vx, double* vy, double* vz,
+                           const double* __restrict__ ax, const double* __restrict__ ay,
+                           const double* __restrict__ az)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  vx[i] = fma(ax[i], dt, vx[i]);
+  vy[i] = fma(ay[i], dt, vy[i]);
+  vz[i] = fma(az[i], dt, vz[i]);
+}
+
+// push_f_v_r(dt) then push_f_v(dt/2), plus the largest squared displacement
+// since the last list build (as bits of a non-negative double, which order
+// like the doubles themselves).
+__global__ void __launch_bounds__(256) k_drift_kick(
+    int64_t n, double dt, double dt2, double hdt, double* rx, double* ry, double* rz, double* vx,
+    double* vy, double* vz, const double* __restrict__ ax, const double* __restrict__ ay,
+    const double* __restrict__ az, const double* __restrict__ rrx, const double* __restrict__ rry,
+    const double* __restrict__ rrz, unsigned long long* __restrict__ maxd2)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double d2 = 0.0;
+  if (i < n) {
+    const double a0 = ax[i], a1 = ay[i], a2 = az[i];
+    double v0 = vx[i], v1 = vy[i], v2 = vz[i];
+    const double x = rx[i] + fma(v0, dt, a0 * dt2);
+    const double y = ry[i] + fma(v1, dt, a1 * dt2);
+    const double z = rz[i] + fma(v2, dt, a2 * dt2);
+    rx[i] = x;
+    ry[i] = y;
+    rz[i] = z;
+    vx[i] = fma(a0, hdt, v0);
+    vy[i] = fma(a1, hdt, v1);
+    vz[i] = fma(a2, hdt, v2);
+    d2 = xsmd_d2(x - rrx[i], y - rry[i], z - rrz[i]);
+  }
+  for (int off = 32; off > 0; off >>= 1) d2 = fmax(d2, __shfl_xor(d2, off));
+  if ((threadIdx.x & 63) == 0 && d2 > 0.0) atomicMax(maxd2, (unsigned long long)__double_as_longlong(d2));
+}
+
+constexpr int kThermoBlocks = 1024;
+constexpr int kThermoQ = 26;
+
+template <class T>
+__device__ __forceinline__ T wave_sum(T v)
+{
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+
+// Per-block partial sums of the 26 quantities of simulation_thermodynamic_state
+// (virial 9, v(x)v m 9, m v 3, m v^2 3, ep, m) over a fixed grid, so the
+// summation order -- and the result -- does not depend on timing.
+__global__ void __launch_bounds__(256) k_thermo_partial(
+    int64_t n, int64_t npad, const double* __restrict__ vx, const double* __restrict__ vy,
+    const double* __restrict__ vz, const double* __restrict__ ep, const double* __restrict__ vir,
+    const uint8_t* __restrict__ type, const double* __restrict__ mass, double* __restrict__ part)
+{
+  double acc[kThermoQ];
+#pragma unroll
+  for (int q = 0; q < kThermoQ; q++) acc[q] = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const double m = mass[type[i]];
+    const double v[3] = {vx[i], vy[i], vz[i]};
+#pragma unroll
+    for (int q = 0; q < 9; q++) acc[q] += vir[q * npad + i];
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+      for (int b = 0; b < 3; b++) acc[9 + 3 * a + b] += (v[a] * v[b]) * m;
+#pragma unroll
+    for (int a = 0; a < 3; a++) acc[18 + a] += v[a] * m;
+#pragma unroll
+    for (int a = 0; a < 3; a++) acc[21 + a] += v[a] * v[a] * m;
+    acc[24] += ep[i];
+    acc[25] += m;
+  }
+  __shared__ double sh[4][kThermoQ];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+  for (int q = 0; q < kThermoQ; q++) {
+    const double s = wave_sum(acc[q]);
+    if (lane == 0) sh[w][q] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x < kThermoQ) {
+    const int q = threadIdx.x;
+    part[(int64_t)blockIdx.x * kThermoQ + q] = ((sh[0][q] + sh[1][q]) + sh[2][q]) + sh[3][q];
+  }
+}
+
+__global__ void k_thermo_final(int nblocks, const double* __restrict__ part, double* __restrict__ out)
+{
+  const int q = threadIdx.x;
+  if (q >= kThermoQ) return;
+  double s = 0.0;
+  for (int b = 0; b < nblocks; b++) s += part[(int64_t)b * kThermoQ + q];
+  out[q] = s;
+}
+
+}  // namespace
+
+void launch_zero(Ctx& c)
+{
+  if (c.n == 0) return;
+  k_zero<<<grid_for(c.n), kBlock, 0, c.stream>>>(c.n, c.npad, c.ax.p, c.ay.p, c.az.p, c.ep.p, c.vir.p);
+  XS_HIP(hipGetLastError());
+}
+
+void launch_force_to_accel(Ctx& c)
+{
+  if (c.n == 0) return;
+  k_force_to_accel<<<grid_for(c.n), kBlock, 0, c.stream>>>(c.n, c.ax.p, c.ay.p, c.az.p, c.type.p,
+                                                           c.mass_d.p);
+  XS_HIP(hipGetLastError());
+}
+
+void launch_push_f_v_r(Ctx& c, double dt)
+{
+  if (c.n == 0) return;
+  const double dt2 = dt * dt * 0.5;
+  k_push_f_v_r<<<grid_for(c.n), kBlock, 0, c.stream>>>(c.n, dt, dt2, c.rx.p, c.ry.p, c.rz.p, c.vx.p,
+                                                       c.vy.p, c.vz.p, c.ax.p, c.ay.p, c.az.p);
+  XS_HIP(hipGetLastError());
+}
+
+void launch_push_f_v(Ctx& c, double dt)
+{
+  if (c.n == 0) return;
+  k_push_f_v<<<grid_for(c.n), kBlock, 0, c.stream>>>(c.n, dt, c.vx.p, c.vy.p, c.vz.p, c.ax.p, c.ay.p,
+                                                     c.az.p);
+  XS_HIP(hipGetLastError());
+}
+
+void launch_drift_kick(Ctx& c, double dt)
+{
+  if (c.n == 0) return;
+  const double dt2 = dt * dt * 0.5;
+  k_drift_kick<<<grid_for(c.n), kBlock, 0, c.stream>>>(
+      c.n, dt, dt2, dt * 0.5, c.rx.p, c.ry.p, c.rz.p, c.vx.p, c.vy.p, c.vz.p, c.ax.p, c.ay.p, c.az.p,
+      c.rrx.p, c.rry.p, c.rrz.p, c.dscalar.p + 1);
+  XS_HIP(hipGetLastError());
+}
+
+void launch_thermo(Ctx& c, double out[27])
+{
+  c.red_part.reserve((size_t)kThermoBlocks * kThermoQ);
+  c.red_out.reserve(kThermoQ);
+  double s[kThermoQ] = {0};
+  if (c.n > 0) {
+    const int nb = (int)std::min<int64_t>(kThermoBlocks, grid_for(c.n));
+    k_thermo_partial<<<nb, kBlock, 0, c.stream>>>(c.n, c.npad, c.vx.p, c.vy.p, c.vz.p, c.ep.p, c.vir.p,
+                                                  c.type.p, c.mass_d.p, c.red_part.p);
+    XS_HIP(hipGetLastError());
+    k_thermo_final<<<1, 64, 0, c.stream>>>(nb, c.red_part.p, c.red_out.p);
+    XS_HIP(hipGetLastError());
+    XS_HIP(hipMemcpyAsync(s, c.red_out.p, sizeof(s), hipMemcpyDeviceToHost, c.stream));
+    XS_HIP(hipStreamSynchronize(c.stream));
+  }
+  for (int q = 0; q < 9; q++) out[q] = s[q];
+  for (int q = 9; q < 18; q++) out[q] = 0.5 * s[q];
+  for (int q = 18; q < 21; q++) out[q] = s[q];
+  for (int q = 21; q < 24; q++) out[q] = 0.5 * s[q];
+  out[24] = s[24];
+  out[25] = s[25];
+  out[26] = (double)c.n;
+}
+
+}  // namespace xsmd

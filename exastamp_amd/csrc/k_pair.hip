@@ -1,0 +1,514 @@
+// k_pair.hip -- Lennard-Jones and EAM pair kernels over the full neighbour list.
+//
+// One thread per central atom walks its ELL list (coalesced entry loads), gathers
+// neighbour positions, and accumulates its own force only: no atomics, no
+// Newton-symmetric scatter, so results do not depend on scheduling.
+//
+// Precision: template parameter Real is the type of the pair and spline
+// arithmetic.  Real = double is the plain FP64 form.  Real = float is the mixed
+// form: positions, the minimum-image displacement, d^2, the cutoff test, the
+// per-atom force/energy/virial accumulation and the EAM density sum stay in
+// FP64; distance, spline evaluation and the pair scalar are FP32.
+#include "xsmd_internal.hpp"
+
+namespace xsmd {
+namespace {
+
+template <class Real>
+struct Q4 {
+  Real a, b, c, d;
+};
+
+__device__ __forceinline__ Q4<float> load4(const float* __restrict__ p)
+{
+  const float4 v = *reinterpret_cast<const float4*>(p);
+  return {v.x, v.y, v.z, v.w};
+}
+__device__ __forceinline__ Q4<double> load4(const double* __restrict__ p)
+{
+  const double2 u = reinterpret_cast<const double2*>(p)[0];
+  const double2 w = reinterpret_cast<const double2*>(p)[1];
+  return {u.x, u.y, w.x, w.y};
+}
+
+// 1/sqrt(d2) and sqrt(d2): one hardware rsq in FP32, correctly rounded sqrt
+// and division in FP64 (what plain FP64 code compiles to).
+__device__ __forceinline__ void dist(float d2, float& r, float& rinv)
+{
+  rinv = __builtin_amdgcn_rsqf(d2);
+  r = d2 * rinv;
+}
+__device__ __forceinline__ void dist(double d2, double& r, double& rinv)
+{
+  r = sqrt(d2);
+  rinv = 1.0 / r;
+}
+__device__ __forceinline__ float recip(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double recip(double x) { return 1.0 / x; }
+
+// XCD-aware block order: hardware hands workgroup b to XCD b % 8, so map it to
+// logical block (b % 8) * (G/8) + b / 8 and each XCD's L2 sees one contiguous
+// (hence spatially compact) range of atoms.  G is padded to a multiple of 8.
+__device__ __forceinline__ int64_t logical_block()
+{
+  const unsigned b = blockIdx.x, g = gridDim.x;
+  return (int64_t)(b & 7u) * (g >> 3) + (b >> 3);
+}
+
+struct PairCommon {
+  int64_t n, npad;
+  const double* __restrict__ rx;
+  const double* __restrict__ ry;
+  const double* __restrict__ rz;
+  const uint8_t* __restrict__ type;
+  const uint16_t* __restrict__ rep;
+  const BoxDev* __restrict__ box;
+  const uint32_t* __restrict__ nbr;
+  const int* __restrict__ nnb;
+  double rc2;
+};
+
+struct Outputs {
+  double* ax;
+  double* ay;
+  double* az;
+  double* ep;
+  double* vir;
+  const double* mass;  // fused epilogue
+  double* vx;
+  double* vy;
+  double* vz;
+  double kick_dt;
+};
+
+// Displacement r_j(image) - r_i in FP64 and its squared norm.
+struct Disp {
+  double x, y, z, d2;
+  uint32_t j;
+};
+__device__ __forceinline__ Disp displacement(const PairCommon& pc, uint32_t e, double xi, double yi,
+                                             double zi, const double* L)
+{
+  Disp d;
+  d.j = xsmd_nbr_index(e);
+  double xj = pc.rx[d.j], yj = pc.ry[d.j], zj = pc.rz[d.j];
+  if (e >> XSMD_NBR_INDEX_BITS) {
+    xj = xsmd_image(xj, xsmd_nbr_img_x(e), L[0]);
+    yj = xsmd_image(yj, xsmd_nbr_img_y(e), L[1]);
+    zj = xsmd_image(zj, xsmd_nbr_img_z(e), L[2]);
+  }
+  d.x = xj - xi;
+  d.y = yj - yi;
+  d.z = zj - zi;
+  d.d2 = xsmd_d2(d.x, d.y, d.z);
+  return d;
+}
+
+template <bool ENERGY, bool VIRIAL, bool FUSED>
+__device__ __forceinline__ void write_outputs(const Outputs& o, const PairCommon& pc, int64_t i, double fx,
+                                              double fy, double fz, double e, const double* v)
+{
+  if constexpr (FUSED) {
+    const double m = o.mass[pc.type[i]];
+    const double a0 = fx / m, a1 = fy / m, a2 = fz / m;
+    o.ax[i] = a0;
+    o.ay[i] = a1;
+    o.az[i] = a2;
+    o.vx[i] = fma(a0, o.kick_dt, o.vx[i]);
+    o.vy[i] = fma(a1, o.kick_dt, o.vy[i]);
+    o.vz[i] = fma(a2, o.kick_dt, o.vz[i]);
+  } else {
+    o.ax[i] += fx;
+    o.ay[i] += fy;
+    o.az[i] += fz;
+  }
+  if constexpr (ENERGY) o.ep[i] += e;
+  if constexpr (VIRIAL) {
+#pragma unroll
+    for (int q = 0; q < 9; q++) {
+      if constexpr (FUSED) o.vir[q * pc.npad + i] = v[q];
+      else o.vir[q * pc.npad + i] += v[q];
+    }
+  }
+}
+
+__device__ __forceinline__ void add_virial(double* v, double fe0, double fe1, double fe2, const Disp& d)
+{
+  const double fe[3] = {fe0, fe1, fe2};
+  const double dr[3] = {d.x, d.y, d.z};
+#pragma unroll
+  for (int a = 0; a < 3; a++)
+#pragma unroll
+    for (int b = 0; b < 3; b++) v[3 * a + b] += (fe[a] * dr[b]) * -0.5;
+}
+
+// ------------------------------------------------------------------------------
+// Lennard-Jones: lj_compute_energy (lennard_jones.h:40-50) with the energy shift
+// and per-atom accumulation of pair_potential_force_op_singlemat.h:101-123.
+// Written in r^2 only: (sigma/r)^2 = sigma^2 / d2 and de/r = -24 eps (2 s^12 - s^6) / d2.
+// ------------------------------------------------------------------------------
+template <class Real>
+struct LJParams {
+  Real sigma2, eps4, eps24, ecut;
+};
+
+template <class Real, bool ENERGY, bool VIRIAL, bool FUSED>
+__global__ void __launch_bounds__(256) k_lj(PairCommon pc, LJParams<Real> lp, Outputs o)
+{
+  const int64_t i = logical_block() * blockDim.x + threadIdx.x;
+  if (i >= pc.n) return;
+  const double* L = pc.box[pc.rep ? pc.rep[i] : 0].L;
+  const double xi = pc.rx[i], yi = pc.ry[i], zi = pc.rz[i];
+  double fx = 0.0, fy = 0.0, fz = 0.0, e = 0.0;
+  double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  const int cnt = pc.nnb[i];
+  for (int k = 0; k < cnt; k++) {
+    const Disp d = displacement(pc, pc.nbr[(int64_t)k * pc.npad + i], xi, yi, zi, L);
+    if (d.d2 > pc.rc2) continue;
+    const Real ir2 = recip((Real)d.d2);
+    const Real s2 = lp.sigma2 * ir2;
+    const Real s6 = s2 * s2 * s2;
+    const Real s12 = s6 * s6;
+    const double de_r = (double)(-lp.eps24 * (Real(2) * s12 - s6) * ir2);
+    const double fe0 = de_r * d.x, fe1 = de_r * d.y, fe2 = de_r * d.z;
+    fx += fe0;
+    fy += fe1;
+    fz += fe2;
+    if constexpr (ENERGY) e += 0.5 * (double)(lp.eps4 * (s12 - s6) - lp.ecut);
+    if constexpr (VIRIAL) add_virial(v, fe0, fe1, fe2, d);
+  }
+  write_outputs<ENERGY, VIRIAL, FUSED>(o, pc, i, fx, fy, fz, e, v);
+}
+
+// ------------------------------------------------------------------------------
+// EAM (eam_alloy): spline evaluation as eam_alloy.h:198-313, pass semantics of
+// SymRhoOp / Rho2EmbOp / SymForceOp with NewtonSym = false
+// (eam_force_op_multimat.h:114-343).
+// ------------------------------------------------------------------------------
+template <class Real>
+__device__ __forceinline__ void knot_r(const EamTabDev<Real>& t, Real r, int& m, Real& p)
+{
+  p = r * t.rdr + Real(1);
+  m = (int)p;
+  m = min(m, t.nr - 1);
+  p -= (Real)m;
+  p = fmin(p, Real(1));
+}
+
+template <class Real>
+__device__ __forceinline__ const Real* row_r(const Real* tab, const EamTabDev<Real>& t, int which, int m)
+{
+  return tab + ((int64_t)which * (t.nr + 1) + m) * 8;
+}
+
+template <class Real>
+__device__ __forceinline__ Real cubic(const Q4<Real>& c, Real p)
+{
+  return ((c.a * p + c.b) * p + c.c) * p + c.d;
+}
+template <class Real>
+__device__ __forceinline__ Real quad(const Q4<Real>& c, Real p)
+{
+  return (c.a * p + c.b) * p + c.c;
+}
+
+// F(rho), F'(rho) in eV-converted internal units, with the linear extrapolation
+// of F above rhomax (eam_alloy.h:289-313).
+template <class Real>
+__device__ __forceinline__ void embed(const EamTabDev<Real>& t, int ti, double rho, Real& F, Real& Fp)
+{
+  const Real rr = (Real)rho;
+  Real p = rr * t.rdrho + Real(1);
+  int m = (int)p;
+  m = max(1, min(m, t.nrho - 1));
+  p -= (Real)m;
+  p = fmin(p, Real(1));
+  const Real* row = t.frho + ((int64_t)t.t2frho[ti + 1] * (t.nrho + 1) + m) * 8;
+  const Q4<Real> v = load4(row);
+  const Q4<Real> dv = load4(row + 4);
+  Fp = quad(dv, p);
+  F = cubic(v, p);
+  if (rr > t.rhomax) F += Fp * (rr - t.rhomax);
+  F *= t.conv;
+  Fp *= t.conv;
+}
+
+// Pass 1 (+ fused pass 1b): rho_i = sum_j rho_{type_j}(r_ij) in FP64, then
+// F'(rho_i) (and F(rho_i) into ep when ENERGY).
+template <class Real, bool MULTI, bool RHO, bool EMB, bool ENERGY, bool SET_EP>
+__global__ void __launch_bounds__(256) k_eam_rho(PairCommon pc, EamTabDev<Real> t, double* __restrict__ rho_io,
+                                                 Real* __restrict__ fp_out, double* __restrict__ ep)
+{
+  const int64_t i = logical_block() * blockDim.x + threadIdx.x;
+  if (i >= pc.n) return;
+  const int ti = MULTI ? pc.type[i] : 0;
+  double rho = 0.0;
+  if constexpr (RHO) {
+    const double* L = pc.box[pc.rep ? pc.rep[i] : 0].L;
+    const double xi = pc.rx[i], yi = pc.ry[i], zi = pc.rz[i];
+    const int cnt = pc.nnb[i];
+    for (int k = 0; k < cnt; k++) {
+      const Disp d = displacement(pc, pc.nbr[(int64_t)k * pc.npad + i], xi, yi, zi, L);
+      if (d.d2 > pc.rc2) continue;
+      const int tj = MULTI ? pc.type[d.j] : 0;
+      Real r, rinv;
+      dist((Real)d.d2, r, rinv);
+      int m;
+      Real p;
+      knot_r(t, r, m, p);
+      const Q4<Real> c = load4(row_r(t.rhor, t, t.t2rhor[tj + 1][ti + 1], m));
+      rho += (double)cubic(c, p);
+    }
+  } else {
+    rho = rho_io[i];
+  }
+  if constexpr (EMB) {
+    Real F, Fp;
+    embed(t, ti, rho, F, Fp);
+    fp_out[i] = Fp;
+    if constexpr (ENERGY) {
+      if constexpr (SET_EP) ep[i] = (double)F;
+      else ep[i] += (double)F;
+    }
+  } else {
+    rho_io[i] = rho;
+  }
+}
+
+// Pass 2: psi'(r) = F'_i rho'_j(r) + F'_j rho'_i(r) + phi'(r), f_i += psi'/r (r_j - r_i).
+template <class Real, bool MULTI, bool ENERGY, bool VIRIAL, bool FUSED>
+__global__ void __launch_bounds__(256) k_eam_force(PairCommon pc, EamTabDev<Real> t, const Real* __restrict__ fp,
+                                                   Outputs o)
+{
+  const int64_t i = logical_block() * blockDim.x + threadIdx.x;
+  if (i >= pc.n) return;
+  const int ti = MULTI ? pc.type[i] : 0;
+  const double* L = pc.box[pc.rep ? pc.rep[i] : 0].L;
+  const double xi = pc.rx[i], yi = pc.ry[i], zi = pc.rz[i];
+  const Real fpi = fp[i];
+  double fx = 0.0, fy = 0.0, fz = 0.0, e = 0.0;
+  double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  const int cnt = pc.nnb[i];
+  for (int k = 0; k < cnt; k++) {
+    const Disp d = displacement(pc, pc.nbr[(int64_t)k * pc.npad + i], xi, yi, zi, L);
+    if (d.d2 > pc.rc2) continue;
+    const int tj = MULTI ? pc.type[d.j] : 0;
+    const Real fpj = fp[d.j];
+    Real r, rinv;
+    dist((Real)d.d2, r, rinv);
+    int m;
+    Real p;
+    knot_r(t, r, m, p);
+    const Q4<Real> dri = load4(row_r(t.rhor, t, t.t2rhor[ti + 1][tj + 1], m) + 4);
+    const Real rhoip = quad(dri, p);
+    Real rhojp = rhoip;
+    if constexpr (MULTI) rhojp = quad(load4(row_r(t.rhor, t, t.t2rhor[tj + 1][ti + 1], m) + 4), p);
+    const Real* zrow = row_r(t.z2r, t, t.t2z2r[ti + 1][tj + 1], m);
+    const Q4<Real> zv = load4(zrow);
+    const Q4<Real> zd = load4(zrow + 4);
+    const Real z2p = quad(zd, p);
+    const Real z2 = cubic(zv, p);
+    Real phi = z2 * rinv;
+    const Real phip = (z2p * rinv - phi * rinv) * t.conv;
+    phi *= t.conv;
+    const Real psip = fpi * rhojp + fpj * rhoip + phip;
+    const double fpair = (double)(psip * rinv);
+    const double fe0 = d.x * fpair, fe1 = d.y * fpair, fe2 = d.z * fpair;
+    fx += fe0;
+    fy += fe1;
+    fz += fe2;
+    if constexpr (ENERGY) e += 0.5 * (double)phi;
+    if constexpr (VIRIAL) add_virial(v, fe0, fe1, fe2, d);
+  }
+  write_outputs<ENERGY, VIRIAL, FUSED>(o, pc, i, fx, fy, fz, e, v);
+}
+
+PairCommon make_common(Ctx& c, double rcut)
+{
+  PairCommon pc;
+  pc.n = c.n;
+  pc.npad = c.npad;
+  pc.rx = c.rx.p;
+  pc.ry = c.ry.p;
+  pc.rz = c.rz.p;
+  pc.type = c.type.p;
+  pc.rep = c.has_rep ? c.rep.p : nullptr;
+  pc.box = c.box_d.p;
+  pc.nbr = c.nbr.p;
+  pc.nnb = c.nnb.p;
+  pc.rc2 = rcut * rcut;
+  return pc;
+}
+
+Outputs make_outputs(Ctx& c, const ForceEpilogue& ep)
+{
+  Outputs o;
+  o.ax = c.ax.p;
+  o.ay = c.ay.p;
+  o.az = c.az.p;
+  o.ep = c.ep.p;
+  o.vir = c.vir.p;
+  o.mass = c.mass_d.p;
+  o.vx = c.vx.p;
+  o.vy = c.vy.p;
+  o.vz = c.vz.p;
+  o.kick_dt = ep.kick_dt;
+  return o;
+}
+
+unsigned pair_grid(int64_t n)
+{
+  const unsigned g = grid_for(n);
+  return (g + 7u) / 8u * 8u;
+}
+
+template <class Real>
+EamTabDev<Real> make_tab(Ctx& c);
+
+template <>
+EamTabDev<double> make_tab<double>(Ctx& c)
+{
+  EamTabDev<double> t;
+  t.frho = c.eam.frho_d.p;
+  t.rhor = c.eam.rhor_d.p;
+  t.z2r = c.eam.z2r_d.p;
+  t.nr = c.eam.nr;
+  t.nrho = c.eam.nrho;
+  t.rdr = c.eam.rdr;
+  t.rdrho = c.eam.rdrho;
+  t.rhomax = c.eam.rhomax;
+  std::memcpy(t.t2rhor, c.eam.t2rhor, sizeof(t.t2rhor));
+  std::memcpy(t.t2z2r, c.eam.t2z2r, sizeof(t.t2z2r));
+  std::memcpy(t.t2frho, c.eam.t2frho, sizeof(t.t2frho));
+  return t;
+}
+
+template <>
+EamTabDev<float> make_tab<float>(Ctx& c)
+{
+  EamTabDev<float> t;
+  t.frho = c.eam.frho_f.p;
+  t.rhor = c.eam.rhor_f.p;
+  t.z2r = c.eam.z2r_f.p;
+  t.nr = c.eam.nr;
+  t.nrho = c.eam.nrho;
+  t.rdr = (float)c.eam.rdr;
+  t.rdrho = (float)c.eam.rdrho;
+  t.rhomax = (float)c.eam.rhomax;
+  std::memcpy(t.t2rhor, c.eam.t2rhor, sizeof(t.t2rhor));
+  std::memcpy(t.t2z2r, c.eam.t2z2r, sizeof(t.t2z2r));
+  std::memcpy(t.t2frho, c.eam.t2frho, sizeof(t.t2frho));
+  return t;
+}
+
+// 1 eV in Da.A^2/ps^2 (CODATA-2018: e / (amu * 1e4)); conversion_z2r = 1 eV.A and
+// conversion_frho = 1 eV of eam_alloy.h:156-157 in exaStamp's internal units.
+constexpr double kEV = 1.602176634e-19 / (1.66053906660e-27 * 1.0e4);
+
+template <class Real, bool MULTI>
+void eam_dispatch(Ctx& c, double rcut, bool do_rho, bool do_emb, bool do_force, unsigned flags,
+                  const ForceEpilogue& epi)
+{
+  EamTabDev<Real> t = make_tab<Real>(c);
+  t.conv = (Real)kEV;
+  const PairCommon pc = make_common(c, rcut);
+  const unsigned g = pair_grid(c.n);
+  const bool energy = flags & XS_ENERGY;
+  Real* fpbuf;
+  if constexpr (sizeof(Real) == 4) fpbuf = c.fpf.p;
+  else fpbuf = c.rho.p;
+
+  if (do_rho || do_emb) {
+    if (!do_rho && c.rho_state != RhoState::Rho) throw Error(XS_E_STATE, "eam_rho2emb needs eam_rho first");
+#define XS_RHO(R, E, EN, SET) k_eam_rho<Real, MULTI, R, E, EN, SET><<<g, kBlock, 0, c.stream>>>(pc, t, c.rho.p, fpbuf, c.ep.p)
+    if (do_rho && do_emb) {
+      if (energy && epi.fused) XS_RHO(true, true, true, true);
+      else if (energy) XS_RHO(true, true, true, false);
+      else XS_RHO(true, true, false, false);
+    } else if (do_rho) {
+      XS_RHO(true, false, false, false);
+    } else {
+      if (energy) XS_RHO(false, true, true, false);
+      else XS_RHO(false, true, false, false);
+    }
+#undef XS_RHO
+    XS_HIP(hipGetLastError());
+    c.rho_state = do_emb ? (sizeof(Real) == 4 ? RhoState::FpFloat : RhoState::FpDouble) : RhoState::Rho;
+  }
+  if (do_force) {
+    const RhoState want = sizeof(Real) == 4 ? RhoState::FpFloat : RhoState::FpDouble;
+    if (c.rho_state != want)
+      throw Error(XS_E_STATE, "eam_force needs F'(rho) from eam_rho2emb in the same precision");
+    const Outputs o = make_outputs(c, epi);
+    const bool vir = flags & XS_VIRIAL;
+#define XS_FORCE(EN, VI, FU) k_eam_force<Real, MULTI, EN, VI, FU><<<g, kBlock, 0, c.stream>>>(pc, t, fpbuf, o)
+    if (epi.fused) {
+      if (energy && vir) XS_FORCE(true, true, true);
+      else if (energy) XS_FORCE(true, false, true);
+      else if (vir) XS_FORCE(false, true, true);
+      else XS_FORCE(false, false, true);
+    } else {
+      if (energy && vir) XS_FORCE(true, true, false);
+      else if (energy) XS_FORCE(true, false, false);
+      else if (vir) XS_FORCE(false, true, false);
+      else XS_FORCE(false, false, false);
+    }
+#undef XS_FORCE
+    XS_HIP(hipGetLastError());
+  }
+}
+
+template <class Real>
+void lj_dispatch(Ctx& c, double eps, double sigma, double rcut, unsigned flags, const ForceEpilogue& epi)
+{
+  // energy_cutoff (pair_potential_impl.hxx:487-498): E(rcut) in FP64
+  double ecut = 0.0;
+  if (rcut > 0.0) {
+    const double ratio = sigma / rcut, r2 = ratio * ratio, r6 = r2 * r2 * r2, r12 = r6 * r6;
+    ecut = 4. * eps * (r12 - r6);
+  }
+  LJParams<Real> lp{(Real)(sigma * sigma), (Real)(4.0 * eps), (Real)(24.0 * eps), (Real)ecut};
+  const PairCommon pc = make_common(c, rcut);
+  const Outputs o = make_outputs(c, epi);
+  const unsigned g = pair_grid(c.n);
+  const bool energy = flags & XS_ENERGY, vir = flags & XS_VIRIAL;
+#define XS_LJ(EN, VI, FU) k_lj<Real, EN, VI, FU><<<g, kBlock, 0, c.stream>>>(pc, lp, o)
+  if (epi.fused) {
+    if (energy && vir) XS_LJ(true, true, true);
+    else if (energy) XS_LJ(true, false, true);
+    else if (vir) XS_LJ(false, true, true);
+    else XS_LJ(false, false, true);
+  } else {
+    if (energy && vir) XS_LJ(true, true, false);
+    else if (energy) XS_LJ(true, false, false);
+    else if (vir) XS_LJ(false, true, false);
+    else XS_LJ(false, false, false);
+  }
+#undef XS_LJ
+  XS_HIP(hipGetLastError());
+}
+
+}  // namespace
+
+void launch_lj(Ctx& c, double eps, double sigma, double rcut, unsigned flags, const ForceEpilogue& epi)
+{
+  if (c.n == 0) return;
+  if (flags & XS_MIXED) lj_dispatch<float>(c, eps, sigma, rcut, flags, epi);
+  else lj_dispatch<double>(c, eps, sigma, rcut, flags, epi);
+}
+
+void launch_eam(Ctx& c, double rcut, bool do_rho, bool do_emb, bool do_force, unsigned flags,
+                const ForceEpilogue& epi)
+{
+  if (c.n == 0) return;
+  const bool multi = c.eam.nel > 1;
+  if (flags & XS_MIXED) {
+    if (multi) eam_dispatch<float, true>(c, rcut, do_rho, do_emb, do_force, flags, epi);
+    else eam_dispatch<float, false>(c, rcut, do_rho, do_emb, do_force, flags, epi);
+  } else {
+    if (multi) eam_dispatch<double, true>(c, rcut, do_rho, do_emb, do_force, flags, epi);
+    else eam_dispatch<double, false>(c, rcut, do_rho, do_emb, do_force, flags, epi);
+  }
+}
+
+}  // namespace xsmd

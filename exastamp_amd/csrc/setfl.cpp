@@ -1,0 +1,162 @@
+// setfl.cpp -- setfl (.eam.alloy) reader, spline coefficient build and upload of
+// the EAM tables to the device in FP64 and FP32.
+#include "xsmd_internal.hpp"
+
+#include <fstream>
+#include <limits>
+
+// Coefficients must be bit-identical to the reference's, so no FMA contraction.
+#pragma clang fp contract(off)
+
+namespace xsmd {
+
+// Cubic spline through knots f[0..n-1] (knot m+1 of the 1-based scheme), with the
+// slopes from the fourth-order central difference in the interior and lower order
+// at the ends: the construction of EamAlloyTools::interpolate
+// (eam_alloy.cpp:29-58), evaluated in the same order so every coefficient has the
+// same bits.  coeffs: (n+1) rows of 8, row 0 unused; row m = [c0..c6, 0] with
+// value(p) = ((c3 p + c4) p + c5) p + c6 and derivative = (c0 p + c1) p + c2.
+void spline_interpolate(int n, double delta, const double* f, double* coeffs)
+{
+  if (n < 5) throw Error(XS_E_ARG, "spline needs at least 5 knots");
+  std::vector<double> v(n + 2, 0.0), s(n + 2, 0.0), a(n + 2, 0.0), b(n + 2, 0.0);
+  for (int m = 1; m <= n; m++) v[m] = f[m - 1];
+  s[1] = v[2] - v[1];
+  s[2] = 0.5 * (v[3] - v[1]);
+  s[n - 1] = 0.5 * (v[n] - v[n - 2]);
+  s[n] = v[n] - v[n - 1];
+  for (int m = 3; m <= n - 2; m++) s[m] = ((v[m - 2] - v[m + 2]) + 8.0 * (v[m + 1] - v[m - 1])) / 12.0;
+  for (int m = 1; m <= n - 1; m++) {
+    const double dv = v[m + 1] - v[m];
+    b[m] = 3.0 * dv - 2.0 * s[m] - s[m + 1];   // p^2 coefficient
+    a[m] = s[m] + s[m + 1] - 2.0 * dv;          // p^3 coefficient
+  }
+  for (int k = 0; k < 8; k++) coeffs[k] = 0.0;
+  for (int m = 1; m <= n; m++) {
+    double* row = coeffs + 8 * (size_t)m;
+    row[6] = v[m];
+    row[5] = s[m];
+    row[4] = b[m];
+    row[3] = a[m];
+    row[2] = s[m] / delta;
+    row[1] = 2.0 * b[m] / delta;
+    row[0] = 3.0 * a[m] / delta;
+    row[7] = 0.0;
+  }
+}
+
+// setfl reader with the reference's token order (eam_alloy.cpp:86-201): three
+// comment lines; "nel name..."; "nrho drho nr dr rcut"; per element
+// "Z mass a0 lattice" then F(rho)[nrho] and rho(r)[nr]; then r*phi(r)[nr] for the
+// packed lower triangle 11, 21, 22, 31, ...
+void read_setfl(const char* path, int& nel, int& nrho, double& drho, int& nr, double& dr,
+                double& rcut, std::vector<double>& frho, std::vector<double>& rhor,
+                std::vector<double>& z2r, std::vector<double>& masses)
+{
+  std::ifstream in(path);
+  if (!in) throw Error(XS_E_IO, std::string("cannot open setfl file ") + path);
+  for (int i = 0; i < 3; ++i) in.ignore(std::numeric_limits<std::streamsize>::max(), '\n');
+  long long ne = 0;
+  in >> ne;
+  if (!in || ne < 1 || ne > 7) throw Error(XS_E_IO, "setfl: bad element count");
+  nel = (int)ne;
+  for (int i = 0; i < nel; i++) {
+    std::string name;
+    in >> name;
+  }
+  long long a = 0, b = 0;
+  in >> a >> drho >> b >> dr >> rcut;
+  if (!in || a < 5 || b < 5) throw Error(XS_E_IO, "setfl: bad grid line");
+  nrho = (int)a;
+  nr = (int)b;
+  frho.assign((size_t)nel * nrho, 0.0);
+  rhor.assign((size_t)nel * nr, 0.0);
+  masses.assign(nel, 0.0);
+  for (int e = 0; e < nel; e++) {
+    int z;
+    double a0;
+    std::string lat;
+    in >> std::ws >> z >> masses[e] >> a0 >> lat;
+    for (int k = 0; k < nrho; k++) in >> frho[(size_t)e * nrho + k];
+    for (int k = 0; k < nr; k++) in >> rhor[(size_t)e * nr + k];
+  }
+  const int nz = nel * (nel + 1) / 2;
+  z2r.assign((size_t)nz * nr, 0.0);
+  for (int t = 0; t < nz; t++)
+    for (int k = 0; k < nr; k++) in >> z2r[(size_t)t * nr + k];
+  if (!in) throw Error(XS_E_IO, "setfl: file truncated");
+}
+
+namespace {
+
+// [c0..c6,0] rows -> device order [c3 c4 c5 c6 c0 c1 c2 0]
+template <class Real>
+void reorder_rows(const std::vector<double>& src, std::vector<Real>& dst)
+{
+  const size_t rows = src.size() / 8;
+  dst.resize(src.size());
+  for (size_t r = 0; r < rows; r++) {
+    const double* s = &src[8 * r];
+    Real* d = &dst[8 * r];
+    d[0] = (Real)s[3];
+    d[1] = (Real)s[4];
+    d[2] = (Real)s[5];
+    d[3] = (Real)s[6];
+    d[4] = (Real)s[0];
+    d[5] = (Real)s[1];
+    d[6] = (Real)s[2];
+    d[7] = (Real)0;
+  }
+}
+
+template <class Real>
+void upload(Ctx& c, DBuf<Real>& buf, const std::vector<double>& coeffs)
+{
+  std::vector<Real> tmp;
+  reorder_rows(coeffs, tmp);
+  buf.reserve(tmp.size());
+  XS_HIP(hipMemcpyAsync(buf.p, tmp.data(), tmp.size() * sizeof(Real), hipMemcpyHostToDevice, c.stream));
+  XS_HIP(hipStreamSynchronize(c.stream));
+}
+
+std::vector<double> build_all(int ntab, int n, double delta, const std::vector<double>& knots)
+{
+  std::vector<double> out((size_t)ntab * (n + 1) * 8, 0.0);
+  for (int t = 0; t < ntab; t++)
+    spline_interpolate(n, delta, knots.data() + (size_t)t * n, out.data() + (size_t)t * (n + 1) * 8);
+  return out;
+}
+
+}  // namespace
+
+// Type maps of initialize_types_table (eam_alloy.h:70-133) for nel types mapped
+// to elements in order, then both precisions of every table to the device.
+void eam_upload(Ctx& c)
+{
+  EamHost& e = c.eam;
+  const int nel = e.nel;
+  std::memset(e.t2rhor, 0, sizeof(e.t2rhor));
+  std::memset(e.t2z2r, 0, sizeof(e.t2z2r));
+  std::memset(e.t2frho, 0, sizeof(e.t2frho));
+  for (int i = 1; i <= nel; i++) {
+    e.t2frho[i] = (int8_t)(i - 1);
+    for (int j = 1; j <= nel; j++) {
+      e.t2rhor[i][j] = (int8_t)(i - 1);
+      int hiv = i - 1, lov = j - 1;
+      if (hiv < lov) std::swap(hiv, lov);
+      e.t2z2r[i][j] = (int8_t)(hiv * (hiv + 1) / 2 + lov);
+    }
+  }
+  const int nz = nel * (nel + 1) / 2;
+  const std::vector<double> cf = build_all(nel, e.nrho, e.drho, e.frho_knots);
+  const std::vector<double> cr = build_all(nel, e.nr, e.dr, e.rhor_knots);
+  const std::vector<double> cz = build_all(nz, e.nr, e.dr, e.z2r_knots);
+  upload(c, e.frho_d, cf);
+  upload(c, e.rhor_d, cr);
+  upload(c, e.z2r_d, cz);
+  upload(c, e.frho_f, cf);
+  upload(c, e.rhor_f, cr);
+  upload(c, e.z2r_f, cz);
+}
+
+}  // namespace xsmd

@@ -1,0 +1,195 @@
+// xsmd_internal.hpp -- context, device buffers and kernel-launch declarations
+// shared by the libxsmd.so translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#define XSMD_HD __host__ __device__ __forceinline__
+#include "xsmd.h"
+#include "xsmd_pair_geometry.h"
+
+namespace xsmd {
+
+// Internal error: carries the C-ABI status; converted at the C boundary.
+struct Error : std::runtime_error {
+  xs_status code;
+  Error(xs_status c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+inline void hip_check(hipError_t e, const char* what)
+{
+  if (e != hipSuccess) throw Error(XS_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+#define XS_HIP(call) ::xsmd::hip_check((call), #call)
+
+// Device array owned by a context; grows, never shrinks.
+template <class T>
+struct DBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DBuf() = default;
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  ~DBuf() { release(); }
+  void reserve(size_t count)
+  {
+    if (count <= n && p) return;
+    release();
+    if (count == 0) count = 1;
+    if (hipMalloc(&p, count * sizeof(T)) != hipSuccess) {
+      p = nullptr;
+      n = 0;
+      throw Error(XS_E_ALLOC, "device allocation of " + std::to_string(count * sizeof(T)) + " bytes failed");
+    }
+    n = count;
+  }
+  void release()
+  {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  void swap(DBuf& o)
+  {
+    std::swap(p, o.p);
+    std::swap(n, o.n);
+  }
+};
+
+// Box of one replica, as kernels read it: lo[3], L[3], n/L[3].
+struct BoxDev {
+  double lo[3];
+  double L[3];
+  double inv_cs[3];
+};
+
+// Spline tables on the device.  Each row of the reference's SplineCoeffs
+// (eam_alloy.h:37-42: c0..c2 derivative quadratic, c3..c6 value cubic) is
+// stored as 8 values reordered to [c3 c4 c5 c6 | c0 c1 c2 0] so the value
+// cubic and the derivative quadratic are each one aligned 4-vector load.
+template <class Real>
+struct EamTabDev {
+  const Real* frho = nullptr;  // nel tables of (nrho+1) rows
+  const Real* rhor = nullptr;  // nel tables of (nr+1) rows
+  const Real* z2r = nullptr;   // nel(nel+1)/2 tables of (nr+1) rows
+  int nr = 0, nrho = 0;
+  Real rdr = 0, rdrho = 0, rhomax = 0, conv = 0;
+  int8_t t2rhor[8][8] = {};
+  int8_t t2z2r[8][8] = {};
+  int8_t t2frho[8] = {};
+};
+
+struct EamHost {
+  int nel = 0, nr = 0, nrho = 0;
+  double dr = 0, drho = 0, rdr = 0, rdrho = 0, rc = 0, rhomax = 0;
+  std::vector<double> frho_knots, rhor_knots, z2r_knots;
+  int8_t t2rhor[8][8] = {};
+  int8_t t2z2r[8][8] = {};
+  int8_t t2frho[8] = {};
+  DBuf<double> frho_d, rhor_d, z2r_d;
+  DBuf<float> frho_f, rhor_f, z2r_f;
+};
+
+enum class RhoState { None, Rho, FpDouble, FpFloat };
+
+struct Ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+
+  int nspecies = 0;
+  double mass[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  DBuf<double> inv_mass_d;  // unused slots 0
+  DBuf<double> mass_d;
+
+  int64_t n = 0;
+  int64_t npad = 0;
+  int nrep = 1;
+  int periodic[3] = {1, 1, 1};
+  std::vector<BoxDev> box_h;
+  DBuf<BoxDev> box_d;
+
+  // per-atom state in cell-sorted order (and the permutation targets)
+  DBuf<double> rx, ry, rz, vx, vy, vz, ax, ay, az;
+  DBuf<double> rx2, ry2, rz2, vx2, vy2, vz2, ax2, ay2, az2;
+  DBuf<double> rrx, rry, rrz;  // positions at the last list build
+  DBuf<uint8_t> type, type2;
+  DBuf<int64_t> id, id2;
+  DBuf<uint16_t> rep, rep2;
+  bool has_rep = false;
+  DBuf<double> ep, vir, rho;
+  DBuf<float> fpf;
+  RhoState rho_state = RhoState::None;
+
+  // cell binning
+  int nc[3] = {1, 1, 1};
+  int64_t ncell_rep = 1, ncell_tot = 1;
+  DBuf<uint32_t> keys, keys2, vals, vals2;
+  DBuf<uint8_t> sort_tmp;
+  DBuf<int> cstart, cend;
+
+  // neighbour list, ELL column-major: nbr[k*npad + i], k < nnb[i] <= nmax
+  DBuf<uint32_t> nbr;
+  DBuf<int> nnb;
+  int nmax = 0;
+  bool list_valid = false;
+  double list_rcut = 0, list_skin = 0;
+
+  // small device scalars + pinned host mirror
+  DBuf<unsigned long long> dscalar;  // [0] max count, [1] max displacement^2 bits
+  unsigned long long* hscalar = nullptr;
+  DBuf<double> red_part;  // thermo partial sums
+  DBuf<double> red_out;
+
+  bool has_lj = false;
+  double lj_eps = 0, lj_sigma = 0, lj_rcut = 0;
+  bool has_eam = false;
+  EamHost eam;
+
+  ~Ctx();
+};
+
+// ---- launches (k_neighbors.hip) ----
+void build_neighbor_list(Ctx& c, double rcut, double skin);
+double max_displacement2(Ctx& c);
+
+// ---- launches (k_integrate.hip) ----
+void launch_zero(Ctx& c);
+void launch_force_to_accel(Ctx& c);
+void launch_push_f_v_r(Ctx& c, double dt);
+void launch_push_f_v(Ctx& c, double dt);
+// fused first half of a step: r += v dt + a dt^2/2 ; v += a dt/2 ; max |r - r_ref|^2
+void launch_drift_kick(Ctx& c, double dt);
+void launch_thermo(Ctx& c, double out[27]);
+
+// ---- launches (k_pair.hip) ----
+struct ForceEpilogue {
+  bool fused = false;   // overwrite a with f/m and kick v by a*kick_dt (MD loop)
+  double kick_dt = 0;
+};
+void launch_lj(Ctx& c, double eps, double sigma, double rcut, unsigned flags, const ForceEpilogue& ep);
+void launch_eam(Ctx& c, double rcut, bool do_rho, bool do_rho2emb, bool do_force, unsigned flags,
+                const ForceEpilogue& ep);
+
+// ---- setfl + splines (setfl.cpp) ----
+void spline_interpolate(int n, double delta, const double* f, double* coeffs);
+void read_setfl(const char* path, int& nel, int& nrho, double& drho, int& nr, double& dr,
+                double& rcut, std::vector<double>& frho, std::vector<double>& rhor,
+                std::vector<double>& z2r, std::vector<double>& masses);
+void eam_upload(Ctx& c);
+
+// ---- helpers ----
+constexpr int kBlock = 256;
+inline unsigned grid_for(int64_t n, int block = kBlock)
+{
+  return (unsigned)((n + block - 1) / block);
+}
+
+}  // namespace xsmd

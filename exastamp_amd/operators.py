@@ -1,0 +1,296 @@
+"""exaStamp operators of the short-range force path, over libxsmd's C ABI.
+
+Each method of ``Simulation`` is named after the exaStamp/exaNBody operator it
+stands in for and takes that operator's slots as keyword arguments (units as
+deck quantities such as ``"0.0104 eV"`` are accepted wherever the reference
+takes a Quantity).  Where the reference aborts through fatal_error() this
+raises ``XsError`` (from the C ABI) or ``ValueError`` (slot validation).
+
+    sim = Simulation()
+    sim.species([("Cu", 63.546)])
+    sim.set_config(cfg)
+    sim.eam_alloy_init(parameters=tables)            # or {"file": "Cu.eam.alloy"}
+    sim.chunk_neighbors(rcut_max=4.95, rcut_inc=0.3)
+    sim.zero_force_energy()
+    sim.eam_alloy_force(rcut=4.95, trigger_thermo_state=True, compute_virial=True)
+    sim.force_to_accel()
+"""
+from __future__ import annotations
+
+import ctypes as C
+import dataclasses
+from pathlib import Path
+
+import numpy as np
+
+from . import _lib
+from ._lib import XS_ENERGY, XS_MIXED, XS_VIRIAL, XsError
+from .inputs import Config
+from .potentials import SetflTables
+from .units import EV, KB, PRESSURE_UNIT_PA, parse_quantity
+
+
+@dataclasses.dataclass
+class ThermodynamicState:
+    """simulation_thermodynamic_state outputs (simulation_thermodynamic_state.cpp:165-225)."""
+    raw: np.ndarray                 # the 27 reduced doubles
+    volume: float
+
+    @property
+    def virial(self) -> np.ndarray:
+        return self.raw[0:9].reshape(3, 3)
+
+    @property
+    def kinetic_tensor(self) -> np.ndarray:
+        return self.raw[9:18].reshape(3, 3)
+
+    @property
+    def momentum(self) -> np.ndarray:
+        return self.raw[18:21]
+
+    @property
+    def kinetic_energy(self) -> float:
+        return float(self.raw[21:24].sum())
+
+    @property
+    def potential_energy(self) -> float:
+        return float(self.raw[24])
+
+    @property
+    def mass(self) -> float:
+        return float(self.raw[25])
+
+    @property
+    def particle_count(self) -> int:
+        return int(self.raw[26])
+
+    @property
+    def total_energy(self) -> float:
+        return self.kinetic_energy + self.potential_energy
+
+    @property
+    def temperature(self) -> float:
+        """Kelvin: 2 (KE - p^2/2M) / (3 N kB)."""
+        ke = self.raw[21:24] - 0.5 * self.momentum ** 2 / self.mass
+        return float(2.0 * ke.sum() / (3.0 * self.particle_count * KB))
+
+    @property
+    def pressure_pa(self) -> float:
+        """Hydrostatic pressure (kinetic tensor + virial) / V, in Pa."""
+        kt = 2.0 * (self.kinetic_tensor - 0.5 * np.outer(self.momentum, self.momentum) / self.mass)
+        p = (np.trace(kt) + np.trace(self.virial)) / (3.0 * self.volume)
+        return float(p * PRESSURE_UNIT_PA)
+
+    def per_atom_ev(self, e: float) -> float:
+        return e / EV / self.particle_count
+
+
+class Simulation:
+    def __init__(self, device: int = 0):
+        self.ctx = _lib.Context(device)
+        self.species_names: list[str] = []
+        self.masses: list[float] = []
+        self.n = 0
+        self.cfg: Config | None = None
+        self.rcut_max = 0.0
+        self.lj_params: tuple[float, float] | None = None
+        self.eam: SetflTables | None = None
+
+    # --- set-up -------------------------------------------------------------
+    def species(self, spec):
+        """particle_type_add_properties / ParticleSpecies: [(name, mass), ...]."""
+        self.species_names = [s[0] for s in spec]
+        self.masses = [parse_quantity(s[1]) for s in spec]
+        m = np.asarray(self.masses, dtype=np.float64)
+        self.ctx.xs_species_set(len(self.masses), _lib.dptr(m))
+
+    def set_config(self, cfg: Config):
+        """Load atoms (what setup_system + init_particles produce)."""
+        r = np.ascontiguousarray(cfg.r, dtype=np.float64)
+        v = np.ascontiguousarray(cfg.v, dtype=np.float64)
+        cols = [np.ascontiguousarray(r[:, a]) for a in range(3)] + [np.ascontiguousarray(v[:, a]) for a in range(3)]
+        t = np.ascontiguousarray(cfg.type, dtype=np.uint8)
+        rep = None if cfg.rep is None else np.ascontiguousarray(cfg.rep, dtype=np.uint16)
+        lo = np.ascontiguousarray(cfg.lo, dtype=np.float64).ravel()
+        hi = np.ascontiguousarray(cfg.hi, dtype=np.float64).ravel()
+        per = (C.c_int * 3)(*[int(p) for p in cfg.periodic])
+        self.ctx.xs_system_set(cfg.n, *[_lib.dptr(c) for c in cols], _lib.ptr(t, C.c_uint8),
+                               _lib.ptr(rep, C.c_uint16), cfg.nrep, _lib.dptr(lo), _lib.dptr(hi), per)
+        self.n = cfg.n
+        self.cfg = cfg
+        self._keep = cols  # keep host buffers alive until the copy completed
+
+    # --- neighbour list -------------------------------------------------------
+    def chunk_neighbors(self, rcut_max=None, rcut_inc=0.0, config=None):
+        """chunk_neighbors (config_move_particles.msp:54-68); rcut_inc is the skin."""
+        config = config or {}
+        chunk = int(config.get("chunk_size", 1))
+        rmax = parse_quantity(rcut_max) if rcut_max is not None else self.rcut_max
+        if rmax <= 0:
+            raise ValueError("chunk_neighbors needs rcut_max > 0 (run a force operator first or pass it)")
+        self.ctx.xs_chunk_neighbors(rmax, parse_quantity(rcut_inc), chunk)
+
+    def particle_displ_over(self, threshold) -> bool:
+        out = C.c_int(0)
+        self.ctx.xs_particle_displ_over(parse_quantity(threshold), C.byref(out))
+        return bool(out.value)
+
+    # --- operators ------------------------------------------------------------
+    def zero_force_energy(self):
+        self.ctx.xs_zero_force_energy()
+
+    def force_to_accel(self):
+        self.ctx.xs_force_to_accel()
+
+    @staticmethod
+    def _flags(trigger_thermo_state, compute_virial, mixed):
+        f = 0
+        if trigger_thermo_state:
+            f |= XS_ENERGY
+            if compute_virial:
+                f |= XS_VIRIAL
+        if mixed:
+            f |= XS_MIXED
+        return f
+
+    def lj_compute_force(self, parameters, rcut, ghost=False, compute_virial=True,
+                         trigger_thermo_state=True, mixed=False, **_slots):
+        """lj_compute_force{parameters: {epsilon, sigma}, rcut} (pair_potential_impl.hxx:108-121).
+        The reference always fills ep (and the virial when the grid has one); here
+        they are filled when trigger_thermo_state (and compute_virial) are set."""
+        eps = parse_quantity(parameters["epsilon"])
+        sig = parse_quantity(parameters["sigma"])
+        rc = parse_quantity(rcut)
+        self.rcut_max = max(self.rcut_max, rc)
+        self.lj_params = (eps, sig)
+        self.ctx.xs_lj_compute_force(eps, sig, rc, self._flags(trigger_thermo_state, compute_virial, mixed))
+
+    lj_compute_force_symetric = lj_compute_force  # full list gives the same forces
+
+    def lj_parameters(self, parameters, rcut):
+        """Set the LJ parameters slot without computing (for md_run)."""
+        eps = parse_quantity(parameters["epsilon"])
+        sig = parse_quantity(parameters["sigma"])
+        rc = parse_quantity(rcut)
+        self.rcut_max = max(self.rcut_max, rc)
+        self.lj_params = (eps, sig)
+        self.ctx.xs_lj_parameters(eps, sig, rc)
+
+    def eam_alloy_init(self, parameters):
+        """eam_alloy_init{parameters: {file: ...}} or an in-memory SetflTables."""
+        if isinstance(parameters, SetflTables):
+            t = parameters
+        else:
+            f = parameters["file"] if isinstance(parameters, dict) else parameters
+            from .potentials import read_setfl
+            t = read_setfl(Path(f))
+        nz = t.nel * (t.nel + 1) // 2
+        if t.z2r.shape != (nz, t.nr):
+            raise ValueError("z2r table count does not match the element count")
+        fr = np.ascontiguousarray(t.frho, dtype=np.float64)
+        rr = np.ascontiguousarray(t.rhor, dtype=np.float64)
+        zr = np.ascontiguousarray(t.z2r, dtype=np.float64)
+        self.ctx.xs_eam_alloy_tables(t.nel, t.nrho, t.drho, t.nr, t.dr, t.rc, _lib.dptr(fr), _lib.dptr(rr),
+                                     _lib.dptr(zr))
+        self.eam = t
+
+    def eam_alloy_force(self, rcut, eam_rho=True, eam_rho2emb=True, eam_ghost=True, eam_force=True,
+                        eam_symmetry=False, compute_virial=False, trigger_thermo_state=False,
+                        mixed=False, types=None, **_slots):
+        """eam_alloy_force (eam_potential_multimat.cu:88-109 slots).  eam_ghost and
+        eam_symmetry select ghost / Newton-symmetric variants in the reference; the
+        full-list GPU path computes the same quantities for either value."""
+        if types:
+            unknown = [t for t in types if t not in self.species_names]
+            if unknown:
+                raise ValueError(f"eam_alloy_force: unknown types {unknown}")
+        rc = parse_quantity(rcut)
+        self.rcut_max = max(self.rcut_max, rc)
+        flags = self._flags(trigger_thermo_state, compute_virial, mixed)
+        self.ctx.xs_eam_alloy_force(rc, int(bool(eam_rho)), int(bool(eam_rho2emb)), int(bool(eam_force)), flags)
+
+    def push_f_v_r(self, dt, dt_scale=1.0, xform_mode="INV_XFORM"):
+        self.ctx.xs_push_f_v_r(parse_quantity(dt), float(dt_scale))
+
+    def push_f_v(self, dt, dt_scale=1.0, xform_mode="IDENTITY"):
+        self.ctx.xs_push_f_v(parse_quantity(dt), float(dt_scale))
+
+    def simulation_thermodynamic_state(self) -> ThermodynamicState:
+        out = np.zeros(27)
+        self.ctx.xs_thermo(_lib.dptr(out))
+        return ThermodynamicState(out, self.volume())
+
+    def volume(self) -> float:
+        c = self.cfg
+        return float(np.prod(c.hi - c.lo, axis=1).sum()) if c is not None else 0.0
+
+    # --- fused NVE loop -------------------------------------------------------
+    def md_run(self, potential, nsteps, dt, rcut, skin, mixed=False, thermo_every=0):
+        """verlet_nve for nsteps (with first_iteration); returns (stats, thermo list)."""
+        pot = {"lj": _lib.XS_POT_LJ, "eam": _lib.XS_POT_EAM}[potential]
+        nth = (nsteps // thermo_every + 1) if thermo_every > 0 else 0
+        out = np.zeros(max(nth, 1) * 27)
+        st = _lib.RunStats()
+        flags = XS_MIXED if mixed else 0
+        self.ctx.xs_md_run(pot, int(nsteps), parse_quantity(dt), parse_quantity(rcut), parse_quantity(skin),
+                           flags, int(thermo_every), _lib.dptr(out) if nth else None, C.byref(st))
+        states = [ThermodynamicState(out[27 * k:27 * k + 27].copy(), self.volume()) for k in range(nth)]
+        return st, states
+
+    # --- read back (id order) --------------------------------------------------
+    def state(self):
+        n = self.n
+        arrs = [np.zeros(n) for _ in range(6)]
+        self.ctx.xs_system_get(*[_lib.dptr(a) for a in arrs])
+        return np.stack(arrs[:3], axis=1), np.stack(arrs[3:], axis=1)
+
+    def set_velocities(self, v):
+        cols = [np.ascontiguousarray(v[:, a], dtype=np.float64) for a in range(3)]
+        self.ctx.xs_velocity_set(*[_lib.dptr(c) for c in cols])
+
+    def accelerations(self) -> np.ndarray:
+        out = np.zeros(3 * self.n)
+        self.ctx.xs_field_get(_lib.XS_FIELD_ACCEL, _lib.dptr(out))
+        return out.reshape(3, self.n).T.copy()
+
+    def forces(self) -> np.ndarray:
+        """Accelerations times masses (valid after force_to_accel)."""
+        m = np.asarray(self.masses)[self.cfg.type]
+        return self.accelerations() * m[:, None]
+
+    def ep(self) -> np.ndarray:
+        out = np.zeros(self.n)
+        self.ctx.xs_field_get(_lib.XS_FIELD_EP, _lib.dptr(out))
+        return out
+
+    def virial(self) -> np.ndarray:
+        out = np.zeros(9 * self.n)
+        self.ctx.xs_field_get(_lib.XS_FIELD_VIRIAL, _lib.dptr(out))
+        return out.reshape(self.n, 3, 3)
+
+    def rho_dEmb(self) -> np.ndarray:
+        out = np.zeros(self.n)
+        self.ctx.xs_field_get(_lib.XS_FIELD_RHO_DEMB, _lib.dptr(out))
+        return out
+
+    def neighbors(self):
+        """(counts[n], ids[total], imgs[total,3]) in id space, entries in list order."""
+        counts = np.zeros(self.n, dtype=np.int64)
+        total = C.c_int64(0)
+        self.ctx.xs_neighbors_get(_lib.ptr(counts, C.c_int64), C.byref(total), None, None)
+        ids = np.zeros(total.value, dtype=np.int64)
+        imgs = np.zeros((total.value, 3), dtype=np.int32)
+        self.ctx.xs_neighbors_get(_lib.ptr(counts, C.c_int64), C.byref(total), _lib.ptr(ids, C.c_int64),
+                                  _lib.ptr(imgs, C.c_int32))
+        return counts, ids, imgs
+
+    def order(self) -> np.ndarray:
+        perm = np.zeros(self.n, dtype=np.int64)
+        self.ctx.xs_order_get(_lib.ptr(perm, C.c_int64))
+        return perm
+
+    def close(self):
+        self.ctx.close()
+
+
+__all__ = ["Simulation", "ThermodynamicState", "XsError"]

@@ -1,0 +1,171 @@
+/*
+ * xsmd.h -- C ABI of libxsmd.so, the MI355X (gfx950) short-range force path
+ * that stands in for exaStamp's operators of the same names.
+ *
+ * Conventions
+ *  - Every entry point returns an xs_status (0 = XS_OK) and never aborts; the
+ *    message of the last failure on a context is xs_last_error(ctx).  The
+ *    reference operators call fatal_error()/std::abort() instead
+ *    (e.g. pair_potential_impl.hxx:135-147, eam_potential_multimat.cu:235-238).
+ *  - Host pointers are plain host memory and are copied in/out.  The only
+ *    device pointers are those named *_dev.
+ *  - Per-atom host arrays are indexed by particle id = the position the atom
+ *    had in the arrays given to xs_system_set.  Internally atoms live in
+ *    cell-sorted order, like exaNBody's grid, and are re-sorted at every
+ *    neighbour-list rebuild.
+ *  - Units are exaStamp's internal units (include/exaStamp/unit_system.h:26-36):
+ *    Angstrom, Dalton, picosecond; energy in Da.A^2/ps^2.
+ *  - Calls on one context are ordered on the context's HIP stream and are not
+ *    thread-safe.  Separate contexts may be used from separate threads.
+ *  - One context holds one system made of nrep independent periodic boxes
+ *    ("replicas"), used for batches of small configurations; nrep = 1 is the
+ *    usual single box.
+ */
+#ifndef XSMD_H
+#define XSMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct xs_ctx xs_ctx;
+
+typedef enum {
+  XS_OK = 0,
+  XS_E_ARG = 1,      /* invalid argument */
+  XS_E_STATE = 2,    /* call out of order (no system, no list, no tables...) */
+  XS_E_HIP = 3,      /* HIP runtime error */
+  XS_E_ALLOC = 4,    /* device allocation failed */
+  XS_E_IO = 5,       /* file could not be read or parsed */
+  XS_E_LIMIT = 6     /* a size limit of this implementation was exceeded */
+} xs_status;
+
+/* compute flags */
+#define XS_ENERGY 1u   /* per-atom potential energy ep */
+#define XS_VIRIAL 2u   /* per-atom virial tensor (9 components, row-major) */
+#define XS_MIXED 4u    /* FP32 pair/spline math; FP64 positions, displacements,
+                          accumulation and reductions.  Without it: all FP64. */
+
+/* potentials for xs_md_run */
+#define XS_POT_LJ 0
+#define XS_POT_EAM 1
+
+/* per-atom fields for xs_field_get */
+#define XS_FIELD_EP 0        /* n doubles */
+#define XS_FIELD_VIRIAL 1    /* 9n doubles */
+#define XS_FIELD_RHO_DEMB 2  /* n doubles: rho after the density pass, F'(rho) after rho2emb */
+#define XS_FIELD_ACCEL 3     /* 3n doubles (x then y then z blocks) */
+
+int xs_version(void);
+
+/* --- context --------------------------------------------------------------- */
+/* Replaces the device set-up of hw_device_init/init_cuda (data/config/main-config.msp:87-91). */
+xs_status xs_ctx_create(int device, xs_ctx** out);
+xs_status xs_ctx_destroy(xs_ctx* ctx);
+const char* xs_last_error(const xs_ctx* ctx);
+/* Wait for all work queued on the context's stream. */
+xs_status xs_synchronize(xs_ctx* ctx);
+
+/* --- species and system ------------------------------------------------------ */
+/* Masses per atom type (ParticleSpecies, src/particle_species/include/exaStamp/
+ * particle_species/particle_specie.h:52-122).  1 <= nspecies <= 7. */
+xs_status xs_species_set(xs_ctx* ctx, int nspecies, const double* masses);
+
+/* Load the atoms (replaces the grid filled by setup_system/lattice,
+ * src/setup_system/bulk_lattice.cpp:17-31, and the rx..vz/type fields of
+ * include/exaStamp/fields.h:64,69-79).  rep (may be NULL when nrep == 1) gives
+ * the replica of each atom; lo/hi hold 3*nrep box bounds; periodic[3] applies
+ * to all replicas.  Accelerations, energies and virials are zeroed. */
+xs_status xs_system_set(xs_ctx* ctx, int64_t n, const double* rx, const double* ry,
+                        const double* rz, const double* vx, const double* vy,
+                        const double* vz, const uint8_t* type, const uint16_t* rep,
+                        int nrep, const double* lo, const double* hi, const int periodic[3]);
+int64_t xs_system_size(const xs_ctx* ctx);
+/* Copy state back in id order; any pointer may be NULL. */
+xs_status xs_system_get(xs_ctx* ctx, double* rx, double* ry, double* rz, double* vx,
+                        double* vy, double* vz);
+xs_status xs_field_get(xs_ctx* ctx, int field, double* out);
+/* Overwrite velocities (id order), e.g. for time-reversal tests. */
+xs_status xs_velocity_set(xs_ctx* ctx, const double* vx, const double* vy, const double* vz);
+
+/* --- neighbour list ------------------------------------------------------------ */
+/* chunk_neighbors (exaNBody; slots data/config/config_move_particles.msp:54-68,
+ * consumed at eam_potential_multimat.cu:94,180-181 and pair_potential_impl.hxx:111):
+ * wrap atoms into the box, re-sort them by cell (move_particles), then build the
+ * full per-atom list of every (atom, periodic image) with d^2 < (rcut_max+rcut_inc)^2.
+ * The list is atom-granular, the chunk_size = 1 form; chunk_size must be a power of
+ * two and any value gives identical interaction sets (a larger chunk only widens
+ * the reference's list, never the set within rcut). */
+xs_status xs_chunk_neighbors(xs_ctx* ctx, double rcut_max, double rcut_inc, int chunk_size);
+/* particle_displ_over (config_move_particles.msp:19-23): *result = 1 when some atom
+ * moved more than threshold since the last list build. */
+xs_status xs_particle_displ_over(xs_ctx* ctx, double threshold, int* result);
+/* Neighbour list in id space for checking: counts[n] (entries per atom); returns the
+ * total through *total.  ids/imgs (3 ints per entry) may be NULL; entries of one atom
+ * are in list order, atoms in id order. */
+xs_status xs_neighbors_get(xs_ctx* ctx, int64_t* counts, int64_t* total, int64_t* ids,
+                           int32_t* imgs);
+/* perm[k] = id of the atom at internal (cell-sorted) position k. */
+xs_status xs_order_get(xs_ctx* ctx, int64_t* perm);
+
+/* --- operators -------------------------------------------------------------------- */
+/* zero_force_energy (src/compute/zero_force_energy.cu:38-147): a = ep = virial = 0. */
+xs_status xs_zero_force_energy(xs_ctx* ctx);
+/* force_to_accel (src/compute/force_to_accel.cu:38-122): a /= mass[type]. */
+xs_status xs_force_to_accel(xs_ctx* ctx);
+/* lj_compute_force (pair_potential_impl.hxx:39-513 with lennard_jones.h:33-50):
+ * adds LJ forces, shifted energies (e - E(rcut)) and virials to the atoms. */
+xs_status xs_lj_compute_force(xs_ctx* ctx, double epsilon, double sigma, double rcut,
+                              unsigned flags);
+/* The parameters slot of lj_compute_force alone (LennardJonesParms, lennard_jones.h:33-37,
+ * decoded at :60-71), for xs_md_run. */
+xs_status xs_lj_parameters(xs_ctx* ctx, double epsilon, double sigma, double rcut);
+/* eam_alloy_init + parameters (eam_alloy.cpp:66-278): tabulated knots in setfl
+ * layout.  frho: nel*nrho values, rhor: nel*nr, z2r: nel(nel+1)/2*nr (packed lower
+ * triangle 11,21,22,31,...), z2r = r*phi in eV.A, F in eV. */
+xs_status xs_eam_alloy_tables(xs_ctx* ctx, int nel, int nrho, double drho, int nr, double dr,
+                              double rcut, const double* frho, const double* rhor,
+                              const double* z2r);
+/* Same from a setfl (.eam.alloy) file; masses (nullable, >= 7 doubles) and the
+ * element count are returned. */
+xs_status xs_eam_alloy_load(xs_ctx* ctx, const char* path, double* masses, int* nel);
+/* eam_alloy_force (eam_potential_multimat.cu:65-259) with its eam_rho / eam_rho2emb /
+ * eam_force slots; eam_ghost and eam_symmetry are not needed (no ghosts; full list). */
+xs_status xs_eam_alloy_force(xs_ctx* ctx, double rcut, int eam_rho, int eam_rho2emb,
+                             int eam_force, unsigned flags);
+/* push_f_v_r (config_numerical_schemes.msp:23-25): r += v*dt + a*dt^2/2,
+ * dt = dt * dt_scale. */
+xs_status xs_push_f_v_r(xs_ctx* ctx, double dt, double dt_scale);
+/* push_f_v (config_numerical_schemes.msp:24-28): v += a*dt, dt = dt * dt_scale. */
+xs_status xs_push_f_v(xs_ctx* ctx, double dt, double dt_scale);
+/* simulation_thermodynamic_state reduction (src/thermo_state/
+ * simulation_thermodynamic_state.cpp:101-163): out[27] = virial(9), kinetic tensor
+ * (9), momentum(3), kinetic energy(3), potential energy, mass, particle count. */
+xs_status xs_thermo(xs_ctx* ctx, double out27[27]);
+
+/* --- fused NVE loop ---------------------------------------------------------------- */
+typedef struct xs_run_stats {
+  int64_t steps;
+  int64_t rebuilds;     /* neighbour-list builds including the first */
+  int64_t max_neighbors;
+  double seconds;       /* wall time of the loop (GPU synchronised) */
+} xs_run_stats;
+
+/* verlet_nve (config_numerical_schemes.msp:44-52) for nsteps steps, starting with
+ * first_iteration's force computation (main-config.msp:145-152), rebuilding the list
+ * when an atom moved more than skin/2.  The potential was set by a prior
+ * xs_lj_compute_force / xs_eam_alloy_tables call.  flags: XS_MIXED selects the
+ * precision; when thermo_every > 0, energies and virials are computed on every
+ * thermo_every-th step (and step 0) and the 27-double state is written to
+ * thermo_out[(step/thermo_every)*27].  stats may be NULL. */
+xs_status xs_md_run(xs_ctx* ctx, int potential, int nsteps, double dt, double rcut,
+                    double skin, unsigned flags, int thermo_every, double* thermo_out,
+                    xs_run_stats* stats);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* XSMD_H */

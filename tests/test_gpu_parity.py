@@ -1,0 +1,403 @@
+"""GPU parity: the HIP path through the C ABI against the FP64 CPU oracle.
+
+Tolerances (stated per quantity):
+  neighbour lists          bit-exact: same atom order, same entries in the same order
+  FP64 forces              max_i |F_gpu - F_ref| / rms|F_ref| <= 1e-11
+  FP64 energy / virial     relative <= 1e-12 (sums), per-atom <= 1e-10 of the rms
+  mixed forces             max_i |F_mixed - F_ref| / rms|F_ref| <= 1e-5
+  NVE trajectories (FP64)  per-step potential/kinetic energy relative <= 1e-9
+"""
+import numpy as np
+import pytest
+
+from conftest import cu_tables, cuni_tables
+
+pytestmark = pytest.mark.gpu
+
+AR_MASS = 39.948
+CU_MASS = 63.546
+
+
+def _units():
+    from exastamp_amd import units
+    return units
+
+
+def lj_c1_config(seed=11, cells=(10, 10, 10)):
+    from exastamp_amd import inputs
+    return inputs.lattice_config(cells, 5.26, [AR_MASS], T=100.0, seed=seed)
+
+
+def cu_config(cells=(10, 10, 10), T=600.0, jitter=0.05, seed=3):
+    from exastamp_amd import inputs
+    return inputs.lattice_config(cells, 3.615, [CU_MASS], T=T, jitter=jitter, seed=seed)
+
+
+def make_sim(cfg, masses, names=None):
+    from exastamp_amd.operators import Simulation
+    sim = Simulation(0)
+    names = names or [f"T{k}" for k in range(len(masses))]
+    sim.species(list(zip(names, masses)))
+    sim.set_config(cfg)
+    return sim
+
+
+def make_oracle(oracle_mod, cfg, masses):
+    o = oracle_mod.Oracle()
+    o.set_config(cfg, masses)
+    return o
+
+
+def frel(f, fref):
+    return float(np.abs(f - fref).max() / np.sqrt((fref ** 2).sum(1).mean()))
+
+
+def lj_params():
+    u = _units()
+    return 0.0104 * u.EV, 3.4, 8.5
+
+
+# ----------------------------------------------------------------------------
+# neighbour lists
+# ----------------------------------------------------------------------------
+
+def _compare_lists(sim, o):
+    assert np.array_equal(sim.order(), o.order()), "cell-sorted atom order differs"
+    c1, i1, m1 = sim.neighbors()
+    c2, i2, m2 = o.neighbors()
+    assert np.array_equal(c1, c2)
+    assert np.array_equal(i1, i2)
+    assert np.array_equal(m1, m2)
+    return c1, i1, m1
+
+
+def _sorted_sets(counts, ids, imgs):
+    off = np.concatenate([[0], np.cumsum(counts)])
+    out = []
+    for a in range(counts.size):
+        s = slice(off[a], off[a + 1])
+        key = np.lexsort((imgs[s, 0], imgs[s, 1], imgs[s, 2], ids[s]))
+        out.append((ids[s][key], imgs[s][key]))
+    return out
+
+
+def test_neighbor_list_bitexact_lj_c1(oracle_mod):
+    eps, sig, rc = lj_params()
+    cfg = lj_c1_config()
+    sim = make_sim(cfg, [AR_MASS])
+    sim.chunk_neighbors(rcut_max=rc, rcut_inc=0.3)
+    o = make_oracle(oracle_mod, cfg, [AR_MASS])
+    o.build_neighbors(rc, 0.3)
+    c, ids, imgs = _compare_lists(sim, o)
+    # against brute force over all pairs and images (set equality per atom)
+    cb, ib, mb = o.brute_neighbors(rc + 0.3)
+    assert np.array_equal(c, cb)
+    for (a1, b1), (a2, b2) in zip(_sorted_sets(c, ids, imgs), _sorted_sets(cb, ib, mb)):
+        assert np.array_equal(a1, a2) and np.array_equal(b1, b2)
+    assert c.min() >= 78  # 12+6+24+12+24 FCC shells within 8.8 A
+
+
+@pytest.mark.parametrize("chunk_size", [1, 4])
+def test_neighbor_list_bitexact_eam_jitter(oracle_mod, chunk_size):
+    cfg = cu_config((9, 8, 7), jitter=0.12, seed=5)
+    sim = make_sim(cfg, [CU_MASS])
+    sim.chunk_neighbors(rcut_max=4.95, rcut_inc=0.3, config={"chunk_size": chunk_size})
+    o = make_oracle(oracle_mod, cfg, [CU_MASS])
+    o.build_neighbors(4.95, 0.3)
+    _compare_lists(sim, o)
+
+
+def test_rebuild_after_motion_bitexact(oracle_mod):
+    """Second build after atoms moved and crossed the periodic boundary."""
+    cfg = cu_config((6, 6, 6), jitter=0.05, seed=8)
+    sim = make_sim(cfg, [CU_MASS])
+    o = make_oracle(oracle_mod, cfg, [CU_MASS])
+    sim.chunk_neighbors(rcut_max=4.95, rcut_inc=0.3)
+    o.build_neighbors(4.95, 0.3)
+    # drift everything by a large velocity (free flight, zero acceleration)
+    v = np.tile([[31.0, -17.0, 9.0]], (cfg.n, 1))
+    sim.set_velocities(v)
+    from exastamp_amd import inputs
+    cfg2 = inputs.Config(cfg.r, v, cfg.type, cfg.lo, cfg.hi)
+    o.set_config(cfg2, [CU_MASS])
+    o.build_neighbors(4.95, 0.3)
+    # oracle needs the same pre-build order as the GPU: replay the GPU history
+    sim.push_f_v_r(0.01)
+    o.push_f_v_r(0.01)
+    assert sim.particle_displ_over(0.15) == o.displ_over(0.15) == True  # noqa: E712
+    sim.chunk_neighbors(rcut_max=4.95, rcut_inc=0.3)
+    o.build_neighbors(4.95, 0.3)
+    _compare_lists(sim, o)
+    r_gpu, _ = sim.state()
+    assert np.array_equal(r_gpu, o.state()["r"])
+
+
+# ----------------------------------------------------------------------------
+# Lennard-Jones
+# ----------------------------------------------------------------------------
+
+def _lj_pair(oracle_mod, mixed):
+    eps, sig, rc = lj_params()
+    cfg = lj_c1_config(seed=12)
+    from exastamp_amd import inputs
+    cfg = inputs.Config(cfg.r + inputs.rng(99).normal(scale=0.08, size=cfg.r.shape), cfg.v, cfg.type, cfg.lo,
+                        cfg.hi)
+    sim = make_sim(cfg, [AR_MASS])
+    sim.chunk_neighbors(rcut_max=rc, rcut_inc=0.3)
+    sim.zero_force_energy()
+    sim.lj_compute_force(parameters={"epsilon": "0.0104 eV", "sigma": "3.4 ang"}, rcut="8.5 ang",
+                         trigger_thermo_state=True, compute_virial=True, mixed=mixed)
+    sim.force_to_accel()
+    o = make_oracle(oracle_mod, cfg, [AR_MASS])
+    o.set_lj(eps, sig, rc)
+    o.build_neighbors(rc, 0.3)
+    o.compute_forces("lj")
+    return sim, o
+
+
+def test_lj_fp64_matches_oracle(oracle_mod):
+    sim, o = _lj_pair(oracle_mod, mixed=False)
+    ref = o.state()
+    assert frel(sim.forces(), o.forces()) <= 1e-11
+    ep, ep_ref = sim.ep(), ref["ep"]
+    assert abs(ep.sum() - ep_ref.sum()) <= 1e-12 * abs(ep_ref.sum())
+    assert np.abs(ep - ep_ref).max() <= 1e-10 * np.sqrt((ep_ref ** 2).mean())
+    vir, vref = sim.virial().sum(0), ref["virial"].sum(0)
+    assert np.abs(vir - vref).max() <= 1e-12 * np.abs(vref).max()
+
+
+def test_lj_mixed_within_tolerance(oracle_mod):
+    sim, o = _lj_pair(oracle_mod, mixed=True)
+    ref = o.state()
+    err = frel(sim.forces(), o.forces())
+    print(f"LJ mixed max|dF|/rms|F| = {err:.3e}")
+    assert err <= 1e-5
+    assert abs(sim.ep().sum() - ref["ep"].sum()) <= 1e-6 * abs(ref["ep"].sum())
+
+
+def test_lj_nve_c1_trajectory(oracle_mod):
+    """C1: LJ Ar 4000 atoms, 100 NVE steps, dt = 2 fs, per-step energies and virial."""
+    eps, sig, rc = lj_params()
+    cfg = lj_c1_config()
+    sim = make_sim(cfg, [AR_MASS])
+    sim.lj_parameters(parameters={"epsilon": eps, "sigma": sig}, rcut=rc)
+    st, th = sim.md_run("lj", 100, 0.002, rc, 0.3, thermo_every=1)
+    o = make_oracle(oracle_mod, cfg, [AR_MASS])
+    o.set_lj(eps, sig, rc)
+    nreb, th_ref = o.md_run("lj", 100, 0.002, rc, 0.3)
+    assert st.rebuilds == nreb
+    pe = np.array([t.potential_energy for t in th])
+    ke = np.array([t.kinetic_energy for t in th])
+    vtr = np.array([np.trace(t.virial) for t in th])
+    assert np.abs(pe - th_ref[:, 24]).max() <= 1e-9 * np.abs(th_ref[:, 24]).max()
+    assert np.abs(ke - th_ref[:, 21:24].sum(1)).max() <= 1e-9 * th_ref[:, 21:24].sum(1).max()
+    vref = th_ref[:, 0] + th_ref[:, 4] + th_ref[:, 8]
+    assert np.abs(vtr - vref).max() <= 1e-8 * np.abs(vref).max()
+    r, v = sim.state()
+    assert np.abs(r - o.state()["r"]).max() < 1e-8
+
+
+# ----------------------------------------------------------------------------
+# EAM
+# ----------------------------------------------------------------------------
+
+def _eam_pair(oracle_mod, cfg, tab, masses, mixed, names=None):
+    sim = make_sim(cfg, masses, names)
+    sim.eam_alloy_init(tab)
+    sim.chunk_neighbors(rcut_max=tab.rc, rcut_inc=0.3)
+    sim.zero_force_energy()
+    sim.eam_alloy_force(rcut=tab.rc, trigger_thermo_state=True, compute_virial=True, mixed=mixed)
+    sim.force_to_accel()
+    o = make_oracle(oracle_mod, cfg, masses)
+    o.set_eam(tab)
+    o.build_neighbors(tab.rc, 0.3)
+    o.compute_forces("eam")
+    return sim, o
+
+
+def test_eam_fp64_matches_oracle(oracle_mod):
+    tab = cu_tables()
+    sim, o = _eam_pair(oracle_mod, cu_config(), tab, [CU_MASS], mixed=False)
+    ref = o.state()
+    assert frel(sim.forces(), o.forces()) <= 1e-11
+    ep, ep_ref = sim.ep(), ref["ep"]
+    assert abs(ep.sum() - ep_ref.sum()) <= 1e-12 * abs(ep_ref.sum())
+    assert np.abs(ep - ep_ref).max() <= 1e-10 * np.sqrt((ep_ref ** 2).mean())
+    vir, vref = sim.virial().sum(0), ref["virial"].sum(0)
+    assert np.abs(vir - vref).max() <= 1e-10 * np.abs(vref).max()
+    fp, fp_ref = sim.rho_dEmb(), ref["rho_dEmb"]
+    assert np.abs(fp - fp_ref).max() <= 1e-12 * np.abs(fp_ref).max()
+
+
+def test_eam_mixed_within_tolerance(oracle_mod):
+    tab = cu_tables()
+    sim, o = _eam_pair(oracle_mod, cu_config(), tab, [CU_MASS], mixed=True)
+    ref = o.state()
+    err = frel(sim.forces(), o.forces())
+    print(f"EAM mixed max|dF|/rms|F| = {err:.3e}")
+    assert err <= 1e-5
+    assert abs(sim.ep().sum() - ref["ep"].sum()) <= 1e-6 * abs(ref["ep"].sum())
+
+
+def test_eam_split_passes_equal_fused():
+    """eam_rho / eam_rho2emb / eam_force as three operator calls (the split deck of
+    data/regression_new/potentials/eam/eam_alloy/single_specy_nosym_cs1.msp) give the
+    same bits as one call with all passes."""
+    tab = cu_tables()
+    cfg = cu_config((7, 7, 7), seed=21)
+    sim = make_sim(cfg, [CU_MASS])
+    sim.eam_alloy_init(tab)
+    sim.chunk_neighbors(rcut_max=4.95, rcut_inc=0.3)
+    for mixed in (False, True):
+        sim.zero_force_energy()
+        sim.eam_alloy_force(rcut=4.95, trigger_thermo_state=True, mixed=mixed)
+        f1, e1 = sim.accelerations(), sim.ep()
+        sim.zero_force_energy()
+        kw = dict(rcut=4.95, trigger_thermo_state=True, mixed=mixed)
+        sim.eam_alloy_force(eam_rho=True, eam_rho2emb=False, eam_force=False, **kw)
+        sim.eam_alloy_force(eam_rho=False, eam_rho2emb=True, eam_force=False, **kw)
+        sim.eam_alloy_force(eam_rho=False, eam_rho2emb=False, eam_force=True, **kw)
+        assert np.array_equal(f1, sim.accelerations())
+        assert np.array_equal(e1, sim.ep())
+
+
+def test_eam_operator_loop_equals_md_run():
+    """Operator-by-operator verlet_nve (push_f_v_r, push_f_v, particle_displ_over,
+    chunk_neighbors, zero_force_energy, eam_alloy_force, force_to_accel, push_f_v)
+    reproduces the fused xs_md_run bit for bit."""
+    tab = cu_tables()
+    cfg = cu_config((6, 6, 6), T=900.0, seed=4)
+    dt, rc, skin = 0.001, 4.95, 0.3
+    a = make_sim(cfg, [CU_MASS])
+    a.eam_alloy_init(tab)
+    a.chunk_neighbors(rcut_max=rc, rcut_inc=skin)
+    a.zero_force_energy()
+    a.eam_alloy_force(rcut=rc)
+    a.force_to_accel()
+    for _ in range(30):
+        a.push_f_v_r(dt, 1.0)
+        a.push_f_v(dt, 0.5)
+        if a.particle_displ_over(0.5 * skin):
+            a.chunk_neighbors(rcut_max=rc, rcut_inc=skin)
+        a.zero_force_energy()
+        a.eam_alloy_force(rcut=rc)
+        a.force_to_accel()
+        a.push_f_v(dt, 0.5)
+    b = make_sim(cfg, [CU_MASS])
+    b.eam_alloy_init(tab)
+    b.md_run("eam", 30, dt, rc, skin)
+    ra, va = a.state()
+    rb, vb = b.state()
+    assert np.array_equal(ra, rb) and np.array_equal(va, vb)
+
+
+def test_eam_nve_vs_oracle(oracle_mod):
+    """EAM Cu 100 NVE steps, dt = 1 fs (C2 shape at 4000 atoms): FP64 GPU trajectory
+    energies vs the oracle, and mixed-precision energy conservation."""
+    tab = cu_tables()
+    cfg = cu_config((10, 10, 10), T=0.0, jitter=0.05, seed=2)
+    sim = make_sim(cfg, [CU_MASS])
+    sim.eam_alloy_init(tab)
+    st, th = sim.md_run("eam", 100, 0.001, 4.95, 0.3, thermo_every=1)
+    o = make_oracle(oracle_mod, cfg, [CU_MASS])
+    o.set_eam(tab)
+    nreb, th_ref = o.md_run("eam", 100, 0.001, 4.95, 0.3)
+    assert st.rebuilds == nreb
+    pe = np.array([t.potential_energy for t in th])
+    assert np.abs(pe - th_ref[:, 24]).max() <= 1e-9 * np.abs(th_ref[:, 24]).max()
+    etot64 = np.array([t.total_energy for t in th])
+    sim.set_config(cfg)
+    _, thm = sim.md_run("eam", 100, 0.001, 4.95, 0.3, mixed=True, thermo_every=1)
+    etotm = np.array([t.total_energy for t in thm])
+    drift64 = abs(etot64[-1] - etot64[0])
+    driftm = abs(etotm[-1] - etotm[0])
+    scale = np.abs(pe).max()
+    print(f"EAM NVE |dE|/|PE|: fp64 {drift64 / scale:.3e}, mixed {driftm / scale:.3e}")
+    # the target: mixed-precision NVE drift no worse than 2x the FP64 drift
+    assert driftm <= 2.0 * drift64 + 1e-9 * scale
+
+
+def test_eam_alloy_slab_two_species(oracle_mod):
+    """Cu/Ni, 50/50 substitution, vacancies, free surface along z (C5 shape, small)."""
+    from exastamp_amd import inputs
+    tab = cuni_tables()
+    masses = [63.546, 58.693]
+    cfg = inputs.alloy_slab_config((8, 8, 6), 3.57, masses, vacancy_frac=0.01, vacuum=10.0, T=500.0, seed=9)
+    sim, o = _eam_pair(oracle_mod, cfg, tab, masses, mixed=False, names=["Cu", "Ni"])
+    _compare_lists(sim, o)
+    assert frel(sim.forces(), o.forces()) <= 1e-11
+    assert abs(sim.ep().sum() - o.state()["ep"].sum()) <= 1e-12 * abs(o.state()["ep"].sum())
+    simm, _ = _eam_pair(oracle_mod, cfg, tab, masses, mixed=True, names=["Cu", "Ni"])
+    assert frel(simm.forces(), o.forces()) <= 1e-5
+
+
+def test_replica_batch_matches_per_box_oracle(oracle_mod):
+    """A batch of independently strained periodic boxes (the C4 layout) equals the
+    oracle run on each box alone."""
+    from exastamp_amd import inputs
+    tab = cu_tables(500)
+    batch = inputs.strained_batch(5, (4, 4, 4), 3.615, [CU_MASS], seed=13)
+    sim = make_sim(batch, [CU_MASS])
+    sim.eam_alloy_init(tab)
+    sim.chunk_neighbors(rcut_max=tab.rc, rcut_inc=0.3)
+    sim.zero_force_energy()
+    sim.eam_alloy_force(rcut=tab.rc, trigger_thermo_state=True)
+    sim.force_to_accel()
+    f = sim.forces()
+    ep = sim.ep()
+    for k, box in enumerate(inputs.split_replicas(batch)):
+        o = make_oracle(oracle_mod, box, [CU_MASS])
+        o.set_eam(tab)
+        o.build_neighbors(tab.rc, 0.3)
+        o.compute_forces("eam")
+        sel = batch.rep == k
+        assert frel(f[sel], o.forces()) <= 1e-11
+        assert abs(ep[sel].sum() - o.state()["ep"].sum()) <= 1e-12 * abs(o.state()["ep"].sum())
+
+
+def test_eam_c3_scale_properties():
+    """EAM Cu at the C3 size (128^3 cells, 8 388 608 atoms): perfect lattice has zero
+    force; jittered lattice has zero net force; mixed vs FP64 within 1e-5."""
+    from exastamp_amd import inputs
+    tab = cu_tables()
+    cfg = inputs.lattice_config((128, 128, 128), 3.615, [CU_MASS], T=0.0, jitter=0.0, seed=0)
+    sim = make_sim(cfg, [CU_MASS])
+    sim.eam_alloy_init(tab)
+    sim.chunk_neighbors(rcut_max=4.95, rcut_inc=0.3)
+    sim.zero_force_energy()
+    sim.eam_alloy_force(rcut=4.95)
+    sim.force_to_accel()
+    u = _units()
+    assert np.abs(sim.forces()).max() / u.EV < 1e-9
+    cfg = inputs.lattice_config((128, 128, 128), 3.615, [CU_MASS], T=600.0, jitter=0.05, seed=1)
+    sim.set_config(cfg)
+    sim.chunk_neighbors(rcut_max=4.95, rcut_inc=0.3)
+    sim.zero_force_energy()
+    sim.eam_alloy_force(rcut=4.95)
+    sim.force_to_accel()
+    f64 = sim.forces()
+    rms = np.sqrt((f64 ** 2).sum(1).mean())
+    assert np.abs(f64.sum(0)).max() < 1e-8 * rms * cfg.n
+    sim.zero_force_energy()
+    sim.eam_alloy_force(rcut=4.95, mixed=True)
+    sim.force_to_accel()
+    err = float(np.abs(sim.forces() - f64).max() / rms)
+    print(f"C3 mixed vs FP64 max|dF|/rms|F| = {err:.3e}")
+    assert err <= 1e-5
+
+
+def test_errors_are_reported_not_fatal():
+    from exastamp_amd.operators import Simulation, XsError
+    sim = Simulation(0)
+    with pytest.raises(XsError, match="no system"):
+        sim.chunk_neighbors(rcut_max=1.0)
+    with pytest.raises(XsError, match="species"):
+        sim.set_config(cu_config((3, 3, 3)))
+    sim.species([("Cu", CU_MASS)])
+    cfg = cu_config((3, 3, 3))
+    sim.set_config(cfg)
+    with pytest.raises(XsError, match="neighbour list"):
+        sim.eam_alloy_init(cu_tables(500))
+        sim.eam_alloy_force(rcut=4.95)
+    with pytest.raises(XsError, match="power of two"):
+        sim.chunk_neighbors(rcut_max=4.95, rcut_inc=0.3, config={"chunk_size": 3})
