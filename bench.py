@@ -1,0 +1,246 @@
+#!/usr/bin/env python3
+"""EAM Cu NVE throughput on one MI355X: mixed precision vs plain FP64.
+
+Workload (the headline configuration): FCC copper, a = 3.615 A, 128^3 cubic cells
+= 8 388 608 atoms, periodic; synthetic Johnson-type setfl tables with 5000 knots
+(rc = 4.95 A); neighbour skin 0.3 A; Maxwell-Boltzmann velocities at 600 K from a
+seed; velocity-Verlet NVE with dt = 1 fs.  A step is one NVE step of the whole
+system (drift/kick, displacement check and list rebuild when triggered, EAM
+density pass + embedding, EAM force pass + force->acceleration + kick).
+
+Prints ONE JSON line.  value = atom-steps/s of the mixed-precision form over the
+K timed steps (inputs resident in HBM, list rebuilds included, setup excluded).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "atom-steps/s EAM Cu 8.4M atoms, mixed vs FP64 GPU, at ≤1e-5 rel force error"
+HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+FP64_PEAK_TFLOPS = 78.6
+# Algorithmic per-atom figures of the EAM force pass as fused here (force pass +
+# force_to_accel + half kick): bytes of the compulsory FP64 state stream (read r,
+# F', v; write a, v) and FLOPs per interacting pair / per atom.
+FORCE_BYTES_PER_ATOM = 104.0
+FORCE_FLOPS_PER_PAIR = 49.0
+FORCE_FLOPS_PER_ATOM = 3.0 + 6.0
+
+
+def parse():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--cells", type=int, default=128, help="FCC cells per edge (128 -> 8.39M atoms)")
+    ap.add_argument("--fp64-steps", type=int, default=None, help="timed steps of the FP64 form (default: --steps)")
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--seed", type=int, default=2024)
+    return ap.parse_args()
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def setup(sim, cfg, tab):
+    sim.species([("Cu", 63.546)])
+    sim.set_config(cfg)
+    sim.eam_alloy_init(tab)
+
+
+def energy(sim, rc, mixed):
+    """Total energy at the current state (recomputes forces, outside timed regions)."""
+    sim.zero_force_energy()
+    sim.eam_alloy_force(rcut=rc, trigger_thermo_state=True, compute_virial=True, mixed=mixed)
+    sim.force_to_accel()
+    return sim.simulation_thermodynamic_state()
+
+
+def run_form(sim, cfg, tab, mixed, steps, warmup, dt, rc, skin):
+    from exastamp_amd import _lib
+    from exastamp_amd.units import EV
+    import ctypes as C
+    sim.set_config(cfg)
+    st_w, _ = sim.md_run("eam", warmup, dt, rc, skin, mixed=mixed)
+    th0 = energy(sim, rc, mixed)
+    sim.ctx.xs_profile_enable(1)
+    sim.ctx.xs_synchronize()
+    t0 = time.perf_counter()
+    st = _lib.RunStats()
+    flags = (_lib.XS_MIXED if mixed else 0) | _lib.XS_CONTINUE
+    sim.ctx.xs_md_run(_lib.XS_POT_EAM, steps, dt, rc, skin, flags, 0, None, C.byref(st))
+    sim.ctx.xs_synchronize()
+    t1 = time.perf_counter()
+    ms = np.zeros(len(_lib.PROF_CLASSES))
+    cnt = np.zeros(len(_lib.PROF_CLASSES), dtype=np.int64)
+    sim.ctx.xs_profile_get(_lib.dptr(ms), _lib.ptr(cnt, C.c_int64))
+    sim.ctx.xs_profile_enable(0)
+    thK = energy(sim, rc, mixed)
+    n = cfg.n
+    sec = t1 - t0
+    drift = (thK.total_energy - th0.total_energy) / EV / n / (steps * dt)
+    kern = {name: {"ms_total": float(ms[k]), "launches": int(cnt[k]),
+                   "ms_avg": float(ms[k] / cnt[k]) if cnt[k] else None}
+            for k, name in enumerate(_lib.PROF_CLASSES)}
+    return {
+        "value": n * steps / sec, "seconds": sec, "ms_per_step": 1e3 * sec / steps,
+        "rebuilds": int(st.rebuilds), "kernels": kern,
+        "drift_ev_per_atom_per_ps": float(drift), "T0": th0.temperature, "T_end": thK.temperature,
+        "warmup_rebuilds": int(st_w.rebuilds),
+    }
+
+
+def force_error(sim, rc):
+    """Mixed vs FP64 forces on the current positions."""
+    sim.zero_force_energy()
+    sim.eam_alloy_force(rcut=rc, mixed=False)
+    sim.force_to_accel()
+    f64 = sim.accelerations()
+    sim.zero_force_energy()
+    sim.eam_alloy_force(rcut=rc, mixed=True)
+    sim.force_to_accel()
+    f32 = sim.accelerations()
+    mag = np.sqrt((f64 ** 2).sum(1))
+    rms = float(np.sqrt((mag ** 2).mean()))
+    d = np.sqrt(((f32 - f64) ** 2).sum(1))
+    sel = mag > 0.1 * rms
+    return {
+        "max_abs_over_rms": float(d.max() / rms),
+        "rms_abs_over_rms": float(np.sqrt((d ** 2).mean()) / rms),
+        "max_rel_atoms_above_0.1rms": float((d[sel] / mag[sel]).max()),
+        "metric": "max_i |F_mixed,i - F_fp64,i| / rms_j |F_fp64,j|",
+    }
+
+
+def cpu_baseline(cfg, tab, steps, threads, dt, rc, skin):
+    """The FP64 oracle (OpenMP, full list, the reference's CPU structure) on the same
+    system for a bounded number of steps."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import pyoracle
+    if threads <= 0:
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or len(os.sched_getaffinity(0))
+    o = pyoracle.Oracle(threads=threads)
+    o.set_config(cfg, [63.546])
+    o.set_eam(tab)
+    t0 = time.perf_counter()
+    o.build_neighbors(rc, skin)
+    o.compute_forces("eam", 0)
+    t_setup = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    nreb = o.md_steps("eam", steps, dt, rc, skin)
+    sec = time.perf_counter() - t0
+    o.close()
+    return {
+        "value": cfg.n * steps / sec, "unit": "atom-steps/s", "cores": threads, "kind": "port",
+        "sample": f"FP64 oracle, same {cfg.n}-atom system, {steps} NVE steps after setup "
+                  f"({t_setup:.1f} s setup excluded), {nreb} rebuilds in the sample",
+        "seconds": sec,
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported on this build "
+              "(the path would shard by spatial domain decomposition, see DESIGN.md).", file=sys.stderr)
+        sys.exit(2)
+    from exastamp_amd import inputs, potentials
+    from exastamp_amd.operators import Simulation
+
+    dt, rc, skin = 0.001, 4.95, 0.3
+    nc = args.cells
+    log(f"building {nc}^3-cell Cu lattice + 5000-knot tables")
+    tab = potentials.johnson_tables(nr=5000, nrho=5000, rc=rc)
+    cfg = inputs.lattice_config((nc, nc, nc), 3.615, [63.546], T=600.0, jitter=0.0, seed=args.seed)
+    sim = Simulation(0)
+    setup(sim, cfg, tab)
+
+    log(f"mixed precision: {args.warmup} warmup + {args.steps} timed steps, {cfg.n} atoms")
+    mixed = run_form(sim, cfg, tab, True, args.steps, args.warmup, dt, rc, skin)
+    ferr = force_error(sim, rc)
+    entries, within = np.zeros(1, dtype=np.int64), np.zeros(1, dtype=np.int64)
+    import ctypes as C
+    sim.ctx.xs_pair_count(rc, entries.ctypes.data_as(C.POINTER(C.c_int64)),
+                          within.ctypes.data_as(C.POINTER(C.c_int64)))
+    pairs_per_atom = float(within[0]) / cfg.n
+    log(f"mixed: {mixed['value']:.4g} atom-steps/s, force err {ferr['max_abs_over_rms']:.2e}")
+
+    fp64 = None
+    if not args.no_fp64:
+        k64 = args.fp64_steps or args.steps
+        log(f"FP64: {args.warmup} warmup + {k64} timed steps")
+        fp64 = run_form(sim, cfg, tab, False, k64, args.warmup, dt, rc, skin)
+        log(f"fp64: {fp64['value']:.4g} atom-steps/s")
+
+    # roofline of the dominant kernel (EAM force pass, mixed)
+    kf = mixed["kernels"]["eam_force"]
+    avg_s = kf["ms_avg"] * 1e-3
+    gbs = FORCE_BYTES_PER_ATOM * cfg.n / avg_s / 1e9
+    tflops = (FORCE_FLOPS_PER_PAIR * pairs_per_atom + FORCE_FLOPS_PER_ATOM) * cfg.n / avg_s / 1e12
+    roof = {
+        "kernel": "k_eam_force<float> (EAM force pass + force_to_accel + half kick)",
+        "bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
+        "traffic": None,
+        "bytes_per_atom": FORCE_BYTES_PER_ATOM, "avg_launch_ms": kf["ms_avg"],
+        "valu": {"achieved_tflops": tflops, "peak_tflops": FP32_PEAK_TFLOPS, "frac": tflops / FP32_PEAK_TFLOPS,
+                 "flops_per_pair": FORCE_FLOPS_PER_PAIR, "pairs_per_atom": pairs_per_atom},
+    }
+
+    cpu = None
+    if not args.no_cpu:
+        log(f"CPU oracle baseline: {args.cpu_steps} steps")
+        cpu = cpu_baseline(cfg, tab, args.cpu_steps, args.cpu_threads, dt, rc, skin)
+        log(f"cpu: {cpu['value']:.4g} atom-steps/s on {cpu['cores']} threads")
+
+    out = {
+        "metric": METRIC,
+        "value": mixed["value"],
+        "unit": "atom-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": mixed["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32/f64 mixed (FP32 pair+spline math; FP64 positions, displacements, accumulation, reductions)",
+        "data": "synthetic (seeded FCC lattice + Maxwell-Boltzmann 600 K; synthetic Johnson-type 5000-knot tables)",
+        "config": {
+            "workload": f"EAM Cu FCC {nc}^3 cells = {cfg.n} atoms, periodic, rc 4.95 A, skin 0.3 A, "
+                        f"NVE dt 1 fs, T0 600 K",
+            "atoms": cfg.n, "dt_ps": dt, "rcut": rc, "skin": skin, "table_knots": 5000,
+            "parallelism": "single GPU",
+        },
+        "fp64": None if fp64 is None else {k: fp64[k] for k in ("value", "ms_per_step", "rebuilds",
+                                                                  "drift_ev_per_atom_per_ps")},
+        "mixed_over_fp64": None if fp64 is None else mixed["value"] / fp64["value"],
+        "force_error": ferr,
+        "nve_drift_ev_per_atom_per_ps": {"mixed": mixed["drift_ev_per_atom_per_ps"],
+                                         "fp64": None if fp64 is None else fp64["drift_ev_per_atom_per_ps"]},
+        "rebuilds": mixed["rebuilds"],
+        "pairs_per_atom_within_rc": pairs_per_atom,
+        "kernels_mixed": mixed["kernels"],
+        "kernels_fp64": None if fp64 is None else fp64["kernels"],
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    sim.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

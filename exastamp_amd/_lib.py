@@ -16,12 +16,15 @@ LIB_PATH = Path(__file__).resolve().parent / "libxsmd.so"
 XS_ENERGY = 1
 XS_VIRIAL = 2
 XS_MIXED = 4
+XS_CONTINUE = 8
 XS_POT_LJ = 0
 XS_POT_EAM = 1
 XS_FIELD_EP = 0
 XS_FIELD_VIRIAL = 1
 XS_FIELD_RHO_DEMB = 2
 XS_FIELD_ACCEL = 3
+
+PROF_CLASSES = ["neighbors", "eam_rho", "eam_force", "lj", "drift", "thermo"]
 
 STATUS = {0: "XS_OK", 1: "XS_E_ARG", 2: "XS_E_STATE", 3: "XS_E_HIP", 4: "XS_E_ALLOC", 5: "XS_E_IO",
           6: "XS_E_LIMIT"}
@@ -62,6 +65,7 @@ SIGNATURES = {
     "xs_particle_displ_over": (C.c_int, [_CTX, C.c_double, C.POINTER(C.c_int)]),
     "xs_neighbors_get": (C.c_int, [_CTX, _I64, _I64, _I64, _I32]),
     "xs_order_get": (C.c_int, [_CTX, _I64]),
+    "xs_pair_count": (C.c_int, [_CTX, C.c_double, _I64, _I64]),
     "xs_zero_force_energy": (C.c_int, [_CTX]),
     "xs_force_to_accel": (C.c_int, [_CTX]),
     "xs_lj_compute_force": (C.c_int, [_CTX, C.c_double, C.c_double, C.c_double, C.c_uint]),
@@ -73,6 +77,8 @@ SIGNATURES = {
     "xs_push_f_v_r": (C.c_int, [_CTX, C.c_double, C.c_double]),
     "xs_push_f_v": (C.c_int, [_CTX, C.c_double, C.c_double]),
     "xs_thermo": (C.c_int, [_CTX, _D]),
+    "xs_profile_enable": (C.c_int, [_CTX, C.c_int]),
+    "xs_profile_get": (C.c_int, [_CTX, _D, _I64]),
     "xs_md_run": (C.c_int, [_CTX, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_uint, C.c_int,
                             _D, C.POINTER(RunStats)]),
 }

@@ -79,12 +79,12 @@ void allocate_atoms(Ctx& c, int64_t n)
 {
   c.npad = std::max<int64_t>(64, (n + 63) / 64 * 64);
   const int64_t np = c.npad;
-  for (DBuf<double>* b : {&c.rx, &c.ry, &c.rz, &c.vx, &c.vy, &c.vz, &c.ax, &c.ay, &c.az, &c.rx2, &c.ry2,
-                          &c.rz2, &c.vx2, &c.vy2, &c.vz2, &c.ax2, &c.ay2, &c.az2, &c.rrx, &c.rry, &c.rrz,
-                          &c.ep, &c.rho})
+  for (DBuf<double>* b : {&c.vx, &c.vy, &c.vz, &c.ax, &c.ay, &c.az, &c.vx2, &c.vy2, &c.vz2, &c.ax2, &c.ay2,
+                          &c.az2, &c.rrx, &c.rry, &c.rrz, &c.ep, &c.rho})
     b->reserve(np);
+  c.pos.reserve(np);
+  c.pos2.reserve(np);
   c.vir.reserve(9 * np);
-  c.fpf.reserve(np);
   c.type.reserve(np);
   c.type2.reserve(np);
   c.id.reserve(np);
@@ -250,9 +250,12 @@ xs_status xs_system_set(xs_ctx* h, int64_t n, const double* rx, const double* ry
     c.list_valid = false;
     c.rho_state = RhoState::None;
     if (n > 0) {
-      to_device(c, c.rx, rx, n);
-      to_device(c, c.ry, ry, n);
-      to_device(c, c.rz, rz, n);
+      std::vector<Pos4> p4(n);
+      for (int64_t i = 0; i < n; i++) p4[i] = Pos4{rx[i], ry[i], rz[i], 0.0};
+      to_device(c, c.pos, p4.data(), n);
+      to_device(c, c.rrx, rx, n);
+      to_device(c, c.rry, ry, n);
+      to_device(c, c.rrz, rz, n);
       to_device(c, c.vx, vx, n);
       to_device(c, c.vy, vy, n);
       to_device(c, c.vz, vz, n);
@@ -261,9 +264,6 @@ xs_status xs_system_set(xs_ctx* h, int64_t n, const double* rx, const double* ry
       for (int64_t i = 0; i < n; i++) id[i] = i;
       to_device(c, c.id, id.data(), n);
       if (c.has_rep) to_device(c, c.rep, rep, n);
-      XS_HIP(hipMemcpyAsync(c.rrx.p, c.rx.p, n * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
-      XS_HIP(hipMemcpyAsync(c.rry.p, c.ry.p, n * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
-      XS_HIP(hipMemcpyAsync(c.rrz.p, c.rz.p, n * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
       launch_zero(c);
     }
     XS_HIP(hipStreamSynchronize(c.stream));
@@ -279,9 +279,15 @@ int64_t xs_system_size(const xs_ctx* h)
 xs_status xs_system_get(xs_ctx* h, double* rx, double* ry, double* rz, double* vx, double* vy, double* vz)
 {
   return guarded(h, [&](Ctx& c) {
-    to_id_order(c, c.rx, rx);
-    to_id_order(c, c.ry, ry);
-    to_id_order(c, c.rz, rz);
+    if (rx || ry || rz) {
+      const std::vector<int64_t> id = from_device(c, c.id, c.n);
+      const std::vector<Pos4> p = from_device(c, c.pos, c.n);
+      for (int64_t k = 0; k < c.n; k++) {
+        if (rx) rx[id[k]] = p[k].x;
+        if (ry) ry[id[k]] = p[k].y;
+        if (rz) rz[id[k]] = p[k].z;
+      }
+    }
     to_id_order(c, c.vx, vx);
     to_id_order(c, c.vy, vy);
     to_id_order(c, c.vz, vz);
@@ -310,10 +316,10 @@ xs_status xs_field_get(xs_ctx* h, int field, double* out)
       case XS_FIELD_EP: to_id_order(c, c.ep, out); break;
       case XS_FIELD_VIRIAL: to_id_order(c, c.vir, out, 9); break;
       case XS_FIELD_RHO_DEMB:
-        if (c.rho_state == RhoState::FpFloat) {
+        if (c.rho_state == RhoState::FpFloat || c.rho_state == RhoState::FpDouble) {
           const std::vector<int64_t> id = from_device(c, c.id, c.n);
-          const std::vector<float> v = from_device(c, c.fpf, c.n);
-          for (int64_t k = 0; k < c.n; k++) out[id[k]] = v[k];
+          const std::vector<Pos4> p = from_device(c, c.pos, c.n);
+          for (int64_t k = 0; k < c.n; k++) out[id[k]] = p[k].w;
         } else {
           to_id_order(c, c.rho, out);
         }
@@ -386,6 +392,17 @@ xs_status xs_order_get(xs_ctx* h, int64_t* perm)
     require(perm != nullptr, XS_E_ARG, "null perm");
     const std::vector<int64_t> id = from_device(c, c.id, c.n);
     std::copy(id.begin(), id.end(), perm);
+  });
+}
+
+xs_status xs_pair_count(xs_ctx* h, double rcut, int64_t* entries, int64_t* within_rcut)
+{
+  return guarded(h, [&](Ctx& c) {
+    require_list(c);
+    int64_t e, w;
+    pair_count(c, rcut, e, w);
+    if (entries) *entries = e;
+    if (within_rcut) *within_rcut = w;
   });
 }
 
@@ -474,11 +491,19 @@ xs_status xs_md_run(xs_ctx* h, int potential, int nsteps, double dt, double rcut
     require(thermo_every <= 0 || thermo_out, XS_E_ARG, "thermo_out required when thermo_every > 0");
     require(potential == XS_POT_LJ ? c.has_lj : c.has_eam, XS_E_STATE, "potential not set");
     const unsigned prec = flags & XS_MIXED;
+    const bool cont = (flags & XS_CONTINUE) != 0;
+    if (cont) {
+      require_list(c);
+      require(std::abs(c.list_rcut - rcut) < 1e-12 && std::abs(c.list_skin - skin) < 1e-12, XS_E_ARG,
+              "continued run must keep rcut and skin of the current list");
+    }
     const auto t0 = std::chrono::steady_clock::now();
     int64_t rebuilds = 0;
 
-    build_neighbor_list(c, rcut, skin);
-    rebuilds++;
+    if (!cont) {
+      build_neighbor_list(c, rcut, skin);
+      rebuilds++;
+    }
     auto is_thermo = [&](int step) { return thermo_every > 0 && step % thermo_every == 0; };
     auto forces = [&](int step, double kick) {
       const bool th = is_thermo(step);
@@ -491,7 +516,7 @@ xs_status xs_md_run(xs_ctx* h, int potential, int nsteps, double dt, double rcut
       if (th) launch_thermo(c, thermo_out + 27 * (size_t)(step / thermo_every));
     };
     // first_iteration: forces at t = 0, no kick
-    forces(0, 0.0);
+    if (!cont) forces(0, 0.0);
     const double thr2 = (0.5 * skin) * (0.5 * skin);
     for (int step = 1; step <= nsteps; step++) {
       XS_HIP(hipMemsetAsync(c.dscalar.p + 1, 0, sizeof(unsigned long long), c.stream));
@@ -499,6 +524,7 @@ xs_status xs_md_run(xs_ctx* h, int potential, int nsteps, double dt, double rcut
       XS_HIP(hipMemcpyAsync(c.hscalar + 1, c.dscalar.p + 1, sizeof(unsigned long long), hipMemcpyDeviceToHost,
                             c.stream));
       XS_HIP(hipStreamSynchronize(c.stream));
+      if (c.prof.on) c.prof.flush();
       double d2;
       std::memcpy(&d2, &c.hscalar[1], sizeof(double));
       if (d2 > thr2) {
@@ -509,11 +535,34 @@ xs_status xs_md_run(xs_ctx* h, int potential, int nsteps, double dt, double rcut
     }
     XS_HIP(hipStreamSynchronize(c.stream));
     const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (c.prof.on) c.prof.flush();
     if (stats) {
       stats->steps = nsteps;
       stats->rebuilds = rebuilds;
       stats->max_neighbors = c.nmax;
       stats->seconds = sec;
+    }
+  });
+}
+
+xs_status xs_profile_enable(xs_ctx* h, int on)
+{
+  return guarded(h, [&](Ctx& c) {
+    XS_HIP(hipStreamSynchronize(c.stream));
+    c.prof.flush();
+    c.prof.reset();
+    c.prof.on = on != 0;
+  });
+}
+
+xs_status xs_profile_get(xs_ctx* h, double* ms_total, int64_t* launches)
+{
+  return guarded(h, [&](Ctx& c) {
+    XS_HIP(hipStreamSynchronize(c.stream));
+    c.prof.flush();
+    for (int k = 0; k < XS_PROF_NCLASSES; k++) {
+      if (ms_total) ms_total[k] = c.prof.ms[k];
+      if (launches) launches[k] = c.prof.count[k];
     }
   });
 }

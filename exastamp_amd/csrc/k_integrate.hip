@@ -32,16 +32,18 @@ __global__ void k_force_to_accel(int64_t n, double* ax, double* ay, double* az,
 }
 
 // x += v*dt + a*dt2 with dt2 = dt*dt/2, evaluated as x + fma(v, dt, a*dt2).
-__global__ void k_push_f_v_r(int64_t n, double dt, double dt2, double* rx, double* ry, double* rz,
+__global__ void k_push_f_v_r(int64_t n, double dt, double dt2, Pos4* __restrict__ pos,
                              const double* __restrict__ vx, const double* __restrict__ vy,
                              const double* __restrict__ vz, const double* __restrict__ ax,
                              const double* __restrict__ ay, const double* __restrict__ az)
 {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  rx[i] += fma(vx[i], dt, ax[i] * dt2);
-  ry[i] += fma(vy[i], dt, ay[i] * dt2);
-  rz[i] += fma(vz[i], dt, az[i] * dt2);
+  Pos4 p = load_pos(pos, i);
+  p.x += fma(vx[i], dt, ax[i] * dt2);
+  p.y += fma(vy[i], dt, ay[i] * dt2);
+  p.z += fma(vz[i], dt, az[i] * dt2);
+  store_pos(pos, i, p);
 }
 
 __global__ void k_push_f_v(int64_t n, double dt, double* vx, double* vy, double* vz,
@@ -57,31 +59,37 @@ __global__ void k_push_f_v(int64_t n, double dt, double* vx, double* vy, double*
 
 // push_f_v_r(dt) then push_f_v(dt/2), plus the largest squared displacement
 // since the last list build (as bits of a non-negative double, which order
-// like the doubles themselves).
+// like the doubles themselves): reduced per block, one atomic per block.
 __global__ void __launch_bounds__(256) k_drift_kick(
-    int64_t n, double dt, double dt2, double hdt, double* rx, double* ry, double* rz, double* vx,
-    double* vy, double* vz, const double* __restrict__ ax, const double* __restrict__ ay,
+    int64_t n, double dt, double dt2, double hdt, Pos4* __restrict__ pos, double* vx, double* vy,
+    double* vz, const double* __restrict__ ax, const double* __restrict__ ay,
     const double* __restrict__ az, const double* __restrict__ rrx, const double* __restrict__ rry,
     const double* __restrict__ rrz, unsigned long long* __restrict__ maxd2)
 {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   double d2 = 0.0;
-  if (i < n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
     const double a0 = ax[i], a1 = ay[i], a2 = az[i];
-    double v0 = vx[i], v1 = vy[i], v2 = vz[i];
-    const double x = rx[i] + fma(v0, dt, a0 * dt2);
-    const double y = ry[i] + fma(v1, dt, a1 * dt2);
-    const double z = rz[i] + fma(v2, dt, a2 * dt2);
-    rx[i] = x;
-    ry[i] = y;
-    rz[i] = z;
+    const double v0 = vx[i], v1 = vy[i], v2 = vz[i];
+    Pos4 p = load_pos(pos, i);
+    p.x += fma(v0, dt, a0 * dt2);
+    p.y += fma(v1, dt, a1 * dt2);
+    p.z += fma(v2, dt, a2 * dt2);
+    store_pos(pos, i, p);
     vx[i] = fma(a0, hdt, v0);
     vy[i] = fma(a1, hdt, v1);
     vz[i] = fma(a2, hdt, v2);
-    d2 = xsmd_d2(x - rrx[i], y - rry[i], z - rrz[i]);
+    d2 = fmax(d2, xsmd_d2(p.x - rrx[i], p.y - rry[i], p.z - rrz[i]));
   }
   for (int off = 32; off > 0; off >>= 1) d2 = fmax(d2, __shfl_xor(d2, off));
-  if ((threadIdx.x & 63) == 0 && d2 > 0.0) atomicMax(maxd2, (unsigned long long)__double_as_longlong(d2));
+  __shared__ double sh[kBlock / 64];
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = d2;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = sh[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); w++) m = fmax(m, sh[w]);
+    if (m > 0.0) atomicMax(maxd2, (unsigned long long)__double_as_longlong(m));
+  }
 }
 
 constexpr int kThermoBlocks = 1024;
@@ -166,8 +174,8 @@ void launch_push_f_v_r(Ctx& c, double dt)
 {
   if (c.n == 0) return;
   const double dt2 = dt * dt * 0.5;
-  k_push_f_v_r<<<grid_for(c.n), kBlock, 0, c.stream>>>(c.n, dt, dt2, c.rx.p, c.ry.p, c.rz.p, c.vx.p,
-                                                       c.vy.p, c.vz.p, c.ax.p, c.ay.p, c.az.p);
+  k_push_f_v_r<<<grid_for(c.n), kBlock, 0, c.stream>>>(c.n, dt, dt2, c.pos.p, c.vx.p, c.vy.p, c.vz.p,
+                                                       c.ax.p, c.ay.p, c.az.p);
   XS_HIP(hipGetLastError());
 }
 
@@ -183,9 +191,10 @@ void launch_drift_kick(Ctx& c, double dt)
 {
   if (c.n == 0) return;
   const double dt2 = dt * dt * 0.5;
-  k_drift_kick<<<grid_for(c.n), kBlock, 0, c.stream>>>(
-      c.n, dt, dt2, dt * 0.5, c.rx.p, c.ry.p, c.rz.p, c.vx.p, c.vy.p, c.vz.p, c.ax.p, c.ay.p, c.az.p,
-      c.rrx.p, c.rry.p, c.rrz.p, c.dscalar.p + 1);
+  ProfScope prof(c.prof, c.stream, XS_PROF_DRIFT);
+  const unsigned g = std::min<unsigned>(grid_for(c.n), 4096);
+  k_drift_kick<<<g, kBlock, 0, c.stream>>>(c.n, dt, dt2, dt * 0.5, c.pos.p, c.vx.p, c.vy.p, c.vz.p, c.ax.p,
+                                           c.ay.p, c.az.p, c.rrx.p, c.rry.p, c.rrz.p, c.dscalar.p + 1);
   XS_HIP(hipGetLastError());
 }
 
@@ -195,6 +204,7 @@ void launch_thermo(Ctx& c, double out[27])
   c.red_out.reserve(kThermoQ);
   double s[kThermoQ] = {0};
   if (c.n > 0) {
+    ProfScope prof(c.prof, c.stream, XS_PROF_THERMO);
     const int nb = (int)std::min<int64_t>(kThermoBlocks, grid_for(c.n));
     k_thermo_partial<<<nb, kBlock, 0, c.stream>>>(c.n, c.npad, c.vx.p, c.vy.p, c.vz.p, c.ep.p, c.vir.p,
                                                   c.type.p, c.mass_d.p, c.red_part.p);

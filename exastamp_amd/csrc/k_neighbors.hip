@@ -17,30 +17,37 @@
 namespace xsmd {
 namespace {
 
-__global__ void k_wrap_cellkey(int64_t n, double* __restrict__ rx, double* __restrict__ ry,
-                               double* __restrict__ rz, const uint16_t* __restrict__ rep,
-                               const BoxDev* __restrict__ box, int px, int py, int pz, int nx,
-                               int ny, int nz, int64_t ncell_rep, uint32_t* __restrict__ keys,
+__global__ void k_wrap_cellkey(int64_t n, Pos4* __restrict__ pos, const uint16_t* __restrict__ rep,
+                               const BoxDev* __restrict__ box, int px, int py, int pz, int nx, int ny,
+                               int nz, int64_t ncell_rep, uint32_t* __restrict__ keys,
                                uint32_t* __restrict__ vals)
 {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int r = rep ? rep[i] : 0;
   const BoxDev& b = box[r];
-  double x = rx[i], y = ry[i], z = rz[i];
-  if (px) { x = xsmd_wrap(x, b.lo[0], b.L[0]); rx[i] = x; }
-  if (py) { y = xsmd_wrap(y, b.lo[1], b.L[1]); ry[i] = y; }
-  if (pz) { z = xsmd_wrap(z, b.lo[2], b.L[2]); rz[i] = z; }
-  const int cx = xsmd_cell_coord(x, b.lo[0], b.inv_cs[0], nx);
-  const int cy = xsmd_cell_coord(y, b.lo[1], b.inv_cs[1], ny);
-  const int cz = xsmd_cell_coord(z, b.lo[2], b.inv_cs[2], nz);
+  Pos4 p = load_pos(pos, i);
+  if (px || py || pz) {
+    if (px) p.x = xsmd_wrap(p.x, b.lo[0], b.L[0]);
+    if (py) p.y = xsmd_wrap(p.y, b.lo[1], b.L[1]);
+    if (pz) p.z = xsmd_wrap(p.z, b.lo[2], b.L[2]);
+    store_pos(pos, i, p);
+  }
+  const int cx = xsmd_cell_coord(p.x, b.lo[0], b.inv_cs[0], nx);
+  const int cy = xsmd_cell_coord(p.y, b.lo[1], b.inv_cs[1], ny);
+  const int cz = xsmd_cell_coord(p.z, b.lo[2], b.inv_cs[2], nz);
   keys[i] = (uint32_t)((int64_t)r * ncell_rep + ((int64_t)cz * ny + cy) * nx + cx);
   vals[i] = (uint32_t)i;
 }
 
 struct PermuteArgs {
-  const double* src[12];
-  double* dst[12];
+  const Pos4* pos_src;
+  Pos4* pos_dst;
+  double* rrx;
+  double* rry;
+  double* rrz;
+  const double* src[6];
+  double* dst[6];
   const uint8_t* type_src;
   uint8_t* type_dst;
   const int64_t* id_src;
@@ -49,13 +56,20 @@ struct PermuteArgs {
   uint16_t* rep_dst;
 };
 
+// Gather every per-atom array into the new order; the new positions also become
+// the reference positions of the displacement check.
 __global__ void k_permute(int64_t n, const uint32_t* __restrict__ perm, PermuteArgs a)
 {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const uint32_t p = perm[i];
+  const Pos4 q = load_pos(a.pos_src, p);
+  store_pos(a.pos_dst, i, q);
+  a.rrx[i] = q.x;
+  a.rry[i] = q.y;
+  a.rrz[i] = q.z;
 #pragma unroll
-  for (int d = 0; d < 12; d++) a.dst[d][i] = a.src[d][p];
+  for (int d = 0; d < 6; d++) a.dst[d][i] = a.src[d][p];
   a.type_dst[i] = a.type_src[p];
   a.id_dst[i] = a.id_src[p];
   if (a.rep_src) a.rep_dst[i] = a.rep_src[p];
@@ -74,8 +88,7 @@ __global__ void k_cell_bounds(int64_t n, const uint32_t* __restrict__ keys, int*
 // One thread per atom; stencil order dz, dy, dx (outer to inner), in-cell order
 // ascending -- the same visit order as the oracle, so lists match entry by entry.
 __global__ void __launch_bounds__(256) k_build(
-    int64_t n, const double* __restrict__ rx, const double* __restrict__ ry,
-    const double* __restrict__ rz, const uint32_t* __restrict__ keys,
+    int64_t n, const Pos4* __restrict__ pos, const uint32_t* __restrict__ keys,
     const BoxDev* __restrict__ box, int px, int py, int pz, int nx, int ny, int nz,
     int64_t ncell_rep, const int* __restrict__ cstart, const int* __restrict__ cend, double rl2,
     uint32_t* __restrict__ nbr, int* __restrict__ nnb, int nmax, int64_t npad,
@@ -90,7 +103,7 @@ __global__ void __launch_bounds__(256) k_build(
   const int cy = (int)((cl / nx) % ny);
   const int cz = (int)(cl / ((int64_t)nx * ny));
   const BoxDev& b = box[r];
-  const double xi = rx[i], yi = ry[i], zi = rz[i];
+  const Pos4 pi = load_pos(pos, i);
   int cnt = 0;
   for (int dz = -1; dz <= 1; dz++) {
     int z = cz + dz, kz = 0;
@@ -109,9 +122,10 @@ __global__ void __launch_bounds__(256) k_build(
         const bool self_img = (kx == 0 && ky == 0 && kz == 0);
         for (int j = j0; j < j1; j++) {
           if (self_img && j == (int)i) continue;
-          const double ddx = xsmd_image(rx[j], kx, b.L[0]) - xi;
-          const double ddy = xsmd_image(ry[j], ky, b.L[1]) - yi;
-          const double ddz = xsmd_image(rz[j], kz, b.L[2]) - zi;
+          const Pos4 pj = load_pos(pos, j);
+          const double ddx = xsmd_image(pj.x, kx, b.L[0]) - pi.x;
+          const double ddy = xsmd_image(pj.y, ky, b.L[1]) - pi.y;
+          const double ddz = xsmd_image(pj.z, kz, b.L[2]) - pi.z;
           if (xsmd_d2(ddx, ddy, ddz) < rl2) {
             if (cnt < nmax) nbr[(int64_t)cnt * npad + i] = xsmd_nbr_encode((uint32_t)j, kx, ky, kz);
             cnt++;
@@ -124,10 +138,71 @@ __global__ void __launch_bounds__(256) k_build(
   if (cnt > nmax) atomicMax(maxcnt, (unsigned long long)cnt);
 }
 
+// block-wide max of a non-negative double, one atomic per block
+__device__ __forceinline__ void block_max_to(double v, unsigned long long* out)
+{
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off));
+  __shared__ double sh[kBlock / 64];
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = sh[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); w++) m = fmax(m, sh[w]);
+    if (m > 0.0) atomicMax(out, (unsigned long long)__double_as_longlong(m));
+  }
+}
+
+__global__ void __launch_bounds__(256) k_max_displ2(int64_t n, const Pos4* __restrict__ pos,
+                                                    const double* __restrict__ rrx,
+                                                    const double* __restrict__ rry,
+                                                    const double* __restrict__ rrz,
+                                                    unsigned long long* __restrict__ out)
+{
+  double m = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const Pos4 p = load_pos(pos, i);
+    m = fmax(m, xsmd_d2(p.x - rrx[i], p.y - rry[i], p.z - rrz[i]));
+  }
+  block_max_to(m, out);
+}
+
+__global__ void k_pair_count(int64_t n, int64_t npad, const Pos4* __restrict__ pos,
+                             const uint16_t* __restrict__ rep, const BoxDev* __restrict__ box,
+                             const uint32_t* __restrict__ nbr, const int* __restrict__ nnb, double rc2,
+                             unsigned long long* __restrict__ out)
+{
+  unsigned long long e = 0, w = 0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const double* L = box[rep ? rep[i] : 0].L;
+    const Pos4 pi = load_pos(pos, i);
+    const int cnt = nnb[i];
+    e += cnt;
+    for (int k = 0; k < cnt; k++) {
+      const uint32_t en = nbr[(int64_t)k * npad + i];
+      const Pos4 pj = load_pos(pos, xsmd_nbr_index(en));
+      const double dx = xsmd_image(pj.x, xsmd_nbr_img_x(en), L[0]) - pi.x;
+      const double dy = xsmd_image(pj.y, xsmd_nbr_img_y(en), L[1]) - pi.y;
+      const double dz = xsmd_image(pj.z, xsmd_nbr_img_z(en), L[2]) - pi.z;
+      if (xsmd_d2(dx, dy, dz) <= rc2) w++;
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    e += __shfl_xor(e, off);
+    w += __shfl_xor(w, off);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicAdd(out, e);
+    atomicAdd(out + 1, w);
+  }
+}
+
 }  // namespace
 
 void build_neighbor_list(Ctx& c, double rcut, double skin)
 {
+  ProfScope prof(c.prof, c.stream, XS_PROF_NEIGHBORS);
   const int64_t n = c.n;
   if (n == 0) {
     c.list_valid = true;
@@ -152,9 +227,9 @@ void build_neighbor_list(Ctx& c, double rcut, double skin)
   if (c.ncell_tot >= (int64_t)1 << 32) throw Error(XS_E_LIMIT, "too many cells");
 
   const unsigned g = grid_for(n);
-  k_wrap_cellkey<<<g, kBlock, 0, c.stream>>>(n, c.rx.p, c.ry.p, c.rz.p, c.has_rep ? c.rep.p : nullptr,
-                                             c.box_d.p, c.periodic[0], c.periodic[1], c.periodic[2],
-                                             c.nc[0], c.nc[1], c.nc[2], c.ncell_rep, c.keys.p, c.vals.p);
+  k_wrap_cellkey<<<g, kBlock, 0, c.stream>>>(n, c.pos.p, c.has_rep ? c.rep.p : nullptr, c.box_d.p,
+                                             c.periodic[0], c.periodic[1], c.periodic[2], c.nc[0], c.nc[1],
+                                             c.nc[2], c.ncell_rep, c.keys.p, c.vals.p);
   XS_HIP(hipGetLastError());
 
   int nbits = 1;
@@ -167,11 +242,14 @@ void build_neighbor_list(Ctx& c, double rcut, double skin)
                                             c.vals2.p, (int)n, 0, nbits, c.stream));
 
   PermuteArgs pa{};
-  const double* src[12] = {c.rx.p, c.ry.p, c.rz.p, c.vx.p, c.vy.p, c.vz.p,
-                           c.ax.p, c.ay.p, c.az.p, c.rx.p, c.ry.p, c.rz.p};
-  double* dst[12] = {c.rx2.p, c.ry2.p, c.rz2.p, c.vx2.p, c.vy2.p, c.vz2.p,
-                     c.ax2.p, c.ay2.p, c.az2.p, c.rrx.p, c.rry.p, c.rrz.p};
-  for (int d = 0; d < 12; d++) {
+  pa.pos_src = c.pos.p;
+  pa.pos_dst = c.pos2.p;
+  pa.rrx = c.rrx.p;
+  pa.rry = c.rry.p;
+  pa.rrz = c.rrz.p;
+  const double* src[6] = {c.vx.p, c.vy.p, c.vz.p, c.ax.p, c.ay.p, c.az.p};
+  double* dst[6] = {c.vx2.p, c.vy2.p, c.vz2.p, c.ax2.p, c.ay2.p, c.az2.p};
+  for (int d = 0; d < 6; d++) {
     pa.src[d] = src[d];
     pa.dst[d] = dst[d];
   }
@@ -183,9 +261,7 @@ void build_neighbor_list(Ctx& c, double rcut, double skin)
   pa.rep_dst = c.has_rep ? c.rep2.p : nullptr;
   k_permute<<<g, kBlock, 0, c.stream>>>(n, c.vals2.p, pa);
   XS_HIP(hipGetLastError());
-  c.rx.swap(c.rx2);
-  c.ry.swap(c.ry2);
-  c.rz.swap(c.rz2);
+  c.pos.swap(c.pos2);
   c.vx.swap(c.vx2);
   c.vy.swap(c.vy2);
   c.vz.swap(c.vz2);
@@ -217,10 +293,9 @@ void build_neighbor_list(Ctx& c, double rcut, double skin)
     c.nbr.reserve((size_t)c.nmax * c.npad);
     c.nnb.reserve(c.npad);
     XS_HIP(hipMemsetAsync(c.dscalar.p, 0, sizeof(unsigned long long), c.stream));
-    k_build<<<g, kBlock, 0, c.stream>>>(n, c.rx.p, c.ry.p, c.rz.p, c.keys2.p, c.box_d.p, c.periodic[0],
-                                        c.periodic[1], c.periodic[2], c.nc[0], c.nc[1], c.nc[2],
-                                        c.ncell_rep, c.cstart.p, c.cend.p, rl2, c.nbr.p, c.nnb.p,
-                                        c.nmax, c.npad, c.dscalar.p);
+    k_build<<<g, kBlock, 0, c.stream>>>(n, c.pos.p, c.keys2.p, c.box_d.p, c.periodic[0], c.periodic[1],
+                                        c.periodic[2], c.nc[0], c.nc[1], c.nc[2], c.ncell_rep, c.cstart.p,
+                                        c.cend.p, rl2, c.nbr.p, c.nnb.p, c.nmax, c.npad, c.dscalar.p);
     XS_HIP(hipGetLastError());
     XS_HIP(hipMemcpyAsync(c.hscalar, c.dscalar.p, sizeof(unsigned long long), hipMemcpyDeviceToHost,
                           c.stream));
@@ -235,35 +310,31 @@ void build_neighbor_list(Ctx& c, double rcut, double skin)
   c.list_skin = skin;
 }
 
-namespace {
-
-__global__ void k_max_displ2(int64_t n, const double* __restrict__ rx, const double* __restrict__ ry,
-                             const double* __restrict__ rz, const double* __restrict__ rrx,
-                             const double* __restrict__ rry, const double* __restrict__ rrz,
-                             unsigned long long* __restrict__ out)
+void pair_count(Ctx& c, double rcut, int64_t& entries, int64_t& within)
 {
-  double m = 0.0;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    const double d2 = xsmd_d2(rx[i] - rrx[i], ry[i] - rry[i], rz[i] - rrz[i]);
-    m = fmax(m, d2);
-  }
-  for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off));
-  if ((threadIdx.x & 63) == 0) atomicMax(out, (unsigned long long)__double_as_longlong(m));
+  entries = within = 0;
+  if (c.n == 0) return;
+  XS_HIP(hipMemsetAsync(c.dscalar.p + 2, 0, 2 * sizeof(unsigned long long), c.stream));
+  const unsigned g = std::min<unsigned>(grid_for(c.n), 4096);
+  k_pair_count<<<g, kBlock, 0, c.stream>>>(c.n, c.npad, c.pos.p, c.has_rep ? c.rep.p : nullptr, c.box_d.p,
+                                           c.nbr.p, c.nnb.p, rcut * rcut, c.dscalar.p + 2);
+  XS_HIP(hipGetLastError());
+  XS_HIP(hipMemcpyAsync(c.hscalar + 2, c.dscalar.p + 2, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                        c.stream));
+  XS_HIP(hipStreamSynchronize(c.stream));
+  entries = (int64_t)c.hscalar[2];
+  within = (int64_t)c.hscalar[3];
 }
-
-}  // namespace
 
 double max_displacement2(Ctx& c)
 {
   if (c.n == 0) return 0.0;
   XS_HIP(hipMemsetAsync(c.dscalar.p + 1, 0, sizeof(unsigned long long), c.stream));
   const unsigned g = std::min<unsigned>(grid_for(c.n), 2048);
-  k_max_displ2<<<g, kBlock, 0, c.stream>>>(c.n, c.rx.p, c.ry.p, c.rz.p, c.rrx.p, c.rry.p, c.rrz.p,
-                                          c.dscalar.p + 1);
+  k_max_displ2<<<g, kBlock, 0, c.stream>>>(c.n, c.pos.p, c.rrx.p, c.rry.p, c.rrz.p, c.dscalar.p + 1);
   XS_HIP(hipGetLastError());
-  XS_HIP(hipMemcpyAsync(c.hscalar + 1, c.dscalar.p + 1, sizeof(unsigned long long),
-                        hipMemcpyDeviceToHost, c.stream));
+  XS_HIP(hipMemcpyAsync(c.hscalar + 1, c.dscalar.p + 1, sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                        c.stream));
   XS_HIP(hipStreamSynchronize(c.stream));
   double d;
   std::memcpy(&d, &c.hscalar[1], sizeof(double));

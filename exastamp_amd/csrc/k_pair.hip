@@ -1,8 +1,14 @@
 // k_pair.hip -- Lennard-Jones and EAM pair kernels over the full neighbour list.
 //
 // One thread per central atom walks its ELL list (coalesced entry loads), gathers
-// neighbour positions, and accumulates its own force only: no atomics, no
+// neighbour position records, and accumulates its own force only: no atomics, no
 // Newton-symmetric scatter, so results do not depend on scheduling.
+//
+// The neighbour loop is branch-free and batched: U entries are loaded, then U
+// position records (one 32-byte sector each), then U pairs are evaluated with
+// the cutoff test applied as a select.  Lanes of a wavefront diverge on the
+// cutoff anyway, so computing the out-of-range entries costs nothing extra, and
+// the batch keeps several gathers in flight per wavefront.
 //
 // Precision: template parameter Real is the type of the pair and spline
 // arithmetic.  Real = double is the plain FP64 form.  Real = float is the mixed
@@ -57,9 +63,7 @@ __device__ __forceinline__ int64_t logical_block()
 
 struct PairCommon {
   int64_t n, npad;
-  const double* __restrict__ rx;
-  const double* __restrict__ ry;
-  const double* __restrict__ rz;
+  const Pos4* __restrict__ pos;
   const uint8_t* __restrict__ type;
   const uint16_t* __restrict__ rep;
   const BoxDev* __restrict__ box;
@@ -81,27 +85,44 @@ struct Outputs {
   double kick_dt;
 };
 
-// Displacement r_j(image) - r_i in FP64 and its squared norm.
-struct Disp {
-  double x, y, z, d2;
+constexpr int kBatch = 4;
+
+// One neighbour: FP64 displacement r_j(image) - r_i, squared norm, the neighbour's
+// record scalar w, and whether it is a real entry inside the cutoff.
+struct Nb {
+  double x, y, z, d2, w;
   uint32_t j;
+  bool in;
 };
-__device__ __forceinline__ Disp displacement(const PairCommon& pc, uint32_t e, double xi, double yi,
-                                             double zi, const double* L)
+
+// Load entries k0..k0+U-1 (missing ones become the atom itself, masked out) and
+// their position records, then form the displacements.
+template <int U>
+__device__ __forceinline__ void fetch(const PairCommon& pc, int64_t i, int k0, int cnt, const Pos4& pi,
+                                      const double* L, Nb (&nb)[U])
 {
-  Disp d;
-  d.j = xsmd_nbr_index(e);
-  double xj = pc.rx[d.j], yj = pc.ry[d.j], zj = pc.rz[d.j];
-  if (e >> XSMD_NBR_INDEX_BITS) {
-    xj = xsmd_image(xj, xsmd_nbr_img_x(e), L[0]);
-    yj = xsmd_image(yj, xsmd_nbr_img_y(e), L[1]);
-    zj = xsmd_image(zj, xsmd_nbr_img_z(e), L[2]);
+  uint32_t e[U];
+#pragma unroll
+  for (int u = 0; u < U; u++) e[u] = (k0 + u < cnt) ? pc.nbr[(int64_t)(k0 + u) * pc.npad + i] : (uint32_t)i;
+  Pos4 pj[U];
+#pragma unroll
+  for (int u = 0; u < U; u++) pj[u] = load_pos(pc.pos, xsmd_nbr_index(e[u]));
+#pragma unroll
+  for (int u = 0; u < U; u++) {
+    double xj = pj[u].x, yj = pj[u].y, zj = pj[u].z;
+    if (e[u] >> XSMD_NBR_INDEX_BITS) {
+      xj = xsmd_image(xj, xsmd_nbr_img_x(e[u]), L[0]);
+      yj = xsmd_image(yj, xsmd_nbr_img_y(e[u]), L[1]);
+      zj = xsmd_image(zj, xsmd_nbr_img_z(e[u]), L[2]);
+    }
+    nb[u].x = xj - pi.x;
+    nb[u].y = yj - pi.y;
+    nb[u].z = zj - pi.z;
+    nb[u].d2 = xsmd_d2(nb[u].x, nb[u].y, nb[u].z);
+    nb[u].w = pj[u].w;
+    nb[u].j = xsmd_nbr_index(e[u]);
+    nb[u].in = (k0 + u < cnt) && nb[u].d2 <= pc.rc2;
   }
-  d.x = xj - xi;
-  d.y = yj - yi;
-  d.z = zj - zi;
-  d.d2 = xsmd_d2(d.x, d.y, d.z);
-  return d;
 }
 
 template <bool ENERGY, bool VIRIAL, bool FUSED>
@@ -132,7 +153,8 @@ __device__ __forceinline__ void write_outputs(const Outputs& o, const PairCommon
   }
 }
 
-__device__ __forceinline__ void add_virial(double* v, double fe0, double fe1, double fe2, const Disp& d)
+// onika tensor(fe, dr) * -0.5 added to v (row-major m11 m12 ... m33)
+__device__ __forceinline__ void add_virial(double* v, double fe0, double fe1, double fe2, const Nb& d)
 {
   const double fe[3] = {fe0, fe1, fe2};
   const double dr[3] = {d.x, d.y, d.z};
@@ -158,24 +180,27 @@ __global__ void __launch_bounds__(256) k_lj(PairCommon pc, LJParams<Real> lp, Ou
   const int64_t i = logical_block() * blockDim.x + threadIdx.x;
   if (i >= pc.n) return;
   const double* L = pc.box[pc.rep ? pc.rep[i] : 0].L;
-  const double xi = pc.rx[i], yi = pc.ry[i], zi = pc.rz[i];
+  const Pos4 pi = load_pos(pc.pos, i);
   double fx = 0.0, fy = 0.0, fz = 0.0, e = 0.0;
   double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   const int cnt = pc.nnb[i];
-  for (int k = 0; k < cnt; k++) {
-    const Disp d = displacement(pc, pc.nbr[(int64_t)k * pc.npad + i], xi, yi, zi, L);
-    if (d.d2 > pc.rc2) continue;
-    const Real ir2 = recip((Real)d.d2);
-    const Real s2 = lp.sigma2 * ir2;
-    const Real s6 = s2 * s2 * s2;
-    const Real s12 = s6 * s6;
-    const double de_r = (double)(-lp.eps24 * (Real(2) * s12 - s6) * ir2);
-    const double fe0 = de_r * d.x, fe1 = de_r * d.y, fe2 = de_r * d.z;
-    fx += fe0;
-    fy += fe1;
-    fz += fe2;
-    if constexpr (ENERGY) e += 0.5 * (double)(lp.eps4 * (s12 - s6) - lp.ecut);
-    if constexpr (VIRIAL) add_virial(v, fe0, fe1, fe2, d);
+  for (int k0 = 0; k0 < cnt; k0 += kBatch) {
+    Nb nb[kBatch];
+    fetch<kBatch>(pc, i, k0, cnt, pi, L, nb);
+#pragma unroll
+    for (int u = 0; u < kBatch; u++) {
+      const Real ir2 = recip((Real)(nb[u].in ? nb[u].d2 : 1.0));
+      const Real s2 = lp.sigma2 * ir2;
+      const Real s6 = s2 * s2 * s2;
+      const Real s12 = s6 * s6;
+      const double de_r = nb[u].in ? (double)(-lp.eps24 * (Real(2) * s12 - s6) * ir2) : 0.0;
+      const double fe0 = de_r * nb[u].x, fe1 = de_r * nb[u].y, fe2 = de_r * nb[u].z;
+      fx += fe0;
+      fy += fe1;
+      fz += fe2;
+      if constexpr (ENERGY) e += nb[u].in ? 0.5 * (double)(lp.eps4 * (s12 - s6) - lp.ecut) : 0.0;
+      if constexpr (VIRIAL) add_virial(v, fe0, fe1, fe2, nb[u]);
+    }
   }
   write_outputs<ENERGY, VIRIAL, FUSED>(o, pc, i, fx, fy, fz, e, v);
 }
@@ -190,7 +215,7 @@ __device__ __forceinline__ void knot_r(const EamTabDev<Real>& t, Real r, int& m,
 {
   p = r * t.rdr + Real(1);
   m = (int)p;
-  m = min(m, t.nr - 1);
+  m = max(0, min(m, t.nr - 1));
   p -= (Real)m;
   p = fmin(p, Real(1));
 }
@@ -210,6 +235,12 @@ template <class Real>
 __device__ __forceinline__ Real quad(const Q4<Real>& c, Real p)
 {
   return (c.a * p + c.b) * p + c.c;
+}
+// derivative of the value cubic with respect to p
+template <class Real>
+__device__ __forceinline__ Real dcubic(const Q4<Real>& c, Real p)
+{
+  return (Real(3) * c.a * p + Real(2) * c.b) * p + c.c;
 }
 
 // F(rho), F'(rho) in eV-converted internal units, with the linear extrapolation
@@ -234,10 +265,13 @@ __device__ __forceinline__ void embed(const EamTabDev<Real>& t, int ti, double r
 }
 
 // Pass 1 (+ fused pass 1b): rho_i = sum_j rho_{type_j}(r_ij) in FP64, then
-// F'(rho_i) (and F(rho_i) into ep when ENERGY).
+// F'(rho_i) into the position record's w (and F(rho_i) into ep when ENERGY).
+// Single-element systems read the rho cubic from the first half of the combined
+// pair row; alloys read the per-element rho table.
 template <class Real, bool MULTI, bool RHO, bool EMB, bool ENERGY, bool SET_EP>
-__global__ void __launch_bounds__(256) k_eam_rho(PairCommon pc, EamTabDev<Real> t, double* __restrict__ rho_io,
-                                                 Real* __restrict__ fp_out, double* __restrict__ ep)
+__global__ void __launch_bounds__(256) k_eam_rho(PairCommon pc, EamTabDev<Real> t, const Real* __restrict__ pair,
+                                                 double* __restrict__ rho_io, double* __restrict__ pos_w,
+                                                 double* __restrict__ ep)
 {
   const int64_t i = logical_block() * blockDim.x + threadIdx.x;
   if (i >= pc.n) return;
@@ -245,19 +279,24 @@ __global__ void __launch_bounds__(256) k_eam_rho(PairCommon pc, EamTabDev<Real> 
   double rho = 0.0;
   if constexpr (RHO) {
     const double* L = pc.box[pc.rep ? pc.rep[i] : 0].L;
-    const double xi = pc.rx[i], yi = pc.ry[i], zi = pc.rz[i];
+    const Pos4 pi = load_pos(pc.pos, i);
     const int cnt = pc.nnb[i];
-    for (int k = 0; k < cnt; k++) {
-      const Disp d = displacement(pc, pc.nbr[(int64_t)k * pc.npad + i], xi, yi, zi, L);
-      if (d.d2 > pc.rc2) continue;
-      const int tj = MULTI ? pc.type[d.j] : 0;
-      Real r, rinv;
-      dist((Real)d.d2, r, rinv);
-      int m;
-      Real p;
-      knot_r(t, r, m, p);
-      const Q4<Real> c = load4(row_r(t.rhor, t, t.t2rhor[tj + 1][ti + 1], m));
-      rho += (double)cubic(c, p);
+    for (int k0 = 0; k0 < cnt; k0 += kBatch) {
+      Nb nb[kBatch];
+      fetch<kBatch>(pc, i, k0, cnt, pi, L, nb);
+#pragma unroll
+      for (int u = 0; u < kBatch; u++) {
+        Real r, rinv;
+        dist((Real)(nb[u].in ? nb[u].d2 : 1.0), r, rinv);
+        int m;
+        Real p;
+        knot_r(t, r, m, p);
+        const Real* row;
+        if constexpr (MULTI) row = row_r(t.rhor, t, t.t2rhor[pc.type[nb[u].j] + 1][ti + 1], m);
+        else row = pair + (int64_t)m * 8;
+        const Real val = cubic(load4(row), p);
+        rho += nb[u].in ? (double)val : 0.0;
+      }
     }
   } else {
     rho = rho_io[i];
@@ -265,7 +304,7 @@ __global__ void __launch_bounds__(256) k_eam_rho(PairCommon pc, EamTabDev<Real> 
   if constexpr (EMB) {
     Real F, Fp;
     embed(t, ti, rho, F, Fp);
-    fp_out[i] = Fp;
+    pos_w[4 * i + 3] = (double)Fp;
     if constexpr (ENERGY) {
       if constexpr (SET_EP) ep[i] = (double)F;
       else ep[i] += (double)F;
@@ -275,50 +314,64 @@ __global__ void __launch_bounds__(256) k_eam_rho(PairCommon pc, EamTabDev<Real> 
   }
 }
 
-// Pass 2: psi'(r) = F'_i rho'_j(r) + F'_j rho'_i(r) + phi'(r), f_i += psi'/r (r_j - r_i).
+// Pass 2: psi'(r) = F'_i rho'_j(r) + F'_j rho'_i(r) + phi'(r), f_i += psi'/r (r_j - r_i),
+// with phi = z2r / r (eam_alloy_mm_force, eam_alloy.h:222-265).
 template <class Real, bool MULTI, bool ENERGY, bool VIRIAL, bool FUSED>
-__global__ void __launch_bounds__(256) k_eam_force(PairCommon pc, EamTabDev<Real> t, const Real* __restrict__ fp,
+__global__ void __launch_bounds__(256) k_eam_force(PairCommon pc, EamTabDev<Real> t, const Real* __restrict__ pair,
                                                    Outputs o)
 {
   const int64_t i = logical_block() * blockDim.x + threadIdx.x;
   if (i >= pc.n) return;
   const int ti = MULTI ? pc.type[i] : 0;
   const double* L = pc.box[pc.rep ? pc.rep[i] : 0].L;
-  const double xi = pc.rx[i], yi = pc.ry[i], zi = pc.rz[i];
-  const Real fpi = fp[i];
+  const Pos4 pi = load_pos(pc.pos, i);
+  const Real fpi = (Real)pi.w;
   double fx = 0.0, fy = 0.0, fz = 0.0, e = 0.0;
   double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   const int cnt = pc.nnb[i];
-  for (int k = 0; k < cnt; k++) {
-    const Disp d = displacement(pc, pc.nbr[(int64_t)k * pc.npad + i], xi, yi, zi, L);
-    if (d.d2 > pc.rc2) continue;
-    const int tj = MULTI ? pc.type[d.j] : 0;
-    const Real fpj = fp[d.j];
-    Real r, rinv;
-    dist((Real)d.d2, r, rinv);
-    int m;
-    Real p;
-    knot_r(t, r, m, p);
-    const Q4<Real> dri = load4(row_r(t.rhor, t, t.t2rhor[ti + 1][tj + 1], m) + 4);
-    const Real rhoip = quad(dri, p);
-    Real rhojp = rhoip;
-    if constexpr (MULTI) rhojp = quad(load4(row_r(t.rhor, t, t.t2rhor[tj + 1][ti + 1], m) + 4), p);
-    const Real* zrow = row_r(t.z2r, t, t.t2z2r[ti + 1][tj + 1], m);
-    const Q4<Real> zv = load4(zrow);
-    const Q4<Real> zd = load4(zrow + 4);
-    const Real z2p = quad(zd, p);
-    const Real z2 = cubic(zv, p);
-    Real phi = z2 * rinv;
-    const Real phip = (z2p * rinv - phi * rinv) * t.conv;
-    phi *= t.conv;
-    const Real psip = fpi * rhojp + fpj * rhoip + phip;
-    const double fpair = (double)(psip * rinv);
-    const double fe0 = d.x * fpair, fe1 = d.y * fpair, fe2 = d.z * fpair;
-    fx += fe0;
-    fy += fe1;
-    fz += fe2;
-    if constexpr (ENERGY) e += 0.5 * (double)phi;
-    if constexpr (VIRIAL) add_virial(v, fe0, fe1, fe2, d);
+  for (int k0 = 0; k0 < cnt; k0 += kBatch) {
+    Nb nb[kBatch];
+    fetch<kBatch>(pc, i, k0, cnt, pi, L, nb);
+#pragma unroll
+    for (int u = 0; u < kBatch; u++) {
+      const Real fpj = (Real)nb[u].w;
+      Real r, rinv;
+      dist((Real)(nb[u].in ? nb[u].d2 : 1.0), r, rinv);
+      int m;
+      Real p;
+      knot_r(t, r, m, p);
+      Real psip, phi;
+      if constexpr (MULTI) {
+        const int tj = pc.type[nb[u].j];
+        const Real rhoip = quad(load4(row_r(t.rhor, t, t.t2rhor[ti + 1][tj + 1], m) + 4), p);
+        const Real rhojp = quad(load4(row_r(t.rhor, t, t.t2rhor[tj + 1][ti + 1], m) + 4), p);
+        const Real* zrow = row_r(t.z2r, t, t.t2z2r[ti + 1][tj + 1], m);
+        const Real z2p = quad(load4(zrow + 4), p);
+        const Real z2 = cubic(load4(zrow), p);
+        phi = z2 * rinv;
+        const Real phip = (z2p * rinv - phi * rinv) * t.conv;
+        phi *= t.conv;
+        psip = fpi * rhojp + fpj * rhoip + phip;
+      } else {
+        const Real* row = pair + (int64_t)m * 8;
+        const Q4<Real> rc = load4(row);
+        const Q4<Real> zc = load4(row + 4);
+        const Real rhop = dcubic(rc, p) * t.rdr;
+        const Real z2p = dcubic(zc, p) * t.rdr;
+        const Real z2 = cubic(zc, p);
+        phi = z2 * rinv;
+        const Real phip = (z2p * rinv - phi * rinv) * t.conv;
+        phi *= t.conv;
+        psip = (fpi + fpj) * rhop + phip;
+      }
+      const double fpair = nb[u].in ? (double)(psip * rinv) : 0.0;
+      const double fe0 = nb[u].x * fpair, fe1 = nb[u].y * fpair, fe2 = nb[u].z * fpair;
+      fx += fe0;
+      fy += fe1;
+      fz += fe2;
+      if constexpr (ENERGY) e += nb[u].in ? 0.5 * (double)phi : 0.0;
+      if constexpr (VIRIAL) add_virial(v, fe0, fe1, fe2, nb[u]);
+    }
   }
   write_outputs<ENERGY, VIRIAL, FUSED>(o, pc, i, fx, fy, fz, e, v);
 }
@@ -328,9 +381,7 @@ PairCommon make_common(Ctx& c, double rcut)
   PairCommon pc;
   pc.n = c.n;
   pc.npad = c.npad;
-  pc.rx = c.rx.p;
-  pc.ry = c.ry.p;
-  pc.rz = c.rz.p;
+  pc.pos = c.pos.p;
   pc.type = c.type.p;
   pc.rep = c.has_rep ? c.rep.p : nullptr;
   pc.box = c.box_d.p;
@@ -411,16 +462,19 @@ void eam_dispatch(Ctx& c, double rcut, bool do_rho, bool do_emb, bool do_force, 
 {
   EamTabDev<Real> t = make_tab<Real>(c);
   t.conv = (Real)kEV;
+  const Real* pair;
+  if constexpr (sizeof(Real) == 4) pair = c.eam.pair_f.p;
+  else pair = c.eam.pair_d.p;
   const PairCommon pc = make_common(c, rcut);
   const unsigned g = pair_grid(c.n);
   const bool energy = flags & XS_ENERGY;
-  Real* fpbuf;
-  if constexpr (sizeof(Real) == 4) fpbuf = c.fpf.p;
-  else fpbuf = c.rho.p;
+  double* pos_w = reinterpret_cast<double*>(c.pos.p);
 
   if (do_rho || do_emb) {
     if (!do_rho && c.rho_state != RhoState::Rho) throw Error(XS_E_STATE, "eam_rho2emb needs eam_rho first");
-#define XS_RHO(R, E, EN, SET) k_eam_rho<Real, MULTI, R, E, EN, SET><<<g, kBlock, 0, c.stream>>>(pc, t, c.rho.p, fpbuf, c.ep.p)
+    ProfScope prof(c.prof, c.stream, XS_PROF_EAM_RHO);
+#define XS_RHO(R, E, EN, SET) \
+  k_eam_rho<Real, MULTI, R, E, EN, SET><<<g, kBlock, 0, c.stream>>>(pc, t, pair, c.rho.p, pos_w, c.ep.p)
     if (do_rho && do_emb) {
       if (energy && epi.fused) XS_RHO(true, true, true, true);
       else if (energy) XS_RHO(true, true, true, false);
@@ -441,7 +495,8 @@ void eam_dispatch(Ctx& c, double rcut, bool do_rho, bool do_emb, bool do_force, 
       throw Error(XS_E_STATE, "eam_force needs F'(rho) from eam_rho2emb in the same precision");
     const Outputs o = make_outputs(c, epi);
     const bool vir = flags & XS_VIRIAL;
-#define XS_FORCE(EN, VI, FU) k_eam_force<Real, MULTI, EN, VI, FU><<<g, kBlock, 0, c.stream>>>(pc, t, fpbuf, o)
+    ProfScope prof(c.prof, c.stream, XS_PROF_EAM_FORCE);
+#define XS_FORCE(EN, VI, FU) k_eam_force<Real, MULTI, EN, VI, FU><<<g, kBlock, 0, c.stream>>>(pc, t, pair, o)
     if (epi.fused) {
       if (energy && vir) XS_FORCE(true, true, true);
       else if (energy) XS_FORCE(true, false, true);
@@ -472,6 +527,7 @@ void lj_dispatch(Ctx& c, double eps, double sigma, double rcut, unsigned flags, 
   const Outputs o = make_outputs(c, epi);
   const unsigned g = pair_grid(c.n);
   const bool energy = flags & XS_ENERGY, vir = flags & XS_VIRIAL;
+  ProfScope prof(c.prof, c.stream, XS_PROF_LJ);
 #define XS_LJ(EN, VI, FU) k_lj<Real, EN, VI, FU><<<g, kBlock, 0, c.stream>>>(pc, lp, o)
   if (epi.fused) {
     if (energy && vir) XS_LJ(true, true, true);

@@ -157,6 +157,26 @@ void eam_upload(Ctx& c)
   upload(c, e.frho_f, cf);
   upload(c, e.rhor_f, cr);
   upload(c, e.z2r_f, cz);
+
+  // combined row of element 0 / pair 00 for single-element kernels:
+  // [rho c3 c4 c5 c6 | z2r c3 c4 c5 c6]
+  const size_t rows = (size_t)e.nr + 1;
+  std::vector<double> pair(8 * rows);
+  for (size_t m = 0; m < rows; m++)
+    for (int k = 0; k < 4; k++) {
+      pair[8 * m + k] = cr[8 * m + 3 + k];
+      pair[8 * m + 4 + k] = cz[8 * m + 3 + k];
+    }
+  auto up = [&](auto& buf, auto zero) {
+    using T = decltype(zero);
+    std::vector<T> tmp(pair.size());
+    for (size_t q = 0; q < pair.size(); q++) tmp[q] = (T)pair[q];
+    buf.reserve(tmp.size());
+    XS_HIP(hipMemcpyAsync(buf.p, tmp.data(), tmp.size() * sizeof(T), hipMemcpyHostToDevice, c.stream));
+    XS_HIP(hipStreamSynchronize(c.stream));
+  };
+  up(e.pair_d, 0.0);
+  up(e.pair_f, 0.0f);
 }
 
 }  // namespace xsmd

@@ -63,6 +63,27 @@ struct DBuf {
   }
 };
 
+// Position record of one atom in HBM: x, y, z and a per-atom scalar w (the
+// embedding derivative F'(rho) between the two EAM passes).  A neighbour gather is
+// one 32-byte sector (two 16-byte loads) instead of several scattered words.
+struct alignas(32) Pos4 {
+  double x, y, z, w;
+};
+
+__device__ __forceinline__ Pos4 load_pos(const Pos4* __restrict__ p, int64_t j)
+{
+  const double2* q = reinterpret_cast<const double2*>(p + j);
+  const double2 a = q[0];
+  const double2 b = q[1];
+  return {a.x, a.y, b.x, b.y};
+}
+__device__ __forceinline__ void store_pos(Pos4* __restrict__ p, int64_t j, const Pos4& v)
+{
+  double2* q = reinterpret_cast<double2*>(p + j);
+  q[0] = make_double2(v.x, v.y);
+  q[1] = make_double2(v.z, v.w);
+}
+
 // Box of one replica, as kernels read it: lo[3], L[3], n/L[3].
 struct BoxDev {
   double lo[3];
@@ -95,9 +116,93 @@ struct EamHost {
   int8_t t2frho[8] = {};
   DBuf<double> frho_d, rhor_d, z2r_d;
   DBuf<float> frho_f, rhor_f, z2r_f;
+  // single-element pair table: row m = [rho c3 c4 c5 c6 | z2r c3 c4 c5 c6]; the
+  // derivative quadratics follow from the value cubics, so one 8-value row
+  // serves both EAM passes
+  DBuf<double> pair_d;
+  DBuf<float> pair_f;
 };
 
+// What the per-atom EAM scalar currently holds: rho (c.rho) after the density
+// pass alone, or F'(rho) (pos[i].w) after the embedding step.
 enum class RhoState { None, Rho, FpDouble, FpFloat };
+
+// Event-pair timing per kernel class (see XS_PROF_* in xsmd.h).
+struct Profiler {
+  bool on = false;
+  std::vector<hipEvent_t> pool;
+  struct Pending {
+    int cls;
+    hipEvent_t a, b;
+  };
+  std::vector<Pending> pending;
+  double ms[XS_PROF_NCLASSES] = {};
+  int64_t count[XS_PROF_NCLASSES] = {};
+
+  hipEvent_t take()
+  {
+    if (pool.empty()) {
+      hipEvent_t e;
+      hip_check(hipEventCreate(&e), "hipEventCreate");
+      return e;
+    }
+    hipEvent_t e = pool.back();
+    pool.pop_back();
+    return e;
+  }
+  // Accumulate every recorded pair; the caller has synchronised the stream.
+  void flush()
+  {
+    for (const Pending& p : pending) {
+      float t = 0.f;
+      hip_check(hipEventElapsedTime(&t, p.a, p.b), "hipEventElapsedTime");
+      ms[p.cls] += t;
+      count[p.cls]++;
+      pool.push_back(p.a);
+      pool.push_back(p.b);
+    }
+    pending.clear();
+  }
+  void reset()
+  {
+    pending.clear();
+    for (int k = 0; k < XS_PROF_NCLASSES; k++) {
+      ms[k] = 0;
+      count[k] = 0;
+    }
+  }
+  ~Profiler()
+  {
+    for (const Pending& p : pending) {
+      (void)hipEventDestroy(p.a);
+      (void)hipEventDestroy(p.b);
+    }
+    for (hipEvent_t e : pool) (void)hipEventDestroy(e);
+  }
+};
+
+// Records a start event at construction and a stop event at destruction on the
+// given stream when profiling is on.
+struct ProfScope {
+  Profiler& p;
+  hipStream_t s;
+  int cls;
+  hipEvent_t a = nullptr;
+  ProfScope(Profiler& prof, hipStream_t stream, int c) : p(prof), s(stream), cls(c)
+  {
+    if (p.on) {
+      a = p.take();
+      hip_check(hipEventRecord(a, s), "hipEventRecord");
+    }
+  }
+  ~ProfScope()
+  {
+    if (a) {
+      hipEvent_t b = p.take();
+      if (hipEventRecord(b, s) == hipSuccess) p.pending.push_back({cls, a, b});
+    }
+  }
+};
 
 struct Ctx {
   int device = 0;
@@ -117,15 +222,15 @@ struct Ctx {
   DBuf<BoxDev> box_d;
 
   // per-atom state in cell-sorted order (and the permutation targets)
-  DBuf<double> rx, ry, rz, vx, vy, vz, ax, ay, az;
-  DBuf<double> rx2, ry2, rz2, vx2, vy2, vz2, ax2, ay2, az2;
+  DBuf<Pos4> pos, pos2;
+  DBuf<double> vx, vy, vz, ax, ay, az;
+  DBuf<double> vx2, vy2, vz2, ax2, ay2, az2;
   DBuf<double> rrx, rry, rrz;  // positions at the last list build
   DBuf<uint8_t> type, type2;
   DBuf<int64_t> id, id2;
   DBuf<uint16_t> rep, rep2;
   bool has_rep = false;
   DBuf<double> ep, vir, rho;
-  DBuf<float> fpf;
   RhoState rho_state = RhoState::None;
 
   // cell binning
@@ -153,12 +258,15 @@ struct Ctx {
   bool has_eam = false;
   EamHost eam;
 
+  Profiler prof;
+
   ~Ctx();
 };
 
 // ---- launches (k_neighbors.hip) ----
 void build_neighbor_list(Ctx& c, double rcut, double skin);
 double max_displacement2(Ctx& c);
+void pair_count(Ctx& c, double rcut, int64_t& entries, int64_t& within);
 
 // ---- launches (k_integrate.hip) ----
 void launch_zero(Ctx& c);

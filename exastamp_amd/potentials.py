@@ -143,10 +143,11 @@ def _fcc_shells(a, rmax):
     g = np.arange(-2 * n, 2 * n + 1)
     i, j, k = np.meshgrid(g, g, g, indexing="ij")
     sel = ((i + j + k) % 2 == 0) & ~((i == 0) & (j == 0) & (k == 0))
-    d = 0.5 * a * np.sqrt(i[sel] ** 2 + j[sel] ** 2 + k[sel] ** 2)
-    d = d[d < rmax]
-    u, c = np.unique(np.round(d, 10), return_counts=True)
-    return u, c
+    s = (i[sel] ** 2 + j[sel] ** 2 + k[sel] ** 2).astype(np.int64)  # exact shell index
+    u, c = np.unique(s, return_counts=True)
+    d = 0.5 * a * np.sqrt(u.astype(np.float64))
+    keep = d < rmax
+    return d[keep], c[keep]
 
 
 def _rho_fn(el, rs, rc):

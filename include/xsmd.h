@@ -47,6 +47,8 @@ typedef enum {
 #define XS_VIRIAL 2u   /* per-atom virial tensor (9 components, row-major) */
 #define XS_MIXED 4u    /* FP32 pair/spline math; FP64 positions, displacements,
                           accumulation and reductions.  Without it: all FP64. */
+#define XS_CONTINUE 8u /* xs_md_run: continue a previous run (valid list and current
+                          accelerations): skip the initial build and force step */
 
 /* potentials for xs_md_run */
 #define XS_POT_LJ 0
@@ -109,6 +111,10 @@ xs_status xs_neighbors_get(xs_ctx* ctx, int64_t* counts, int64_t* total, int64_t
                            int32_t* imgs);
 /* perm[k] = id of the atom at internal (cell-sorted) position k. */
 xs_status xs_order_get(xs_ctx* ctx, int64_t* perm);
+/* Neighbour statistics (the chunk_neighbors_stats diagnostic of
+ * data/samples/perf/microjet_lj.msp:25-28): list entries and entries with
+ * d^2 <= rcut^2 at the current positions. */
+xs_status xs_pair_count(xs_ctx* ctx, double rcut, int64_t* entries, int64_t* within_rcut);
 
 /* --- operators -------------------------------------------------------------------- */
 /* zero_force_energy (src/compute/zero_force_energy.cu:38-147): a = ep = virial = 0. */
@@ -163,6 +169,22 @@ typedef struct xs_run_stats {
 xs_status xs_md_run(xs_ctx* ctx, int potential, int nsteps, double dt, double rcut,
                     double skin, unsigned flags, int thermo_every, double* thermo_out,
                     xs_run_stats* stats);
+
+/* --- timing ------------------------------------------------------------------------ */
+/* GPU time per kernel class, measured with HIP events recorded on the context's stream
+ * around every launch of that class (the per-operator timers of onika's profiling
+ * mode, data/config/config_defaults.msp:17-23). */
+#define XS_PROF_NEIGHBORS 0   /* whole list rebuild: wrap, sort, permute, build */
+#define XS_PROF_EAM_RHO 1     /* density pass (+ embedding) */
+#define XS_PROF_EAM_FORCE 2   /* force pass (+ force_to_accel and half kick when fused) */
+#define XS_PROF_LJ 3
+#define XS_PROF_DRIFT 4       /* push_f_v_r + push_f_v + displacement check */
+#define XS_PROF_THERMO 5
+#define XS_PROF_NCLASSES 6
+/* on != 0 enables and clears the counters; on == 0 disables. */
+xs_status xs_profile_enable(xs_ctx* ctx, int on);
+/* ms_total[XS_PROF_NCLASSES] and launches[XS_PROF_NCLASSES] (either may be NULL). */
+xs_status xs_profile_get(xs_ctx* ctx, double* ms_total, int64_t* launches);
 
 #ifdef __cplusplus
 }

@@ -780,10 +780,21 @@ int or_md_run(or_ctx* c, int potential, int nsteps, double dt, double rcut, doub
 {
   const int eflags = thermo_out ? (OR_ENERGY | OR_VIRIAL) : 0;
   or_build_neighbors(c, rcut, skin);
-  int nrebuild = 1;
   int rc = or_compute_forces(c, potential, eflags);
   if (rc) return rc;
   if (thermo_out) or_thermo(c, thermo_out);
+  rc = or_md_steps(c, potential, nsteps, dt, rcut, skin, thermo_out ? thermo_out + 27 : nullptr);
+  return rc < 0 ? rc : rc + 1;
+}
+
+// The verlet_nve steps alone, continuing from current accelerations and list;
+// returns the number of rebuilds.  thermo_out (may be NULL) receives nsteps x 27.
+int or_md_steps(or_ctx* c, int potential, int nsteps, double dt, double rcut, double skin,
+                double* thermo_out)
+{
+  if (!c->list_valid) return -1;
+  const int eflags = thermo_out ? (OR_ENERGY | OR_VIRIAL) : 0;
+  int nrebuild = 0;
   for (int step = 1; step <= nsteps; step++) {
     or_push_f_v_r(c, dt * 1.0);
     or_push_f_v(c, dt * 0.5);
@@ -793,7 +804,7 @@ int or_md_run(or_ctx* c, int potential, int nsteps, double dt, double rcut, doub
     }
     or_compute_forces(c, potential, eflags);
     or_push_f_v(c, dt * 0.5);
-    if (thermo_out) or_thermo(c, thermo_out + 27 * (size_t)step);
+    if (thermo_out) or_thermo(c, thermo_out + 27 * (size_t)(step - 1));
   }
   return nrebuild;
 }

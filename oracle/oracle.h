@@ -74,6 +74,10 @@ int or_thermo(or_ctx* c, double out[27]);
  * number of neighbour-list rebuilds (>= 1) or a negative error. */
 int or_md_run(or_ctx* c, int potential, int nsteps, double dt, double rcut, double skin,
               double* thermo_out);
+/* The steps alone, continuing from the current list and accelerations; thermo_out
+ * (may be NULL) receives nsteps x 27.  Returns the number of rebuilds. */
+int or_md_steps(or_ctx* c, int potential, int nsteps, double dt, double rcut, double skin,
+                double* thermo_out);
 
 /* --- read back (id order) ------------------------------------------------ */
 int64_t or_n(const or_ctx* c);

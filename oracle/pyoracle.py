@@ -49,6 +49,7 @@ def load():
             "or_displ_over": (C.c_int, [C.c_void_p, C.c_double]),
             "or_thermo": (C.c_int, [C.c_void_p, _D]),
             "or_md_run": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, _D]),
+            "or_md_steps": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, _D]),
             "or_n": (C.c_int64, [C.c_void_p]),
             "or_get_state": (C.c_int, [C.c_void_p] + [_D] * 12),
             "or_neighbor_count": (C.c_int64, [C.c_void_p, _I64]),
@@ -171,6 +172,14 @@ class Oracle:
         out = np.zeros((nsteps + 1) * 27) if thermo else None
         nreb = self._ok(self.lib.or_md_run(self.h, pot, nsteps, dt, rcut, skin, _d(out)), "md_run")
         return nreb, (out.reshape(nsteps + 1, 27) if thermo else None)
+
+    def md_steps(self, potential, nsteps, dt, rcut, skin):
+        """Continue for nsteps (no initial build/force); returns the rebuild count."""
+        pot = {"lj": OR_POT_LJ, "eam": OR_POT_EAM}[potential]
+        rc = self.lib.or_md_steps(self.h, pot, nsteps, dt, rcut, skin, None)
+        if rc < 0:
+            raise RuntimeError(f"oracle md_steps failed ({rc})")
+        return rc
 
     def state(self):
         n = self.n

@@ -1,0 +1,77 @@
+"""Host-side logic: units, seeded inputs, table generation (no GPU)."""
+import numpy as np
+import pytest
+
+
+def test_units_parse_deck_quantities():
+    from exastamp_amd import units
+    # 1 eV = e / (amu * 1e4) Da.A^2/ps^2 (CODATA-2018) ~ 9648.53
+    assert units.parse_quantity("0.0104 eV") == pytest.approx(0.0104 * 1.602176634e-19 / 1.66053906660e-23,
+                                                             rel=1e-14)
+    assert 9648.5 < units.EV < 9648.6
+    assert units.parse_quantity("3.4 ang") == 3.4
+    assert units.parse_quantity("39.948 Da") == 39.948
+    assert units.parse_quantity("1.0e-3 ps") == 1e-3
+    assert units.parse_quantity("2 fs") == pytest.approx(0.002)
+    assert units.parse_quantity("50. m/s") == pytest.approx(0.5)  # A/ps
+    assert units.parse_quantity(4.95) == 4.95
+    with pytest.raises(ValueError):
+        units.parse_quantity("3 furlongs")
+    # kB T at 300 K in eV
+    assert units.KB * 300 / units.EV == pytest.approx(0.025852, rel=1e-4)
+
+
+def test_maxwell_boltzmann_exact_temperature_zero_momentum():
+    from exastamp_amd import inputs, units
+    cfg = inputs.lattice_config((6, 6, 6), 3.615, [63.546], T=600.0, seed=3)
+    m = 63.546
+    p = (m * cfg.v).sum(0)
+    assert np.abs(p).max() < 1e-9 * m * np.abs(cfg.v).max() * cfg.n
+    ke = 0.5 * m * (cfg.v ** 2).sum()
+    assert units.temperature_from_ke(ke, cfg.n) == pytest.approx(600.0, rel=1e-12)
+
+
+def test_inputs_are_seeded_and_deterministic():
+    from exastamp_amd import inputs
+    a = inputs.lattice_config((4, 4, 4), 3.615, [63.546], T=300.0, jitter=0.05, seed=11)
+    b = inputs.lattice_config((4, 4, 4), 3.615, [63.546], T=300.0, jitter=0.05, seed=11)
+    c = inputs.lattice_config((4, 4, 4), 3.615, [63.546], T=300.0, jitter=0.05, seed=12)
+    assert np.array_equal(a.r, b.r) and np.array_equal(a.v, b.v)
+    assert not np.array_equal(a.r, c.r)
+    assert a.n == 256
+
+
+def test_fcc_neighbour_shell_counts():
+    """12, 6, 24, 12, 24 neighbours in the first five FCC shells (Cu: 42 within 4.95 A,
+    54 within 5.25 A)."""
+    from exastamp_amd.potentials import _fcc_shells
+    d, c = _fcc_shells(3.615, 5.8)
+    assert list(c[:5]) == [12, 6, 24, 12, 24]
+    assert c[d < 4.95].sum() == 42 and c[d < 5.25].sum() == 54
+
+
+def test_alloy_slab_and_batch_layouts():
+    from exastamp_amd import inputs
+    s = inputs.alloy_slab_config((8, 8, 8), 3.57, [63.546, 58.693], vacancy_frac=0.01, vacuum=10.0, seed=1)
+    assert s.periodic == (1, 1, 0)
+    assert 0.4 < s.type.mean() < 0.6
+    assert 0.97 * 2048 < s.n < 2048
+    assert s.r[:, 2].min() >= 10.0 and s.r[:, 2].max() <= s.hi[0, 2] - 10.0 + 1e-9
+    b = inputs.strained_batch(6, (3, 3, 3), 3.615, [63.546], seed=2)
+    assert b.nrep == 6 and b.n == 6 * 108
+    L = b.hi[:, 0]
+    assert np.all(np.abs(L / (3 * 3.615) - 1) <= 0.03 + 1e-12)
+    parts = inputs.split_replicas(b)
+    assert sum(p.n for p in parts) == b.n
+
+
+def test_synthetic_tables_zero_pressure_lattice():
+    """r0 of the Morse term is solved so that dE/da = 0 at the lattice constant."""
+    from exastamp_amd import potentials
+    t = potentials.johnson_tables(nr=2000, nrho=2000)
+    assert t.nel == 1 and t.frho.shape == (1, 2000) and t.z2r.shape == (1, 2000)
+    assert t.rhor[0, -1] == pytest.approx(0.0, abs=1e-12) and t.z2r[0, -1] == pytest.approx(0.0, abs=1e-12)
+    theta = t.theta()
+    assert theta.size == 6000
+    t2 = t.with_theta(theta * 1.01)
+    assert np.allclose(t2.frho, 1.01 * t.frho)
