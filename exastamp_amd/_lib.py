@@ -24,7 +24,7 @@ XS_FIELD_VIRIAL = 1
 XS_FIELD_RHO_DEMB = 2
 XS_FIELD_ACCEL = 3
 
-PROF_CLASSES = ["neighbors", "eam_rho", "eam_force", "lj", "drift", "thermo"]
+PROF_CLASSES = ["neighbors", "eam_rho", "eam_force", "lj", "drift", "thermo", "adjoint"]
 
 STATUS = {0: "XS_OK", 1: "XS_E_ARG", 2: "XS_E_STATE", 3: "XS_E_HIP", 4: "XS_E_ALLOC", 5: "XS_E_IO",
           6: "XS_E_LIMIT"}
@@ -77,6 +77,8 @@ SIGNATURES = {
     "xs_push_f_v_r": (C.c_int, [_CTX, C.c_double, C.c_double]),
     "xs_push_f_v": (C.c_int, [_CTX, C.c_double, C.c_double]),
     "xs_thermo": (C.c_int, [_CTX, _D]),
+    "xs_eam_adjoint": (C.c_int, [_CTX, C.c_double, _D, _D, _D, C.c_uint]),
+    "xs_spline_adjoint": (C.c_int, [C.c_int, C.c_double, _D, _D]),
     "xs_profile_enable": (C.c_int, [_CTX, C.c_int]),
     "xs_profile_get": (C.c_int, [_CTX, _D, _I64]),
     "xs_md_run": (C.c_int, [_CTX, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_uint, C.c_int,

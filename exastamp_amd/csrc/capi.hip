@@ -249,6 +249,7 @@ xs_status xs_system_set(xs_ctx* h, int64_t n, const double* rx, const double* ry
     allocate_atoms(c, n);
     c.list_valid = false;
     c.rho_state = RhoState::None;
+    c.rho_ckpt = false;
     if (n > 0) {
       std::vector<Pos4> p4(n);
       for (int64_t i = 0; i < n; i++) p4[i] = Pos4{rx[i], ry[i], rz[i], 0.0};
@@ -462,6 +463,31 @@ xs_status xs_eam_alloy_force(xs_ctx* h, double rcut, int eam_rho, int eam_rho2em
     }
     launch_eam(c, rcut, eam_rho != 0, eam_rho2emb != 0, eam_force != 0, flags, ForceEpilogue{});
   });
+}
+
+xs_status xs_eam_adjoint(xs_ctx* h, double rcut, const double* dL_dF, const double* dL_dE, double* grad,
+                         unsigned flags)
+{
+  return guarded(h, [&](Ctx& c) {
+    require(dL_dF && dL_dE && grad, XS_E_ARG, "null adjoint argument");
+    require(c.has_eam, XS_E_STATE, "eam tables not set");
+    require_list(c);
+    const bool mixed = (flags & XS_MIXED) != 0;
+    require((c.rho_state == RhoState::FpFloat || c.rho_state == RhoState::FpDouble) && c.rho_ckpt, XS_E_STATE,
+            "adjoint needs the rho/F' checkpoint of an eam_alloy_force call at the current positions");
+    eam_adjoint(c, rcut, dL_dF, dL_dE, grad, mixed);
+  });
+}
+
+xs_status xs_spline_adjoint(int n, double delta, const double* G, double* out)
+{
+  if (n < 5 || !G || !out || !(delta > 0)) return XS_E_ARG;
+  try {
+    spline_interpolate_adjoint(n, delta, G, out);
+  } catch (...) {
+    return XS_E_ARG;
+  }
+  return XS_OK;
 }
 
 xs_status xs_push_f_v_r(xs_ctx* h, double dt, double dt_scale)

@@ -1,0 +1,348 @@
+// k_adjoint.hip -- reverse mode of the EAM force/energy with respect to the
+// tabulated knots (no counterpart in exaStamp; the fitting capability built on
+// its eam_alloy tables).
+//
+// For a loss L(F, E) with g_i = dL/dF_i and per-replica gE = dL/dE:
+//   dL/dtheta = sum_i gE_i dE/dtheta - dD/dtheta,   D = sum_i g_i . dE/dx_i,
+// because F_i = -dE/dx_i.  With u_ij = (x_i - x_j)/r and s_ij = u_ij . (g_i - g_j),
+//   D = sum_i F'_i drho_i + 1/2 sum_i sum_j phi'_ij s_ij,   drho_i = sum_j rho'_tj(r_ij) s_ij.
+// Differentiating through both EAM passes (rho_i inside F'_i is the mixed second
+// derivative) gives, per atom and per pair of the full list,
+//   F table at rho_i : value weight  gE_i conv,
+//                      slope weight  conv (gE_i (rho_i - rhomax)^+ - drho_i)
+//   rho table at r_ij: value weight  gE_i F'_i - lambda_i,  slope weight  -F'_i s_ij,
+//                      lambda_i = F''(rho_i) drho_i
+//   z2r table at r_ij: value weight  conv (gE_i / (2r) + s_ij / (2r^2)),
+//                      slope weight  -conv s_ij / (2r)
+// where "slope" multiplies the derivative of the spline with respect to r (or rho).
+// rho_i and F'_i come from the forward pass (checkpointed, not recomputed).
+//
+// Each weight is spread on the four value coefficients c3..c6 of the spline row
+// it falls in (the slope coefficients c0..c2 are 3c3/D, 2c4/D, c5/D), accumulated
+// in FP64; the host then applies the transpose of the knot -> coefficient map.
+#include "xsmd_internal.hpp"
+
+#include <algorithm>
+
+namespace xsmd {
+namespace {
+
+__device__ __forceinline__ int64_t lblock()
+{
+  const unsigned b = blockIdx.x, g = gridDim.x;
+  return (int64_t)(b & 7u) * (g >> 3) + (b >> 3);
+}
+
+// Only the table storage follows Real: distances, knot fractions, weights and
+// accumulations are FP64 (the sweeps are bound by gathers and atomics, and the
+// knot fraction of an FP32 r would carry the rounding of r times the knot count).
+template <class Real>
+struct AdjTab {
+  const Real* frho;  // reference layout rows [c3 c4 c5 c6 | c0 c1 c2 0]
+  const Real* rhor;
+  const Real* z2r;
+  int nr, nrho;
+  double rdr, rdrho, rhomax, conv;
+  int8_t t2rhor[8][8];
+  int8_t t2z2r[8][8];
+  int8_t t2frho[8];
+};
+
+struct AdjArgs {
+  int64_t n, npad;
+  const Pos4* pos;        // x y z F'_i
+  const Pos4* gvec;       // g_i (dL/dF_i) in cell order, w unused
+  const double* rho;      // checkpointed rho_i
+  const uint8_t* type;
+  const uint16_t* rep;
+  const BoxDev* box;
+  const uint32_t* nbr;
+  const int* nnb;
+  const double* gE;       // per replica
+  double rc2;
+  double* lam;            // lambda_i (B1 out, B2 in)
+  double* gF;             // F table coefficient gradients, 4 per row
+  double* gR;             // rho tables
+  double* gZ;             // z2r tables
+};
+
+__device__ __forceinline__ void spread(double* row4, double wv, double ws, double p, double rscale)
+{
+  // value v = c3 p^3 + c4 p^2 + c5 p + c6 ; slope dv/dx = (3 c3 p^2 + 2 c4 p + c5) * rscale
+  const double s = ws * rscale;
+  atomicAdd(row4 + 0, wv * p * p * p + s * 3.0 * p * p);
+  atomicAdd(row4 + 1, wv * p * p + s * 2.0 * p);
+  atomicAdd(row4 + 2, wv * p + s);
+  atomicAdd(row4 + 3, wv);
+}
+
+__device__ __forceinline__ void knot(double x, double rscale, int nmax1, int mmin, int& m, double& p,
+                                     bool& clamped)
+{
+  p = x * rscale + 1.0;
+  m = (int)p;
+  m = max(mmin, min(m, nmax1));
+  p -= (double)m;
+  clamped = p > 1.0;
+  p = fmin(p, 1.0);
+}
+
+template <class Real>
+__device__ __forceinline__ double quad4(const Real* row, double p)
+{
+  return ((double)row[4] * p + (double)row[5]) * p + (double)row[6];
+}
+
+// B1: per atom, drho_i and lambda_i, plus the F-table gradient.
+template <class Real, bool MULTI>
+__global__ void __launch_bounds__(256) k_adj_b1(AdjArgs a, AdjTab<Real> t)
+{
+  const int64_t i = lblock() * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  const int ti = MULTI ? a.type[i] : 0;
+  const int rep = a.rep ? a.rep[i] : 0;
+  const double* L = a.box[rep].L;
+  const Pos4 pi = load_pos(a.pos, i);
+  const Pos4 gi = load_pos(a.gvec, i);
+  double drho = 0.0;
+  const int cnt = a.nnb[i];
+  for (int k = 0; k < cnt; k++) {
+    const uint32_t e = a.nbr[(int64_t)k * a.npad + i];
+    const uint32_t j = xsmd_nbr_index(e);
+    const Pos4 pj = load_pos(a.pos, j);
+    const double dx = xsmd_image(pj.x, xsmd_nbr_img_x(e), L[0]) - pi.x;
+    const double dy = xsmd_image(pj.y, xsmd_nbr_img_y(e), L[1]) - pi.y;
+    const double dz = xsmd_image(pj.z, xsmd_nbr_img_z(e), L[2]) - pi.z;
+    const double d2 = xsmd_d2(dx, dy, dz);
+    if (d2 > a.rc2) continue;
+    const Pos4 gj = load_pos(a.gvec, j);
+    const double r = sqrt(d2);
+    // s_ij = u_ij . (g_i - g_j), u_ij = -(dx,dy,dz)/r
+    const double s = (dx * (gj.x - gi.x) + dy * (gj.y - gi.y) + dz * (gj.z - gi.z)) / r;
+    const int tj = MULTI ? a.type[j] : 0;
+    int m;
+    double p;
+    bool cl;
+    knot(r, t.rdr, t.nr - 1, 0, m, p, cl);
+    const Real* row = t.rhor + ((int64_t)t.t2rhor[tj + 1][ti + 1] * (t.nr + 1) + m) * 8;
+    drho += quad4(row, p) * s;
+  }
+  const double rho = a.rho[i];
+  int m;
+  double p;
+  bool cl;
+  knot(rho, t.rdrho, t.nrho - 1, 1, m, p, cl);
+  const int ft = t.t2frho[ti + 1];
+  const Real* row = t.frho + ((int64_t)ft * (t.nrho + 1) + m) * 8;
+  // F'' = d/drho of (c0 p + c1) p + c2, zero where p was clamped
+  const double fpp = cl ? 0.0 : (2.0 * (double)row[4] * p + (double)row[5]) * t.rdrho * t.conv;
+  a.lam[i] = fpp * drho;
+  const double ge = a.gE[rep];
+  const double conv = t.conv;
+  const double over = rho > t.rhomax ? rho - t.rhomax : 0.0;
+  spread(a.gF + ((int64_t)ft * (t.nrho + 1) + m) * 4, ge * conv, conv * (ge * over - drho), p, t.rdrho);
+}
+
+// B2: per pair, the rho- and z2r-table gradients.
+template <class Real, bool MULTI>
+__global__ void __launch_bounds__(256) k_adj_b2(AdjArgs a, AdjTab<Real> t)
+{
+  const int64_t i = lblock() * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  const int ti = MULTI ? a.type[i] : 0;
+  const int rep = a.rep ? a.rep[i] : 0;
+  const double* L = a.box[rep].L;
+  const Pos4 pi = load_pos(a.pos, i);
+  const Pos4 gi = load_pos(a.gvec, i);
+  const double fpi = pi.w;
+  const double lam = a.lam[i];
+  const double ge = a.gE[rep];
+  const double conv = t.conv;
+  const double rdr = t.rdr;
+  const int cnt = a.nnb[i];
+  for (int k = 0; k < cnt; k++) {
+    const uint32_t e = a.nbr[(int64_t)k * a.npad + i];
+    const uint32_t j = xsmd_nbr_index(e);
+    const Pos4 pj = load_pos(a.pos, j);
+    const double dx = xsmd_image(pj.x, xsmd_nbr_img_x(e), L[0]) - pi.x;
+    const double dy = xsmd_image(pj.y, xsmd_nbr_img_y(e), L[1]) - pi.y;
+    const double dz = xsmd_image(pj.z, xsmd_nbr_img_z(e), L[2]) - pi.z;
+    const double d2 = xsmd_d2(dx, dy, dz);
+    if (d2 > a.rc2) continue;
+    const Pos4 gj = load_pos(a.gvec, j);
+    const double r = sqrt(d2);
+    const double s = (dx * (gj.x - gi.x) + dy * (gj.y - gi.y) + dz * (gj.z - gi.z)) / r;
+    const int tj = MULTI ? a.type[j] : 0;
+    int m;
+    double p;
+    bool cl;
+    knot(r, t.rdr, t.nr - 1, 0, m, p, cl);
+    const int tr = t.t2rhor[tj + 1][ti + 1];
+    spread(a.gR + ((int64_t)tr * (t.nr + 1) + m) * 4, ge * fpi - lam, -fpi * s, p, rdr);
+    const int tz = t.t2z2r[ti + 1][tj + 1];
+    const double ir = 1.0 / r;
+    spread(a.gZ + ((int64_t)tz * (t.nr + 1) + m) * 4, conv * (0.5 * ge * ir + 0.5 * s * ir * ir),
+                 -conv * 0.5 * s * ir, p, rdr);
+  }
+}
+
+// g (dL/dF) from id order into cell order as position-like records
+__global__ void k_gather_g(int64_t n, const int64_t* __restrict__ id, const double* __restrict__ g_id,
+                           Pos4* __restrict__ gvec)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int64_t q = id[i];
+  store_pos(gvec, i, Pos4{g_id[3 * q], g_id[3 * q + 1], g_id[3 * q + 2], 0.0});
+}
+
+template <class Real>
+AdjTab<Real> adj_tab(Ctx& c);
+
+template <>
+AdjTab<double> adj_tab<double>(Ctx& c)
+{
+  AdjTab<double> t;
+  t.frho = c.eam.frho_d.p;
+  t.rhor = c.eam.rhor_d.p;
+  t.z2r = c.eam.z2r_d.p;
+  t.rdr = c.eam.rdr;
+  t.rdrho = c.eam.rdrho;
+  t.rhomax = c.eam.rhomax;
+  t.conv = 1.602176634e-19 / (1.66053906660e-27 * 1.0e4);
+  t.nr = c.eam.nr;
+  t.nrho = c.eam.nrho;
+  std::memcpy(t.t2rhor, c.eam.t2rhor, sizeof(t.t2rhor));
+  std::memcpy(t.t2z2r, c.eam.t2z2r, sizeof(t.t2z2r));
+  std::memcpy(t.t2frho, c.eam.t2frho, sizeof(t.t2frho));
+  return t;
+}
+
+template <>
+AdjTab<float> adj_tab<float>(Ctx& c)
+{
+  AdjTab<float> t;
+  t.frho = c.eam.frho_f.p;
+  t.rhor = c.eam.rhor_f.p;
+  t.z2r = c.eam.z2r_f.p;
+  t.rdr = c.eam.rdr;
+  t.rdrho = c.eam.rdrho;
+  t.rhomax = c.eam.rhomax;
+  t.conv = 1.602176634e-19 / (1.66053906660e-27 * 1.0e4);
+  t.nr = c.eam.nr;
+  t.nrho = c.eam.nrho;
+  std::memcpy(t.t2rhor, c.eam.t2rhor, sizeof(t.t2rhor));
+  std::memcpy(t.t2z2r, c.eam.t2z2r, sizeof(t.t2z2r));
+  std::memcpy(t.t2frho, c.eam.t2frho, sizeof(t.t2frho));
+  return t;
+}
+
+template <class Real>
+void adj_run(Ctx& c, const AdjArgs& a)
+{
+  const AdjTab<Real> t = adj_tab<Real>(c);
+  const unsigned g = (grid_for(c.n) + 7u) / 8u * 8u;
+  if (c.eam.nel > 1) {
+    k_adj_b1<Real, true><<<g, kBlock, 0, c.stream>>>(a, t);
+    k_adj_b2<Real, true><<<g, kBlock, 0, c.stream>>>(a, t);
+  } else {
+    k_adj_b1<Real, false><<<g, kBlock, 0, c.stream>>>(a, t);
+    k_adj_b2<Real, false><<<g, kBlock, 0, c.stream>>>(a, t);
+  }
+  XS_HIP(hipGetLastError());
+}
+
+}  // namespace
+
+// Transpose of the knot -> coefficient map of spline_interpolate for one table:
+// G holds dL/d(c3,c4,c5,c6) per row (rows 0..n), out receives dL/df for the n knots.
+void spline_interpolate_adjoint(int n, double delta, const double* G, double* out)
+{
+  std::vector<double> gf(n + 2, 0.0), gs(n + 2, 0.0);
+  (void)delta;  // slope coefficients were folded into c3..c5 on the device
+  for (int m = 1; m <= n; m++) {
+    const double* g = G + 4 * (size_t)m;
+    gf[m] += g[3];  // c6 = f
+    gs[m] += g[2];  // c5 = s
+    if (m <= n - 1) {
+      // c4 = 3 (f[m+1] - f[m]) - 2 s[m] - s[m+1]
+      gf[m + 1] += 3.0 * g[1];
+      gf[m] -= 3.0 * g[1];
+      gs[m] -= 2.0 * g[1];
+      gs[m + 1] -= g[1];
+      // c3 = s[m] + s[m+1] - 2 (f[m+1] - f[m])
+      gs[m] += g[0];
+      gs[m + 1] += g[0];
+      gf[m + 1] -= 2.0 * g[0];
+      gf[m] += 2.0 * g[0];
+    }
+  }
+  // slopes
+  gf[2] += gs[1];
+  gf[1] -= gs[1];
+  gf[3] += 0.5 * gs[2];
+  gf[1] -= 0.5 * gs[2];
+  gf[n] += 0.5 * gs[n - 1];
+  gf[n - 2] -= 0.5 * gs[n - 1];
+  gf[n] += gs[n];
+  gf[n - 1] -= gs[n];
+  for (int m = 3; m <= n - 2; m++) {
+    const double w = gs[m] / 12.0;
+    gf[m - 2] += w;
+    gf[m + 2] -= w;
+    gf[m + 1] += 8.0 * w;
+    gf[m - 1] -= 8.0 * w;
+  }
+  for (int m = 1; m <= n; m++) out[m - 1] = gf[m];
+}
+
+void eam_adjoint(Ctx& c, double rcut, const double* dLdF_id, const double* dLdE, double* grad, bool mixed)
+{
+  EamHost& e = c.eam;
+  const int nel = e.nel, nz = nel * (nel + 1) / 2;
+  const size_t rows_r = (size_t)e.nr + 1, rows_f = (size_t)e.nrho + 1;
+  const size_t nF = (size_t)nel * rows_f * 4, nR = (size_t)nel * rows_r * 4, nZ = (size_t)nz * rows_r * 4;
+  c.adj_g.reserve(nF + nR + nZ);
+  c.adj_lam.reserve(c.npad);
+  c.adj_gvec.reserve(c.npad);
+  c.adj_in.reserve(3 * (size_t)c.n + c.nrep);
+  XS_HIP(hipMemsetAsync(c.adj_g.p, 0, sizeof(double) * (nF + nR + nZ), c.stream));
+  XS_HIP(hipMemcpyAsync(c.adj_in.p, dLdF_id, sizeof(double) * 3 * c.n, hipMemcpyHostToDevice, c.stream));
+  XS_HIP(hipMemcpyAsync(c.adj_in.p + 3 * c.n, dLdE, sizeof(double) * c.nrep, hipMemcpyHostToDevice, c.stream));
+  if (c.n > 0) {
+    k_gather_g<<<grid_for(c.n), kBlock, 0, c.stream>>>(c.n, c.id.p, c.adj_in.p, c.adj_gvec.p);
+    XS_HIP(hipGetLastError());
+    AdjArgs a;
+    a.n = c.n;
+    a.npad = c.npad;
+    a.pos = c.pos.p;
+    a.gvec = c.adj_gvec.p;
+    a.rho = c.rho.p;
+    a.type = c.type.p;
+    a.rep = c.has_rep ? c.rep.p : nullptr;
+    a.box = c.box_d.p;
+    a.nbr = c.nbr.p;
+    a.nnb = c.nnb.p;
+    a.gE = c.adj_in.p + 3 * c.n;
+    a.rc2 = rcut * rcut;
+    a.lam = c.adj_lam.p;
+    a.gF = c.adj_g.p;
+    a.gR = c.adj_g.p + nF;
+    a.gZ = c.adj_g.p + nF + nR;
+    ProfScope prof(c.prof, c.stream, XS_PROF_ADJOINT);
+    if (mixed) adj_run<float>(c, a);
+    else adj_run<double>(c, a);
+  }
+  std::vector<double> G(nF + nR + nZ);
+  XS_HIP(hipMemcpyAsync(G.data(), c.adj_g.p, sizeof(double) * G.size(), hipMemcpyDeviceToHost, c.stream));
+  XS_HIP(hipStreamSynchronize(c.stream));
+  double* out = grad;
+  for (int t = 0; t < nel; t++, out += e.nrho)
+    spline_interpolate_adjoint(e.nrho, e.drho, G.data() + (size_t)t * rows_f * 4, out);
+  for (int t = 0; t < nel; t++, out += e.nr)
+    spline_interpolate_adjoint(e.nr, e.dr, G.data() + nF + (size_t)t * rows_r * 4, out);
+  for (int t = 0; t < nz; t++, out += e.nr)
+    spline_interpolate_adjoint(e.nr, e.dr, G.data() + nF + nR + (size_t)t * rows_r * 4, out);
+}
+
+}  // namespace xsmd

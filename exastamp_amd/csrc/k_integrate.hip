@@ -173,6 +173,7 @@ void launch_force_to_accel(Ctx& c)
 void launch_push_f_v_r(Ctx& c, double dt)
 {
   if (c.n == 0) return;
+  c.rho_ckpt = false;
   const double dt2 = dt * dt * 0.5;
   k_push_f_v_r<<<grid_for(c.n), kBlock, 0, c.stream>>>(c.n, dt, dt2, c.pos.p, c.vx.p, c.vy.p, c.vz.p,
                                                        c.ax.p, c.ay.p, c.az.p);
@@ -190,6 +191,7 @@ void launch_push_f_v(Ctx& c, double dt)
 void launch_drift_kick(Ctx& c, double dt)
 {
   if (c.n == 0) return;
+  c.rho_ckpt = false;
   const double dt2 = dt * dt * 0.5;
   ProfScope prof(c.prof, c.stream, XS_PROF_DRIFT);
   const unsigned g = std::min<unsigned>(grid_for(c.n), 4096);

@@ -272,6 +272,7 @@ void build_neighbor_list(Ctx& c, double rcut, double skin)
   c.id.swap(c.id2);
   if (c.has_rep) c.rep.swap(c.rep2);
   c.rho_state = RhoState::None;  // per-atom EAM state is in the old order
+  c.rho_ckpt = false;
 
   c.cstart.reserve(c.ncell_tot);
   c.cend.reserve(c.ncell_tot);

@@ -298,6 +298,7 @@ __global__ void __launch_bounds__(256) k_eam_rho(PairCommon pc, EamTabDev<Real> 
         rho += nb[u].in ? (double)val : 0.0;
       }
     }
+    rho_io[i] = rho;  // kept: the density checkpoint of the adjoint
   } else {
     rho = rho_io[i];
   }
@@ -309,8 +310,6 @@ __global__ void __launch_bounds__(256) k_eam_rho(PairCommon pc, EamTabDev<Real> 
       if constexpr (SET_EP) ep[i] = (double)F;
       else ep[i] += (double)F;
     }
-  } else {
-    rho_io[i] = rho;
   }
 }
 
@@ -488,6 +487,7 @@ void eam_dispatch(Ctx& c, double rcut, bool do_rho, bool do_emb, bool do_force, 
 #undef XS_RHO
     XS_HIP(hipGetLastError());
     c.rho_state = do_emb ? (sizeof(Real) == 4 ? RhoState::FpFloat : RhoState::FpDouble) : RhoState::Rho;
+    if (do_rho) c.rho_ckpt = true;
   }
   if (do_force) {
     const RhoState want = sizeof(Real) == 4 ? RhoState::FpFloat : RhoState::FpDouble;

@@ -232,6 +232,7 @@ struct Ctx {
   bool has_rep = false;
   DBuf<double> ep, vir, rho;
   RhoState rho_state = RhoState::None;
+  bool rho_ckpt = false;  // rho (c.rho) and F' (pos.w) belong to the current positions
 
   // cell binning
   int nc[3] = {1, 1, 1};
@@ -257,6 +258,10 @@ struct Ctx {
   double lj_eps = 0, lj_sigma = 0, lj_rcut = 0;
   bool has_eam = false;
   EamHost eam;
+
+  // EAM adjoint scratch: coefficient gradients, lambda_i, dL/dF records, inputs
+  DBuf<double> adj_g, adj_lam, adj_in;
+  DBuf<Pos4> adj_gvec;
 
   Profiler prof;
 
@@ -285,6 +290,10 @@ struct ForceEpilogue {
 void launch_lj(Ctx& c, double eps, double sigma, double rcut, unsigned flags, const ForceEpilogue& ep);
 void launch_eam(Ctx& c, double rcut, bool do_rho, bool do_rho2emb, bool do_force, unsigned flags,
                 const ForceEpilogue& ep);
+
+// ---- EAM adjoint (k_adjoint.hip) ----
+void eam_adjoint(Ctx& c, double rcut, const double* dLdF_id, const double* dLdE, double* grad, bool mixed);
+void spline_interpolate_adjoint(int n, double delta, const double* G, double* out);
 
 // ---- setfl + splines (setfl.cpp) ----
 void spline_interpolate(int n, double delta, const double* f, double* coeffs);

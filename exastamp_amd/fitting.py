@@ -1,0 +1,63 @@
+"""PyTorch fitting harness for EAM tables, bound over libxsmd's C ABI.
+
+``EamForcesEnergies`` is a torch.autograd.Function: forward loads the knot vector
+theta (setfl order: F tables, rho tables, r*phi tables) into the context, runs
+eam_alloy_force on the GPU and returns forces (n, 3) and per-replica energies in
+eV units; backward calls the hand-written adjoint kernels (xs_eam_adjoint) with
+the incoming gradients and returns dL/dtheta.  The forward's rho_i and F'(rho_i)
+are the checkpoint the adjoint reads, so the system must not move between the
+two calls.
+
+    sim = Simulation(); sim.species(...); sim.set_config(batch); sim.eam_alloy_init(tables)
+    sim.chunk_neighbors(rcut_max=tables.rc, rcut_inc=0.3)
+    theta = torch.tensor(tables.theta(), requires_grad=True)
+    F, E = EamForcesEnergies.apply(theta, sim, True, True)
+    loss = fit_loss(F, E, F_target, E_target, atoms_per_replica)
+    loss.backward()
+
+forward_mixed selects the forward precision (FP32 pair/spline math), adjoint_mixed
+the FP32 table copies in the adjoint sweeps.  A force-matching gradient is only as
+accurate as the residual F - F_target it is built from: with an FP32 forward the
+force error (~1e-6 of |F|) divided by the residual size bounds the gradient error.
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from .units import EV
+
+
+class EamForcesEnergies(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, theta, sim, forward_mixed=True, adjoint_mixed=True):
+        t = sim.eam.with_theta(theta.detach().to("cpu", torch.float64).numpy())
+        sim.eam_alloy_init(t)
+        sim.zero_force_energy()
+        sim.eam_alloy_force(rcut=t.rc, trigger_thermo_state=True, mixed=forward_mixed)
+        sim.force_to_accel()
+        F = torch.from_numpy(sim.forces() / EV).to(theta.device)
+        E = torch.from_numpy(sim.replica_energies() / EV).to(theta.device)
+        ctx.sim = sim
+        ctx.adjoint_mixed = adjoint_mixed
+        return F, E
+
+    @staticmethod
+    def backward(ctx, gF, gE):
+        # outputs are in eV units: dL/dF_internal = dL/dF_eV / EV, likewise for E
+        gf = gF.detach().to("cpu", torch.float64).numpy() / EV
+        ge = gE.detach().to("cpu", torch.float64).numpy() / EV
+        g = ctx.sim.eam_adjoint(gf, ge, mixed=ctx.adjoint_mixed)
+        return torch.from_numpy(g).to(gF.device), None, None, None
+
+
+def fit_loss(F, E, F_target, E_target, atoms_per_replica, energy_weight=0.1):
+    """Force MSE (eV/A)^2 over all components plus energy_weight * MSE of the
+    per-atom energy (eV) over replicas."""
+    n = torch.as_tensor(atoms_per_replica, dtype=F.dtype, device=F.device)
+    return ((F - F_target) ** 2).mean() + energy_weight * (((E - E_target) / n) ** 2).mean()
+
+
+def fit_loss_np(F, E, F_target, E_target, atoms_per_replica, energy_weight=0.1):
+    n = np.asarray(atoms_per_replica, dtype=np.float64)
+    return float(((F - F_target) ** 2).mean() + energy_weight * (((E - E_target) / n) ** 2).mean())

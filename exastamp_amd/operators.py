@@ -209,6 +209,28 @@ class Simulation:
         flags = self._flags(trigger_thermo_state, compute_virial, mixed)
         self.ctx.xs_eam_alloy_force(rc, int(bool(eam_rho)), int(bool(eam_rho2emb)), int(bool(eam_force)), flags)
 
+    def eam_adjoint(self, dL_dF, dL_dE, mixed=False, rcut=None):
+        """dL/dtheta for the knots (setfl order: F, rho, z2r tables) given dL/dF
+        (n, 3; forces in internal units) and dL/dE per replica.  Needs a preceding
+        eam_alloy_force at the current positions (its rho/F' checkpoint)."""
+        t = self.eam
+        if t is None:
+            raise ValueError("eam_adjoint needs eam_alloy_init first")
+        g = np.ascontiguousarray(np.asarray(dL_dF, dtype=np.float64).reshape(self.n, 3))
+        ge = np.ascontiguousarray(np.asarray(dL_dE, dtype=np.float64).reshape(-1))
+        if ge.size != self.cfg.nrep:
+            raise ValueError("dL_dE needs one value per replica")
+        out = np.zeros(t.theta().size)
+        rc = parse_quantity(rcut) if rcut is not None else t.rc
+        self.ctx.xs_eam_adjoint(rc, _lib.dptr(g), _lib.dptr(ge), _lib.dptr(out), XS_MIXED if mixed else 0)
+        return out
+
+    def replica_energies(self) -> np.ndarray:
+        """Total potential energy of each replica (from the per-atom ep)."""
+        ep = self.ep()
+        rep = self.cfg.rep if self.cfg.rep is not None else np.zeros(self.n, dtype=np.int64)
+        return np.bincount(rep, weights=ep, minlength=self.cfg.nrep)
+
     def push_f_v_r(self, dt, dt_scale=1.0, xform_mode="INV_XFORM"):
         self.ctx.xs_push_f_v_r(parse_quantity(dt), float(dt_scale))
 

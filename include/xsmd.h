@@ -151,6 +151,26 @@ xs_status xs_push_f_v(xs_ctx* ctx, double dt, double dt_scale);
  * (9), momentum(3), kinetic energy(3), potential energy, mass, particle count. */
 xs_status xs_thermo(xs_ctx* ctx, double out27[27]);
 
+/* --- EAM adjoint --------------------------------------------------------------------- */
+/* No counterpart in exaStamp: reverse mode of eam_alloy_force with respect to the
+ * tabulated knots, for fitting tables to reference forces and energies.  Call after
+ * xs_eam_alloy_force with eam_rho, eam_rho2emb and eam_force in the same call (its
+ * rho_i and F'(rho_i) are the checkpoint; they are not recomputed).
+ *   dL_dF : 3n doubles, dL/dF for each atom's force (id order, x y z per atom),
+ *           forces in internal units (Da.A/ps^2)
+ *   dL_dE : nrep doubles, dL/dE for each replica's total potential energy
+ *   grad  : receives dL/dtheta for the knots in setfl order: F tables (nel*nrho),
+ *           rho tables (nel*nr), r*phi tables (nel(nel+1)/2*nr), each per file unit
+ *           (eV, density, eV.A).
+ * flags: XS_MIXED reads the FP32 copies of the tables (distances, knot fractions,
+ * weights and accumulation stay FP64).  The checkpoint may come from either
+ * precision of the forward. */
+xs_status xs_eam_adjoint(xs_ctx* ctx, double rcut, const double* dL_dF, const double* dL_dE,
+                         double* grad, unsigned flags);
+/* Knot-gradient of the spline construction alone (transpose of the knot -> c3..c6
+ * map): G has (n+1) rows of 4 coefficient gradients, out n knots.  Host only. */
+xs_status xs_spline_adjoint(int n, double delta, const double* G, double* out);
+
 /* --- fused NVE loop ---------------------------------------------------------------- */
 typedef struct xs_run_stats {
   int64_t steps;
@@ -180,7 +200,8 @@ xs_status xs_md_run(xs_ctx* ctx, int potential, int nsteps, double dt, double rc
 #define XS_PROF_LJ 3
 #define XS_PROF_DRIFT 4       /* push_f_v_r + push_f_v + displacement check */
 #define XS_PROF_THERMO 5
-#define XS_PROF_NCLASSES 6
+#define XS_PROF_ADJOINT 6     /* both EAM adjoint sweeps */
+#define XS_PROF_NCLASSES 7
 /* on != 0 enables and clears the counters; on == 0 disables. */
 xs_status xs_profile_enable(xs_ctx* ctx, int on);
 /* ms_total[XS_PROF_NCLASSES] and launches[XS_PROF_NCLASSES] (either may be NULL). */

@@ -1,0 +1,132 @@
+"""EAM adjoint (dLoss/dtheta for the tabulated knots) against FP64 central finite
+differences of the oracle's forces and energies.
+
+Workload: the C4 layout at test size -- a batch of independently strained,
+thermally displaced Cu boxes, 500-knot synthetic tables, targets from a seeded
++-2 % perturbation of every knot, loss = force MSE + 0.1 * per-atom energy MSE
+(eV units).  Tolerances: FP64 adjoint <= 1e-6 relative to the FD gradient, mixed
+(FP32 spline/pair math, FP64 accumulation) <= 1e-5, on 32 random knots among
+those the configurations sample.
+"""
+import numpy as np
+import pytest
+
+from conftest import cu_tables
+
+pytestmark = pytest.mark.gpu
+CU = 63.546
+
+
+def _batch(nconf=6, cells=(4, 4, 4), seed=21):
+    from exastamp_amd import inputs
+    return inputs.strained_batch(nconf, cells, 3.615, [CU], seed=seed)
+
+
+def _oracle_fe(boxes, tab):
+    """Forces (eV/A) and energies (eV) of each box from the FP64 oracle."""
+    import pyoracle
+    from exastamp_amd import units
+    Fs, Es = [], []
+    for b in boxes:
+        o = pyoracle.Oracle(threads=8)
+        o.set_config(b, [CU])
+        o.set_eam(tab)
+        o.build_neighbors(tab.rc, 0.3)
+        o.compute_forces("eam")
+        Fs.append(o.forces() / units.EV)
+        Es.append(o.state()["ep"].sum() / units.EV)
+        o.close()
+    return np.concatenate(Fs), np.array(Es)
+
+
+def _setup(mixed_target_seed=5):
+    from exastamp_amd import inputs
+    from exastamp_amd.operators import Simulation
+    tab = cu_tables(500)
+    batch = _batch()
+    boxes = inputs.split_replicas(batch)
+    theta0 = tab.theta()
+    g = inputs.rng(mixed_target_seed)
+    tab_t = tab.with_theta(theta0 * (1.0 + g.uniform(-0.02, 0.02, theta0.size)))
+    Ft, Et = _oracle_fe(boxes, tab_t)
+    sim = Simulation(0)
+    sim.species([("Cu", CU)])
+    sim.set_config(batch)
+    sim.eam_alloy_init(tab)
+    sim.chunk_neighbors(rcut_max=tab.rc, rcut_inc=0.3)
+    return tab, batch, boxes, Ft, Et, sim
+
+
+def test_adjoint_matches_central_differences():
+    import torch
+    from exastamp_amd.fitting import EamForcesEnergies, fit_loss, fit_loss_np
+    tab, batch, boxes, Ft, Et, sim = _setup()
+    nper = np.bincount(batch.rep)
+    # the boxes' atom order in the batch is replica-major, as split_replicas returns
+    grads = {}
+    for fwd_mixed, adj_mixed in ((False, False), (False, True), (True, True)):
+        theta = torch.tensor(tab.theta(), requires_grad=True)
+        F, E = EamForcesEnergies.apply(theta, sim, fwd_mixed, adj_mixed)
+        loss = fit_loss(F, E, torch.from_numpy(Ft), torch.from_numpy(Et), nper)
+        loss.backward()
+        grads[(fwd_mixed, adj_mixed)] = theta.grad.numpy().copy()
+    g64 = grads[(False, False)]
+    # FP64 central differences of the oracle loss on 32 random sampled knots
+    active = np.flatnonzero(np.abs(g64) > 1e-3 * np.abs(g64).max())
+    pick = np.random.default_rng(3).choice(active, size=32, replace=False)
+    theta0 = tab.theta()
+    h = 1e-6
+    fd = []
+    for k in pick:
+        vals = []
+        for s in (+1, -1):
+            th = theta0.copy()
+            th[k] += s * h
+            F, E = _oracle_fe(boxes, tab.with_theta(th))
+            vals.append(fit_loss_np(F, E, Ft, Et, nper))
+        fd.append((vals[0] - vals[1]) / (2 * h))
+    fd = np.array(fd)
+    err = {k: np.abs(v[pick] - fd) / np.abs(fd) for k, v in grads.items()}
+    res = np.abs(F - Ft).mean() / np.abs(Ft).mean()
+    print(f"adjoint vs FD (32 knots): FP64 fwd+adj max rel {err[(False, False)].max():.3e}; "
+          f"FP64 fwd + FP32-table adj {err[(False, True)].max():.3e}; "
+          f"mixed fwd + FP32-table adj {err[(True, True)].max():.3e} "
+          f"(force residual |F-Ft|/|Ft| = {res:.2e})")
+    assert err[(False, False)].max() <= 1e-6
+    assert err[(False, True)].max() <= 1e-5
+    # an FP32 forward bounds the gradient by (force error) / (residual size)
+    assert err[(True, True)].max() <= 1e-2
+
+
+def test_adjoint_requires_checkpoint():
+    from exastamp_amd.operators import XsError
+    tab, batch, boxes, Ft, Et, sim = _setup()
+    with pytest.raises(XsError, match="checkpoint"):
+        sim.eam_adjoint(np.zeros((sim.n, 3)), np.zeros(batch.nrep))
+    sim.zero_force_energy()
+    sim.eam_alloy_force(rcut=tab.rc, trigger_thermo_state=True)
+    sim.push_f_v_r(0.001)
+    with pytest.raises(XsError, match="checkpoint"):
+        sim.eam_adjoint(np.zeros((sim.n, 3)), np.zeros(batch.nrep))
+
+
+def test_energy_only_gradient_is_linear_in_dE():
+    """With dL/dF = 0 and dL/dE = 1 per replica, the gradient is dE_total/dtheta:
+    check it against a central difference of the GPU energy itself (FP64)."""
+    tab, batch, boxes, Ft, Et, sim = _setup()
+    sim.zero_force_energy()
+    sim.eam_alloy_force(rcut=tab.rc, trigger_thermo_state=True)
+    g = sim.eam_adjoint(np.zeros((sim.n, 3)), np.ones(batch.nrep))
+    theta0 = tab.theta()
+    k = int(np.argmax(np.abs(g)))
+    h = 1e-6
+    es = []
+    for s in (+1, -1):
+        th = theta0.copy()
+        th[k] += s * h
+        sim.eam_alloy_init(tab.with_theta(th))
+        sim.zero_force_energy()
+        sim.eam_alloy_force(rcut=tab.rc, trigger_thermo_state=True)
+        es.append(sim.ep().sum())
+    fd = (es[0] - es[1]) / (2 * h)
+    assert g[k] == pytest.approx(fd, rel=1e-6)
