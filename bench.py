@@ -48,6 +48,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--seed", type=int, default=2024)
     return ap.parse_args()
 
@@ -123,6 +124,97 @@ def force_error(sim, rc):
         "rms_abs_over_rms": float(np.sqrt((d ** 2).mean()) / rms),
         "max_rel_atoms_above_0.1rms": float((d[sel] / mag[sel]).max()),
         "metric": "max_i |F_mixed,i - F_fp64,i| / rms_j |F_fp64,j|",
+    }
+
+
+def _oracle_forces_energies(boxes, masses, tab, threads):
+    """FP64 oracle forces (eV/A) and per-box energies (eV)."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import pyoracle
+    from exastamp_amd.units import EV
+    Fs, Es = [], []
+    for b in boxes:
+        o = pyoracle.Oracle(threads=threads)
+        o.set_config(b, masses)
+        o.set_eam(tab)
+        o.build_neighbors(tab.rc, 0.3)
+        o.compute_forces("eam")
+        Fs.append(o.forces() / EV)
+        Es.append(o.state()["ep"].sum() / EV)
+        o.close()
+    return np.concatenate(Fs), np.array(Es)
+
+
+def adjoint_leg(name, cfg, masses, names, tab, nfd, repeats, threads):
+    """Forward + adjoint dLoss/dtheta on one batch: GPU time per forward+backward,
+    device memory, and gradient error vs FP64 central differences of the oracle.
+    Loss = force MSE (eV/A)^2 + 0.1 per-atom energy MSE (eV^2), the gradients of which
+    are formed here with numpy (the torch harness in exastamp_amd.fitting does the same)."""
+    import ctypes as C
+    from exastamp_amd import _lib, inputs
+    from exastamp_amd.fitting import fit_loss_np
+    from exastamp_amd.operators import Simulation
+    from exastamp_amd.units import EV
+    boxes = inputs.split_replicas(cfg)
+    theta0 = tab.theta()
+    tab_t = tab.with_theta(theta0 * (1.0 + inputs.rng(5).uniform(-0.02, 0.02, theta0.size)))
+    Ft, Et = _oracle_forces_energies(boxes, masses, tab_t, threads)
+    nper = np.bincount(cfg.rep) if cfg.rep is not None else np.array([cfg.n])
+    sim = Simulation(0)
+    sim.species(list(zip(names, masses)))
+    sim.set_config(cfg)
+    sim.eam_alloy_init(tab)
+    sim.chunk_neighbors(rcut_max=tab.rc, rcut_inc=0.3)
+
+    def step(fwd_mixed, adj_mixed):
+        sim.eam_alloy_init(tab.with_theta(theta0))
+        sim.zero_force_energy()
+        sim.eam_alloy_force(rcut=tab.rc, trigger_thermo_state=True, mixed=fwd_mixed)
+        sim.force_to_accel()
+        F = sim.forces() / EV
+        E = sim.replica_energies() / EV
+        gF = 2.0 * (F - Ft) / F.size / EV
+        gE = 0.1 * 2.0 * (E - Et) / nper ** 2 / E.size / EV
+        return sim.eam_adjoint(gF, gE, mixed=adj_mixed)
+
+    grads = {("mixed", "mixed"): step(True, True), ("fp64", "fp64"): step(False, False),
+             ("fp64", "mixed"): step(False, True)}
+    sim.ctx.xs_profile_enable(1)
+    t0 = time.perf_counter()
+    for _ in range(repeats):
+        step(True, True)
+    sim.ctx.xs_synchronize()
+    wall = (time.perf_counter() - t0) / repeats
+    ms = np.zeros(len(_lib.PROF_CLASSES))
+    cnt = np.zeros(len(_lib.PROF_CLASSES), dtype=np.int64)
+    sim.ctx.xs_profile_get(_lib.dptr(ms), _lib.ptr(cnt, C.c_int64))
+    sim.ctx.xs_profile_enable(0)
+    gpu_ms = {k: float(ms[i] / repeats) for i, k in enumerate(_lib.PROF_CLASSES) if cnt[i]}
+    g64 = grads[("fp64", "fp64")]
+    active = np.flatnonzero(np.abs(g64) > 1e-3 * np.abs(g64).max())
+    pick = np.random.default_rng(3).choice(active, size=min(nfd, active.size), replace=False)
+    h = 1e-6
+    fd = []
+    for k in pick:
+        v = []
+        for sgn in (+1, -1):
+            th = theta0.copy()
+            th[k] += sgn * h
+            F, E = _oracle_forces_energies(boxes, masses, tab.with_theta(th), threads)
+            v.append(fit_loss_np(F, E, Ft, Et, nper))
+        fd.append((v[0] - v[1]) / (2 * h))
+    fd = np.array(fd)
+    err = {f"fwd_{a}_adj_{b}": float((np.abs(g[pick] - fd) / np.abs(fd)).max()) for (a, b), g in grads.items()}
+    free = C.c_size_t(0)
+    total = C.c_size_t(0)
+    sim.ctx.xs_device_memory(C.byref(free), C.byref(total))
+    sim.close()
+    return {
+        "workload": name, "atoms": int(cfg.n), "replicas": int(cfg.nrep), "knots": int(theta0.size),
+        "ms_forward_backward_wall": 1e3 * wall, "gpu_ms_per_forward_backward": gpu_ms,
+        "grad_rel_err_vs_fd_max": err, "fd_knots": int(len(pick)), "fd_h": h,
+        "device_used_bytes": int(total.value - free.value),
+        "checkpoint_bytes": int(16 * cfg.n),
     }
 
 
@@ -206,6 +298,24 @@ def main():
         cpu = cpu_baseline(cfg, tab, args.cpu_steps, args.cpu_threads, dt, rc, skin)
         log(f"cpu: {cpu['value']:.4g} atom-steps/s on {cpu['cores']} threads")
 
+    adjoint = None
+    if not args.no_adjoint:
+        from exastamp_amd import inputs as inp
+        threads = args.cpu_threads or int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or len(os.sched_getaffinity(0))
+        sim.close()
+        log("adjoint C4: 64 strained Cu boxes x 864 atoms, 3 x 500-knot tables, FD on 32 knots")
+        c4 = inp.strained_batch(64, (6, 6, 6), 3.615, [63.546], seed=41)
+        a4 = adjoint_leg("C4 batch", c4, [63.546], ["Cu"], potentials.johnson_tables(nr=500, nrho=500), 32, 20,
+                         threads)
+        log(f"C4: {a4['ms_forward_backward_wall']:.2f} ms/forward+backward, FD err {a4['grad_rel_err_vs_fd_max']}")
+        log("adjoint C5: Cu/Ni 64^3 FCC slab, 1% vacancies, 7 x 2000-knot tables, FD on 4 knots")
+        c5 = inp.alloy_slab_config((64, 64, 64), 3.57, [63.546, 58.693], vacancy_frac=0.01, vacuum=10.0,
+                                   T=300.0, seed=42)
+        t5 = potentials.johnson_tables((potentials.CU, potentials.NI), nr=2000, nrho=2000)
+        a5 = adjoint_leg("C5 Cu/Ni slab", c5, [63.546, 58.693], ["Cu", "Ni"], t5, 4, 5, threads)
+        log(f"C5: {a5['ms_forward_backward_wall']:.2f} ms/forward+backward, FD err {a5['grad_rel_err_vs_fd_max']}")
+        adjoint = {"C4": a4, "C5": a5}
+
     out = {
         "metric": METRIC,
         "value": mixed["value"],
@@ -237,8 +347,10 @@ def main():
         "kernels_fp64": None if fp64 is None else fp64["kernels"],
         "roofline": roof,
         "cpu_baseline": cpu,
+        "adjoint": adjoint,
     }
-    sim.close()
+    if adjoint is None:
+        sim.close()
     print(json.dumps(out), flush=True)
 
 

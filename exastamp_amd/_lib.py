@@ -54,6 +54,7 @@ SIGNATURES = {
     "xs_ctx_destroy": (C.c_int, [_CTX]),
     "xs_last_error": (C.c_char_p, [_CTX]),
     "xs_synchronize": (C.c_int, [_CTX]),
+    "xs_device_memory": (C.c_int, [_CTX, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
     "xs_species_set": (C.c_int, [_CTX, C.c_int, _D]),
     "xs_system_set": (C.c_int, [_CTX, C.c_int64, _D, _D, _D, _D, _D, _D, _U8, _U16, C.c_int, _D, _D,
                                 C.POINTER(C.c_int)]),
@@ -66,6 +67,7 @@ SIGNATURES = {
     "xs_neighbors_get": (C.c_int, [_CTX, _I64, _I64, _I64, _I32]),
     "xs_order_get": (C.c_int, [_CTX, _I64]),
     "xs_pair_count": (C.c_int, [_CTX, C.c_double, _I64, _I64]),
+    "xs_list_info": (C.c_int, [_CTX, C.POINTER(C.c_int), _I64, C.POINTER(C.c_int)]),
     "xs_zero_force_energy": (C.c_int, [_CTX]),
     "xs_force_to_accel": (C.c_int, [_CTX]),
     "xs_lj_compute_force": (C.c_int, [_CTX, C.c_double, C.c_double, C.c_double, C.c_uint]),
@@ -79,6 +81,7 @@ SIGNATURES = {
     "xs_thermo": (C.c_int, [_CTX, _D]),
     "xs_eam_adjoint": (C.c_int, [_CTX, C.c_double, _D, _D, _D, C.c_uint]),
     "xs_spline_adjoint": (C.c_int, [C.c_int, C.c_double, _D, _D]),
+    "xs_stream_copy": (C.c_int, [_CTX, C.c_size_t, C.c_int, _D]),
     "xs_profile_enable": (C.c_int, [_CTX, C.c_int]),
     "xs_profile_get": (C.c_int, [_CTX, _D, _I64]),
     "xs_md_run": (C.c_int, [_CTX, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_uint, C.c_int,

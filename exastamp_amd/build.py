@@ -19,7 +19,7 @@ CSRC = PKG / "csrc"
 BUILD = PKG / "_build"
 LIB = PKG / "libxsmd.so"
 
-SOURCES = ["capi.hip", "k_neighbors.hip", "k_integrate.hip", "k_pair.hip", "k_adjoint.hip", "setfl.cpp"]
+SOURCES = ["capi.hip", "k_neighbors.hip", "k_integrate.hip", "k_pair.hip", "k_adjoint.hip", "k_block.hip", "setfl.cpp"]
 
 ARCH = "gfx950"
 # packed FP32 instructions are disabled for all device code (-packed-fp32-ops)

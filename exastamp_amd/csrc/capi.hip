@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 
 namespace xsmd {
 
@@ -170,6 +171,8 @@ xs_status xs_ctx_create(int device, xs_ctx** out)
     c->dscalar.reserve(4);
     c->mass_d.reserve(8);
     c->box_d.reserve(1);
+    const char* kind = std::getenv("XSMD_LIST");
+    c->block_disabled = kind && std::strcmp(kind, "global") == 0;
   } catch (const Error&) {
     delete c;
     return XS_E_HIP;
@@ -197,6 +200,16 @@ const char* xs_last_error(const xs_ctx* h)
 xs_status xs_synchronize(xs_ctx* h)
 {
   return guarded(h, [&](Ctx& c) { XS_HIP(hipStreamSynchronize(c.stream)); });
+}
+
+xs_status xs_device_memory(xs_ctx* h, size_t* free_bytes, size_t* total_bytes)
+{
+  return guarded(h, [&](Ctx&) {
+    size_t f = 0, t = 0;
+    XS_HIP(hipMemGetInfo(&f, &t));
+    if (free_bytes) *free_bytes = f;
+    if (total_bytes) *total_bytes = t;
+  });
 }
 
 xs_status xs_species_set(xs_ctx* h, int nspecies, const double* masses)
@@ -342,7 +355,7 @@ xs_status xs_chunk_neighbors(xs_ctx* h, double rcut_max, double rcut_inc, int ch
             "chunk_size must be a power of two");
     require(rcut_max > 0 && rcut_inc >= 0, XS_E_ARG, "need rcut_max > 0 and rcut_inc >= 0");
     require(!c.box_h.empty(), XS_E_STATE, "no system loaded (call xs_system_set)");
-    build_neighbor_list(c, rcut_max, rcut_inc);
+    build_neighbor_list(c, rcut_max, rcut_inc, c.has_eam);
   });
 }
 
@@ -360,6 +373,7 @@ xs_status xs_neighbors_get(xs_ctx* h, int64_t* counts, int64_t* total, int64_t* 
   return guarded(h, [&](Ctx& c) {
     require_list(c);
     require(counts && total, XS_E_ARG, "null counts/total");
+    ensure_global_list(c);
     const std::vector<int64_t> id = from_device(c, c.id, c.n);
     const std::vector<int> cnt = from_device(c, c.nnb, c.n);
     int64_t tot = 0;
@@ -404,6 +418,17 @@ xs_status xs_pair_count(xs_ctx* h, double rcut, int64_t* entries, int64_t* withi
     pair_count(c, rcut, e, w);
     if (entries) *entries = e;
     if (within_rcut) *within_rcut = w;
+  });
+}
+
+xs_status xs_list_info(xs_ctx* h, int* kind, int64_t* blocks, int* halo_capacity)
+{
+  return guarded(h, [&](Ctx& c) {
+    require_list(c);
+    const bool blk = c.list_kind == ListKind::Block;
+    if (kind) *kind = blk ? 1 : 0;
+    if (blocks) *blocks = blk ? c.bgeom.nblocks : 0;
+    if (halo_capacity) *halo_capacity = blk ? c.bcap : 0;
   });
 }
 
@@ -527,7 +552,7 @@ xs_status xs_md_run(xs_ctx* h, int potential, int nsteps, double dt, double rcut
     int64_t rebuilds = 0;
 
     if (!cont) {
-      build_neighbor_list(c, rcut, skin);
+      build_neighbor_list(c, rcut, skin, potential == XS_POT_EAM);
       rebuilds++;
     }
     auto is_thermo = [&](int step) { return thermo_every > 0 && step % thermo_every == 0; };
@@ -554,7 +579,7 @@ xs_status xs_md_run(xs_ctx* h, int potential, int nsteps, double dt, double rcut
       double d2;
       std::memcpy(&d2, &c.hscalar[1], sizeof(double));
       if (d2 > thr2) {
-        build_neighbor_list(c, rcut, skin);
+        build_neighbor_list(c, rcut, skin, potential == XS_POT_EAM);
         rebuilds++;
       }
       forces(step, 0.5 * dt);
@@ -568,6 +593,14 @@ xs_status xs_md_run(xs_ctx* h, int potential, int nsteps, double dt, double rcut
       stats->max_neighbors = c.nmax;
       stats->seconds = sec;
     }
+  });
+}
+
+xs_status xs_stream_copy(xs_ctx* h, size_t bytes, int reps, double* gbs)
+{
+  return guarded(h, [&](Ctx& c) {
+    require(gbs != nullptr, XS_E_ARG, "null output");
+    *gbs = stream_copy(c, bytes, reps);
   });
 }
 

@@ -298,6 +298,7 @@ void spline_interpolate_adjoint(int n, double delta, const double* G, double* ou
 
 void eam_adjoint(Ctx& c, double rcut, const double* dLdF_id, const double* dLdE, double* grad, bool mixed)
 {
+  ensure_global_list(c);
   EamHost& e = c.eam;
   const int nel = e.nel, nz = nel * (nel + 1) / 2;
   const size_t rows_r = (size_t)e.nr + 1, rows_f = (size_t)e.nrho + 1;

@@ -92,6 +92,15 @@ __global__ void __launch_bounds__(256) k_drift_kick(
   }
 }
 
+// Plain streaming copy, 16 bytes per lane per iteration.
+__global__ void __launch_bounds__(256) k_stream_copy(int64_t n16, const double2* __restrict__ src,
+                                                     double2* __restrict__ dst)
+{
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16;
+       i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
 constexpr int kThermoBlocks = 1024;
 constexpr int kThermoQ = 26;
 
@@ -223,6 +232,37 @@ void launch_thermo(Ctx& c, double out[27])
   out[24] = s[24];
   out[25] = s[25];
   out[26] = (double)c.n;
+}
+
+}  // namespace xsmd
+
+namespace xsmd {
+
+double stream_copy(Ctx& c, size_t bytes, int reps)
+{
+  const int64_t n16 = (int64_t)(bytes / 16);
+  if (n16 <= 0 || reps <= 0) throw Error(XS_E_ARG, "stream copy needs bytes >= 16 and reps >= 1");
+  DBuf<double2> a, b;
+  a.reserve(n16);
+  b.reserve(n16);
+  XS_HIP(hipMemsetAsync(a.p, 0, n16 * 16, c.stream));
+  hipEvent_t e0, e1;
+  XS_HIP(hipEventCreate(&e0));
+  XS_HIP(hipEventCreate(&e1));
+  const unsigned g = std::min<unsigned>(grid_for(n16), 256u * 64u);
+  float best = 1e30f;
+  for (int r = 0; r < reps; r++) {
+    XS_HIP(hipEventRecord(e0, c.stream));
+    k_stream_copy<<<g, kBlock, 0, c.stream>>>(n16, a.p, b.p);
+    XS_HIP(hipEventRecord(e1, c.stream));
+    XS_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    XS_HIP(hipEventElapsedTime(&ms, e0, e1));
+    best = std::min(best, ms);
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return 2.0 * (double)n16 * 16.0 / (best * 1e-3) / 1e9;
 }
 
 }  // namespace xsmd

@@ -75,22 +75,12 @@ __global__ void k_permute(int64_t n, const uint32_t* __restrict__ perm, PermuteA
   if (a.rep_src) a.rep_dst[i] = a.rep_src[p];
 }
 
-__global__ void k_cell_bounds(int64_t n, const uint32_t* __restrict__ keys, int* __restrict__ cstart,
-                              int* __restrict__ cend)
-{
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const uint32_t k = keys[i];
-  if (i == 0 || keys[i - 1] != k) cstart[k] = (int)i;
-  if (i == n - 1 || keys[i + 1] != k) cend[k] = (int)(i + 1);
-}
-
 // One thread per atom; stencil order dz, dy, dx (outer to inner), in-cell order
 // ascending -- the same visit order as the oracle, so lists match entry by entry.
 __global__ void __launch_bounds__(256) k_build(
     int64_t n, const Pos4* __restrict__ pos, const uint32_t* __restrict__ keys,
     const BoxDev* __restrict__ box, int px, int py, int pz, int nx, int ny, int nz,
-    int64_t ncell_rep, const int* __restrict__ cstart, const int* __restrict__ cend, double rl2,
+    int64_t ncell_rep, const int* __restrict__ cell_off, double rl2,
     uint32_t* __restrict__ nbr, int* __restrict__ nnb, int nmax, int64_t npad,
     unsigned long long* __restrict__ maxcnt)
 {
@@ -118,7 +108,7 @@ __global__ void __launch_bounds__(256) k_build(
         if (x < 0) { if (!px) continue; x += nx; kx = -1; }
         else if (x >= nx) { if (!px) continue; x -= nx; kx = 1; }
         const int64_t cell = r * ncell_rep + ((int64_t)z * ny + y) * nx + x;
-        const int j0 = cstart[cell], j1 = cend[cell];
+        const int j0 = cell_off[cell], j1 = cell_off[cell + 1];
         const bool self_img = (kx == 0 && ky == 0 && kz == 0);
         for (int j = j0; j < j1; j++) {
           if (self_img && j == (int)i) continue;
@@ -200,12 +190,14 @@ __global__ void k_pair_count(int64_t n, int64_t npad, const Pos4* __restrict__ p
 
 }  // namespace
 
-void build_neighbor_list(Ctx& c, double rcut, double skin)
+void build_neighbor_list(Ctx& c, double rcut, double skin, bool prefer_block)
 {
   ProfScope prof(c.prof, c.stream, XS_PROF_NEIGHBORS);
   const int64_t n = c.n;
   if (n == 0) {
     c.list_valid = true;
+    c.list_kind = ListKind::Global;
+    c.global_from_block = false;
     return;
   }
   const double rl = rcut + skin;
@@ -274,12 +266,7 @@ void build_neighbor_list(Ctx& c, double rcut, double skin)
   c.rho_state = RhoState::None;  // per-atom EAM state is in the old order
   c.rho_ckpt = false;
 
-  c.cstart.reserve(c.ncell_tot);
-  c.cend.reserve(c.ncell_tot);
-  XS_HIP(hipMemsetAsync(c.cstart.p, 0, sizeof(int) * c.ncell_tot, c.stream));
-  XS_HIP(hipMemsetAsync(c.cend.p, 0, sizeof(int) * c.ncell_tot, c.stream));
-  k_cell_bounds<<<g, kBlock, 0, c.stream>>>(n, c.keys2.p, c.cstart.p, c.cend.p);
-  XS_HIP(hipGetLastError());
+  cell_offsets(c);
 
   // list capacity: estimate from the density, grow to the measured maximum on overflow
   if (c.nmax == 0) {
@@ -290,13 +277,22 @@ void build_neighbor_list(Ctx& c, double rcut, double skin)
     c.nmax = std::max(16, (int)std::ceil(est * 1.25 / 8.0) * 8);
   }
   const double rl2 = rl * rl;
-  for (int attempt = 0; attempt < 2; attempt++) {
+  c.global_from_block = false;
+  c.list_kind = ListKind::Global;
+  if (prefer_block && !c.block_disabled) {
+    block_setup(c);
+    if (block_feasible(c)) {
+      block_build(c, rl2);
+      c.list_kind = ListKind::Block;
+    }
+  }
+  for (int attempt = 0; c.list_kind == ListKind::Global && attempt < 2; attempt++) {
     c.nbr.reserve((size_t)c.nmax * c.npad);
     c.nnb.reserve(c.npad);
     XS_HIP(hipMemsetAsync(c.dscalar.p, 0, sizeof(unsigned long long), c.stream));
     k_build<<<g, kBlock, 0, c.stream>>>(n, c.pos.p, c.keys2.p, c.box_d.p, c.periodic[0], c.periodic[1],
-                                        c.periodic[2], c.nc[0], c.nc[1], c.nc[2], c.ncell_rep, c.cstart.p,
-                                        c.cend.p, rl2, c.nbr.p, c.nnb.p, c.nmax, c.npad, c.dscalar.p);
+                                        c.periodic[2], c.nc[0], c.nc[1], c.nc[2], c.ncell_rep, c.cell_off.p, rl2,
+                                        c.nbr.p, c.nnb.p, c.nmax, c.npad, c.dscalar.p);
     XS_HIP(hipGetLastError());
     XS_HIP(hipMemcpyAsync(c.hscalar, c.dscalar.p, sizeof(unsigned long long), hipMemcpyDeviceToHost,
                           c.stream));
@@ -315,6 +311,7 @@ void pair_count(Ctx& c, double rcut, int64_t& entries, int64_t& within)
 {
   entries = within = 0;
   if (c.n == 0) return;
+  ensure_global_list(c);
   XS_HIP(hipMemsetAsync(c.dscalar.p + 2, 0, 2 * sizeof(unsigned long long), c.stream));
   const unsigned g = std::min<unsigned>(grid_for(c.n), 4096);
   k_pair_count<<<g, kBlock, 0, c.stream>>>(c.n, c.npad, c.pos.p, c.has_rep ? c.rep.p : nullptr, c.box_d.p,

@@ -1,4 +1,4 @@
-// k_pair.hip -- Lennard-Jones and EAM pair kernels over the full neighbour list.
+// k_pair.hip -- Lennard-Jones and EAM pair kernels over the global neighbour list.
 //
 // One thread per central atom walks its ELL list (coalesced entry loads), gathers
 // neighbour position records, and accumulates its own force only: no atomics, no
@@ -10,56 +10,14 @@
 // cutoff anyway, so computing the out-of-range entries costs nothing extra, and
 // the batch keeps several gathers in flight per wavefront.
 //
-// Precision: template parameter Real is the type of the pair and spline
-// arithmetic.  Real = double is the plain FP64 form.  Real = float is the mixed
-// form: positions, the minimum-image displacement, d^2, the cutoff test, the
-// per-atom force/energy/virial accumulation and the EAM density sum stay in
-// FP64; distance, spline evaluation and the pair scalar are FP32.
-#include "xsmd_internal.hpp"
+// EAM single-element systems with a cell-block list use k_block.hip instead;
+// these kernels serve LJ, alloys, replica batches and the global-list path.
+#include "pair_math.hpp"
 
 namespace xsmd {
 namespace {
 
-template <class Real>
-struct Q4 {
-  Real a, b, c, d;
-};
-
-__device__ __forceinline__ Q4<float> load4(const float* __restrict__ p)
-{
-  const float4 v = *reinterpret_cast<const float4*>(p);
-  return {v.x, v.y, v.z, v.w};
-}
-__device__ __forceinline__ Q4<double> load4(const double* __restrict__ p)
-{
-  const double2 u = reinterpret_cast<const double2*>(p)[0];
-  const double2 w = reinterpret_cast<const double2*>(p)[1];
-  return {u.x, u.y, w.x, w.y};
-}
-
-// 1/sqrt(d2) and sqrt(d2): one hardware rsq in FP32, correctly rounded sqrt
-// and division in FP64 (what plain FP64 code compiles to).
-__device__ __forceinline__ void dist(float d2, float& r, float& rinv)
-{
-  rinv = __builtin_amdgcn_rsqf(d2);
-  r = d2 * rinv;
-}
-__device__ __forceinline__ void dist(double d2, double& r, double& rinv)
-{
-  r = sqrt(d2);
-  rinv = 1.0 / r;
-}
-__device__ __forceinline__ float recip(float x) { return __builtin_amdgcn_rcpf(x); }
-__device__ __forceinline__ double recip(double x) { return 1.0 / x; }
-
-// XCD-aware block order: hardware hands workgroup b to XCD b % 8, so map it to
-// logical block (b % 8) * (G/8) + b / 8 and each XCD's L2 sees one contiguous
-// (hence spatially compact) range of atoms.  G is padded to a multiple of 8.
-__device__ __forceinline__ int64_t logical_block()
-{
-  const unsigned b = blockIdx.x, g = gridDim.x;
-  return (int64_t)(b & 7u) * (g >> 3) + (b >> 3);
-}
+using namespace pm;
 
 struct PairCommon {
   int64_t n, npad;
@@ -70,19 +28,6 @@ struct PairCommon {
   const uint32_t* __restrict__ nbr;
   const int* __restrict__ nnb;
   double rc2;
-};
-
-struct Outputs {
-  double* ax;
-  double* ay;
-  double* az;
-  double* ep;
-  double* vir;
-  const double* mass;  // fused epilogue
-  double* vx;
-  double* vy;
-  double* vz;
-  double kick_dt;
 };
 
 constexpr int kBatch = 4;
@@ -125,45 +70,6 @@ __device__ __forceinline__ void fetch(const PairCommon& pc, int64_t i, int k0, i
   }
 }
 
-template <bool ENERGY, bool VIRIAL, bool FUSED>
-__device__ __forceinline__ void write_outputs(const Outputs& o, const PairCommon& pc, int64_t i, double fx,
-                                              double fy, double fz, double e, const double* v)
-{
-  if constexpr (FUSED) {
-    const double m = o.mass[pc.type[i]];
-    const double a0 = fx / m, a1 = fy / m, a2 = fz / m;
-    o.ax[i] = a0;
-    o.ay[i] = a1;
-    o.az[i] = a2;
-    o.vx[i] = fma(a0, o.kick_dt, o.vx[i]);
-    o.vy[i] = fma(a1, o.kick_dt, o.vy[i]);
-    o.vz[i] = fma(a2, o.kick_dt, o.vz[i]);
-  } else {
-    o.ax[i] += fx;
-    o.ay[i] += fy;
-    o.az[i] += fz;
-  }
-  if constexpr (ENERGY) o.ep[i] += e;
-  if constexpr (VIRIAL) {
-#pragma unroll
-    for (int q = 0; q < 9; q++) {
-      if constexpr (FUSED) o.vir[q * pc.npad + i] = v[q];
-      else o.vir[q * pc.npad + i] += v[q];
-    }
-  }
-}
-
-// onika tensor(fe, dr) * -0.5 added to v (row-major m11 m12 ... m33)
-__device__ __forceinline__ void add_virial(double* v, double fe0, double fe1, double fe2, const Nb& d)
-{
-  const double fe[3] = {fe0, fe1, fe2};
-  const double dr[3] = {d.x, d.y, d.z};
-#pragma unroll
-  for (int a = 0; a < 3; a++)
-#pragma unroll
-    for (int b = 0; b < 3; b++) v[3 * a + b] += (fe[a] * dr[b]) * -0.5;
-}
-
 // ------------------------------------------------------------------------------
 // Lennard-Jones: lj_compute_energy (lennard_jones.h:40-50) with the energy shift
 // and per-atom accumulation of pair_potential_force_op_singlemat.h:101-123.
@@ -199,75 +105,20 @@ __global__ void __launch_bounds__(256) k_lj(PairCommon pc, LJParams<Real> lp, Ou
       fy += fe1;
       fz += fe2;
       if constexpr (ENERGY) e += nb[u].in ? 0.5 * (double)(lp.eps4 * (s12 - s6) - lp.ecut) : 0.0;
-      if constexpr (VIRIAL) add_virial(v, fe0, fe1, fe2, nb[u]);
+      if constexpr (VIRIAL) add_virial(v, fe0, fe1, fe2, nb[u].x, nb[u].y, nb[u].z);
     }
   }
-  write_outputs<ENERGY, VIRIAL, FUSED>(o, pc, i, fx, fy, fz, e, v);
+  write_outputs<ENERGY, VIRIAL, FUSED>(o, i, fx, fy, fz, e, v);
 }
 
 // ------------------------------------------------------------------------------
-// EAM (eam_alloy): spline evaluation as eam_alloy.h:198-313, pass semantics of
-// SymRhoOp / Rho2EmbOp / SymForceOp with NewtonSym = false
-// (eam_force_op_multimat.h:114-343).
+// EAM (eam_alloy): pass semantics of SymRhoOp / Rho2EmbOp / SymForceOp with
+// NewtonSym = false (eam_force_op_multimat.h:114-343).
 // ------------------------------------------------------------------------------
-template <class Real>
-__device__ __forceinline__ void knot_r(const EamTabDev<Real>& t, Real r, int& m, Real& p)
-{
-  p = r * t.rdr + Real(1);
-  m = (int)p;
-  m = max(0, min(m, t.nr - 1));
-  p -= (Real)m;
-  p = fmin(p, Real(1));
-}
 
-template <class Real>
-__device__ __forceinline__ const Real* row_r(const Real* tab, const EamTabDev<Real>& t, int which, int m)
-{
-  return tab + ((int64_t)which * (t.nr + 1) + m) * 8;
-}
-
-template <class Real>
-__device__ __forceinline__ Real cubic(const Q4<Real>& c, Real p)
-{
-  return ((c.a * p + c.b) * p + c.c) * p + c.d;
-}
-template <class Real>
-__device__ __forceinline__ Real quad(const Q4<Real>& c, Real p)
-{
-  return (c.a * p + c.b) * p + c.c;
-}
-// derivative of the value cubic with respect to p
-template <class Real>
-__device__ __forceinline__ Real dcubic(const Q4<Real>& c, Real p)
-{
-  return (Real(3) * c.a * p + Real(2) * c.b) * p + c.c;
-}
-
-// F(rho), F'(rho) in eV-converted internal units, with the linear extrapolation
-// of F above rhomax (eam_alloy.h:289-313).
-template <class Real>
-__device__ __forceinline__ void embed(const EamTabDev<Real>& t, int ti, double rho, Real& F, Real& Fp)
-{
-  const Real rr = (Real)rho;
-  Real p = rr * t.rdrho + Real(1);
-  int m = (int)p;
-  m = max(1, min(m, t.nrho - 1));
-  p -= (Real)m;
-  p = fmin(p, Real(1));
-  const Real* row = t.frho + ((int64_t)t.t2frho[ti + 1] * (t.nrho + 1) + m) * 8;
-  const Q4<Real> v = load4(row);
-  const Q4<Real> dv = load4(row + 4);
-  Fp = quad(dv, p);
-  F = cubic(v, p);
-  if (rr > t.rhomax) F += Fp * (rr - t.rhomax);
-  F *= t.conv;
-  Fp *= t.conv;
-}
-
-// Pass 1 (+ fused pass 1b): rho_i = sum_j rho_{type_j}(r_ij) in FP64, then
-// F'(rho_i) into the position record's w (and F(rho_i) into ep when ENERGY).
-// Single-element systems read the rho cubic from the first half of the combined
-// pair row; alloys read the per-element rho table.
+// Pass 1 (+ fused pass 1b): rho_i = sum_j rho_{type_j}(r_ij) in FP64 (kept as the
+// adjoint's checkpoint), then F'(rho_i) into the position record's w (and F(rho_i)
+// into ep when ENERGY).
 template <class Real, bool MULTI, bool RHO, bool EMB, bool ENERGY, bool SET_EP>
 __global__ void __launch_bounds__(256) k_eam_rho(PairCommon pc, EamTabDev<Real> t, const Real* __restrict__ pair,
                                                  double* __restrict__ rho_io, double* __restrict__ pos_w,
@@ -286,19 +137,12 @@ __global__ void __launch_bounds__(256) k_eam_rho(PairCommon pc, EamTabDev<Real> 
       fetch<kBatch>(pc, i, k0, cnt, pi, L, nb);
 #pragma unroll
       for (int u = 0; u < kBatch; u++) {
-        Real r, rinv;
-        dist((Real)(nb[u].in ? nb[u].d2 : 1.0), r, rinv);
-        int m;
-        Real p;
-        knot_r(t, r, m, p);
-        const Real* row;
-        if constexpr (MULTI) row = row_r(t.rhor, t, t.t2rhor[pc.type[nb[u].j] + 1][ti + 1], m);
-        else row = pair + (int64_t)m * 8;
-        const Real val = cubic(load4(row), p);
+        const int tj = MULTI ? pc.type[nb[u].j] : 0;
+        const Real val = eam_rho<Real, MULTI>(t, pair, ti, tj, nb[u].in ? nb[u].d2 : 1.0);
         rho += nb[u].in ? (double)val : 0.0;
       }
     }
-    rho_io[i] = rho;  // kept: the density checkpoint of the adjoint
+    rho_io[i] = rho;
   } else {
     rho = rho_io[i];
   }
@@ -313,8 +157,7 @@ __global__ void __launch_bounds__(256) k_eam_rho(PairCommon pc, EamTabDev<Real> 
   }
 }
 
-// Pass 2: psi'(r) = F'_i rho'_j(r) + F'_j rho'_i(r) + phi'(r), f_i += psi'/r (r_j - r_i),
-// with phi = z2r / r (eam_alloy_mm_force, eam_alloy.h:222-265).
+// Pass 2: f_i += psi'/r (r_j - r_i), with the per-atom epilogue.
 template <class Real, bool MULTI, bool ENERGY, bool VIRIAL, bool FUSED>
 __global__ void __launch_bounds__(256) k_eam_force(PairCommon pc, EamTabDev<Real> t, const Real* __restrict__ pair,
                                                    Outputs o)
@@ -333,46 +176,19 @@ __global__ void __launch_bounds__(256) k_eam_force(PairCommon pc, EamTabDev<Real
     fetch<kBatch>(pc, i, k0, cnt, pi, L, nb);
 #pragma unroll
     for (int u = 0; u < kBatch; u++) {
-      const Real fpj = (Real)nb[u].w;
-      Real r, rinv;
-      dist((Real)(nb[u].in ? nb[u].d2 : 1.0), r, rinv);
-      int m;
-      Real p;
-      knot_r(t, r, m, p);
-      Real psip, phi;
-      if constexpr (MULTI) {
-        const int tj = pc.type[nb[u].j];
-        const Real rhoip = quad(load4(row_r(t.rhor, t, t.t2rhor[ti + 1][tj + 1], m) + 4), p);
-        const Real rhojp = quad(load4(row_r(t.rhor, t, t.t2rhor[tj + 1][ti + 1], m) + 4), p);
-        const Real* zrow = row_r(t.z2r, t, t.t2z2r[ti + 1][tj + 1], m);
-        const Real z2p = quad(load4(zrow + 4), p);
-        const Real z2 = cubic(load4(zrow), p);
-        phi = z2 * rinv;
-        const Real phip = (z2p * rinv - phi * rinv) * t.conv;
-        phi *= t.conv;
-        psip = fpi * rhojp + fpj * rhoip + phip;
-      } else {
-        const Real* row = pair + (int64_t)m * 8;
-        const Q4<Real> rc = load4(row);
-        const Q4<Real> zc = load4(row + 4);
-        const Real rhop = dcubic(rc, p) * t.rdr;
-        const Real z2p = dcubic(zc, p) * t.rdr;
-        const Real z2 = cubic(zc, p);
-        phi = z2 * rinv;
-        const Real phip = (z2p * rinv - phi * rinv) * t.conv;
-        phi *= t.conv;
-        psip = (fpi + fpj) * rhop + phip;
-      }
-      const double fpair = nb[u].in ? (double)(psip * rinv) : 0.0;
+      const int tj = MULTI ? pc.type[nb[u].j] : 0;
+      Real fp_r, phi;
+      eam_pair<Real, MULTI>(t, pair, ti, tj, fpi, (Real)nb[u].w, nb[u].in ? nb[u].d2 : 1.0, fp_r, phi);
+      const double fpair = nb[u].in ? (double)fp_r : 0.0;
       const double fe0 = nb[u].x * fpair, fe1 = nb[u].y * fpair, fe2 = nb[u].z * fpair;
       fx += fe0;
       fy += fe1;
       fz += fe2;
       if constexpr (ENERGY) e += nb[u].in ? 0.5 * (double)phi : 0.0;
-      if constexpr (VIRIAL) add_virial(v, fe0, fe1, fe2, nb[u]);
+      if constexpr (VIRIAL) add_virial(v, fe0, fe1, fe2, nb[u].x, nb[u].y, nb[u].z);
     }
   }
-  write_outputs<ENERGY, VIRIAL, FUSED>(o, pc, i, fx, fy, fz, e, v);
+  write_outputs<ENERGY, VIRIAL, FUSED>(o, i, fx, fy, fz, e, v);
 }
 
 PairCommon make_common(Ctx& c, double rcut)
@@ -390,87 +206,24 @@ PairCommon make_common(Ctx& c, double rcut)
   return pc;
 }
 
-Outputs make_outputs(Ctx& c, const ForceEpilogue& ep)
-{
-  Outputs o;
-  o.ax = c.ax.p;
-  o.ay = c.ay.p;
-  o.az = c.az.p;
-  o.ep = c.ep.p;
-  o.vir = c.vir.p;
-  o.mass = c.mass_d.p;
-  o.vx = c.vx.p;
-  o.vy = c.vy.p;
-  o.vz = c.vz.p;
-  o.kick_dt = ep.kick_dt;
-  return o;
-}
-
 unsigned pair_grid(int64_t n)
 {
   const unsigned g = grid_for(n);
   return (g + 7u) / 8u * 8u;
 }
 
-template <class Real>
-EamTabDev<Real> make_tab(Ctx& c);
-
-template <>
-EamTabDev<double> make_tab<double>(Ctx& c)
-{
-  EamTabDev<double> t;
-  t.frho = c.eam.frho_d.p;
-  t.rhor = c.eam.rhor_d.p;
-  t.z2r = c.eam.z2r_d.p;
-  t.nr = c.eam.nr;
-  t.nrho = c.eam.nrho;
-  t.rdr = c.eam.rdr;
-  t.rdrho = c.eam.rdrho;
-  t.rhomax = c.eam.rhomax;
-  std::memcpy(t.t2rhor, c.eam.t2rhor, sizeof(t.t2rhor));
-  std::memcpy(t.t2z2r, c.eam.t2z2r, sizeof(t.t2z2r));
-  std::memcpy(t.t2frho, c.eam.t2frho, sizeof(t.t2frho));
-  return t;
-}
-
-template <>
-EamTabDev<float> make_tab<float>(Ctx& c)
-{
-  EamTabDev<float> t;
-  t.frho = c.eam.frho_f.p;
-  t.rhor = c.eam.rhor_f.p;
-  t.z2r = c.eam.z2r_f.p;
-  t.nr = c.eam.nr;
-  t.nrho = c.eam.nrho;
-  t.rdr = (float)c.eam.rdr;
-  t.rdrho = (float)c.eam.rdrho;
-  t.rhomax = (float)c.eam.rhomax;
-  std::memcpy(t.t2rhor, c.eam.t2rhor, sizeof(t.t2rhor));
-  std::memcpy(t.t2z2r, c.eam.t2z2r, sizeof(t.t2z2r));
-  std::memcpy(t.t2frho, c.eam.t2frho, sizeof(t.t2frho));
-  return t;
-}
-
-// 1 eV in Da.A^2/ps^2 (CODATA-2018: e / (amu * 1e4)); conversion_z2r = 1 eV.A and
-// conversion_frho = 1 eV of eam_alloy.h:156-157 in exaStamp's internal units.
-constexpr double kEV = 1.602176634e-19 / (1.66053906660e-27 * 1.0e4);
-
 template <class Real, bool MULTI>
 void eam_dispatch(Ctx& c, double rcut, bool do_rho, bool do_emb, bool do_force, unsigned flags,
                   const ForceEpilogue& epi)
 {
-  EamTabDev<Real> t = make_tab<Real>(c);
-  t.conv = (Real)kEV;
-  const Real* pair;
-  if constexpr (sizeof(Real) == 4) pair = c.eam.pair_f.p;
-  else pair = c.eam.pair_d.p;
+  const EamTabDev<Real> t = make_eam_tab<Real>(c);
+  const Real* pair = eam_pair_table<Real>(c);
   const PairCommon pc = make_common(c, rcut);
   const unsigned g = pair_grid(c.n);
   const bool energy = flags & XS_ENERGY;
   double* pos_w = reinterpret_cast<double*>(c.pos.p);
 
   if (do_rho || do_emb) {
-    if (!do_rho && c.rho_state != RhoState::Rho) throw Error(XS_E_STATE, "eam_rho2emb needs eam_rho first");
     ProfScope prof(c.prof, c.stream, XS_PROF_EAM_RHO);
 #define XS_RHO(R, E, EN, SET) \
   k_eam_rho<Real, MULTI, R, E, EN, SET><<<g, kBlock, 0, c.stream>>>(pc, t, pair, c.rho.p, pos_w, c.ep.p)
@@ -486,13 +239,8 @@ void eam_dispatch(Ctx& c, double rcut, bool do_rho, bool do_emb, bool do_force, 
     }
 #undef XS_RHO
     XS_HIP(hipGetLastError());
-    c.rho_state = do_emb ? (sizeof(Real) == 4 ? RhoState::FpFloat : RhoState::FpDouble) : RhoState::Rho;
-    if (do_rho) c.rho_ckpt = true;
   }
   if (do_force) {
-    const RhoState want = sizeof(Real) == 4 ? RhoState::FpFloat : RhoState::FpDouble;
-    if (c.rho_state != want)
-      throw Error(XS_E_STATE, "eam_force needs F'(rho) from eam_rho2emb in the same precision");
     const Outputs o = make_outputs(c, epi);
     const bool vir = flags & XS_VIRIAL;
     ProfScope prof(c.prof, c.stream, XS_PROF_EAM_FORCE);
@@ -546,9 +294,81 @@ void lj_dispatch(Ctx& c, double eps, double sigma, double rcut, unsigned flags, 
 
 }  // namespace
 
+// 1 eV in Da.A^2/ps^2 (CODATA-2018: e / (amu * 1e4)); conversion_z2r = 1 eV.A and
+// conversion_frho = 1 eV of eam_alloy.h:156-157 in exaStamp's internal units.
+constexpr double kEV = 1.602176634e-19 / (1.66053906660e-27 * 1.0e4);
+
+template <>
+EamTabDev<double> make_eam_tab<double>(Ctx& c)
+{
+  EamTabDev<double> t;
+  t.frho = c.eam.frho_d.p;
+  t.rhor = c.eam.rhor_d.p;
+  t.z2r = c.eam.z2r_d.p;
+  t.nr = c.eam.nr;
+  t.nrho = c.eam.nrho;
+  t.rdr = c.eam.rdr;
+  t.rdrho = c.eam.rdrho;
+  t.rhomax = c.eam.rhomax;
+  t.conv = kEV;
+  std::memcpy(t.t2rhor, c.eam.t2rhor, sizeof(t.t2rhor));
+  std::memcpy(t.t2z2r, c.eam.t2z2r, sizeof(t.t2z2r));
+  std::memcpy(t.t2frho, c.eam.t2frho, sizeof(t.t2frho));
+  return t;
+}
+
+template <>
+EamTabDev<float> make_eam_tab<float>(Ctx& c)
+{
+  EamTabDev<float> t;
+  t.frho = c.eam.frho_f.p;
+  t.rhor = c.eam.rhor_f.p;
+  t.z2r = c.eam.z2r_f.p;
+  t.nr = c.eam.nr;
+  t.nrho = c.eam.nrho;
+  t.rdr = (float)c.eam.rdr;
+  t.rdrho = (float)c.eam.rdrho;
+  t.rhomax = (float)c.eam.rhomax;
+  t.conv = (float)kEV;
+  std::memcpy(t.t2rhor, c.eam.t2rhor, sizeof(t.t2rhor));
+  std::memcpy(t.t2z2r, c.eam.t2z2r, sizeof(t.t2z2r));
+  std::memcpy(t.t2frho, c.eam.t2frho, sizeof(t.t2frho));
+  return t;
+}
+
+template <>
+const double* eam_pair_table<double>(Ctx& c)
+{
+  return c.eam.pair_d.p;
+}
+template <>
+const float* eam_pair_table<float>(Ctx& c)
+{
+  return c.eam.pair_f.p;
+}
+
+Outputs make_outputs(Ctx& c, const ForceEpilogue& ep)
+{
+  Outputs o;
+  o.ax = c.ax.p;
+  o.ay = c.ay.p;
+  o.az = c.az.p;
+  o.ep = c.ep.p;
+  o.vir = c.vir.p;
+  o.mass = c.mass_d.p;
+  o.type = c.type.p;
+  o.vx = c.vx.p;
+  o.vy = c.vy.p;
+  o.vz = c.vz.p;
+  o.kick_dt = ep.kick_dt;
+  o.npad = c.npad;
+  return o;
+}
+
 void launch_lj(Ctx& c, double eps, double sigma, double rcut, unsigned flags, const ForceEpilogue& epi)
 {
   if (c.n == 0) return;
+  ensure_global_list(c);
   if (flags & XS_MIXED) lj_dispatch<float>(c, eps, sigma, rcut, flags, epi);
   else lj_dispatch<double>(c, eps, sigma, rcut, flags, epi);
 }
@@ -557,13 +377,26 @@ void launch_eam(Ctx& c, double rcut, bool do_rho, bool do_emb, bool do_force, un
                 const ForceEpilogue& epi)
 {
   if (c.n == 0) return;
+  if (do_emb && !do_rho && c.rho_state != RhoState::Rho) throw Error(XS_E_STATE, "eam_rho2emb needs eam_rho first");
+  const bool mixed = (flags & XS_MIXED) != 0;
+  if (do_force && !(do_rho && do_emb)) {
+    const RhoState want = mixed ? RhoState::FpFloat : RhoState::FpDouble;
+    if (c.rho_state != want)
+      throw Error(XS_E_STATE, "eam_force needs F'(rho) from eam_rho2emb in the same precision");
+  }
   const bool multi = c.eam.nel > 1;
-  if (flags & XS_MIXED) {
+  if (c.list_kind == ListKind::Block) {
+    launch_eam_block(c, rcut, do_rho, do_emb, do_force, flags, epi);
+  } else if (mixed) {
     if (multi) eam_dispatch<float, true>(c, rcut, do_rho, do_emb, do_force, flags, epi);
     else eam_dispatch<float, false>(c, rcut, do_rho, do_emb, do_force, flags, epi);
   } else {
     if (multi) eam_dispatch<double, true>(c, rcut, do_rho, do_emb, do_force, flags, epi);
     else eam_dispatch<double, false>(c, rcut, do_rho, do_emb, do_force, flags, epi);
+  }
+  if (do_rho || do_emb) {
+    c.rho_state = do_emb ? (mixed ? RhoState::FpFloat : RhoState::FpDouble) : RhoState::Rho;
+    if (do_rho) c.rho_ckpt = true;
   }
 }
 

@@ -1,0 +1,225 @@
+// pair_math.hpp -- device arithmetic shared by the global-list kernels (k_pair.hip)
+// and the cell-block kernels (k_block.hip): spline evaluation (eam_alloy.h:198-313),
+// the EAM pair scalars (eam_alloy_mm_force), the per-atom output epilogue.
+//
+// Real is the type of the pair and spline arithmetic: double for the plain FP64
+// form, float for the mixed form (FP64 displacement, d^2, cutoff test and all
+// accumulations in either case).
+#pragma once
+
+#include "xsmd_internal.hpp"
+
+namespace xsmd {
+namespace pm {
+
+template <class Real>
+struct Q4 {
+  Real a, b, c, d;
+};
+
+__device__ __forceinline__ Q4<float> load4(const float* __restrict__ p)
+{
+  const float4 v = *reinterpret_cast<const float4*>(p);
+  return {v.x, v.y, v.z, v.w};
+}
+__device__ __forceinline__ Q4<double> load4(const double* __restrict__ p)
+{
+  const double2 u = reinterpret_cast<const double2*>(p)[0];
+  const double2 w = reinterpret_cast<const double2*>(p)[1];
+  return {u.x, u.y, w.x, w.y};
+}
+
+// 1/sqrt(d2) and sqrt(d2): one hardware rsq in FP32, correctly rounded sqrt
+// and division in FP64 (what plain FP64 code compiles to).
+__device__ __forceinline__ void dist(float d2, float& r, float& rinv)
+{
+  rinv = __builtin_amdgcn_rsqf(d2);
+  r = d2 * rinv;
+}
+__device__ __forceinline__ void dist(double d2, double& r, double& rinv)
+{
+  r = sqrt(d2);
+  rinv = 1.0 / r;
+}
+__device__ __forceinline__ float recip(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double recip(double x) { return 1.0 / x; }
+
+// XCD-aware block order: hardware hands workgroup b to XCD b % 8, so map it to
+// logical block (b % 8) * (G/8) + b / 8 and each XCD's L2 sees one contiguous
+// (hence spatially compact) range of atoms.  G is padded to a multiple of 8.
+__device__ __forceinline__ int64_t logical_block()
+{
+  const unsigned b = blockIdx.x, g = gridDim.x;
+  return (int64_t)(b & 7u) * (g >> 3) + (b >> 3);
+}
+
+struct Outputs {
+  double* ax;
+  double* ay;
+  double* az;
+  double* ep;
+  double* vir;
+  const double* mass;  // fused epilogue
+  const uint8_t* type;
+  double* vx;
+  double* vy;
+  double* vz;
+  double kick_dt;
+  int64_t npad;
+};
+
+// Per-atom epilogue: add the force (operator form) or write a = f/m and kick v
+// (fused MD form); energy and virial when requested.
+template <bool ENERGY, bool VIRIAL, bool FUSED>
+__device__ __forceinline__ void write_outputs(const Outputs& o, int64_t i, double fx, double fy, double fz,
+                                              double e, const double* v)
+{
+  if constexpr (FUSED) {
+    const double m = o.mass[o.type[i]];
+    const double a0 = fx / m, a1 = fy / m, a2 = fz / m;
+    o.ax[i] = a0;
+    o.ay[i] = a1;
+    o.az[i] = a2;
+    o.vx[i] = fma(a0, o.kick_dt, o.vx[i]);
+    o.vy[i] = fma(a1, o.kick_dt, o.vy[i]);
+    o.vz[i] = fma(a2, o.kick_dt, o.vz[i]);
+  } else {
+    o.ax[i] += fx;
+    o.ay[i] += fy;
+    o.az[i] += fz;
+  }
+  if constexpr (ENERGY) o.ep[i] += e;
+  if constexpr (VIRIAL) {
+#pragma unroll
+    for (int q = 0; q < 9; q++) {
+      if constexpr (FUSED) o.vir[q * o.npad + i] = v[q];
+      else o.vir[q * o.npad + i] += v[q];
+    }
+  }
+}
+
+// onika tensor(fe, dr) * -0.5 added to v (row-major m11 m12 ... m33)
+__device__ __forceinline__ void add_virial(double* v, double fe0, double fe1, double fe2, double dx, double dy,
+                                           double dz)
+{
+  const double fe[3] = {fe0, fe1, fe2};
+  const double dr[3] = {dx, dy, dz};
+#pragma unroll
+  for (int a = 0; a < 3; a++)
+#pragma unroll
+    for (int b = 0; b < 3; b++) v[3 * a + b] += (fe[a] * dr[b]) * -0.5;
+}
+
+template <class Real>
+__device__ __forceinline__ void knot_r(const EamTabDev<Real>& t, Real r, int& m, Real& p)
+{
+  p = r * t.rdr + Real(1);
+  m = (int)p;
+  m = max(0, min(m, t.nr - 1));
+  p -= (Real)m;
+  p = fmin(p, Real(1));
+}
+
+template <class Real>
+__device__ __forceinline__ const Real* row_r(const Real* tab, const EamTabDev<Real>& t, int which, int m)
+{
+  return tab + ((int64_t)which * (t.nr + 1) + m) * 8;
+}
+
+template <class Real>
+__device__ __forceinline__ Real cubic(const Q4<Real>& c, Real p)
+{
+  return ((c.a * p + c.b) * p + c.c) * p + c.d;
+}
+template <class Real>
+__device__ __forceinline__ Real quad(const Q4<Real>& c, Real p)
+{
+  return (c.a * p + c.b) * p + c.c;
+}
+// derivative of the value cubic with respect to p
+template <class Real>
+__device__ __forceinline__ Real dcubic(const Q4<Real>& c, Real p)
+{
+  return (Real(3) * c.a * p + Real(2) * c.b) * p + c.c;
+}
+
+// F(rho), F'(rho) in eV-converted internal units, with the linear extrapolation
+// of F above rhomax (eam_alloy.h:289-313).
+template <class Real>
+__device__ __forceinline__ void embed(const EamTabDev<Real>& t, int ti, double rho, Real& F, Real& Fp)
+{
+  const Real rr = (Real)rho;
+  Real p = rr * t.rdrho + Real(1);
+  int m = (int)p;
+  m = max(1, min(m, t.nrho - 1));
+  p -= (Real)m;
+  p = fmin(p, Real(1));
+  const Real* row = t.frho + ((int64_t)t.t2frho[ti + 1] * (t.nrho + 1) + m) * 8;
+  const Q4<Real> v = load4(row);
+  const Q4<Real> dv = load4(row + 4);
+  Fp = quad(dv, p);
+  F = cubic(v, p);
+  if (rr > t.rhomax) F += Fp * (rr - t.rhomax);
+  F *= t.conv;
+  Fp *= t.conv;
+}
+
+// rho_{type_j}(r) contribution to atom i (SymRhoOp, eam_force_op_multimat.h:142-171).
+// Single-element tables read the rho cubic from the first half of the combined row.
+template <class Real, bool MULTI>
+__device__ __forceinline__ Real eam_rho(const EamTabDev<Real>& t, const Real* __restrict__ pair, int ti, int tj,
+                                        double d2)
+{
+  Real r, rinv;
+  dist((Real)d2, r, rinv);
+  int m;
+  Real p;
+  knot_r(t, r, m, p);
+  const Real* row;
+  if constexpr (MULTI) row = row_r(t.rhor, t, t.t2rhor[tj + 1][ti + 1], m);
+  else row = pair + (int64_t)m * 8;
+  return cubic(load4(row), p);
+}
+
+// psi'(r)/r and phi(r) of eam_alloy_mm_force (eam_alloy.h:222-265):
+// psi' = F'_i rho'_j + F'_j rho'_i + phi', phi = z2r / r.
+template <class Real, bool MULTI>
+__device__ __forceinline__ void eam_pair(const EamTabDev<Real>& t, const Real* __restrict__ pair, int ti, int tj,
+                                         Real fpi, Real fpj, double d2, Real& fpair, Real& phi)
+{
+  Real r, rinv;
+  dist((Real)d2, r, rinv);
+  int m;
+  Real p;
+  knot_r(t, r, m, p);
+  Real psip;
+  if constexpr (MULTI) {
+    const Real rhoip = quad(load4(row_r(t.rhor, t, t.t2rhor[ti + 1][tj + 1], m) + 4), p);
+    const Real rhojp = quad(load4(row_r(t.rhor, t, t.t2rhor[tj + 1][ti + 1], m) + 4), p);
+    const Real* zrow = row_r(t.z2r, t, t.t2z2r[ti + 1][tj + 1], m);
+    const Real z2p = quad(load4(zrow + 4), p);
+    const Real z2 = cubic(load4(zrow), p);
+    phi = z2 * rinv;
+    const Real phip = (z2p * rinv - phi * rinv) * t.conv;
+    phi *= t.conv;
+    psip = fpi * rhojp + fpj * rhoip + phip;
+  } else {
+    const Real* row = pair + (int64_t)m * 8;
+    const Q4<Real> rc = load4(row);
+    const Q4<Real> zc = load4(row + 4);
+    const Real rhop = dcubic(rc, p) * t.rdr;
+    const Real z2p = dcubic(zc, p) * t.rdr;
+    const Real z2 = cubic(zc, p);
+    phi = z2 * rinv;
+    const Real phip = (z2p * rinv - phi * rinv) * t.conv;
+    phi *= t.conv;
+    psip = (fpi + fpj) * rhop + phip;
+  }
+  fpair = psip * rinv;
+}
+
+}  // namespace pm
+
+pm::Outputs make_outputs(Ctx& c, const ForceEpilogue& ep);
+
+}  // namespace xsmd

@@ -127,6 +127,25 @@ struct EamHost {
 // pass alone, or F'(rho) (pos[i].w) after the embedding step.
 enum class RhoState { None, Rho, FpDouble, FpFloat };
 
+// Which neighbour list is current: the global ELL list of (index, image)
+// entries, or the cell-block list of 16-bit halo slots (k_block.hip).
+enum class ListKind { Global, Block };
+
+// Cell-block decomposition: bricks of bx*by*bz cells, nbx*nby*nbz per replica.
+struct BlockGeom {
+  int nx, ny, nz;
+  int bx, by, bz;
+  int nbx, nby, nbz;
+  int px, py, pz;
+  int nh_max;        // (bx+2)(by+2)(bz+2) halo cells
+  int64_t ncell_rep;
+  int64_t nblocks;   // over all replicas
+};
+
+constexpr double kBlockAtoms = 200.0;      // target central atoms per brick
+constexpr int kMaxHaloCells = 216;
+constexpr size_t kBlockLdsLimit = 64 * 1024;  // default dynamic-LDS limit of a launch
+
 // Event-pair timing per kernel class (see XS_PROF_* in xsmd.h).
 struct Profiler {
   bool on = false;
@@ -239,7 +258,6 @@ struct Ctx {
   int64_t ncell_rep = 1, ncell_tot = 1;
   DBuf<uint32_t> keys, keys2, vals, vals2;
   DBuf<uint8_t> sort_tmp;
-  DBuf<int> cstart, cend;
 
   // neighbour list, ELL column-major: nbr[k*npad + i], k < nnb[i] <= nmax
   DBuf<uint32_t> nbr;
@@ -247,6 +265,15 @@ struct Ctx {
   int nmax = 0;
   bool list_valid = false;
   double list_rcut = 0, list_skin = 0;
+  // cell-block list (same layout, 16-bit halo slots) and its geometry
+  ListKind list_kind = ListKind::Global;
+  bool global_from_block = false;  // nbr holds the block list converted to global form
+  bool block_disabled = false;     // XSMD_LIST=global at context creation
+  DBuf<uint16_t> nbr16;
+  DBuf<int> cell_off;              // first atom of each cell, ncell_tot + 1
+  BlockGeom bgeom{};
+  int bcap = 0;                    // LDS slots per block (max halo atoms, rounded)
+  int bcentral = 0;                // max central atoms per block
 
   // small device scalars + pinned host mirror
   DBuf<unsigned long long> dscalar;  // [0] max count, [1] max displacement^2 bits
@@ -269,7 +296,18 @@ struct Ctx {
 };
 
 // ---- launches (k_neighbors.hip) ----
-void build_neighbor_list(Ctx& c, double rcut, double skin);
+// prefer_block: build the cell-block list when its LDS footprint allows
+void build_neighbor_list(Ctx& c, double rcut, double skin, bool prefer_block = false);
+
+// ---- cell blocks (k_block.hip) ----
+void cell_offsets(Ctx& c);
+void block_setup(Ctx& c);
+bool block_feasible(const Ctx& c);
+size_t block_lds_bytes(const Ctx& c, bool with_type);
+unsigned block_grid(const Ctx& c);
+void block_build(Ctx& c, double rl2);
+void block_to_global(Ctx& c);
+void ensure_global_list(Ctx& c);
 double max_displacement2(Ctx& c);
 void pair_count(Ctx& c, double rcut, int64_t& entries, int64_t& within);
 
@@ -281,6 +319,7 @@ void launch_push_f_v(Ctx& c, double dt);
 // fused first half of a step: r += v dt + a dt^2/2 ; v += a dt/2 ; max |r - r_ref|^2
 void launch_drift_kick(Ctx& c, double dt);
 void launch_thermo(Ctx& c, double out[27]);
+double stream_copy(Ctx& c, size_t bytes, int reps);
 
 // ---- launches (k_pair.hip) ----
 struct ForceEpilogue {
@@ -290,6 +329,12 @@ struct ForceEpilogue {
 void launch_lj(Ctx& c, double eps, double sigma, double rcut, unsigned flags, const ForceEpilogue& ep);
 void launch_eam(Ctx& c, double rcut, bool do_rho, bool do_rho2emb, bool do_force, unsigned flags,
                 const ForceEpilogue& ep);
+void launch_eam_block(Ctx& c, double rcut, bool do_rho, bool do_emb, bool do_force, unsigned flags,
+                      const ForceEpilogue& ep);
+template <class Real>
+EamTabDev<Real> make_eam_tab(Ctx& c);
+template <class Real>
+const Real* eam_pair_table(Ctx& c);
 
 // ---- EAM adjoint (k_adjoint.hip) ----
 void eam_adjoint(Ctx& c, double rcut, const double* dLdF_id, const double* dLdE, double* grad, bool mixed);

@@ -295,6 +295,19 @@ class Simulation:
         self.ctx.xs_field_get(_lib.XS_FIELD_RHO_DEMB, _lib.dptr(out))
         return out
 
+    def chunk_neighbors_stats(self, rcut) -> dict:
+        """List entries and entries within rcut at the current positions (the
+        chunk_neighbors_stats diagnostic, data/samples/perf/microjet_lj.msp:25-28)."""
+        e, w = C.c_int64(0), C.c_int64(0)
+        self.ctx.xs_pair_count(parse_quantity(rcut), C.byref(e), C.byref(w))
+        return {"entries": e.value, "within_rcut": w.value}
+
+    def list_info(self) -> dict:
+        """{'kind': 'global'|'block', 'blocks': bricks, 'halo_capacity': LDS slots}."""
+        kind, blocks, cap = C.c_int(0), C.c_int64(0), C.c_int(0)
+        self.ctx.xs_list_info(C.byref(kind), C.byref(blocks), C.byref(cap))
+        return {"kind": ("global", "block")[kind.value], "blocks": blocks.value, "halo_capacity": cap.value}
+
     def neighbors(self):
         """(counts[n], ids[total], imgs[total,3]) in id space, entries in list order."""
         counts = np.zeros(self.n, dtype=np.int64)

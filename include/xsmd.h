@@ -24,6 +24,7 @@
 #ifndef XSMD_H
 #define XSMD_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -69,6 +70,8 @@ xs_status xs_ctx_destroy(xs_ctx* ctx);
 const char* xs_last_error(const xs_ctx* ctx);
 /* Wait for all work queued on the context's stream. */
 xs_status xs_synchronize(xs_ctx* ctx);
+/* Free and total device memory of the context's GPU (hipMemGetInfo). */
+xs_status xs_device_memory(xs_ctx* ctx, size_t* free_bytes, size_t* total_bytes);
 
 /* --- species and system ------------------------------------------------------ */
 /* Masses per atom type (ParticleSpecies, src/particle_species/include/exaStamp/
@@ -115,6 +118,12 @@ xs_status xs_order_get(xs_ctx* ctx, int64_t* perm);
  * data/samples/perf/microjet_lj.msp:25-28): list entries and entries with
  * d^2 <= rcut^2 at the current positions. */
 xs_status xs_pair_count(xs_ctx* ctx, double rcut, int64_t* entries, int64_t* within_rcut);
+/* Which list the last build produced (diagnostic; no counterpart in the reference):
+ * *kind = 0 for the global (index, image) list, 1 for the cell-block list of the
+ * LDS-staged EAM kernels; *blocks = bricks of cells, *halo_capacity = LDS slots per
+ * brick (both 0 for the global list).  The environment variable XSMD_LIST=global,
+ * read when a context is created, keeps every build on the global list. */
+xs_status xs_list_info(xs_ctx* ctx, int* kind, int64_t* blocks, int* halo_capacity);
 
 /* --- operators -------------------------------------------------------------------- */
 /* zero_force_energy (src/compute/zero_force_energy.cu:38-147): a = ep = virial = 0. */
@@ -202,6 +211,11 @@ xs_status xs_md_run(xs_ctx* ctx, int potential, int nsteps, double dt, double rc
 #define XS_PROF_THERMO 5
 #define XS_PROF_ADJOINT 6     /* both EAM adjoint sweeps */
 #define XS_PROF_NCLASSES 7
+/* Streaming copy of `bytes` between two device buffers (allocated and freed by the
+ * call), repeated `reps` times; *gbs receives (read + write bytes) / time of the
+ * fastest repeat.  The achievable-bandwidth reference for the roofline, and a
+ * known-size kernel (k_stream_copy) for calibrating HBM byte counters. */
+xs_status xs_stream_copy(xs_ctx* ctx, size_t bytes, int reps, double* gbs);
 /* on != 0 enables and clears the counters; on == 0 disables. */
 xs_status xs_profile_enable(xs_ctx* ctx, int on);
 /* ms_total[XS_PROF_NCLASSES] and launches[XS_PROF_NCLASSES] (either may be NULL). */

@@ -1,0 +1,135 @@
+"""Cell-block (LDS-staged) EAM path against the global-list path and the oracle.
+
+The block list is the global list in another encoding (16-bit halo slots instead of
+(index, image) words): converted back it must be identical entry by entry, and the
+block kernels evaluate the same pairs in the same order with the same arithmetic as
+the global-list kernels, so forces, energies, virials and F'(rho) carry the same bits.
+The global path is selected with XSMD_LIST=global, read when a context is created.
+
+Tolerances: neighbour lists and every per-atom output bit-exact between the two
+paths; against the oracle as in test_gpu_parity (FP64 1e-11, mixed 1e-5 of rms|F|).
+"""
+import numpy as np
+import pytest
+
+from conftest import cu_tables, cuni_tables
+from test_gpu_parity import CU_MASS, cu_config, frel, make_oracle, make_sim
+
+pytestmark = pytest.mark.gpu
+
+CUNI = [63.546, 58.693]
+
+
+def _case(name):
+    from exastamp_amd import inputs
+    if name == "cu_jitter":
+        return cu_config((9, 8, 7), jitter=0.12, seed=5), [CU_MASS], None, cu_tables()
+    if name == "cu_one_cell":        # 7.2 A box: one cell per axis, stencil sees 3 images
+        return cu_config((2, 2, 2), jitter=0.05, seed=6), [CU_MASS], None, cu_tables()
+    if name == "cu_two_cells":       # two cells per axis: -1 and +1 neighbours are one cell
+        return cu_config((3, 4, 3), jitter=0.05, seed=7), [CU_MASS], None, cu_tables()
+    if name == "cuni_slab":          # two species, free surface along z
+        cfg = inputs.alloy_slab_config((8, 8, 6), 3.57, CUNI, vacancy_frac=0.01, vacuum=10.0, T=500.0, seed=9)
+        return cfg, CUNI, ["Cu", "Ni"], cuni_tables()
+    if name == "replicas":           # several boxes, different strains
+        return inputs.strained_batch(5, (4, 4, 4), 3.615, [CU_MASS], seed=13), [CU_MASS], None, cu_tables(500)
+    raise KeyError(name)
+
+
+CASES = ["cu_jitter", "cu_one_cell", "cu_two_cells", "cuni_slab", "replicas"]
+
+
+def _sim(monkeypatch, kind, cfg, masses, names, tab):
+    if kind == "global":
+        monkeypatch.setenv("XSMD_LIST", "global")
+    else:
+        monkeypatch.delenv("XSMD_LIST", raising=False)
+    sim = make_sim(cfg, masses, names)
+    sim.eam_alloy_init(tab)
+    sim.chunk_neighbors(rcut_max=tab.rc, rcut_inc=0.3)
+    assert sim.list_info()["kind"] == kind
+    return sim
+
+
+def _outputs(sim, rc, mixed):
+    sim.zero_force_energy()
+    sim.eam_alloy_force(rcut=rc, trigger_thermo_state=True, compute_virial=True, mixed=mixed)
+    return sim.accelerations(), sim.ep(), sim.virial(), sim.rho_dEmb()
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_block_list_equals_global_list(monkeypatch, name):
+    cfg, masses, names, tab = _case(name)
+    b = _sim(monkeypatch, "block", cfg, masses, names, tab)
+    g = _sim(monkeypatch, "global", cfg, masses, names, tab)
+    info = b.list_info()
+    assert info["blocks"] >= 1 and info["halo_capacity"] >= 1
+    assert np.array_equal(b.order(), g.order())
+    for x, y in zip(b.neighbors(), g.neighbors()):
+        assert np.array_equal(x, y)
+
+
+@pytest.mark.parametrize("name", CASES)
+@pytest.mark.parametrize("mixed", [False, True])
+def test_block_forces_bitexact_vs_global(monkeypatch, name, mixed):
+    cfg, masses, names, tab = _case(name)
+    b = _sim(monkeypatch, "block", cfg, masses, names, tab)
+    g = _sim(monkeypatch, "global", cfg, masses, names, tab)
+    for x, y in zip(_outputs(b, tab.rc, mixed), _outputs(g, tab.rc, mixed)):
+        assert np.array_equal(x, y)
+
+
+@pytest.mark.parametrize("name", ["cu_one_cell", "cu_two_cells"])
+def test_block_small_boxes_vs_oracle(oracle_mod, monkeypatch, name):
+    """Boxes of one and two cells per axis against the oracle directly."""
+    cfg, masses, names, tab = _case(name)
+    sim = _sim(monkeypatch, "block", cfg, masses, names, tab)
+    _outputs(sim, tab.rc, False)
+    o = make_oracle(oracle_mod, cfg, masses)
+    o.set_eam(tab)
+    o.build_neighbors(tab.rc, 0.3)
+    o.compute_forces("eam")
+    assert np.array_equal(sim.order(), o.order())
+    for x, y in zip(sim.neighbors(), o.neighbors()):
+        assert np.array_equal(x, y)
+    assert frel(sim.forces(), o.forces()) <= 1e-11
+    ref = o.state()["ep"].sum()
+    assert abs(sim.ep().sum() - ref) <= 1e-12 * abs(ref)
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_block_md_run_bitexact_vs_global(monkeypatch, mixed):
+    """40 fused NVE steps with rebuilds: identical trajectories on both list kinds."""
+    tab = cu_tables()
+    cfg = cu_config((6, 6, 6), T=1200.0, seed=4)
+    out = []
+    for kind in ("block", "global"):
+        if kind == "global":
+            monkeypatch.setenv("XSMD_LIST", "global")
+        else:
+            monkeypatch.delenv("XSMD_LIST", raising=False)
+        sim = make_sim(cfg, [CU_MASS])
+        sim.eam_alloy_init(tab)
+        stats = sim.md_run("eam", 40, 0.002, tab.rc, 0.3, mixed=mixed)
+        assert sim.list_info()["kind"] == kind
+        out.append((sim.state(), stats))
+    (ra, va), sa = out[0]
+    (rb, vb), sb = out[1]
+    assert np.array_equal(ra, rb) and np.array_equal(va, vb)
+
+
+def test_block_adjoint_and_pair_count_use_converted_list(monkeypatch):
+    """Operators that read the global list (pair count, adjoint) work after a block build
+    and give the global-build results."""
+    tab = cu_tables(500)
+    cfg = cu_config((5, 5, 5), jitter=0.05, seed=12)
+    res = []
+    for kind in ("block", "global"):
+        sim = _sim(monkeypatch, kind, cfg, [CU_MASS], None, tab)
+        _outputs(sim, tab.rc, False)
+        pc = sim.chunk_neighbors_stats(tab.rc)
+        g = np.random.default_rng(1).normal(size=(cfg.n, 3))
+        grad = sim.eam_adjoint(g, np.ones(1))
+        res.append((pc, grad))
+    assert res[0][0] == res[1][0] and res[0][0]["within_rcut"] > 0
+    assert np.array_equal(res[0][1], res[1][1])
