@@ -46,7 +46,7 @@ def parse():
     ap.add_argument("--fp64-steps", type=int, default=None, help="timed steps of the FP64 form (default: --steps)")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=12)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--seed", type=int, default=2024)
@@ -264,6 +264,7 @@ def main():
     log(f"mixed precision: {args.warmup} warmup + {args.steps} timed steps, {cfg.n} atoms")
     mixed = run_form(sim, cfg, tab, True, args.steps, args.warmup, dt, rc, skin)
     ferr = force_error(sim, rc)
+    list_info = sim.list_info()
     entries, within = np.zeros(1, dtype=np.int64), np.zeros(1, dtype=np.int64)
     import ctypes as C
     sim.ctx.xs_pair_count(rc, entries.ctypes.data_as(C.POINTER(C.c_int64)),
@@ -284,7 +285,8 @@ def main():
     gbs = FORCE_BYTES_PER_ATOM * cfg.n / avg_s / 1e9
     tflops = (FORCE_FLOPS_PER_PAIR * pairs_per_atom + FORCE_FLOPS_PER_ATOM) * cfg.n / avg_s / 1e12
     roof = {
-        "kernel": "k_eam_force<float> (EAM force pass + force_to_accel + half kick)",
+        "kernel": ("k_block_eam_force<float>" if list_info["kind"] == "block" else "k_eam_force<float>")
+                  + " (EAM force pass + force_to_accel + half kick)",
         "bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
         "traffic": None,
         "bytes_per_atom": FORCE_BYTES_PER_ATOM, "avg_launch_ms": kf["ms_avg"],
@@ -343,6 +345,8 @@ def main():
                                          "fp64": None if fp64 is None else fp64["drift_ev_per_atom_per_ps"]},
         "rebuilds": mixed["rebuilds"],
         "pairs_per_atom_within_rc": pairs_per_atom,
+        "list_entries_per_atom": float(entries[0]) / cfg.n,
+        "neighbor_list": list_info,
         "kernels_mixed": mixed["kernels"],
         "kernels_fp64": None if fp64 is None else fp64["kernels"],
         "roofline": roof,

@@ -173,6 +173,10 @@ xs_status xs_ctx_create(int device, xs_ctx** out)
     c->box_d.reserve(1);
     const char* kind = std::getenv("XSMD_LIST");
     c->block_disabled = kind && std::strcmp(kind, "global") == 0;
+    if (const char* ba = std::getenv("XSMD_BLOCK_ATOMS")) {
+      const double v = std::atof(ba);
+      if (v >= 1.0 && v <= 4096.0) c->block_atoms = v;
+    }
   } catch (const Error&) {
     delete c;
     return XS_E_HIP;

@@ -310,6 +310,17 @@ struct BlockArgs {
 };
 
 constexpr int kBBatch = 4;
+// EAM passes run two lanes per central atom: the two halves of a wave take the
+// even and odd list entries of the same 32 atoms and combine their sums with one
+// cross-half exchange, doubling the waves that hide the table-gather latency for
+// the LDS a brick needs.
+constexpr int kPassThreads = 512;
+constexpr int kPassAtoms = kPassThreads / 2;
+// Minimum waves per SIMD the register allocation must allow: three 8-wave
+// workgroups per CU (what the LDS of a ~200-atom brick permits) need 6; the
+// virial form of the force pass carries 9 more sums and is held to 4.
+template <bool VIRIAL>
+constexpr int kPassWaves = VIRIAL ? 4 : 6;
 
 // FP64 displacement to the staged record of slot c (image already applied)
 struct LNb {
@@ -318,13 +329,17 @@ struct LNb {
   bool in;
 };
 
+// entries k0, k0+2, ..., k0+2(U-1) of atom gi (this lane's parity of the list)
 template <int U>
 __device__ __forceinline__ void lfetch(const BlockArgs& a, const Pos4* rec, int64_t gi, int self, int k0, int cnt,
                                        const Pos4& pi, LNb (&nb)[U])
 {
   int c[U];
 #pragma unroll
-  for (int u = 0; u < U; u++) c[u] = (k0 + u < cnt) ? (int)a.nbr16[(int64_t)(k0 + u) * a.npad + gi] : self;
+  for (int u = 0; u < U; u++) {
+    const int k = k0 + 2 * u;
+    c[u] = (k < cnt) ? (int)a.nbr16[(int64_t)k * a.npad + gi] : self;
+  }
 #pragma unroll
   for (int u = 0; u < U; u++) {
     const Pos4 pj = rec[c[u]];
@@ -334,14 +349,23 @@ __device__ __forceinline__ void lfetch(const BlockArgs& a, const Pos4* rec, int6
     nb[u].d2 = xsmd_d2(nb[u].x, nb[u].y, nb[u].z);
     nb[u].w = pj.w;
     nb[u].c = c[u];
-    nb[u].in = (k0 + u < cnt) && nb[u].d2 <= a.rc2;
+    nb[u].in = (k0 + 2 * u < cnt) && nb[u].d2 <= a.rc2;
   }
 }
 
-template <class Real, bool MULTI, bool ENERGY, bool SET_EP>
-__global__ void __launch_bounds__(256) k_block_eam_rho(BlockArgs a, EamTabDev<Real> t, const Real* __restrict__ pair,
-                                                       double* __restrict__ rho_io, double* __restrict__ pos_w,
-                                                       double* __restrict__ ep)
+// sum of v over the two halves of the wave (lanes l and l ^ 32)
+__device__ __forceinline__ double half_sum(double v) { return v + __shfl_xor(v, 32); }
+
+// this lane's atom slot and list parity
+__device__ __forceinline__ int pass_slot() { return (threadIdx.x >> 6) * 32 + (threadIdx.x & 31); }
+__device__ __forceinline__ int pass_half() { return (threadIdx.x >> 5) & 1; }
+
+// rho_i, then (EMB) F(rho_i) and F'(rho_i) for the central atoms of one brick
+template <class Real, bool MULTI, bool EMB, bool ENERGY, bool SET_EP>
+__global__ void __launch_bounds__(kPassThreads, kPassWaves<false>) k_block_eam_rho(BlockArgs a, EamTabDev<Real> t,
+                                                                const Real* __restrict__ pair,
+                                                                double* __restrict__ rho_io,
+                                                                double* __restrict__ pos_w, double* __restrict__ ep)
 {
   const int64_t b = lblock();
   if (b >= a.g.nblocks) return;
@@ -350,14 +374,15 @@ __global__ void __launch_bounds__(256) k_block_eam_rho(BlockArgs a, EamTabDev<Re
   const int total = build_halo(a.g, s, a.cell_off, l.hc, l.total);
   if (total > a.cap) return;  // excluded by the host (cap = measured maximum)
   stage(s, l.hc, total, a.pos, a.box[s.r].L, a.type, l.rec, l.ltype);
-  for (int q = threadIdx.x;; q += blockDim.x) {
+  const int half = pass_half();
+  for (int q = pass_slot();; q += kPassAtoms) {
     Central ca;
     if (!central_atom(s, l.hc, q, ca)) break;
     const Pos4 pi = l.rec[ca.local];
     const int ti = MULTI ? l.ltype[ca.local] : 0;
     const int cnt = a.nnb[ca.global];
     double rho = 0.0;
-    for (int k0 = 0; k0 < cnt; k0 += kBBatch) {
+    for (int k0 = half; k0 < cnt; k0 += 2 * kBBatch) {
       LNb nb[kBBatch];
       lfetch<kBBatch>(a, l.rec, ca.global, ca.local, k0, cnt, pi, nb);
 #pragma unroll
@@ -367,20 +392,25 @@ __global__ void __launch_bounds__(256) k_block_eam_rho(BlockArgs a, EamTabDev<Re
         rho += nb[u].in ? (double)val : 0.0;
       }
     }
-    rho_io[ca.global] = rho;
-    Real F, Fp;
-    pm::embed(t, ti, rho, F, Fp);
-    pos_w[4 * ca.global + 3] = (double)Fp;
-    if constexpr (ENERGY) {
-      if constexpr (SET_EP) ep[ca.global] = (double)F;
-      else ep[ca.global] += (double)F;
+    rho = half_sum(rho);
+    if (half == 0) {
+      rho_io[ca.global] = rho;
+      if constexpr (EMB) {
+        Real F, Fp;
+        pm::embed(t, ti, rho, F, Fp);
+        pos_w[4 * ca.global + 3] = (double)Fp;
+        if constexpr (ENERGY) {
+          if constexpr (SET_EP) ep[ca.global] = (double)F;
+          else ep[ca.global] += (double)F;
+        }
+      }
     }
   }
 }
 
 template <class Real, bool MULTI, bool ENERGY, bool VIRIAL, bool FUSED>
-__global__ void __launch_bounds__(256) k_block_eam_force(BlockArgs a, EamTabDev<Real> t,
-                                                         const Real* __restrict__ pair, pm::Outputs o)
+__global__ void __launch_bounds__(kPassThreads, kPassWaves<VIRIAL>) k_block_eam_force(BlockArgs a, EamTabDev<Real> t,
+                                                                  const Real* __restrict__ pair, pm::Outputs o)
 {
   const int64_t b = lblock();
   if (b >= a.g.nblocks) return;
@@ -389,7 +419,8 @@ __global__ void __launch_bounds__(256) k_block_eam_force(BlockArgs a, EamTabDev<
   const int total = build_halo(a.g, s, a.cell_off, l.hc, l.total);
   if (total > a.cap) return;  // excluded by the host (cap = measured maximum)
   stage(s, l.hc, total, a.pos, a.box[s.r].L, a.type, l.rec, l.ltype);
-  for (int q = threadIdx.x;; q += blockDim.x) {
+  const int half = pass_half();
+  for (int q = pass_slot();; q += kPassAtoms) {
     Central ca;
     if (!central_atom(s, l.hc, q, ca)) break;
     const Pos4 pi = l.rec[ca.local];
@@ -398,7 +429,7 @@ __global__ void __launch_bounds__(256) k_block_eam_force(BlockArgs a, EamTabDev<
     const int cnt = a.nnb[ca.global];
     double fx = 0.0, fy = 0.0, fz = 0.0, e = 0.0;
     double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    for (int k0 = 0; k0 < cnt; k0 += kBBatch) {
+    for (int k0 = half; k0 < cnt; k0 += 2 * kBBatch) {
       LNb nb[kBBatch];
       lfetch<kBBatch>(a, l.rec, ca.global, ca.local, k0, cnt, pi, nb);
 #pragma unroll
@@ -415,7 +446,15 @@ __global__ void __launch_bounds__(256) k_block_eam_force(BlockArgs a, EamTabDev<
         if constexpr (VIRIAL) pm::add_virial(v, fe0, fe1, fe2, nb[u].x, nb[u].y, nb[u].z);
       }
     }
-    pm::write_outputs<ENERGY, VIRIAL, FUSED>(o, ca.global, fx, fy, fz, e, v);
+    fx = half_sum(fx);
+    fy = half_sum(fy);
+    fz = half_sum(fz);
+    if constexpr (ENERGY) e = half_sum(e);
+    if constexpr (VIRIAL) {
+#pragma unroll
+      for (int k = 0; k < 9; k++) v[k] = half_sum(v[k]);
+    }
+    if (half == 0) pm::write_outputs<ENERGY, VIRIAL, FUSED>(o, ca.global, fx, fy, fz, e, v);
   }
 }
 
@@ -442,11 +481,12 @@ void block_eam_dispatch(Ctx& c, double rcut, bool do_rho, bool do_emb, bool do_f
   double* pos_w = reinterpret_cast<double*>(c.pos.p);
   if (do_rho) {
     ProfScope prof(c.prof, c.stream, XS_PROF_EAM_RHO);
-#define XS_BRHO(EN, SET) \
-  k_block_eam_rho<Real, MULTI, EN, SET><<<g, kBlock, lds, c.stream>>>(a, t, pair, c.rho.p, pos_w, c.ep.p)
-    if (energy && epi.fused) XS_BRHO(true, true);
-    else if (energy) XS_BRHO(true, false);
-    else XS_BRHO(false, false);
+#define XS_BRHO(EMB, EN, SET) \
+  k_block_eam_rho<Real, MULTI, EMB, EN, SET><<<g, kPassThreads, lds, c.stream>>>(a, t, pair, c.rho.p, pos_w, c.ep.p)
+    if (!do_emb) XS_BRHO(false, false, false);
+    else if (energy && epi.fused) XS_BRHO(true, true, true);
+    else if (energy) XS_BRHO(true, true, false);
+    else XS_BRHO(true, false, false);
 #undef XS_BRHO
     XS_HIP(hipGetLastError());
   }
@@ -455,7 +495,7 @@ void block_eam_dispatch(Ctx& c, double rcut, bool do_rho, bool do_emb, bool do_f
     const bool vir = flags & XS_VIRIAL;
     ProfScope prof(c.prof, c.stream, XS_PROF_EAM_FORCE);
 #define XS_BF(EN, VI, FU) \
-  k_block_eam_force<Real, MULTI, EN, VI, FU><<<g, kBlock, lds, c.stream>>>(a, t, pair, o)
+  k_block_eam_force<Real, MULTI, EN, VI, FU><<<g, kPassThreads, lds, c.stream>>>(a, t, pair, o)
     if (epi.fused) {
       if (energy && vir) XS_BF(true, true, true);
       else if (energy) XS_BF(true, false, true);
@@ -489,7 +529,7 @@ void block_setup(Ctx& c)
 {
   // brick of about kBlockAtoms atoms, as cubic as the grid allows
   const double per_cell = (double)c.n / (double)c.ncell_tot;
-  const int want = std::max(1, (int)std::lround(kBlockAtoms / std::max(per_cell, 0.5)));
+  const int want = std::max(1, (int)std::lround(c.block_atoms / std::max(per_cell, 0.5)));
   BlockGeom& g = c.bgeom;
   g.nx = c.nc[0];
   g.ny = c.nc[1];
@@ -572,27 +612,26 @@ void block_to_global(Ctx& c)
 
 namespace xsmd {
 
-// EAM on the block list.  The density pass always runs rho + embedding together
-// here; a lone eam_rho or eam_rho2emb call (the split-pass deck) goes through the
-// global-list kernels, whose arithmetic is the same.
+// EAM on the block list.  The embedding step alone (eam_rho2emb after a separate
+// eam_rho, as in the split-pass deck) reads no list and runs the per-atom kernel of
+// the global path; its arithmetic is the same as the fused block pass.
 void launch_eam_block(Ctx& c, double rcut, bool do_rho, bool do_emb, bool do_force, unsigned flags,
                       const ForceEpilogue& epi)
 {
   const bool mixed = (flags & XS_MIXED) != 0;
   const bool multi = c.eam.nel > 1;
-  if (do_rho != do_emb) {
-    ensure_global_list(c);
+  if (do_emb && !do_rho) {
     const ListKind keep = c.list_kind;
     c.list_kind = ListKind::Global;
     try {
-      launch_eam(c, rcut, do_rho, do_emb, false, flags, epi);
+      launch_eam(c, rcut, false, true, false, flags, epi);
     } catch (...) {
       c.list_kind = keep;
       throw;
     }
     c.list_kind = keep;
-    if (do_force) launch_eam_block(c, rcut, false, false, true, flags, epi);
-    return;
+    if (!do_force) return;
+    do_emb = false;
   }
   if (mixed) {
     if (multi) block_eam_dispatch<float, true>(c, rcut, do_rho, do_emb, do_force, flags, epi);

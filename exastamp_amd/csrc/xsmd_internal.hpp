@@ -269,6 +269,7 @@ struct Ctx {
   ListKind list_kind = ListKind::Global;
   bool global_from_block = false;  // nbr holds the block list converted to global form
   bool block_disabled = false;     // XSMD_LIST=global at context creation
+  double block_atoms = kBlockAtoms;  // XSMD_BLOCK_ATOMS at context creation
   DBuf<uint16_t> nbr16;
   DBuf<int> cell_off;              // first atom of each cell, ncell_tot + 1
   BlockGeom bgeom{};

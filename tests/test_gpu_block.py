@@ -1,13 +1,16 @@
 """Cell-block (LDS-staged) EAM path against the global-list path and the oracle.
 
 The block list is the global list in another encoding (16-bit halo slots instead of
-(index, image) words): converted back it must be identical entry by entry, and the
-block kernels evaluate the same pairs in the same order with the same arithmetic as
-the global-list kernels, so forces, energies, virials and F'(rho) carry the same bits.
+(index, image) words): converted back it must be identical entry by entry.  The block
+kernels evaluate the same pairs with the same per-pair arithmetic as the global-list
+kernels but sum each atom's list as two interleaved halves (two lanes per atom), so
+per-atom sums differ from the global path only by FP64 summation order.
 The global path is selected with XSMD_LIST=global, read when a context is created.
 
-Tolerances: neighbour lists and every per-atom output bit-exact between the two
-paths; against the oracle as in test_gpu_parity (FP64 1e-11, mixed 1e-5 of rms|F|).
+Tolerances: neighbour lists bit-exact; per-atom outputs between the two paths within
+1e-12 of their rms (summation order only, in both precision forms); trajectories of
+40 NVE steps within 1e-9 A; against the oracle as in test_gpu_parity (FP64 1e-11,
+mixed 1e-5 of rms|F|).
 """
 import numpy as np
 import pytest
@@ -69,14 +72,19 @@ def test_block_list_equals_global_list(monkeypatch, name):
         assert np.array_equal(x, y)
 
 
+def _close(x, y, tol=1e-12):
+    scale = np.sqrt(np.mean(y.astype(np.float64) ** 2))
+    return np.abs(x - y).max() <= tol * max(scale, 1e-300)
+
+
 @pytest.mark.parametrize("name", CASES)
 @pytest.mark.parametrize("mixed", [False, True])
-def test_block_forces_bitexact_vs_global(monkeypatch, name, mixed):
+def test_block_forces_match_global(monkeypatch, name, mixed):
     cfg, masses, names, tab = _case(name)
     b = _sim(monkeypatch, "block", cfg, masses, names, tab)
     g = _sim(monkeypatch, "global", cfg, masses, names, tab)
     for x, y in zip(_outputs(b, tab.rc, mixed), _outputs(g, tab.rc, mixed)):
-        assert np.array_equal(x, y)
+        assert _close(x, y)
 
 
 @pytest.mark.parametrize("name", ["cu_one_cell", "cu_two_cells"])
@@ -85,6 +93,7 @@ def test_block_small_boxes_vs_oracle(oracle_mod, monkeypatch, name):
     cfg, masses, names, tab = _case(name)
     sim = _sim(monkeypatch, "block", cfg, masses, names, tab)
     _outputs(sim, tab.rc, False)
+    sim.force_to_accel()
     o = make_oracle(oracle_mod, cfg, masses)
     o.set_eam(tab)
     o.build_neighbors(tab.rc, 0.3)
@@ -98,8 +107,8 @@ def test_block_small_boxes_vs_oracle(oracle_mod, monkeypatch, name):
 
 
 @pytest.mark.parametrize("mixed", [False, True])
-def test_block_md_run_bitexact_vs_global(monkeypatch, mixed):
-    """40 fused NVE steps with rebuilds: identical trajectories on both list kinds."""
+def test_block_md_run_matches_global(monkeypatch, mixed):
+    """40 fused NVE steps with rebuilds: the same trajectory on both list kinds."""
     tab = cu_tables()
     cfg = cu_config((6, 6, 6), T=1200.0, seed=4)
     out = []
@@ -115,12 +124,14 @@ def test_block_md_run_bitexact_vs_global(monkeypatch, mixed):
         out.append((sim.state(), stats))
     (ra, va), sa = out[0]
     (rb, vb), sb = out[1]
-    assert np.array_equal(ra, rb) and np.array_equal(va, vb)
+    assert np.abs(ra - rb).max() <= 1e-9
+    assert _close(va, vb, 1e-9)
 
 
 def test_block_adjoint_and_pair_count_use_converted_list(monkeypatch):
     """Operators that read the global list (pair count, adjoint) work after a block build
-    and give the global-build results."""
+    and give the global-build results (pair counts exactly; the adjoint to 1e-12 of its
+    largest entry, since its table gradients are accumulated with atomics)."""
     tab = cu_tables(500)
     cfg = cu_config((5, 5, 5), jitter=0.05, seed=12)
     res = []
@@ -132,4 +143,5 @@ def test_block_adjoint_and_pair_count_use_converted_list(monkeypatch):
         grad = sim.eam_adjoint(g, np.ones(1))
         res.append((pc, grad))
     assert res[0][0] == res[1][0] and res[0][0]["within_rcut"] > 0
-    assert np.array_equal(res[0][1], res[1][1])
+    ga, gb = res[0][1], res[1][1]
+    assert np.abs(ga - gb).max() <= 1e-12 * np.abs(gb).max()

@@ -239,15 +239,21 @@ def test_eam_mixed_within_tolerance(oracle_mod):
     assert abs(sim.ep().sum() - ref["ep"].sum()) <= 1e-6 * abs(ref["ep"].sum())
 
 
-def test_eam_split_passes_equal_fused():
+@pytest.mark.parametrize("kind", ["block", "global"])
+def test_eam_split_passes_equal_fused(monkeypatch, kind):
     """eam_rho / eam_rho2emb / eam_force as three operator calls (the split deck of
     data/regression_new/potentials/eam/eam_alloy/single_specy_nosym_cs1.msp) give the
-    same bits as one call with all passes."""
+    same bits as one call with all passes, on either neighbour-list kind."""
+    if kind == "global":
+        monkeypatch.setenv("XSMD_LIST", "global")
+    else:
+        monkeypatch.delenv("XSMD_LIST", raising=False)
     tab = cu_tables()
     cfg = cu_config((7, 7, 7), seed=21)
     sim = make_sim(cfg, [CU_MASS])
     sim.eam_alloy_init(tab)
     sim.chunk_neighbors(rcut_max=4.95, rcut_inc=0.3)
+    assert sim.list_info()["kind"] == kind
     for mixed in (False, True):
         sim.zero_force_energy()
         sim.eam_alloy_force(rcut=4.95, trigger_thermo_state=True, mixed=mixed)
