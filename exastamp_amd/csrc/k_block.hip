@@ -72,6 +72,25 @@ __device__ __forceinline__ bool wrap_axis(int c, int n, int periodic, int& w, in
   return true;
 }
 
+// Staged halo records in LDS as two 16-byte arrays, {x, y} and {z, w}: a random
+// 16-byte read then starts on any of 16 bank groups instead of the 8 that 32-byte
+// records allow, which halves the expected bank-conflict depth of a gather.
+struct LRec {
+  double2* xy;
+  double2* zw;
+  __device__ __forceinline__ Pos4 get(int c) const
+  {
+    const double2 a = xy[c];
+    const double2 b = zw[c];
+    return {a.x, a.y, b.x, b.y};
+  }
+  __device__ __forceinline__ void put(int c, const Pos4& p) const
+  {
+    xy[c] = make_double2(p.x, p.y);
+    zw[c] = make_double2(p.z, p.w);
+  }
+};
+
 // Halo table of block b in LDS (hc[0..nh)), filled by all threads; returns the
 // number of halo atoms.  Ends with __syncthreads().
 __device__ int build_halo(const BlockGeom& g, const BlockShape& s, const int* __restrict__ cell_off,
@@ -92,13 +111,23 @@ __device__ int build_halo(const BlockGeom& g, const BlockShape& s, const int* __
     hc[h] = e;
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    int acc = 0;
-    for (int h = 0; h < nh; h++) {
-      hc[h].lstart = acc;
-      acc += hc[h].count;
+  // exclusive prefix sum of the counts in the first wave, 64 cells at a time
+  if (threadIdx.x < 64) {
+    const int lane = threadIdx.x;
+    int carry = 0;
+    for (int base = 0; base < nh; base += 64) {
+      const int h = base + lane;
+      const int cnt = (h < nh) ? hc[h].count : 0;
+      int v = cnt;
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) {
+        const int t = __shfl_up(v, off);
+        if (lane >= off) v += t;
+      }
+      if (h < nh) hc[h].lstart = carry + v - cnt;
+      carry += __shfl(v, 63);
     }
-    *total_out = acc;
+    if (lane == 0) *total_out = carry;
   }
   __syncthreads();
   return *total_out;
@@ -118,9 +147,9 @@ __device__ __forceinline__ int find_cell(const HaloCell* hc, int nh, int c)
   return lo;
 }
 
-// Stage halo atoms: rec[c] = position (+ image shift) and w; type when requested.
+// Stage halo atoms: record c = position (+ image shift) and w; type when requested.
 __device__ void stage(const BlockShape& s, const HaloCell* hc, int total, const Pos4* __restrict__ pos,
-                      const double* L, const uint8_t* __restrict__ type, Pos4* rec, uint8_t* ltype)
+                      const double* L, const uint8_t* __restrict__ type, const LRec& rec, uint8_t* ltype)
 {
   const int nh = s.hx * s.hy * s.hz;
   for (int c = threadIdx.x; c < total; c += blockDim.x) {
@@ -134,7 +163,7 @@ __device__ void stage(const BlockShape& s, const HaloCell* hc, int total, const 
       p.y = xsmd_image(p.y, xsmd_nbr_img_y(code), L[1]);
       p.z = xsmd_image(p.z, xsmd_nbr_img_z(code), L[2]);
     }
-    rec[c] = p;
+    rec.put(c, p);
     if (ltype) ltype[c] = type[j];
   }
   __syncthreads();
@@ -175,7 +204,7 @@ __device__ __forceinline__ int64_t lblock()
 }
 
 struct LdsLayout {
-  Pos4* rec;
+  LRec rec;
   HaloCell* hc;
   uint8_t* ltype;
   int* total;
@@ -185,7 +214,8 @@ __device__ __forceinline__ LdsLayout lds_layout(int cap, int nh_max, bool with_t
 {
   extern __shared__ __align__(32) unsigned char smem[];
   LdsLayout l;
-  l.rec = reinterpret_cast<Pos4*>(smem);
+  l.rec.xy = reinterpret_cast<double2*>(smem);
+  l.rec.zw = reinterpret_cast<double2*>(smem + (size_t)cap * sizeof(double2));
   l.hc = reinterpret_cast<HaloCell*>(smem + (size_t)cap * sizeof(Pos4));
   l.total = reinterpret_cast<int*>(l.hc + nh_max);
   l.ltype = with_type ? reinterpret_cast<uint8_t*>(l.total + 4) : nullptr;
@@ -214,7 +244,7 @@ __global__ void __launch_bounds__(256) k_block_build(BlockGeom g, const int* __r
   for (int q = threadIdx.x;; q += blockDim.x) {
     Central a;
     if (!central_atom(s, l.hc, q, a)) break;
-    const Pos4 pi = l.rec[a.local];
+    const Pos4 pi = l.rec.get(a.local);
     const int ix = a.h % s.hx, iy = (a.h / s.hx) % s.hy, iz = a.h / (s.hx * s.hy);
     int cnt = 0;
     for (int dz = -1; dz <= 1; dz++)
@@ -223,7 +253,7 @@ __global__ void __launch_bounds__(256) k_block_build(BlockGeom g, const int* __r
           const HaloCell e = l.hc[((iz + dz) * s.hy + (iy + dy)) * s.hx + (ix + dx)];
           for (int c = e.lstart; c < e.lstart + e.count; c++) {
             if (c == a.local) continue;
-            const Pos4 pj = l.rec[c];
+            const Pos4 pj = l.rec.get(c);
             if (xsmd_d2(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z) < rl2) {
               if (cnt < nmax) nbr16[(int64_t)cnt * npad + a.global] = (uint16_t)c;
               cnt++;
@@ -331,9 +361,10 @@ struct LNb {
 
 // entries k0, k0+2, ..., k0+2(U-1) of atom gi (this lane's parity of the list)
 template <int U>
-__device__ __forceinline__ void lfetch(const BlockArgs& a, const Pos4* rec, int64_t gi, int self, int k0, int cnt,
+__device__ __forceinline__ void lfetch(const BlockArgs& a, const LRec& rec, int64_t gi, int self, int k0, int cnt,
                                        const Pos4& pi, LNb (&nb)[U])
 {
+  // entries past this atom's count become the atom itself (a masked pair at d2 = 0)
   int c[U];
 #pragma unroll
   for (int u = 0; u < U; u++) {
@@ -342,7 +373,7 @@ __device__ __forceinline__ void lfetch(const BlockArgs& a, const Pos4* rec, int6
   }
 #pragma unroll
   for (int u = 0; u < U; u++) {
-    const Pos4 pj = rec[c[u]];
+    const Pos4 pj = rec.get(c[u]);
     nb[u].x = pj.x - pi.x;
     nb[u].y = pj.y - pi.y;
     nb[u].z = pj.z - pi.z;
@@ -378,7 +409,7 @@ __global__ void __launch_bounds__(kPassThreads, kPassWaves<false>) k_block_eam_r
   for (int q = pass_slot();; q += kPassAtoms) {
     Central ca;
     if (!central_atom(s, l.hc, q, ca)) break;
-    const Pos4 pi = l.rec[ca.local];
+    const Pos4 pi = l.rec.get(ca.local);
     const int ti = MULTI ? l.ltype[ca.local] : 0;
     const int cnt = a.nnb[ca.global];
     double rho = 0.0;
@@ -423,7 +454,7 @@ __global__ void __launch_bounds__(kPassThreads, kPassWaves<VIRIAL>) k_block_eam_
   for (int q = pass_slot();; q += kPassAtoms) {
     Central ca;
     if (!central_atom(s, l.hc, q, ca)) break;
-    const Pos4 pi = l.rec[ca.local];
+    const Pos4 pi = l.rec.get(ca.local);
     const int ti = MULTI ? l.ltype[ca.local] : 0;
     const Real fpi = (Real)pi.w;
     const int cnt = a.nnb[ca.global];
