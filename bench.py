@@ -86,6 +86,7 @@ def run_form(sim, cfg, tab, mixed, steps, warmup, dt, rc, skin):
     sim.ctx.xs_md_run(_lib.XS_POT_EAM, steps, dt, rc, skin, flags, 0, None, C.byref(st))
     sim.ctx.xs_synchronize()
     t1 = time.perf_counter()
+    pass_info = sim.eam_pass_info()
     ms = np.zeros(len(_lib.PROF_CLASSES))
     cnt = np.zeros(len(_lib.PROF_CLASSES), dtype=np.int64)
     sim.ctx.xs_profile_get(_lib.dptr(ms), _lib.ptr(cnt, C.c_int64))
@@ -101,8 +102,27 @@ def run_form(sim, cfg, tab, mixed, steps, warmup, dt, rc, skin):
         "value": n * steps / sec, "seconds": sec, "ms_per_step": 1e3 * sec / steps,
         "rebuilds": int(st.rebuilds), "kernels": kern,
         "drift_ev_per_atom_per_ps": float(drift), "T0": th0.temperature, "T_end": thK.temperature,
-        "warmup_rebuilds": int(st_w.rebuilds),
+        "warmup_rebuilds": int(st_w.rebuilds), "pass_info": pass_info,
     }
+
+
+def force_kernel_name(list_info, pass_info):
+    if list_info["kind"] != "block":
+        return "k_eam_force<float, false, false, false, true>"
+    if pass_info["force_table_rows"] > 0:
+        return "k_tab_eam_force<false, false, true>"
+    return "k_block_eam_force<float, false, false, false, true>"
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` on this workload from the committed counter
+    summary (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated), or None."""
+    for f in sorted((ROOT / "profiles").glob("*.md")):
+        for line in f.read_text().splitlines():
+            parts = line.strip().strip("`").split("\t")
+            if len(parts) == 3 and parts[0] == "TRAFFIC" and parts[1] == kernel:
+                return float(parts[2])
+    return None
 
 
 def force_error(sim, rc):
@@ -281,14 +301,14 @@ def main():
 
     # roofline of the dominant kernel (EAM force pass, mixed)
     kf = mixed["kernels"]["eam_force"]
+    kname = force_kernel_name(list_info, mixed["pass_info"])
     avg_s = kf["ms_avg"] * 1e-3
     gbs = FORCE_BYTES_PER_ATOM * cfg.n / avg_s / 1e9
     tflops = (FORCE_FLOPS_PER_PAIR * pairs_per_atom + FORCE_FLOPS_PER_ATOM) * cfg.n / avg_s / 1e12
     roof = {
-        "kernel": ("k_block_eam_force<float>" if list_info["kind"] == "block" else "k_eam_force<float>")
-                  + " (EAM force pass + force_to_accel + half kick)",
+        "kernel": kname + " (EAM force pass + force_to_accel + half kick)",
         "bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
-        "traffic": None,
+        "traffic": measured_traffic(kname),
         "bytes_per_atom": FORCE_BYTES_PER_ATOM, "avg_launch_ms": kf["ms_avg"],
         "valu": {"achieved_tflops": tflops, "peak_tflops": FP32_PEAK_TFLOPS, "frac": tflops / FP32_PEAK_TFLOPS,
                  "flops_per_pair": FORCE_FLOPS_PER_PAIR, "pairs_per_atom": pairs_per_atom},

@@ -167,12 +167,19 @@ xs_status xs_ctx_create(int device, xs_ctx** out)
   try {
     XS_HIP(hipSetDevice(device));
     XS_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-    XS_HIP(hipHostMalloc((void**)&c->hscalar, 4 * sizeof(unsigned long long), hipHostMallocDefault));
-    c->dscalar.reserve(4);
+    XS_HIP(hipHostMalloc((void**)&c->hscalar, 8 * sizeof(unsigned long long), hipHostMallocDefault));
+    c->dscalar.reserve(8);
     c->mass_d.reserve(8);
     c->box_d.reserve(1);
     const char* kind = std::getenv("XSMD_LIST");
     c->block_disabled = kind && std::strcmp(kind, "global") == 0;
+    hipDeviceProp_t prop;
+    XS_HIP(hipGetDeviceProperties(&prop, device));
+    c->num_cu = prop.multiProcessorCount;
+    c->lds_max = prop.sharedMemPerBlockOptin > prop.sharedMemPerBlock ? prop.sharedMemPerBlockOptin
+                                                                       : prop.sharedMemPerBlock;
+    if (const char* lt = std::getenv("XSMD_LDS_TABLE")) c->lds_table_enabled = std::strcmp(lt, "1") == 0;
+    if (const char* lr = std::getenv("XSMD_LDS_ROWS")) c->lds_table_rows = std::max(0, std::atoi(lr));
     if (const char* ba = std::getenv("XSMD_BLOCK_ATOMS")) {
       const double v = std::atof(ba);
       if (v >= 1.0 && v <= 4096.0) c->block_atoms = v;
@@ -436,6 +443,14 @@ xs_status xs_list_info(xs_ctx* h, int* kind, int64_t* blocks, int* halo_capacity
   });
 }
 
+xs_status xs_eam_pass_info(xs_ctx* h, int* rho_table_rows, int* force_table_rows)
+{
+  return guarded(h, [&](Ctx& c) {
+    if (rho_table_rows) *rho_table_rows = c.last_rho_rows;
+    if (force_table_rows) *force_table_rows = c.last_force_rows;
+  });
+}
+
 xs_status xs_zero_force_energy(xs_ctx* h)
 {
   return guarded(h, [&](Ctx& c) { launch_zero(c); });
@@ -554,39 +569,55 @@ xs_status xs_md_run(xs_ctx* h, int potential, int nsteps, double dt, double rcut
     }
     const auto t0 = std::chrono::steady_clock::now();
     int64_t rebuilds = 0;
+    const double thr2 = (0.5 * skin) * (0.5 * skin);
 
     if (!cont) {
       build_neighbor_list(c, rcut, skin, potential == XS_POT_EAM);
       rebuilds++;
     }
     auto is_thermo = [&](int step) { return thermo_every > 0 && step % thermo_every == 0; };
-    auto forces = [&](int step, double kick) {
+    // forces of `step` and its closing kick; with want_drift the force launch may
+    // also apply the next step's drift/kick (returns whether it did)
+    auto forces = [&](int step, double kick, bool want_drift) {
       const bool th = is_thermo(step);
       const unsigned f = prec | (th ? (XS_ENERGY | XS_VIRIAL) : 0u);
       if (th && potential == XS_POT_LJ) launch_zero(c);
       ForceEpilogue epi;
       epi.fused = true;
       epi.kick_dt = kick;
+      epi.drift = want_drift && !th;
+      epi.drift_dt = dt;
+      epi.drift_thr2 = thr2;
       compute_forces(c, potential, rcut, f, epi);
       if (th) launch_thermo(c, thermo_out + 27 * (size_t)(step / thermo_every));
+      return c.drift_done;
     };
     // first_iteration: forces at t = 0, no kick
-    if (!cont) forces(0, 0.0);
-    const double thr2 = (0.5 * skin) * (0.5 * skin);
+    bool drifted = false;
+    if (!cont) drifted = forces(0, 0.0, nsteps >= 1);
     for (int step = 1; step <= nsteps; step++) {
-      XS_HIP(hipMemsetAsync(c.dscalar.p + 1, 0, sizeof(unsigned long long), c.stream));
-      launch_drift_kick(c, dt);
-      XS_HIP(hipMemcpyAsync(c.hscalar + 1, c.dscalar.p + 1, sizeof(unsigned long long), hipMemcpyDeviceToHost,
-                            c.stream));
-      XS_HIP(hipStreamSynchronize(c.stream));
+      bool over;
+      if (!drifted) {
+        XS_HIP(hipMemsetAsync(c.dscalar.p + 1, 0, sizeof(unsigned long long), c.stream));
+        launch_drift_kick(c, dt);
+        XS_HIP(hipMemcpyAsync(c.hscalar + 1, c.dscalar.p + 1, sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                              c.stream));
+        XS_HIP(hipStreamSynchronize(c.stream));
+        double d2;
+        std::memcpy(&d2, &c.hscalar[1], sizeof(double));
+        over = d2 > thr2;
+      } else {  // the force pass of the previous step moved the atoms and tested them
+        XS_HIP(hipMemcpyAsync(c.hscalar + 5, c.dscalar.p + 5, sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                              c.stream));
+        XS_HIP(hipStreamSynchronize(c.stream));
+        over = c.hscalar[5] != 0;
+      }
       if (c.prof.on) c.prof.flush();
-      double d2;
-      std::memcpy(&d2, &c.hscalar[1], sizeof(double));
-      if (d2 > thr2) {
+      if (over) {
         build_neighbor_list(c, rcut, skin, potential == XS_POT_EAM);
         rebuilds++;
       }
-      forces(step, 0.5 * dt);
+      drifted = forces(step, 0.5 * dt, step < nsteps);
     }
     XS_HIP(hipStreamSynchronize(c.stream));
     const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();

@@ -16,6 +16,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 
 namespace xsmd {
 namespace {
@@ -222,14 +223,47 @@ __device__ __forceinline__ LdsLayout lds_layout(int cap, int nh_max, bool with_t
   return l;
 }
 
+// Two lanes per central atom: the two halves of a wave take the same 32 atoms
+// (list build: alternate candidates; EAM passes: even and odd list entries).
+constexpr int kPassThreads = 512;
+constexpr int kPassAtoms = kPassThreads / 2;
+
+// sum of v over the two halves of the wave (lanes l and l ^ 32)
+__device__ __forceinline__ double half_sum(double v) { return v + __shfl_xor(v, 32); }
+
+// this lane's atom slot and which half of the wave it is in
+__device__ __forceinline__ int pass_slot() { return (threadIdx.x >> 6) * 32 + (threadIdx.x & 31); }
+__device__ __forceinline__ int pass_half() { return (threadIdx.x >> 5) & 1; }
+
 // ---------------------------------------------------------------------------
 // list build
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_block_build(BlockGeom g, const int* __restrict__ cell_off,
-                                                     const Pos4* __restrict__ pos, const BoxDev* __restrict__ box,
-                                                     double rl2, uint16_t* __restrict__ nbr16, int* __restrict__ nnb,
-                                                     int nmax, int64_t npad, int cap, int nh_max,
-                                                     unsigned long long* __restrict__ flags)
+// Squared distance from p to the box of the halo cell at unwrapped grid coordinate
+// (gx, gy, gz), widened by a margin that covers the rounding of cell assignment.
+// Along a non-periodic axis the edge cells also hold the atoms clamped into them,
+// so their boxes extend to infinity there.
+__device__ __forceinline__ double cell_box_d2(const BlockGeom& g, const BoxDev& b, int gx, int gy, int gz,
+                                              const Pos4& p)
+{
+  const double ux = xsmd_cell_gap(p.x, b.lo[0], b.L[0], g.nx, gx, g.px);
+  const double uy = xsmd_cell_gap(p.y, b.lo[1], b.L[1], g.ny, gy, g.py);
+  const double uz = xsmd_cell_gap(p.z, b.lo[2], b.L[2], g.nz, gz, g.pz);
+  return xsmd_d2(ux, uy, uz);
+}
+
+// Block list: stencil order dz, dy, dx, in-cell order ascending (the order of the
+// global build).  Two lanes per atom test alternate candidates of a cell and place
+// their hits in candidate order; a neighbour cell whose box lies beyond rl is
+// skipped (its atoms would all fail the exact test), unless some atom sits outside
+// its cell's box (nocull, set by the cell-key kernel).
+__global__ void __launch_bounds__(kPassThreads) k_block_build(BlockGeom g, const int* __restrict__ cell_off,
+                                                              const Pos4* __restrict__ pos,
+                                                              const BoxDev* __restrict__ box, double rl2,
+                                                              uint16_t* __restrict__ nbr16, int* __restrict__ nnb,
+                                                              int nmax, int64_t npad, int cap, int nh_max,
+                                                              const unsigned long long* __restrict__ nocull,
+                                                              int* __restrict__ brick_rows,
+                                                              unsigned long long* __restrict__ flags)
 {
   const int64_t b = lblock();
   if (b >= g.nblocks) return;
@@ -240,8 +274,12 @@ __global__ void __launch_bounds__(256) k_block_build(BlockGeom g, const int* __r
     if (threadIdx.x == 0) atomicMax(flags + 1, (unsigned long long)total);
     return;
   }
-  stage(s, l.hc, total, pos, box[s.r].L, nullptr, l.rec, nullptr);
-  for (int q = threadIdx.x;; q += blockDim.x) {
+  const BoxDev bx = box[s.r];
+  const bool cull = *nocull == 0ull;
+  if (threadIdx.x == 0) l.total[1] = 0;  // longest list of the brick (made visible by stage's barrier)
+  stage(s, l.hc, total, pos, bx.L, nullptr, l.rec, nullptr);
+  const int half = pass_half();
+  for (int q = pass_slot();; q += kPassAtoms) {
     Central a;
     if (!central_atom(s, l.hc, q, a)) break;
     const Pos4 pi = l.rec.get(a.local);
@@ -251,18 +289,31 @@ __global__ void __launch_bounds__(256) k_block_build(BlockGeom g, const int* __r
       for (int dy = -1; dy <= 1; dy++)
         for (int dx = -1; dx <= 1; dx++) {
           const HaloCell e = l.hc[((iz + dz) * s.hy + (iy + dy)) * s.hx + (ix + dx)];
-          for (int c = e.lstart; c < e.lstart + e.count; c++) {
-            if (c == a.local) continue;
-            const Pos4 pj = l.rec.get(c);
-            if (xsmd_d2(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z) < rl2) {
-              if (cnt < nmax) nbr16[(int64_t)cnt * npad + a.global] = (uint16_t)c;
-              cnt++;
+          if (e.count == 0) continue;
+          if (cull && cell_box_d2(g, bx, s.cx0 - 1 + ix + dx, s.cy0 - 1 + iy + dy, s.cz0 - 1 + iz + dz, pi) >= rl2)
+            continue;
+          for (int off = 0; off < e.count; off += 2) {
+            const int k = off + half;
+            const int c = e.lstart + k;
+            bool hit = false;
+            if (k < e.count && c != a.local) {
+              const Pos4 pj = l.rec.get(c);
+              hit = xsmd_d2(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z) < rl2;
             }
+            const int other = __shfl_xor((int)hit, 32);
+            const int slot = cnt + (half ? other : 0);
+            if (hit && slot < nmax) nbr16[(int64_t)slot * npad + a.global] = (uint16_t)c;
+            cnt += (int)hit + other;
           }
         }
-    nnb[a.global] = cnt;
-    if (cnt > nmax) atomicMax(flags, (unsigned long long)cnt);
+    if (half == 0) {
+      nnb[a.global] = cnt;
+      atomicMax(l.total + 1, cnt);
+      if (cnt > nmax) atomicMax(flags, (unsigned long long)cnt);
+    }
   }
+  __syncthreads();
+  if (threadIdx.x == 0) brick_rows[b] = l.total[1];
 }
 
 // block-local entries -> global (index, image) entries of the same list
@@ -344,8 +395,6 @@ constexpr int kBBatch = 4;
 // even and odd list entries of the same 32 atoms and combine their sums with one
 // cross-half exchange, doubling the waves that hide the table-gather latency for
 // the LDS a brick needs.
-constexpr int kPassThreads = 512;
-constexpr int kPassAtoms = kPassThreads / 2;
 // Minimum waves per SIMD the register allocation must allow: three 8-wave
 // workgroups per CU (what the LDS of a ~200-atom brick permits) need 6; the
 // virial form of the force pass carries 9 more sums and is held to 4.
@@ -384,12 +433,6 @@ __device__ __forceinline__ void lfetch(const BlockArgs& a, const LRec& rec, int6
   }
 }
 
-// sum of v over the two halves of the wave (lanes l and l ^ 32)
-__device__ __forceinline__ double half_sum(double v) { return v + __shfl_xor(v, 32); }
-
-// this lane's atom slot and list parity
-__device__ __forceinline__ int pass_slot() { return (threadIdx.x >> 6) * 32 + (threadIdx.x & 31); }
-__device__ __forceinline__ int pass_half() { return (threadIdx.x >> 5) & 1; }
 
 // rho_i, then (EMB) F(rho_i) and F'(rho_i) for the central atoms of one brick
 template <class Real, bool MULTI, bool EMB, bool ENERGY, bool SET_EP>
@@ -439,10 +482,12 @@ __global__ void __launch_bounds__(kPassThreads, kPassWaves<false>) k_block_eam_r
   }
 }
 
-template <class Real, bool MULTI, bool ENERGY, bool VIRIAL, bool FUSED>
+template <class Real, bool MULTI, bool ENERGY, bool VIRIAL, bool FUSED, bool DRIFT>
 __global__ void __launch_bounds__(kPassThreads, kPassWaves<VIRIAL>) k_block_eam_force(BlockArgs a, EamTabDev<Real> t,
-                                                                  const Real* __restrict__ pair, pm::Outputs o)
+                                                                  const Real* __restrict__ pair, pm::Outputs o,
+                                                                  pm::Drift dr)
 {
+  static_assert(!DRIFT || (FUSED && !ENERGY && !VIRIAL), "the drift epilogue is the plain MD-loop step");
   const int64_t b = lblock();
   if (b >= a.g.nblocks) return;
   LdsLayout l = lds_layout(a.cap, a.g.nh_max, MULTI);
@@ -485,8 +530,589 @@ __global__ void __launch_bounds__(kPassThreads, kPassWaves<VIRIAL>) k_block_eam_
 #pragma unroll
       for (int k = 0; k < 9; k++) v[k] = half_sum(v[k]);
     }
-    if (half == 0) pm::write_outputs<ENERGY, VIRIAL, FUSED>(o, ca.global, fx, fy, fz, e, v);
+    if (half == 0) {
+      if constexpr (DRIFT) pm::write_outputs_drift(o, dr, ca.global, pi, fx, fy, fz);
+      else pm::write_outputs<ENERGY, VIRIAL, FUSED>(o, ca.global, fx, fy, fz, e, v);
+    }
   }
+}
+
+// ---------------------------------------------------------------------------
+// Persistent passes with the spline table in LDS (single element, FP32 tables)
+// ---------------------------------------------------------------------------
+// A wave's table gathers touch one cache line per distinct row, so rows read from
+// HBM/L2 keep the texture data path busy for ~50 cycles per gather.  Here one
+// 1024-thread workgroup per CU copies rows [m_lo, m_lo + rows) of the combined
+// table into LDS once per launch and then walks a contiguous run of bricks, so
+// table rows and neighbour records both come from LDS.  A pair whose knot index
+// lies below m_lo (an unusually short pair) reads its row from global memory.
+// Four lanes per central atom take list entries k = h, h + 4, ...; the lanes of
+// one parity h are adjacent within the wave, so list reads stay contiguous.
+constexpr int kTabThreads = 1024;
+constexpr int kTabLanes = 4;
+constexpr int kTabGroup = 64 / kTabLanes;           // atoms per wave
+constexpr int kTabAtoms = kTabThreads / kTabLanes;  // atoms per pass over a brick
+constexpr int kTBatch = 2;
+
+struct TabArgs {
+  BlockArgs a;
+  int m_lo, rows;     // LDS holds table rows m_lo .. m_lo + rows - 1 (the last is nr - 1)
+  int64_t per_wg;     // bricks per workgroup
+  const int* __restrict__ brick_rows;  // longest list of each brick
+  int nmax;           // rows of the list
+};
+
+__device__ __forceinline__ int tab_slot() { return (threadIdx.x >> 6) * kTabGroup + (threadIdx.x & (kTabGroup - 1)); }
+__device__ __forceinline__ int tab_h() { return (threadIdx.x & 63) / kTabGroup; }
+
+// sum over the four lanes of one atom (lanes l, l ^ 16, l ^ 32, l ^ 48)
+__device__ __forceinline__ double group_sum(double v)
+{
+  v += __shfl_xor(v, 16);
+  v += __shfl_xor(v, 32);
+  return v;
+}
+
+struct TabLayout {
+  float* tab;
+  LRec rec;
+  HaloCell* hc;
+  int* total;
+};
+
+__device__ __forceinline__ TabLayout tab_layout(int rows, int row_floats, int cap, int nh_max)
+{
+  extern __shared__ __align__(32) unsigned char smem[];
+  TabLayout l;
+  l.tab = reinterpret_cast<float*>(smem);
+  unsigned char* p = smem + (size_t)rows * row_floats * sizeof(float);
+  l.rec.xy = reinterpret_cast<double2*>(p);
+  l.rec.zw = reinterpret_cast<double2*>(p + (size_t)cap * sizeof(double2));
+  l.hc = reinterpret_cast<HaloCell*>(p + (size_t)cap * sizeof(Pos4));
+  l.total = reinterpret_cast<int*>(l.hc + nh_max);
+  return l;
+}
+
+// list slots of entries k0, k0 + 4, ... of atom gi (the atom itself past the count).
+// The row read is clamped into [0, cnt - 1], so every load is unconditional and in
+// bounds, and the loads of the next batch can stay in flight while this one is used.
+template <int U>
+__device__ __forceinline__ void tlist(const BlockArgs& a, int64_t gi, int self, int k0, int cnt, int (&c)[U])
+{
+#pragma unroll
+  for (int u = 0; u < U; u++) {
+    const int k = k0 + kTabLanes * u;
+    const int kk = max(0, min(k, cnt - 1));
+    const int raw = a.nbr16[(int64_t)kk * a.npad + gi];
+    c[u] = (k < cnt) ? raw : self;
+  }
+}
+
+// neighbour records of list slots c (entries k0, k0 + 4, ...); entries past the
+// count are masked and evaluated at d2 = rc^2, whose knot is in LDS
+template <int U>
+__device__ __forceinline__ void tgather(const BlockArgs& a, const LRec& rec, const int (&c)[U], int k0, int cnt,
+                                        const Pos4& pi, LNb (&nb)[U])
+{
+#pragma unroll
+  for (int u = 0; u < U; u++) {
+    const Pos4 pj = rec.get(c[u]);
+    nb[u].x = pj.x - pi.x;
+    nb[u].y = pj.y - pi.y;
+    nb[u].z = pj.z - pi.z;
+    nb[u].d2 = xsmd_d2(nb[u].x, nb[u].y, nb[u].z);
+    nb[u].w = pj.w;
+    nb[u].c = c[u];
+    nb[u].in = (k0 + kTabLanes * u < cnt) && nb[u].d2 <= a.rc2;
+  }
+}
+
+// value cubic of rho at knot m: LDS row when m >= m_lo, else the HBM row
+__device__ __forceinline__ pm::Q4<float> tab_rho_row(const float* ltab, int m_lo, const float* __restrict__ g, int m)
+{
+  if (m >= m_lo) {
+    const float4 v = reinterpret_cast<const float4*>(ltab)[m - m_lo];
+    return {v.x, v.y, v.z, v.w};
+  }
+  return pm::load4(g + (int64_t)m * 8);
+}
+
+__device__ __forceinline__ void tab_pair_rows(const float* ltab, int m_lo, const float* __restrict__ g, int m,
+                                              pm::Q4<float>& rc, pm::Q4<float>& zc)
+{
+  if (m >= m_lo) {
+    const float4* q = reinterpret_cast<const float4*>(ltab) + (int64_t)(m - m_lo) * 2;
+    const float4 u = q[0];
+    const float4 w = q[1];
+    rc = {u.x, u.y, u.z, u.w};
+    zc = {w.x, w.y, w.z, w.w};
+  } else {
+    rc = pm::load4(g + (int64_t)m * 8);
+    zc = pm::load4(g + (int64_t)m * 8 + 4);
+  }
+}
+
+// ---- per-brick pipeline of the persistent passes ----
+// Global loads of a brick form one round trip: the halo cells of brick b + 1
+// (cell offsets and the brick's longest list) are fetched while brick b computes;
+// at the start of a brick every lane issues its atom's count and list entries and
+// then the halo record loads, all before the first wait; the pair loop itself
+// reads only LDS and registers.
+constexpr int kTabPL = 16;  // list entries per lane held in registers (rows < 4 * kTabPL), two per VGPR
+
+// this thread's halo cell of a brick (threads >= the cell count hold nothing)
+struct HaloNext {
+  int gstart, count, img, rows;
+};
+
+__device__ __forceinline__ void halo_fetch(const BlockGeom& g, const int* __restrict__ cell_off,
+                                           const int* __restrict__ brick_rows, int64_t b, HaloNext& hn)
+{
+  const BlockShape s = block_shape(g, b);
+  const int nh = s.hx * s.hy * s.hz;
+  hn = {0, 0, 0, brick_rows[b]};
+  const int h = threadIdx.x;
+  if (h < nh) {
+    const int ix = h % s.hx, iy = (h / s.hx) % s.hy, iz = h / (s.hx * s.hy);
+    int wx, wy, wz, kx, ky, kz;
+    if (wrap_axis(s.cx0 - 1 + ix, g.nx, g.px, wx, kx) && wrap_axis(s.cy0 - 1 + iy, g.ny, g.py, wy, ky) &&
+        wrap_axis(s.cz0 - 1 + iz, g.nz, g.pz, wz, kz)) {
+      const int64_t c = (int64_t)s.r * g.ncell_rep + ((int64_t)wz * g.ny + wy) * g.nx + wx;
+      hn.gstart = cell_off[c];
+      hn.count = cell_off[c + 1] - hn.gstart;
+      hn.img = (int)(xsmd_nbr_encode(0u, kx, ky, kz) >> XSMD_NBR_INDEX_BITS);
+    }
+  }
+}
+
+// halo table of the brick from the fetched cells, with the exclusive prefix of the
+// counts (as build_halo); ends with __syncthreads()
+__device__ int halo_commit(const BlockShape& s, const HaloNext& hn, HaloCell* hc, int* total_out)
+{
+  const int nh = s.hx * s.hy * s.hz;
+  if ((int)threadIdx.x < nh) hc[threadIdx.x] = HaloCell{hn.gstart, 0, hn.count, hn.img};
+  __syncthreads();
+  if (threadIdx.x < 64) {
+    const int lane = threadIdx.x;
+    int carry = 0;
+    for (int base = 0; base < nh; base += 64) {
+      const int h = base + lane;
+      const int cnt = (h < nh) ? hc[h].count : 0;
+      int v = cnt;
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) {
+        const int t = __shfl_up(v, off);
+        if (lane >= off) v += t;
+      }
+      if (h < nh) hc[h].lstart = carry + v - cnt;
+      carry += __shfl(v, 63);
+    }
+    if (lane == 0) *total_out = carry;
+  }
+  __syncthreads();
+  return *total_out;
+}
+
+// Halo records, at most two per thread (cap <= 2 * kTabThreads): all loads are
+// issued before any record is written.  Ends with __syncthreads().
+__device__ __forceinline__ void stage2(const BlockShape& s, const HaloCell* hc, int total,
+                                       const Pos4* __restrict__ pos, const double* L, const LRec& rec)
+{
+  const int nh = s.hx * s.hy * s.hz;
+  Pos4 p[2];
+  int img[2];
+#pragma unroll
+  for (int r = 0; r < 2; r++) {
+    const int c = threadIdx.x + r * kTabThreads;
+    img[r] = 0;
+    if (c < total) {
+      const HaloCell e = hc[find_cell(hc, nh, c)];
+      p[r] = load_pos(pos, e.gstart + (c - e.lstart));
+      img[r] = e.img;
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < 2; r++) {
+    const int c = threadIdx.x + r * kTabThreads;
+    if (c < total) {
+      Pos4 q = p[r];
+      if (img[r]) {
+        const uint32_t code = (uint32_t)img[r] << XSMD_NBR_INDEX_BITS;
+        q.x = xsmd_image(q.x, xsmd_nbr_img_x(code), L[0]);
+        q.y = xsmd_image(q.y, xsmd_nbr_img_y(code), L[1]);
+        q.z = xsmd_image(q.z, xsmd_nbr_img_z(code), L[2]);
+      }
+      rec.put(c, q);
+    }
+  }
+  __syncthreads();
+}
+
+// list entries h, h + 4, ... of atom gi into registers (entry u in half u & 1 of
+// E[u / 2]), rows < 4 * min(kTabPL, units) (units = the brick's longest list / 4,
+// rounded up); rows past this atom's count hold another atom's data and are masked
+// by the count later
+__device__ __forceinline__ void list_fetch(const BlockArgs& a, int nmax, int64_t gi, int h, int units,
+                                           uint32_t (&E)[kTabPL / 2])
+{
+#pragma unroll
+  for (int u = 0; u < kTabPL; u += 2) {
+    uint32_t lo = 0, hi = 0;
+    if (u < units) lo = a.nbr16[(int64_t)min(h + kTabLanes * u, nmax - 1) * a.npad + gi];
+    if (u + 1 < units) hi = a.nbr16[(int64_t)min(h + kTabLanes * (u + 1), nmax - 1) * a.npad + gi];
+    E[u / 2] = lo | (hi << 16);
+  }
+}
+
+__device__ __forceinline__ int list_entry(const uint32_t (&E)[kTabPL / 2], int u)
+{
+  return (int)((E[u / 2] >> (16 * (u & 1))) & 0xffffu);
+}
+
+// EAM density of central atom ca: entries from E (and, past 4 * kTabPL entries, from HBM)
+template <int U>
+__device__ __forceinline__ double tab_rho_entries(const TabArgs& ta, const EamTabDev<float>& t,
+                                                  const float* __restrict__ pair, const float* ltab, const LRec& rec,
+                                                  const Central& ca, const Pos4& pi, int h, int cnt,
+                                                  const uint32_t (&E)[kTabPL / 2])
+{
+  const BlockArgs& a = ta.a;
+  double rho = 0.0;
+#pragma unroll 1
+  for (int u0 = 0; u0 < kTabPL; u0 += U) {
+    if (h + kTabLanes * u0 >= cnt) break;
+    int c[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) c[u] = (h + kTabLanes * (u0 + u) < cnt) ? list_entry(E, u0 + u) : ca.local;
+    LNb nb[U];
+    tgather<U>(a, rec, c, h + kTabLanes * u0, cnt, pi, nb);
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      float r, rinv, p;
+      int m;
+      pm::dist((float)(nb[u].in ? nb[u].d2 : a.rc2), r, rinv);
+      pm::knot_r(t, r, m, p);
+      const float val = pm::cubic(tab_rho_row(ltab, ta.m_lo, pair, m), p);
+      rho += nb[u].in ? (double)val : 0.0;
+    }
+  }
+  for (int k0 = h + kTabLanes * kTabPL; k0 < cnt; k0 += kTabLanes * U) {
+    int c[U];
+    tlist<U>(a, ca.global, ca.local, k0, cnt, c);
+    LNb nb[U];
+    tgather<U>(a, rec, c, k0, cnt, pi, nb);
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      float r, rinv, p;
+      int m;
+      pm::dist((float)(nb[u].in ? nb[u].d2 : a.rc2), r, rinv);
+      pm::knot_r(t, r, m, p);
+      const float val = pm::cubic(tab_rho_row(ltab, ta.m_lo, pair, m), p);
+      rho += nb[u].in ? (double)val : 0.0;
+    }
+  }
+  return rho;
+}
+
+struct ForceSums {
+  double fx, fy, fz, e;
+  double v[9];
+};
+
+template <int U, bool ENERGY, bool VIRIAL>
+__device__ __forceinline__ void tab_force_batch(const TabArgs& ta, const EamTabDev<float>& t,
+                                                const float* __restrict__ pair, const float* ltab, float fpi,
+                                                const LNb (&nb)[U], ForceSums& fs)
+{
+#pragma unroll
+  for (int u = 0; u < U; u++) {
+    float r, rinv, p, fp_r, phi;
+    int m;
+    pm::dist((float)(nb[u].in ? nb[u].d2 : ta.a.rc2), r, rinv);
+    pm::knot_r(t, r, m, p);
+    pm::Q4<float> rc, zc;
+    tab_pair_rows(ltab, ta.m_lo, pair, m, rc, zc);
+    pm::eam_pair_rows(t, rc, zc, p, rinv, fpi, (float)nb[u].w, fp_r, phi);
+    const double fpair = nb[u].in ? (double)fp_r : 0.0;
+    const double fe0 = nb[u].x * fpair, fe1 = nb[u].y * fpair, fe2 = nb[u].z * fpair;
+    fs.fx += fe0;
+    fs.fy += fe1;
+    fs.fz += fe2;
+    if constexpr (ENERGY) fs.e += nb[u].in ? 0.5 * (double)phi : 0.0;
+    if constexpr (VIRIAL) pm::add_virial(fs.v, fe0, fe1, fe2, nb[u].x, nb[u].y, nb[u].z);
+  }
+}
+
+template <int U, bool ENERGY, bool VIRIAL>
+__device__ __forceinline__ void tab_force_entries(const TabArgs& ta, const EamTabDev<float>& t,
+                                                  const float* __restrict__ pair, const float* ltab,
+                                                  const LRec& rec, const Central& ca, const Pos4& pi, int h,
+                                                  int cnt, const uint32_t (&E)[kTabPL / 2], ForceSums& fs)
+{
+  const BlockArgs& a = ta.a;
+  const float fpi = (float)pi.w;
+#pragma unroll 1
+  for (int u0 = 0; u0 < kTabPL; u0 += U) {
+    if (h + kTabLanes * u0 >= cnt) break;
+    int c[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) c[u] = (h + kTabLanes * (u0 + u) < cnt) ? list_entry(E, u0 + u) : ca.local;
+    LNb nb[U];
+    tgather<U>(a, rec, c, h + kTabLanes * u0, cnt, pi, nb);
+    tab_force_batch<U, ENERGY, VIRIAL>(ta, t, pair, ltab, fpi, nb, fs);
+  }
+  for (int k0 = h + kTabLanes * kTabPL; k0 < cnt; k0 += kTabLanes * U) {
+    int c[U];
+    tlist<U>(a, ca.global, ca.local, k0, cnt, c);
+    LNb nb[U];
+    tgather<U>(a, rec, c, k0, cnt, pi, nb);
+    tab_force_batch<U, ENERGY, VIRIAL>(ta, t, pair, ltab, fpi, nb, fs);
+  }
+}
+
+template <bool EMB, bool ENERGY, bool SET_EP>
+__global__ void __launch_bounds__(kTabThreads) k_tab_eam_rho(TabArgs ta, EamTabDev<float> t,
+                                                             const float* __restrict__ pair,
+                                                             double* __restrict__ rho_io, double* __restrict__ pos_w,
+                                                             double* __restrict__ ep)
+{
+  const BlockArgs& a = ta.a;
+  const TabLayout l = tab_layout(ta.rows, 4, a.cap, a.g.nh_max);
+  // rho value cubics: the first half of each combined row
+  for (int i = threadIdx.x; i < ta.rows; i += blockDim.x)
+    reinterpret_cast<float4*>(l.tab)[i] = reinterpret_cast<const float4*>(pair)[(int64_t)(ta.m_lo + i) * 2];
+  const int64_t b0 = lblock() * ta.per_wg;
+  const int64_t b1 = min(b0 + ta.per_wg, a.g.nblocks);
+  const int h = tab_h();
+  HaloNext hn;
+  if (b0 < b1) halo_fetch(a.g, a.cell_off, ta.brick_rows, b0, hn);
+  for (int64_t b = b0; b < b1; b++) {
+    const BlockShape s = block_shape(a.g, b);
+    const int total = halo_commit(s, hn, l.hc, l.total);
+    const int units = (hn.rows + kTabLanes - 1) / kTabLanes;
+    Central ca;
+    const bool have = central_atom(s, l.hc, tab_slot(), ca);
+    int cnt = 0;
+    uint32_t E[kTabPL / 2];
+    if (have) {
+      cnt = a.nnb[ca.global];
+      list_fetch(a, ta.nmax, ca.global, h, units, E);
+    }
+    if (total <= a.cap) stage2(s, l.hc, total, a.pos, a.box[s.r].L, l.rec);  // always (cap = measured maximum)
+    else __syncthreads();
+    if (b + 1 < b1) halo_fetch(a.g, a.cell_off, ta.brick_rows, b + 1, hn);
+    if (total <= a.cap) {
+      for (int q = tab_slot(); have; q += kTabAtoms) {
+        if (q != tab_slot()) {  // atoms past the first pass (bricks of more than kTabAtoms atoms)
+          if (!central_atom(s, l.hc, q, ca)) break;
+          cnt = a.nnb[ca.global];
+          list_fetch(a, ta.nmax, ca.global, h, units, E);
+        }
+        const Pos4 pi = l.rec.get(ca.local);
+        double rho = tab_rho_entries<kTBatch>(ta, t, pair, l.tab, l.rec, ca, pi, h, cnt, E);
+        rho = group_sum(rho);
+        if (h == 0) {
+          rho_io[ca.global] = rho;
+          if constexpr (EMB) {
+            float F, Fp;
+            pm::embed(t, 0, rho, F, Fp);
+            pos_w[4 * ca.global + 3] = (double)Fp;
+            if constexpr (ENERGY) {
+              if constexpr (SET_EP) ep[ca.global] = (double)F;
+              else ep[ca.global] += (double)F;
+            }
+          }
+        }
+      }
+    }
+    __syncthreads();  // this brick's LDS is read by every lane before the next is staged
+  }
+}
+
+template <bool ENERGY, bool VIRIAL, bool FUSED, bool DRIFT>
+__global__ void __launch_bounds__(kTabThreads) k_tab_eam_force(TabArgs ta, EamTabDev<float> t,
+                                                               const float* __restrict__ pair, pm::Outputs o,
+                                                               pm::Drift dr)
+{
+  static_assert(!DRIFT || (FUSED && !ENERGY && !VIRIAL), "the drift epilogue is the plain MD-loop step");
+  const BlockArgs& a = ta.a;
+  const TabLayout l = tab_layout(ta.rows, 8, a.cap, a.g.nh_max);
+  for (int i = threadIdx.x; i < 2 * ta.rows; i += blockDim.x)
+    reinterpret_cast<float4*>(l.tab)[i] = reinterpret_cast<const float4*>(pair)[(int64_t)ta.m_lo * 2 + i];
+  const int64_t b0 = lblock() * ta.per_wg;
+  const int64_t b1 = min(b0 + ta.per_wg, a.g.nblocks);
+  const int h = tab_h();
+  HaloNext hn;
+  if (b0 < b1) halo_fetch(a.g, a.cell_off, ta.brick_rows, b0, hn);
+  for (int64_t b = b0; b < b1; b++) {
+    const BlockShape s = block_shape(a.g, b);
+    const int total = halo_commit(s, hn, l.hc, l.total);
+    const int units = (hn.rows + kTabLanes - 1) / kTabLanes;
+    Central ca;
+    const bool have = central_atom(s, l.hc, tab_slot(), ca);
+    int cnt = 0;
+    uint32_t E[kTabPL / 2];
+    if (have) {
+      cnt = a.nnb[ca.global];
+      list_fetch(a, ta.nmax, ca.global, h, units, E);
+    }
+    if (total <= a.cap) stage2(s, l.hc, total, a.pos, a.box[s.r].L, l.rec);  // always (cap = measured maximum)
+    else __syncthreads();
+    if (b + 1 < b1) halo_fetch(a.g, a.cell_off, ta.brick_rows, b + 1, hn);
+    if (total <= a.cap) {
+      for (int q = tab_slot(); have; q += kTabAtoms) {
+        if (q != tab_slot()) {
+          if (!central_atom(s, l.hc, q, ca)) break;
+          cnt = a.nnb[ca.global];
+          list_fetch(a, ta.nmax, ca.global, h, units, E);
+        }
+        const Pos4 pi = l.rec.get(ca.local);
+        ForceSums fs{0.0, 0.0, 0.0, 0.0, {0, 0, 0, 0, 0, 0, 0, 0, 0}};
+        tab_force_entries<kTBatch, ENERGY, VIRIAL>(ta, t, pair, l.tab, l.rec, ca, pi, h, cnt, E, fs);
+        fs.fx = group_sum(fs.fx);
+        fs.fy = group_sum(fs.fy);
+        fs.fz = group_sum(fs.fz);
+        if constexpr (ENERGY) fs.e = group_sum(fs.e);
+        if constexpr (VIRIAL) {
+#pragma unroll
+          for (int k = 0; k < 9; k++) fs.v[k] = group_sum(fs.v[k]);
+        }
+        if (h == 0) {
+          if constexpr (DRIFT) pm::write_outputs_drift(o, dr, ca.global, pi, fs.fx, fs.fy, fs.fz);
+          else pm::write_outputs<ENERGY, VIRIAL, FUSED>(o, ca.global, fs.fx, fs.fy, fs.fz, fs.e, fs.v);
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Fused next-step drift of the MD loop (ForceEpilogue::drift) for a force launch
+// with the plain MD-loop epilogue; false when it does not apply.
+bool drift_setup(Ctx& c, const ForceEpilogue& epi, unsigned flags, pm::Drift& dr)
+{
+  if (!(epi.fused && epi.drift) || (flags & (XS_ENERGY | XS_VIRIAL))) return false;
+  c.pos2.reserve(c.npad);
+  dr.pos_out = c.pos2.p;
+  dr.rrx = c.rrx.p;
+  dr.rry = c.rry.p;
+  dr.rrz = c.rrz.p;
+  dr.dt = epi.drift_dt;
+  dr.dt2 = epi.drift_dt * epi.drift_dt * 0.5;
+  dr.hdt = epi.drift_dt * 0.5;
+  dr.thr2 = epi.drift_thr2;
+  dr.over = c.dscalar.p + 5;
+  XS_HIP(hipMemsetAsync(c.dscalar.p + 5, 0, sizeof(unsigned long long), c.stream));
+  return true;
+}
+
+// after the launch: the next step's positions (pos2) become current
+void drift_commit(Ctx& c)
+{
+  c.pos.swap(c.pos2);
+  c.rho_ckpt = false;
+  c.drift_done = true;
+}
+
+// LDS plan of the persistent passes: rows of the table that fit beside the halo.
+struct TabPlan {
+  bool ok;
+  int m_lo, rows;
+  size_t bytes;
+};
+
+TabPlan tab_plan(const Ctx& c, int row_floats)
+{
+  TabPlan tp{false, 0, 0, 0};
+  if (!c.lds_table_enabled || c.eam.nel != 1 || c.num_cu <= 0 || c.bcap > 2 * kTabThreads) return tp;
+  const size_t halo = (size_t)c.bcap * sizeof(Pos4) + (size_t)c.bgeom.nh_max * sizeof(HaloCell) + 4 * sizeof(int);
+  if (halo >= c.lds_max) return tp;
+  const size_t row_bytes = (size_t)row_floats * sizeof(float);
+  const int nr = c.eam.nr;
+  int rows = (int)std::min<size_t>((size_t)nr, (c.lds_max - halo) / row_bytes);
+  if (c.lds_table_rows > 0) rows = std::min(rows, c.lds_table_rows);
+  // short of a quarter of the table, the HBM fallback would carry too many pairs
+  if (rows < std::max(1, nr / 4)) return tp;
+  tp.ok = true;
+  tp.rows = rows;
+  tp.m_lo = nr - rows;
+  tp.bytes = (size_t)rows * row_bytes + halo;
+  return tp;
+}
+
+template <class K>
+void set_lds_limit(K kernel, size_t bytes)
+{
+  XS_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)bytes));
+}
+
+// Persistent LDS-table passes; false when the table does not fit (caller falls back).
+bool tab_eam_dispatch(Ctx& c, const BlockArgs& a, bool do_rho, bool do_emb, bool do_force, unsigned flags,
+                      const ForceEpilogue& epi)
+{
+  const TabPlan pr = tab_plan(c, 4), pf = tab_plan(c, 8);
+  if ((do_rho && !pr.ok) || (do_force && !pf.ok)) return false;
+  if (do_rho) c.last_rho_rows = pr.rows;
+  if (do_force) c.last_force_rows = pf.rows;
+  const EamTabDev<float> t = make_eam_tab<float>(c);
+  const float* pair = eam_pair_table<float>(c);
+  const unsigned g = (unsigned)std::max<int64_t>(8, std::min<int64_t>((c.num_cu + 7) / 8 * 8,
+                                                                         (a.g.nblocks + 7) / 8 * 8));
+  TabArgs ta;
+  ta.a = a;
+  ta.brick_rows = c.brick_rows.p;
+  ta.nmax = c.nmax;
+  ta.per_wg = (a.g.nblocks + g - 1) / g;
+  const bool energy = flags & XS_ENERGY;
+  double* pos_w = reinterpret_cast<double*>(c.pos.p);
+  if (do_rho) {
+    ta.m_lo = pr.m_lo;
+    ta.rows = pr.rows;
+    ProfScope prof(c.prof, c.stream, XS_PROF_EAM_RHO);
+#define XS_TRHO(EMB, EN, SET)                                                                               \
+  do {                                                                                                      \
+    set_lds_limit(k_tab_eam_rho<EMB, EN, SET>, pr.bytes);                                                   \
+    k_tab_eam_rho<EMB, EN, SET><<<g, kTabThreads, pr.bytes, c.stream>>>(ta, t, pair, c.rho.p, pos_w, c.ep.p); \
+  } while (0)
+    if (!do_emb) XS_TRHO(false, false, false);
+    else if (energy && epi.fused) XS_TRHO(true, true, true);
+    else if (energy) XS_TRHO(true, true, false);
+    else XS_TRHO(true, false, false);
+#undef XS_TRHO
+    XS_HIP(hipGetLastError());
+  }
+  if (do_force) {
+    ta.m_lo = pf.m_lo;
+    ta.rows = pf.rows;
+    const pm::Outputs o = make_outputs(c, epi);
+    const bool vir = flags & XS_VIRIAL;
+    pm::Drift dr{};
+    const bool drift = drift_setup(c, epi, flags, dr);
+    ProfScope prof(c.prof, c.stream, XS_PROF_EAM_FORCE);
+#define XS_TF(EN, VI, FU, DR)                                                                    \
+  do {                                                                                           \
+    set_lds_limit(k_tab_eam_force<EN, VI, FU, DR>, pf.bytes);                                    \
+    k_tab_eam_force<EN, VI, FU, DR><<<g, kTabThreads, pf.bytes, c.stream>>>(ta, t, pair, o, dr); \
+  } while (0)
+    if (drift) {
+      XS_TF(false, false, true, true);
+    } else if (epi.fused) {
+      if (energy && vir) XS_TF(true, true, true, false);
+      else if (energy) XS_TF(true, false, true, false);
+      else if (vir) XS_TF(false, true, true, false);
+      else XS_TF(false, false, true, false);
+    } else {
+      if (energy && vir) XS_TF(true, true, false, false);
+      else if (energy) XS_TF(true, false, false, false);
+      else if (vir) XS_TF(false, true, false, false);
+      else XS_TF(false, false, false, false);
+    }
+#undef XS_TF
+    XS_HIP(hipGetLastError());
+    if (drift) drift_commit(c);
+  }
+  return true;
 }
 
 template <class Real, bool MULTI>
@@ -506,6 +1132,11 @@ void block_eam_dispatch(Ctx& c, double rcut, bool do_rho, bool do_emb, bool do_f
   a.npad = c.npad;
   a.cap = c.bcap;
   a.rc2 = rcut * rcut;
+  if constexpr (std::is_same_v<Real, float> && !MULTI) {
+    if (tab_eam_dispatch(c, a, do_rho, do_emb, do_force, flags, epi)) return;
+  }
+  if (do_rho) c.last_rho_rows = 0;
+  if (do_force) c.last_force_rows = 0;
   const size_t lds = block_lds_bytes(c, MULTI);
   const unsigned g = block_grid(c);
   const bool energy = flags & XS_ENERGY;
@@ -524,22 +1155,27 @@ void block_eam_dispatch(Ctx& c, double rcut, bool do_rho, bool do_emb, bool do_f
   if (do_force) {
     const pm::Outputs o = make_outputs(c, epi);
     const bool vir = flags & XS_VIRIAL;
+    pm::Drift dr{};
+    const bool drift = drift_setup(c, epi, flags, dr);
     ProfScope prof(c.prof, c.stream, XS_PROF_EAM_FORCE);
-#define XS_BF(EN, VI, FU) \
-  k_block_eam_force<Real, MULTI, EN, VI, FU><<<g, kPassThreads, lds, c.stream>>>(a, t, pair, o)
-    if (epi.fused) {
-      if (energy && vir) XS_BF(true, true, true);
-      else if (energy) XS_BF(true, false, true);
-      else if (vir) XS_BF(false, true, true);
-      else XS_BF(false, false, true);
+#define XS_BF(EN, VI, FU, DR) \
+  k_block_eam_force<Real, MULTI, EN, VI, FU, DR><<<g, kPassThreads, lds, c.stream>>>(a, t, pair, o, dr)
+    if (drift) {
+      XS_BF(false, false, true, true);
+    } else if (epi.fused) {
+      if (energy && vir) XS_BF(true, true, true, false);
+      else if (energy) XS_BF(true, false, true, false);
+      else if (vir) XS_BF(false, true, true, false);
+      else XS_BF(false, false, true, false);
     } else {
-      if (energy && vir) XS_BF(true, true, false);
-      else if (energy) XS_BF(true, false, false);
-      else if (vir) XS_BF(false, true, false);
-      else XS_BF(false, false, false);
+      if (energy && vir) XS_BF(true, true, false, false);
+      else if (energy) XS_BF(true, false, false, false);
+      else if (vir) XS_BF(false, true, false, false);
+      else XS_BF(false, false, false, false);
     }
 #undef XS_BF
     XS_HIP(hipGetLastError());
+    if (drift) drift_commit(c);
   }
 }
 
@@ -614,10 +1250,12 @@ void block_build(Ctx& c, double rl2)
   for (int attempt = 0; attempt < 2; attempt++) {
     c.nbr16.reserve((size_t)c.nmax * c.npad);
     c.nnb.reserve(c.npad);
+    c.brick_rows.reserve(c.bgeom.nblocks);
     XS_HIP(hipMemsetAsync(c.dscalar.p, 0, 2 * sizeof(unsigned long long), c.stream));
-    k_block_build<<<block_grid(c), kBlock, lds, c.stream>>>(c.bgeom, c.cell_off.p, c.pos.p, c.box_d.p, rl2,
+    k_block_build<<<block_grid(c), kPassThreads, lds, c.stream>>>(c.bgeom, c.cell_off.p, c.pos.p, c.box_d.p, rl2,
                                                             c.nbr16.p, c.nnb.p, c.nmax, c.npad, c.bcap,
-                                                            c.bgeom.nh_max, c.dscalar.p);
+                                                            c.bgeom.nh_max, c.dscalar.p + 4, c.brick_rows.p,
+                                                            c.dscalar.p);
     XS_HIP(hipGetLastError());
     XS_HIP(hipMemcpyAsync(c.hscalar, c.dscalar.p, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
                           c.stream));

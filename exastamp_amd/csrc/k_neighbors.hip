@@ -20,7 +20,7 @@ namespace {
 __global__ void k_wrap_cellkey(int64_t n, Pos4* __restrict__ pos, const uint16_t* __restrict__ rep,
                                const BoxDev* __restrict__ box, int px, int py, int pz, int nx, int ny,
                                int nz, int64_t ncell_rep, uint32_t* __restrict__ keys,
-                               uint32_t* __restrict__ vals)
+                               uint32_t* __restrict__ vals, unsigned long long* __restrict__ outside)
 {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -38,6 +38,11 @@ __global__ void k_wrap_cellkey(int64_t n, Pos4* __restrict__ pos, const uint16_t
   const int cz = xsmd_cell_coord(p.z, b.lo[2], b.inv_cs[2], nz);
   keys[i] = (uint32_t)((int64_t)r * ncell_rep + ((int64_t)cz * ny + cy) * nx + cx);
   vals[i] = (uint32_t)i;
+  // an atom outside its own cell's slab (more than one box length away along a
+  // periodic axis) voids the cell-box culling of the block build
+  if (xsmd_cell_gap(p.x, b.lo[0], b.L[0], nx, cx, px) > 0.0 || xsmd_cell_gap(p.y, b.lo[1], b.L[1], ny, cy, py) > 0.0 ||
+      xsmd_cell_gap(p.z, b.lo[2], b.L[2], nz, cz, pz) > 0.0)
+    atomicOr(outside, 1ull);
 }
 
 struct PermuteArgs {
@@ -219,9 +224,10 @@ void build_neighbor_list(Ctx& c, double rcut, double skin, bool prefer_block)
   if (c.ncell_tot >= (int64_t)1 << 32) throw Error(XS_E_LIMIT, "too many cells");
 
   const unsigned g = grid_for(n);
+  XS_HIP(hipMemsetAsync(c.dscalar.p + 4, 0, sizeof(unsigned long long), c.stream));
   k_wrap_cellkey<<<g, kBlock, 0, c.stream>>>(n, c.pos.p, c.has_rep ? c.rep.p : nullptr, c.box_d.p,
                                              c.periodic[0], c.periodic[1], c.periodic[2], c.nc[0], c.nc[1],
-                                             c.nc[2], c.ncell_rep, c.keys.p, c.vals.p);
+                                             c.nc[2], c.ncell_rep, c.keys.p, c.vals.p, c.dscalar.p + 4);
   XS_HIP(hipGetLastError());
 
   int nbits = 1;

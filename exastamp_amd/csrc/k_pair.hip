@@ -367,6 +367,7 @@ Outputs make_outputs(Ctx& c, const ForceEpilogue& ep)
 
 void launch_lj(Ctx& c, double eps, double sigma, double rcut, unsigned flags, const ForceEpilogue& epi)
 {
+  c.drift_done = false;
   if (c.n == 0) return;
   ensure_global_list(c);
   if (flags & XS_MIXED) lj_dispatch<float>(c, eps, sigma, rcut, flags, epi);
@@ -376,6 +377,7 @@ void launch_lj(Ctx& c, double eps, double sigma, double rcut, unsigned flags, co
 void launch_eam(Ctx& c, double rcut, bool do_rho, bool do_emb, bool do_force, unsigned flags,
                 const ForceEpilogue& epi)
 {
+  c.drift_done = false;
   if (c.n == 0) return;
   if (do_emb && !do_rho && c.rho_state != RhoState::Rho) throw Error(XS_E_STATE, "eam_rho2emb needs eam_rho first");
   const bool mixed = (flags & XS_MIXED) != 0;
@@ -385,6 +387,10 @@ void launch_eam(Ctx& c, double rcut, bool do_rho, bool do_emb, bool do_force, un
       throw Error(XS_E_STATE, "eam_force needs F'(rho) from eam_rho2emb in the same precision");
   }
   const bool multi = c.eam.nel > 1;
+  if (c.list_kind != ListKind::Block) {
+    if (do_rho) c.last_rho_rows = 0;
+    if (do_force) c.last_force_rows = 0;
+  }
   if (c.list_kind == ListKind::Block) {
     launch_eam_block(c, rcut, do_rho, do_emb, do_force, flags, epi);
   } else if (mixed) {
@@ -397,6 +403,10 @@ void launch_eam(Ctx& c, double rcut, bool do_rho, bool do_emb, bool do_force, un
   if (do_rho || do_emb) {
     c.rho_state = do_emb ? (mixed ? RhoState::FpFloat : RhoState::FpDouble) : RhoState::Rho;
     if (do_rho) c.rho_ckpt = true;
+  }
+  if (c.drift_done) {  // positions moved on to the next step: rho and F' are stale
+    c.rho_state = RhoState::None;
+    c.rho_ckpt = false;
   }
 }
 

@@ -98,6 +98,46 @@ __device__ __forceinline__ void write_outputs(const Outputs& o, int64_t i, doubl
   }
 }
 
+// Next-step drift fused into the MD-loop epilogue: the arithmetic of k_drift_kick
+// applied to the velocity this step's kick produced.  The loop only needs to know
+// whether some atom moved more than the rebuild threshold, so an atom past it
+// stores 1 to *over (every writer stores the same value).
+struct Drift {
+  Pos4* pos_out;
+  const double* rrx;
+  const double* rry;
+  const double* rrz;
+  double dt, dt2, hdt;
+  double thr2;
+  unsigned long long* over;
+};
+
+// Fused epilogue with drift: a = f/m, v += a*kick_dt (this step), then
+// push_f_v_r(dt) and push_f_v(dt/2) of the next step from the record p of atom i,
+// and the rebuild test on the displacement since the last list build.
+__device__ __forceinline__ void write_outputs_drift(const Outputs& o, const Drift& d, int64_t i, const Pos4& p,
+                                                    double fx, double fy, double fz)
+{
+  const double m = o.mass[o.type[i]];
+  const double a0 = fx / m, a1 = fy / m, a2 = fz / m;
+  o.ax[i] = a0;
+  o.ay[i] = a1;
+  o.az[i] = a2;
+  const double v0 = fma(a0, o.kick_dt, o.vx[i]);
+  const double v1 = fma(a1, o.kick_dt, o.vy[i]);
+  const double v2 = fma(a2, o.kick_dt, o.vz[i]);
+  Pos4 q = p;
+  q.x += fma(v0, d.dt, a0 * d.dt2);
+  q.y += fma(v1, d.dt, a1 * d.dt2);
+  q.z += fma(v2, d.dt, a2 * d.dt2);
+  q.w = 0.0;
+  store_pos(d.pos_out, i, q);
+  o.vx[i] = fma(a0, d.hdt, v0);
+  o.vy[i] = fma(a1, d.hdt, v1);
+  o.vz[i] = fma(a2, d.hdt, v2);
+  if (xsmd_d2(q.x - d.rrx[i], q.y - d.rry[i], q.z - d.rrz[i]) > d.thr2) *d.over = 1ull;
+}
+
 // onika tensor(fe, dr) * -0.5 added to v (row-major m11 m12 ... m33)
 __device__ __forceinline__ void add_virial(double* v, double fe0, double fe1, double fe2, double dx, double dy,
                                            double dz)
@@ -181,6 +221,23 @@ __device__ __forceinline__ Real eam_rho(const EamTabDev<Real>& t, const Real* __
   return cubic(load4(row), p);
 }
 
+// Single-element psi'(r)/r and phi(r) from the two halves of a combined row:
+// rc = rho value cubic, zc = z2r value cubic; the derivative quadratics follow
+// from the value cubics (times 1/dr).
+template <class Real>
+__device__ __forceinline__ void eam_pair_rows(const EamTabDev<Real>& t, const Q4<Real>& rc, const Q4<Real>& zc, Real p,
+                                              Real rinv, Real fpi, Real fpj, Real& fpair, Real& phi)
+{
+  const Real rhop = dcubic(rc, p) * t.rdr;
+  const Real z2p = dcubic(zc, p) * t.rdr;
+  const Real z2 = cubic(zc, p);
+  phi = z2 * rinv;
+  const Real phip = (z2p * rinv - phi * rinv) * t.conv;
+  phi *= t.conv;
+  const Real psip = (fpi + fpj) * rhop + phip;
+  fpair = psip * rinv;
+}
+
 // psi'(r)/r and phi(r) of eam_alloy_mm_force (eam_alloy.h:222-265):
 // psi' = F'_i rho'_j + F'_j rho'_i + phi', phi = z2r / r.
 template <class Real, bool MULTI>
@@ -192,7 +249,6 @@ __device__ __forceinline__ void eam_pair(const EamTabDev<Real>& t, const Real* _
   int m;
   Real p;
   knot_r(t, r, m, p);
-  Real psip;
   if constexpr (MULTI) {
     const Real rhoip = quad(load4(row_r(t.rhor, t, t.t2rhor[ti + 1][tj + 1], m) + 4), p);
     const Real rhojp = quad(load4(row_r(t.rhor, t, t.t2rhor[tj + 1][ti + 1], m) + 4), p);
@@ -202,20 +258,12 @@ __device__ __forceinline__ void eam_pair(const EamTabDev<Real>& t, const Real* _
     phi = z2 * rinv;
     const Real phip = (z2p * rinv - phi * rinv) * t.conv;
     phi *= t.conv;
-    psip = fpi * rhojp + fpj * rhoip + phip;
+    const Real psip = fpi * rhojp + fpj * rhoip + phip;
+    fpair = psip * rinv;
   } else {
     const Real* row = pair + (int64_t)m * 8;
-    const Q4<Real> rc = load4(row);
-    const Q4<Real> zc = load4(row + 4);
-    const Real rhop = dcubic(rc, p) * t.rdr;
-    const Real z2p = dcubic(zc, p) * t.rdr;
-    const Real z2 = cubic(zc, p);
-    phi = z2 * rinv;
-    const Real phip = (z2p * rinv - phi * rinv) * t.conv;
-    phi *= t.conv;
-    psip = (fpi + fpj) * rhop + phip;
+    eam_pair_rows(t, load4(row), load4(row + 4), p, rinv, fpi, fpj, fpair, phi);
   }
-  fpair = psip * rinv;
 }
 
 }  // namespace pm

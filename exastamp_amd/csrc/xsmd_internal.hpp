@@ -275,9 +275,19 @@ struct Ctx {
   BlockGeom bgeom{};
   int bcap = 0;                    // LDS slots per block (max halo atoms, rounded)
   int bcentral = 0;                // max central atoms per block
+  DBuf<int> brick_rows;            // longest list of each block
+  // persistent LDS-table passes: CUs and LDS per workgroup of the device;
+  // XSMD_LDS_TABLE=1 enables them, XSMD_LDS_ROWS caps the table rows in LDS
+  int num_cu = 0;
+  size_t lds_max = 65536;
+  bool lds_table_enabled = false;
+  int lds_table_rows = 0;
+  int last_rho_rows = 0, last_force_rows = 0;  // LDS table rows of the last EAM passes (0: not used)
+  bool drift_done = false;  // the last force launch applied ForceEpilogue::drift
 
   // small device scalars + pinned host mirror
-  DBuf<unsigned long long> dscalar;  // [0] max count, [1] max displacement^2 bits
+  DBuf<unsigned long long> dscalar;  // [0] max count, [1] max displacement^2 bits, [2..3] halo maxima,
+                                     // [4] atom outside its cell, [5] fused-drift rebuild flag
   unsigned long long* hscalar = nullptr;
   DBuf<double> red_part;  // thermo partial sums
   DBuf<double> red_out;
@@ -326,6 +336,14 @@ double stream_copy(Ctx& c, size_t bytes, int reps);
 struct ForceEpilogue {
   bool fused = false;   // overwrite a with f/m and kick v by a*kick_dt (MD loop)
   double kick_dt = 0;
+  // MD loop, fused form only: also apply the next step's push_f_v_r(drift_dt) and
+  // push_f_v(drift_dt/2), writing the new positions to pos2 (then made current),
+  // and set dscalar[5] to 1 if some atom moved more than sqrt(drift_thr2) since the
+  // last build.  A launcher that can do this sets Ctx::drift_done; otherwise the
+  // loop runs k_drift_kick itself.
+  bool drift = false;
+  double drift_dt = 0;
+  double drift_thr2 = 0;
 };
 void launch_lj(Ctx& c, double eps, double sigma, double rcut, unsigned flags, const ForceEpilogue& ep);
 void launch_eam(Ctx& c, double rcut, bool do_rho, bool do_rho2emb, bool do_force, unsigned flags,

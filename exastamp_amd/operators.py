@@ -308,6 +308,12 @@ class Simulation:
         self.ctx.xs_list_info(C.byref(kind), C.byref(blocks), C.byref(cap))
         return {"kind": ("global", "block")[kind.value], "blocks": blocks.value, "halo_capacity": cap.value}
 
+    def eam_pass_info(self) -> dict:
+        """Spline-table rows the last EAM density / force passes kept in LDS (0: HBM)."""
+        r, f = C.c_int(0), C.c_int(0)
+        self.ctx.xs_eam_pass_info(C.byref(r), C.byref(f))
+        return {"rho_table_rows": r.value, "force_table_rows": f.value}
+
     def neighbors(self):
         """(counts[n], ids[total], imgs[total,3]) in id space, entries in list order."""
         counts = np.zeros(self.n, dtype=np.int64)

@@ -65,6 +65,22 @@ XSMD_HD int xsmd_cell_coord(double x, double lo, double inv_cs, int n)
   return c < 0 ? 0 : (c >= n ? n - 1 : c);
 }
 
+/* Gap along one axis between coordinate x and the slab of cell c (n cells over
+ * [lo, lo + L)), the slab widened by 1e-6 A to cover the rounding of cell
+ * assignment; on a non-periodic axis the edge cells also hold the atoms clamped
+ * into them, so their slabs are unbounded on the outer side.  0 inside. */
+XSMD_HD double xsmd_cell_gap(double x, double lo, double L, int n, int c, int periodic)
+{
+  const double cs = L / n;
+  const double margin = 1e-6;
+  double a = lo + c * cs - margin;
+  double b = lo + (c + 1) * cs + margin;
+  double g = 0.0;
+  if (!(c == 0 && !periodic) && x < a) g = a - x;
+  if (!(c == n - 1 && !periodic) && x > b) g = x - b;
+  return g;
+}
+
 /* Number of cells along an axis of length L for a minimum cell size cs. */
 XSMD_HD int xsmd_cells_along(double L, double cs)
 {

@@ -3,14 +3,17 @@
 The block list is the global list in another encoding (16-bit halo slots instead of
 (index, image) words): converted back it must be identical entry by entry.  The block
 kernels evaluate the same pairs with the same per-pair arithmetic as the global-list
-kernels but sum each atom's list as two interleaved halves (two lanes per atom), so
-per-atom sums differ from the global path only by FP64 summation order.
+kernels but sum each atom's list in interleaved parts (two or four lanes per atom),
+so per-atom sums differ from the global path only by FP64 summation order.
 The global path is selected with XSMD_LIST=global, read when a context is created.
 
-Tolerances: neighbour lists bit-exact; per-atom outputs between the two paths within
-1e-12 of their rms (summation order only, in both precision forms); trajectories of
-40 NVE steps within 1e-9 A; against the oracle as in test_gpu_parity (FP64 1e-11,
-mixed 1e-5 of rms|F|).
+Tolerances: neighbour lists bit-exact.  FP64 form: per-atom outputs within 1e-10 of
+their rms (the sums of ~50 pair terms of up to ~10x the net force reorder at
+~1e-12).  Mixed form: within 1e-6 of the rms, because a last-bit change of the FP64
+density can flip the FP32 rounding of rho before the embedding lookup, which moves
+F'(rho) by one FP32 ulp (6e-8 relative) and the forces with it.  Trajectories of 40
+NVE steps: 1e-9 A (FP64), 1e-6 A (mixed).  Against the oracle as in test_gpu_parity
+(FP64 1e-11, mixed 1e-5 of rms|F|).
 """
 import numpy as np
 import pytest
@@ -72,7 +75,7 @@ def test_block_list_equals_global_list(monkeypatch, name):
         assert np.array_equal(x, y)
 
 
-def _close(x, y, tol=1e-12):
+def _close(x, y, tol=1e-10):
     scale = np.sqrt(np.mean(y.astype(np.float64) ** 2))
     return np.abs(x - y).max() <= tol * max(scale, 1e-300)
 
@@ -84,7 +87,7 @@ def test_block_forces_match_global(monkeypatch, name, mixed):
     b = _sim(monkeypatch, "block", cfg, masses, names, tab)
     g = _sim(monkeypatch, "global", cfg, masses, names, tab)
     for x, y in zip(_outputs(b, tab.rc, mixed), _outputs(g, tab.rc, mixed)):
-        assert _close(x, y)
+        assert _close(x, y, 1e-6 if mixed else 1e-10)
 
 
 @pytest.mark.parametrize("name", ["cu_one_cell", "cu_two_cells"])
@@ -124,8 +127,8 @@ def test_block_md_run_matches_global(monkeypatch, mixed):
         out.append((sim.state(), stats))
     (ra, va), sa = out[0]
     (rb, vb), sb = out[1]
-    assert np.abs(ra - rb).max() <= 1e-9
-    assert _close(va, vb, 1e-9)
+    assert np.abs(ra - rb).max() <= (1e-6 if mixed else 1e-9)
+    assert _close(va, vb, 1e-6 if mixed else 1e-9)
 
 
 def test_block_adjoint_and_pair_count_use_converted_list(monkeypatch):
@@ -145,3 +148,58 @@ def test_block_adjoint_and_pair_count_use_converted_list(monkeypatch):
     assert res[0][0] == res[1][0] and res[0][0]["within_rcut"] > 0
     ga, gb = res[0][1], res[1][1]
     assert np.abs(ga - gb).max() <= 1e-12 * np.abs(gb).max()
+
+
+def _env(monkeypatch, **kv):
+    for k in ("XSMD_LIST", "XSMD_LDS_TABLE", "XSMD_LDS_ROWS"):
+        monkeypatch.delenv(k, raising=False)
+    for k, v in kv.items():
+        monkeypatch.setenv(k, v)
+
+
+@pytest.mark.parametrize("name", ["cu_jitter", "cu_one_cell", "replicas"])
+def test_lds_table_passes_match_hbm_table(monkeypatch, name):
+    """Single-element FP32 passes with the spline table in LDS (persistent workgroups,
+    XSMD_LDS_TABLE=1) against the default block passes reading the table from HBM: the
+    same per-pair arithmetic,
+    another summation order (mixed tolerance, 1e-6 of the rms).  A table capped to a
+    third of its rows sends the short pairs to the HBM fallback of the same kernels:
+    bit-identical to the full LDS table."""
+    cfg, masses, names, tab = _case(name)
+    out = {}
+    for label, env in (("lds", {"XSMD_LDS_TABLE": "1"}), ("hbm", {}),
+                       ("lds_capped", {"XSMD_LDS_TABLE": "1", "XSMD_LDS_ROWS": str(tab.nr // 3)})):
+        _env(monkeypatch, **env)
+        sim = make_sim(cfg, masses, names)
+        sim.eam_alloy_init(tab)
+        sim.chunk_neighbors(rcut_max=tab.rc, rcut_inc=0.3)
+        assert sim.list_info()["kind"] == "block"
+        out[label] = _outputs(sim, tab.rc, True)
+        info = sim.eam_pass_info()
+        if label == "hbm":
+            assert info == {"rho_table_rows": 0, "force_table_rows": 0}
+        elif label == "lds":
+            assert info["rho_table_rows"] > 0 and info["force_table_rows"] > 0
+        else:
+            assert info["force_table_rows"] == tab.nr // 3
+    for x, y in zip(out["lds_capped"], out["lds"]):
+        assert np.array_equal(x, y)
+    for x, y in zip(out["lds"], out["hbm"]):
+        assert _close(x, y, 1e-6)
+
+
+def test_lds_table_md_run_vs_oracle_energy(monkeypatch, oracle_mod):
+    """100 mixed NVE steps on the LDS-table passes (fused run, Cu 600 K): the potential
+    energy of every step within 1e-6 of the FP64 oracle trajectory's."""
+    _env(monkeypatch, XSMD_LDS_TABLE="1")
+    tab = cu_tables()
+    cfg = cu_config((8, 8, 8), T=600.0, jitter=0.0, seed=31)
+    sim = make_sim(cfg, [CU_MASS])
+    sim.eam_alloy_init(tab)
+    _, th = sim.md_run("eam", 100, 0.001, tab.rc, 0.3, mixed=True, thermo_every=1)
+    assert sim.eam_pass_info()["force_table_rows"] > 0
+    o = make_oracle(oracle_mod, cfg, [CU_MASS])
+    o.set_eam(tab)
+    _, th_ref = o.md_run("eam", 100, 0.001, tab.rc, 0.3)
+    pe = np.array([t.potential_energy for t in th])
+    assert np.abs(pe - th_ref[:, 24]).max() <= 1e-6 * np.abs(th_ref[:, 24]).max()

@@ -267,10 +267,12 @@ def test_eam_split_passes_equal_fused(monkeypatch, kind):
         assert np.array_equal(e1, sim.ep())
 
 
-def test_eam_operator_loop_equals_md_run():
+@pytest.mark.parametrize("mixed", [False, True])
+def test_eam_operator_loop_equals_md_run(mixed):
     """Operator-by-operator verlet_nve (push_f_v_r, push_f_v, particle_displ_over,
     chunk_neighbors, zero_force_energy, eam_alloy_force, force_to_accel, push_f_v)
-    reproduces the fused xs_md_run bit for bit."""
+    reproduces the fused xs_md_run bit for bit (in the mixed form the fused run also
+    folds each next step's drift into the force pass)."""
     tab = cu_tables()
     cfg = cu_config((6, 6, 6), T=900.0, seed=4)
     dt, rc, skin = 0.001, 4.95, 0.3
@@ -278,7 +280,7 @@ def test_eam_operator_loop_equals_md_run():
     a.eam_alloy_init(tab)
     a.chunk_neighbors(rcut_max=rc, rcut_inc=skin)
     a.zero_force_energy()
-    a.eam_alloy_force(rcut=rc)
+    a.eam_alloy_force(rcut=rc, mixed=mixed)
     a.force_to_accel()
     for _ in range(30):
         a.push_f_v_r(dt, 1.0)
@@ -286,12 +288,12 @@ def test_eam_operator_loop_equals_md_run():
         if a.particle_displ_over(0.5 * skin):
             a.chunk_neighbors(rcut_max=rc, rcut_inc=skin)
         a.zero_force_energy()
-        a.eam_alloy_force(rcut=rc)
+        a.eam_alloy_force(rcut=rc, mixed=mixed)
         a.force_to_accel()
         a.push_f_v(dt, 0.5)
     b = make_sim(cfg, [CU_MASS])
     b.eam_alloy_init(tab)
-    b.md_run("eam", 30, dt, rc, skin)
+    b.md_run("eam", 30, dt, rc, skin, mixed=mixed)
     ra, va = a.state()
     rb, vb = b.state()
     assert np.array_equal(ra, rb) and np.array_equal(va, vb)
