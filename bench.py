@@ -110,8 +110,8 @@ def force_kernel_name(list_info, pass_info):
     if list_info["kind"] != "block":
         return "k_eam_force<float, false, false, false, true>"
     if pass_info["force_table_rows"] > 0:
-        return "k_tab_eam_force<false, false, true>"
-    return "k_block_eam_force<float, false, false, false, true>"
+        return "k_split_eam_force<false, false, true, true>"
+    return "k_block_eam_force<float, false, false, false, true, true>"
 
 
 def measured_traffic(kernel):

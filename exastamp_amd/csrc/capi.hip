@@ -178,8 +178,8 @@ xs_status xs_ctx_create(int device, xs_ctx** out)
     c->num_cu = prop.multiProcessorCount;
     c->lds_max = prop.sharedMemPerBlockOptin > prop.sharedMemPerBlock ? prop.sharedMemPerBlockOptin
                                                                        : prop.sharedMemPerBlock;
-    if (const char* lt = std::getenv("XSMD_LDS_TABLE")) c->lds_table_enabled = std::strcmp(lt, "1") == 0;
-    if (const char* lr = std::getenv("XSMD_LDS_ROWS")) c->lds_table_rows = std::max(0, std::atoi(lr));
+    if (const char* st = std::getenv("XSMD_SPLIT_TABLE")) c->split_enabled = std::strcmp(st, "1") == 0;
+    if (const char* sr = std::getenv("XSMD_SPLIT_ROWS")) c->split_rows = std::max(0, std::atoi(sr));
     if (const char* ba = std::getenv("XSMD_BLOCK_ATOMS")) {
       const double v = std::atof(ba);
       if (v >= 1.0 && v <= 4096.0) c->block_atoms = v;
@@ -443,11 +443,11 @@ xs_status xs_list_info(xs_ctx* h, int* kind, int64_t* blocks, int* halo_capacity
   });
 }
 
-xs_status xs_eam_pass_info(xs_ctx* h, int* rho_table_rows, int* force_table_rows)
+xs_status xs_eam_pass_info(xs_ctx* h, int* rho_lds_rows, int* force_lds_rows)
 {
   return guarded(h, [&](Ctx& c) {
-    if (rho_table_rows) *rho_table_rows = c.last_rho_rows;
-    if (force_table_rows) *force_table_rows = c.last_force_rows;
+    if (rho_lds_rows) *rho_lds_rows = c.last_rho_rows;
+    if (force_lds_rows) *force_lds_rows = c.last_force_rows;
   });
 }
 

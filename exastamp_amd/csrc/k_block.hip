@@ -73,24 +73,36 @@ __device__ __forceinline__ bool wrap_axis(int c, int n, int periodic, int& w, in
   return true;
 }
 
-// Staged halo records in LDS as two 16-byte arrays, {x, y} and {z, w}: a random
-// 16-byte read then starts on any of 16 bank groups instead of the 8 that 32-byte
-// records allow, which halves the expected bank-conflict depth of a gather.
+// Staged halo records in LDS as arrays of doubles x[], y[], z[] and, for the
+// passes that read it, w[]: a random 8-byte LDS read costs a wave ~16 CU cycles
+// against ~30 for a 16-byte read, so three or four 8-byte reads (~23, ~31) beat the
+// two 16-byte reads of {x, y}, {z, w} (~66).  Passes without w alias w to x.
 struct LRec {
-  double2* xy;
-  double2* zw;
-  __device__ __forceinline__ Pos4 get(int c) const
-  {
-    const double2 a = xy[c];
-    const double2 b = zw[c];
-    return {a.x, a.y, b.x, b.y};
-  }
+  double* x;
+  double* y;
+  double* z;
+  double* w;
+  bool has_w;
+  __device__ __forceinline__ Pos4 get(int c) const { return {x[c], y[c], z[c], w[c]}; }
   __device__ __forceinline__ void put(int c, const Pos4& p) const
   {
-    xy[c] = make_double2(p.x, p.y);
-    zw[c] = make_double2(p.z, p.w);
+    x[c] = p.x;
+    y[c] = p.y;
+    z[c] = p.z;
+    if (has_w) w[c] = p.w;
   }
 };
+
+// record arrays at p for cap slots; returns the bytes they take
+__device__ __forceinline__ size_t lrec_at(unsigned char* p, int cap, bool with_w, LRec& r)
+{
+  r.x = reinterpret_cast<double*>(p);
+  r.y = r.x + cap;
+  r.z = r.y + cap;
+  r.w = with_w ? r.z + cap : r.x;
+  r.has_w = with_w;
+  return (size_t)cap * sizeof(double) * (with_w ? 4 : 3);
+}
 
 // Halo table of block b in LDS (hc[0..nh)), filled by all threads; returns the
 // number of halo atoms.  Ends with __syncthreads().
@@ -211,13 +223,13 @@ struct LdsLayout {
   int* total;
 };
 
-__device__ __forceinline__ LdsLayout lds_layout(int cap, int nh_max, bool with_type)
+// LDS of the block kernels: records, halo table, counters, per-record types
+__device__ __forceinline__ LdsLayout lds_layout(int cap, int nh_max, bool with_type, bool with_w)
 {
   extern __shared__ __align__(32) unsigned char smem[];
   LdsLayout l;
-  l.rec.xy = reinterpret_cast<double2*>(smem);
-  l.rec.zw = reinterpret_cast<double2*>(smem + (size_t)cap * sizeof(double2));
-  l.hc = reinterpret_cast<HaloCell*>(smem + (size_t)cap * sizeof(Pos4));
+  const size_t rb = lrec_at(smem, cap, with_w, l.rec);
+  l.hc = reinterpret_cast<HaloCell*>(smem + rb);
   l.total = reinterpret_cast<int*>(l.hc + nh_max);
   l.ltype = with_type ? reinterpret_cast<uint8_t*>(l.total + 4) : nullptr;
   return l;
@@ -267,7 +279,7 @@ __global__ void __launch_bounds__(kPassThreads) k_block_build(BlockGeom g, const
 {
   const int64_t b = lblock();
   if (b >= g.nblocks) return;
-  LdsLayout l = lds_layout(cap, nh_max, false);
+  LdsLayout l = lds_layout(cap, nh_max, false, false);
   const BlockShape s = block_shape(g, b);
   const int total = build_halo(g, s, cell_off, l.hc, l.total);
   if (total > cap) {  // cannot happen when cap = measured maximum; never write past LDS
@@ -324,7 +336,7 @@ __global__ void __launch_bounds__(256) k_block_to_global(BlockGeom g, const int*
 {
   const int64_t b = lblock();
   if (b >= g.nblocks) return;
-  LdsLayout l = lds_layout(0, nh_max, false);
+  LdsLayout l = lds_layout(0, nh_max, false, false);
   const BlockShape s = block_shape(g, b);
   const int nh = s.hx * s.hy * s.hz;
   build_halo(g, s, cell_off, l.hc, l.total);
@@ -443,7 +455,7 @@ __global__ void __launch_bounds__(kPassThreads, kPassWaves<false>) k_block_eam_r
 {
   const int64_t b = lblock();
   if (b >= a.g.nblocks) return;
-  LdsLayout l = lds_layout(a.cap, a.g.nh_max, MULTI);
+  LdsLayout l = lds_layout(a.cap, a.g.nh_max, MULTI, false);
   const BlockShape s = block_shape(a.g, b);
   const int total = build_halo(a.g, s, a.cell_off, l.hc, l.total);
   if (total > a.cap) return;  // excluded by the host (cap = measured maximum)
@@ -490,7 +502,7 @@ __global__ void __launch_bounds__(kPassThreads, kPassWaves<VIRIAL>) k_block_eam_
   static_assert(!DRIFT || (FUSED && !ENERGY && !VIRIAL), "the drift epilogue is the plain MD-loop step");
   const int64_t b = lblock();
   if (b >= a.g.nblocks) return;
-  LdsLayout l = lds_layout(a.cap, a.g.nh_max, MULTI);
+  LdsLayout l = lds_layout(a.cap, a.g.nh_max, MULTI, true);
   const BlockShape s = block_shape(a.g, b);
   const int total = build_halo(a.g, s, a.cell_off, l.hc, l.total);
   if (total > a.cap) return;  // excluded by the host (cap = measured maximum)
@@ -538,80 +550,50 @@ __global__ void __launch_bounds__(kPassThreads, kPassWaves<VIRIAL>) k_block_eam_
 }
 
 // ---------------------------------------------------------------------------
-// Persistent passes with the spline table in LDS (single element, FP32 tables)
+// Force pass with the rho rows in LDS (single element, FP32 tables)
 // ---------------------------------------------------------------------------
-// A wave's table gathers touch one cache line per distinct row, so rows read from
-// HBM/L2 keep the texture data path busy for ~50 cycles per gather.  Here one
-// 1024-thread workgroup per CU copies rows [m_lo, m_lo + rows) of the combined
-// table into LDS once per launch and then walks a contiguous run of bricks, so
-// table rows and neighbour records both come from LDS.  A pair whose knot index
-// lies below m_lo (an unusually short pair) reads its row from global memory.
-// Four lanes per central atom take list entries k = h, h + 4, ...; the lanes of
-// one parity h are adjacent within the wave, so list reads stay contiguous.
-constexpr int kTabThreads = 1024;
-constexpr int kTabLanes = 4;
-constexpr int kTabGroup = 64 / kTabLanes;           // atoms per wave
-constexpr int kTabAtoms = kTabThreads / kTabLanes;  // atoms per pass over a brick
-constexpr int kTBatch = 2;
+// A random 16-byte row gather costs the CU ~95 cycles per wave through the
+// texture path from L1/L2 but ~30 from LDS, and the block force pass spends its
+// time in exactly two such gathers per pair (rho cubic, r*phi cubic) next to two
+// LDS record reads.  Here the three rho coefficients the force needs (c3..c5, for
+// rho') of the knots nearest rc sit in LDS as three float arrays, the r*phi rows
+// stay in HBM, which balances the texture path against LDS.  The table slice is
+// loaded once per workgroup: workgroups are persistent (two per CU) and walk
+// contiguous runs of bricks.  A pair whose knot lies below the slice (an unusually
+// short pair) reads its rho row from HBM.
+constexpr int kSplitThreads = 512;  // two lanes per atom, as the block passes
 
-struct TabArgs {
+struct SplitArgs {
   BlockArgs a;
-  int m_lo, rows;     // LDS holds table rows m_lo .. m_lo + rows - 1 (the last is nr - 1)
-  int64_t per_wg;     // bricks per workgroup
-  const int* __restrict__ brick_rows;  // longest list of each brick
-  int nmax;           // rows of the list
+  int m_lo, rows;   // LDS holds rho rows m_lo .. m_lo + rows - 1 (the last is nr - 1)
+  int64_t per_wg;   // bricks per workgroup
 };
 
-__device__ __forceinline__ int tab_slot() { return (threadIdx.x >> 6) * kTabGroup + (threadIdx.x & (kTabGroup - 1)); }
-__device__ __forceinline__ int tab_h() { return (threadIdx.x & 63) / kTabGroup; }
-
-// sum over the four lanes of one atom (lanes l, l ^ 16, l ^ 32, l ^ 48)
-__device__ __forceinline__ double group_sum(double v)
-{
-  v += __shfl_xor(v, 16);
-  v += __shfl_xor(v, 32);
-  return v;
-}
-
-struct TabLayout {
-  float* tab;
-  LRec rec;
-  HaloCell* hc;
-  int* total;
-};
-
-__device__ __forceinline__ TabLayout tab_layout(int rows, int row_floats, int cap, int nh_max)
-{
-  extern __shared__ __align__(32) unsigned char smem[];
-  TabLayout l;
-  l.tab = reinterpret_cast<float*>(smem);
-  unsigned char* p = smem + (size_t)rows * row_floats * sizeof(float);
-  l.rec.xy = reinterpret_cast<double2*>(p);
-  l.rec.zw = reinterpret_cast<double2*>(p + (size_t)cap * sizeof(double2));
-  l.hc = reinterpret_cast<HaloCell*>(p + (size_t)cap * sizeof(Pos4));
-  l.total = reinterpret_cast<int*>(l.hc + nh_max);
-  return l;
-}
-
-// list slots of entries k0, k0 + 4, ... of atom gi (the atom itself past the count).
-// The row read is clamped into [0, cnt - 1], so every load is unconditional and in
-// bounds, and the loads of the next batch can stay in flight while this one is used.
+// Raw list slots of entries k0, k0 + 2, ... of atom gi.  The row read is clamped
+// into [0, cnt - 1], so every load is unconditional and in bounds, and nothing
+// reads the values until the next batch (plist_use), so those loads stay in flight
+// while this batch is evaluated (the persistent passes run four waves per SIMD).
 template <int U>
-__device__ __forceinline__ void tlist(const BlockArgs& a, int64_t gi, int self, int k0, int cnt, int (&c)[U])
+__device__ __forceinline__ void plist(const BlockArgs& a, int64_t gi, int k0, int cnt, int (&raw)[U])
 {
 #pragma unroll
   for (int u = 0; u < U; u++) {
-    const int k = k0 + kTabLanes * u;
-    const int kk = max(0, min(k, cnt - 1));
-    const int raw = a.nbr16[(int64_t)kk * a.npad + gi];
-    c[u] = (k < cnt) ? raw : self;
+    const int kk = max(0, min(k0 + 2 * u, cnt - 1));
+    raw[u] = a.nbr16[(int64_t)kk * a.npad + gi];
   }
 }
 
-// neighbour records of list slots c (entries k0, k0 + 4, ...); entries past the
-// count are masked and evaluated at d2 = rc^2, whose knot is in LDS
+// slots of entries k0, k0 + 2, ...: the raw values, or the atom itself past the count
 template <int U>
-__device__ __forceinline__ void tgather(const BlockArgs& a, const LRec& rec, const int (&c)[U], int k0, int cnt,
+__device__ __forceinline__ void plist_use(const int (&raw)[U], int self, int k0, int cnt, int (&c)[U])
+{
+#pragma unroll
+  for (int u = 0; u < U; u++) c[u] = (k0 + 2 * u < cnt) ? (raw[u] & 0xffff) : self;
+}
+
+// neighbour records of list slots c (entries k0, k0 + 2, ...)
+template <int U>
+__device__ __forceinline__ void pgather(const BlockArgs& a, const LRec& rec, const int (&c)[U], int k0, int cnt,
                                         const Pos4& pi, LNb (&nb)[U])
 {
 #pragma unroll
@@ -623,295 +605,169 @@ __device__ __forceinline__ void tgather(const BlockArgs& a, const LRec& rec, con
     nb[u].d2 = xsmd_d2(nb[u].x, nb[u].y, nb[u].z);
     nb[u].w = pj.w;
     nb[u].c = c[u];
-    nb[u].in = (k0 + kTabLanes * u < cnt) && nb[u].d2 <= a.rc2;
+    nb[u].in = (k0 + 2 * u < cnt) && nb[u].d2 <= a.rc2;
   }
 }
 
-// value cubic of rho at knot m: LDS row when m >= m_lo, else the HBM row
-__device__ __forceinline__ pm::Q4<float> tab_rho_row(const float* ltab, int m_lo, const float* __restrict__ g, int m)
+__device__ __forceinline__ size_t split_table_bytes(int rows)
 {
-  if (m >= m_lo) {
-    const float4 v = reinterpret_cast<const float4*>(ltab)[m - m_lo];
-    return {v.x, v.y, v.z, v.w};
-  }
-  return pm::load4(g + (int64_t)m * 8);
+  return ((size_t)3 * rows * sizeof(float) + 15) / 16 * 16;
 }
 
-__device__ __forceinline__ void tab_pair_rows(const float* ltab, int m_lo, const float* __restrict__ g, int m,
-                                              pm::Q4<float>& rc, pm::Q4<float>& zc)
+// two workgroups per CU (LDS) make four waves per SIMD: the full 128 VGPRs
+template <bool ENERGY, bool VIRIAL, bool FUSED, bool DRIFT>
+__global__ void __launch_bounds__(kSplitThreads, 4)
+    k_split_eam_force(SplitArgs sa, EamTabDev<float> t, const float* __restrict__ pair, pm::Outputs o, pm::Drift dr)
 {
-  if (m >= m_lo) {
-    const float4* q = reinterpret_cast<const float4*>(ltab) + (int64_t)(m - m_lo) * 2;
-    const float4 u = q[0];
-    const float4 w = q[1];
-    rc = {u.x, u.y, u.z, u.w};
-    zc = {w.x, w.y, w.z, w.w};
-  } else {
-    rc = pm::load4(g + (int64_t)m * 8);
-    zc = pm::load4(g + (int64_t)m * 8 + 4);
+  static_assert(!DRIFT || (FUSED && !ENERGY && !VIRIAL), "the drift epilogue is the plain MD-loop step");
+  const BlockArgs& a = sa.a;
+  extern __shared__ __align__(32) unsigned char smem[];
+  float* c3 = reinterpret_cast<float*>(smem);
+  float* c4 = c3 + sa.rows;
+  float* c5 = c4 + sa.rows;
+  unsigned char* base = smem + split_table_bytes(sa.rows);
+  LRec rec;
+  HaloCell* hc = reinterpret_cast<HaloCell*>(base + lrec_at(base, a.cap, true, rec));
+  int* total_p = reinterpret_cast<int*>(hc + a.g.nh_max);
+  for (int i = threadIdx.x; i < sa.rows; i += blockDim.x) {
+    const float* row = pair + (int64_t)(sa.m_lo + i) * 8;
+    c3[i] = row[0];
+    c4[i] = row[1];
+    c5[i] = row[2];
   }
-}
-
-// ---- per-brick pipeline of the persistent passes ----
-// Global loads of a brick form one round trip: the halo cells of brick b + 1
-// (cell offsets and the brick's longest list) are fetched while brick b computes;
-// at the start of a brick every lane issues its atom's count and list entries and
-// then the halo record loads, all before the first wait; the pair loop itself
-// reads only LDS and registers.
-constexpr int kTabPL = 16;  // list entries per lane held in registers (rows < 4 * kTabPL), two per VGPR
-
-// this thread's halo cell of a brick (threads >= the cell count hold nothing)
-struct HaloNext {
-  int gstart, count, img, rows;
-};
-
-__device__ __forceinline__ void halo_fetch(const BlockGeom& g, const int* __restrict__ cell_off,
-                                           const int* __restrict__ brick_rows, int64_t b, HaloNext& hn)
-{
-  const BlockShape s = block_shape(g, b);
-  const int nh = s.hx * s.hy * s.hz;
-  hn = {0, 0, 0, brick_rows[b]};
-  const int h = threadIdx.x;
-  if (h < nh) {
-    const int ix = h % s.hx, iy = (h / s.hx) % s.hy, iz = h / (s.hx * s.hy);
-    int wx, wy, wz, kx, ky, kz;
-    if (wrap_axis(s.cx0 - 1 + ix, g.nx, g.px, wx, kx) && wrap_axis(s.cy0 - 1 + iy, g.ny, g.py, wy, ky) &&
-        wrap_axis(s.cz0 - 1 + iz, g.nz, g.pz, wz, kz)) {
-      const int64_t c = (int64_t)s.r * g.ncell_rep + ((int64_t)wz * g.ny + wy) * g.nx + wx;
-      hn.gstart = cell_off[c];
-      hn.count = cell_off[c + 1] - hn.gstart;
-      hn.img = (int)(xsmd_nbr_encode(0u, kx, ky, kz) >> XSMD_NBR_INDEX_BITS);
-    }
-  }
-}
-
-// halo table of the brick from the fetched cells, with the exclusive prefix of the
-// counts (as build_halo); ends with __syncthreads()
-__device__ int halo_commit(const BlockShape& s, const HaloNext& hn, HaloCell* hc, int* total_out)
-{
-  const int nh = s.hx * s.hy * s.hz;
-  if ((int)threadIdx.x < nh) hc[threadIdx.x] = HaloCell{hn.gstart, 0, hn.count, hn.img};
-  __syncthreads();
-  if (threadIdx.x < 64) {
-    const int lane = threadIdx.x;
-    int carry = 0;
-    for (int base = 0; base < nh; base += 64) {
-      const int h = base + lane;
-      const int cnt = (h < nh) ? hc[h].count : 0;
-      int v = cnt;
-#pragma unroll
-      for (int off = 1; off < 64; off <<= 1) {
-        const int t = __shfl_up(v, off);
-        if (lane >= off) v += t;
-      }
-      if (h < nh) hc[h].lstart = carry + v - cnt;
-      carry += __shfl(v, 63);
-    }
-    if (lane == 0) *total_out = carry;
-  }
-  __syncthreads();
-  return *total_out;
-}
-
-// Halo records, at most two per thread (cap <= 2 * kTabThreads): all loads are
-// issued before any record is written.  Ends with __syncthreads().
-__device__ __forceinline__ void stage2(const BlockShape& s, const HaloCell* hc, int total,
-                                       const Pos4* __restrict__ pos, const double* L, const LRec& rec)
-{
-  const int nh = s.hx * s.hy * s.hz;
-  Pos4 p[2];
-  int img[2];
-#pragma unroll
-  for (int r = 0; r < 2; r++) {
-    const int c = threadIdx.x + r * kTabThreads;
-    img[r] = 0;
-    if (c < total) {
-      const HaloCell e = hc[find_cell(hc, nh, c)];
-      p[r] = load_pos(pos, e.gstart + (c - e.lstart));
-      img[r] = e.img;
-    }
-  }
-#pragma unroll
-  for (int r = 0; r < 2; r++) {
-    const int c = threadIdx.x + r * kTabThreads;
-    if (c < total) {
-      Pos4 q = p[r];
-      if (img[r]) {
-        const uint32_t code = (uint32_t)img[r] << XSMD_NBR_INDEX_BITS;
-        q.x = xsmd_image(q.x, xsmd_nbr_img_x(code), L[0]);
-        q.y = xsmd_image(q.y, xsmd_nbr_img_y(code), L[1]);
-        q.z = xsmd_image(q.z, xsmd_nbr_img_z(code), L[2]);
-      }
-      rec.put(c, q);
-    }
-  }
-  __syncthreads();
-}
-
-// list entries h, h + 4, ... of atom gi into registers (entry u in half u & 1 of
-// E[u / 2]), rows < 4 * min(kTabPL, units) (units = the brick's longest list / 4,
-// rounded up); rows past this atom's count hold another atom's data and are masked
-// by the count later
-__device__ __forceinline__ void list_fetch(const BlockArgs& a, int nmax, int64_t gi, int h, int units,
-                                           uint32_t (&E)[kTabPL / 2])
-{
-#pragma unroll
-  for (int u = 0; u < kTabPL; u += 2) {
-    uint32_t lo = 0, hi = 0;
-    if (u < units) lo = a.nbr16[(int64_t)min(h + kTabLanes * u, nmax - 1) * a.npad + gi];
-    if (u + 1 < units) hi = a.nbr16[(int64_t)min(h + kTabLanes * (u + 1), nmax - 1) * a.npad + gi];
-    E[u / 2] = lo | (hi << 16);
-  }
-}
-
-__device__ __forceinline__ int list_entry(const uint32_t (&E)[kTabPL / 2], int u)
-{
-  return (int)((E[u / 2] >> (16 * (u & 1))) & 0xffffu);
-}
-
-// EAM density of central atom ca: entries from E (and, past 4 * kTabPL entries, from HBM)
-template <int U>
-__device__ __forceinline__ double tab_rho_entries(const TabArgs& ta, const EamTabDev<float>& t,
-                                                  const float* __restrict__ pair, const float* ltab, const LRec& rec,
-                                                  const Central& ca, const Pos4& pi, int h, int cnt,
-                                                  const uint32_t (&E)[kTabPL / 2])
-{
-  const BlockArgs& a = ta.a;
-  double rho = 0.0;
-#pragma unroll 1
-  for (int u0 = 0; u0 < kTabPL; u0 += U) {
-    if (h + kTabLanes * u0 >= cnt) break;
-    int c[U];
-#pragma unroll
-    for (int u = 0; u < U; u++) c[u] = (h + kTabLanes * (u0 + u) < cnt) ? list_entry(E, u0 + u) : ca.local;
-    LNb nb[U];
-    tgather<U>(a, rec, c, h + kTabLanes * u0, cnt, pi, nb);
-#pragma unroll
-    for (int u = 0; u < U; u++) {
-      float r, rinv, p;
-      int m;
-      pm::dist((float)(nb[u].in ? nb[u].d2 : a.rc2), r, rinv);
-      pm::knot_r(t, r, m, p);
-      const float val = pm::cubic(tab_rho_row(ltab, ta.m_lo, pair, m), p);
-      rho += nb[u].in ? (double)val : 0.0;
-    }
-  }
-  for (int k0 = h + kTabLanes * kTabPL; k0 < cnt; k0 += kTabLanes * U) {
-    int c[U];
-    tlist<U>(a, ca.global, ca.local, k0, cnt, c);
-    LNb nb[U];
-    tgather<U>(a, rec, c, k0, cnt, pi, nb);
-#pragma unroll
-    for (int u = 0; u < U; u++) {
-      float r, rinv, p;
-      int m;
-      pm::dist((float)(nb[u].in ? nb[u].d2 : a.rc2), r, rinv);
-      pm::knot_r(t, r, m, p);
-      const float val = pm::cubic(tab_rho_row(ltab, ta.m_lo, pair, m), p);
-      rho += nb[u].in ? (double)val : 0.0;
-    }
-  }
-  return rho;
-}
-
-struct ForceSums {
-  double fx, fy, fz, e;
-  double v[9];
-};
-
-template <int U, bool ENERGY, bool VIRIAL>
-__device__ __forceinline__ void tab_force_batch(const TabArgs& ta, const EamTabDev<float>& t,
-                                                const float* __restrict__ pair, const float* ltab, float fpi,
-                                                const LNb (&nb)[U], ForceSums& fs)
-{
-#pragma unroll
-  for (int u = 0; u < U; u++) {
-    float r, rinv, p, fp_r, phi;
-    int m;
-    pm::dist((float)(nb[u].in ? nb[u].d2 : ta.a.rc2), r, rinv);
-    pm::knot_r(t, r, m, p);
-    pm::Q4<float> rc, zc;
-    tab_pair_rows(ltab, ta.m_lo, pair, m, rc, zc);
-    pm::eam_pair_rows(t, rc, zc, p, rinv, fpi, (float)nb[u].w, fp_r, phi);
-    const double fpair = nb[u].in ? (double)fp_r : 0.0;
-    const double fe0 = nb[u].x * fpair, fe1 = nb[u].y * fpair, fe2 = nb[u].z * fpair;
-    fs.fx += fe0;
-    fs.fy += fe1;
-    fs.fz += fe2;
-    if constexpr (ENERGY) fs.e += nb[u].in ? 0.5 * (double)phi : 0.0;
-    if constexpr (VIRIAL) pm::add_virial(fs.v, fe0, fe1, fe2, nb[u].x, nb[u].y, nb[u].z);
-  }
-}
-
-template <int U, bool ENERGY, bool VIRIAL>
-__device__ __forceinline__ void tab_force_entries(const TabArgs& ta, const EamTabDev<float>& t,
-                                                  const float* __restrict__ pair, const float* ltab,
-                                                  const LRec& rec, const Central& ca, const Pos4& pi, int h,
-                                                  int cnt, const uint32_t (&E)[kTabPL / 2], ForceSums& fs)
-{
-  const BlockArgs& a = ta.a;
-  const float fpi = (float)pi.w;
-#pragma unroll 1
-  for (int u0 = 0; u0 < kTabPL; u0 += U) {
-    if (h + kTabLanes * u0 >= cnt) break;
-    int c[U];
-#pragma unroll
-    for (int u = 0; u < U; u++) c[u] = (h + kTabLanes * (u0 + u) < cnt) ? list_entry(E, u0 + u) : ca.local;
-    LNb nb[U];
-    tgather<U>(a, rec, c, h + kTabLanes * u0, cnt, pi, nb);
-    tab_force_batch<U, ENERGY, VIRIAL>(ta, t, pair, ltab, fpi, nb, fs);
-  }
-  for (int k0 = h + kTabLanes * kTabPL; k0 < cnt; k0 += kTabLanes * U) {
-    int c[U];
-    tlist<U>(a, ca.global, ca.local, k0, cnt, c);
-    LNb nb[U];
-    tgather<U>(a, rec, c, k0, cnt, pi, nb);
-    tab_force_batch<U, ENERGY, VIRIAL>(ta, t, pair, ltab, fpi, nb, fs);
-  }
-}
-
-template <bool EMB, bool ENERGY, bool SET_EP>
-__global__ void __launch_bounds__(kTabThreads) k_tab_eam_rho(TabArgs ta, EamTabDev<float> t,
-                                                             const float* __restrict__ pair,
-                                                             double* __restrict__ rho_io, double* __restrict__ pos_w,
-                                                             double* __restrict__ ep)
-{
-  const BlockArgs& a = ta.a;
-  const TabLayout l = tab_layout(ta.rows, 4, a.cap, a.g.nh_max);
-  // rho value cubics: the first half of each combined row
-  for (int i = threadIdx.x; i < ta.rows; i += blockDim.x)
-    reinterpret_cast<float4*>(l.tab)[i] = reinterpret_cast<const float4*>(pair)[(int64_t)(ta.m_lo + i) * 2];
-  const int64_t b0 = lblock() * ta.per_wg;
-  const int64_t b1 = min(b0 + ta.per_wg, a.g.nblocks);
-  const int h = tab_h();
-  HaloNext hn;
-  if (b0 < b1) halo_fetch(a.g, a.cell_off, ta.brick_rows, b0, hn);
+  const int64_t b0 = lblock() * sa.per_wg;
+  const int64_t b1 = min(b0 + sa.per_wg, a.g.nblocks);
+  const int half = pass_half();
   for (int64_t b = b0; b < b1; b++) {
     const BlockShape s = block_shape(a.g, b);
-    const int total = halo_commit(s, hn, l.hc, l.total);
-    const int units = (hn.rows + kTabLanes - 1) / kTabLanes;
-    Central ca;
-    const bool have = central_atom(s, l.hc, tab_slot(), ca);
-    int cnt = 0;
-    uint32_t E[kTabPL / 2];
-    if (have) {
-      cnt = a.nnb[ca.global];
-      list_fetch(a, ta.nmax, ca.global, h, units, E);
-    }
-    if (total <= a.cap) stage2(s, l.hc, total, a.pos, a.box[s.r].L, l.rec);  // always (cap = measured maximum)
-    else __syncthreads();
-    if (b + 1 < b1) halo_fetch(a.g, a.cell_off, ta.brick_rows, b + 1, hn);
-    if (total <= a.cap) {
-      for (int q = tab_slot(); have; q += kTabAtoms) {
-        if (q != tab_slot()) {  // atoms past the first pass (bricks of more than kTabAtoms atoms)
-          if (!central_atom(s, l.hc, q, ca)) break;
-          cnt = a.nnb[ca.global];
-          list_fetch(a, ta.nmax, ca.global, h, units, E);
+    const int total = build_halo(a.g, s, a.cell_off, hc, total_p);
+    if (total <= a.cap) {  // always (cap = measured maximum)
+      stage(s, hc, total, a.pos, a.box[s.r].L, nullptr, rec, nullptr);
+      for (int q = pass_slot();; q += kPassAtoms) {
+        Central ca;
+        if (!central_atom(s, hc, q, ca)) break;
+        const Pos4 pi = rec.get(ca.local);
+        const float fpi = (float)pi.w;
+        const int cnt = a.nnb[ca.global];
+        double fx = 0.0, fy = 0.0, fz = 0.0, e = 0.0;
+        double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        int nxt[kBBatch];
+        plist<kBBatch>(a, ca.global, half, cnt, nxt);
+        for (int k0 = half; k0 < cnt; k0 += 2 * kBBatch) {
+          int cur[kBBatch];
+          plist_use<kBBatch>(nxt, ca.local, k0, cnt, cur);
+          LNb nb[kBBatch];
+          pgather<kBBatch>(a, rec, cur, k0, cnt, pi, nb);
+          float rr[kBBatch], rinv[kBBatch], pp[kBBatch];
+          pm::Q4<float> rc[kBBatch], zc[kBBatch];
+#pragma unroll
+          for (int u = 0; u < kBBatch; u++) {
+            int m;
+            // masked entries are evaluated at rc, whose knot is in LDS
+            pm::dist((float)(nb[u].in ? nb[u].d2 : a.rc2), rr[u], rinv[u]);
+            pm::knot_r(t, rr[u], m, pp[u]);
+            zc[u] = pm::load4(pair + (int64_t)m * 8 + 4);
+            if (m >= sa.m_lo) {
+              const int k = m - sa.m_lo;
+              rc[u] = {c3[k], c4[k], c5[k], 0.0f};
+            } else {
+              rc[u] = pm::load4(pair + (int64_t)m * 8);
+            }
+          }
+          // the next batch's list loads go out after this batch's row loads, so that
+          // waiting for the rows (in issue order) does not wait for them
+          __builtin_amdgcn_sched_barrier(0);
+          plist<kBBatch>(a, ca.global, k0 + 2 * kBBatch, cnt, nxt);
+          __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+          for (int u = 0; u < kBBatch; u++) {
+            float fp_r, phi;
+            pm::eam_pair_rows(t, rc[u], zc[u], pp[u], rinv[u], fpi, (float)nb[u].w, fp_r, phi);
+            const double fpair = nb[u].in ? (double)fp_r : 0.0;
+            const double fe0 = nb[u].x * fpair, fe1 = nb[u].y * fpair, fe2 = nb[u].z * fpair;
+            fx += fe0;
+            fy += fe1;
+            fz += fe2;
+            if constexpr (ENERGY) e += nb[u].in ? 0.5 * (double)phi : 0.0;
+            if constexpr (VIRIAL) pm::add_virial(v, fe0, fe1, fe2, nb[u].x, nb[u].y, nb[u].z);
+          }
         }
-        const Pos4 pi = l.rec.get(ca.local);
-        double rho = tab_rho_entries<kTBatch>(ta, t, pair, l.tab, l.rec, ca, pi, h, cnt, E);
-        rho = group_sum(rho);
-        if (h == 0) {
+        fx = half_sum(fx);
+        fy = half_sum(fy);
+        fz = half_sum(fz);
+        if constexpr (ENERGY) e = half_sum(e);
+        if constexpr (VIRIAL) {
+#pragma unroll
+          for (int k = 0; k < 9; k++) v[k] = half_sum(v[k]);
+        }
+        if (half == 0) {
+          if constexpr (DRIFT) pm::write_outputs_drift(o, dr, ca.global, pi, fx, fy, fz);
+          else pm::write_outputs<ENERGY, VIRIAL, FUSED>(o, ca.global, fx, fy, fz, e, v);
+        }
+      }
+    }
+    __syncthreads();  // this brick's LDS is read by every lane before the next is staged
+  }
+}
+
+// Density pass with the rho value rows of the knots nearest rc in LDS (16-byte
+// rows), persistent like the split force pass: the records (three 8-byte reads)
+// and the table row (one 16-byte read) of a pair all come from LDS.
+template <bool EMB, bool ENERGY, bool SET_EP>
+__global__ void __launch_bounds__(kSplitThreads, 4)
+    k_split_eam_rho(SplitArgs sa, EamTabDev<float> t, const float* __restrict__ pair, double* __restrict__ rho_io,
+                    double* __restrict__ pos_w, double* __restrict__ ep)
+{
+  const BlockArgs& a = sa.a;
+  extern __shared__ __align__(32) unsigned char smem[];
+  float4* rows = reinterpret_cast<float4*>(smem);
+  unsigned char* base = smem + (size_t)sa.rows * sizeof(float4);
+  LRec rec;
+  HaloCell* hc = reinterpret_cast<HaloCell*>(base + lrec_at(base, a.cap, false, rec));
+  int* total_p = reinterpret_cast<int*>(hc + a.g.nh_max);
+  for (int i = threadIdx.x; i < sa.rows; i += blockDim.x)
+    rows[i] = reinterpret_cast<const float4*>(pair)[(int64_t)(sa.m_lo + i) * 2];
+  const int64_t b0 = lblock() * sa.per_wg;
+  const int64_t b1 = min(b0 + sa.per_wg, a.g.nblocks);
+  const int half = pass_half();
+  for (int64_t b = b0; b < b1; b++) {
+    const BlockShape s = block_shape(a.g, b);
+    const int total = build_halo(a.g, s, a.cell_off, hc, total_p);
+    if (total <= a.cap) {  // always (cap = measured maximum)
+      stage(s, hc, total, a.pos, a.box[s.r].L, nullptr, rec, nullptr);
+      for (int q = pass_slot();; q += kPassAtoms) {
+        Central ca;
+        if (!central_atom(s, hc, q, ca)) break;
+        const Pos4 pi = rec.get(ca.local);
+        const int cnt = a.nnb[ca.global];
+        double rho = 0.0;
+        int nxt[kBBatch];
+        plist<kBBatch>(a, ca.global, half, cnt, nxt);
+        for (int k0 = half; k0 < cnt; k0 += 2 * kBBatch) {
+          int cur[kBBatch];
+          plist_use<kBBatch>(nxt, ca.local, k0, cnt, cur);
+          plist<kBBatch>(a, ca.global, k0 + 2 * kBBatch, cnt, nxt);
+          LNb nb[kBBatch];
+          pgather<kBBatch>(a, rec, cur, k0, cnt, pi, nb);
+#pragma unroll
+          for (int u = 0; u < kBBatch; u++) {
+            float r, rinv, p;
+            int m;
+            pm::dist((float)(nb[u].in ? nb[u].d2 : a.rc2), r, rinv);
+            pm::knot_r(t, r, m, p);
+            pm::Q4<float> rc;
+            if (m >= sa.m_lo) {
+              const float4 q4 = rows[m - sa.m_lo];
+              rc = {q4.x, q4.y, q4.z, q4.w};
+            } else {
+              rc = pm::load4(pair + (int64_t)m * 8);
+            }
+            const float val = pm::cubic(rc, p);
+            rho += nb[u].in ? (double)val : 0.0;
+          }
+        }
+        rho = half_sum(rho);
+        if (half == 0) {
           rho_io[ca.global] = rho;
           if constexpr (EMB) {
             float F, Fp;
@@ -922,64 +778,6 @@ __global__ void __launch_bounds__(kTabThreads) k_tab_eam_rho(TabArgs ta, EamTabD
               else ep[ca.global] += (double)F;
             }
           }
-        }
-      }
-    }
-    __syncthreads();  // this brick's LDS is read by every lane before the next is staged
-  }
-}
-
-template <bool ENERGY, bool VIRIAL, bool FUSED, bool DRIFT>
-__global__ void __launch_bounds__(kTabThreads) k_tab_eam_force(TabArgs ta, EamTabDev<float> t,
-                                                               const float* __restrict__ pair, pm::Outputs o,
-                                                               pm::Drift dr)
-{
-  static_assert(!DRIFT || (FUSED && !ENERGY && !VIRIAL), "the drift epilogue is the plain MD-loop step");
-  const BlockArgs& a = ta.a;
-  const TabLayout l = tab_layout(ta.rows, 8, a.cap, a.g.nh_max);
-  for (int i = threadIdx.x; i < 2 * ta.rows; i += blockDim.x)
-    reinterpret_cast<float4*>(l.tab)[i] = reinterpret_cast<const float4*>(pair)[(int64_t)ta.m_lo * 2 + i];
-  const int64_t b0 = lblock() * ta.per_wg;
-  const int64_t b1 = min(b0 + ta.per_wg, a.g.nblocks);
-  const int h = tab_h();
-  HaloNext hn;
-  if (b0 < b1) halo_fetch(a.g, a.cell_off, ta.brick_rows, b0, hn);
-  for (int64_t b = b0; b < b1; b++) {
-    const BlockShape s = block_shape(a.g, b);
-    const int total = halo_commit(s, hn, l.hc, l.total);
-    const int units = (hn.rows + kTabLanes - 1) / kTabLanes;
-    Central ca;
-    const bool have = central_atom(s, l.hc, tab_slot(), ca);
-    int cnt = 0;
-    uint32_t E[kTabPL / 2];
-    if (have) {
-      cnt = a.nnb[ca.global];
-      list_fetch(a, ta.nmax, ca.global, h, units, E);
-    }
-    if (total <= a.cap) stage2(s, l.hc, total, a.pos, a.box[s.r].L, l.rec);  // always (cap = measured maximum)
-    else __syncthreads();
-    if (b + 1 < b1) halo_fetch(a.g, a.cell_off, ta.brick_rows, b + 1, hn);
-    if (total <= a.cap) {
-      for (int q = tab_slot(); have; q += kTabAtoms) {
-        if (q != tab_slot()) {
-          if (!central_atom(s, l.hc, q, ca)) break;
-          cnt = a.nnb[ca.global];
-          list_fetch(a, ta.nmax, ca.global, h, units, E);
-        }
-        const Pos4 pi = l.rec.get(ca.local);
-        ForceSums fs{0.0, 0.0, 0.0, 0.0, {0, 0, 0, 0, 0, 0, 0, 0, 0}};
-        tab_force_entries<kTBatch, ENERGY, VIRIAL>(ta, t, pair, l.tab, l.rec, ca, pi, h, cnt, E, fs);
-        fs.fx = group_sum(fs.fx);
-        fs.fy = group_sum(fs.fy);
-        fs.fz = group_sum(fs.fz);
-        if constexpr (ENERGY) fs.e = group_sum(fs.e);
-        if constexpr (VIRIAL) {
-#pragma unroll
-          for (int k = 0; k < 9; k++) fs.v[k] = group_sum(fs.v[k]);
-        }
-        if (h == 0) {
-          if constexpr (DRIFT) pm::write_outputs_drift(o, dr, ca.global, pi, fs.fx, fs.fy, fs.fz);
-          else pm::write_outputs<ENERGY, VIRIAL, FUSED>(o, ca.global, fs.fx, fs.fy, fs.fz, fs.e, fs.v);
         }
       }
     }
@@ -1014,31 +812,41 @@ void drift_commit(Ctx& c)
   c.drift_done = true;
 }
 
-// LDS plan of the persistent passes: rows of the table that fit beside the halo.
-struct TabPlan {
+// LDS plan of a split pass: table rows (row_bytes each) that fit beside the halo
+// records when two workgroups share a CU.
+struct SplitPlan {
   bool ok;
   int m_lo, rows;
   size_t bytes;
 };
 
-TabPlan tab_plan(const Ctx& c, int row_floats)
+SplitPlan split_plan(const Ctx& c, size_t row_bytes, bool with_w)
 {
-  TabPlan tp{false, 0, 0, 0};
-  if (!c.lds_table_enabled || c.eam.nel != 1 || c.num_cu <= 0 || c.bcap > 2 * kTabThreads) return tp;
-  const size_t halo = (size_t)c.bcap * sizeof(Pos4) + (size_t)c.bgeom.nh_max * sizeof(HaloCell) + 4 * sizeof(int);
-  if (halo >= c.lds_max) return tp;
-  const size_t row_bytes = (size_t)row_floats * sizeof(float);
+  SplitPlan sp{false, 0, 0, 0};
+  if (!c.split_enabled || c.eam.nel != 1 || c.num_cu <= 0) return sp;
+  const size_t budget = c.lds_max / 2;
+  const size_t halo = block_lds_bytes(c, false, with_w);
+  if (halo + 16 >= budget) return sp;
   const int nr = c.eam.nr;
-  int rows = (int)std::min<size_t>((size_t)nr, (c.lds_max - halo) / row_bytes);
-  if (c.lds_table_rows > 0) rows = std::min(rows, c.lds_table_rows);
+  int rows = (int)std::min<size_t>((size_t)nr, (budget - halo - 16) / row_bytes);
+  if (c.split_rows > 0) rows = std::min(rows, c.split_rows);
   // short of a quarter of the table, the HBM fallback would carry too many pairs
-  if (rows < std::max(1, nr / 4)) return tp;
-  tp.ok = true;
-  tp.rows = rows;
-  tp.m_lo = nr - rows;
-  tp.bytes = (size_t)rows * row_bytes + halo;
-  return tp;
+  if (rows < std::max(1, nr / 4)) return sp;
+  sp.ok = true;
+  sp.rows = rows;
+  sp.m_lo = nr - rows;
+  sp.bytes = ((size_t)rows * row_bytes + 15) / 16 * 16 + halo;
+  return sp;
 }
+
+unsigned split_grid(const Ctx& c, int64_t nblocks)
+{
+  const int64_t want = 2 * (int64_t)c.num_cu;
+  return (unsigned)std::max<int64_t>(8, (std::min<int64_t>(want, nblocks) + 7) / 8 * 8);
+}
+
+// Split-table density pass; false when it does not apply (the caller falls back).
+bool split_rho_dispatch(Ctx& c, const BlockArgs& a, bool do_emb, unsigned flags, const ForceEpilogue& epi);
 
 template <class K>
 void set_lds_limit(K kernel, size_t bytes)
@@ -1047,71 +855,76 @@ void set_lds_limit(K kernel, size_t bytes)
                              (int)bytes));
 }
 
-// Persistent LDS-table passes; false when the table does not fit (caller falls back).
-bool tab_eam_dispatch(Ctx& c, const BlockArgs& a, bool do_rho, bool do_emb, bool do_force, unsigned flags,
-                      const ForceEpilogue& epi)
+// Split-table force pass; false when it does not apply (the caller falls back).
+bool split_force_dispatch(Ctx& c, const BlockArgs& a, unsigned flags, const ForceEpilogue& epi)
 {
-  const TabPlan pr = tab_plan(c, 4), pf = tab_plan(c, 8);
-  if ((do_rho && !pr.ok) || (do_force && !pf.ok)) return false;
-  if (do_rho) c.last_rho_rows = pr.rows;
-  if (do_force) c.last_force_rows = pf.rows;
+  const SplitPlan sp = split_plan(c, 3 * sizeof(float), true);
+  if (!sp.ok) return false;
+  c.last_force_rows = sp.rows;
   const EamTabDev<float> t = make_eam_tab<float>(c);
   const float* pair = eam_pair_table<float>(c);
-  const unsigned g = (unsigned)std::max<int64_t>(8, std::min<int64_t>((c.num_cu + 7) / 8 * 8,
-                                                                         (a.g.nblocks + 7) / 8 * 8));
-  TabArgs ta;
-  ta.a = a;
-  ta.brick_rows = c.brick_rows.p;
-  ta.nmax = c.nmax;
-  ta.per_wg = (a.g.nblocks + g - 1) / g;
+  const unsigned g = split_grid(c, a.g.nblocks);
+  SplitArgs sa;
+  sa.a = a;
+  sa.m_lo = sp.m_lo;
+  sa.rows = sp.rows;
+  sa.per_wg = (a.g.nblocks + g - 1) / g;
+  const bool energy = flags & XS_ENERGY, vir = flags & XS_VIRIAL;
+  const pm::Outputs o = make_outputs(c, epi);
+  pm::Drift dr{};
+  const bool drift = drift_setup(c, epi, flags, dr);
+  ProfScope prof(c.prof, c.stream, XS_PROF_EAM_FORCE);
+#define XS_SF(EN, VI, FU, DR)                                                                        \
+  do {                                                                                               \
+    set_lds_limit(k_split_eam_force<EN, VI, FU, DR>, sp.bytes);                                      \
+    k_split_eam_force<EN, VI, FU, DR><<<g, kSplitThreads, sp.bytes, c.stream>>>(sa, t, pair, o, dr); \
+  } while (0)
+  if (drift) {
+    XS_SF(false, false, true, true);
+  } else if (epi.fused) {
+    if (energy && vir) XS_SF(true, true, true, false);
+    else if (energy) XS_SF(true, false, true, false);
+    else if (vir) XS_SF(false, true, true, false);
+    else XS_SF(false, false, true, false);
+  } else {
+    if (energy && vir) XS_SF(true, true, false, false);
+    else if (energy) XS_SF(true, false, false, false);
+    else if (vir) XS_SF(false, true, false, false);
+    else XS_SF(false, false, false, false);
+  }
+#undef XS_SF
+  XS_HIP(hipGetLastError());
+  if (drift) drift_commit(c);
+  return true;
+}
+
+bool split_rho_dispatch(Ctx& c, const BlockArgs& a, bool do_emb, unsigned flags, const ForceEpilogue& epi)
+{
+  const SplitPlan sp = split_plan(c, sizeof(float4), false);
+  if (!sp.ok) return false;
+  c.last_rho_rows = sp.rows;
+  const EamTabDev<float> t = make_eam_tab<float>(c);
+  const float* pair = eam_pair_table<float>(c);
+  const unsigned g = split_grid(c, a.g.nblocks);
+  SplitArgs sa;
+  sa.a = a;
+  sa.m_lo = sp.m_lo;
+  sa.rows = sp.rows;
+  sa.per_wg = (a.g.nblocks + g - 1) / g;
   const bool energy = flags & XS_ENERGY;
   double* pos_w = reinterpret_cast<double*>(c.pos.p);
-  if (do_rho) {
-    ta.m_lo = pr.m_lo;
-    ta.rows = pr.rows;
-    ProfScope prof(c.prof, c.stream, XS_PROF_EAM_RHO);
-#define XS_TRHO(EMB, EN, SET)                                                                               \
-  do {                                                                                                      \
-    set_lds_limit(k_tab_eam_rho<EMB, EN, SET>, pr.bytes);                                                   \
-    k_tab_eam_rho<EMB, EN, SET><<<g, kTabThreads, pr.bytes, c.stream>>>(ta, t, pair, c.rho.p, pos_w, c.ep.p); \
+  ProfScope prof(c.prof, c.stream, XS_PROF_EAM_RHO);
+#define XS_SR(EMB, EN, SET)                                                                                  \
+  do {                                                                                                       \
+    set_lds_limit(k_split_eam_rho<EMB, EN, SET>, sp.bytes);                                                  \
+    k_split_eam_rho<EMB, EN, SET><<<g, kSplitThreads, sp.bytes, c.stream>>>(sa, t, pair, c.rho.p, pos_w, c.ep.p); \
   } while (0)
-    if (!do_emb) XS_TRHO(false, false, false);
-    else if (energy && epi.fused) XS_TRHO(true, true, true);
-    else if (energy) XS_TRHO(true, true, false);
-    else XS_TRHO(true, false, false);
-#undef XS_TRHO
-    XS_HIP(hipGetLastError());
-  }
-  if (do_force) {
-    ta.m_lo = pf.m_lo;
-    ta.rows = pf.rows;
-    const pm::Outputs o = make_outputs(c, epi);
-    const bool vir = flags & XS_VIRIAL;
-    pm::Drift dr{};
-    const bool drift = drift_setup(c, epi, flags, dr);
-    ProfScope prof(c.prof, c.stream, XS_PROF_EAM_FORCE);
-#define XS_TF(EN, VI, FU, DR)                                                                    \
-  do {                                                                                           \
-    set_lds_limit(k_tab_eam_force<EN, VI, FU, DR>, pf.bytes);                                    \
-    k_tab_eam_force<EN, VI, FU, DR><<<g, kTabThreads, pf.bytes, c.stream>>>(ta, t, pair, o, dr); \
-  } while (0)
-    if (drift) {
-      XS_TF(false, false, true, true);
-    } else if (epi.fused) {
-      if (energy && vir) XS_TF(true, true, true, false);
-      else if (energy) XS_TF(true, false, true, false);
-      else if (vir) XS_TF(false, true, true, false);
-      else XS_TF(false, false, true, false);
-    } else {
-      if (energy && vir) XS_TF(true, true, false, false);
-      else if (energy) XS_TF(true, false, false, false);
-      else if (vir) XS_TF(false, true, false, false);
-      else XS_TF(false, false, false, false);
-    }
-#undef XS_TF
-    XS_HIP(hipGetLastError());
-    if (drift) drift_commit(c);
-  }
+  if (!do_emb) XS_SR(false, false, false);
+  else if (energy && epi.fused) XS_SR(true, true, true);
+  else if (energy) XS_SR(true, true, false);
+  else XS_SR(true, false, false);
+#undef XS_SR
+  XS_HIP(hipGetLastError());
   return true;
 }
 
@@ -1132,19 +945,20 @@ void block_eam_dispatch(Ctx& c, double rcut, bool do_rho, bool do_emb, bool do_f
   a.npad = c.npad;
   a.cap = c.bcap;
   a.rc2 = rcut * rcut;
-  if constexpr (std::is_same_v<Real, float> && !MULTI) {
-    if (tab_eam_dispatch(c, a, do_rho, do_emb, do_force, flags, epi)) return;
-  }
-  if (do_rho) c.last_rho_rows = 0;
-  if (do_force) c.last_force_rows = 0;
-  const size_t lds = block_lds_bytes(c, MULTI);
+  const size_t lds_rho = block_lds_bytes(c, MULTI, false);
+  const size_t lds = block_lds_bytes(c, MULTI, true);
   const unsigned g = block_grid(c);
   const bool energy = flags & XS_ENERGY;
   double* pos_w = reinterpret_cast<double*>(c.pos.p);
-  if (do_rho) {
+  bool rho_done = false;
+  if constexpr (std::is_same_v<Real, float> && !MULTI) {
+    if (do_rho) rho_done = split_rho_dispatch(c, a, do_emb, flags, epi);
+  }
+  if (do_rho && !rho_done) {
+    c.last_rho_rows = 0;
     ProfScope prof(c.prof, c.stream, XS_PROF_EAM_RHO);
 #define XS_BRHO(EMB, EN, SET) \
-  k_block_eam_rho<Real, MULTI, EMB, EN, SET><<<g, kPassThreads, lds, c.stream>>>(a, t, pair, c.rho.p, pos_w, c.ep.p)
+  k_block_eam_rho<Real, MULTI, EMB, EN, SET><<<g, kPassThreads, lds_rho, c.stream>>>(a, t, pair, c.rho.p, pos_w, c.ep.p)
     if (!do_emb) XS_BRHO(false, false, false);
     else if (energy && epi.fused) XS_BRHO(true, true, true);
     else if (energy) XS_BRHO(true, true, false);
@@ -1153,6 +967,10 @@ void block_eam_dispatch(Ctx& c, double rcut, bool do_rho, bool do_emb, bool do_f
     XS_HIP(hipGetLastError());
   }
   if (do_force) {
+    if constexpr (std::is_same_v<Real, float> && !MULTI) {
+      if (split_force_dispatch(c, a, flags, epi)) return;
+    }
+    c.last_force_rows = 0;
     const pm::Outputs o = make_outputs(c, epi);
     const bool vir = flags & XS_VIRIAL;
     pm::Drift dr{};
@@ -1228,15 +1046,15 @@ void block_setup(Ctx& c)
   c.bcentral = (int)c.hscalar[3];
 }
 
-size_t block_lds_bytes(const Ctx& c, bool with_type)
+size_t block_lds_bytes(const Ctx& c, bool with_type, bool with_w)
 {
-  return (size_t)c.bcap * sizeof(Pos4) + (size_t)c.bgeom.nh_max * sizeof(HaloCell) + 4 * sizeof(int) +
-         (with_type ? (size_t)c.bcap : 0);
+  return (size_t)c.bcap * sizeof(double) * (with_w ? 4 : 3) + (size_t)c.bgeom.nh_max * sizeof(HaloCell) +
+         4 * sizeof(int) + (with_type ? (size_t)c.bcap : 0);
 }
 
 bool block_feasible(const Ctx& c)
 {
-  return c.bcap > 0 && c.bcap < 65536 && block_lds_bytes(c, true) <= kBlockLdsLimit;
+  return c.bcap > 0 && c.bcap < 65536 && block_lds_bytes(c, true, true) <= kBlockLdsLimit;
 }
 
 unsigned block_grid(const Ctx& c)
@@ -1246,7 +1064,7 @@ unsigned block_grid(const Ctx& c)
 
 void block_build(Ctx& c, double rl2)
 {
-  const size_t lds = block_lds_bytes(c, false);
+  const size_t lds = block_lds_bytes(c, false, false);
   for (int attempt = 0; attempt < 2; attempt++) {
     c.nbr16.reserve((size_t)c.nmax * c.npad);
     c.nnb.reserve(c.npad);
@@ -1271,7 +1089,7 @@ void block_build(Ctx& c, double rl2)
 void block_to_global(Ctx& c)
 {
   c.nbr.reserve((size_t)c.nmax * c.npad);
-  const size_t lds = block_lds_bytes(c, false) - (size_t)c.bcap * sizeof(Pos4);
+  const size_t lds = block_lds_bytes(c, false, false) - (size_t)c.bcap * 3 * sizeof(double);
   k_block_to_global<<<block_grid(c), kBlock, lds, c.stream>>>(c.bgeom, c.cell_off.p, c.nbr16.p, c.nnb.p, c.nbr.p,
                                                               c.npad, c.bgeom.nh_max);
   XS_HIP(hipGetLastError());

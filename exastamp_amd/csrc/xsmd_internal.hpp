@@ -276,13 +276,14 @@ struct Ctx {
   int bcap = 0;                    // LDS slots per block (max halo atoms, rounded)
   int bcentral = 0;                // max central atoms per block
   DBuf<int> brick_rows;            // longest list of each block
-  // persistent LDS-table passes: CUs and LDS per workgroup of the device;
-  // XSMD_LDS_TABLE=1 enables them, XSMD_LDS_ROWS caps the table rows in LDS
+  // split-table passes (rho rows in LDS, persistent workgroups): CUs and LDS per
+  // workgroup of the device; XSMD_SPLIT_TABLE=1 enables them, XSMD_SPLIT_ROWS caps
+  // the rows in LDS
   int num_cu = 0;
   size_t lds_max = 65536;
-  bool lds_table_enabled = false;
-  int lds_table_rows = 0;
-  int last_rho_rows = 0, last_force_rows = 0;  // LDS table rows of the last EAM passes (0: not used)
+  bool split_enabled = false;
+  int split_rows = 0;
+  int last_rho_rows = 0, last_force_rows = 0;  // table rows in LDS of the last EAM passes (0: none)
   bool drift_done = false;  // the last force launch applied ForceEpilogue::drift
 
   // small device scalars + pinned host mirror
@@ -314,7 +315,7 @@ void build_neighbor_list(Ctx& c, double rcut, double skin, bool prefer_block = f
 void cell_offsets(Ctx& c);
 void block_setup(Ctx& c);
 bool block_feasible(const Ctx& c);
-size_t block_lds_bytes(const Ctx& c, bool with_type);
+size_t block_lds_bytes(const Ctx& c, bool with_type, bool with_w);
 unsigned block_grid(const Ctx& c);
 void block_build(Ctx& c, double rl2);
 void block_to_global(Ctx& c);

@@ -309,7 +309,7 @@ class Simulation:
         return {"kind": ("global", "block")[kind.value], "blocks": blocks.value, "halo_capacity": cap.value}
 
     def eam_pass_info(self) -> dict:
-        """Spline-table rows the last EAM density / force passes kept in LDS (0: HBM)."""
+        """Table rows the last EAM density / force passes kept in LDS (0: all from HBM)."""
         r, f = C.c_int(0), C.c_int(0)
         self.ctx.xs_eam_pass_info(C.byref(r), C.byref(f))
         return {"rho_table_rows": r.value, "force_table_rows": f.value}
