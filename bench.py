@@ -31,10 +31,14 @@ FP32_PEAK_TFLOPS = 157.3
 FP64_PEAK_TFLOPS = 78.6
 # Algorithmic per-atom figures of the EAM force pass as fused here (force pass +
 # force_to_accel + half kick): bytes of the compulsory FP64 state stream (read r,
-# F', v; write a, v) and FLOPs per interacting pair / per atom.
+# F', v; write a, v) and FLOPs per interacting pair / per atom.  In the MD loop the
+# pass also applies the next step's drift and kick (positions written, reference
+# positions of the displacement test read): 48 B more, 18 FLOP more.
 FORCE_BYTES_PER_ATOM = 104.0
+DRIFT_BYTES_PER_ATOM = 48.0
 FORCE_FLOPS_PER_PAIR = 49.0
 FORCE_FLOPS_PER_ATOM = 3.0 + 6.0
+DRIFT_FLOPS_PER_ATOM = 12.0 + 6.0
 
 
 def parse():
@@ -49,6 +53,8 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=12)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--stream-copy-gib", type=float, default=0.0,
+                    help="also time a plain HBM copy of this many GiB (the counter calibration kernel)")
     ap.add_argument("--seed", type=int, default=2024)
     return ap.parse_args()
 
@@ -302,17 +308,28 @@ def main():
     # roofline of the dominant kernel (EAM force pass, mixed)
     kf = mixed["kernels"]["eam_force"]
     kname = force_kernel_name(list_info, mixed["pass_info"])
+    drift_fused = kname.endswith(", true>")
+    bytes_atom = FORCE_BYTES_PER_ATOM + (DRIFT_BYTES_PER_ATOM if drift_fused else 0.0)
+    flops_atom = FORCE_FLOPS_PER_ATOM + (DRIFT_FLOPS_PER_ATOM if drift_fused else 0.0)
     avg_s = kf["ms_avg"] * 1e-3
-    gbs = FORCE_BYTES_PER_ATOM * cfg.n / avg_s / 1e9
-    tflops = (FORCE_FLOPS_PER_PAIR * pairs_per_atom + FORCE_FLOPS_PER_ATOM) * cfg.n / avg_s / 1e12
+    gbs = bytes_atom * cfg.n / avg_s / 1e9
+    tflops = (FORCE_FLOPS_PER_PAIR * pairs_per_atom + flops_atom) * cfg.n / avg_s / 1e12
     roof = {
-        "kernel": kname + " (EAM force pass + force_to_accel + half kick)",
+        "kernel": kname + " (EAM force pass + force_to_accel + half kick"
+                  + (" + next-step drift/kick)" if drift_fused else ")"),
         "bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
         "traffic": measured_traffic(kname),
-        "bytes_per_atom": FORCE_BYTES_PER_ATOM, "avg_launch_ms": kf["ms_avg"],
+        "bytes_per_atom": bytes_atom, "avg_launch_ms": kf["ms_avg"],
         "valu": {"achieved_tflops": tflops, "peak_tflops": FP32_PEAK_TFLOPS, "frac": tflops / FP32_PEAK_TFLOPS,
                  "flops_per_pair": FORCE_FLOPS_PER_PAIR, "pairs_per_atom": pairs_per_atom},
     }
+
+    stream = None
+    if args.stream_copy_gib > 0:
+        g = C.c_double(0.0)
+        sim.ctx.xs_stream_copy(int(args.stream_copy_gib * (1 << 30)), 3, C.byref(g))
+        stream = {"gib": args.stream_copy_gib, "gbs": g.value}
+        log(f"stream copy: {g.value:.1f} GB/s")
 
     cpu = None
     if not args.no_cpu:
@@ -370,6 +387,7 @@ def main():
         "kernels_mixed": mixed["kernels"],
         "kernels_fp64": None if fp64 is None else fp64["kernels"],
         "roofline": roof,
+        "stream_copy": stream,
         "cpu_baseline": cpu,
         "adjoint": adjoint,
     }
