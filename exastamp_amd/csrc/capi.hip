@@ -180,6 +180,11 @@ xs_status xs_ctx_create(int device, xs_ctx** out)
                                                                        : prop.sharedMemPerBlock;
     if (const char* st = std::getenv("XSMD_SPLIT_TABLE")) c->split_enabled = std::strcmp(st, "1") == 0;
     if (const char* sr = std::getenv("XSMD_SPLIT_ROWS")) c->split_rows = std::max(0, std::atoi(sr));
+    if (const char* ls = std::getenv("XSMD_LIST_SORT")) c->list_sort_disabled = std::strcmp(ls, "0") == 0;
+    if (const char* sb = std::getenv("XSMD_SORT_BITS")) {
+      const int v = std::atoi(sb);
+      if (v >= 1 && v <= 15) c->sort_bits = v;
+    }
     if (const char* ba = std::getenv("XSMD_BLOCK_ATOMS")) {
       const double v = std::atof(ba);
       if (v >= 1.0 && v <= 4096.0) c->block_atoms = v;

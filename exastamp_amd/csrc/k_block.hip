@@ -221,6 +221,7 @@ struct LdsLayout {
   HaloCell* hc;
   uint8_t* ltype;
   int* total;
+  unsigned char* end;  // first byte past the layout, 16-byte aligned
 };
 
 // LDS of the block kernels: records, halo table, counters, per-record types
@@ -232,6 +233,8 @@ __device__ __forceinline__ LdsLayout lds_layout(int cap, int nh_max, bool with_t
   l.hc = reinterpret_cast<HaloCell*>(smem + rb);
   l.total = reinterpret_cast<int*>(l.hc + nh_max);
   l.ltype = with_type ? reinterpret_cast<uint8_t*>(l.total + 4) : nullptr;
+  const size_t used = rb + (size_t)nh_max * sizeof(HaloCell) + 4 * sizeof(int) + (with_type ? (size_t)cap : 0);
+  l.end = smem + (used + 15) / 16 * 16;
   return l;
 }
 
@@ -250,6 +253,66 @@ __device__ __forceinline__ int pass_half() { return (threadIdx.x >> 5) & 1; }
 // ---------------------------------------------------------------------------
 // list build
 // ---------------------------------------------------------------------------
+// number of central atoms of a brick (the row walk of central_atom)
+__device__ __forceinline__ int central_count(const BlockShape& s, const HaloCell* hc)
+{
+  int n = 0;
+  for (int iz = 1; iz <= s.ez; iz++)
+    for (int iy = 1; iy <= s.ey; iy++) {
+      const int h0 = (iz * s.hy + iy) * s.hx + 1;
+      const int h1 = h0 + s.ex - 1;
+      n += hc[h1].lstart + hc[h1].count - hc[h0].lstart;
+    }
+  return n;
+}
+
+// Stable sort of one 32-bit key per lane over the 64 lanes of a wave by key bits
+// [lo, lo + nbits): least-significant-digit radix sort, one bit per pass (ballot,
+// prefix popcounts, forward permute).  Keys whose sort bits are all ones go last.
+__device__ __forceinline__ uint32_t wave_radix_sort(uint32_t key, int lo, int nbits)
+{
+  const int lane = threadIdx.x & 63;
+  const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  for (int bit = lo; bit < lo + nbits; bit++) {
+    const bool one = (key >> bit) & 1u;
+    const unsigned long long zeros = __ballot(!one);
+    const int nz = __popcll(zeros);
+    const int dest = one ? nz + __popcll(~zeros & below) : __popcll(zeros & below);
+    key = (uint32_t)__builtin_amdgcn_ds_permute(dest << 2, (int)key);
+  }
+  return key;
+}
+
+// two independent sorts interleaved (the permutes of one wait on the LDS crossbar
+// while the other's ballots issue)
+__device__ __forceinline__ void wave_radix_sort2(uint32_t& k0, uint32_t& k1, int lo, int nbits)
+{
+  const int lane = threadIdx.x & 63;
+  const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  for (int bit = lo; bit < lo + nbits; bit++) {
+    const bool one0 = (k0 >> bit) & 1u, one1 = (k1 >> bit) & 1u;
+    const unsigned long long z0 = __ballot(!one0), z1 = __ballot(!one1);
+    const int d0 = one0 ? __popcll(z0) + __popcll(~z0 & below) : __popcll(z0 & below);
+    const int d1 = one1 ? __popcll(z1) + __popcll(~z1 & below) : __popcll(z1 & below);
+    k0 = (uint32_t)__builtin_amdgcn_ds_permute(d0 << 2, (int)k0);
+    k1 = (uint32_t)__builtin_amdgcn_ds_permute(d1 << 2, (int)k1);
+  }
+}
+
+// Lists of at most 64 entries are kept sorted by distance (see k_block_sort); this
+// many rows are all the sort touches.
+constexpr int kSortRows = 64;
+
+// sort key of slot c for central slot i: distance bin (nbits) above the slot
+__device__ __forceinline__ uint32_t dist_key(const LRec& rec, int c, int i, float inv_rl, int nbits)
+{
+  const double dx = rec.x[c] - rec.x[i], dy = rec.y[c] - rec.y[i], dz = rec.z[c] - rec.z[i];
+  const float r = sqrtf((float)xsmd_d2(dx, dy, dz));
+  const int nb = 1 << nbits;
+  const uint32_t bin = (uint32_t)min(nb - 1, (int)(r * inv_rl * (float)nb));
+  return (bin << 16) | (uint32_t)c;
+}
+
 // Squared distance from p to the box of the halo cell at unwrapped grid coordinate
 // (gx, gy, gz), widened by a margin that covers the rounding of cell assignment.
 // Along a non-periodic axis the edge cells also hold the atoms clamped into them,
@@ -328,7 +391,77 @@ __global__ void __launch_bounds__(kPassThreads) k_block_build(BlockGeom g, const
   if (threadIdx.x == 0) brick_rows[b] = l.total[1];
 }
 
-// block-local entries -> global (index, image) entries of the same list
+// Orders each atom's block-list entries by distance (bins of rl / 2^nbits, stencil
+// order within a bin), for lists of at most kSortRows entries.  At a given list
+// position the atoms of a wave then look up nearby spline rows, which cuts the
+// cache lines each table gather of the EAM passes touches by about 40 %.  The lists
+// of a brick are staged through LDS (buf[row][atom]) so that global reads and writes
+// coalesce; one wave sorts two atoms at a time.
+constexpr int kSortThreads = 512;
+
+__global__ void __launch_bounds__(kSortThreads) k_block_sort(BlockGeom g, const int* __restrict__ cell_off,
+                                                             const Pos4* __restrict__ pos,
+                                                             const BoxDev* __restrict__ box,
+                                                             uint16_t* __restrict__ nbr16,
+                                                             const int* __restrict__ nnb, int64_t npad, int cap,
+                                                             int nh_max, int slots, float inv_rl, int nbits)
+{
+  const int64_t b = lblock();
+  if (b >= g.nblocks) return;
+  LdsLayout l = lds_layout(cap, nh_max, false, false);
+  int* gq = reinterpret_cast<int*>(l.end);  // per central atom: global index, local slot, count
+  int* lq = gq + slots;
+  int* cq = lq + slots;
+  uint16_t* buf = reinterpret_cast<uint16_t*>(cq + slots);  // [row][slots]
+  const BlockShape s = block_shape(g, b);
+  const int total = build_halo(g, s, cell_off, l.hc, l.total);
+  if (total > cap) return;  // excluded by the host (cap = measured maximum)
+  const int na = central_count(s, l.hc);  // <= slots (the measured maximum)
+  for (int q = threadIdx.x; q < na; q += blockDim.x) {
+    Central ca;
+    central_atom(s, l.hc, q, ca);
+    gq[q] = (int)ca.global;
+    lq[q] = ca.local;
+    cq[q] = nnb[ca.global];
+  }
+  stage(s, l.hc, total, pos, box[s.r].L, nullptr, l.rec, nullptr);  // ends with a barrier
+  // thread t stages atom t % slots, rows t / slots, t / slots + step, ...:
+  // consecutive atoms are consecutive in memory within a cell row, so each row's
+  // reads (and the writes below) coalesce (slots is a power of two no larger than
+  // the workgroup)
+  const int step = kSortThreads / slots;
+  const int sq = threadIdx.x % slots, sk = threadIdx.x / slots;
+  const int scnt = sq < na ? cq[sq] : 0;
+  const bool sorts = scnt >= 2 && scnt <= kSortRows;
+  if (sorts) {
+#pragma unroll 8
+    for (int k = sk; k < scnt; k += step) buf[k * slots + sq] = nbr16[(int64_t)k * npad + gq[sq]];
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int nw = blockDim.x >> 6;
+  for (int q0 = threadIdx.x >> 6; q0 < na; q0 += 2 * nw) {
+    const int q1 = q0 + nw;
+    const int cnt0 = cq[q0], cnt1 = q1 < na ? cq[q1] : 0;
+    const bool s0 = cnt0 >= 2 && cnt0 <= kSortRows, s1 = cnt1 >= 2 && cnt1 <= kSortRows;
+    if (!s0 && !s1) continue;
+    uint32_t key0 = 0xffffffffu, key1 = 0xffffffffu;
+    if (s0 && lane < cnt0) key0 = dist_key(l.rec, buf[lane * slots + q0], lq[q0], inv_rl, nbits);
+    if (s1 && lane < cnt1) key1 = dist_key(l.rec, buf[lane * slots + q1], lq[q1], inv_rl, nbits);
+    wave_radix_sort2(key0, key1, 16, nbits);
+    if (s0 && lane < cnt0) buf[lane * slots + q0] = (uint16_t)(key0 & 0xffffu);
+    if (s1 && lane < cnt1) buf[lane * slots + q1] = (uint16_t)(key1 & 0xffffu);
+  }
+  __syncthreads();
+  if (sorts) {
+#pragma unroll 8
+    for (int k = sk; k < scnt; k += step) nbr16[(int64_t)k * npad + gq[sq]] = buf[k * slots + sq];
+  }
+}
+
+// Block-local entries -> global (index, image) entries of the same list, in the
+// stencil order of the global build: a distance-sorted list goes back to ascending
+// halo slot order (the stencil order) first.  One wave per central atom.
 __global__ void __launch_bounds__(256) k_block_to_global(BlockGeom g, const int* __restrict__ cell_off,
                                                          const uint16_t* __restrict__ nbr16,
                                                          const int* __restrict__ nnb, uint32_t* __restrict__ nbr,
@@ -340,15 +473,28 @@ __global__ void __launch_bounds__(256) k_block_to_global(BlockGeom g, const int*
   const BlockShape s = block_shape(g, b);
   const int nh = s.hx * s.hy * s.hz;
   build_halo(g, s, cell_off, l.hc, l.total);
-  for (int q = threadIdx.x;; q += blockDim.x) {
+  const int lane = threadIdx.x & 63;
+  for (int q = threadIdx.x >> 6;; q += blockDim.x >> 6) {
     Central a;
     if (!central_atom(s, l.hc, q, a)) break;
     const int cnt = nnb[a.global];
-    for (int k = 0; k < cnt; k++) {
-      const int c = nbr16[(int64_t)k * npad + a.global];
-      const HaloCell e = l.hc[find_cell(l.hc, nh, c)];
-      nbr[(int64_t)k * npad + a.global] =
-          (uint32_t)(e.gstart + (c - e.lstart)) | ((uint32_t)e.img << XSMD_NBR_INDEX_BITS);
+    if (cnt <= kSortRows) {
+      uint32_t key = 0xffffu;
+      if (lane < cnt) key = nbr16[(int64_t)lane * npad + a.global];
+      key = wave_radix_sort(key, 0, 16);
+      if (lane < cnt) {
+        const int c = (int)key;
+        const HaloCell e = l.hc[find_cell(l.hc, nh, c)];
+        nbr[(int64_t)lane * npad + a.global] =
+            (uint32_t)(e.gstart + (c - e.lstart)) | ((uint32_t)e.img << XSMD_NBR_INDEX_BITS);
+      }
+    } else {
+      for (int k = lane; k < cnt; k += 64) {
+        const int c = nbr16[(int64_t)k * npad + a.global];
+        const HaloCell e = l.hc[find_cell(l.hc, nh, c)];
+        nbr[(int64_t)k * npad + a.global] =
+            (uint32_t)(e.gstart + (c - e.lstart)) | ((uint32_t)e.img << XSMD_NBR_INDEX_BITS);
+      }
     }
   }
 }
@@ -1062,6 +1208,25 @@ unsigned block_grid(const Ctx& c)
   return (unsigned)((c.bgeom.nblocks + 7) / 8 * 8);
 }
 
+// Distance order of the block lists (k_block_sort); skipped when disabled or when
+// the staging does not fit (the lists then stay in stencil order).
+void block_sort(Ctx& c, double rl)
+{
+  if (c.list_sort_disabled || c.bgeom.nblocks == 0) return;
+  int slots = 64;
+  while (slots < c.bcentral) slots *= 2;
+  if (slots > kSortThreads) return;  // bricks this large keep the stencil order
+  const size_t lds = (block_lds_bytes(c, false, false) + 15) / 16 * 16 + (size_t)3 * slots * sizeof(int) +
+                     (size_t)kSortRows * slots * sizeof(uint16_t);
+  if (lds > c.lds_max) return;
+  XS_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_block_sort),
+                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  k_block_sort<<<block_grid(c), kSortThreads, lds, c.stream>>>(c.bgeom, c.cell_off.p, c.pos.p, c.box_d.p, c.nbr16.p,
+                                                               c.nnb.p, c.npad, c.bcap, c.bgeom.nh_max, slots,
+                                                               (float)(1.0 / rl), c.sort_bits);
+  XS_HIP(hipGetLastError());
+}
+
 void block_build(Ctx& c, double rl2)
 {
   const size_t lds = block_lds_bytes(c, false, false);
@@ -1071,16 +1236,19 @@ void block_build(Ctx& c, double rl2)
     c.brick_rows.reserve(c.bgeom.nblocks);
     XS_HIP(hipMemsetAsync(c.dscalar.p, 0, 2 * sizeof(unsigned long long), c.stream));
     k_block_build<<<block_grid(c), kPassThreads, lds, c.stream>>>(c.bgeom, c.cell_off.p, c.pos.p, c.box_d.p, rl2,
-                                                            c.nbr16.p, c.nnb.p, c.nmax, c.npad, c.bcap,
-                                                            c.bgeom.nh_max, c.dscalar.p + 4, c.brick_rows.p,
-                                                            c.dscalar.p);
+                                                                  c.nbr16.p, c.nnb.p, c.nmax, c.npad, c.bcap,
+                                                                  c.bgeom.nh_max, c.dscalar.p + 4, c.brick_rows.p,
+                                                                  c.dscalar.p);
     XS_HIP(hipGetLastError());
     XS_HIP(hipMemcpyAsync(c.hscalar, c.dscalar.p, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
                           c.stream));
     XS_HIP(hipStreamSynchronize(c.stream));
     if (c.hscalar[1] != 0) throw Error(XS_E_STATE, "block halo exceeds its measured capacity");
     const unsigned long long need = c.hscalar[0];
-    if (need == 0) return;
+    if (need == 0) {
+      block_sort(c, std::sqrt(rl2));
+      return;
+    }
     if (attempt == 1) throw Error(XS_E_STATE, "neighbour list overflow after resize");
     c.nmax = (int)((need + 7) / 8 * 8 + 8);
   }

@@ -283,6 +283,8 @@ struct Ctx {
   size_t lds_max = 65536;
   bool split_enabled = false;
   int split_rows = 0;
+  bool list_sort_disabled = false;  // XSMD_LIST_SORT=0: block lists stay in stencil order
+  int sort_bits = 7;                // distance bins of the list sort: 2^sort_bits over [0, rl)
   int last_rho_rows = 0, last_force_rows = 0;  // table rows in LDS of the last EAM passes (0: none)
   bool drift_done = false;  // the last force launch applied ForceEpilogue::drift
 

@@ -151,7 +151,7 @@ def test_block_adjoint_and_pair_count_use_converted_list(monkeypatch):
 
 
 def _env(monkeypatch, **kv):
-    for k in ("XSMD_LIST", "XSMD_SPLIT_TABLE", "XSMD_SPLIT_ROWS"):
+    for k in ("XSMD_LIST", "XSMD_SPLIT_TABLE", "XSMD_SPLIT_ROWS", "XSMD_LIST_SORT"):
         monkeypatch.delenv(k, raising=False)
     for k, v in kv.items():
         monkeypatch.setenv(k, v)
@@ -204,3 +204,24 @@ def test_split_table_md_run_vs_oracle_energy(monkeypatch, oracle_mod):
     _, th_ref = o.md_run("eam", 100, 0.001, tab.rc, 0.3)
     pe = np.array([t.potential_energy for t in th])
     assert np.abs(pe - th_ref[:, 24]).max() <= 1e-6 * np.abs(th_ref[:, 24]).max()
+
+
+@pytest.mark.parametrize("name", ["cu_jitter", "cuni_slab", "replicas"])
+def test_distance_sorted_lists(monkeypatch, name):
+    """Block lists are reordered by distance after the build (XSMD_LIST_SORT=0 keeps
+    the stencil order): the converted global list is the same entry by entry, and
+    the passes agree to summation order (FP64 1e-10, mixed 1e-6 of the rms)."""
+    cfg, masses, names, tab = _case(name)
+    sims = {}
+    for label, env in (("sorted", {}), ("stencil", {"XSMD_LIST_SORT": "0"})):
+        _env(monkeypatch, **env)
+        sim = make_sim(cfg, masses, names)
+        sim.eam_alloy_init(tab)
+        sim.chunk_neighbors(rcut_max=tab.rc, rcut_inc=0.3)
+        assert sim.list_info()["kind"] == "block"
+        sims[label] = sim
+    for x, y in zip(sims["sorted"].neighbors(), sims["stencil"].neighbors()):
+        assert np.array_equal(x, y)
+    for mixed in (False, True):
+        for x, y in zip(_outputs(sims["sorted"], tab.rc, mixed), _outputs(sims["stencil"], tab.rc, mixed)):
+            assert _close(x, y, 1e-6 if mixed else 1e-10)
