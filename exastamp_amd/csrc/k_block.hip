@@ -326,6 +326,17 @@ __device__ __forceinline__ double cell_box_d2(const BlockGeom& g, const BoxDev& 
   return xsmd_d2(ux, uy, uz);
 }
 
+// Block-list storage.  The entries of atom i come in groups of 8: group q holds
+// entries 8q .. 8q + 7 in the 16 bytes at nbr16 + (q * npad + i) * 8, the even
+// entries first (8q, 8q + 2, 8q + 4, 8q + 6, then 8q + 1, ..., 8q + 7).  The two
+// lanes of an atom walk its even and its odd entries, so each lane reads four of
+// its entries with one 8-byte load and the 32 atoms of a wave read 512 contiguous
+// bytes.  The list capacity (nmax) is a multiple of 8.
+__device__ __forceinline__ int64_t bl_index(int64_t npad, int64_t i, int k)
+{
+  return (((int64_t)(k >> 3) * npad + i) << 3) + ((k & 1) << 2) + ((k & 7) >> 1);
+}
+
 // Block list: stencil order dz, dy, dx, in-cell order ascending (the order of the
 // global build).  Two lanes per atom test alternate candidates of a cell and place
 // their hits in candidate order; a neighbour cell whose box lies beyond rl is
@@ -377,7 +388,7 @@ __global__ void __launch_bounds__(kPassThreads) k_block_build(BlockGeom g, const
             }
             const int other = __shfl_xor((int)hit, 32);
             const int slot = cnt + (half ? other : 0);
-            if (hit && slot < nmax) nbr16[(int64_t)slot * npad + a.global] = (uint16_t)c;
+            if (hit && slot < nmax) nbr16[bl_index(npad, a.global, slot)] = (uint16_t)c;
             cnt += (int)hit + other;
           }
         }
@@ -425,17 +436,25 @@ __global__ void __launch_bounds__(kSortThreads) k_block_sort(BlockGeom g, const 
     cq[q] = nnb[ca.global];
   }
   stage(s, l.hc, total, pos, box[s.r].L, nullptr, l.rec, nullptr);  // ends with a barrier
-  // thread t stages atom t % slots, rows t / slots, t / slots + step, ...:
-  // consecutive atoms are consecutive in memory within a cell row, so each row's
-  // reads (and the writes below) coalesce (slots is a power of two no larger than
-  // the workgroup)
+  // thread t stages the 8-entry groups t / slots, t / slots + step, ... of atom
+  // t % slots (one 16-byte load each; consecutive atoms are consecutive in memory
+  // within a cell row, so the loads and the stores below coalesce).  Whole groups
+  // are staged (at most kSortRows rows), so the entries past the count go back
+  // unchanged.  slots is a power of two no larger than the workgroup.
   const int step = kSortThreads / slots;
   const int sq = threadIdx.x % slots, sk = threadIdx.x / slots;
   const int scnt = sq < na ? cq[sq] : 0;
   const bool sorts = scnt >= 2 && scnt <= kSortRows;
   if (sorts) {
-#pragma unroll 8
-    for (int k = sk; k < scnt; k += step) buf[k * slots + sq] = nbr16[(int64_t)k * npad + gq[sq]];
+    for (int grp = sk; 8 * grp < scnt; grp += step) {
+      const uint4 w = *reinterpret_cast<const uint4*>(nbr16 + bl_index(npad, gq[sq], 8 * grp));
+      const uint32_t h[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+      for (int j = 0; j < 8; j++) {
+        const int k = 8 * grp + (j < 4 ? 2 * j : 2 * (j - 4) + 1);
+        buf[k * slots + sq] = (uint16_t)(h[j >> 1] >> (16 * (j & 1)));
+      }
+    }
   }
   __syncthreads();
   const int lane = threadIdx.x & 63;
@@ -454,8 +473,16 @@ __global__ void __launch_bounds__(kSortThreads) k_block_sort(BlockGeom g, const 
   }
   __syncthreads();
   if (sorts) {
-#pragma unroll 8
-    for (int k = sk; k < scnt; k += step) nbr16[(int64_t)k * npad + gq[sq]] = buf[k * slots + sq];
+    for (int grp = sk; 8 * grp < scnt; grp += step) {
+      uint32_t h[4];
+#pragma unroll
+      for (int j2 = 0; j2 < 4; j2++) {
+        const int ka = 8 * grp + (2 * j2 < 4 ? 4 * j2 : 4 * (j2 - 2) + 1);  // entry of j = 2 j2
+        const int kb = ka + 2;                                              // entry of j = 2 j2 + 1
+        h[j2] = (uint32_t)buf[ka * slots + sq] | ((uint32_t)buf[kb * slots + sq] << 16);
+      }
+      *reinterpret_cast<uint4*>(nbr16 + bl_index(npad, gq[sq], 8 * grp)) = make_uint4(h[0], h[1], h[2], h[3]);
+    }
   }
 }
 
@@ -480,7 +507,7 @@ __global__ void __launch_bounds__(256) k_block_to_global(BlockGeom g, const int*
     const int cnt = nnb[a.global];
     if (cnt <= kSortRows) {
       uint32_t key = 0xffffu;
-      if (lane < cnt) key = nbr16[(int64_t)lane * npad + a.global];
+      if (lane < cnt) key = nbr16[bl_index(npad, a.global, lane)];
       key = wave_radix_sort(key, 0, 16);
       if (lane < cnt) {
         const int c = (int)key;
@@ -490,7 +517,7 @@ __global__ void __launch_bounds__(256) k_block_to_global(BlockGeom g, const int*
       }
     } else {
       for (int k = lane; k < cnt; k += 64) {
-        const int c = nbr16[(int64_t)k * npad + a.global];
+        const int c = nbr16[bl_index(npad, a.global, k)];
         const HaloCell e = l.hc[find_cell(l.hc, nh, c)];
         nbr[(int64_t)k * npad + a.global] =
             (uint32_t)(e.gstart + (c - e.lstart)) | ((uint32_t)e.img << XSMD_NBR_INDEX_BITS);
@@ -566,18 +593,27 @@ struct LNb {
   bool in;
 };
 
-// entries k0, k0+2, ..., k0+2(U-1) of atom gi (this lane's parity of the list)
-template <int U>
-__device__ __forceinline__ void lfetch(const BlockArgs& a, const LRec& rec, int64_t gi, int self, int k0, int cnt,
-                                       const Pos4& pi, LNb (&nb)[U])
+// List slots of entries k0, k0 + 2, k0 + 4, k0 + 6 of atom gi (this lane's parity of
+// the list).  k0 = half + 8 q, so one 8-byte load (lword) brings the four entries of
+// group q (in bounds for k0 < nmax); lslots turns entries past this atom's count
+// into the atom itself (a masked pair at d2 = 0).  The passes load the next group's
+// word one iteration ahead (clamped to the current group at the end of the list).
+__device__ __forceinline__ uint2 lword(const BlockArgs& a, int64_t gi, int k0)
 {
-  // entries past this atom's count become the atom itself (a masked pair at d2 = 0)
-  int c[U];
+  return *reinterpret_cast<const uint2*>(a.nbr16 + bl_index(a.npad, gi, k0));
+}
+__device__ __forceinline__ void lslots(uint2 w, int self, int k0, int cnt, int (&c)[4])
+{
+  const int raw[4] = {(int)(w.x & 0xffffu), (int)(w.x >> 16), (int)(w.y & 0xffffu), (int)(w.y >> 16)};
 #pragma unroll
-  for (int u = 0; u < U; u++) {
-    const int k = k0 + 2 * u;
-    c[u] = (k < cnt) ? (int)a.nbr16[(int64_t)k * a.npad + gi] : self;
-  }
+  for (int u = 0; u < 4; u++) c[u] = (k0 + 2 * u < cnt) ? raw[u] : self;
+}
+
+// neighbour records of slots c[0..U-1] (entries k0, k0 + 2, ...)
+template <int U>
+__device__ __forceinline__ void lgather(const BlockArgs& a, const LRec& rec, const int* c, int k0, int cnt,
+                                        const Pos4& pi, LNb (&nb)[U])
+{
 #pragma unroll
   for (int u = 0; u < U; u++) {
     const Pos4 pj = rec.get(c[u]);
@@ -590,7 +626,6 @@ __device__ __forceinline__ void lfetch(const BlockArgs& a, const LRec& rec, int6
     nb[u].in = (k0 + 2 * u < cnt) && nb[u].d2 <= a.rc2;
   }
 }
-
 
 // rho_i, then (EMB) F(rho_i) and F'(rho_i) for the central atoms of one brick
 template <class Real, bool MULTI, bool EMB, bool ENERGY, bool SET_EP>
@@ -614,9 +649,14 @@ __global__ void __launch_bounds__(kPassThreads, kPassWaves<false>) k_block_eam_r
     const int ti = MULTI ? l.ltype[ca.local] : 0;
     const int cnt = a.nnb[ca.global];
     double rho = 0.0;
+    uint2 wnext = lword(a, ca.global, half);
     for (int k0 = half; k0 < cnt; k0 += 2 * kBBatch) {
+      const uint2 w = wnext;
+      wnext = lword(a, ca.global, k0 + 8 < cnt ? k0 + 8 : k0);
+      int c[kBBatch];
+      lslots(w, ca.local, k0, cnt, c);
       LNb nb[kBBatch];
-      lfetch<kBBatch>(a, l.rec, ca.global, ca.local, k0, cnt, pi, nb);
+      lgather<kBBatch>(a, l.rec, c, k0, cnt, pi, nb);
 #pragma unroll
       for (int u = 0; u < kBBatch; u++) {
         const int tj = MULTI ? l.ltype[nb[u].c] : 0;
@@ -663,21 +703,33 @@ __global__ void __launch_bounds__(kPassThreads, kPassWaves<VIRIAL>) k_block_eam_
     const int cnt = a.nnb[ca.global];
     double fx = 0.0, fy = 0.0, fz = 0.0, e = 0.0;
     double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    uint2 wnext = lword(a, ca.global, half);
     for (int k0 = half; k0 < cnt; k0 += 2 * kBBatch) {
-      LNb nb[kBBatch];
-      lfetch<kBBatch>(a, l.rec, ca.global, ca.local, k0, cnt, pi, nb);
+      const uint2 w = wnext;
+      wnext = lword(a, ca.global, k0 + 8 < cnt ? k0 + 8 : k0);
+      int c[kBBatch];
+      lslots(w, ca.local, k0, cnt, c);
+      // two entries at a time: their four table rows stay in flight together within
+      // the register budget of six waves per SIMD (the barrier keeps the compiler
+      // from hoisting the second pair's gathers above the first pair's rows)
 #pragma unroll
-      for (int u = 0; u < kBBatch; u++) {
-        const int tj = MULTI ? l.ltype[nb[u].c] : 0;
-        Real fp_r, phi;
-        pm::eam_pair<Real, MULTI>(t, pair, ti, tj, fpi, (Real)nb[u].w, nb[u].in ? nb[u].d2 : 1.0, fp_r, phi);
-        const double fpair = nb[u].in ? (double)fp_r : 0.0;
-        const double fe0 = nb[u].x * fpair, fe1 = nb[u].y * fpair, fe2 = nb[u].z * fpair;
-        fx += fe0;
-        fy += fe1;
-        fz += fe2;
-        if constexpr (ENERGY) e += nb[u].in ? 0.5 * (double)phi : 0.0;
-        if constexpr (VIRIAL) pm::add_virial(v, fe0, fe1, fe2, nb[u].x, nb[u].y, nb[u].z);
+      for (int sub = 0; sub < kBBatch; sub += 2) {
+        LNb nb[2];
+        lgather<2>(a, l.rec, c + sub, k0 + 2 * sub, cnt, pi, nb);
+#pragma unroll
+        for (int u = 0; u < 2; u++) {
+          const int tj = MULTI ? l.ltype[nb[u].c] : 0;
+          Real fp_r, phi;
+          pm::eam_pair<Real, MULTI>(t, pair, ti, tj, fpi, (Real)nb[u].w, nb[u].in ? nb[u].d2 : 1.0, fp_r, phi);
+          const double fpair = nb[u].in ? (double)fp_r : 0.0;
+          const double fe0 = nb[u].x * fpair, fe1 = nb[u].y * fpair, fe2 = nb[u].z * fpair;
+          fx += fe0;
+          fy += fe1;
+          fz += fe2;
+          if constexpr (ENERGY) e += nb[u].in ? 0.5 * (double)phi : 0.0;
+          if constexpr (VIRIAL) pm::add_virial(v, fe0, fe1, fe2, nb[u].x, nb[u].y, nb[u].z);
+        }
+        __builtin_amdgcn_sched_barrier(0);
       }
     }
     fx = half_sum(fx);
@@ -725,7 +777,7 @@ __device__ __forceinline__ void plist(const BlockArgs& a, int64_t gi, int k0, in
 #pragma unroll
   for (int u = 0; u < U; u++) {
     const int kk = max(0, min(k0 + 2 * u, cnt - 1));
-    raw[u] = a.nbr16[(int64_t)kk * a.npad + gi];
+    raw[u] = a.nbr16[bl_index(a.npad, gi, kk)];
   }
 }
 
@@ -776,7 +828,7 @@ __global__ void __launch_bounds__(kSplitThreads, 4)
   HaloCell* hc = reinterpret_cast<HaloCell*>(base + lrec_at(base, a.cap, true, rec));
   int* total_p = reinterpret_cast<int*>(hc + a.g.nh_max);
   for (int i = threadIdx.x; i < sa.rows; i += blockDim.x) {
-    const float* row = pair + (int64_t)(sa.m_lo + i) * 8;
+    const float* row = pm::pair_rho_row(pair, sa.m_lo + i);
     c3[i] = row[0];
     c4[i] = row[1];
     c5[i] = row[2];
@@ -812,12 +864,12 @@ __global__ void __launch_bounds__(kSplitThreads, 4)
             // masked entries are evaluated at rc, whose knot is in LDS
             pm::dist((float)(nb[u].in ? nb[u].d2 : a.rc2), rr[u], rinv[u]);
             pm::knot_r(t, rr[u], m, pp[u]);
-            zc[u] = pm::load4(pair + (int64_t)m * 8 + 4);
+            zc[u] = pm::load4(pm::pair_z2_row(pair, t, m));
             if (m >= sa.m_lo) {
               const int k = m - sa.m_lo;
               rc[u] = {c3[k], c4[k], c5[k], 0.0f};
             } else {
-              rc[u] = pm::load4(pair + (int64_t)m * 8);
+              rc[u] = pm::load4(pm::pair_rho_row(pair, m));
             }
           }
           // the next batch's list loads go out after this batch's row loads, so that
@@ -872,7 +924,7 @@ __global__ void __launch_bounds__(kSplitThreads, 4)
   HaloCell* hc = reinterpret_cast<HaloCell*>(base + lrec_at(base, a.cap, false, rec));
   int* total_p = reinterpret_cast<int*>(hc + a.g.nh_max);
   for (int i = threadIdx.x; i < sa.rows; i += blockDim.x)
-    rows[i] = reinterpret_cast<const float4*>(pair)[(int64_t)(sa.m_lo + i) * 2];
+    rows[i] = *reinterpret_cast<const float4*>(pm::pair_rho_row(pair, sa.m_lo + i));
   const int64_t b0 = lblock() * sa.per_wg;
   const int64_t b1 = min(b0 + sa.per_wg, a.g.nblocks);
   const int half = pass_half();
@@ -906,7 +958,7 @@ __global__ void __launch_bounds__(kSplitThreads, 4)
               const float4 q4 = rows[m - sa.m_lo];
               rc = {q4.x, q4.y, q4.z, q4.w};
             } else {
-              rc = pm::load4(pair + (int64_t)m * 8);
+              rc = pm::load4(pm::pair_rho_row(pair, m));
             }
             const float val = pm::cubic(rc, p);
             rho += nb[u].in ? (double)val : 0.0;

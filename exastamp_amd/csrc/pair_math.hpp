@@ -204,8 +204,23 @@ __device__ __forceinline__ void embed(const EamTabDev<Real>& t, int ti, double r
   Fp *= t.conv;
 }
 
+// Single-element pair table: two planes of (nr + 1) rows of 4 values, the rho value
+// cubic [c3 c4 c5 c6] of knot m at row m of plane 0 and the z2r value cubic at row m
+// of plane 1.  Rows of 4 values keep 8 (FP32) or 4 (FP64) knots per 128-byte line,
+// so the clustered knots that the lanes of a wave look up touch few lines.
+template <class Real>
+__device__ __forceinline__ const Real* pair_rho_row(const Real* __restrict__ pair, int m)
+{
+  return pair + (int64_t)m * 4;
+}
+template <class Real>
+__device__ __forceinline__ const Real* pair_z2_row(const Real* __restrict__ pair, const EamTabDev<Real>& t, int m)
+{
+  return pair + ((int64_t)t.nr + 1 + m) * 4;
+}
+
 // rho_{type_j}(r) contribution to atom i (SymRhoOp, eam_force_op_multimat.h:142-171).
-// Single-element tables read the rho cubic from the first half of the combined row.
+// Single-element tables read the rho cubic from plane 0 of the pair table.
 template <class Real, bool MULTI>
 __device__ __forceinline__ Real eam_rho(const EamTabDev<Real>& t, const Real* __restrict__ pair, int ti, int tj,
                                         double d2)
@@ -217,11 +232,11 @@ __device__ __forceinline__ Real eam_rho(const EamTabDev<Real>& t, const Real* __
   knot_r(t, r, m, p);
   const Real* row;
   if constexpr (MULTI) row = row_r(t.rhor, t, t.t2rhor[tj + 1][ti + 1], m);
-  else row = pair + (int64_t)m * 8;
+  else row = pair_rho_row(pair, m);
   return cubic(load4(row), p);
 }
 
-// Single-element psi'(r)/r and phi(r) from the two halves of a combined row:
+// Single-element psi'(r)/r and phi(r) from the two rows of knot m:
 // rc = rho value cubic, zc = z2r value cubic; the derivative quadratics follow
 // from the value cubics (times 1/dr).
 template <class Real>
@@ -261,8 +276,7 @@ __device__ __forceinline__ void eam_pair(const EamTabDev<Real>& t, const Real* _
     const Real psip = fpi * rhojp + fpj * rhoip + phip;
     fpair = psip * rinv;
   } else {
-    const Real* row = pair + (int64_t)m * 8;
-    eam_pair_rows(t, load4(row), load4(row + 4), p, rinv, fpi, fpj, fpair, phi);
+    eam_pair_rows(t, load4(pair_rho_row(pair, m)), load4(pair_z2_row(pair, t, m)), p, rinv, fpi, fpj, fpair, phi);
   }
 }
 

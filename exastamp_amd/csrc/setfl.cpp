@@ -158,14 +158,14 @@ void eam_upload(Ctx& c)
   upload(c, e.rhor_f, cr);
   upload(c, e.z2r_f, cz);
 
-  // combined row of element 0 / pair 00 for single-element kernels:
-  // [rho c3 c4 c5 c6 | z2r c3 c4 c5 c6]
+  // pair table of element 0 / pair 00 for single-element kernels: plane 0 holds the
+  // rho value cubics [c3 c4 c5 c6] of rows 0..nr, plane 1 the z2r value cubics
   const size_t rows = (size_t)e.nr + 1;
   std::vector<double> pair(8 * rows);
   for (size_t m = 0; m < rows; m++)
     for (int k = 0; k < 4; k++) {
-      pair[8 * m + k] = cr[8 * m + 3 + k];
-      pair[8 * m + 4 + k] = cz[8 * m + 3 + k];
+      pair[4 * m + k] = cr[8 * m + 3 + k];
+      pair[4 * (rows + m) + k] = cz[8 * m + 3 + k];
     }
   auto up = [&](auto& buf, auto zero) {
     using T = decltype(zero);

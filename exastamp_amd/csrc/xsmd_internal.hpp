@@ -116,9 +116,9 @@ struct EamHost {
   int8_t t2frho[8] = {};
   DBuf<double> frho_d, rhor_d, z2r_d;
   DBuf<float> frho_f, rhor_f, z2r_f;
-  // single-element pair table: row m = [rho c3 c4 c5 c6 | z2r c3 c4 c5 c6]; the
-  // derivative quadratics follow from the value cubics, so one 8-value row
-  // serves both EAM passes
+  // single-element pair table: plane 0 row m = rho [c3 c4 c5 c6], plane 1 row m =
+  // z2r [c3 c4 c5 c6]; the derivative quadratics follow from the value cubics, so
+  // the two planes serve both EAM passes
   DBuf<double> pair_d;
   DBuf<float> pair_f;
 };
