@@ -2,16 +2,19 @@
 //
 // The cell grid is cut into bricks of bx*by*bz cells.  One workgroup owns one
 // brick: it stages the brick's halo -- the brick plus one cell on every side,
-// periodic images applied once at staging -- into LDS as 32-byte records
-// {x, y, z, w}, then each thread walks the list of one central atom.  List
-// entries are 16-bit halo-local slots, so a pair costs one LDS record read
-// instead of a global gather, and the list is half the size of the global one.
+// periodic images applied once at staging -- into LDS as arrays of doubles
+// (x, y, z, and w = F'(rho) in the force pass), then two lanes walk the list of
+// each central atom.  List entries are 16-bit halo-local slots, so a pair costs
+// three 8-byte LDS reads instead of a global gather, and the list is half the
+// size of the global one.
 //
 // The halo of a brick is an (ex+2)x(ey+2)x(ez+2) box of cells enumerated z-major,
 // x fastest.  Atoms of one cell are contiguous in HBM (cell-sorted order) and are
-// staged contiguously, cells in enumeration order, so the local slot order equals
-// the visit order of the global build: a block list maps back to exactly the
-// global list (same entries, same order).
+// staged contiguously, cells in enumeration order, so ascending slot order equals
+// the visit order of the global build.  The build finds the same pairs as the
+// global build; k_block_sort then orders each list by distance, and
+// k_block_to_global restores slot order, so a converted block list equals the
+// global list entry for entry.
 #include "pair_math.hpp"
 
 #include <algorithm>
