@@ -303,8 +303,11 @@ __device__ __forceinline__ void wave_radix_sort2(uint32_t& k0, uint32_t& k1, int
 }
 
 // Lists of at most 64 entries are kept sorted by distance (see k_block_sort); this
-// many rows are all the sort touches.
+// many entries are all the sort touches.  Its LDS rows are kSortStride entries
+// apart: 144 bytes, a multiple of 16 whose 16 consecutive rows start in distinct
+// groups of four banks.
 constexpr int kSortRows = 64;
+constexpr int kSortStride = 72;
 
 // sort key of slot c for central slot i: distance bin (nbits) above the slot
 __device__ __forceinline__ uint32_t dist_key(const LRec& rec, int c, int i, float inv_rl, int nbits)
@@ -341,18 +344,43 @@ __device__ __forceinline__ int64_t bl_index(int64_t npad, int64_t i, int k)
 }
 
 // Block list: stencil order dz, dy, dx, in-cell order ascending (the order of the
-// global build).  Two lanes per atom test alternate candidates of a cell and place
-// their hits in candidate order; a neighbour cell whose box lies beyond rl is
-// skipped (its atoms would all fail the exact test), unless some atom sits outside
-// its cell's box (nocull, set by the cell-key kernel).
-__global__ void __launch_bounds__(kPassThreads) k_block_build(BlockGeom g, const int* __restrict__ cell_off,
-                                                              const Pos4* __restrict__ pos,
-                                                              const BoxDev* __restrict__ box, double rl2,
-                                                              uint16_t* __restrict__ nbr16, int* __restrict__ nnb,
-                                                              int nmax, int64_t npad, int cap, int nh_max,
-                                                              const unsigned long long* __restrict__ nocull,
-                                                              int* __restrict__ brick_rows,
-                                                              unsigned long long* __restrict__ flags)
+// global build, ascending halo slot).  One lane per central atom walks the
+// candidates itself, two at a time (their LDS reads overlap), and packs its hits
+// into the current 8-entry group in registers; a full group goes out as one
+// 16-byte store.  A neighbour cell whose box lies beyond rl is skipped (its atoms
+// would all fail the exact test), unless some atom sits outside its cell's box
+// (nocull, set by the cell-key kernel).  Entries past nmax are counted, not
+// stored (the host grows nmax and builds again).
+constexpr int kBuildThreads = 256;
+
+// The current 8-entry group of a lane is a 128-bit FIFO of 16-bit slots in w[0..3]:
+// a new entry enters at the top and the others move down one place (a funnel
+// shift per word), so after eight entries w holds them in list order, two per
+// word, the first in the low half of w[0].
+__device__ __forceinline__ void fifo_push(uint32_t (&w)[4], uint32_t c)
+{
+  w[0] = __builtin_amdgcn_alignbit(w[1], w[0], 16);
+  w[1] = __builtin_amdgcn_alignbit(w[2], w[1], 16);
+  w[2] = __builtin_amdgcn_alignbit(w[3], w[2], 16);
+  w[3] = __builtin_amdgcn_alignbit(c, w[3], 16);
+}
+
+// list order (0, 1), (2, 3), (4, 5), (6, 7) -> the bl_index order (0, 2), (4, 6),
+// (1, 3), (5, 7)
+__device__ __forceinline__ uint4 group_interleave(const uint32_t (&o)[4])
+{
+  return make_uint4((o[0] & 0xffffu) | (o[1] << 16), (o[2] & 0xffffu) | (o[3] << 16),
+                    (o[0] >> 16) | (o[1] & 0xffff0000u), (o[2] >> 16) | (o[3] & 0xffff0000u));
+}
+
+__global__ void __launch_bounds__(kBuildThreads) k_block_build(BlockGeom g, const int* __restrict__ cell_off,
+                                                               const Pos4* __restrict__ pos,
+                                                               const BoxDev* __restrict__ box, double rl2,
+                                                               uint16_t* __restrict__ nbr16, int* __restrict__ nnb,
+                                                               int nmax, int64_t npad, int cap, int nh_max,
+                                                               const unsigned long long* __restrict__ nocull,
+                                                               int* __restrict__ brick_rows,
+                                                               unsigned long long* __restrict__ flags)
 {
   const int64_t b = lblock();
   if (b >= g.nblocks) return;
@@ -367,13 +395,26 @@ __global__ void __launch_bounds__(kPassThreads) k_block_build(BlockGeom g, const
   const bool cull = *nocull == 0ull;
   if (threadIdx.x == 0) l.total[1] = 0;  // longest list of the brick (made visible by stage's barrier)
   stage(s, l.hc, total, pos, bx.L, nullptr, l.rec, nullptr);
-  const int half = pass_half();
-  for (int q = pass_slot();; q += kPassAtoms) {
+  for (int q = threadIdx.x;; q += blockDim.x) {
     Central a;
     if (!central_atom(s, l.hc, q, a)) break;
-    const Pos4 pi = l.rec.get(a.local);
+    const double xi = l.rec.x[a.local], yi = l.rec.y[a.local], zi = l.rec.z[a.local];
+    const Pos4 pi = {xi, yi, zi, 0.0};
     const int ix = a.h % s.hx, iy = (a.h / s.hx) % s.hy, iz = a.h / (s.hx * s.hy);
     int cnt = 0;
+    uint32_t w[4] = {0u, 0u, 0u, 0u};
+    // hit at slot c: into the group; a full group is stored (when it fits in nmax)
+    // at gp, which then moves to the atom's next group (bl_index(npad, i, 8 q))
+    uint16_t* gp = nbr16 + bl_index(npad, a.global, 0);
+    const int64_t gstep = npad * 8;
+    auto hit = [&](int c) {
+      fifo_push(w, (uint32_t)c);
+      cnt++;
+      if ((cnt & 7) == 0) {
+        if (cnt <= nmax) *reinterpret_cast<uint4*>(gp) = group_interleave(w);
+        gp += gstep;
+      }
+    };
     for (int dz = -1; dz <= 1; dz++)
       for (int dy = -1; dy <= 1; dy++)
         for (int dx = -1; dx <= 1; dx++) {
@@ -381,25 +422,22 @@ __global__ void __launch_bounds__(kPassThreads) k_block_build(BlockGeom g, const
           if (e.count == 0) continue;
           if (cull && cell_box_d2(g, bx, s.cx0 - 1 + ix + dx, s.cy0 - 1 + iy + dy, s.cz0 - 1 + iz + dz, pi) >= rl2)
             continue;
-          for (int off = 0; off < e.count; off += 2) {
-            const int k = off + half;
-            const int c = e.lstart + k;
-            bool hit = false;
-            if (k < e.count && c != a.local) {
-              const Pos4 pj = l.rec.get(c);
-              hit = xsmd_d2(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z) < rl2;
-            }
-            const int other = __shfl_xor((int)hit, 32);
-            const int slot = cnt + (half ? other : 0);
-            if (hit && slot < nmax) nbr16[bl_index(npad, a.global, slot)] = (uint16_t)c;
-            cnt += (int)hit + other;
+          const int end = e.lstart + e.count;
+          for (int c = e.lstart; c < end; c += 2) {
+            const int c1 = min(c + 1, end - 1);  // the pair's second candidate (or c again)
+            const double d0 = xsmd_d2(l.rec.x[c] - xi, l.rec.y[c] - yi, l.rec.z[c] - zi);
+            const double d1 = xsmd_d2(l.rec.x[c1] - xi, l.rec.y[c1] - yi, l.rec.z[c1] - zi);
+            if (d0 < rl2 && c != a.local) hit(c);
+            if (c + 1 < end && d1 < rl2 && c1 != a.local) hit(c1);
           }
         }
-    if (half == 0) {
-      nnb[a.global] = cnt;
-      atomicMax(l.total + 1, cnt);
-      if (cnt > nmax) atomicMax(flags, (unsigned long long)cnt);
+    if ((cnt & 7) != 0 && cnt <= nmax) {  // the last, partial group: move it down (unused places hold slot 0)
+      for (int k = cnt & 7; k < 8; k++) fifo_push(w, 0u);
+      *reinterpret_cast<uint4*>(gp) = group_interleave(w);
     }
+    nnb[a.global] = cnt;
+    atomicMax(l.total + 1, cnt);
+    if (cnt > nmax) atomicMax(flags, (unsigned long long)cnt);
   }
   __syncthreads();
   if (threadIdx.x == 0) brick_rows[b] = l.total[1];
@@ -409,8 +447,8 @@ __global__ void __launch_bounds__(kPassThreads) k_block_build(BlockGeom g, const
 // order within a bin), for lists of at most kSortRows entries.  At a given list
 // position the atoms of a wave then look up nearby spline rows, which cuts the
 // cache lines each table gather of the EAM passes touches by about 40 %.  The lists
-// of a brick are staged through LDS (buf[row][atom]) so that global reads and writes
-// coalesce; one wave sorts two atoms at a time.
+// of a brick are staged through LDS (one row per atom) so that global reads and
+// writes coalesce; one wave sorts two atoms at a time.
 constexpr int kSortThreads = 512;
 
 __global__ void __launch_bounds__(kSortThreads) k_block_sort(BlockGeom g, const int* __restrict__ cell_off,
@@ -426,7 +464,7 @@ __global__ void __launch_bounds__(kSortThreads) k_block_sort(BlockGeom g, const 
   int* gq = reinterpret_cast<int*>(l.end);  // per central atom: global index, local slot, count
   int* lq = gq + slots;
   int* cq = lq + slots;
-  uint16_t* buf = reinterpret_cast<uint16_t*>(cq + slots);  // [row][slots]
+  uint16_t* buf = reinterpret_cast<uint16_t*>(cq + slots);  // [atom][kSortStride]
   const BlockShape s = block_shape(g, b);
   const int total = build_halo(g, s, cell_off, l.hc, l.total);
   if (total > cap) return;  // excluded by the host (cap = measured maximum)
@@ -439,11 +477,12 @@ __global__ void __launch_bounds__(kSortThreads) k_block_sort(BlockGeom g, const 
     cq[q] = nnb[ca.global];
   }
   stage(s, l.hc, total, pos, box[s.r].L, nullptr, l.rec, nullptr);  // ends with a barrier
-  // thread t stages the 8-entry groups t / slots, t / slots + step, ... of atom
-  // t % slots (one 16-byte load each; consecutive atoms are consecutive in memory
-  // within a cell row, so the loads and the stores below coalesce).  Whole groups
-  // are staged (at most kSortRows rows), so the entries past the count go back
-  // unchanged.  slots is a power of two no larger than the workgroup.
+  // Thread t stages the 8-entry groups t / slots, t / slots + step, ... of atom
+  // t % slots: one 16-byte load each (consecutive atoms are consecutive in memory
+  // within a cell row, so the loads and the stores below coalesce), stored in list
+  // order into the atom's LDS row, one 16-byte write.  Whole groups are staged (at
+  // most kSortRows entries), so the entries past the count go back unchanged.
+  // slots is a power of two no larger than the workgroup.
   const int step = kSortThreads / slots;
   const int sq = threadIdx.x % slots, sk = threadIdx.x / slots;
   const int scnt = sq < na ? cq[sq] : 0;
@@ -451,12 +490,11 @@ __global__ void __launch_bounds__(kSortThreads) k_block_sort(BlockGeom g, const 
   if (sorts) {
     for (int grp = sk; 8 * grp < scnt; grp += step) {
       const uint4 w = *reinterpret_cast<const uint4*>(nbr16 + bl_index(npad, gq[sq], 8 * grp));
-      const uint32_t h[4] = {w.x, w.y, w.z, w.w};
-#pragma unroll
-      for (int j = 0; j < 8; j++) {
-        const int k = 8 * grp + (j < 4 ? 2 * j : 2 * (j - 4) + 1);
-        buf[k * slots + sq] = (uint16_t)(h[j >> 1] >> (16 * (j & 1)));
-      }
+      // w holds entries (0, 2), (4, 6), (1, 3), (5, 7) of the group; list order is
+      // (0, 1), (2, 3), (4, 5), (6, 7)
+      const uint4 o = make_uint4((w.x & 0xffffu) | (w.z << 16), (w.x >> 16) | (w.z & 0xffff0000u),
+                                 (w.y & 0xffffu) | (w.w << 16), (w.y >> 16) | (w.w & 0xffff0000u));
+      *reinterpret_cast<uint4*>(buf + sq * kSortStride + 8 * grp) = o;
     }
   }
   __syncthreads();
@@ -468,23 +506,20 @@ __global__ void __launch_bounds__(kSortThreads) k_block_sort(BlockGeom g, const 
     const bool s0 = cnt0 >= 2 && cnt0 <= kSortRows, s1 = cnt1 >= 2 && cnt1 <= kSortRows;
     if (!s0 && !s1) continue;
     uint32_t key0 = 0xffffffffu, key1 = 0xffffffffu;
-    if (s0 && lane < cnt0) key0 = dist_key(l.rec, buf[lane * slots + q0], lq[q0], inv_rl, nbits);
-    if (s1 && lane < cnt1) key1 = dist_key(l.rec, buf[lane * slots + q1], lq[q1], inv_rl, nbits);
+    if (s0 && lane < cnt0) key0 = dist_key(l.rec, buf[q0 * kSortStride + lane], lq[q0], inv_rl, nbits);
+    if (s1 && lane < cnt1) key1 = dist_key(l.rec, buf[q1 * kSortStride + lane], lq[q1], inv_rl, nbits);
     wave_radix_sort2(key0, key1, 16, nbits);
-    if (s0 && lane < cnt0) buf[lane * slots + q0] = (uint16_t)(key0 & 0xffffu);
-    if (s1 && lane < cnt1) buf[lane * slots + q1] = (uint16_t)(key1 & 0xffffu);
+    if (s0 && lane < cnt0) buf[q0 * kSortStride + lane] = (uint16_t)(key0 & 0xffffu);
+    if (s1 && lane < cnt1) buf[q1 * kSortStride + lane] = (uint16_t)(key1 & 0xffffu);
   }
   __syncthreads();
   if (sorts) {
     for (int grp = sk; 8 * grp < scnt; grp += step) {
-      uint32_t h[4];
-#pragma unroll
-      for (int j2 = 0; j2 < 4; j2++) {
-        const int ka = 8 * grp + (2 * j2 < 4 ? 4 * j2 : 4 * (j2 - 2) + 1);  // entry of j = 2 j2
-        const int kb = ka + 2;                                              // entry of j = 2 j2 + 1
-        h[j2] = (uint32_t)buf[ka * slots + sq] | ((uint32_t)buf[kb * slots + sq] << 16);
-      }
-      *reinterpret_cast<uint4*>(nbr16 + bl_index(npad, gq[sq], 8 * grp)) = make_uint4(h[0], h[1], h[2], h[3]);
+      const uint4 o = *reinterpret_cast<const uint4*>(buf + sq * kSortStride + 8 * grp);
+      // back to (0, 2), (4, 6), (1, 3), (5, 7)
+      const uint4 w = make_uint4((o.x & 0xffffu) | (o.y << 16), (o.z & 0xffffu) | (o.w << 16),
+                                 (o.x >> 16) | (o.y & 0xffff0000u), (o.z >> 16) | (o.w & 0xffff0000u));
+      *reinterpret_cast<uint4*>(nbr16 + bl_index(npad, gq[sq], 8 * grp)) = w;
     }
   }
 }
@@ -1272,7 +1307,7 @@ void block_sort(Ctx& c, double rl)
   while (slots < c.bcentral) slots *= 2;
   if (slots > kSortThreads) return;  // bricks this large keep the stencil order
   const size_t lds = (block_lds_bytes(c, false, false) + 15) / 16 * 16 + (size_t)3 * slots * sizeof(int) +
-                     (size_t)kSortRows * slots * sizeof(uint16_t);
+                     (size_t)kSortStride * slots * sizeof(uint16_t);
   if (lds > c.lds_max) return;
   XS_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_block_sort),
                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
@@ -1290,7 +1325,7 @@ void block_build(Ctx& c, double rl2)
     c.nnb.reserve(c.npad);
     c.brick_rows.reserve(c.bgeom.nblocks);
     XS_HIP(hipMemsetAsync(c.dscalar.p, 0, 2 * sizeof(unsigned long long), c.stream));
-    k_block_build<<<block_grid(c), kPassThreads, lds, c.stream>>>(c.bgeom, c.cell_off.p, c.pos.p, c.box_d.p, rl2,
+    k_block_build<<<block_grid(c), kBuildThreads, lds, c.stream>>>(c.bgeom, c.cell_off.p, c.pos.p, c.box_d.p, rl2,
                                                                   c.nbr16.p, c.nnb.p, c.nmax, c.npad, c.bcap,
                                                                   c.bgeom.nh_max, c.dscalar.p + 4, c.brick_rows.p,
                                                                   c.dscalar.p);
