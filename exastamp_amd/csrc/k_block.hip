@@ -403,16 +403,32 @@ __global__ void __launch_bounds__(kBuildThreads) k_block_build(BlockGeom g, cons
     const int ix = a.h % s.hx, iy = (a.h / s.hx) % s.hy, iz = a.h / (s.hx * s.hy);
     int cnt = 0;
     uint32_t w[4] = {0u, 0u, 0u, 0u};
-    // hit at slot c: into the group; a full group is stored (when it fits in nmax)
-    // at gp, which then moves to the atom's next group (bl_index(npad, i, 8 q))
+    // A hit at slot c goes into the group FIFO w; a full group moves to r and is
+    // stored (when it fits in nmax) after the current stencil cell, so that the
+    // wave issues at most one store per cell rather than one per candidate pair.
+    // gp is the atom's next group (bl_index(npad, i, 8 q)); gq its index q.
+    uint32_t r[4] = {0u, 0u, 0u, 0u};
+    bool ready = false;
     uint16_t* gp = nbr16 + bl_index(npad, a.global, 0);
     const int64_t gstep = npad * 8;
+    const int gmax = nmax >> 3;  // groups that fit
+    int gq = 0;
+    auto flush = [&]() {
+      if (gq < gmax) *reinterpret_cast<uint4*>(gp) = group_interleave(r);
+      gp += gstep;
+      gq++;
+      ready = false;
+    };
     auto hit = [&](int c) {
       fifo_push(w, (uint32_t)c);
       cnt++;
       if ((cnt & 7) == 0) {
-        if (cnt <= nmax) *reinterpret_cast<uint4*>(gp) = group_interleave(w);
-        gp += gstep;
+        if (ready) flush();  // a second full group within one cell
+        r[0] = w[0];
+        r[1] = w[1];
+        r[2] = w[2];
+        r[3] = w[3];
+        ready = true;
       }
     };
     for (int dz = -1; dz <= 1; dz++)
@@ -430,10 +446,15 @@ __global__ void __launch_bounds__(kBuildThreads) k_block_build(BlockGeom g, cons
             if (d0 < rl2 && c != a.local) hit(c);
             if (c + 1 < end && d1 < rl2 && c1 != a.local) hit(c1);
           }
+          if (ready) flush();
         }
-    if ((cnt & 7) != 0 && cnt <= nmax) {  // the last, partial group: move it down (unused places hold slot 0)
+    if ((cnt & 7) != 0) {  // the last, partial group: move it down (unused places hold slot 0)
       for (int k = cnt & 7; k < 8; k++) fifo_push(w, 0u);
-      *reinterpret_cast<uint4*>(gp) = group_interleave(w);
+      r[0] = w[0];
+      r[1] = w[1];
+      r[2] = w[2];
+      r[3] = w[3];
+      flush();
     }
     nnb[a.global] = cnt;
     atomicMax(l.total + 1, cnt);
