@@ -20,6 +20,14 @@
 // Each weight is spread on the four value coefficients c3..c6 of the spline row
 // it falls in (the slope coefficients c0..c2 are 3c3/D, 2c4/D, c5/D), accumulated
 // in FP64; the host then applies the transpose of the knot -> coefficient map.
+//
+// Accumulation.  All pairs of a shell fall on a few hundred rows, and all atoms on
+// a few F rows, so atomics on the gradient rows in HBM serialise.  Instead a
+// launch covers a set of row windows (slots) that fit in LDS: persistent
+// workgroups walk all atoms (or pairs), add the contributions that fall in a window
+// into their LDS copy (LDS FP64 atomics), and write it out as a partial; a second
+// kernel sums the partials over the workgroups in a fixed order.  Windows cover
+// every row of every table, one launch per group of windows.
 #include "xsmd_internal.hpp"
 
 #include <algorithm>
@@ -61,19 +69,62 @@ struct AdjArgs {
   const double* gE;       // per replica
   double rc2;
   double* lam;            // lambda_i (B1 out, B2 in)
-  double* gF;             // F table coefficient gradients, 4 per row
-  double* gR;             // rho tables
-  double* gZ;             // z2r tables
+  double* fc;             // F-table contribution of atom i (4 coefficients; B1 out)
+  int* fm;                // its F-table row
+  int nel;                // elements: table ids F 0..nel-1, rho nel..2nel-1, z2r 2nel..
 };
 
-__device__ __forceinline__ void spread(double* row4, double wv, double ws, double p, double rscale)
+// LDS row windows of one launch: rows [lo, lo + rows) of table `table` at LDS
+// rows off .. off + rows - 1
+constexpr int kAdjSlots = 8;
+constexpr int kAdjRows = 2560;  // LDS rows per launch (80 KB)
+constexpr int kAdjThreads = 512;
+struct AdjSlot {
+  int table, lo, rows, off;
+  int64_t g0;  // row of the gradient array that holds row lo of the table
+};
+struct AdjSlots {
+  int n, total;
+  AdjSlot s[kAdjSlots];
+};
+
+// the four coefficient weights of a value weight wv and a slope weight ws at
+// fraction p: v = c3 p^3 + c4 p^2 + c5 p + c6, dv/dx = (3 c3 p^2 + 2 c4 p + c5) rscale
+__device__ __forceinline__ void coeff4(double wv, double ws, double p, double rscale, double (&o)[4])
 {
-  // value v = c3 p^3 + c4 p^2 + c5 p + c6 ; slope dv/dx = (3 c3 p^2 + 2 c4 p + c5) * rscale
   const double s = ws * rscale;
-  atomicAdd(row4 + 0, wv * p * p * p + s * 3.0 * p * p);
-  atomicAdd(row4 + 1, wv * p * p + s * 2.0 * p);
-  atomicAdd(row4 + 2, wv * p + s);
-  atomicAdd(row4 + 3, wv);
+  o[0] = wv * p * p * p + s * 3.0 * p * p;
+  o[1] = wv * p * p + s * 2.0 * p;
+  o[2] = wv * p + s;
+  o[3] = wv;
+}
+
+// add o to row m of table T if a window of this launch holds it
+__device__ __forceinline__ void lds_add(double* acc, const AdjSlots& S, int T, int m, const double (&o)[4])
+{
+  for (int k = 0; k < S.n; k++) {
+    const AdjSlot& w = S.s[k];
+    if (w.table == T && m >= w.lo && m < w.lo + w.rows) {
+      double* row = acc + 4 * (w.off + m - w.lo);
+      atomicAdd(row + 0, o[0]);
+      atomicAdd(row + 1, o[1]);
+      atomicAdd(row + 2, o[2]);
+      atomicAdd(row + 3, o[3]);
+    }
+  }
+}
+
+__device__ __forceinline__ void lds_zero(double* acc, int total)
+{
+  for (int q = threadIdx.x; q < 4 * total; q += blockDim.x) acc[q] = 0.0;
+  __syncthreads();
+}
+
+__device__ __forceinline__ void lds_out(const double* acc, int total, double* __restrict__ part)
+{
+  __syncthreads();
+  double* dst = part + (int64_t)blockIdx.x * kAdjRows * 4;
+  for (int q = threadIdx.x; q < 4 * total; q += blockDim.x) dst[q] = acc[q];
 }
 
 __device__ __forceinline__ void knot(double x, double rscale, int nmax1, int mmin, int& m, double& p,
@@ -140,15 +191,33 @@ __global__ void __launch_bounds__(256) k_adj_b1(AdjArgs a, AdjTab<Real> t)
   const double ge = a.gE[rep];
   const double conv = t.conv;
   const double over = rho > t.rhomax ? rho - t.rhomax : 0.0;
-  spread(a.gF + ((int64_t)ft * (t.nrho + 1) + m) * 4, ge * conv, conv * (ge * over - drho), p, t.rdrho);
+  double o[4];
+  coeff4(ge * conv, conv * (ge * over - drho), p, t.rdrho, o);
+  store_pos(reinterpret_cast<Pos4*>(a.fc), i, Pos4{o[0], o[1], o[2], o[3]});
+  a.fm[i] = m;
+}
+
+// F-table rows: the per-atom contributions of B1 into this launch's windows
+template <class Real, bool MULTI>
+__global__ void __launch_bounds__(kAdjThreads) k_adj_fsum(AdjArgs a, AdjTab<Real> t, AdjSlots S,
+                                                          double* __restrict__ part)
+{
+  extern __shared__ double acc[];
+  lds_zero(acc, S.total);
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < a.n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int T = t.t2frho[(MULTI ? a.type[i] : 0) + 1];
+    const Pos4 c = load_pos(reinterpret_cast<const Pos4*>(a.fc), i);
+    const double o[4] = {c.x, c.y, c.z, c.w};
+    lds_add(acc, S, T, a.fm[i], o);
+  }
+  lds_out(acc, S.total, part);
 }
 
 // B2: per pair, the rho- and z2r-table gradients.
 template <class Real, bool MULTI>
-__global__ void __launch_bounds__(256) k_adj_b2(AdjArgs a, AdjTab<Real> t)
+__device__ __forceinline__ void adj_b2_atom(const AdjArgs& a, const AdjTab<Real>& t, const AdjSlots& S,
+                                            double* acc, int64_t i)
 {
-  const int64_t i = lblock() * blockDim.x + threadIdx.x;
-  if (i >= a.n) return;
   const int ti = MULTI ? a.type[i] : 0;
   const int rep = a.rep ? a.rep[i] : 0;
   const double* L = a.box[rep].L;
@@ -177,15 +246,39 @@ __global__ void __launch_bounds__(256) k_adj_b2(AdjArgs a, AdjTab<Real> t)
     double p;
     bool cl;
     knot(r, t.rdr, t.nr - 1, 0, m, p, cl);
-    const int tr = t.t2rhor[tj + 1][ti + 1];
-    spread(a.gR + ((int64_t)tr * (t.nr + 1) + m) * 4, ge * fpi - lam, -fpi * s, p, rdr);
-    const int tz = t.t2z2r[ti + 1][tj + 1];
+    double o[4];
+    coeff4(ge * fpi - lam, -fpi * s, p, rdr, o);
+    lds_add(acc, S, a.nel + t.t2rhor[tj + 1][ti + 1], m, o);
     const double ir = 1.0 / r;
-    spread(a.gZ + ((int64_t)tz * (t.nr + 1) + m) * 4, conv * (0.5 * ge * ir + 0.5 * s * ir * ir),
-                 -conv * 0.5 * s * ir, p, rdr);
+    coeff4(conv * (0.5 * ge * ir + 0.5 * s * ir * ir), -conv * 0.5 * s * ir, p, rdr, o);
+    lds_add(acc, S, 2 * a.nel + t.t2z2r[ti + 1][tj + 1], m, o);
   }
 }
 
+template <class Real, bool MULTI>
+__global__ void __launch_bounds__(kAdjThreads) k_adj_b2(AdjArgs a, AdjTab<Real> t, AdjSlots S,
+                                                        double* __restrict__ part)
+{
+  extern __shared__ double acc[];
+  lds_zero(acc, S.total);
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < a.n; i += (int64_t)gridDim.x * blockDim.x)
+    adj_b2_atom<Real, MULTI>(a, t, S, acc, i);
+  lds_out(acc, S.total, part);
+}
+
+// Sum of the partials of `groups` workgroups (fixed order) into the gradient rows
+// of this launch's windows (each row belongs to exactly one window of one launch).
+__global__ void k_adj_reduce(AdjSlots S, int groups, const double* __restrict__ part, double* __restrict__ g)
+{
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= 4 * S.total) return;
+  double v = 0.0;
+  for (int w = 0; w < groups; w++) v += part[(int64_t)w * kAdjRows * 4 + q];
+  const int row = q >> 2;
+  int k = 0;
+  while (k + 1 < S.n && row >= S.s[k + 1].off) k++;
+  g[(S.s[k].g0 + row - S.s[k].off) * 4 + (q & 3)] = v;
+}
 // g (dL/dF) from id order into cell order as position-like records
 __global__ void k_gather_g(int64_t n, const int64_t* __restrict__ id, const double* __restrict__ g_id,
                            Pos4* __restrict__ gvec)
@@ -237,19 +330,71 @@ AdjTab<float> adj_tab<float>(Ctx& c)
   return t;
 }
 
+// Row windows of every table (F tables use their rows_f rows, rho and z2r tables
+// rows_r), packed into launches of at most kAdjRows LDS rows and kAdjSlots windows;
+// `pairs` selects the rho/z2r tables (B2) or the F tables (fsum).
+std::vector<AdjSlots> adj_launches(int nel, int nz, int64_t rows_f, int64_t rows_r, bool pairs)
+{
+  std::vector<AdjSlots> out;
+  AdjSlots cur{};
+  auto add = [&](int T, int64_t rows, int64_t g0) {
+    for (int64_t lo = 0; lo < rows; lo += kAdjRows) {
+      const int w = (int)std::min<int64_t>(kAdjRows, rows - lo);
+      if (cur.n == kAdjSlots || cur.total + w > kAdjRows) {
+        out.push_back(cur);
+        cur = AdjSlots{};
+      }
+      cur.s[cur.n++] = AdjSlot{T, (int)lo, w, cur.total, g0 + lo};
+      cur.total += w;
+    }
+  };
+  if (!pairs) {
+    for (int t = 0; t < nel; t++) add(t, rows_f, t * rows_f);
+  } else {
+    const int64_t f_end = nel * rows_f;
+    for (int t = 0; t < nel; t++) add(nel + t, rows_r, f_end + t * rows_r);
+    for (int t = 0; t < nz; t++) add(2 * nel + t, rows_r, f_end + (nel + t) * rows_r);
+  }
+  if (cur.n) out.push_back(cur);
+  return out;
+}
+
 template <class Real>
 void adj_run(Ctx& c, const AdjArgs& a)
 {
   const AdjTab<Real> t = adj_tab<Real>(c);
+  EamHost& e = c.eam;
+  const int nel = e.nel, nz = nel * (nel + 1) / 2;
+  const int64_t rows_f = e.nrho + 1, rows_r = e.nr + 1;
   const unsigned g = (grid_for(c.n) + 7u) / 8u * 8u;
-  if (c.eam.nel > 1) {
-    k_adj_b1<Real, true><<<g, kBlock, 0, c.stream>>>(a, t);
-    k_adj_b2<Real, true><<<g, kBlock, 0, c.stream>>>(a, t);
-  } else {
-    k_adj_b1<Real, false><<<g, kBlock, 0, c.stream>>>(a, t);
-    k_adj_b2<Real, false><<<g, kBlock, 0, c.stream>>>(a, t);
-  }
+  const bool multi = nel > 1;
+  if (multi) k_adj_b1<Real, true><<<g, kBlock, 0, c.stream>>>(a, t);
+  else k_adj_b1<Real, false><<<g, kBlock, 0, c.stream>>>(a, t);
   XS_HIP(hipGetLastError());
+  // persistent workgroups: two per CU (80 KB of LDS each), fewer for small systems
+  const int groups = (int)std::max<int64_t>(1, std::min<int64_t>(2 * std::max(c.num_cu, 1),
+                                                                   (c.n + kAdjThreads - 1) / kAdjThreads));
+  c.adj_part.reserve((size_t)groups * kAdjRows * 4);
+  for (int pass = 0; pass < 2; pass++) {
+    for (const AdjSlots& S : adj_launches(nel, nz, rows_f, rows_r, pass == 1)) {
+      const size_t lds = (size_t)S.total * 4 * sizeof(double);
+      if (pass == 0) {
+        auto k = multi ? k_adj_fsum<Real, true> : k_adj_fsum<Real, false>;
+        XS_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)lds));
+        k<<<groups, kAdjThreads, lds, c.stream>>>(a, t, S, c.adj_part.p);
+      } else {
+        auto k = multi ? k_adj_b2<Real, true> : k_adj_b2<Real, false>;
+        XS_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)lds));
+        k<<<groups, kAdjThreads, lds, c.stream>>>(a, t, S, c.adj_part.p);
+      }
+      XS_HIP(hipGetLastError());
+      k_adj_reduce<<<(unsigned)((4 * S.total + kBlock - 1) / kBlock), kBlock, 0, c.stream>>>(S, groups, c.adj_part.p,
+                                                                                          c.adj_g.p);
+      XS_HIP(hipGetLastError());
+    }
+  }
 }
 
 }  // namespace
@@ -327,9 +472,11 @@ void eam_adjoint(Ctx& c, double rcut, const double* dLdF_id, const double* dLdE,
     a.gE = c.adj_in.p + 3 * c.n;
     a.rc2 = rcut * rcut;
     a.lam = c.adj_lam.p;
-    a.gF = c.adj_g.p;
-    a.gR = c.adj_g.p + nF;
-    a.gZ = c.adj_g.p + nF + nR;
+    c.adj_fc.reserve(4 * (size_t)c.npad);
+    c.adj_fm.reserve(c.npad);
+    a.fc = c.adj_fc.p;
+    a.fm = c.adj_fm.p;
+    a.nel = nel;
     ProfScope prof(c.prof, c.stream, XS_PROF_ADJOINT);
     if (mixed) adj_run<float>(c, a);
     else adj_run<double>(c, a);

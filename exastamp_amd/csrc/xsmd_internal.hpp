@@ -302,6 +302,8 @@ struct Ctx {
 
   // EAM adjoint scratch: coefficient gradients, lambda_i, dL/dF records, inputs
   DBuf<double> adj_g, adj_lam, adj_in;
+  DBuf<double> adj_fc, adj_part;  // per-atom F-table contributions; per-workgroup partial rows
+  DBuf<int> adj_fm;               // per-atom F-table row
   DBuf<Pos4> adj_gvec;
 
   Profiler prof;
