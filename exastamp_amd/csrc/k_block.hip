@@ -716,6 +716,12 @@ __global__ void __launch_bounds__(kPassThreads, kPassWaves<false>) k_block_eam_r
       lslots(w, ca.local, k0, cnt, c);
       LNb nb[kBBatch];
       lgather<kBBatch>(a, l.rec, c, k0, cnt, pi, nb);
+      // a batch with no pair inside rc on any lane of the wave (the tails of the
+      // distance-sorted lists) skips the table reads; skipped terms would add 0
+      bool any = false;
+#pragma unroll
+      for (int u = 0; u < kBBatch; u++) any |= nb[u].in;
+      if (!__any(any)) continue;
 #pragma unroll
       for (int u = 0; u < kBBatch; u++) {
         const int tj = MULTI ? l.ltype[nb[u].c] : 0;
@@ -775,6 +781,7 @@ __global__ void __launch_bounds__(kPassThreads, kPassWaves<VIRIAL>) k_block_eam_
       for (int sub = 0; sub < kBBatch; sub += 2) {
         LNb nb[2];
         lgather<2>(a, l.rec, c + sub, k0 + 2 * sub, cnt, pi, nb);
+        if (!__any(nb[0].in || nb[1].in)) continue;  // as in the density pass
 #pragma unroll
         for (int u = 0; u < 2; u++) {
           const int tj = MULTI ? l.ltype[nb[u].c] : 0;
