@@ -187,7 +187,10 @@ xs_status xs_ctx_create(int device, xs_ctx** out)
     }
     if (const char* ba = std::getenv("XSMD_BLOCK_ATOMS")) {
       const double v = std::atof(ba);
-      if (v >= 1.0 && v <= 4096.0) c->block_atoms = v;
+      if (v >= 1.0 && v <= 4096.0) {
+        c->block_atoms = v;
+        c->block_atoms_set = true;
+      }
     }
   } catch (const Error&) {
     delete c;
@@ -437,7 +440,7 @@ xs_status xs_pair_count(xs_ctx* h, double rcut, int64_t* entries, int64_t* withi
   });
 }
 
-xs_status xs_list_info(xs_ctx* h, int* kind, int64_t* blocks, int* halo_capacity)
+xs_status xs_list_info(xs_ctx* h, int* kind, int64_t* blocks, int* halo_capacity, int* central_max, int* brick)
 {
   return guarded(h, [&](Ctx& c) {
     require_list(c);
@@ -445,6 +448,12 @@ xs_status xs_list_info(xs_ctx* h, int* kind, int64_t* blocks, int* halo_capacity
     if (kind) *kind = blk ? 1 : 0;
     if (blocks) *blocks = blk ? c.bgeom.nblocks : 0;
     if (halo_capacity) *halo_capacity = blk ? c.bcap : 0;
+    if (central_max) *central_max = blk ? c.bcentral : 0;
+    if (brick) {
+      brick[0] = blk ? c.bgeom.bx : 0;
+      brick[1] = blk ? c.bgeom.by : 0;
+      brick[2] = blk ? c.bgeom.bz : 0;
+    }
   });
 }
 

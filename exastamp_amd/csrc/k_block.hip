@@ -1274,23 +1274,33 @@ void cell_offsets(Ctx& c)
   XS_HIP(hipGetLastError());
 }
 
-void block_setup(Ctx& c)
+// Brick shape: the one that stages the fewest halo cells per grid cell (partial
+// bricks at the grid edge count as whole ones) among the shapes whose expected
+// central atoms stay within `bound` (one round of the 256 atom slots of a pass
+// workgroup, with headroom for density fluctuations) and, unless XSMD_BLOCK_ATOMS
+// set the bound, whose expected halo leaves LDS for three pass workgroups per CU
+// (32-byte records, 15 % headroom).
+void block_shape_choose(const Ctx& c, BlockGeom& g, double bound, double per_cell)
 {
-  // brick of about kBlockAtoms atoms, as cubic as the grid allows
-  const double per_cell = (double)c.n / (double)c.ncell_tot;
-  const int want = std::max(1, (int)std::lround(c.block_atoms / std::max(per_cell, 0.5)));
-  BlockGeom& g = c.bgeom;
-  g.nx = c.nc[0];
-  g.ny = c.nc[1];
-  g.nz = c.nc[2];
-  g.px = c.periodic[0];
-  g.py = c.periodic[1];
-  g.pz = c.periodic[2];
-  g.ncell_rep = c.ncell_rep;
-  int bz = std::max(1, std::min(g.nz, (int)std::floor(std::cbrt((double)want))));
-  int by = std::max(1, std::min(g.ny, (int)std::floor(std::sqrt((double)want / bz))));
-  int bx = std::max(1, std::min(g.nx, (int)std::lround((double)want / (by * bz))));
-  while ((bx + 2) * (by + 2) * (bz + 2) > kMaxHaloCells && bx > 1) bx--;
+  const double lds_halo_cells = (double)(kLdsPerCu / 3 - 2048) / (32.0 * 1.15 * per_cell);
+  int bx = 1, by = 1, bz = 1;
+  double best = 1e300;
+  for (int z = 1; z <= std::min(g.nz, 8); z++)
+    for (int y = 1; y <= std::min(g.ny, 8); y++)
+      for (int x = 1; x <= std::min(g.nx, 8); x++) {
+        const int halo = (x + 2) * (y + 2) * (z + 2);
+        if (x * y * z > 1 && (x * y * z * per_cell > bound || halo > kMaxHaloCells ||
+                              (!c.block_atoms_set && halo > lds_halo_cells)))
+          continue;
+        const double nb = (double)((g.nx + x - 1) / x) * ((g.ny + y - 1) / y) * ((g.nz + z - 1) / z);
+        const double cost = nb * halo / ((double)g.nx * g.ny * g.nz);
+        if (cost < best * (1.0 - 1e-9)) {
+          best = cost;
+          bx = x;
+          by = y;
+          bz = z;
+        }
+      }
   g.bx = bx;
   g.by = by;
   g.bz = bz;
@@ -1299,15 +1309,39 @@ void block_setup(Ctx& c)
   g.nbz = (g.nz + bz - 1) / bz;
   g.nblocks = (int64_t)g.nbx * g.nby * g.nbz * c.nrep;
   g.nh_max = (bx + 2) * (by + 2) * (bz + 2);
+}
 
-  XS_HIP(hipMemsetAsync(c.dscalar.p + 2, 0, 2 * sizeof(unsigned long long), c.stream));
-  k_block_halo_max<<<grid_for(g.nblocks), kBlock, 0, c.stream>>>(g, c.cell_off.p, c.dscalar.p + 2);
-  XS_HIP(hipGetLastError());
-  XS_HIP(hipMemcpyAsync(c.hscalar + 2, c.dscalar.p + 2, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
-                        c.stream));
-  XS_HIP(hipStreamSynchronize(c.stream));
-  c.bcap = (int)((c.hscalar[2] + 63) / 64 * 64);
-  c.bcentral = (int)c.hscalar[3];
+void block_setup(Ctx& c)
+{
+  BlockGeom& g = c.bgeom;
+  g.nx = c.nc[0];
+  g.ny = c.nc[1];
+  g.nz = c.nc[2];
+  g.px = c.periodic[0];
+  g.py = c.periodic[1];
+  g.pz = c.periodic[2];
+  g.ncell_rep = c.ncell_rep;
+  // The shape is chosen from the mean atoms per cell; where atoms crowd into part
+  // of the grid (slabs, clusters) the measured halo can exceed the LDS budget, and
+  // the bound shrinks until it fits (or the bricks are single cells).
+  double per_cell = std::max((double)c.n / (double)c.ncell_tot, 1e-3);
+  double bound = c.block_atoms;
+  for (int attempt = 0;; attempt++) {
+    block_shape_choose(c, g, bound, per_cell);
+    XS_HIP(hipMemsetAsync(c.dscalar.p + 2, 0, 2 * sizeof(unsigned long long), c.stream));
+    k_block_halo_max<<<grid_for(g.nblocks), kBlock, 0, c.stream>>>(g, c.cell_off.p, c.dscalar.p + 2);
+    XS_HIP(hipGetLastError());
+    XS_HIP(hipMemcpyAsync(c.hscalar + 2, c.dscalar.p + 2, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                          c.stream));
+    XS_HIP(hipStreamSynchronize(c.stream));
+    c.bcap = (int)((c.hscalar[2] + 63) / 64 * 64);
+    c.bcentral = (int)c.hscalar[3];
+    const size_t budget = c.block_atoms_set ? kBlockLdsLimit : std::min(kBlockLdsLimit, kLdsPerCu / 3);
+    if (block_lds_bytes(c, true, true) <= budget || g.nh_max == 27 || attempt == 6) break;
+    // the densest halo set the capacity: size the next shape for that density
+    per_cell = std::max(per_cell, (double)c.hscalar[2] / g.nh_max);
+    bound *= 0.75;
+  }
 }
 
 size_t block_lds_bytes(const Ctx& c, bool with_type, bool with_w)

@@ -142,7 +142,8 @@ struct BlockGeom {
   int64_t nblocks;   // over all replicas
 };
 
-constexpr double kBlockAtoms = 200.0;      // target central atoms per brick
+constexpr double kBlockAtoms = 230.0;      // largest expected central atoms per brick
+constexpr size_t kLdsPerCu = 160 * 1024;   // LDS of one CU
 constexpr int kMaxHaloCells = 216;
 constexpr size_t kBlockLdsLimit = 64 * 1024;  // default dynamic-LDS limit of a launch
 
@@ -270,6 +271,7 @@ struct Ctx {
   bool global_from_block = false;  // nbr holds the block list converted to global form
   bool block_disabled = false;     // XSMD_LIST=global at context creation
   double block_atoms = kBlockAtoms;  // XSMD_BLOCK_ATOMS at context creation
+  bool block_atoms_set = false;
   DBuf<uint16_t> nbr16;
   DBuf<int> cell_off;              // first atom of each cell, ncell_tot + 1
   BlockGeom bgeom{};

@@ -303,10 +303,13 @@ class Simulation:
         return {"entries": e.value, "within_rcut": w.value}
 
     def list_info(self) -> dict:
-        """{'kind': 'global'|'block', 'blocks': bricks, 'halo_capacity': LDS slots}."""
-        kind, blocks, cap = C.c_int(0), C.c_int64(0), C.c_int(0)
-        self.ctx.xs_list_info(C.byref(kind), C.byref(blocks), C.byref(cap))
-        return {"kind": ("global", "block")[kind.value], "blocks": blocks.value, "halo_capacity": cap.value}
+        """{'kind': 'global'|'block', 'blocks': bricks, 'halo_capacity': LDS slots,
+        'central_max': most central atoms of a brick, 'brick': cells per brick (x, y, z)}."""
+        kind, blocks, cap, cen = C.c_int(0), C.c_int64(0), C.c_int(0), C.c_int(0)
+        brick = (C.c_int * 3)()
+        self.ctx.xs_list_info(C.byref(kind), C.byref(blocks), C.byref(cap), C.byref(cen), brick)
+        return {"kind": ("global", "block")[kind.value], "blocks": blocks.value, "halo_capacity": cap.value,
+                "central_max": cen.value, "brick": tuple(brick)}
 
     def eam_pass_info(self) -> dict:
         """Table rows the last EAM density / force passes kept in LDS (0: all from HBM)."""

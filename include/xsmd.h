@@ -121,9 +121,12 @@ xs_status xs_pair_count(xs_ctx* ctx, double rcut, int64_t* entries, int64_t* wit
 /* Which list the last build produced (diagnostic; no counterpart in the reference):
  * *kind = 0 for the global (index, image) list, 1 for the cell-block list of the
  * LDS-staged EAM kernels; *blocks = bricks of cells, *halo_capacity = LDS slots per
- * brick (both 0 for the global list).  The environment variable XSMD_LIST=global,
- * read when a context is created, keeps every build on the global list. */
-xs_status xs_list_info(xs_ctx* ctx, int* kind, int64_t* blocks, int* halo_capacity);
+ * brick, *central_max = most central atoms in one brick, brick[0..2] = cells per
+ * brick along x, y, z (all 0 for the global list; any pointer may be NULL).  The
+ * environment variable XSMD_LIST=global, read when a context is created, keeps every
+ * build on the global list; XSMD_BLOCK_ATOMS=n bounds the expected central atoms
+ * per brick (default 230). */
+xs_status xs_list_info(xs_ctx* ctx, int* kind, int64_t* blocks, int* halo_capacity, int* central_max, int* brick);
 /* Table rows the last EAM density and force passes kept in LDS (diagnostic; no
  * counterpart in the reference): 0 when a pass read all its rows from HBM.  With
  * XSMD_SPLIT_TABLE=1 the single-element FP32 passes on the cell-block list run as

@@ -151,7 +151,7 @@ def test_block_adjoint_and_pair_count_use_converted_list(monkeypatch):
 
 
 def _env(monkeypatch, **kv):
-    for k in ("XSMD_LIST", "XSMD_SPLIT_TABLE", "XSMD_SPLIT_ROWS", "XSMD_LIST_SORT"):
+    for k in ("XSMD_LIST", "XSMD_SPLIT_TABLE", "XSMD_SPLIT_ROWS", "XSMD_LIST_SORT", "XSMD_BLOCK_ATOMS"):
         monkeypatch.delenv(k, raising=False)
     for k, v in kv.items():
         monkeypatch.setenv(k, v)
@@ -225,3 +225,22 @@ def test_distance_sorted_lists(monkeypatch, name):
     for mixed in (False, True):
         for x, y in zip(_outputs(sims["sorted"], tab.rc, mixed), _outputs(sims["stencil"], tab.rc, mixed)):
             assert _close(x, y, 1e-6 if mixed else 1e-10)
+
+
+def test_bricks_beyond_one_round(monkeypatch):
+    """Bricks of more central atoms than the 256 atom slots of a pass workgroup walk
+    their atoms in two rounds: FCC Cu 16^3 unit cells on an 11^3 cell grid (12.3 atoms
+    per cell) with XSMD_BLOCK_ATOMS=340 gives 3x3x3-cell bricks of ~330 atoms.  Lists
+    equal to the global build, forces as in the block-vs-global tests."""
+    cfg, masses, names, tab = cu_config((16, 16, 16), jitter=0.08, seed=21), [CU_MASS], None, cu_tables()
+    _env(monkeypatch, XSMD_BLOCK_ATOMS="340")
+    b = _sim(monkeypatch, "block", cfg, masses, names, tab)
+    _env(monkeypatch)
+    g = _sim(monkeypatch, "global", cfg, masses, names, tab)
+    info = b.list_info()
+    assert info["central_max"] > 256, info
+    for x, y in zip(b.neighbors(), g.neighbors()):
+        assert np.array_equal(x, y)
+    for mixed, tol in ((False, 1e-10), (True, 1e-6)):
+        for x, y in zip(_outputs(b, tab.rc, mixed), _outputs(g, tab.rc, mixed)):
+            assert _close(x, y, tol)
