@@ -1,0 +1,711 @@
+"""Runner for exaStamp YAML operator-graph decks, restricted to the force path.
+
+exaStamp runs a simulation as a graph of named operators described in YAML
+``.msp`` decks (data/config/main-config.msp and the files it includes; a user deck
+overrides nodes by name).  This module reads such decks -- ``includes``, node
+aliases, batches, blocks with ``rebind`` / ``body`` / ``loop`` / ``condition``,
+``+name`` list appends, ``--set-a-b-c value`` overrides -- and executes the subset
+of the graph this library implements: ``setup_system`` (species, particle types,
+domain, FCC/BCC/SC ``lattice``, ``gaussian_noise_r/v``), ``init_parameters``
+(``eam_alloy_init``), the NVE scheme (``verlet_first_half``,
+``check_and_update_particles``, ``compute_all_forces_energy``,
+``verlet_second_half``), the force operators (``lj_compute_force``,
+``eam_alloy_force`` with its pass flags), ``chunk_neighbors`` and the
+thermodynamic state on the screen/file frequencies of ``global``.
+
+The default graph below restates the node names and the control flow of the
+reference's default NVE simulation (main-config.msp:145-170,232-255,
+config_numerical_schemes.msp:8-52, config_move_particles.msp:17-152,
+config_globals.msp:5-29) for that subset; a deck overrides it exactly as it
+overrides the reference's defaults.  Operators of the reference outside the path
+(load balancing, snapshots, restarts, analysis, ghost exchanges -- one GPU holds
+the whole periodic box, images are resolved inside the library) are accepted as
+no-ops; any other operator name raises ``DeckError``, as exaStamp fails on an
+unknown operator.
+
+Differences that a deck cannot see in its results: the Newton-symmetric variants
+(``eam_symmetry: true``, ``lj_compute_force_symetric``) run the full-list kernels
+(same forces up to FP64 summation order); ``chunk_size`` is accepted and does not
+change the list; the random streams of ``gaussian_noise_r/v`` are numpy Philox
+streams (seed ``--seed``), not exaNBody's.
+
+    python -m exastamp_amd.deck single_specy_nosym.msp --set-global-max_iteration 20
+"""
+from __future__ import annotations
+
+import argparse
+import copy
+import dataclasses
+import sys
+from pathlib import Path
+
+import numpy as np
+import yaml
+
+from .inputs import Config
+from .units import EV, parse_quantity
+
+
+class DeckError(RuntimeError):
+    """A deck the runner cannot execute (unknown operator, bad slot value)."""
+
+
+# Default graph: the reference's node names and control flow for the NVE path.
+DEFAULT_GRAPH = """
+global:
+  max_iteration: 1000
+  timestep: 0
+  physical_time: 0.
+  dt: 1.0e-3 ps
+  rcut_inc: 1.0 ang
+  simulation_thermostate_screen_frequency: 10
+  simulation_thermostate_file_frequency: 10
+  thermostate_file: "thermodynamic_state.csv"
+  trigger_thermo_state: true
+  md_loop_continue: true
+init_parameters: nop
+generate_default_species: species
+compute_force_prolog: zero_force_energy
+compute_force: nop
+compute_force_epilog: force_to_accel
+compute_all_forces_energy:
+  - compute_force_prolog
+  - compute_force
+  - compute_force_epilog
+verlet_first_half:
+  - push_f_v_r: { dt_scale: 1.0, xform_mode: INV_XFORM }
+  - push_f_v: { dt_scale: 0.5, xform_mode: IDENTITY }
+verlet_second_half:
+  - push_f_v: { dt_scale: 0.5, xform_mode: IDENTITY }
+numerical_scheme: verlet_nve
+verlet_nve:
+  name: NVE_scheme
+  body:
+    - verlet_first_half
+    - check_and_update_particles
+    - compute_all_forces_energy
+    - verlet_second_half
+chunk_neighbors:
+  config:
+    chunk_size: 1
+begin_iteration:
+  - trigger_thermostate_screen
+  - trigger_thermostate_file
+end_iteration:
+  - thermostate_compute_if_triggered
+  - thermostate_screen_if_triggered
+  - thermostate_file_if_triggered
+first_iteration:
+  - init_particles
+  - compute_all_forces_energy
+  - default_thermostate_compute
+  - default_thermostate_screen
+  - default_thermostate_file
+  - next_time_step
+md_trajectory_loop:
+  loop: true
+  name: md_loop
+  condition: md_loop_continue
+  body:
+    - begin_iteration
+    - numerical_scheme
+    - end_iteration
+    - next_time_step
+    - md_loop_stop
+md_loop_stop:
+  rebind: { end_at: max_iteration , result: md_loop_continue }
+  body:
+    - sim_continue
+simulation_epilog: nop
+simulation:
+  name: exaStamp_simulation
+  body:
+    - global
+    - init_parameters
+    - generate_default_species
+    - setup_system
+    - init_rcut_max
+    - first_iteration
+    - md_trajectory_loop
+    - simulation_epilog
+"""
+
+# Operators of the reference that have nothing to do on one GPU or outside the path.
+NOOPS = {
+    "nop", "print_logo_banner", "print_version_info", "message", "hw_device_init", "hw_device_finalize",
+    "mpi_comm_world", "init_cuda", "finalize_cuda", "update_ghost_config", "make_empty_grid", "grid_flavor",
+    "grid_flavor_full", "grid_flavor_multimat", "grid_flavor_minimal", "particle_regions", "init_prolog",
+    "init_epilog", "init_rcb_grid", "grid_post_processing",
+    "reduce_species_after_read", "print_domain", "performance_adviser", "memory_stats", "grid_stats",
+    "load_balance_auto_tune_start", "load_balance_auto_tune_end", "nose_hoover_additional_step",
+    "md_loop_prolog", "md_loop_epilog", "trigger_restart", "trigger_analysis", "trigger_snapshot",
+    "write_restart_if_triggered", "perform_analysis_if_triggered", "write_snapshot_if_triggered",
+    "write_final_restart", "write_restart", "ghost_update_r", "ghost_update_opt", "ghost_update_all_no_fv",
+    "update_force_energy_from_ghost", "ghost_config", "ghost_config_opt", "configuration", "preinit_rcut_max",
+    "nbh_dist", "xxx_simulation_epilog",
+}
+
+
+def _merge(base, over):
+    """Deck override: maps merge recursively, anything else replaces; '+key' appends."""
+    if not isinstance(base, dict) or not isinstance(over, dict):
+        return copy.deepcopy(over)
+    out = dict(base)
+    for k, v in over.items():
+        if isinstance(k, str) and k.startswith("+"):
+            key = k[1:]
+            prev = out.get(key) or []
+            out[key] = list(prev if isinstance(prev, list) else [prev]) + list(v if isinstance(v, list) else [v])
+        elif k in out:
+            out[k] = _merge(out[k], v)
+        else:
+            out[k] = copy.deepcopy(v)
+    return out
+
+
+def _read_yaml(path: Path) -> dict:
+    with open(path) as f:
+        doc = yaml.safe_load(f)
+    if doc is None:
+        return {}
+    if not isinstance(doc, dict):
+        raise DeckError(f"{path}: a deck is a map of node definitions")
+    return doc
+
+
+def load_deck(path, search=(), sets=()) -> dict:
+    """The deck at `path` merged over the default graph, includes resolved (relative
+    to the including file, then the directories in `search`), then `sets`
+    overrides applied: (["global", "max_iteration"], "20") pairs."""
+    path = Path(path)
+    seen = []
+
+    def load(p: Path) -> dict:
+        p = p.resolve()
+        if p in seen:
+            raise DeckError(f"include cycle through {p}")
+        seen.append(p)
+        doc = _read_yaml(p)
+        merged = {}
+        for inc in doc.pop("includes", None) or []:
+            q = _find(str(inc), p.parent, search)
+            merged = _merge(merged, load(q))
+        seen.pop()
+        return _merge(merged, doc)
+
+    cfg = _merge(yaml.safe_load(DEFAULT_GRAPH), load(path))
+    for keys, value in sets:
+        node = cfg
+        for k in keys[:-1]:
+            node = node.setdefault(k, {})
+            if not isinstance(node, dict):
+                raise DeckError(f"--set-{'-'.join(keys)}: {k} is not a map")
+        node[keys[-1]] = yaml.safe_load(value)
+    cfg["__deck_dir__"] = str(path.resolve().parent)
+    cfg["__search__"] = [str(s) for s in search]
+    return cfg
+
+
+def _find(name: str, here: Path, search) -> Path:
+    for d in [here, *[Path(s) for s in search]]:
+        if (d / name).exists():
+            return d / name
+    raise DeckError(f"cannot find {name!r} (looked in {here} and {list(map(str, search))})")
+
+
+# --- lattices (exaNBody's lattice operator; bulk_lattice.cpp:17-31 wraps it) -------
+_BASES = {
+    "SC": np.zeros((1, 3)),
+    "BCC": np.array([[0.0, 0.0, 0.0], [0.5, 0.5, 0.5]]),
+    "FCC": np.array([[0.0, 0.0, 0.0], [0.5, 0.5, 0.0], [0.5, 0.0, 0.5], [0.0, 0.5, 0.5]]),
+}
+
+
+def _quantity_list(v):
+    return [parse_quantity(x) for x in v]
+
+
+@dataclasses.dataclass
+class ThermoRecord:
+    step: int
+    time_ps: float
+    particles: int
+    total_ev: float      # per particle
+    kinetic_ev: float
+    potential_ev: float
+    temperature_k: float
+    pressure_pa: float
+    volume: float
+
+    def line(self) -> str:
+        return (f"{self.step:9d}  {self.time_ps:.6e}  {self.particles:12d}  {self.total_ev: .10e}  "
+                f"{self.kinetic_ev: .10e}  {self.potential_ev: .10e}  {self.temperature_k:.6f}  "
+                f"{self.pressure_pa:.6e}  {self.volume:.6f}")
+
+
+THERMO_HEADER = ("# Step     Time (ps)     Particles  Tot. E. (eV/part)  Kin. E. (eV/part)  Pot. E. (eV/part)  "
+                 "Temp. (K)  Pressure (Pa)  Vol. (ang^3)")
+
+
+class Runner:
+    """Executes a loaded deck on one GPU through the operator mirror."""
+
+    def __init__(self, cfg: dict, device: int = 0, mixed: bool = False, seed: int = 0, out=None,
+                 thermo_file: str | None = None):
+        from .operators import Simulation  # the C ABI is loaded only when a deck runs
+        self.cfg = cfg
+        self.mixed = mixed
+        self.seed = seed
+        self.out = out if out is not None else sys.stdout
+        self.thermo_file = thermo_file
+        self.sim = Simulation(device)
+        self.slots: dict = {"rcut_max": 0.0}
+        self.rebind: list[dict] = []
+        self.records: list[ThermoRecord] = []
+        self.species: list[tuple[str, float]] = []
+        self.type_map: dict[str, int] = {}
+        self.domain: dict = {}
+        self.atoms: Config | None = None
+        self.init_atoms: Config | None = None  # the configuration the MD started from
+        self.tables_loaded = None
+        self.dry = False
+        self._file = None
+        self.rebuilds = 0
+
+    # --- slots with rebind ----------------------------------------------------
+    def _slot_name(self, name):
+        for m in reversed(self.rebind):
+            if name in m:
+                return m[name]
+        return name
+
+    def get(self, name, default=None):
+        return self.slots.get(self._slot_name(name), default)
+
+    def set(self, name, value):
+        self.slots[self._slot_name(name)] = value
+
+    # --- graph execution ------------------------------------------------------
+    def run(self) -> list[ThermoRecord]:
+        try:
+            self.node("simulation")
+        finally:
+            if self._file:
+                self._file.close()
+        return self.records
+
+    def node(self, name: str, params=None):
+        ops = _OPS
+        d = self.cfg.get(name, None) if name in self.cfg else _MISSING
+        if isinstance(d, str):
+            return self.node(d, params)
+        if isinstance(d, list) and name in ops:  # an operator given a list value (species: [...])
+            return ops[name](self, _merge({"value": d}, params or {}))
+        if isinstance(d, list):
+            if params:
+                raise DeckError(f"{name} is a batch; it takes no parameters")
+            for item in d:
+                self.item(item)
+            return None
+        if isinstance(d, dict) and "body" in d:
+            return self.block(name, d)
+        own = d if isinstance(d, dict) else {}
+        p = _merge(own, params or {})
+        if name in ops:
+            return ops[name](self, p)
+        if name in NOOPS:
+            return None
+        raise DeckError(f"unknown operator {name!r}")
+
+    def item(self, item):
+        if isinstance(item, str):
+            return self.node(item)
+        if isinstance(item, dict) and len(item) == 1:
+            (k, v), = item.items()
+            return self.node(k, v if isinstance(v, dict) else ({} if v is None else {"value": v}))
+        raise DeckError(f"cannot read batch item {item!r}")
+
+    def block(self, name, d):
+        self.rebind.append(dict(d.get("rebind") or {}))
+        try:
+            body = d.get("body") or []
+            if d.get("loop"):
+                cond = d.get("condition")
+                if cond is None:
+                    raise DeckError(f"loop {name} has no condition")
+                while bool(self.get(cond, True)):
+                    for item in body:
+                        self.item(item)
+            else:
+                if d.get("condition") is not None and not bool(self.get(d["condition"], True)):
+                    return None
+                for item in body:
+                    self.item(item)
+        finally:
+            self.rebind.pop()
+        return None
+
+    # --- helpers ----------------------------------------------------------------
+    def path(self, name: str) -> Path:
+        return _find(name, Path(self.cfg["__deck_dir__"]), self.cfg.get("__search__", []))
+
+    def dt(self) -> float:
+        return parse_quantity(self.get("dt"))
+
+    def thermo(self) -> ThermoRecord:
+        st = self.sim.simulation_thermodynamic_state()
+        n = st.particle_count
+        rec = ThermoRecord(step=int(self.get("timestep", 0)), time_ps=float(self.get("physical_time", 0.0)),
+                           particles=n, total_ev=st.total_energy / EV / n, kinetic_ev=st.kinetic_energy / EV / n,
+                           potential_ev=st.potential_energy / EV / n, temperature_k=st.temperature,
+                           pressure_pa=st.pressure_pa, volume=st.volume)
+        self.records.append(rec)
+        return rec
+
+
+_MISSING = object()
+
+
+def _op_global(r: Runner, p):
+    for k, v in p.items():
+        r.set(k, v)
+
+
+def _op_species(r: Runner, p):
+    spec = p.get("species", p.get("value"))
+    if spec is None and p and "verbose" not in p and "fail_if_empty" not in p:
+        spec = p
+    if not spec:
+        if p.get("fail_if_empty") and not r.species:
+            raise DeckError("no species defined")
+        return
+    out = []
+    for s in spec if isinstance(spec, list) else [spec]:
+        for name, props in s.items():
+            out.append((name, parse_quantity(props["mass"])))
+    r.species = out
+
+
+def _op_particle_types(r: Runner, p):
+    r.type_map = {k: int(v) for k, v in (p.get("particle_type_map") or {}).items()}
+
+
+def _op_particle_type_add_properties(r: Runner, p):
+    masses = {}
+    for name, props in p.items():
+        if isinstance(props, dict) and "mass" in props:
+            masses[name] = parse_quantity(props["mass"])
+    if not r.type_map:
+        r.type_map = {n: i for i, n in enumerate(masses)}
+    spec = sorted(r.type_map.items(), key=lambda kv: kv[1])
+    known = dict(r.species)
+    r.species = [(n, masses.get(n, known.get(n))) for n, _ in spec]
+    if any(m is None for _, m in r.species):
+        raise DeckError("particle_type_add_properties: a type has no mass")
+
+
+def _op_domain(r: Runner, p):
+    d = dict(r.domain)
+    d.update(p)
+    xf = d.get("xform")
+    if xf is not None and not np.allclose(np.asarray(xf, dtype=float), np.eye(3)):
+        raise DeckError("domain: only the identity xform is supported")
+    r.domain = d
+
+
+def _op_lattice(r: Runner, p):
+    struct = str(p.get("structure", "FCC")).upper()
+    if struct not in _BASES:
+        raise DeckError(f"lattice: structure {struct} is not supported (SC, BCC, FCC)")
+    basis = _BASES[struct]
+    types = p.get("types") or []
+    if len(types) != len(basis):
+        raise DeckError(f"lattice: {struct} needs {len(basis)} types, got {len(types)}")
+    size = np.asarray(_quantity_list(p["size"]))
+    b = r.domain.get("bounds")
+    if b is None:
+        raise DeckError("lattice: the domain has no bounds")
+    lo = np.asarray(_quantity_list(b[0]))
+    hi = np.asarray(_quantity_list(b[1]))
+    reps = np.rint((hi - lo) / size).astype(int)
+    if np.any(reps < 1) or not np.allclose(reps * size, hi - lo, rtol=1e-9, atol=1e-9):
+        raise DeckError("lattice: the domain is not a whole number of unit cells")
+    ii, jj, kk = np.meshgrid(np.arange(reps[0]), np.arange(reps[1]), np.arange(reps[2]), indexing="ij")
+    cells = np.stack([ii.ravel(), jj.ravel(), kk.ravel()], axis=1).astype(float)
+    pos = (cells[:, None, :] + basis[None, :, :]).reshape(-1, 3) * size + lo
+    tmap = r.type_map or {n: i for i, (n, _) in enumerate(r.species)}
+    tb = np.asarray([tmap[t] for t in types], dtype=np.uint8)
+    typ = np.tile(tb, cells.shape[0])
+    per = tuple(int(bool(x)) for x in r.domain.get("periodic", [True, True, True]))
+    r.atoms = Config(r=pos, v=np.zeros_like(pos), type=typ, lo=lo[None, :], hi=hi[None, :], periodic=per)
+
+
+def _op_gaussian_noise_r(r: Runner, p):
+    if r.atoms is None:
+        raise DeckError("gaussian_noise_r before any atoms")
+    sigma = parse_quantity(p.get("sigma", 0.0))
+    g = np.random.Generator(np.random.Philox(key=r.seed * 2 + 0))
+    a = r.atoms
+    a.r = a.r + g.normal(scale=sigma, size=a.r.shape)
+    lo, hi = a.lo[0], a.hi[0]
+    for k in range(3):  # wrap along periodic axes, as the grid does on insertion
+        if a.periodic[k]:
+            a.r[:, k] = lo[k] + np.mod(a.r[:, k] - lo[k], hi[k] - lo[k])
+
+
+def _op_gaussian_noise_v(r: Runner, p):
+    if r.atoms is None:
+        raise DeckError("gaussian_noise_v before any atoms")
+    sigma = parse_quantity(p.get("sigma", 0.0))
+    g = np.random.Generator(np.random.Philox(key=r.seed * 2 + 1))
+    r.atoms.v = r.atoms.v + g.normal(scale=sigma, size=r.atoms.v.shape)
+
+
+def _op_setup_species_check(r: Runner):
+    if not r.species:
+        raise DeckError("no species defined")
+
+
+def _op_init_rcut_max(r: Runner, p):
+    # executing the force nodes on an empty system only publishes their rcut
+    r.dry = True
+    try:
+        r.node("compute_force")
+    finally:
+        r.dry = False
+
+
+def _op_init_particles(r: Runner, p):
+    if r.atoms is None:
+        raise DeckError("init_particles: setup_system produced no atoms")
+    _op_setup_species_check(r)
+    r.sim.species(r.species)
+    r.sim.set_config(r.atoms)
+    r.init_atoms = copy.deepcopy(r.atoms)
+    if r.tables_loaded is not None:
+        r.sim.eam_alloy_init(r.tables_loaded)
+    _build_list(r)
+
+
+def _build_list(r: Runner):
+    rmax = float(r.get("rcut_max", 0.0))
+    if rmax <= 0.0:
+        raise DeckError("no force operator published rcut_max (compute_force is empty?)")
+    conf = (r.cfg.get("chunk_neighbors") or {}).get("config") or {}
+    r.sim.chunk_neighbors(rcut_max=rmax, rcut_inc=parse_quantity(r.get("rcut_inc")), config=conf)
+    r.rebuilds += 1
+
+
+def _op_chunk_neighbors(r: Runner, p):
+    _build_list(r)
+
+
+def _op_check_and_update_particles(r: Runner, p):
+    # particle_displ_over with max_displ = rcut_inc / 2 (config_move_particles.msp:19-23,116-118)
+    if r.sim.particle_displ_over(0.5 * parse_quantity(r.get("rcut_inc"))):
+        _build_list(r)
+
+
+def _publish_rcut(r: Runner, rc):
+    r.set("rcut_max", max(float(r.get("rcut_max", 0.0) or 0.0), rc))
+
+
+def _trigger(r: Runner) -> bool:
+    return bool(r.get("trigger_thermo_state", False))
+
+
+def _op_lj(r: Runner, p):
+    rc = parse_quantity(p["rcut"])
+    _publish_rcut(r, rc)
+    if r.dry:
+        return
+    r.sim.lj_compute_force(parameters=p["parameters"], rcut=rc, trigger_thermo_state=_trigger(r),
+                           compute_virial=bool(p.get("compute_virial", True)), mixed=r.mixed)
+
+
+def _tables_from(r: Runner, params):
+    f = params.get("file") if isinstance(params, dict) else None
+    if f is None:
+        raise DeckError("eam_alloy parameters need a file")
+    from .potentials import read_setfl
+    return read_setfl(r.path(str(f)))
+
+
+def _op_eam_alloy_init(r: Runner, p):
+    t = _tables_from(r, p.get("parameters") or {})
+    r.set("parameters", t)
+    r.tables_loaded = t
+
+
+def _op_eam_alloy_force(r: Runner, p):
+    rc = parse_quantity(p["rcut"])
+    _publish_rcut(r, rc)
+    if p.get("parameters") is not None and isinstance(p["parameters"], dict) and r.tables_loaded is None:
+        r.tables_loaded = _tables_from(r, p["parameters"])   # single-pass decks give the file inline
+        if r.atoms is not None and r.sim.n:
+            r.sim.eam_alloy_init(r.tables_loaded)
+    if r.dry:
+        return
+    if r.tables_loaded is None:
+        raise DeckError("eam_alloy_force: no tables (eam_alloy_init did not run)")
+    flag = lambda k: bool(p.get(k, True))
+    r.sim.eam_alloy_force(rcut=rc, eam_rho=flag("eam_rho"), eam_rho2emb=flag("eam_rho2emb"),
+                          eam_ghost=flag("eam_ghost"), eam_force=flag("eam_force"),
+                          eam_symmetry=bool(p.get("eam_symmetry", False)), trigger_thermo_state=_trigger(r),
+                          compute_virial=bool(p.get("compute_virial", False)), mixed=r.mixed)
+
+
+def _op_zero(r: Runner, p):
+    if not r.dry:
+        r.sim.zero_force_energy()
+
+
+def _op_accel(r: Runner, p):
+    if not r.dry:
+        r.sim.force_to_accel()
+
+
+def _op_push_f_v_r(r: Runner, p):
+    r.sim.push_f_v_r(r.dt(), dt_scale=float(p.get("dt_scale", 1.0)), xform_mode=p.get("xform_mode", "INV_XFORM"))
+
+
+def _op_push_f_v(r: Runner, p):
+    r.sim.push_f_v(r.dt(), dt_scale=float(p.get("dt_scale", 0.5)), xform_mode=p.get("xform_mode", "IDENTITY"))
+
+
+def _freq_hit(r: Runner, key) -> bool:
+    f = int(r.get(key, -1) or -1)
+    step = int(r.get("timestep", 0))
+    last = step >= int(r.get("max_iteration", 0))
+    return f > 0 and (step % f == 0 or last)
+
+
+def _op_trigger_screen(r: Runner, p):
+    r.set("trigger_thermo_screen", _freq_hit(r, "simulation_thermostate_screen_frequency"))
+    r.set("trigger_thermo_state", bool(r.get("trigger_thermo_screen")) or bool(r.get("trigger_thermo_file")))
+
+
+def _op_trigger_file(r: Runner, p):
+    r.set("trigger_thermo_file", _freq_hit(r, "simulation_thermostate_file_frequency"))
+    r.set("trigger_thermo_state", bool(r.get("trigger_thermo_screen")) or bool(r.get("trigger_thermo_file")))
+
+
+def _op_thermo_compute_if(r: Runner, p):
+    if _trigger(r):
+        r.thermo()
+
+
+def _op_thermo_screen_if(r: Runner, p):
+    if r.get("trigger_thermo_screen") and r.records and r.records[-1].step == int(r.get("timestep", 0)):
+        print(r.records[-1].line(), file=r.out)
+
+
+def _op_thermo_file_if(r: Runner, p):
+    if r.get("trigger_thermo_file") and r.records and r.records[-1].step == int(r.get("timestep", 0)):
+        _write_file(r, r.records[-1])
+
+
+def _write_file(r: Runner, rec: ThermoRecord):
+    name = r.thermo_file
+    if name is None:
+        return
+    if r._file is None:
+        r._file = open(name, "w")
+        print(THERMO_HEADER, file=r._file)
+    print(rec.line(), file=r._file)
+
+
+def _op_default_thermo_compute(r: Runner, p):
+    r.thermo()
+
+
+def _op_default_thermo_screen(r: Runner, p):
+    if p.get("print_header", True):
+        print(THERMO_HEADER, file=r.out)
+    print(r.records[-1].line(), file=r.out)
+
+
+def _op_default_thermo_file(r: Runner, p):
+    if int(r.get("simulation_thermostate_file_frequency", -1) or -1) > 0:
+        _write_file(r, r.records[-1])
+
+
+def _op_next_time_step(r: Runner, p):
+    r.set("timestep", int(r.get("timestep", 0)) + 1)
+    r.set("physical_time", float(r.get("physical_time", 0.0)) + r.dt())
+
+
+def _op_sim_continue(r: Runner, p):
+    # rebind: end_at -> max_iteration, result -> md_loop_continue
+    r.set("result", int(r.get("timestep", 0)) <= int(r.get("end_at", 0)))
+
+
+_OPS = {
+    "global": _op_global,
+    "species": _op_species,
+    "particle_types": _op_particle_types,
+    "particle_type_add_properties": _op_particle_type_add_properties,
+    "domain": _op_domain,
+    "lattice": _op_lattice,
+    "bulk_lattice": _op_lattice,
+    "gaussian_noise_r": _op_gaussian_noise_r,
+    "gaussian_noise_v": _op_gaussian_noise_v,
+    "init_rcut_max": _op_init_rcut_max,
+    "init_particles": _op_init_particles,
+    "chunk_neighbors": _op_chunk_neighbors,
+    "check_and_update_particles": _op_check_and_update_particles,
+    "lj_compute_force": _op_lj,
+    "lj_compute_force_symetric": _op_lj,
+    "eam_alloy_init": _op_eam_alloy_init,
+    "eam_alloy_force": _op_eam_alloy_force,
+    "zero_force_energy": _op_zero,
+    "force_to_accel": _op_accel,
+    "push_f_v_r": _op_push_f_v_r,
+    "push_f_v": _op_push_f_v,
+    "trigger_thermostate_screen": _op_trigger_screen,
+    "trigger_thermostate_file": _op_trigger_file,
+    "trigger_thermostate_compute": lambda r, p: None,
+    "thermostate_compute_if_triggered": _op_thermo_compute_if,
+    "thermostate_screen_if_triggered": _op_thermo_screen_if,
+    "thermostate_file_if_triggered": _op_thermo_file_if,
+    "default_thermostate_compute": _op_default_thermo_compute,
+    "default_thermostate_screen": _op_default_thermo_screen,
+    "default_thermostate_file": _op_default_thermo_file,
+    "simulation_thermodynamic_state": _op_default_thermo_compute,
+    "next_time_step": _op_next_time_step,
+    "sim_continue": _op_sim_continue,
+}
+
+
+def parse_sets(argv):
+    """--set-a-b value pairs -> ([a, b], value); other arguments pass through."""
+    sets, rest = [], []
+    it = iter(argv)
+    for a in it:
+        if a.startswith("--set-"):
+            try:
+                v = next(it)
+            except StopIteration:
+                raise DeckError(f"{a} needs a value") from None
+            sets.append((a[len("--set-"):].split("-"), v))
+        else:
+            rest.append(a)
+    return sets, rest
+
+
+def main(argv=None):
+    sets, rest = parse_sets(list(sys.argv[1:] if argv is None else argv))
+    ap = argparse.ArgumentParser(description="Run an exaStamp deck (force-path subset) on one GPU.")
+    ap.add_argument("deck")
+    ap.add_argument("--config-dir", action="append", default=[], help="extra include / table search directory")
+    ap.add_argument("--mixed", action="store_true", help="FP32 pair and spline math (XS_MIXED)")
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--thermo-file", default=None)
+    a = ap.parse_args(rest)
+    cfg = load_deck(a.deck, a.config_dir, sets)
+    Runner(cfg, mixed=a.mixed, seed=a.seed, thermo_file=a.thermo_file).run()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

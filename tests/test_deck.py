@@ -1,0 +1,58 @@
+"""Deck loading (host logic, no GPU): the reference's regression decks under
+tests/golden/decks (copied from data/regression_new/potentials/{pair/lj,eam/eam_alloy})
+resolve their includes, override the default graph by node name, and take
+--set overrides; list appends ('+name') and include cycles behave as stated."""
+import pytest
+
+from conftest import GOLDEN
+
+DECKS = GOLDEN / "decks"
+
+
+def test_lj_deck_resolves():
+    from exastamp_amd import deck
+    cfg = deck.load_deck(DECKS / "lj" / "single_specy_nosym.msp")
+    assert cfg["compute_force"] == "lj_compute_force"
+    assert cfg["lj_compute_force"]["rcut"] == "8.0 ang"
+    assert cfg["global"]["max_iteration"] == 10          # from the included baseline
+    assert cfg["global"]["dt"] == "1.0e-3 ps"
+    assert cfg["global"]["md_loop_continue"] is True     # default graph value kept
+    names = [next(iter(s)) if isinstance(s, dict) else s for s in cfg["setup_system"]]
+    assert names == ["particle_types", "particle_type_add_properties", "domain", "init_rcb_grid", "lattice",
+                     "gaussian_noise_r", "gaussian_noise_v"]
+
+
+def test_eam_split_deck_blocks_and_sets():
+    from exastamp_amd import deck
+    cfg = deck.load_deck(DECKS / "eam_alloy" / "single_specy_nosym_cs1.msp",
+                         sets=[(["global", "max_iteration"], "3"), (["chunk_neighbors", "config", "chunk_size"], "4")])
+    assert cfg["global"]["max_iteration"] == 3
+    assert cfg["chunk_neighbors"]["config"] == {"chunk_size": 4, "build_particle_offset": True}
+    assert cfg["compute_force"]["rebind"] == {"parameters": "eam_alloy_parameters"}
+    assert cfg["compute_force"]["body"] == ["eam_rho", "eam_rho2emb", "ghost_update_opt", "eam_force"]
+    (op, p), = cfg["eam_rho"][0].items()
+    assert op == "eam_alloy_force" and p["eam_rho"] is True and p["eam_force"] is False
+
+
+def test_set_parsing_and_appends(tmp_path):
+    from exastamp_amd import deck
+    sets, rest = deck.parse_sets(["d.msp", "--set-global-max_iteration", "20", "--mixed"])
+    assert sets == [(["global", "max_iteration"], "20")] and rest == ["d.msp", "--mixed"]
+    base = tmp_path / "base.yaml"
+    base.write_text("setup_system:\n  - a\n  - b\nglobal: { dt: 2.0e-3 ps }\n")
+    top = tmp_path / "top.msp"
+    top.write_text("includes: [ base.yaml ]\n+setup_system:\n  - c\nglobal: { max_iteration: 5 }\n")
+    cfg = deck.load_deck(top)
+    assert cfg["setup_system"] == ["a", "b", "c"]
+    assert cfg["global"]["dt"] == "2.0e-3 ps" and cfg["global"]["max_iteration"] == 5
+
+
+def test_include_cycle_and_missing(tmp_path):
+    from exastamp_amd import deck
+    (tmp_path / "a.msp").write_text("includes: [ b.msp ]\n")
+    (tmp_path / "b.msp").write_text("includes: [ a.msp ]\n")
+    with pytest.raises(deck.DeckError, match="cycle"):
+        deck.load_deck(tmp_path / "a.msp")
+    (tmp_path / "c.msp").write_text("includes: [ nowhere.yaml ]\n")
+    with pytest.raises(deck.DeckError, match="cannot find"):
+        deck.load_deck(tmp_path / "c.msp")
