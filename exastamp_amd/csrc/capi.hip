@@ -183,7 +183,7 @@ xs_status xs_ctx_create(int device, xs_ctx** out)
     if (const char* ls = std::getenv("XSMD_LIST_SORT")) c->list_sort_disabled = std::strcmp(ls, "0") == 0;
     if (const char* sb = std::getenv("XSMD_SORT_BITS")) {
       const int v = std::atoi(sb);
-      if (v >= 1 && v <= 15) c->sort_bits = v;
+      if (v >= 1 && v <= 8) c->sort_bits = v;
     }
     if (const char* ba = std::getenv("XSMD_BLOCK_ATOMS")) {
       const double v = std::atof(ba);

@@ -12,7 +12,7 @@
 // x fastest.  Atoms of one cell are contiguous in HBM (cell-sorted order) and are
 // staged contiguously, cells in enumeration order, so ascending slot order equals
 // the visit order of the global build.  The build finds the same pairs as the
-// global build; k_block_sort then orders each list by distance, and
+// global build; k_list_sort then orders each list by distance, and
 // k_block_to_global restores slot order, so a converted block list equals the
 // global list entry for entry.
 #include "pair_math.hpp"
@@ -302,22 +302,8 @@ __device__ __forceinline__ void wave_radix_sort2(uint32_t& k0, uint32_t& k1, int
   }
 }
 
-// Lists of at most 64 entries are kept sorted by distance (see k_block_sort); this
-// many entries are all the sort touches.  Its LDS rows are kSortStride entries
-// apart: 144 bytes, a multiple of 16 whose 16 consecutive rows start in distinct
-// groups of four banks.
+// Lists of at most 64 entries are kept sorted by distance (see k_list_sort).
 constexpr int kSortRows = 64;
-constexpr int kSortStride = 72;
-
-// sort key of slot c for central slot i: distance bin (nbits) above the slot
-__device__ __forceinline__ uint32_t dist_key(const LRec& rec, int c, int i, float inv_rl, int nbits)
-{
-  const double dx = rec.x[c] - rec.x[i], dy = rec.y[c] - rec.y[i], dz = rec.z[c] - rec.z[i];
-  const float r = sqrtf((float)xsmd_d2(dx, dy, dz));
-  const int nb = 1 << nbits;
-  const uint32_t bin = (uint32_t)min(nb - 1, (int)(r * inv_rl * (float)nb));
-  return (bin << 16) | (uint32_t)c;
-}
 
 // Squared distance from p to the box of the halo cell at unwrapped grid coordinate
 // (gx, gy, gz), widened by a margin that covers the rounding of cell assignment.
@@ -373,6 +359,19 @@ __device__ __forceinline__ uint4 group_interleave(const uint32_t (&o)[4])
                     (o[0] >> 16) | (o[1] & 0xffff0000u), (o[2] >> 16) | (o[3] & 0xffff0000u));
 }
 
+// 8-bit distance keys of a group, list order (byte j of b[0..1] = entry j) -> the
+// bl_index order (0, 2, 4, 6 | 1, 3, 5, 7)
+__device__ __forceinline__ uint2 bins_interleave(const uint32_t (&b)[2])
+{
+  const uint32_t e0 = b[0] & 0xffu, e1 = (b[0] >> 8) & 0xffu, e2 = (b[0] >> 16) & 0xffu, e3 = b[0] >> 24;
+  const uint32_t e4 = b[1] & 0xffu, e5 = (b[1] >> 8) & 0xffu, e6 = (b[1] >> 16) & 0xffu, e7 = b[1] >> 24;
+  return make_uint2(e0 | (e2 << 8) | (e4 << 16) | (e6 << 24), e1 | (e3 << 8) | (e5 << 16) | (e7 << 24));
+}
+
+// BINS: each entry also gets its distance key for the sort, min(2^nbits - 1,
+// floor(d2 * bin_scale)) with bin_scale = 2^nbits / rl^2 (bins uniform in d^2), in
+// bins8, grouped like the list.
+template <bool BINS>
 __global__ void __launch_bounds__(kBuildThreads) k_block_build(BlockGeom g, const int* __restrict__ cell_off,
                                                                const Pos4* __restrict__ pos,
                                                                const BoxDev* __restrict__ box, double rl2,
@@ -380,7 +379,9 @@ __global__ void __launch_bounds__(kBuildThreads) k_block_build(BlockGeom g, cons
                                                                int nmax, int64_t npad, int cap, int nh_max,
                                                                const unsigned long long* __restrict__ nocull,
                                                                int* __restrict__ brick_rows,
-                                                               unsigned long long* __restrict__ flags)
+                                                               unsigned long long* __restrict__ flags,
+                                                               uint8_t* __restrict__ bins8, float bin_scale,
+                                                               int bin_max)
 {
   const int64_t b = lblock();
   if (b >= g.nblocks) return;
@@ -408,19 +409,30 @@ __global__ void __launch_bounds__(kBuildThreads) k_block_build(BlockGeom g, cons
     // wave issues at most one store per cell rather than one per candidate pair.
     // gp is the atom's next group (bl_index(npad, i, 8 q)); gq its index q.
     uint32_t r[4] = {0u, 0u, 0u, 0u};
+    uint32_t bw[2] = {0u, 0u}, br[2] = {0u, 0u};  // distance keys of w and r (BINS)
     bool ready = false;
     uint16_t* gp = nbr16 + bl_index(npad, a.global, 0);
+    uint8_t* bp = BINS ? bins8 + bl_index(npad, a.global, 0) : nullptr;
     const int64_t gstep = npad * 8;
     const int gmax = nmax >> 3;  // groups that fit
     int gq = 0;
     auto flush = [&]() {
-      if (gq < gmax) *reinterpret_cast<uint4*>(gp) = group_interleave(r);
+      if (gq < gmax) {
+        *reinterpret_cast<uint4*>(gp) = group_interleave(r);
+        if constexpr (BINS) *reinterpret_cast<uint2*>(bp) = bins_interleave(br);
+      }
       gp += gstep;
+      if constexpr (BINS) bp += gstep;
       gq++;
       ready = false;
     };
-    auto hit = [&](int c) {
+    auto hit = [&](int c, double d2) {
       fifo_push(w, (uint32_t)c);
+      if constexpr (BINS) {
+        const uint32_t bin = (uint32_t)min(bin_max, (int)((float)d2 * bin_scale));
+        bw[0] = __builtin_amdgcn_alignbit(bw[1], bw[0], 8);
+        bw[1] = __builtin_amdgcn_alignbit(bin, bw[1], 8);
+      }
       cnt++;
       if ((cnt & 7) == 0) {
         if (ready) flush();  // a second full group within one cell
@@ -428,6 +440,8 @@ __global__ void __launch_bounds__(kBuildThreads) k_block_build(BlockGeom g, cons
         r[1] = w[1];
         r[2] = w[2];
         r[3] = w[3];
+        br[0] = bw[0];
+        br[1] = bw[1];
         ready = true;
       }
     };
@@ -443,17 +457,23 @@ __global__ void __launch_bounds__(kBuildThreads) k_block_build(BlockGeom g, cons
             const int c1 = min(c + 1, end - 1);  // the pair's second candidate (or c again)
             const double d0 = xsmd_d2(l.rec.x[c] - xi, l.rec.y[c] - yi, l.rec.z[c] - zi);
             const double d1 = xsmd_d2(l.rec.x[c1] - xi, l.rec.y[c1] - yi, l.rec.z[c1] - zi);
-            if (d0 < rl2 && c != a.local) hit(c);
-            if (c + 1 < end && d1 < rl2 && c1 != a.local) hit(c1);
+            if (d0 < rl2 && c != a.local) hit(c, d0);
+            if (c + 1 < end && d1 < rl2 && c1 != a.local) hit(c1, d1);
           }
           if (ready) flush();
         }
     if ((cnt & 7) != 0) {  // the last, partial group: move it down (unused places hold slot 0)
-      for (int k = cnt & 7; k < 8; k++) fifo_push(w, 0u);
+      for (int k = cnt & 7; k < 8; k++) {
+        fifo_push(w, 0u);
+        bw[0] = __builtin_amdgcn_alignbit(bw[1], bw[0], 8);
+        bw[1] = __builtin_amdgcn_alignbit(0u, bw[1], 8);
+      }
       r[0] = w[0];
       r[1] = w[1];
       r[2] = w[2];
       r[3] = w[3];
+      br[0] = bw[0];
+      br[1] = bw[1];
       flush();
     }
     nnb[a.global] = cnt;
@@ -464,84 +484,61 @@ __global__ void __launch_bounds__(kBuildThreads) k_block_build(BlockGeom g, cons
   if (threadIdx.x == 0) brick_rows[b] = l.total[1];
 }
 
-// Orders each atom's block-list entries by distance (bins of rl / 2^nbits, stencil
-// order within a bin), for lists of at most kSortRows entries.  At a given list
-// position the atoms of a wave then look up nearby spline rows, which cuts the
-// cache lines each table gather of the EAM passes touches by about 40 %.  The lists
-// of a brick are staged through LDS (one row per atom) so that global reads and
-// writes coalesce; one wave sorts two atoms at a time.
-constexpr int kSortThreads = 512;
+// Orders each atom's block-list entries by their distance keys (written by the
+// build; stable, so equal keys keep ascending slot order), for lists of at most
+// kSortRows entries.  At a given list position the atoms of a wave then look up
+// nearby spline rows, which cuts the cache lines each table gather of the EAM
+// passes touches by about 40 %.  One wave takes eight atoms: each lane loads one
+// 8-entry group (slots and keys) of one atom, the groups go through LDS to one row
+// per atom, and the wave sorts two rows at a time (ballot, popcount, ds_permute).
+constexpr int kSortThreads = 256;
+constexpr int kSortWaveAtoms = 8;
+constexpr int kSortRowWords = 65;  // 64 keys + 1: the lanes of a group load start in distinct banks
 
-__global__ void __launch_bounds__(kSortThreads) k_block_sort(BlockGeom g, const int* __restrict__ cell_off,
-                                                             const Pos4* __restrict__ pos,
-                                                             const BoxDev* __restrict__ box,
-                                                             uint16_t* __restrict__ nbr16,
-                                                             const int* __restrict__ nnb, int64_t npad, int cap,
-                                                             int nh_max, int slots, float inv_rl, int nbits)
+__global__ void __launch_bounds__(kSortThreads) k_list_sort(int64_t n, uint16_t* __restrict__ nbr16,
+                                                            const uint8_t* __restrict__ bins8,
+                                                            const int* __restrict__ nnb, int64_t npad, int nbits)
 {
-  const int64_t b = lblock();
-  if (b >= g.nblocks) return;
-  LdsLayout l = lds_layout(cap, nh_max, false, false);
-  int* gq = reinterpret_cast<int*>(l.end);  // per central atom: global index, local slot, count
-  int* lq = gq + slots;
-  int* cq = lq + slots;
-  uint16_t* buf = reinterpret_cast<uint16_t*>(cq + slots);  // [atom][kSortStride]
-  const BlockShape s = block_shape(g, b);
-  const int total = build_halo(g, s, cell_off, l.hc, l.total);
-  if (total > cap) return;  // excluded by the host (cap = measured maximum)
-  const int na = central_count(s, l.hc);  // <= slots (the measured maximum)
-  for (int q = threadIdx.x; q < na; q += blockDim.x) {
-    Central ca;
-    central_atom(s, l.hc, q, ca);
-    gq[q] = (int)ca.global;
-    lq[q] = ca.local;
-    cq[q] = nnb[ca.global];
-  }
-  stage(s, l.hc, total, pos, box[s.r].L, nullptr, l.rec, nullptr);  // ends with a barrier
-  // Thread t stages the 8-entry groups t / slots, t / slots + step, ... of atom
-  // t % slots: one 16-byte load each (consecutive atoms are consecutive in memory
-  // within a cell row, so the loads and the stores below coalesce), stored in list
-  // order into the atom's LDS row, one 16-byte write.  Whole groups are staged (at
-  // most kSortRows entries), so the entries past the count go back unchanged.
-  // slots is a power of two no larger than the workgroup.
-  const int step = kSortThreads / slots;
-  const int sq = threadIdx.x % slots, sk = threadIdx.x / slots;
-  const int scnt = sq < na ? cq[sq] : 0;
-  const bool sorts = scnt >= 2 && scnt <= kSortRows;
-  if (sorts) {
-    for (int grp = sk; 8 * grp < scnt; grp += step) {
-      const uint4 w = *reinterpret_cast<const uint4*>(nbr16 + bl_index(npad, gq[sq], 8 * grp));
-      // w holds entries (0, 2), (4, 6), (1, 3), (5, 7) of the group; list order is
-      // (0, 1), (2, 3), (4, 5), (6, 7)
-      const uint4 o = make_uint4((w.x & 0xffffu) | (w.z << 16), (w.x >> 16) | (w.z & 0xffff0000u),
-                                 (w.y & 0xffffu) | (w.w << 16), (w.y >> 16) | (w.w & 0xffff0000u));
-      *reinterpret_cast<uint4*>(buf + sq * kSortStride + 8 * grp) = o;
+  __shared__ uint32_t keys[kSortThreads / 64][kSortWaveAtoms][kSortRowWords];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t a0 = ((int64_t)blockIdx.x * (kSortThreads / 64) + w) * kSortWaveAtoms;
+  const int la = lane >> 3, grp = lane & 7;
+  const int64_t a = a0 + la;
+  const int cnt = a < n ? nnb[a] : 0;
+  const bool sorts = cnt >= 2 && cnt <= kSortRows;
+  uint32_t* row = keys[w][la];
+  if (sorts && 8 * grp < cnt) {
+    const int64_t at = bl_index(npad, a, 8 * grp);
+    const uint4 sl = *reinterpret_cast<const uint4*>(nbr16 + at);
+    const uint2 bn = *reinterpret_cast<const uint2*>(bins8 + at);
+    const uint32_t h[4] = {sl.x, sl.y, sl.z, sl.w};
+#pragma unroll
+    for (int j = 0; j < 8; j++) {  // storage order (0, 2, 4, 6, 1, 3, 5, 7)
+      const int k = 8 * grp + (j < 4 ? 2 * j : 2 * (j - 4) + 1);
+      const uint32_t slot = (h[j >> 1] >> (16 * (j & 1))) & 0xffffu;
+      const uint32_t bin = ((j < 4 ? bn.x : bn.y) >> (8 * (j & 3))) & 0xffu;
+      row[k] = (bin << 16) | slot;
     }
   }
   __syncthreads();
-  const int lane = threadIdx.x & 63;
-  const int nw = blockDim.x >> 6;
-  for (int q0 = threadIdx.x >> 6; q0 < na; q0 += 2 * nw) {
-    const int q1 = q0 + nw;
-    const int cnt0 = cq[q0], cnt1 = q1 < na ? cq[q1] : 0;
-    const bool s0 = cnt0 >= 2 && cnt0 <= kSortRows, s1 = cnt1 >= 2 && cnt1 <= kSortRows;
-    if (!s0 && !s1) continue;
-    uint32_t key0 = 0xffffffffu, key1 = 0xffffffffu;
-    if (s0 && lane < cnt0) key0 = dist_key(l.rec, buf[q0 * kSortStride + lane], lq[q0], inv_rl, nbits);
-    if (s1 && lane < cnt1) key1 = dist_key(l.rec, buf[q1 * kSortStride + lane], lq[q1], inv_rl, nbits);
-    wave_radix_sort2(key0, key1, 16, nbits);
-    if (s0 && lane < cnt0) buf[q0 * kSortStride + lane] = (uint16_t)(key0 & 0xffffu);
-    if (s1 && lane < cnt1) buf[q1 * kSortStride + lane] = (uint16_t)(key1 & 0xffffu);
+#pragma unroll
+  for (int r = 0; r < kSortWaveAtoms; r += 2) {
+    const int c0 = __shfl(cnt, 8 * r), c1 = __shfl(cnt, 8 * (r + 1));
+    const bool s0 = c0 >= 2 && c0 <= kSortRows, s1 = c1 >= 2 && c1 <= kSortRows;
+    if (!s0 && !s1) continue;  // uniform across the wave
+    uint32_t k0 = (s0 && lane < c0) ? keys[w][r][lane] : 0xffffffffu;
+    uint32_t k1 = (s1 && lane < c1) ? keys[w][r + 1][lane] : 0xffffffffu;
+    wave_radix_sort2(k0, k1, 16, nbits);
+    if (s0 && lane < c0) keys[w][r][lane] = k0;
+    if (s1 && lane < c1) keys[w][r + 1][lane] = k1;
   }
   __syncthreads();
-  if (sorts) {
-    for (int grp = sk; 8 * grp < scnt; grp += step) {
-      const uint4 o = *reinterpret_cast<const uint4*>(buf + sq * kSortStride + 8 * grp);
-      // back to (0, 2), (4, 6), (1, 3), (5, 7)
-      const uint4 w = make_uint4((o.x & 0xffffu) | (o.y << 16), (o.z & 0xffffu) | (o.w << 16),
-                                 (o.x >> 16) | (o.y & 0xffff0000u), (o.z >> 16) | (o.w & 0xffff0000u));
-      *reinterpret_cast<uint4*>(nbr16 + bl_index(npad, gq[sq], 8 * grp)) = w;
-    }
+  if (sorts && 8 * grp < cnt) {
+    uint32_t o[4];
+#pragma unroll
+    for (int j2 = 0; j2 < 4; j2++)  // list order (2 j2, 2 j2 + 1)
+      o[j2] = (row[8 * grp + 2 * j2] & 0xffffu) | (row[8 * grp + 2 * j2 + 1] << 16);
+    *reinterpret_cast<uint4*>(nbr16 + bl_index(npad, a, 8 * grp)) = group_interleave(o);
   }
 }
 
@@ -1360,37 +1357,26 @@ unsigned block_grid(const Ctx& c)
   return (unsigned)((c.bgeom.nblocks + 7) / 8 * 8);
 }
 
-// Distance order of the block lists (k_block_sort); skipped when disabled or when
-// the staging does not fit (the lists then stay in stencil order).
-void block_sort(Ctx& c, double rl)
-{
-  if (c.list_sort_disabled || c.bgeom.nblocks == 0) return;
-  int slots = 64;
-  while (slots < c.bcentral) slots *= 2;
-  if (slots > kSortThreads) return;  // bricks this large keep the stencil order
-  const size_t lds = (block_lds_bytes(c, false, false) + 15) / 16 * 16 + (size_t)3 * slots * sizeof(int) +
-                     (size_t)kSortStride * slots * sizeof(uint16_t);
-  if (lds > c.lds_max) return;
-  XS_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_block_sort),
-                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  k_block_sort<<<block_grid(c), kSortThreads, lds, c.stream>>>(c.bgeom, c.cell_off.p, c.pos.p, c.box_d.p, c.nbr16.p,
-                                                               c.nnb.p, c.npad, c.bcap, c.bgeom.nh_max, slots,
-                                                               (float)(1.0 / rl), c.sort_bits);
-  XS_HIP(hipGetLastError());
-}
-
+// Block list build, then (unless XSMD_LIST_SORT=0) the distance order of each list
+// of at most 64 entries: the build writes an 8-bit distance key per entry
+// (2^bits bins uniform in d^2 over [0, rl^2)), k_list_sort orders by it.
 void block_build(Ctx& c, double rl2)
 {
   const size_t lds = block_lds_bytes(c, false, false);
+  const bool sorted = !c.list_sort_disabled;
+  const int bits = std::max(1, std::min(c.sort_bits, 8));
+  const float bin_scale = (float)((double)(1 << bits) / rl2);
   for (int attempt = 0; attempt < 2; attempt++) {
     c.nbr16.reserve((size_t)c.nmax * c.npad);
+    if (sorted) c.bins8.reserve((size_t)c.nmax * c.npad);
     c.nnb.reserve(c.npad);
     c.brick_rows.reserve(c.bgeom.nblocks);
     XS_HIP(hipMemsetAsync(c.dscalar.p, 0, 2 * sizeof(unsigned long long), c.stream));
-    k_block_build<<<block_grid(c), kBuildThreads, lds, c.stream>>>(c.bgeom, c.cell_off.p, c.pos.p, c.box_d.p, rl2,
-                                                                  c.nbr16.p, c.nnb.p, c.nmax, c.npad, c.bcap,
-                                                                  c.bgeom.nh_max, c.dscalar.p + 4, c.brick_rows.p,
-                                                                  c.dscalar.p);
+    auto k = sorted ? k_block_build<true> : k_block_build<false>;
+    k<<<block_grid(c), kBuildThreads, lds, c.stream>>>(c.bgeom, c.cell_off.p, c.pos.p, c.box_d.p, rl2, c.nbr16.p,
+                                                       c.nnb.p, c.nmax, c.npad, c.bcap, c.bgeom.nh_max,
+                                                       c.dscalar.p + 4, c.brick_rows.p, c.dscalar.p,
+                                                       sorted ? c.bins8.p : nullptr, bin_scale, (1 << bits) - 1);
     XS_HIP(hipGetLastError());
     XS_HIP(hipMemcpyAsync(c.hscalar, c.dscalar.p, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
                           c.stream));
@@ -1398,7 +1384,12 @@ void block_build(Ctx& c, double rl2)
     if (c.hscalar[1] != 0) throw Error(XS_E_STATE, "block halo exceeds its measured capacity");
     const unsigned long long need = c.hscalar[0];
     if (need == 0) {
-      block_sort(c, std::sqrt(rl2));
+      if (sorted && c.n > 0) {
+        const int64_t per_block = (int64_t)(kSortThreads / 64) * kSortWaveAtoms;
+        const unsigned grid = (unsigned)((c.n + per_block - 1) / per_block);
+        k_list_sort<<<grid, kSortThreads, 0, c.stream>>>(c.n, c.nbr16.p, c.bins8.p, c.nnb.p, c.npad, bits);
+        XS_HIP(hipGetLastError());
+      }
       return;
     }
     if (attempt == 1) throw Error(XS_E_STATE, "neighbour list overflow after resize");

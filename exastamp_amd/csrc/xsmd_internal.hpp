@@ -273,6 +273,7 @@ struct Ctx {
   double block_atoms = kBlockAtoms;  // XSMD_BLOCK_ATOMS at context creation
   bool block_atoms_set = false;
   DBuf<uint16_t> nbr16;
+  DBuf<uint8_t> bins8;  // distance keys of the block-list entries (for the sort)
   DBuf<int> cell_off;              // first atom of each cell, ncell_tot + 1
   BlockGeom bgeom{};
   int bcap = 0;                    // LDS slots per block (max halo atoms, rounded)
@@ -286,7 +287,7 @@ struct Ctx {
   bool split_enabled = false;
   int split_rows = 0;
   bool list_sort_disabled = false;  // XSMD_LIST_SORT=0: block lists stay in stencil order
-  int sort_bits = 7;                // distance bins of the list sort: 2^sort_bits over [0, rl)
+  int sort_bits = 7;                // distance keys of the list sort: 2^sort_bits bins over [0, rl^2), <= 8
   int last_rho_rows = 0, last_force_rows = 0;  // table rows in LDS of the last EAM passes (0: none)
   bool drift_done = false;  // the last force launch applied ForceEpilogue::drift
 
