@@ -92,7 +92,6 @@ def run_form(sim, cfg, tab, mixed, steps, warmup, dt, rc, skin):
     sim.ctx.xs_md_run(_lib.XS_POT_EAM, steps, dt, rc, skin, flags, 0, None, C.byref(st))
     sim.ctx.xs_synchronize()
     t1 = time.perf_counter()
-    pass_info = sim.eam_pass_info()
     ms = np.zeros(len(_lib.PROF_CLASSES))
     cnt = np.zeros(len(_lib.PROF_CLASSES), dtype=np.int64)
     sim.ctx.xs_profile_get(_lib.dptr(ms), _lib.ptr(cnt, C.c_int64))
@@ -108,15 +107,13 @@ def run_form(sim, cfg, tab, mixed, steps, warmup, dt, rc, skin):
         "value": n * steps / sec, "seconds": sec, "ms_per_step": 1e3 * sec / steps,
         "rebuilds": int(st.rebuilds), "kernels": kern,
         "drift_ev_per_atom_per_ps": float(drift), "T0": th0.temperature, "T_end": thK.temperature,
-        "warmup_rebuilds": int(st_w.rebuilds), "pass_info": pass_info,
+        "warmup_rebuilds": int(st_w.rebuilds),
     }
 
 
-def force_kernel_name(list_info, pass_info):
+def force_kernel_name(list_info):
     if list_info["kind"] != "block":
         return "k_eam_force<float, false, false, false, true>"
-    if pass_info["force_table_rows"] > 0:
-        return "k_split_eam_force<false, false, true, true>"
     return "k_block_eam_force<float, false, false, false, true, true>"
 
 
@@ -307,7 +304,7 @@ def main():
 
     # roofline of the dominant kernel (EAM force pass, mixed)
     kf = mixed["kernels"]["eam_force"]
-    kname = force_kernel_name(list_info, mixed["pass_info"])
+    kname = force_kernel_name(list_info)
     drift_fused = kname.endswith(", true>")
     bytes_atom = FORCE_BYTES_PER_ATOM + (DRIFT_BYTES_PER_ATOM if drift_fused else 0.0)
     flops_atom = FORCE_FLOPS_PER_ATOM + (DRIFT_FLOPS_PER_ATOM if drift_fused else 0.0)

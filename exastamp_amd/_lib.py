@@ -69,7 +69,6 @@ SIGNATURES = {
     "xs_pair_count": (C.c_int, [_CTX, C.c_double, _I64, _I64]),
     "xs_list_info": (C.c_int, [_CTX, C.POINTER(C.c_int), _I64, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                C.POINTER(C.c_int)]),
-    "xs_eam_pass_info": (C.c_int, [_CTX, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "xs_zero_force_energy": (C.c_int, [_CTX]),
     "xs_force_to_accel": (C.c_int, [_CTX]),
     "xs_lj_compute_force": (C.c_int, [_CTX, C.c_double, C.c_double, C.c_double, C.c_uint]),

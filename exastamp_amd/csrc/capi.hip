@@ -169,6 +169,8 @@ xs_status xs_ctx_create(int device, xs_ctx** out)
     XS_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     XS_HIP(hipHostMalloc((void**)&c->hscalar, 8 * sizeof(unsigned long long), hipHostMallocDefault));
     c->dscalar.reserve(8);
+    XS_HIP(hipMemsetAsync(c->dscalar.p, 0, 8 * sizeof(unsigned long long), c->stream));  // flags start clear
+    XS_HIP(hipStreamSynchronize(c->stream));
     c->mass_d.reserve(8);
     c->box_d.reserve(1);
     const char* kind = std::getenv("XSMD_LIST");
@@ -178,8 +180,6 @@ xs_status xs_ctx_create(int device, xs_ctx** out)
     c->num_cu = prop.multiProcessorCount;
     c->lds_max = prop.sharedMemPerBlockOptin > prop.sharedMemPerBlock ? prop.sharedMemPerBlockOptin
                                                                        : prop.sharedMemPerBlock;
-    if (const char* st = std::getenv("XSMD_SPLIT_TABLE")) c->split_enabled = std::strcmp(st, "1") == 0;
-    if (const char* sr = std::getenv("XSMD_SPLIT_ROWS")) c->split_rows = std::max(0, std::atoi(sr));
     if (const char* ls = std::getenv("XSMD_LIST_SORT")) c->list_sort_disabled = std::strcmp(ls, "0") == 0;
     if (const char* sb = std::getenv("XSMD_SORT_BITS")) {
       const int v = std::atoi(sb);
@@ -454,14 +454,6 @@ xs_status xs_list_info(xs_ctx* h, int* kind, int64_t* blocks, int* halo_capacity
       brick[1] = blk ? c.bgeom.by : 0;
       brick[2] = blk ? c.bgeom.bz : 0;
     }
-  });
-}
-
-xs_status xs_eam_pass_info(xs_ctx* h, int* rho_lds_rows, int* force_lds_rows)
-{
-  return guarded(h, [&](Ctx& c) {
-    if (rho_lds_rows) *rho_lds_rows = c.last_rho_rows;
-    if (force_lds_rows) *force_lds_rows = c.last_force_rows;
   });
 }
 

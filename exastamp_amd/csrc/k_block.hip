@@ -810,242 +810,6 @@ __global__ void __launch_bounds__(kPassThreads, kPassWaves<VIRIAL>) k_block_eam_
   }
 }
 
-// ---------------------------------------------------------------------------
-// Force pass with the rho rows in LDS (single element, FP32 tables)
-// ---------------------------------------------------------------------------
-// A random 16-byte row gather costs the CU ~95 cycles per wave through the
-// texture path from L1/L2 but ~30 from LDS, and the block force pass spends its
-// time in exactly two such gathers per pair (rho cubic, r*phi cubic) next to two
-// LDS record reads.  Here the three rho coefficients the force needs (c3..c5, for
-// rho') of the knots nearest rc sit in LDS as three float arrays, the r*phi rows
-// stay in HBM, which balances the texture path against LDS.  The table slice is
-// loaded once per workgroup: workgroups are persistent (two per CU) and walk
-// contiguous runs of bricks.  A pair whose knot lies below the slice (an unusually
-// short pair) reads its rho row from HBM.
-constexpr int kSplitThreads = 512;  // two lanes per atom, as the block passes
-
-struct SplitArgs {
-  BlockArgs a;
-  int m_lo, rows;   // LDS holds rho rows m_lo .. m_lo + rows - 1 (the last is nr - 1)
-  int64_t per_wg;   // bricks per workgroup
-};
-
-// Raw list slots of entries k0, k0 + 2, ... of atom gi.  The row read is clamped
-// into [0, cnt - 1], so every load is unconditional and in bounds, and nothing
-// reads the values until the next batch (plist_use), so those loads stay in flight
-// while this batch is evaluated (the persistent passes run four waves per SIMD).
-template <int U>
-__device__ __forceinline__ void plist(const BlockArgs& a, int64_t gi, int k0, int cnt, int (&raw)[U])
-{
-#pragma unroll
-  for (int u = 0; u < U; u++) {
-    const int kk = max(0, min(k0 + 2 * u, cnt - 1));
-    raw[u] = a.nbr16[bl_index(a.npad, gi, kk)];
-  }
-}
-
-// slots of entries k0, k0 + 2, ...: the raw values, or the atom itself past the count
-template <int U>
-__device__ __forceinline__ void plist_use(const int (&raw)[U], int self, int k0, int cnt, int (&c)[U])
-{
-#pragma unroll
-  for (int u = 0; u < U; u++) c[u] = (k0 + 2 * u < cnt) ? (raw[u] & 0xffff) : self;
-}
-
-// neighbour records of list slots c (entries k0, k0 + 2, ...)
-template <int U>
-__device__ __forceinline__ void pgather(const BlockArgs& a, const LRec& rec, const int (&c)[U], int k0, int cnt,
-                                        const Pos4& pi, LNb (&nb)[U])
-{
-#pragma unroll
-  for (int u = 0; u < U; u++) {
-    const Pos4 pj = rec.get(c[u]);
-    nb[u].x = pj.x - pi.x;
-    nb[u].y = pj.y - pi.y;
-    nb[u].z = pj.z - pi.z;
-    nb[u].d2 = xsmd_d2(nb[u].x, nb[u].y, nb[u].z);
-    nb[u].w = pj.w;
-    nb[u].c = c[u];
-    nb[u].in = (k0 + 2 * u < cnt) && nb[u].d2 <= a.rc2;
-  }
-}
-
-__device__ __forceinline__ size_t split_table_bytes(int rows)
-{
-  return ((size_t)3 * rows * sizeof(float) + 15) / 16 * 16;
-}
-
-// two workgroups per CU (LDS) make four waves per SIMD: the full 128 VGPRs
-template <bool ENERGY, bool VIRIAL, bool FUSED, bool DRIFT>
-__global__ void __launch_bounds__(kSplitThreads, 4)
-    k_split_eam_force(SplitArgs sa, EamTabDev<float> t, const float* __restrict__ pair, pm::Outputs o, pm::Drift dr)
-{
-  static_assert(!DRIFT || (FUSED && !ENERGY && !VIRIAL), "the drift epilogue is the plain MD-loop step");
-  const BlockArgs& a = sa.a;
-  extern __shared__ __align__(32) unsigned char smem[];
-  float* c3 = reinterpret_cast<float*>(smem);
-  float* c4 = c3 + sa.rows;
-  float* c5 = c4 + sa.rows;
-  unsigned char* base = smem + split_table_bytes(sa.rows);
-  LRec rec;
-  HaloCell* hc = reinterpret_cast<HaloCell*>(base + lrec_at(base, a.cap, true, rec));
-  int* total_p = reinterpret_cast<int*>(hc + a.g.nh_max);
-  for (int i = threadIdx.x; i < sa.rows; i += blockDim.x) {
-    const float* row = pm::pair_rho_row(pair, sa.m_lo + i);
-    c3[i] = row[0];
-    c4[i] = row[1];
-    c5[i] = row[2];
-  }
-  const int64_t b0 = lblock() * sa.per_wg;
-  const int64_t b1 = min(b0 + sa.per_wg, a.g.nblocks);
-  const int half = pass_half();
-  for (int64_t b = b0; b < b1; b++) {
-    const BlockShape s = block_shape(a.g, b);
-    const int total = build_halo(a.g, s, a.cell_off, hc, total_p);
-    if (total <= a.cap) {  // always (cap = measured maximum)
-      stage(s, hc, total, a.pos, a.box[s.r].L, nullptr, rec, nullptr);
-      for (int q = pass_slot();; q += kPassAtoms) {
-        Central ca;
-        if (!central_atom(s, hc, q, ca)) break;
-        const Pos4 pi = rec.get(ca.local);
-        const float fpi = (float)pi.w;
-        const int cnt = a.nnb[ca.global];
-        double fx = 0.0, fy = 0.0, fz = 0.0, e = 0.0;
-        double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-        int nxt[kBBatch];
-        plist<kBBatch>(a, ca.global, half, cnt, nxt);
-        for (int k0 = half; k0 < cnt; k0 += 2 * kBBatch) {
-          int cur[kBBatch];
-          plist_use<kBBatch>(nxt, ca.local, k0, cnt, cur);
-          LNb nb[kBBatch];
-          pgather<kBBatch>(a, rec, cur, k0, cnt, pi, nb);
-          float rr[kBBatch], rinv[kBBatch], pp[kBBatch];
-          pm::Q4<float> rc[kBBatch], zc[kBBatch];
-#pragma unroll
-          for (int u = 0; u < kBBatch; u++) {
-            int m;
-            // masked entries are evaluated at rc, whose knot is in LDS
-            pm::dist((float)(nb[u].in ? nb[u].d2 : a.rc2), rr[u], rinv[u]);
-            pm::knot_r(t, rr[u], m, pp[u]);
-            zc[u] = pm::load4(pm::pair_z2_row(pair, t, m));
-            if (m >= sa.m_lo) {
-              const int k = m - sa.m_lo;
-              rc[u] = {c3[k], c4[k], c5[k], 0.0f};
-            } else {
-              rc[u] = pm::load4(pm::pair_rho_row(pair, m));
-            }
-          }
-          // the next batch's list loads go out after this batch's row loads, so that
-          // waiting for the rows (in issue order) does not wait for them
-          __builtin_amdgcn_sched_barrier(0);
-          plist<kBBatch>(a, ca.global, k0 + 2 * kBBatch, cnt, nxt);
-          __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-          for (int u = 0; u < kBBatch; u++) {
-            float fp_r, phi;
-            pm::eam_pair_rows(t, rc[u], zc[u], pp[u], rinv[u], fpi, (float)nb[u].w, fp_r, phi);
-            const double fpair = nb[u].in ? (double)fp_r : 0.0;
-            const double fe0 = nb[u].x * fpair, fe1 = nb[u].y * fpair, fe2 = nb[u].z * fpair;
-            fx += fe0;
-            fy += fe1;
-            fz += fe2;
-            if constexpr (ENERGY) e += nb[u].in ? 0.5 * (double)phi : 0.0;
-            if constexpr (VIRIAL) pm::add_virial(v, fe0, fe1, fe2, nb[u].x, nb[u].y, nb[u].z);
-          }
-        }
-        fx = half_sum(fx);
-        fy = half_sum(fy);
-        fz = half_sum(fz);
-        if constexpr (ENERGY) e = half_sum(e);
-        if constexpr (VIRIAL) {
-#pragma unroll
-          for (int k = 0; k < 9; k++) v[k] = half_sum(v[k]);
-        }
-        if (half == 0) {
-          if constexpr (DRIFT) pm::write_outputs_drift(o, dr, ca.global, pi, fx, fy, fz);
-          else pm::write_outputs<ENERGY, VIRIAL, FUSED>(o, ca.global, fx, fy, fz, e, v);
-        }
-      }
-    }
-    __syncthreads();  // this brick's LDS is read by every lane before the next is staged
-  }
-}
-
-// Density pass with the rho value rows of the knots nearest rc in LDS (16-byte
-// rows), persistent like the split force pass: the records (three 8-byte reads)
-// and the table row (one 16-byte read) of a pair all come from LDS.
-template <bool EMB, bool ENERGY, bool SET_EP>
-__global__ void __launch_bounds__(kSplitThreads, 4)
-    k_split_eam_rho(SplitArgs sa, EamTabDev<float> t, const float* __restrict__ pair, double* __restrict__ rho_io,
-                    double* __restrict__ pos_w, double* __restrict__ ep)
-{
-  const BlockArgs& a = sa.a;
-  extern __shared__ __align__(32) unsigned char smem[];
-  float4* rows = reinterpret_cast<float4*>(smem);
-  unsigned char* base = smem + (size_t)sa.rows * sizeof(float4);
-  LRec rec;
-  HaloCell* hc = reinterpret_cast<HaloCell*>(base + lrec_at(base, a.cap, false, rec));
-  int* total_p = reinterpret_cast<int*>(hc + a.g.nh_max);
-  for (int i = threadIdx.x; i < sa.rows; i += blockDim.x)
-    rows[i] = *reinterpret_cast<const float4*>(pm::pair_rho_row(pair, sa.m_lo + i));
-  const int64_t b0 = lblock() * sa.per_wg;
-  const int64_t b1 = min(b0 + sa.per_wg, a.g.nblocks);
-  const int half = pass_half();
-  for (int64_t b = b0; b < b1; b++) {
-    const BlockShape s = block_shape(a.g, b);
-    const int total = build_halo(a.g, s, a.cell_off, hc, total_p);
-    if (total <= a.cap) {  // always (cap = measured maximum)
-      stage(s, hc, total, a.pos, a.box[s.r].L, nullptr, rec, nullptr);
-      for (int q = pass_slot();; q += kPassAtoms) {
-        Central ca;
-        if (!central_atom(s, hc, q, ca)) break;
-        const Pos4 pi = rec.get(ca.local);
-        const int cnt = a.nnb[ca.global];
-        double rho = 0.0;
-        int nxt[kBBatch];
-        plist<kBBatch>(a, ca.global, half, cnt, nxt);
-        for (int k0 = half; k0 < cnt; k0 += 2 * kBBatch) {
-          int cur[kBBatch];
-          plist_use<kBBatch>(nxt, ca.local, k0, cnt, cur);
-          plist<kBBatch>(a, ca.global, k0 + 2 * kBBatch, cnt, nxt);
-          LNb nb[kBBatch];
-          pgather<kBBatch>(a, rec, cur, k0, cnt, pi, nb);
-#pragma unroll
-          for (int u = 0; u < kBBatch; u++) {
-            float r, rinv, p;
-            int m;
-            pm::dist((float)(nb[u].in ? nb[u].d2 : a.rc2), r, rinv);
-            pm::knot_r(t, r, m, p);
-            pm::Q4<float> rc;
-            if (m >= sa.m_lo) {
-              const float4 q4 = rows[m - sa.m_lo];
-              rc = {q4.x, q4.y, q4.z, q4.w};
-            } else {
-              rc = pm::load4(pm::pair_rho_row(pair, m));
-            }
-            const float val = pm::cubic(rc, p);
-            rho += nb[u].in ? (double)val : 0.0;
-          }
-        }
-        rho = half_sum(rho);
-        if (half == 0) {
-          rho_io[ca.global] = rho;
-          if constexpr (EMB) {
-            float F, Fp;
-            pm::embed(t, 0, rho, F, Fp);
-            pos_w[4 * ca.global + 3] = (double)Fp;
-            if constexpr (ENERGY) {
-              if constexpr (SET_EP) ep[ca.global] = (double)F;
-              else ep[ca.global] += (double)F;
-            }
-          }
-        }
-      }
-    }
-    __syncthreads();
-  }
-}
-
 // Fused next-step drift of the MD loop (ForceEpilogue::drift) for a force launch
 // with the plain MD-loop epilogue; false when it does not apply.
 bool drift_setup(Ctx& c, const ForceEpilogue& epi, unsigned flags, pm::Drift& dr)
@@ -1073,122 +837,6 @@ void drift_commit(Ctx& c)
   c.drift_done = true;
 }
 
-// LDS plan of a split pass: table rows (row_bytes each) that fit beside the halo
-// records when two workgroups share a CU.
-struct SplitPlan {
-  bool ok;
-  int m_lo, rows;
-  size_t bytes;
-};
-
-SplitPlan split_plan(const Ctx& c, size_t row_bytes, bool with_w)
-{
-  SplitPlan sp{false, 0, 0, 0};
-  if (!c.split_enabled || c.eam.nel != 1 || c.num_cu <= 0) return sp;
-  const size_t budget = c.lds_max / 2;
-  const size_t halo = block_lds_bytes(c, false, with_w);
-  if (halo + 16 >= budget) return sp;
-  const int nr = c.eam.nr;
-  int rows = (int)std::min<size_t>((size_t)nr, (budget - halo - 16) / row_bytes);
-  if (c.split_rows > 0) rows = std::min(rows, c.split_rows);
-  // short of a quarter of the table, the HBM fallback would carry too many pairs
-  if (rows < std::max(1, nr / 4)) return sp;
-  sp.ok = true;
-  sp.rows = rows;
-  sp.m_lo = nr - rows;
-  sp.bytes = ((size_t)rows * row_bytes + 15) / 16 * 16 + halo;
-  return sp;
-}
-
-unsigned split_grid(const Ctx& c, int64_t nblocks)
-{
-  const int64_t want = 2 * (int64_t)c.num_cu;
-  return (unsigned)std::max<int64_t>(8, (std::min<int64_t>(want, nblocks) + 7) / 8 * 8);
-}
-
-// Split-table density pass; false when it does not apply (the caller falls back).
-bool split_rho_dispatch(Ctx& c, const BlockArgs& a, bool do_emb, unsigned flags, const ForceEpilogue& epi);
-
-template <class K>
-void set_lds_limit(K kernel, size_t bytes)
-{
-  XS_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
-                             (int)bytes));
-}
-
-// Split-table force pass; false when it does not apply (the caller falls back).
-bool split_force_dispatch(Ctx& c, const BlockArgs& a, unsigned flags, const ForceEpilogue& epi)
-{
-  const SplitPlan sp = split_plan(c, 3 * sizeof(float), true);
-  if (!sp.ok) return false;
-  c.last_force_rows = sp.rows;
-  const EamTabDev<float> t = make_eam_tab<float>(c);
-  const float* pair = eam_pair_table<float>(c);
-  const unsigned g = split_grid(c, a.g.nblocks);
-  SplitArgs sa;
-  sa.a = a;
-  sa.m_lo = sp.m_lo;
-  sa.rows = sp.rows;
-  sa.per_wg = (a.g.nblocks + g - 1) / g;
-  const bool energy = flags & XS_ENERGY, vir = flags & XS_VIRIAL;
-  const pm::Outputs o = make_outputs(c, epi);
-  pm::Drift dr{};
-  const bool drift = drift_setup(c, epi, flags, dr);
-  ProfScope prof(c.prof, c.stream, XS_PROF_EAM_FORCE);
-#define XS_SF(EN, VI, FU, DR)                                                                        \
-  do {                                                                                               \
-    set_lds_limit(k_split_eam_force<EN, VI, FU, DR>, sp.bytes);                                      \
-    k_split_eam_force<EN, VI, FU, DR><<<g, kSplitThreads, sp.bytes, c.stream>>>(sa, t, pair, o, dr); \
-  } while (0)
-  if (drift) {
-    XS_SF(false, false, true, true);
-  } else if (epi.fused) {
-    if (energy && vir) XS_SF(true, true, true, false);
-    else if (energy) XS_SF(true, false, true, false);
-    else if (vir) XS_SF(false, true, true, false);
-    else XS_SF(false, false, true, false);
-  } else {
-    if (energy && vir) XS_SF(true, true, false, false);
-    else if (energy) XS_SF(true, false, false, false);
-    else if (vir) XS_SF(false, true, false, false);
-    else XS_SF(false, false, false, false);
-  }
-#undef XS_SF
-  XS_HIP(hipGetLastError());
-  if (drift) drift_commit(c);
-  return true;
-}
-
-bool split_rho_dispatch(Ctx& c, const BlockArgs& a, bool do_emb, unsigned flags, const ForceEpilogue& epi)
-{
-  const SplitPlan sp = split_plan(c, sizeof(float4), false);
-  if (!sp.ok) return false;
-  c.last_rho_rows = sp.rows;
-  const EamTabDev<float> t = make_eam_tab<float>(c);
-  const float* pair = eam_pair_table<float>(c);
-  const unsigned g = split_grid(c, a.g.nblocks);
-  SplitArgs sa;
-  sa.a = a;
-  sa.m_lo = sp.m_lo;
-  sa.rows = sp.rows;
-  sa.per_wg = (a.g.nblocks + g - 1) / g;
-  const bool energy = flags & XS_ENERGY;
-  double* pos_w = reinterpret_cast<double*>(c.pos.p);
-  ProfScope prof(c.prof, c.stream, XS_PROF_EAM_RHO);
-#define XS_SR(EMB, EN, SET)                                                                                  \
-  do {                                                                                                       \
-    set_lds_limit(k_split_eam_rho<EMB, EN, SET>, sp.bytes);                                                  \
-    k_split_eam_rho<EMB, EN, SET><<<g, kSplitThreads, sp.bytes, c.stream>>>(sa, t, pair, c.rho.p, pos_w, c.ep.p); \
-  } while (0)
-  if (!do_emb) XS_SR(false, false, false);
-  else if (energy && epi.fused) XS_SR(true, true, true);
-  else if (energy) XS_SR(true, true, false);
-  else XS_SR(true, false, false);
-#undef XS_SR
-  XS_HIP(hipGetLastError());
-  return true;
-}
-
 template <class Real, bool MULTI>
 void block_eam_dispatch(Ctx& c, double rcut, bool do_rho, bool do_emb, bool do_force, unsigned flags,
                         const ForceEpilogue& epi)
@@ -1211,12 +859,7 @@ void block_eam_dispatch(Ctx& c, double rcut, bool do_rho, bool do_emb, bool do_f
   const unsigned g = block_grid(c);
   const bool energy = flags & XS_ENERGY;
   double* pos_w = reinterpret_cast<double*>(c.pos.p);
-  bool rho_done = false;
-  if constexpr (std::is_same_v<Real, float> && !MULTI) {
-    if (do_rho) rho_done = split_rho_dispatch(c, a, do_emb, flags, epi);
-  }
-  if (do_rho && !rho_done) {
-    c.last_rho_rows = 0;
+  if (do_rho) {
     ProfScope prof(c.prof, c.stream, XS_PROF_EAM_RHO);
 #define XS_BRHO(EMB, EN, SET) \
   k_block_eam_rho<Real, MULTI, EMB, EN, SET><<<g, kPassThreads, lds_rho, c.stream>>>(a, t, pair, c.rho.p, pos_w, c.ep.p)
@@ -1228,10 +871,6 @@ void block_eam_dispatch(Ctx& c, double rcut, bool do_rho, bool do_emb, bool do_f
     XS_HIP(hipGetLastError());
   }
   if (do_force) {
-    if constexpr (std::is_same_v<Real, float> && !MULTI) {
-      if (split_force_dispatch(c, a, flags, epi)) return;
-    }
-    c.last_force_rows = 0;
     const pm::Outputs o = make_outputs(c, epi);
     const bool vir = flags & XS_VIRIAL;
     pm::Drift dr{};

@@ -387,10 +387,6 @@ void launch_eam(Ctx& c, double rcut, bool do_rho, bool do_emb, bool do_force, un
       throw Error(XS_E_STATE, "eam_force needs F'(rho) from eam_rho2emb in the same precision");
   }
   const bool multi = c.eam.nel > 1;
-  if (c.list_kind != ListKind::Block) {
-    if (do_rho) c.last_rho_rows = 0;
-    if (do_force) c.last_force_rows = 0;
-  }
   if (c.list_kind == ListKind::Block) {
     launch_eam_block(c, rcut, do_rho, do_emb, do_force, flags, epi);
   } else if (mixed) {

@@ -279,16 +279,10 @@ struct Ctx {
   int bcap = 0;                    // LDS slots per block (max halo atoms, rounded)
   int bcentral = 0;                // max central atoms per block
   DBuf<int> brick_rows;            // longest list of each block
-  // split-table passes (rho rows in LDS, persistent workgroups): CUs and LDS per
-  // workgroup of the device; XSMD_SPLIT_TABLE=1 enables them, XSMD_SPLIT_ROWS caps
-  // the rows in LDS
-  int num_cu = 0;
-  size_t lds_max = 65536;
-  bool split_enabled = false;
-  int split_rows = 0;
+  int num_cu = 0;           // CUs of the device (persistent grids)
+  size_t lds_max = 65536;   // dynamic LDS one workgroup may use
   bool list_sort_disabled = false;  // XSMD_LIST_SORT=0: block lists stay in stencil order
   int sort_bits = 7;                // distance keys of the list sort: 2^sort_bits bins over [0, rl^2), <= 8
-  int last_rho_rows = 0, last_force_rows = 0;  // table rows in LDS of the last EAM passes (0: none)
   bool drift_done = false;  // the last force launch applied ForceEpilogue::drift
 
   // small device scalars + pinned host mirror

@@ -311,12 +311,6 @@ class Simulation:
         return {"kind": ("global", "block")[kind.value], "blocks": blocks.value, "halo_capacity": cap.value,
                 "central_max": cen.value, "brick": tuple(brick)}
 
-    def eam_pass_info(self) -> dict:
-        """Table rows the last EAM density / force passes kept in LDS (0: all from HBM)."""
-        r, f = C.c_int(0), C.c_int(0)
-        self.ctx.xs_eam_pass_info(C.byref(r), C.byref(f))
-        return {"rho_table_rows": r.value, "force_table_rows": f.value}
-
     def neighbors(self):
         """(counts[n], ids[total], imgs[total,3]) in id space, entries in list order."""
         counts = np.zeros(self.n, dtype=np.int64)

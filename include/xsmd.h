@@ -127,13 +127,6 @@ xs_status xs_pair_count(xs_ctx* ctx, double rcut, int64_t* entries, int64_t* wit
  * build on the global list; XSMD_BLOCK_ATOMS=n bounds the expected central atoms
  * per brick (default 230). */
 xs_status xs_list_info(xs_ctx* ctx, int* kind, int64_t* blocks, int* halo_capacity, int* central_max, int* brick);
-/* Table rows the last EAM density and force passes kept in LDS (diagnostic; no
- * counterpart in the reference): 0 when a pass read all its rows from HBM.  With
- * XSMD_SPLIT_TABLE=1 the single-element FP32 passes on the cell-block list run as
- * persistent workgroups that keep the rho rows of the knots nearest rc in LDS (the
- * density pass its value rows, the force pass the slope coefficients);
- * XSMD_SPLIT_ROWS=n caps those rows (both read when a context is created). */
-xs_status xs_eam_pass_info(xs_ctx* ctx, int* rho_lds_rows, int* force_lds_rows);
 
 /* --- operators -------------------------------------------------------------------- */
 /* zero_force_energy (src/compute/zero_force_energy.cu:38-147): a = ep = virial = 0. */

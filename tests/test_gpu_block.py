@@ -151,54 +151,23 @@ def test_block_adjoint_and_pair_count_use_converted_list(monkeypatch):
 
 
 def _env(monkeypatch, **kv):
-    for k in ("XSMD_LIST", "XSMD_SPLIT_TABLE", "XSMD_SPLIT_ROWS", "XSMD_LIST_SORT", "XSMD_BLOCK_ATOMS"):
+    for k in ("XSMD_LIST", "XSMD_LIST_SORT", "XSMD_BLOCK_ATOMS", "XSMD_SORT_BITS"):
         monkeypatch.delenv(k, raising=False)
     for k, v in kv.items():
         monkeypatch.setenv(k, v)
 
 
-@pytest.mark.parametrize("name", ["cu_jitter", "cu_one_cell", "replicas"])
-def test_split_table_passes_match_hbm_table(monkeypatch, name):
-    """Single-element FP32 density and force passes with rho rows in LDS (persistent
-    workgroups, XSMD_SPLIT_TABLE=1) against the default block passes reading every row
-    from HBM: the same per-pair arithmetic and summation order in other
-    kernels, whose FP32 contractions the compiler may choose differently (mixed
-    tolerance, 1e-6 of the rms).  Capping the LDS rows to a third of the table sends
-    the short pairs to the HBM fallback of the same kernels: bit-identical."""
-    cfg, masses, names, tab = _case(name)
-    out = {}
-    for label, env in (("split", {"XSMD_SPLIT_TABLE": "1"}), ("hbm", {}),
-                       ("capped", {"XSMD_SPLIT_TABLE": "1", "XSMD_SPLIT_ROWS": str(tab.nr // 3)})):
-        _env(monkeypatch, **env)
-        sim = make_sim(cfg, masses, names)
-        sim.eam_alloy_init(tab)
-        sim.chunk_neighbors(rcut_max=tab.rc, rcut_inc=0.3)
-        assert sim.list_info()["kind"] == "block"
-        out[label] = _outputs(sim, tab.rc, True)
-        info = sim.eam_pass_info()
-        rows = (info["rho_table_rows"], info["force_table_rows"])
-        if label == "hbm":
-            assert rows == (0, 0)
-        elif label == "split":
-            assert min(rows) > 0
-        else:
-            assert rows == (tab.nr // 3, tab.nr // 3)
-    for x, y in zip(out["capped"], out["split"]):
-        assert np.array_equal(x, y)
-    for x, y in zip(out["split"], out["hbm"]):
-        assert _close(x, y, 1e-6)
-
-
-def test_split_table_md_run_vs_oracle_energy(monkeypatch, oracle_mod):
-    """100 mixed NVE steps on the split-table passes (fused run, Cu 600 K): the
-    potential energy of every step within 1e-6 of the FP64 oracle trajectory's."""
-    _env(monkeypatch, XSMD_SPLIT_TABLE="1")
+def test_block_md_run_vs_oracle_energy(monkeypatch, oracle_mod):
+    """100 mixed NVE steps on the default path (block passes, sorted lists, fused
+    drift; Cu 600 K, the bench's density): the potential energy of every step within
+    1e-6 of the FP64 oracle trajectory's."""
+    _env(monkeypatch)
     tab = cu_tables()
     cfg = cu_config((8, 8, 8), T=600.0, jitter=0.0, seed=31)
     sim = make_sim(cfg, [CU_MASS])
     sim.eam_alloy_init(tab)
     _, th = sim.md_run("eam", 100, 0.001, tab.rc, 0.3, mixed=True, thermo_every=1)
-    assert min(sim.eam_pass_info().values()) > 0
+    assert sim.list_info()["kind"] == "block"
     o = make_oracle(oracle_mod, cfg, [CU_MASS])
     o.set_eam(tab)
     _, th_ref = o.md_run("eam", 100, 0.001, tab.rc, 0.3)
