@@ -302,7 +302,7 @@ AdjTab<double> adj_tab<double>(Ctx& c)
   t.rdr = c.eam.rdr;
   t.rdrho = c.eam.rdrho;
   t.rhomax = c.eam.rhomax;
-  t.conv = 1.602176634e-19 / (1.66053906660e-27 * 1.0e4);
+  t.conv = kEV;
   t.nr = c.eam.nr;
   t.nrho = c.eam.nrho;
   std::memcpy(t.t2rhor, c.eam.t2rhor, sizeof(t.t2rhor));
@@ -321,7 +321,7 @@ AdjTab<float> adj_tab<float>(Ctx& c)
   t.rdr = c.eam.rdr;
   t.rdrho = c.eam.rdrho;
   t.rhomax = c.eam.rhomax;
-  t.conv = 1.602176634e-19 / (1.66053906660e-27 * 1.0e4);
+  t.conv = kEV;
   t.nr = c.eam.nr;
   t.nrho = c.eam.nrho;
   std::memcpy(t.t2rhor, c.eam.t2rhor, sizeof(t.t2rhor));

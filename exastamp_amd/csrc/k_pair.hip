@@ -294,9 +294,8 @@ void lj_dispatch(Ctx& c, double eps, double sigma, double rcut, unsigned flags, 
 
 }  // namespace
 
-// 1 eV in Da.A^2/ps^2 (CODATA-2018: e / (amu * 1e4)); conversion_z2r = 1 eV.A and
-// conversion_frho = 1 eV of eam_alloy.h:156-157 in exaStamp's internal units.
-constexpr double kEV = 1.602176634e-19 / (1.66053906660e-27 * 1.0e4);
+// conversion_z2r = 1 eV.A and conversion_frho = 1 eV of eam_alloy.h:156-157 in
+// exaStamp's internal units (kEV, xsmd_internal.hpp).
 
 template <>
 EamTabDev<double> make_eam_tab<double>(Ctx& c)

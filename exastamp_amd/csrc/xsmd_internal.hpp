@@ -142,6 +142,9 @@ struct BlockGeom {
   int64_t nblocks;   // over all replicas
 };
 
+// 1 eV in exaStamp's internal energy unit Da.A^2/ps^2: e / (amu * 1e4 J), exact SI
+// charge, CODATA-2022 atomic mass constant (onika's).
+constexpr double kEV = 1.602176634e-19 / (1.66053906892e-27 * 1.0e4);
 constexpr double kBlockAtoms = 230.0;      // largest expected central atoms per brick
 constexpr size_t kLdsPerCu = 160 * 1024;   // LDS of one CU
 constexpr int kMaxHaloCells = 216;

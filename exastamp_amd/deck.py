@@ -43,7 +43,9 @@ import numpy as np
 import yaml
 
 from .inputs import Config
-from .units import EV, parse_quantity
+from .units import AMU_KG, EV, PRESSURE_UNIT_PA, parse_quantity
+
+AMU_G = AMU_KG * 1e3
 
 
 class DeckError(RuntimeError):
@@ -227,6 +229,11 @@ def _quantity_list(v):
 
 @dataclasses.dataclass
 class ThermoRecord:
+    """One line of the thermodynamic-state screen/CSV output, with the columns and
+    conversions of the reference's print (print_thermodynamic_state.cpp:223-282):
+    energies per particle in eV, temperature in K, Pxx Pyy Pzz = virial diagonal / V
+    (stress_tensor), Pxy Pxz Pyz = (virial + kinetic tensor) / V (full_stress_tensor)
+    in Pa, box lengths, angles, volume in A^3 and density in g/cm^3."""
     step: int
     time_ps: float
     particles: int
@@ -234,17 +241,25 @@ class ThermoRecord:
     kinetic_ev: float
     potential_ev: float
     temperature_k: float
-    pressure_pa: float
+    pressure_pa: float   # hydrostatic, (kinetic + virial) trace / 3V
+    p_diag: tuple        # Pxx Pyy Pzz
+    p_off: tuple         # Pxy Pxz Pyz
+    lengths: tuple       # A B C
     volume: float
+    density: float       # g/cm^3
 
     def line(self) -> str:
-        return (f"{self.step:9d}  {self.time_ps:.6e}  {self.particles:12d}  {self.total_ev: .10e}  "
-                f"{self.kinetic_ev: .10e}  {self.potential_ev: .10e}  {self.temperature_k:.6f}  "
-                f"{self.pressure_pa:.6e}  {self.volume:.6f}")
+        cols = [f"{self.step:9d}", f"{self.time_ps:.6e}", f"{self.particles:13d}", f"{self.total_ev: .10e}",
+                f"{self.kinetic_ev: .10e}", f"{self.potential_ev: .10e}", f"{self.temperature_k:.12f}"]
+        cols += [f"{x:.12f}" for x in (*self.p_diag, *self.p_off, *self.lengths, 90.0, 90.0, 90.0)]
+        cols += [f"{self.volume:.12f}", f"{self.density:.12f}"]
+        return "  ".join(cols)
 
 
 THERMO_HEADER = ("# Step     Time (ps)     Particles  Tot. E. (eV/part)  Kin. E. (eV/part)  Pot. E. (eV/part)  "
-                 "Temp. (K)  Pressure (Pa)  Vol. (ang^3)")
+                 "Temp. (K) Pxx Pyy Pzz Pxy Pxz Pyz (Pa) A/B/C (ang)    alpha/beta/gamma (deg)     Vol. (ang^3)  "
+                 "Rho (g/cm^3)")
+G_PER_CM3_PER_DA_PER_A3 = AMU_G * 1e24
 
 
 class Runner:
@@ -355,10 +370,19 @@ class Runner:
     def thermo(self) -> ThermoRecord:
         st = self.sim.simulation_thermodynamic_state()
         n = st.particle_count
+        v = st.volume
+        kt = 2.0 * (st.kinetic_tensor - 0.5 * np.outer(st.momentum, st.momentum) / st.mass)
+        vir = st.virial
+        a = self.atoms
+        lengths = tuple(float(x) for x in (a.hi[0] - a.lo[0])) if a is not None else (0.0, 0.0, 0.0)
         rec = ThermoRecord(step=int(self.get("timestep", 0)), time_ps=float(self.get("physical_time", 0.0)),
                            particles=n, total_ev=st.total_energy / EV / n, kinetic_ev=st.kinetic_energy / EV / n,
                            potential_ev=st.potential_energy / EV / n, temperature_k=st.temperature,
-                           pressure_pa=st.pressure_pa, volume=st.volume)
+                           pressure_pa=st.pressure_pa,
+                           p_diag=tuple(float(vir[k, k] / v * PRESSURE_UNIT_PA) for k in range(3)),
+                           p_off=tuple(float((vir[i, j] + kt[i, j]) / v * PRESSURE_UNIT_PA)
+                                       for i, j in ((0, 1), (0, 2), (1, 2))),
+                           lengths=lengths, volume=v, density=st.mass / v * G_PER_CM3_PER_DA_PER_A3)
         self.records.append(rec)
         return rec
 

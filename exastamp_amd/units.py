@@ -1,8 +1,12 @@
 """exaStamp's internal unit system and a small quantity parser.
 
 Internal units are Angstrom, Dalton, picosecond, elementary charge, kelvin
-(include/exaStamp/unit_system.h:26-36), so energy is Da.A^2/ps^2.  Constants are
-CODATA-2018 (the values onika uses cannot be checked offline).
+(include/exaStamp/unit_system.h:26-36), so energy is Da.A^2/ps^2.  The elementary
+charge and the Boltzmann constant are exact SI values; the atomic mass constant is
+the CODATA-2022 value, the one onika uses: the density column of the run the
+reference recorded for its AlCu B2 deck
+(data/regression_new/potentials/eam/eam_alloy/thermodynamic_state.csv, 724 224 Da in
+72^3 A^3 at 3.221992473234 g/cm^3) gives 1.6605390689e-27 kg to its 12 digits.
 
 Quantities are written in decks as "<value> <unit>" strings, e.g. "0.0104 eV",
 "3.4 ang", "39.948 Da", "1.0e-3 ps" (lennard_jones.h:66-67 reads them through
@@ -12,7 +16,7 @@ from __future__ import annotations
 
 import re
 
-AMU_KG = 1.66053906660e-27
+AMU_KG = 1.66053906892e-27
 E_CHARGE_C = 1.602176634e-19
 KB_J_PER_K = 1.380649e-23
 

@@ -24,10 +24,13 @@
 
 namespace {
 
-// CODATA-2018 constants.  exaStamp's internal units are Angstrom, Dalton,
+// Exact SI charge and Boltzmann constants; CODATA-2022 atomic mass constant (the
+// value the reference's recorded AlCu run implies through its density column,
+// data/regression_new/potentials/eam/eam_alloy/thermodynamic_state.csv).
+// exaStamp's internal units are Angstrom, Dalton,
 // picosecond (include/exaStamp/unit_system.h:26-36), so the internal energy
 // unit is 1 Da.A^2/ps^2 = amu * 1e4 J (print_thermodynamic_state.cpp:222-224).
-constexpr double kAmu = 1.66053906660e-27;
+constexpr double kAmu = 1.66053906892e-27;
 constexpr double kElementaryCharge = 1.602176634e-19;
 constexpr double kBoltzmann = 1.380649e-23;
 constexpr double kEnergyUnitJ = kAmu * 1.0e4;

@@ -92,6 +92,10 @@ def test_eam_alcu_deck_matches_recorded_run():
     assert first.step == 0 and first.particles == 16000
     assert abs(first.potential_ev - (-2.9354759355)) <= 5e-4
     assert abs(first.kinetic_ev - 1.7674741423e-3) <= 0.03 * 1.7674741423e-3
+    # columns that do not depend on the noise streams equal the recorded ones
+    assert first.lengths == (72.0, 72.0, 72.0) and first.volume == 373248.0
+    assert abs(first.density - 3.221992473234) <= 1e-11
+    assert len(first.line().split()) == 21   # the recorded file's column count
     # one NVE step conserves the total energy per atom to well below the recorded
     # run's own change (1.4e-8 eV)
     assert abs(recs[1].total_ev - first.total_ev) <= 1e-7

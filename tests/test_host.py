@@ -5,8 +5,8 @@ import pytest
 
 def test_units_parse_deck_quantities():
     from exastamp_amd import units
-    # 1 eV = e / (amu * 1e4) Da.A^2/ps^2 (CODATA-2018) ~ 9648.53
-    assert units.parse_quantity("0.0104 eV") == pytest.approx(0.0104 * 1.602176634e-19 / 1.66053906660e-23,
+    # 1 eV = e / (amu * 1e4) Da.A^2/ps^2 (CODATA-2022 amu) ~ 9648.53
+    assert units.parse_quantity("0.0104 eV") == pytest.approx(0.0104 * 1.602176634e-19 / 1.66053906892e-23,
                                                              rel=1e-14)
     assert 9648.5 < units.EV < 9648.6
     assert units.parse_quantity("3.4 ang") == 3.4
@@ -75,3 +75,16 @@ def test_synthetic_tables_zero_pressure_lattice():
     assert theta.size == 6000
     t2 = t.with_theta(theta * 1.01)
     assert np.allclose(t2.frho, 1.01 * t.frho)
+
+
+def test_atomic_mass_constant_from_recorded_density():
+    """The reference's recorded AlCu B2 run (fixture) prints the density of 8000 Al +
+    8000 Cu atoms in a 72 A cube as 3.221992473234 g/cm^3; that pins the atomic mass
+    constant its unit conversions use (CODATA-2022) to the 12 digits printed."""
+    from conftest import GOLDEN
+    from exastamp_amd import units
+    row = (GOLDEN / "alcu_b2_thermodynamic_state.csv").read_text().splitlines()[1].split()
+    volume, rho = float(row[19]), float(row[20])
+    assert volume == 72.0 ** 3
+    mass_da = 8000 * 26.982 + 8000 * 63.546
+    assert abs(mass_da * units.AMU_KG * 1e3 / (volume * 1e-24) / rho - 1.0) < 2e-12
