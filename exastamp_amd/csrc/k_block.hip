@@ -163,9 +163,11 @@ __device__ __forceinline__ int find_cell(const HaloCell* hc, int nh, int c)
   return lo;
 }
 
-// Stage halo atoms: record c = position (+ image shift) and w; type when requested.
-__device__ void stage(const BlockShape& s, const HaloCell* hc, int total, const Pos4* __restrict__ pos,
-                      const double* L, const uint8_t* __restrict__ type, const LRec& rec, uint8_t* ltype)
+// Stage halo atoms: f(c, p, j) for slot c, position p (+ image shift) and w, global
+// index j; ends with __syncthreads().
+template <class F>
+__device__ void stage_each(const BlockShape& s, const HaloCell* hc, int total, const Pos4* __restrict__ pos,
+                           const double* L, F f)
 {
   const int nh = s.hx * s.hy * s.hz;
   for (int c = threadIdx.x; c < total; c += blockDim.x) {
@@ -179,10 +181,19 @@ __device__ void stage(const BlockShape& s, const HaloCell* hc, int total, const 
       p.y = xsmd_image(p.y, xsmd_nbr_img_y(code), L[1]);
       p.z = xsmd_image(p.z, xsmd_nbr_img_z(code), L[2]);
     }
-    rec.put(c, p);
-    if (ltype) ltype[c] = type[j];
+    f(c, p, j);
   }
   __syncthreads();
+}
+
+// Stage halo atoms into the records (and types when requested).
+__device__ void stage(const BlockShape& s, const HaloCell* hc, int total, const Pos4* __restrict__ pos,
+                      const double* L, const uint8_t* __restrict__ type, const LRec& rec, uint8_t* ltype)
+{
+  stage_each(s, hc, total, pos, L, [&](int c, const Pos4& p, int j) {
+    rec.put(c, p);
+    if (ltype) ltype[c] = type[j];
+  });
 }
 
 // Central atom q of the block: its halo cell h, local slot and global index.
@@ -359,6 +370,9 @@ __device__ __forceinline__ uint4 group_interleave(const uint32_t (&o)[4])
                     (o[0] >> 16) | (o[1] & 0xffff0000u), (o[2] >> 16) | (o[3] & 0xffff0000u));
 }
 
+// LDS of the list build: cap + 1 position records of 24 bytes, 16-byte aligned
+__host__ __device__ inline size_t build_rec_bytes(int cap) { return ((size_t)(cap + 1) * 24 + 15) / 16 * 16; }
+
 // 8-bit distance keys of a group, list order (byte j of b[0..1] = entry j) -> the
 // bl_index order (0, 2, 4, 6 | 1, 3, 5, 7)
 __device__ __forceinline__ uint2 bins_interleave(const uint32_t (&b)[2])
@@ -385,7 +399,16 @@ __global__ void __launch_bounds__(kBuildThreads) k_block_build(BlockGeom g, cons
 {
   const int64_t b = lblock();
   if (b >= g.nblocks) return;
-  LdsLayout l = lds_layout(cap, nh_max, false, false);
+  // LDS: cap + 1 position records {x, y, z} (24 bytes; the extra one pads the
+  // second read of a candidate pair), then the halo table
+  extern __shared__ __align__(32) unsigned char smem[];
+  double3* P = reinterpret_cast<double3*>(smem);
+  struct {
+    HaloCell* hc;
+    int* total;
+  } l;
+  l.hc = reinterpret_cast<HaloCell*>(smem + build_rec_bytes(cap));
+  l.total = reinterpret_cast<int*>(l.hc + nh_max);
   const BlockShape s = block_shape(g, b);
   const int total = build_halo(g, s, cell_off, l.hc, l.total);
   if (total > cap) {  // cannot happen when cap = measured maximum; never write past LDS
@@ -395,11 +418,11 @@ __global__ void __launch_bounds__(kBuildThreads) k_block_build(BlockGeom g, cons
   const BoxDev bx = box[s.r];
   const bool cull = *nocull == 0ull;
   if (threadIdx.x == 0) l.total[1] = 0;  // longest list of the brick (made visible by stage's barrier)
-  stage(s, l.hc, total, pos, bx.L, nullptr, l.rec, nullptr);
+  stage_each(s, l.hc, total, pos, bx.L, [&](int c, const Pos4& p, int) { P[c] = double3{p.x, p.y, p.z}; });
   for (int q = threadIdx.x;; q += blockDim.x) {
     Central a;
     if (!central_atom(s, l.hc, q, a)) break;
-    const double xi = l.rec.x[a.local], yi = l.rec.y[a.local], zi = l.rec.z[a.local];
+    const double xi = P[a.local].x, yi = P[a.local].y, zi = P[a.local].z;
     const Pos4 pi = {xi, yi, zi, 0.0};
     const int ix = a.h % s.hx, iy = (a.h / s.hx) % s.hy, iz = a.h / (s.hx * s.hy);
     int cnt = 0;
@@ -454,11 +477,12 @@ __global__ void __launch_bounds__(kBuildThreads) k_block_build(BlockGeom g, cons
             continue;
           const int end = e.lstart + e.count;
           for (int c = e.lstart; c < end; c += 2) {
-            const int c1 = min(c + 1, end - 1);  // the pair's second candidate (or c again)
-            const double d0 = xsmd_d2(l.rec.x[c] - xi, l.rec.y[c] - yi, l.rec.z[c] - zi);
-            const double d1 = xsmd_d2(l.rec.x[c1] - xi, l.rec.y[c1] - yi, l.rec.z[c1] - zi);
+            // record c + 1 <= cap exists (the next cell's first atom, or padding)
+            const double3 p0 = P[c], p1 = P[c + 1];
+            const double d0 = xsmd_d2(p0.x - xi, p0.y - yi, p0.z - zi);
+            const double d1 = xsmd_d2(p1.x - xi, p1.y - yi, p1.z - zi);
             if (d0 < rl2 && c != a.local) hit(c, d0);
-            if (c + 1 < end && d1 < rl2 && c1 != a.local) hit(c1, d1);
+            if (c + 1 < end && d1 < rl2 && c + 1 != a.local) hit(c + 1, d1);
           }
           if (ready) flush();
         }
@@ -1001,7 +1025,7 @@ unsigned block_grid(const Ctx& c)
 // (2^bits bins uniform in d^2 over [0, rl^2)), k_list_sort orders by it.
 void block_build(Ctx& c, double rl2)
 {
-  const size_t lds = block_lds_bytes(c, false, false);
+  const size_t lds = build_rec_bytes(c.bcap) + (size_t)c.bgeom.nh_max * sizeof(HaloCell) + 4 * sizeof(int);
   const bool sorted = !c.list_sort_disabled;
   const int bits = std::max(1, std::min(c.sort_bits, 8));
   const float bin_scale = (float)((double)(1 << bits) / rl2);
