@@ -163,9 +163,10 @@ __device__ __forceinline__ int find_cell(const HaloCell* hc, int nh, int c)
   return lo;
 }
 
-// Stage halo atoms: f(c, p, j) for slot c, position p (+ image shift) and w, global
-// index j; ends with __syncthreads().
-template <class F>
+// Stage halo atoms: f(c, p, j) for slot c, position p (+ image shift) and, with
+// WITH_W, w (else 0: the 24 bytes of x, y, z are read, not the whole record),
+// global index j; ends with __syncthreads().
+template <bool WITH_W, class F>
 __device__ void stage_each(const BlockShape& s, const HaloCell* hc, int total, const Pos4* __restrict__ pos,
                            const double* L, F f)
 {
@@ -174,7 +175,14 @@ __device__ void stage_each(const BlockShape& s, const HaloCell* hc, int total, c
     const int h = find_cell(hc, nh, c);
     const HaloCell e = hc[h];
     const int j = e.gstart + (c - e.lstart);
-    Pos4 p = load_pos(pos, j);
+    Pos4 p;
+    if constexpr (WITH_W) {
+      p = load_pos(pos, j);
+    } else {
+      const double* q = reinterpret_cast<const double*>(pos + j);
+      const double2 a = *reinterpret_cast<const double2*>(q);
+      p = {a.x, a.y, q[2], 0.0};
+    }
     if (e.img) {
       const uint32_t code = (uint32_t)e.img << XSMD_NBR_INDEX_BITS;
       p.x = xsmd_image(p.x, xsmd_nbr_img_x(code), L[0]);
@@ -190,10 +198,12 @@ __device__ void stage_each(const BlockShape& s, const HaloCell* hc, int total, c
 __device__ void stage(const BlockShape& s, const HaloCell* hc, int total, const Pos4* __restrict__ pos,
                       const double* L, const uint8_t* __restrict__ type, const LRec& rec, uint8_t* ltype)
 {
-  stage_each(s, hc, total, pos, L, [&](int c, const Pos4& p, int j) {
+  const auto put = [&](int c, const Pos4& p, int j) {
     rec.put(c, p);
     if (ltype) ltype[c] = type[j];
-  });
+  };
+  if (rec.has_w) stage_each<true>(s, hc, total, pos, L, put);
+  else stage_each<false>(s, hc, total, pos, L, put);
 }
 
 // Central atom q of the block: its halo cell h, local slot and global index.
@@ -418,7 +428,7 @@ __global__ void __launch_bounds__(kBuildThreads) k_block_build(BlockGeom g, cons
   const BoxDev bx = box[s.r];
   const bool cull = *nocull == 0ull;
   if (threadIdx.x == 0) l.total[1] = 0;  // longest list of the brick (made visible by stage's barrier)
-  stage_each(s, l.hc, total, pos, bx.L, [&](int c, const Pos4& p, int) { P[c] = double3{p.x, p.y, p.z}; });
+  stage_each<false>(s, l.hc, total, pos, bx.L, [&](int c, const Pos4& p, int) { P[c] = double3{p.x, p.y, p.z}; });
   for (int q = threadIdx.x;; q += blockDim.x) {
     Central a;
     if (!central_atom(s, l.hc, q, a)) break;
