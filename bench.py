@@ -270,8 +270,9 @@ def cpu_baseline(cfg, tab, steps, threads, dt, rc, skin):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported on this build "
-              "(the path would shard by spatial domain decomposition, see DESIGN.md).", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported: this library runs the force "
+              "path on one GPU (several GPUs would each own a spatial domain of bricks with ghost shells).",
+              file=sys.stderr)
         sys.exit(2)
     from exastamp_amd import inputs, potentials
     from exastamp_amd.operators import Simulation
