@@ -374,7 +374,7 @@ xs_status xs_chunk_neighbors(xs_ctx* h, double rcut_max, double rcut_inc, int ch
             "chunk_size must be a power of two");
     require(rcut_max > 0 && rcut_inc >= 0, XS_E_ARG, "need rcut_max > 0 and rcut_inc >= 0");
     require(!c.box_h.empty(), XS_E_STATE, "no system loaded (call xs_system_set)");
-    build_neighbor_list(c, rcut_max, rcut_inc, c.has_eam);
+    build_neighbor_list(c, rcut_max, rcut_inc, true);
   });
 }
 
@@ -578,7 +578,7 @@ xs_status xs_md_run(xs_ctx* h, int potential, int nsteps, double dt, double rcut
     const double thr2 = (0.5 * skin) * (0.5 * skin);
 
     if (!cont) {
-      build_neighbor_list(c, rcut, skin, potential == XS_POT_EAM);
+      build_neighbor_list(c, rcut, skin, true);
       rebuilds++;
     }
     auto is_thermo = [&](int step) { return thermo_every > 0 && step % thermo_every == 0; };
@@ -620,7 +620,7 @@ xs_status xs_md_run(xs_ctx* h, int potential, int nsteps, double dt, double rcut
       }
       if (c.prof.on) c.prof.flush();
       if (over) {
-        build_neighbor_list(c, rcut, skin, potential == XS_POT_EAM);
+        build_neighbor_list(c, rcut, skin, true);
         rebuilds++;
       }
       drifted = forces(step, 0.5 * dt, step < nsteps);

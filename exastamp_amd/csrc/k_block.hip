@@ -844,6 +844,64 @@ __global__ void __launch_bounds__(kPassThreads, kPassWaves<VIRIAL>) k_block_eam_
   }
 }
 
+// LJ forces (lj_compute_force, pair_potential_force_op_singlemat.h:45-218) on the
+// block list: the staging, list walk and epilogues of k_block_eam_force with the
+// pair math of the global-list kernel (pm::lj_pair).
+template <class Real, bool ENERGY, bool VIRIAL, bool FUSED, bool DRIFT>
+__global__ void __launch_bounds__(kPassThreads, kPassWaves<VIRIAL>) k_block_lj(BlockArgs a, pm::LJ<Real> lp,
+                                                                           pm::Outputs o, pm::Drift dr)
+{
+  static_assert(!DRIFT || (FUSED && !ENERGY && !VIRIAL), "the drift epilogue is the plain MD-loop step");
+  const int64_t b = lblock();
+  if (b >= a.g.nblocks) return;
+  LdsLayout l = lds_layout(a.cap, a.g.nh_max, false, false);
+  const BlockShape s = block_shape(a.g, b);
+  const int total = build_halo(a.g, s, a.cell_off, l.hc, l.total);
+  if (total > a.cap) return;  // excluded by the host (cap = measured maximum)
+  stage(s, l.hc, total, a.pos, a.box[s.r].L, nullptr, l.rec, nullptr);
+  const int half = pass_half();
+  for (int q = pass_slot();; q += kPassAtoms) {
+    Central ca;
+    if (!central_atom(s, l.hc, q, ca)) break;
+    const Pos4 pi = l.rec.get(ca.local);
+    const int cnt = a.nnb[ca.global];
+    double fx = 0.0, fy = 0.0, fz = 0.0, e = 0.0;
+    double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    uint2 wnext = lword(a, ca.global, half);
+    for (int k0 = half; k0 < cnt; k0 += 2 * kBBatch) {
+      const uint2 w = wnext;
+      wnext = lword(a, ca.global, k0 + 8 < cnt ? k0 + 8 : k0);
+      int c[kBBatch];
+      lslots(w, ca.local, k0, cnt, c);
+      LNb nb[kBBatch];
+      lgather<kBBatch>(a, l.rec, c, k0, cnt, pi, nb);
+#pragma unroll
+      for (int u = 0; u < kBBatch; u++) {
+        double de_r, eh;
+        pm::lj_pair(lp, nb[u].d2, nb[u].in, de_r, eh);
+        const double fe0 = nb[u].x * de_r, fe1 = nb[u].y * de_r, fe2 = nb[u].z * de_r;
+        fx += fe0;
+        fy += fe1;
+        fz += fe2;
+        if constexpr (ENERGY) e += eh;
+        if constexpr (VIRIAL) pm::add_virial(v, fe0, fe1, fe2, nb[u].x, nb[u].y, nb[u].z);
+      }
+    }
+    fx = half_sum(fx);
+    fy = half_sum(fy);
+    fz = half_sum(fz);
+    if constexpr (ENERGY) e = half_sum(e);
+    if constexpr (VIRIAL) {
+#pragma unroll
+      for (int k = 0; k < 9; k++) v[k] = half_sum(v[k]);
+    }
+    if (half == 0) {
+      if constexpr (DRIFT) pm::write_outputs_drift(o, dr, ca.global, pi, fx, fy, fz);
+      else pm::write_outputs<ENERGY, VIRIAL, FUSED>(o, ca.global, fx, fy, fz, e, v);
+    }
+  }
+}
+
 // Fused next-step drift of the MD loop (ForceEpilogue::drift) for a force launch
 // with the plain MD-loop epilogue; false when it does not apply.
 bool drift_setup(Ctx& c, const ForceEpilogue& epi, unsigned flags, pm::Drift& dr)
@@ -1111,6 +1169,52 @@ void launch_eam_block(Ctx& c, double rcut, bool do_rho, bool do_emb, bool do_for
     if (multi) block_eam_dispatch<double, true>(c, rcut, do_rho, do_emb, do_force, flags, epi);
     else block_eam_dispatch<double, false>(c, rcut, do_rho, do_emb, do_force, flags, epi);
   }
+}
+
+void launch_lj_block(Ctx& c, double eps, double sigma, double rcut, unsigned flags, const ForceEpilogue& epi)
+{
+  const double ecut = lj_energy_cutoff(eps, sigma, rcut);
+  BlockArgs a;
+  a.g = c.bgeom;
+  a.cell_off = c.cell_off.p;
+  a.pos = c.pos.p;
+  a.type = c.type.p;
+  a.box = c.box_d.p;
+  a.nbr16 = c.nbr16.p;
+  a.nnb = c.nnb.p;
+  a.npad = c.npad;
+  a.cap = c.bcap;
+  a.rc2 = rcut * rcut;
+  const size_t lds = block_lds_bytes(c, false, false);
+  const unsigned g = block_grid(c);
+  const pm::Outputs o = make_outputs(c, epi);
+  const bool energy = flags & XS_ENERGY, vir = flags & XS_VIRIAL;
+  pm::Drift dr{};
+  const bool drift = drift_setup(c, epi, flags, dr);
+  ProfScope prof(c.prof, c.stream, XS_PROF_LJ);
+  auto run = [&](auto zero) {
+    using Real = decltype(zero);
+    const pm::LJ<Real> lp{(Real)(sigma * sigma), (Real)(4.0 * eps), (Real)(24.0 * eps), (Real)ecut};
+#define XS_BL(EN, VI, FU, DR) k_block_lj<Real, EN, VI, FU, DR><<<g, kPassThreads, lds, c.stream>>>(a, lp, o, dr)
+    if (drift) {
+      XS_BL(false, false, true, true);
+    } else if (epi.fused) {
+      if (energy && vir) XS_BL(true, true, true, false);
+      else if (energy) XS_BL(true, false, true, false);
+      else if (vir) XS_BL(false, true, true, false);
+      else XS_BL(false, false, true, false);
+    } else {
+      if (energy && vir) XS_BL(true, true, false, false);
+      else if (energy) XS_BL(true, false, false, false);
+      else if (vir) XS_BL(false, true, false, false);
+      else XS_BL(false, false, false, false);
+    }
+#undef XS_BL
+  };
+  if (flags & XS_MIXED) run(0.0f);
+  else run(0.0);
+  XS_HIP(hipGetLastError());
+  if (drift) drift_commit(c);
 }
 
 void ensure_global_list(Ctx& c)

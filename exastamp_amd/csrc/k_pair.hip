@@ -76,9 +76,7 @@ __device__ __forceinline__ void fetch(const PairCommon& pc, int64_t i, int k0, i
 // Written in r^2 only: (sigma/r)^2 = sigma^2 / d2 and de/r = -24 eps (2 s^12 - s^6) / d2.
 // ------------------------------------------------------------------------------
 template <class Real>
-struct LJParams {
-  Real sigma2, eps4, eps24, ecut;
-};
+using LJParams = pm::LJ<Real>;
 
 template <class Real, bool ENERGY, bool VIRIAL, bool FUSED>
 __global__ void __launch_bounds__(256) k_lj(PairCommon pc, LJParams<Real> lp, Outputs o)
@@ -95,16 +93,13 @@ __global__ void __launch_bounds__(256) k_lj(PairCommon pc, LJParams<Real> lp, Ou
     fetch<kBatch>(pc, i, k0, cnt, pi, L, nb);
 #pragma unroll
     for (int u = 0; u < kBatch; u++) {
-      const Real ir2 = recip((Real)(nb[u].in ? nb[u].d2 : 1.0));
-      const Real s2 = lp.sigma2 * ir2;
-      const Real s6 = s2 * s2 * s2;
-      const Real s12 = s6 * s6;
-      const double de_r = nb[u].in ? (double)(-lp.eps24 * (Real(2) * s12 - s6) * ir2) : 0.0;
+      double de_r, eh;
+      pm::lj_pair(lp, nb[u].d2, nb[u].in, de_r, eh);
       const double fe0 = de_r * nb[u].x, fe1 = de_r * nb[u].y, fe2 = de_r * nb[u].z;
       fx += fe0;
       fy += fe1;
       fz += fe2;
-      if constexpr (ENERGY) e += nb[u].in ? 0.5 * (double)(lp.eps4 * (s12 - s6) - lp.ecut) : 0.0;
+      if constexpr (ENERGY) e += eh;
       if constexpr (VIRIAL) add_virial(v, fe0, fe1, fe2, nb[u].x, nb[u].y, nb[u].z);
     }
   }
@@ -264,12 +259,7 @@ void eam_dispatch(Ctx& c, double rcut, bool do_rho, bool do_emb, bool do_force, 
 template <class Real>
 void lj_dispatch(Ctx& c, double eps, double sigma, double rcut, unsigned flags, const ForceEpilogue& epi)
 {
-  // energy_cutoff (pair_potential_impl.hxx:487-498): E(rcut) in FP64
-  double ecut = 0.0;
-  if (rcut > 0.0) {
-    const double ratio = sigma / rcut, r2 = ratio * ratio, r6 = r2 * r2 * r2, r12 = r6 * r6;
-    ecut = 4. * eps * (r12 - r6);
-  }
+  const double ecut = lj_energy_cutoff(eps, sigma, rcut);
   LJParams<Real> lp{(Real)(sigma * sigma), (Real)(4.0 * eps), (Real)(24.0 * eps), (Real)ecut};
   const PairCommon pc = make_common(c, rcut);
   const Outputs o = make_outputs(c, epi);
@@ -368,7 +358,10 @@ void launch_lj(Ctx& c, double eps, double sigma, double rcut, unsigned flags, co
 {
   c.drift_done = false;
   if (c.n == 0) return;
-  ensure_global_list(c);
+  if (c.list_kind == ListKind::Block) {
+    launch_lj_block(c, eps, sigma, rcut, flags, epi);
+    return;
+  }
   if (flags & XS_MIXED) lj_dispatch<float>(c, eps, sigma, rcut, flags, epi);
   else lj_dispatch<double>(c, eps, sigma, rcut, flags, epi);
 }

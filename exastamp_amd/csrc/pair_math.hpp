@@ -280,6 +280,25 @@ __device__ __forceinline__ void eam_pair(const EamTabDev<Real>& t, const Real* _
   }
 }
 
+// Lennard-Jones pair (lennard_jones.h:40-50, force_op_impl2.hxx:33-78) written in
+// r^2 only: (sigma/r)^2 = sigma^2 / d2, de/r = -24 eps (2 s^12 - s^6) / d2, e shifted
+// by E(rcut) (pair_potential_impl.hxx:487-498).  Pair math in Real, FP64 out.
+template <class Real>
+struct LJ {
+  Real sigma2, eps4, eps24, ecut;
+};
+
+template <class Real>
+__device__ __forceinline__ void lj_pair(const LJ<Real>& lp, double d2, bool in, double& de_r, double& e)
+{
+  const Real ir2 = recip((Real)(in ? d2 : 1.0));
+  const Real s2 = lp.sigma2 * ir2;
+  const Real s6 = s2 * s2 * s2;
+  const Real s12 = s6 * s6;
+  de_r = in ? (double)(-lp.eps24 * (Real(2) * s12 - s6) * ir2) : 0.0;
+  e = in ? 0.5 * (double)(lp.eps4 * (s12 - s6) - lp.ecut) : 0.0;
+}
+
 }  // namespace pm
 
 pm::Outputs make_outputs(Ctx& c, const ForceEpilogue& ep);

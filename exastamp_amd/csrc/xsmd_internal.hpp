@@ -350,7 +350,16 @@ struct ForceEpilogue {
   double drift_dt = 0;
   double drift_thr2 = 0;
 };
+// energy_cutoff (pair_potential_impl.hxx:487-498): E_LJ(rcut) in FP64, 0 for rcut <= 0
+inline double lj_energy_cutoff(double eps, double sigma, double rcut)
+{
+  if (rcut <= 0.0) return 0.0;
+  const double ratio = sigma / rcut, r2 = ratio * ratio, r6 = r2 * r2 * r2, r12 = r6 * r6;
+  return 4. * eps * (r12 - r6);
+}
 void launch_lj(Ctx& c, double eps, double sigma, double rcut, unsigned flags, const ForceEpilogue& ep);
+// LJ on the cell-block list (k_block.hip)
+void launch_lj_block(Ctx& c, double eps, double sigma, double rcut, unsigned flags, const ForceEpilogue& ep);
 void launch_eam(Ctx& c, double rcut, bool do_rho, bool do_rho2emb, bool do_force, unsigned flags,
                 const ForceEpilogue& ep);
 void launch_eam_block(Ctx& c, double rcut, bool do_rho, bool do_emb, bool do_force, unsigned flags,

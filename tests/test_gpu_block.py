@@ -1,4 +1,4 @@
-"""Cell-block (LDS-staged) EAM path against the global-list path and the oracle.
+"""Cell-block (LDS-staged) EAM and LJ paths against the global-list path and the oracle.
 
 The block list is the global list in another encoding (16-bit halo slots instead of
 (index, image) words): converted back it must be identical entry by entry.  The block
@@ -19,7 +19,7 @@ import numpy as np
 import pytest
 
 from conftest import cu_tables, cuni_tables
-from test_gpu_parity import CU_MASS, cu_config, frel, make_oracle, make_sim
+from test_gpu_parity import AR_MASS, CU_MASS, cu_config, frel, lj_c1_config, lj_params, make_oracle, make_sim
 
 pytestmark = pytest.mark.gpu
 
@@ -127,6 +127,82 @@ def test_block_md_run_matches_global(monkeypatch, mixed):
         out.append((sim.state(), stats))
     (ra, va), sa = out[0]
     (rb, vb), sb = out[1]
+    assert np.abs(ra - rb).max() <= (1e-6 if mixed else 1e-9)
+    assert _close(va, vb, 1e-6 if mixed else 1e-9)
+
+
+def _lj_sim(monkeypatch, kind, cfg):
+    if kind == "global":
+        monkeypatch.setenv("XSMD_LIST", "global")
+    else:
+        monkeypatch.delenv("XSMD_LIST", raising=False)
+    eps, sig, rc = lj_params()
+    sim = make_sim(cfg, [AR_MASS])
+    sim.lj_parameters(parameters={"epsilon": eps, "sigma": sig}, rcut=rc)
+    return sim
+
+
+def _lj_jitter_config():
+    from exastamp_amd import inputs
+    cfg = lj_c1_config(seed=21, cells=(9, 8, 7))
+    return inputs.Config(cfg.r + inputs.rng(5).normal(scale=0.08, size=cfg.r.shape), cfg.v, cfg.type, cfg.lo,
+                         cfg.hi)
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_block_lj_forces_match_global(monkeypatch, mixed):
+    """LJ on the block list: forces, energies and virials as on the global list."""
+    eps, sig, rc = lj_params()
+    cfg = _lj_jitter_config()
+    out = []
+    for kind in ("block", "global"):
+        sim = _lj_sim(monkeypatch, kind, cfg)
+        sim.chunk_neighbors(rcut_max=rc, rcut_inc=0.3)
+        assert sim.list_info()["kind"] == kind
+        sim.zero_force_energy()
+        sim.lj_compute_force(parameters={"epsilon": eps, "sigma": sig}, rcut=rc, trigger_thermo_state=True,
+                             compute_virial=True, mixed=mixed)
+        out.append((sim.accelerations(), sim.ep(), sim.virial()))
+    for x, y in zip(*out):
+        assert _close(x, y, 1e-6 if mixed else 1e-10)
+
+
+def test_block_lj_vs_oracle(monkeypatch, oracle_mod):
+    """LJ on the block list against the oracle (FP64 1e-11 of rms|F|)."""
+    eps, sig, rc = lj_params()
+    cfg = _lj_jitter_config()
+    sim = _lj_sim(monkeypatch, "block", cfg)
+    sim.chunk_neighbors(rcut_max=rc, rcut_inc=0.3)
+    assert sim.list_info()["kind"] == "block"
+    sim.zero_force_energy()
+    sim.lj_compute_force(parameters={"epsilon": eps, "sigma": sig}, rcut=rc, trigger_thermo_state=True,
+                         compute_virial=True)
+    sim.force_to_accel()
+    o = make_oracle(oracle_mod, cfg, [AR_MASS])
+    o.set_lj(eps, sig, rc)
+    o.build_neighbors(rc, 0.3)
+    o.compute_forces("lj")
+    assert frel(sim.forces(), o.forces()) <= 1e-11
+    ref = o.state()
+    assert abs(sim.ep().sum() - ref["ep"].sum()) <= 1e-12 * abs(ref["ep"].sum())
+    vir, vref = sim.virial().sum(0), ref["virial"].sum(0)
+    assert np.abs(vir - vref).max() <= 1e-12 * np.abs(vref).max()
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_block_lj_md_run_matches_global(monkeypatch, mixed):
+    """40 fused LJ NVE steps (drift epilogue on the block list) against the global list."""
+    eps, sig, rc = lj_params()
+    cfg = lj_c1_config(seed=23, cells=(8, 8, 8))
+    out = []
+    for kind in ("block", "global"):
+        sim = _lj_sim(monkeypatch, kind, cfg)
+        stats, _ = sim.md_run("lj", 40, 0.004, rc, 0.3, mixed=mixed)
+        assert sim.list_info()["kind"] == kind
+        out.append((sim.state(), stats))
+    (ra, va), sa = out[0]
+    (rb, vb), sb = out[1]
+    assert sa.rebuilds == sb.rebuilds
     assert np.abs(ra - rb).max() <= (1e-6 if mixed else 1e-9)
     assert _close(va, vb, 1e-6 if mixed else 1e-9)
 
