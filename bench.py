@@ -53,6 +53,7 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=12)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-symmetric", action="store_true", help="skip the full-list vs Newton-symmetric operator timing")
     ap.add_argument("--stream-copy-gib", type=float, default=0.0,
                     help="also time a plain HBM copy of this many GiB (the counter calibration kernel)")
     ap.add_argument("--seed", type=int, default=2024)
@@ -267,6 +268,39 @@ def cpu_baseline(cfg, tab, steps, threads, dt, rc, skin):
     }
 
 
+def operator_forms(sim, rc, reps=5):
+    """The eam_alloy_force operator (density, embedding and force passes, mixed) in
+    its full-list form and in its Newton-symmetric form (half list, FP64 atomics):
+    GPU time per call of each pass class, and the symmetric form's force deviation."""
+    from exastamp_amd import _lib
+    import ctypes as C
+    out = {}
+    forces = {}
+    for name, sym in (("full_list", False), ("symmetric", True)):
+        sim.zero_force_energy()
+        sim.eam_alloy_force(rcut=rc, mixed=True, eam_symmetry=sym)   # untimed: builds the half list
+        forces[name] = sim.accelerations()
+        sim.ctx.xs_synchronize()
+        sim.ctx.xs_profile_enable(1)
+        for _ in range(reps):
+            sim.zero_force_energy()
+            sim.eam_alloy_force(rcut=rc, mixed=True, eam_symmetry=sym)
+        sim.ctx.xs_synchronize()
+        ms = np.zeros(len(_lib.PROF_CLASSES))
+        cnt = np.zeros(len(_lib.PROF_CLASSES), dtype=np.int64)
+        sim.ctx.xs_profile_get(_lib.dptr(ms), _lib.ptr(cnt, C.c_int64))
+        sim.ctx.xs_profile_enable(0)
+        k = {c: i for i, c in enumerate(_lib.PROF_CLASSES)}
+        rho, force = ms[k["eam_rho"]] / reps, ms[k["eam_force"]] / reps
+        out[name] = {"density_ms": float(rho), "force_ms": float(force), "total_ms": float(rho + force)}
+    f0, f1 = forces["full_list"], forces["symmetric"]
+    rms = float(np.sqrt((f0 ** 2).sum(1).mean()))
+    out["symmetric_max_dev_over_rms"] = float(np.sqrt(((f1 - f0) ** 2).sum(1)).max() / rms)
+    out["note"] = ("operator calls outside the timed MD loop; symmetric = each pair once from a half list, "
+                   "partner share by FP64 atomics on the global list")
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -295,6 +329,12 @@ def main():
                           within.ctypes.data_as(C.POINTER(C.c_int64)))
     pairs_per_atom = float(within[0]) / cfg.n
     log(f"mixed: {mixed['value']:.4g} atom-steps/s, force err {ferr['max_abs_over_rms']:.2e}")
+
+    sym = None
+    if not args.no_symmetric:
+        sym = operator_forms(sim, rc)
+        log(f"operator eam_alloy_force: full list {sym['full_list']['total_ms']:.3f} ms, "
+            f"symmetric {sym['symmetric']['total_ms']:.3f} ms")
 
     fp64 = None
     if not args.no_fp64:
@@ -384,6 +424,7 @@ def main():
         "neighbor_list": list_info,
         "kernels_mixed": mixed["kernels"],
         "kernels_fp64": None if fp64 is None else fp64["kernels"],
+        "newton_symmetric": sym,
         "roofline": roof,
         "stream_copy": stream,
         "cpu_baseline": cpu,

@@ -17,6 +17,7 @@ XS_ENERGY = 1
 XS_VIRIAL = 2
 XS_MIXED = 4
 XS_CONTINUE = 8
+XS_SYMMETRIC = 16
 XS_POT_LJ = 0
 XS_POT_EAM = 1
 XS_FIELD_EP = 0

@@ -280,6 +280,7 @@ xs_status xs_system_set(xs_ctx* h, int64_t n, const double* rx, const double* ry
     c.n = n;
     allocate_atoms(c, n);
     c.list_valid = false;
+    c.half_valid = false;
     c.rho_state = RhoState::None;
     c.rho_ckpt = false;
     if (n > 0) {
@@ -566,6 +567,7 @@ xs_status xs_md_run(xs_ctx* h, int potential, int nsteps, double dt, double rcut
     require(nsteps >= 0 && dt > 0 && rcut > 0 && skin >= 0, XS_E_ARG, "bad run parameters");
     require(thermo_every <= 0 || thermo_out, XS_E_ARG, "thermo_out required when thermo_every > 0");
     require(potential == XS_POT_LJ ? c.has_lj : c.has_eam, XS_E_STATE, "potential not set");
+    require(!(flags & XS_SYMMETRIC), XS_E_ARG, "xs_md_run evaluates full lists (XS_SYMMETRIC is for the operators)");
     const unsigned prec = flags & XS_MIXED;
     const bool cont = (flags & XS_CONTINUE) != 0;
     if (cont) {

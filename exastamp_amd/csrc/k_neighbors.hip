@@ -199,6 +199,7 @@ void build_neighbor_list(Ctx& c, double rcut, double skin, bool prefer_block)
 {
   ProfScope prof(c.prof, c.stream, XS_PROF_NEIGHBORS);
   const int64_t n = c.n;
+  c.half_valid = false;
   if (n == 0) {
     c.list_valid = true;
     c.list_kind = ListKind::Global;

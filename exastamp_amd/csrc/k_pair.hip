@@ -1,8 +1,10 @@
 // k_pair.hip -- Lennard-Jones and EAM pair kernels over the global neighbour list.
 //
 // One thread per central atom walks its ELL list (coalesced entry loads), gathers
-// neighbour position records, and accumulates its own force only: no atomics, no
-// Newton-symmetric scatter, so results do not depend on scheduling.
+// neighbour position records, and accumulates its own force only: no atomics, so
+// results do not depend on scheduling.  The exception is the Newton-symmetric
+// operator form (XS_SYMMETRIC) at the end of the file, which walks half of the list
+// and adds each partner's share with FP64 atomics.
 //
 // The neighbour loop is branch-free and batched: U entries are loaded, then U
 // position records (one 32-byte sector each), then U pairs are evaluated with
@@ -282,6 +284,239 @@ void lj_dispatch(Ctx& c, double eps, double sigma, double rcut, unsigned flags, 
   XS_HIP(hipGetLastError());
 }
 
+// ------------------------------------------------------------------------------
+// Newton-symmetric operators: SymRhoOp / SymForceOp with NewtonSym = true
+// (eam_force_op_multimat.h:114-343) and lj_compute_force_symetric
+// (pair_potential_singlemat_symetric.cpp:101-332).  Each pair is evaluated once,
+// from the half list; the atom of the list adds its share in registers, the
+// partner's share goes out as FP64 atomic adds (rho_j += rho_{type_i}(r);
+// f_j -= fe; ep_j += phi/2; the partner gets the same virial term), and each atom
+// adds its own sum atomically at the end.  Operator form only (accumulating into
+// the force arrays); the MD loop keeps the full-list kernels.
+// ------------------------------------------------------------------------------
+
+// Half of the global list: entry (j, image) of atom i is kept when j > i, or, for
+// a periodic image of i itself (j == i), when the image's first nonzero
+// component is positive.  Each pair (i, j, image) / (j, i, -image) is then listed
+// once, from one of its two atoms.
+__device__ __forceinline__ bool half_keep(uint32_t i, uint32_t e)
+{
+  const uint32_t j = xsmd_nbr_index(e);
+  if (j != i) return j > i;
+  const int ix = xsmd_nbr_img_x(e), iy = xsmd_nbr_img_y(e), iz = xsmd_nbr_img_z(e);
+  return ix > 0 || (ix == 0 && (iy > 0 || (iy == 0 && iz > 0)));
+}
+
+__global__ void __launch_bounds__(256) k_half_list(int64_t n, int64_t npad, const uint32_t* __restrict__ nbr,
+                                                   const int* __restrict__ nnb, uint32_t* __restrict__ nbr_h,
+                                                   int* __restrict__ nnb_h)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int cnt = nnb[i];
+  int m = 0;
+  for (int k = 0; k < cnt; k++) {
+    const uint32_t e = nbr[(int64_t)k * npad + i];
+    if (half_keep((uint32_t)i, e)) nbr_h[(int64_t)(m++) * npad + i] = e;
+  }
+  nnb_h[i] = m;
+}
+
+__global__ void __launch_bounds__(256) k_fill0(double* __restrict__ x, int64_t n)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = 0.0;
+}
+
+// rho_i += rho_{type_j}(r), rho_j += rho_{type_i}(r) into rho_acc (zeroed before)
+template <class Real, bool MULTI>
+__global__ void __launch_bounds__(256) k_eam_rho_sym(PairCommon pc, EamTabDev<Real> t, const Real* __restrict__ pair,
+                                                     double* __restrict__ rho_acc)
+{
+  const int64_t i = logical_block() * blockDim.x + threadIdx.x;
+  if (i >= pc.n) return;
+  const int ti = MULTI ? pc.type[i] : 0;
+  const double* L = pc.box[pc.rep ? pc.rep[i] : 0].L;
+  const Pos4 pi = load_pos(pc.pos, i);
+  const int cnt = pc.nnb[i];
+  double rho = 0.0;
+  for (int k0 = 0; k0 < cnt; k0 += kBatch) {
+    Nb nb[kBatch];
+    fetch<kBatch>(pc, i, k0, cnt, pi, L, nb);
+#pragma unroll
+    for (int u = 0; u < kBatch; u++) {
+      const int tj = MULTI ? pc.type[nb[u].j] : 0;
+      const double d2 = nb[u].in ? nb[u].d2 : 1.0;
+      const Real to_i = eam_rho<Real, MULTI>(t, pair, ti, tj, d2);
+      Real to_j = to_i;
+      if constexpr (MULTI) to_j = eam_rho<Real, MULTI>(t, pair, tj, ti, d2);
+      if (nb[u].in) {
+        rho += (double)to_i;
+        atomicAdd(rho_acc + nb[u].j, (double)to_j);
+      }
+    }
+  }
+  atomicAdd(rho_acc + i, rho);
+}
+
+// the partner's share of one pair: -fe, phi/2 and the pair's virial term
+template <bool ENERGY, bool VIRIAL>
+__device__ __forceinline__ void scatter_partner(const Outputs& o, uint32_t j, double fe0, double fe1, double fe2,
+                                                double eh, const Nb& nb)
+{
+  atomicAdd(o.ax + j, -fe0);
+  atomicAdd(o.ay + j, -fe1);
+  atomicAdd(o.az + j, -fe2);
+  if constexpr (ENERGY) atomicAdd(o.ep + j, eh);
+  if constexpr (VIRIAL) {
+    double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    add_virial(v, fe0, fe1, fe2, nb.x, nb.y, nb.z);
+#pragma unroll
+    for (int q = 0; q < 9; q++) atomicAdd(o.vir + q * o.npad + j, v[q]);
+  }
+}
+
+template <bool ENERGY, bool VIRIAL>
+__device__ __forceinline__ void add_own(const Outputs& o, int64_t i, double fx, double fy, double fz, double e,
+                                        const double* v)
+{
+  atomicAdd(o.ax + i, fx);
+  atomicAdd(o.ay + i, fy);
+  atomicAdd(o.az + i, fz);
+  if constexpr (ENERGY) atomicAdd(o.ep + i, e);
+  if constexpr (VIRIAL) {
+#pragma unroll
+    for (int q = 0; q < 9; q++) atomicAdd(o.vir + q * o.npad + i, v[q]);
+  }
+}
+
+template <class Real, bool MULTI, bool ENERGY, bool VIRIAL>
+__global__ void __launch_bounds__(256) k_eam_force_sym(PairCommon pc, EamTabDev<Real> t, const Real* __restrict__ pair,
+                                                       Outputs o)
+{
+  const int64_t i = logical_block() * blockDim.x + threadIdx.x;
+  if (i >= pc.n) return;
+  const int ti = MULTI ? pc.type[i] : 0;
+  const double* L = pc.box[pc.rep ? pc.rep[i] : 0].L;
+  const Pos4 pi = load_pos(pc.pos, i);
+  const Real fpi = (Real)pi.w;
+  double fx = 0.0, fy = 0.0, fz = 0.0, e = 0.0;
+  double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  const int cnt = pc.nnb[i];
+  for (int k0 = 0; k0 < cnt; k0 += kBatch) {
+    Nb nb[kBatch];
+    fetch<kBatch>(pc, i, k0, cnt, pi, L, nb);
+#pragma unroll
+    for (int u = 0; u < kBatch; u++) {
+      const int tj = MULTI ? pc.type[nb[u].j] : 0;
+      Real fp_r, phi;
+      eam_pair<Real, MULTI>(t, pair, ti, tj, fpi, (Real)nb[u].w, nb[u].in ? nb[u].d2 : 1.0, fp_r, phi);
+      if (nb[u].in) {
+        const double fpair = (double)fp_r;
+        const double fe0 = nb[u].x * fpair, fe1 = nb[u].y * fpair, fe2 = nb[u].z * fpair;
+        const double eh = 0.5 * (double)phi;
+        fx += fe0;
+        fy += fe1;
+        fz += fe2;
+        if constexpr (ENERGY) e += eh;
+        if constexpr (VIRIAL) add_virial(v, fe0, fe1, fe2, nb[u].x, nb[u].y, nb[u].z);
+        scatter_partner<ENERGY, VIRIAL>(o, nb[u].j, fe0, fe1, fe2, eh, nb[u]);
+      }
+    }
+  }
+  add_own<ENERGY, VIRIAL>(o, i, fx, fy, fz, e, v);
+}
+
+template <class Real, bool ENERGY, bool VIRIAL>
+__global__ void __launch_bounds__(256) k_lj_sym(PairCommon pc, LJParams<Real> lp, Outputs o)
+{
+  const int64_t i = logical_block() * blockDim.x + threadIdx.x;
+  if (i >= pc.n) return;
+  const double* L = pc.box[pc.rep ? pc.rep[i] : 0].L;
+  const Pos4 pi = load_pos(pc.pos, i);
+  double fx = 0.0, fy = 0.0, fz = 0.0, e = 0.0;
+  double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  const int cnt = pc.nnb[i];
+  for (int k0 = 0; k0 < cnt; k0 += kBatch) {
+    Nb nb[kBatch];
+    fetch<kBatch>(pc, i, k0, cnt, pi, L, nb);
+#pragma unroll
+    for (int u = 0; u < kBatch; u++) {
+      double de_r, eh;
+      pm::lj_pair(lp, nb[u].d2, nb[u].in, de_r, eh);
+      if (nb[u].in) {
+        const double fe0 = de_r * nb[u].x, fe1 = de_r * nb[u].y, fe2 = de_r * nb[u].z;
+        fx += fe0;
+        fy += fe1;
+        fz += fe2;
+        if constexpr (ENERGY) e += eh;
+        if constexpr (VIRIAL) add_virial(v, fe0, fe1, fe2, nb[u].x, nb[u].y, nb[u].z);
+        scatter_partner<ENERGY, VIRIAL>(o, nb[u].j, fe0, fe1, fe2, eh, nb[u]);
+      }
+    }
+  }
+  add_own<ENERGY, VIRIAL>(o, i, fx, fy, fz, e, v);
+}
+
+PairCommon half_common(Ctx& c, double rcut)
+{
+  ensure_half_list(c);
+  PairCommon pc = make_common(c, rcut);
+  pc.nbr = c.nbr_half.p;
+  pc.nnb = c.nnb_half.p;
+  return pc;
+}
+
+template <class Real, bool MULTI>
+void eam_sym_dispatch(Ctx& c, double rcut, bool do_rho, bool do_emb, bool do_force, unsigned flags)
+{
+  const EamTabDev<Real> t = make_eam_tab<Real>(c);
+  const Real* pair = eam_pair_table<Real>(c);
+  const PairCommon pc = half_common(c, rcut);
+  const unsigned g = pair_grid(c.n);
+  const bool energy = flags & XS_ENERGY;
+  double* pos_w = reinterpret_cast<double*>(c.pos.p);
+  if (do_rho || do_emb) {
+    ProfScope prof(c.prof, c.stream, XS_PROF_EAM_RHO);
+    if (do_rho) {
+      k_fill0<<<grid_for(c.n), kBlock, 0, c.stream>>>(c.rho.p, c.n);
+      k_eam_rho_sym<Real, MULTI><<<g, kBlock, 0, c.stream>>>(pc, t, pair, c.rho.p);
+    }
+    if (do_emb) {  // F(rho), F'(rho) of the summed densities
+      if (energy) k_eam_rho<Real, MULTI, false, true, true, false><<<g, kBlock, 0, c.stream>>>(pc, t, pair, c.rho.p, pos_w, c.ep.p);
+      else k_eam_rho<Real, MULTI, false, true, false, false><<<g, kBlock, 0, c.stream>>>(pc, t, pair, c.rho.p, pos_w, c.ep.p);
+    }
+    XS_HIP(hipGetLastError());
+  }
+  if (do_force) {
+    const Outputs o = make_outputs(c, ForceEpilogue{});
+    const bool vir = flags & XS_VIRIAL;
+    ProfScope prof(c.prof, c.stream, XS_PROF_EAM_FORCE);
+    if (energy && vir) k_eam_force_sym<Real, MULTI, true, true><<<g, kBlock, 0, c.stream>>>(pc, t, pair, o);
+    else if (energy) k_eam_force_sym<Real, MULTI, true, false><<<g, kBlock, 0, c.stream>>>(pc, t, pair, o);
+    else if (vir) k_eam_force_sym<Real, MULTI, false, true><<<g, kBlock, 0, c.stream>>>(pc, t, pair, o);
+    else k_eam_force_sym<Real, MULTI, false, false><<<g, kBlock, 0, c.stream>>>(pc, t, pair, o);
+    XS_HIP(hipGetLastError());
+  }
+}
+
+template <class Real>
+void lj_sym_dispatch(Ctx& c, double eps, double sigma, double rcut, unsigned flags)
+{
+  const LJParams<Real> lp{(Real)(sigma * sigma), (Real)(4.0 * eps), (Real)(24.0 * eps),
+                          (Real)lj_energy_cutoff(eps, sigma, rcut)};
+  const PairCommon pc = half_common(c, rcut);
+  const Outputs o = make_outputs(c, ForceEpilogue{});
+  const unsigned g = pair_grid(c.n);
+  const bool energy = flags & XS_ENERGY, vir = flags & XS_VIRIAL;
+  ProfScope prof(c.prof, c.stream, XS_PROF_LJ);
+  if (energy && vir) k_lj_sym<Real, true, true><<<g, kBlock, 0, c.stream>>>(pc, lp, o);
+  else if (energy) k_lj_sym<Real, true, false><<<g, kBlock, 0, c.stream>>>(pc, lp, o);
+  else if (vir) k_lj_sym<Real, false, true><<<g, kBlock, 0, c.stream>>>(pc, lp, o);
+  else k_lj_sym<Real, false, false><<<g, kBlock, 0, c.stream>>>(pc, lp, o);
+  XS_HIP(hipGetLastError());
+}
+
 }  // namespace
 
 // conversion_z2r = 1 eV.A and conversion_frho = 1 eV of eam_alloy.h:156-157 in
@@ -354,10 +589,27 @@ Outputs make_outputs(Ctx& c, const ForceEpilogue& ep)
   return o;
 }
 
+void ensure_half_list(Ctx& c)
+{
+  if (c.half_valid) return;
+  ensure_global_list(c);
+  c.nbr_half.reserve((size_t)c.nmax * c.npad);
+  c.nnb_half.reserve(c.npad);
+  k_half_list<<<grid_for(c.n), kBlock, 0, c.stream>>>(c.n, c.npad, c.nbr.p, c.nnb.p, c.nbr_half.p, c.nnb_half.p);
+  XS_HIP(hipGetLastError());
+  c.half_valid = true;
+}
+
 void launch_lj(Ctx& c, double eps, double sigma, double rcut, unsigned flags, const ForceEpilogue& epi)
 {
   c.drift_done = false;
   if (c.n == 0) return;
+  if (flags & XS_SYMMETRIC) {
+    if (epi.fused) throw Error(XS_E_ARG, "the Newton-symmetric form is an operator form");
+    if (flags & XS_MIXED) lj_sym_dispatch<float>(c, eps, sigma, rcut, flags);
+    else lj_sym_dispatch<double>(c, eps, sigma, rcut, flags);
+    return;
+  }
   if (c.list_kind == ListKind::Block) {
     launch_lj_block(c, eps, sigma, rcut, flags, epi);
     return;
@@ -379,7 +631,16 @@ void launch_eam(Ctx& c, double rcut, bool do_rho, bool do_emb, bool do_force, un
       throw Error(XS_E_STATE, "eam_force needs F'(rho) from eam_rho2emb in the same precision");
   }
   const bool multi = c.eam.nel > 1;
-  if (c.list_kind == ListKind::Block) {
+  if (flags & XS_SYMMETRIC) {
+    if (epi.fused) throw Error(XS_E_ARG, "the Newton-symmetric form is an operator form");
+    if (mixed) {
+      if (multi) eam_sym_dispatch<float, true>(c, rcut, do_rho, do_emb, do_force, flags);
+      else eam_sym_dispatch<float, false>(c, rcut, do_rho, do_emb, do_force, flags);
+    } else {
+      if (multi) eam_sym_dispatch<double, true>(c, rcut, do_rho, do_emb, do_force, flags);
+      else eam_sym_dispatch<double, false>(c, rcut, do_rho, do_emb, do_force, flags);
+    }
+  } else if (c.list_kind == ListKind::Block) {
     launch_eam_block(c, rcut, do_rho, do_emb, do_force, flags, epi);
   } else if (mixed) {
     if (multi) eam_dispatch<float, true>(c, rcut, do_rho, do_emb, do_force, flags, epi);

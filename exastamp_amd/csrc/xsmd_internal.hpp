@@ -275,6 +275,11 @@ struct Ctx {
   bool block_disabled = false;     // XSMD_LIST=global at context creation
   double block_atoms = kBlockAtoms;  // XSMD_BLOCK_ATOMS at context creation
   bool block_atoms_set = false;
+  // half list for the Newton-symmetric operators (global encoding; built on demand
+  // from the global form of the current list)
+  DBuf<uint32_t> nbr_half;
+  DBuf<int> nnb_half;
+  bool half_valid = false;
   DBuf<uint16_t> nbr16;
   DBuf<uint8_t> bins8;  // distance keys of the block-list entries (for the sort)
   DBuf<int> cell_off;              // first atom of each cell, ncell_tot + 1
@@ -324,6 +329,8 @@ unsigned block_grid(const Ctx& c);
 void block_build(Ctx& c, double rl2);
 void block_to_global(Ctx& c);
 void ensure_global_list(Ctx& c);
+// the half list (nbr_half / nnb_half) of the current list
+void ensure_half_list(Ctx& c);
 double max_displacement2(Ctx& c);
 void pair_count(Ctx& c, double rcut, int64_t& entries, int64_t& within);
 

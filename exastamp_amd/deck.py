@@ -24,10 +24,11 @@ no-ops; any other operator name raises ``DeckError``, as exaStamp fails on an
 unknown operator.
 
 Differences that a deck cannot see in its results: the Newton-symmetric variants
-(``eam_symmetry: true``, ``lj_compute_force_symetric``) run the full-list kernels
-(same forces up to FP64 summation order); ``chunk_size`` is accepted and does not
-change the list; the random streams of ``gaussian_noise_r/v`` are numpy Philox
-streams (seed ``--seed``), not exaNBody's.
+(``eam_symmetry: true``, ``lj_compute_force_symetric``) evaluate each pair once
+from a half list and add the partner's share with FP64 atomics (same forces up to
+FP64 summation order, which varies from run to run); ``chunk_size`` is accepted
+and does not change the list; the random streams of ``gaussian_noise_r/v`` are
+numpy Philox streams (seed ``--seed``), not exaNBody's.
 
     python -m exastamp_amd.deck single_specy_nosym.msp --set-global-max_iteration 20
 """
@@ -538,13 +539,18 @@ def _trigger(r: Runner) -> bool:
     return bool(r.get("trigger_thermo_state", False))
 
 
-def _op_lj(r: Runner, p):
+def _op_lj(r: Runner, p, symmetric=False):
     rc = parse_quantity(p["rcut"])
     _publish_rcut(r, rc)
     if r.dry:
         return
     r.sim.lj_compute_force(parameters=p["parameters"], rcut=rc, trigger_thermo_state=_trigger(r),
-                           compute_virial=bool(p.get("compute_virial", True)), mixed=r.mixed)
+                           compute_virial=bool(p.get("compute_virial", True)), mixed=r.mixed,
+                           symmetric=symmetric)
+
+
+def _op_lj_symetric(r: Runner, p):
+    _op_lj(r, p, symmetric=True)
 
 
 def _tables_from(r: Runner, params):
@@ -679,7 +685,7 @@ _OPS = {
     "chunk_neighbors": _op_chunk_neighbors,
     "check_and_update_particles": _op_check_and_update_particles,
     "lj_compute_force": _op_lj,
-    "lj_compute_force_symetric": _op_lj,
+    "lj_compute_force_symetric": _op_lj_symetric,
     "eam_alloy_init": _op_eam_alloy_init,
     "eam_alloy_force": _op_eam_alloy_force,
     "zero_force_energy": _op_zero,

@@ -24,7 +24,7 @@ from pathlib import Path
 import numpy as np
 
 from . import _lib
-from ._lib import XS_ENERGY, XS_MIXED, XS_VIRIAL, XsError
+from ._lib import XS_ENERGY, XS_MIXED, XS_SYMMETRIC, XS_VIRIAL, XsError
 from .inputs import Config
 from .potentials import SetflTables
 from .units import EV, KB, PRESSURE_UNIT_PA, parse_quantity
@@ -154,7 +154,7 @@ class Simulation:
         return f
 
     def lj_compute_force(self, parameters, rcut, ghost=False, compute_virial=True,
-                         trigger_thermo_state=True, mixed=False, **_slots):
+                         trigger_thermo_state=True, mixed=False, symmetric=False, **_slots):
         """lj_compute_force{parameters: {epsilon, sigma}, rcut} (pair_potential_impl.hxx:108-121).
         The reference always fills ep (and the virial when the grid has one); here
         they are filled when trigger_thermo_state (and compute_virial) are set."""
@@ -163,9 +163,13 @@ class Simulation:
         rc = parse_quantity(rcut)
         self.rcut_max = max(self.rcut_max, rc)
         self.lj_params = (eps, sig)
-        self.ctx.xs_lj_compute_force(eps, sig, rc, self._flags(trigger_thermo_state, compute_virial, mixed))
+        flags = self._flags(trigger_thermo_state, compute_virial, mixed) | (XS_SYMMETRIC if symmetric else 0)
+        self.ctx.xs_lj_compute_force(eps, sig, rc, flags)
 
-    lj_compute_force_symetric = lj_compute_force  # full list gives the same forces
+    def lj_compute_force_symetric(self, parameters, rcut, **kw):
+        """lj_compute_force_symetric (pair_potential_singlemat_symetric.cpp:101-332):
+        the same forces, each pair evaluated once with Newton's third law."""
+        self.lj_compute_force(parameters, rcut, symmetric=True, **kw)
 
     def lj_parameters(self, parameters, rcut):
         """Set the LJ parameters slot without computing (for md_run)."""
@@ -197,16 +201,18 @@ class Simulation:
     def eam_alloy_force(self, rcut, eam_rho=True, eam_rho2emb=True, eam_ghost=True, eam_force=True,
                         eam_symmetry=False, compute_virial=False, trigger_thermo_state=False,
                         mixed=False, types=None, **_slots):
-        """eam_alloy_force (eam_potential_multimat.cu:88-109 slots).  eam_ghost and
-        eam_symmetry select ghost / Newton-symmetric variants in the reference; the
-        full-list GPU path computes the same quantities for either value."""
+        """eam_alloy_force (eam_potential_multimat.cu:88-109 slots).  eam_ghost is
+        accepted and has no effect (one GPU holds every atom; images are resolved in
+        the pair kernels).  eam_symmetry selects the Newton-symmetric evaluation
+        (each pair once from a half list, the partner's share added atomically):
+        the same quantities up to FP64 summation order."""
         if types:
             unknown = [t for t in types if t not in self.species_names]
             if unknown:
                 raise ValueError(f"eam_alloy_force: unknown types {unknown}")
         rc = parse_quantity(rcut)
         self.rcut_max = max(self.rcut_max, rc)
-        flags = self._flags(trigger_thermo_state, compute_virial, mixed)
+        flags = self._flags(trigger_thermo_state, compute_virial, mixed) | (XS_SYMMETRIC if eam_symmetry else 0)
         self.ctx.xs_eam_alloy_force(rc, int(bool(eam_rho)), int(bool(eam_rho2emb)), int(bool(eam_force)), flags)
 
     def eam_adjoint(self, dL_dF, dL_dE, mixed=False, rcut=None):

@@ -50,6 +50,11 @@ typedef enum {
                           accumulation and reductions.  Without it: all FP64. */
 #define XS_CONTINUE 8u /* xs_md_run: continue a previous run (valid list and current
                           accelerations): skip the initial build and force step */
+#define XS_SYMMETRIC 16u /* Newton-symmetric evaluation (eam_symmetry: true,
+                            lj_compute_force_symetric): each pair once, from a half of
+                            the neighbour list, the partner's share added with FP64
+                            atomics.  Operators only (not xs_md_run); the order of the
+                            atomic sums, hence the last bits, varies from run to run. */
 
 /* potentials for xs_md_run */
 #define XS_POT_LJ 0
@@ -150,7 +155,8 @@ xs_status xs_eam_alloy_tables(xs_ctx* ctx, int nel, int nrho, double drho, int n
  * element count are returned. */
 xs_status xs_eam_alloy_load(xs_ctx* ctx, const char* path, double* masses, int* nel);
 /* eam_alloy_force (eam_potential_multimat.cu:65-259) with its eam_rho / eam_rho2emb /
- * eam_force slots; eam_ghost and eam_symmetry are not needed (no ghosts; full list). */
+ * eam_force slots; eam_ghost is not needed (no ghosts); eam_symmetry is XS_SYMMETRIC
+ * (SymRhoOp / SymForceOp with NewtonSym, eam_force_op_multimat.h:114-343). */
 xs_status xs_eam_alloy_force(xs_ctx* ctx, double rcut, int eam_rho, int eam_rho2emb,
                              int eam_force, unsigned flags);
 /* push_f_v_r (config_numerical_schemes.msp:23-25): r += v*dt + a*dt^2/2,

@@ -6,8 +6,9 @@ tests/golden/decks) run through the deck runner on the GPU.
   relative 1e-9 (as the C1 trajectory test).
 - EAM Ta (16000 atoms, BCC, rc 5.3 A): the split-pass deck (eam_rho, eam_rho2emb,
   ghost_update_opt, eam_force), the single-pass deck and the Newton-symmetric deck
-  give the same energies (1e-12 relative; the symmetric variant runs the full-list
-  kernels), and the split deck matches the oracle's run (1e-9).
+  give the same energies (1e-12 relative; the symmetric deck evaluates each pair
+  once and adds the partner's share atomically, so it differs by FP64 summation
+  order only), and the split deck matches the oracle's run (1e-9).
 - EAM AlCu B2 (16000 atoms): step 0 against the run the reference recorded for this
   lattice (data/regression_new/potentials/eam/eam_alloy/thermodynamic_state.csv,
   fixture): potential energy -2.93548 eV/atom within 5e-4 (our gaussian_noise_r
