@@ -44,7 +44,7 @@ import numpy as np
 import yaml
 
 from .inputs import Config
-from .units import AMU_KG, EV, PRESSURE_UNIT_PA, parse_quantity
+from .units import AMU_KG, EV, KB, PRESSURE_UNIT_PA, parse_quantity
 
 AMU_G = AMU_KG * 1e3
 
@@ -63,6 +63,7 @@ global:
   rcut_inc: 1.0 ang
   simulation_thermostate_screen_frequency: 10
   simulation_thermostate_file_frequency: 10
+  log_mode: mechanical
   thermostate_file: "thermodynamic_state.csv"
   trigger_thermo_state: true
   md_loop_continue: true
@@ -102,7 +103,7 @@ first_iteration:
   - init_particles
   - compute_all_forces_energy
   - default_thermostate_compute
-  - default_thermostate_screen
+  - default_thermostate_screen: { lb_flag: false , move_flag: false , print_header: true }
   - default_thermostate_file
   - next_time_step
 md_trajectory_loop:
@@ -230,11 +231,12 @@ def _quantity_list(v):
 
 @dataclasses.dataclass
 class ThermoRecord:
-    """One line of the thermodynamic-state screen/CSV output, with the columns and
-    conversions of the reference's print (print_thermodynamic_state.cpp:223-282):
-    energies per particle in eV, temperature in K, Pxx Pyy Pzz = virial diagonal / V
-    (stress_tensor), Pxy Pxz Pyz = (virial + kinetic tensor) / V (full_stress_tensor)
-    in Pa, box lengths, angles, volume in A^3 and density in g/cm^3."""
+    """One thermodynamic state with the quantities and conversions of the reference's
+    outputs (print_thermodynamic_state.cpp:223-282, dump_thermodynamic_state.cpp:84-122):
+    energies per particle in eV, temperatures in K, pressures in Pa, box lengths and
+    volume in A and A^3, density in g/cm^3.  p_full holds the full stress tensor
+    (virial + kinetic tensor) / V in the order xx yy zz xy xz yz, p_vir the virial
+    part / V of xx yy zz."""
     step: int
     time_ps: float
     particles: int
@@ -242,24 +244,105 @@ class ThermoRecord:
     kinetic_ev: float
     potential_ev: float
     temperature_k: float
+    t_xyz: tuple         # Tx Ty Tz
     pressure_pa: float   # hydrostatic, (kinetic + virial) trace / 3V
-    p_diag: tuple        # Pxx Pyy Pzz
-    p_off: tuple         # Pxy Pxz Pyz
+    p_full: tuple        # full stress tensor xx yy zz xy xz yz
+    p_vir: tuple         # virial stress xx yy zz
+    vonmises_pa: float
     lengths: tuple       # A B C
     volume: float
+    mass: float          # Da
     density: float       # g/cm^3
 
     def line(self) -> str:
-        cols = [f"{self.step:9d}", f"{self.time_ps:.6e}", f"{self.particles:13d}", f"{self.total_ev: .10e}",
-                f"{self.kinetic_ev: .10e}", f"{self.potential_ev: .10e}", f"{self.temperature_k:.12f}"]
-        cols += [f"{x:.12f}" for x in (*self.p_diag, *self.p_off, *self.lengths, 90.0, 90.0, 90.0)]
-        cols += [f"{self.volume:.12f}", f"{self.density:.12f}"]
-        return "  ".join(cols)
+        """The row dump_thermodynamic_state writes (dump_thermodynamic_state.cpp:101-121),
+        without the newline; angles are 90 degrees (orthogonal boxes)."""
+        return CSV_ROW % (self.step, self.time_ps, self.particles, self.total_ev, self.kinetic_ev,
+                          self.potential_ev, self.temperature_k, *self.p_full, *self.lengths, 90.0, 90.0, 90.0,
+                          self.volume, self.density)
+
+    def log_values(self) -> dict:
+        """Values of the screen log items (print_thermodynamic_state.cpp:254-282)."""
+        return {"stp": float(self.step), "pht": self.time_ps, "prt": float(self.particles), "sta": 0.0,
+                "toe": self.total_ev, "kie": self.kinetic_ev, "poe": self.potential_ev, "ele": 0.0,
+                "tmp": self.temperature_k, "tmx": self.t_xyz[0], "tmy": self.t_xyz[1], "tmz": self.t_xyz[2],
+                "pre": self.pressure_pa, "prx": self.p_vir[0], "pry": self.p_vir[1], "prz": self.p_vir[2],
+                "pxy": self.p_full[3], "pxz": self.p_full[4], "pyz": self.p_full[5], "smi": self.vonmises_pa,
+                "vol": self.volume, "mas": self.mass, "bxa": self.lengths[0], "bxb": self.lengths[1],
+                "bxc": self.lengths[2], "baa": 90.0, "bab": 90.0, "bag": 90.0, "rho": self.density}
 
 
+# the file row and header of dump_thermodynamic_state (dump_thermodynamic_state.cpp:61,101)
+CSV_ROW = ("%9d % .6e %13d  % .10e  % .10e  % .10e  % 9.12f  % 9.12f  % 9.12f  % 9.12f  % 9.12f  % 9.12f "
+           "% 9.12f % 12.12f % 12.12f % 12.12f % 9.12f % 9.12f % 9.12f % 16.12f % 13.12f ")
 THERMO_HEADER = ("# Step     Time (ps)     Particles  Tot. E. (eV/part)  Kin. E. (eV/part)  Pot. E. (eV/part)  "
                  "Temp. (K) Pxx Pyy Pzz Pxy Pxz Pyz (Pa) A/B/C (ang)    alpha/beta/gamma (deg)     Vol. (ang^3)  "
                  "Rho (g/cm^3)")
+
+# Screen log items of print_thermodynamic_state: key -> (header, printf format)
+# (thermodynamic_log_config.h:136-165), and the item lists of its log modes
+# (print_thermodynamic_state.cpp:63-201).  A mode it does not know keeps the
+# default item list (its keyword-list parser adds no items).
+LOG_ITEMS = {
+    "stp": ("Step", "% 9.0f"), "pht": ("Time (ps)", "% .6e"), "prt": ("Particles", "% 12.0f"),
+    "sta": ("Mv/Ext/Imb.", "%.1e"), "toe": ("Tot. E. (eV/part)", "% .10e"),
+    "kie": ("Kin. E. (eV/part)", "% .10e"), "poe": ("Pot. E. (eV/part)", "% .10e"),
+    "ele": ("Elec. E. (eV/part)", "% .10e"), "tmp": ("Temp. (K)", "% 10.3f"), "tmx": ("   Tx (K)", "% 10.3f"),
+    "tmy": ("   Ty (K)", "% 10.3f"), "tmz": ("   Tz (K)", "% 10.3f"), "pre": ("Press. (Pa)", "% 11.3e"),
+    "prx": ("    Px (Pa)", "% 11.3e"), "pry": ("    Py (Pa)", "% 11.3e"), "prz": ("    Pz (Pa)", "% 11.3e"),
+    "pxy": ("   Pxy (Pa)", "% 11.3e"), "pxz": ("   Pxz (Pa)", "% 11.3e"), "pyz": ("   Pyz (Pa)", "% 11.3e"),
+    "smi": ("sMises (Pa)", "% .3e"), "vol": ("Vol. (ang^3)", "% 13.6e"), "mas": ("Mass", "% .3e"),
+    "bxa": ("A (ang)", "% .3f"), "bxb": ("B (ang)", "% .3f"), "bxc": ("C (ang)", "% .3f"),
+    "baa": ("alpha (deg)", "% 12.3f"), "bab": ("beta (deg)", "% 12.3f"), "bag": ("gamma (deg)", "% 12.3f"),
+    "rho": ("Rho (g/cm^3)", "% 12.6f"),
+}
+_BASIC = ["stp", "pht", "toe", "kie", "poe", "tmp", "pre"]
+_FULL = ["stp", "pht", "toe", "kie", "poe", "tmp", "tmx", "tmy", "tmz", "pre", "prx", "pry", "prz"]
+LOG_MODES = {
+    "default": _BASIC + ["sta"], "thermo_basic": _BASIC + ["sta"],
+    "thermo": _FULL + ["sta"], "thermo_full": _FULL + ["sta"],
+    "vol_fluct_ortho_basic": _BASIC + ["vol", "bxa", "bxb", "bxc", "rho", "sta"],
+    "vol_fluct_ortho": _FULL + ["vol", "bxa", "bxb", "bxc", "rho", "sta"],
+    "vol_fluct_ortho_full": _FULL + ["vol", "bxa", "bxb", "bxc", "rho", "sta"],
+    "vol_fluct_tricl_basic": _BASIC + ["vol", "bxa", "bxb", "bxc", "baa", "bab", "bag", "rho", "sta"],
+    "vol_fluct_tricl": _FULL + ["pxy", "pxz", "pyz", "vol", "bxa", "bxb", "bxc", "baa", "bab", "bag", "rho", "sta"],
+    "vol_fluct_tricl_full": _FULL + ["pxy", "pxz", "pyz", "vol", "bxa", "bxb", "bxc", "baa", "bab", "bag", "rho",
+                                     "sta"],
+    "mechanical": ["stp", "pht", "prt", "sta", "toe", "kie", "poe", "tmp", "pre", "smi", "vol", "mas"],
+}
+LOG_DEFAULT_ITEMS = ["stp", "pht", "prt", "sta", "toe", "kie", "poe", "tmp", "pre", "vol"]
+
+
+def _log_width(key: str) -> int:
+    header, fmt = LOG_ITEMS[key]
+    return max(len(fmt % 0.0) + (3 if key == "sta" else 0), len(header))
+
+
+def log_header(mode: str) -> str:
+    """The header line print_thermodynamic_state prints (thermodynamic_log_config.h:86-90)."""
+    return "".join(" %*s" % (_log_width(k), LOG_ITEMS[k][0]) for k in LOG_MODES.get(mode, LOG_DEFAULT_ITEMS))
+
+
+def _moves_char(count: int, one: str, many: str) -> str:
+    if count <= 0:
+        return " "
+    return one if count == 1 else (str(count) if count < 10 else many)
+
+
+def log_line(rec: ThermoRecord, mode: str, moves: int = 0) -> str:
+    """One screen line (thermodynamic_log_config.h:91-127): the status column shows the
+    particle moves (list rebuilds) since the last line; there is no load balancing
+    and no domain extension on one GPU."""
+    vals = rec.log_values()
+    out = []
+    for k in LOG_MODES.get(mode, LOG_DEFAULT_ITEMS):
+        if k == "sta":
+            out.append(" %c %c%*s" % (_moves_char(moves, "m", "M"), " ", _log_width(k) - 3, ""))
+        else:
+            out.append(" " + LOG_ITEMS[k][1] % vals[k])
+    return "".join(out)
+
+
 G_PER_CM3_PER_DA_PER_A3 = AMU_G * 1e24
 
 
@@ -287,6 +370,7 @@ class Runner:
         self.dry = False
         self._file = None
         self.rebuilds = 0
+        self.rebuilds_printed = 0
 
     # --- slots with rebind ----------------------------------------------------
     def _slot_name(self, name):
@@ -374,16 +458,22 @@ class Runner:
         v = st.volume
         kt = 2.0 * (st.kinetic_tensor - 0.5 * np.outer(st.momentum, st.momentum) / st.mass)
         vir = st.virial
+        full = (vir + kt) / v * PRESSURE_UNIT_PA
+        ke = st.raw[21:24] - 0.5 * st.momentum ** 2 / st.mass
+        d, o = np.diag(vir), (vir[0, 1], vir[0, 2], vir[1, 2])
+        mises = np.sqrt(0.5 * ((d[0] - d[1]) ** 2 + (d[1] - d[2]) ** 2 + (d[2] - d[0]) ** 2
+                               + 6.0 * (o[0] ** 2 + o[1] ** 2 + o[2] ** 2))) / v
         a = self.atoms
         lengths = tuple(float(x) for x in (a.hi[0] - a.lo[0])) if a is not None else (0.0, 0.0, 0.0)
         rec = ThermoRecord(step=int(self.get("timestep", 0)), time_ps=float(self.get("physical_time", 0.0)),
                            particles=n, total_ev=st.total_energy / EV / n, kinetic_ev=st.kinetic_energy / EV / n,
                            potential_ev=st.potential_energy / EV / n, temperature_k=st.temperature,
+                           t_xyz=tuple(float(2.0 * x / (n * KB)) for x in ke),
                            pressure_pa=st.pressure_pa,
-                           p_diag=tuple(float(vir[k, k] / v * PRESSURE_UNIT_PA) for k in range(3)),
-                           p_off=tuple(float((vir[i, j] + kt[i, j]) / v * PRESSURE_UNIT_PA)
-                                       for i, j in ((0, 1), (0, 2), (1, 2))),
-                           lengths=lengths, volume=v, density=st.mass / v * G_PER_CM3_PER_DA_PER_A3)
+                           p_full=tuple(float(full[i, j]) for i, j in ((0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2))),
+                           p_vir=tuple(float(vir[k, k] / v * PRESSURE_UNIT_PA) for k in range(3)),
+                           vonmises_pa=float(mises * PRESSURE_UNIT_PA), lengths=lengths, volume=v, mass=st.mass,
+                           density=st.mass / v * G_PER_CM3_PER_DA_PER_A3)
         self.records.append(rec)
         return rec
 
@@ -625,9 +715,17 @@ def _op_thermo_compute_if(r: Runner, p):
         r.thermo()
 
 
+def _screen(r: Runner, header: bool):
+    mode = str(r.get("log_mode", "default"))
+    if header:
+        print(log_header(mode), file=r.out)
+    print(log_line(r.records[-1], mode, r.rebuilds - r.rebuilds_printed), file=r.out)
+    r.rebuilds_printed = r.rebuilds
+
+
 def _op_thermo_screen_if(r: Runner, p):
     if r.get("trigger_thermo_screen") and r.records and r.records[-1].step == int(r.get("timestep", 0)):
-        print(r.records[-1].line(), file=r.out)
+        _screen(r, bool(p.get("print_header", False)))
 
 
 def _op_thermo_file_if(r: Runner, p):
@@ -650,9 +748,7 @@ def _op_default_thermo_compute(r: Runner, p):
 
 
 def _op_default_thermo_screen(r: Runner, p):
-    if p.get("print_header", True):
-        print(THERMO_HEADER, file=r.out)
-    print(r.records[-1].line(), file=r.out)
+    _screen(r, bool(p.get("print_header", False)))
 
 
 def _op_default_thermo_file(r: Runner, p):

@@ -56,3 +56,42 @@ def test_include_cycle_and_missing(tmp_path):
     (tmp_path / "c.msp").write_text("includes: [ nowhere.yaml ]\n")
     with pytest.raises(deck.DeckError, match="cannot find"):
         deck.load_deck(tmp_path / "c.msp")
+
+
+def _record_from_row(row):
+    from exastamp_amd import deck
+    v = row.split()
+    return deck.ThermoRecord(step=int(v[0]), time_ps=float(v[1]), particles=int(v[2]), total_ev=float(v[3]),
+                             kinetic_ev=float(v[4]), potential_ev=float(v[5]), temperature_k=float(v[6]),
+                             t_xyz=(0.0, 0.0, 0.0), pressure_pa=0.0, p_full=tuple(float(x) for x in v[7:13]),
+                             p_vir=(0.0, 0.0, 0.0), vonmises_pa=0.0, lengths=tuple(float(x) for x in v[13:16]),
+                             volume=float(v[19]), mass=0.0, density=float(v[20]))
+
+
+def test_thermo_file_rows_match_recorded_file():
+    """The reference's recorded thermodynamic_state.csv (AlCu deck, fixture): its header
+    and every row, re-printed from the parsed values, come out byte for byte."""
+    from exastamp_amd import deck
+    lines = (GOLDEN / "alcu_b2_thermodynamic_state.csv").read_text().split("\n")
+    assert lines[0] == deck.THERMO_HEADER
+    rows = [l for l in lines[1:] if l.strip()]
+    assert len(rows) >= 2
+    for row in rows:
+        assert _record_from_row(row).line() == row
+
+
+def test_screen_log_modes_and_status_column():
+    """Screen log (print_thermodynamic_state): item lists per mode, the default list for
+    an unknown mode, the moves character of the status column."""
+    from exastamp_amd import deck
+    row = (GOLDEN / "alcu_b2_thermodynamic_state.csv").read_text().split("\n")[1]
+    rec = _record_from_row(row)
+    for mode, items in deck.LOG_MODES.items():
+        assert len(deck.log_line(rec, mode).split()) == len(items) - 1   # blank status column
+        assert deck.log_header(mode).startswith("      Step")
+    assert deck.log_header("stp;pht;vol") == deck.log_header("no such mode")
+    assert deck.log_line(rec, "mechanical").split()[:3] == ["0", "0.000000e+00", "16000"]
+    for moves, ch in ((0, None), (1, "m"), (3, "3"), (12, "M")):
+        cols = deck.log_line(rec, "mechanical", moves).split()
+        assert (len(cols) == 11) if ch is None else (cols[3] == ch)
+    assert "13.672" in deck.log_line(rec, "default")
