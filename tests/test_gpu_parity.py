@@ -325,6 +325,30 @@ def test_eam_nve_vs_oracle(oracle_mod):
     assert driftm <= 2.0 * drift64 + 1e-9 * scale
 
 
+def test_eam_nve_c2_full_size(oracle_mod):
+    """C2 at its full size: EAM Cu FCC 50^3 cells = 500 000 atoms, 0.05 A jitter,
+    100 NVE steps of 1 fs, FP64 on the GPU against the FP64 oracle on the host cores:
+    the same rebuild steps, potential and kinetic energy every 10 steps within 1e-9
+    relative, final positions within 1e-8 A."""
+    tab = cu_tables()
+    cfg = cu_config((50, 50, 50), T=0.0, jitter=0.05, seed=5)
+    assert cfg.n == 500_000
+    sim = make_sim(cfg, [CU_MASS])
+    sim.eam_alloy_init(tab)
+    st, th = sim.md_run("eam", 100, 0.001, 4.95, 0.3, thermo_every=10)
+    o = make_oracle(oracle_mod, cfg, [CU_MASS])
+    o.set_eam(tab)
+    nreb, th_ref = o.md_run("eam", 100, 0.001, 4.95, 0.3)
+    assert st.rebuilds == nreb
+    ref = th_ref[::10]
+    pe = np.array([t.potential_energy for t in th])
+    ke = np.array([t.kinetic_energy for t in th])
+    assert np.abs(pe - ref[:, 24]).max() <= 1e-9 * np.abs(ref[:, 24]).max()
+    assert np.abs(ke - ref[:, 21:24].sum(1)).max() <= 1e-9 * ref[:, 21:24].sum(1).max()
+    r, _ = sim.state()
+    assert np.abs(r - o.state()["r"]).max() < 1e-8
+
+
 def test_eam_alloy_slab_two_species(oracle_mod):
     """Cu/Ni, 50/50 substitution, vacancies, free surface along z (C5 shape, small)."""
     from exastamp_amd import inputs
