@@ -165,15 +165,23 @@ __device__ __forceinline__ int find_cell(const HaloCell* hc, int nh, int c)
 
 // Stage halo atoms: f(c, p, j) for slot c, position p (+ image shift) and, with
 // WITH_W, w (else 0: the 24 bytes of x, y, z are read, not the whole record),
-// global index j; ends with __syncthreads().
+// global index j; ends with __syncthreads().  slot_cell (total <= its capacity)
+// first receives the halo cell of every slot, so that a thread finds the cell of
+// a slot with one LDS read instead of a binary search over the halo table.
+// (Issuing several slots' position loads per thread before using them measured
+// slower: the force pass lost 4 %.)
 template <bool WITH_W, class F>
 __device__ void stage_each(const BlockShape& s, const HaloCell* hc, int total, const Pos4* __restrict__ pos,
-                           const double* L, F f)
+                           const double* L, uint16_t* slot_cell, F f)
 {
   const int nh = s.hx * s.hy * s.hz;
-  for (int c = threadIdx.x; c < total; c += blockDim.x) {
-    const int h = find_cell(hc, nh, c);
+  for (int h = threadIdx.x; h < nh; h += blockDim.x) {
     const HaloCell e = hc[h];
+    for (int k = 0; k < e.count; k++) slot_cell[e.lstart + k] = (uint16_t)h;
+  }
+  __syncthreads();
+  for (int c = threadIdx.x; c < total; c += blockDim.x) {
+    const HaloCell e = hc[slot_cell[c]];
     const int j = e.gstart + (c - e.lstart);
     Pos4 p;
     if constexpr (WITH_W) {
@@ -196,14 +204,15 @@ __device__ void stage_each(const BlockShape& s, const HaloCell* hc, int total, c
 
 // Stage halo atoms into the records (and types when requested).
 __device__ void stage(const BlockShape& s, const HaloCell* hc, int total, const Pos4* __restrict__ pos,
-                      const double* L, const uint8_t* __restrict__ type, const LRec& rec, uint8_t* ltype)
+                      const double* L, const uint8_t* __restrict__ type, const LRec& rec, uint8_t* ltype,
+                      uint16_t* slot_cell)
 {
   const auto put = [&](int c, const Pos4& p, int j) {
     rec.put(c, p);
     if (ltype) ltype[c] = type[j];
   };
-  if (rec.has_w) stage_each<true>(s, hc, total, pos, L, put);
-  else stage_each<false>(s, hc, total, pos, L, put);
+  if (rec.has_w) stage_each<true>(s, hc, total, pos, L, slot_cell, put);
+  else stage_each<false>(s, hc, total, pos, L, slot_cell, put);
 }
 
 // Central atom q of the block: its halo cell h, local slot and global index.
@@ -234,6 +243,54 @@ __device__ __forceinline__ bool central_atom(const BlockShape& s, const HaloCell
   return false;
 }
 
+// Local slots of the central atoms in q order (the order of central_atom):
+// central_local[q] for q < count, count in *count_out.  Wave 0 takes one central
+// row (iz, iy) per lane, scans the row lengths with shuffles and writes the rows'
+// slots; entries beyond ccap are not written (the caller checks count <= ccap).
+// The caller's next __syncthreads publishes the table.
+__device__ void central_table(const BlockShape& s, const HaloCell* hc, uint16_t* central_local, int ccap,
+                              int* count_out)
+{
+  if (threadIdx.x >= 64) return;
+  const int lane = threadIdx.x;
+  const int nrows = s.ez * s.ey;
+  int carry = 0;
+  for (int base = 0; base < nrows; base += 64) {
+    const int r = base + lane;
+    int len = 0, l0 = 0;
+    if (r < nrows) {
+      const int iz = 1 + r / s.ey, iy = 1 + r % s.ey;
+      const int h0 = (iz * s.hy + iy) * s.hx + 1;
+      const int h1 = h0 + s.ex - 1;
+      l0 = hc[h0].lstart;
+      len = hc[h1].lstart + hc[h1].count - l0;
+    }
+    int v = len;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const int t = __shfl_up(v, off);
+      if (lane >= off) v += t;
+    }
+    const int q0 = carry + v - len;
+    for (int k = 0; k < len && q0 + k < ccap; k++) central_local[q0 + k] = (uint16_t)(l0 + k);
+    carry += __shfl(v, 63);
+  }
+  if (lane == 0) *count_out = carry;
+}
+
+// central atom q from the tables (slot_cell and central_local filled)
+__device__ __forceinline__ bool central_at(const HaloCell* hc, const uint16_t* slot_cell,
+                                           const uint16_t* central_local, int count, int q, Central& out)
+{
+  if (q >= count) return false;
+  const int local = central_local[q];
+  const int h = slot_cell[local];
+  out.h = h;
+  out.local = local;
+  out.global = (int64_t)hc[h].gstart + (local - hc[h].lstart);
+  return true;
+}
+
 __device__ __forceinline__ int64_t lblock()
 {
   const unsigned b = blockIdx.x, g = gridDim.x;
@@ -245,20 +302,23 @@ struct LdsLayout {
   HaloCell* hc;
   uint8_t* ltype;
   int* total;
-  unsigned char* end;  // first byte past the layout, 16-byte aligned
+  uint16_t* slot_cell;      // halo cell of each slot (staging)
+  uint16_t* central_local;  // local slots of the central atoms
 };
 
-// LDS of the block kernels: records, halo table, counters, per-record types
-__device__ __forceinline__ LdsLayout lds_layout(int cap, int nh_max, bool with_type, bool with_w)
+// LDS of the block kernels: records, halo table, counters (total[0] halo atoms,
+// total[2] central atoms), slot cells, central slots, per-record types
+// (block_lds_bytes on the host adds up the same sizes)
+__device__ __forceinline__ LdsLayout lds_layout(int cap, int ccap, int nh_max, bool with_type, bool with_w)
 {
   extern __shared__ __align__(32) unsigned char smem[];
   LdsLayout l;
   const size_t rb = lrec_at(smem, cap, with_w, l.rec);
   l.hc = reinterpret_cast<HaloCell*>(smem + rb);
   l.total = reinterpret_cast<int*>(l.hc + nh_max);
-  l.ltype = with_type ? reinterpret_cast<uint8_t*>(l.total + 4) : nullptr;
-  const size_t used = rb + (size_t)nh_max * sizeof(HaloCell) + 4 * sizeof(int) + (with_type ? (size_t)cap : 0);
-  l.end = smem + (used + 15) / 16 * 16;
+  l.slot_cell = reinterpret_cast<uint16_t*>(l.total + 4);
+  l.central_local = l.slot_cell + cap;
+  l.ltype = with_type ? reinterpret_cast<uint8_t*>(l.central_local + ccap) : nullptr;
   return l;
 }
 
@@ -400,7 +460,7 @@ __global__ void __launch_bounds__(kBuildThreads) k_block_build(BlockGeom g, cons
                                                                const Pos4* __restrict__ pos,
                                                                const BoxDev* __restrict__ box, double rl2,
                                                                uint16_t* __restrict__ nbr16, int* __restrict__ nnb,
-                                                               int nmax, int64_t npad, int cap, int nh_max,
+                                                               int nmax, int64_t npad, int cap, int ccap, int nh_max,
                                                                const unsigned long long* __restrict__ nocull,
                                                                int* __restrict__ brick_rows,
                                                                unsigned long long* __restrict__ flags,
@@ -416,9 +476,13 @@ __global__ void __launch_bounds__(kBuildThreads) k_block_build(BlockGeom g, cons
   struct {
     HaloCell* hc;
     int* total;
+    uint16_t* slot_cell;
+    uint16_t* central_local;
   } l;
   l.hc = reinterpret_cast<HaloCell*>(smem + build_rec_bytes(cap));
   l.total = reinterpret_cast<int*>(l.hc + nh_max);
+  l.slot_cell = reinterpret_cast<uint16_t*>(l.total + 4);
+  l.central_local = l.slot_cell + cap;
   const BlockShape s = block_shape(g, b);
   const int total = build_halo(g, s, cell_off, l.hc, l.total);
   if (total > cap) {  // cannot happen when cap = measured maximum; never write past LDS
@@ -428,10 +492,17 @@ __global__ void __launch_bounds__(kBuildThreads) k_block_build(BlockGeom g, cons
   const BoxDev bx = box[s.r];
   const bool cull = *nocull == 0ull;
   if (threadIdx.x == 0) l.total[1] = 0;  // longest list of the brick (made visible by stage's barrier)
-  stage_each<false>(s, l.hc, total, pos, bx.L, [&](int c, const Pos4& p, int) { P[c] = double3{p.x, p.y, p.z}; });
+  central_table(s, l.hc, l.central_local, ccap, l.total + 2);
+  stage_each<false>(s, l.hc, total, pos, bx.L, l.slot_cell,
+                    [&](int c, const Pos4& p, int) { P[c] = double3{p.x, p.y, p.z}; });
+  const int ncentral = l.total[2];
+  if (ncentral > ccap) {  // cannot happen when ccap = measured maximum
+    if (threadIdx.x == 0) atomicMax(flags + 1, (unsigned long long)ncentral);
+    return;
+  }
   for (int q = threadIdx.x;; q += blockDim.x) {
     Central a;
-    if (!central_atom(s, l.hc, q, a)) break;
+    if (!central_at(l.hc, l.slot_cell, l.central_local, ncentral, q, a)) break;
     const double xi = P[a.local].x, yi = P[a.local].y, zi = P[a.local].z;
     const Pos4 pi = {xi, yi, zi, 0.0};
     const int ix = a.h % s.hx, iy = (a.h / s.hx) % s.hy, iz = a.h / (s.hx * s.hy);
@@ -586,7 +657,7 @@ __global__ void __launch_bounds__(256) k_block_to_global(BlockGeom g, const int*
 {
   const int64_t b = lblock();
   if (b >= g.nblocks) return;
-  LdsLayout l = lds_layout(0, nh_max, false, false);
+  LdsLayout l = lds_layout(0, 0, nh_max, false, false);
   const BlockShape s = block_shape(g, b);
   const int nh = s.hx * s.hy * s.hz;
   build_halo(g, s, cell_off, l.hc, l.total);
@@ -661,7 +732,8 @@ struct BlockArgs {
   const uint16_t* __restrict__ nbr16;
   const int* __restrict__ nnb;
   int64_t npad;
-  int cap;
+  int cap;   // halo slots (LDS records)
+  int ccap;  // central atoms per brick at most
   double rc2;
 };
 
@@ -726,15 +798,18 @@ __global__ void __launch_bounds__(kPassThreads, kPassWaves<false>) k_block_eam_r
 {
   const int64_t b = lblock();
   if (b >= a.g.nblocks) return;
-  LdsLayout l = lds_layout(a.cap, a.g.nh_max, MULTI, false);
+  LdsLayout l = lds_layout(a.cap, a.ccap, a.g.nh_max, MULTI, false);
   const BlockShape s = block_shape(a.g, b);
   const int total = build_halo(a.g, s, a.cell_off, l.hc, l.total);
   if (total > a.cap) return;  // excluded by the host (cap = measured maximum)
-  stage(s, l.hc, total, a.pos, a.box[s.r].L, a.type, l.rec, l.ltype);
+  central_table(s, l.hc, l.central_local, a.ccap, l.total + 2);
+  stage(s, l.hc, total, a.pos, a.box[s.r].L, a.type, l.rec, l.ltype, l.slot_cell);
+  const int ncentral = l.total[2];
+  if (ncentral > a.ccap) return;  // excluded by the host (ccap = measured maximum)
   const int half = pass_half();
   for (int q = pass_slot();; q += kPassAtoms) {
     Central ca;
-    if (!central_atom(s, l.hc, q, ca)) break;
+    if (!central_at(l.hc, l.slot_cell, l.central_local, ncentral, q, ca)) break;
     const Pos4 pi = l.rec.get(ca.local);
     const int ti = MULTI ? l.ltype[ca.local] : 0;
     const int cnt = a.nnb[ca.global];
@@ -784,11 +859,11 @@ __global__ void __launch_bounds__(kPassThreads, kPassWaves<VIRIAL>) k_block_eam_
   static_assert(!DRIFT || (FUSED && !ENERGY && !VIRIAL), "the drift epilogue is the plain MD-loop step");
   const int64_t b = lblock();
   if (b >= a.g.nblocks) return;
-  LdsLayout l = lds_layout(a.cap, a.g.nh_max, MULTI, true);
+  LdsLayout l = lds_layout(a.cap, a.ccap, a.g.nh_max, MULTI, true);
   const BlockShape s = block_shape(a.g, b);
   const int total = build_halo(a.g, s, a.cell_off, l.hc, l.total);
   if (total > a.cap) return;  // excluded by the host (cap = measured maximum)
-  stage(s, l.hc, total, a.pos, a.box[s.r].L, a.type, l.rec, l.ltype);
+  stage(s, l.hc, total, a.pos, a.box[s.r].L, a.type, l.rec, l.ltype, l.slot_cell);
   const int half = pass_half();
   for (int q = pass_slot();; q += kPassAtoms) {
     Central ca;
@@ -854,15 +929,18 @@ __global__ void __launch_bounds__(kPassThreads, kPassWaves<VIRIAL>) k_block_lj(B
   static_assert(!DRIFT || (FUSED && !ENERGY && !VIRIAL), "the drift epilogue is the plain MD-loop step");
   const int64_t b = lblock();
   if (b >= a.g.nblocks) return;
-  LdsLayout l = lds_layout(a.cap, a.g.nh_max, false, false);
+  LdsLayout l = lds_layout(a.cap, a.ccap, a.g.nh_max, false, false);
   const BlockShape s = block_shape(a.g, b);
   const int total = build_halo(a.g, s, a.cell_off, l.hc, l.total);
   if (total > a.cap) return;  // excluded by the host (cap = measured maximum)
-  stage(s, l.hc, total, a.pos, a.box[s.r].L, nullptr, l.rec, nullptr);
+  central_table(s, l.hc, l.central_local, a.ccap, l.total + 2);
+  stage(s, l.hc, total, a.pos, a.box[s.r].L, nullptr, l.rec, nullptr, l.slot_cell);
+  const int ncentral = l.total[2];
+  if (ncentral > a.ccap) return;  // excluded by the host (ccap = measured maximum)
   const int half = pass_half();
   for (int q = pass_slot();; q += kPassAtoms) {
     Central ca;
-    if (!central_atom(s, l.hc, q, ca)) break;
+    if (!central_at(l.hc, l.slot_cell, l.central_local, ncentral, q, ca)) break;
     const Pos4 pi = l.rec.get(ca.local);
     const int cnt = a.nnb[ca.global];
     double fx = 0.0, fy = 0.0, fz = 0.0, e = 0.0;
@@ -945,6 +1023,7 @@ void block_eam_dispatch(Ctx& c, double rcut, bool do_rho, bool do_emb, bool do_f
   a.nnb = c.nnb.p;
   a.npad = c.npad;
   a.cap = c.bcap;
+  a.ccap = c.bcentral;
   a.rc2 = rcut * rcut;
   const size_t lds_rho = block_lds_bytes(c, MULTI, false);
   const size_t lds = block_lds_bytes(c, MULTI, true);
@@ -1075,7 +1154,7 @@ void block_setup(Ctx& c)
 size_t block_lds_bytes(const Ctx& c, bool with_type, bool with_w)
 {
   return (size_t)c.bcap * sizeof(double) * (with_w ? 4 : 3) + (size_t)c.bgeom.nh_max * sizeof(HaloCell) +
-         4 * sizeof(int) + (with_type ? (size_t)c.bcap : 0);
+         4 * sizeof(int) + (size_t)(c.bcap + c.bcentral) * sizeof(uint16_t) + (with_type ? (size_t)c.bcap : 0);
 }
 
 bool block_feasible(const Ctx& c)
@@ -1093,7 +1172,8 @@ unsigned block_grid(const Ctx& c)
 // (2^bits bins uniform in d^2 over [0, rl^2)), k_list_sort orders by it.
 void block_build(Ctx& c, double rl2)
 {
-  const size_t lds = build_rec_bytes(c.bcap) + (size_t)c.bgeom.nh_max * sizeof(HaloCell) + 4 * sizeof(int);
+  const size_t lds = build_rec_bytes(c.bcap) + (size_t)c.bgeom.nh_max * sizeof(HaloCell) + 4 * sizeof(int) +
+                     (size_t)(c.bcap + c.bcentral) * sizeof(uint16_t);
   const bool sorted = !c.list_sort_disabled;
   const int bits = std::max(1, std::min(c.sort_bits, 8));
   const float bin_scale = (float)((double)(1 << bits) / rl2);
@@ -1105,14 +1185,14 @@ void block_build(Ctx& c, double rl2)
     XS_HIP(hipMemsetAsync(c.dscalar.p, 0, 2 * sizeof(unsigned long long), c.stream));
     auto k = sorted ? k_block_build<true> : k_block_build<false>;
     k<<<block_grid(c), kBuildThreads, lds, c.stream>>>(c.bgeom, c.cell_off.p, c.pos.p, c.box_d.p, rl2, c.nbr16.p,
-                                                       c.nnb.p, c.nmax, c.npad, c.bcap, c.bgeom.nh_max,
+                                                       c.nnb.p, c.nmax, c.npad, c.bcap, c.bcentral, c.bgeom.nh_max,
                                                        c.dscalar.p + 4, c.brick_rows.p, c.dscalar.p,
                                                        sorted ? c.bins8.p : nullptr, bin_scale, (1 << bits) - 1);
     XS_HIP(hipGetLastError());
     XS_HIP(hipMemcpyAsync(c.hscalar, c.dscalar.p, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
                           c.stream));
     XS_HIP(hipStreamSynchronize(c.stream));
-    if (c.hscalar[1] != 0) throw Error(XS_E_STATE, "block halo exceeds its measured capacity");
+    if (c.hscalar[1] != 0) throw Error(XS_E_STATE, "block halo or central atoms exceed their measured capacity");
     const unsigned long long need = c.hscalar[0];
     if (need == 0) {
       if (sorted && c.n > 0) {
@@ -1184,6 +1264,7 @@ void launch_lj_block(Ctx& c, double eps, double sigma, double rcut, unsigned fla
   a.nnb = c.nnb.p;
   a.npad = c.npad;
   a.cap = c.bcap;
+  a.ccap = c.bcentral;
   a.rc2 = rcut * rcut;
   const size_t lds = block_lds_bytes(c, false, false);
   const unsigned g = block_grid(c);
