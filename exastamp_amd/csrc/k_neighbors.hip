@@ -202,6 +202,8 @@ void build_neighbor_list(Ctx& c, double rcut, double skin, bool prefer_block)
   c.half_valid = false;
   if (n == 0) {
     c.list_valid = true;
+    c.list_rcut = rcut;
+    c.list_skin = skin;
     c.list_kind = ListKind::Global;
     c.global_from_block = false;
     return;
