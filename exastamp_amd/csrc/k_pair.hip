@@ -12,8 +12,9 @@
 // cutoff anyway, so computing the out-of-range entries costs nothing extra, and
 // the batch keeps several gathers in flight per wavefront.
 //
-// EAM single-element systems with a cell-block list use k_block.hip instead;
-// these kernels serve LJ, alloys, replica batches and the global-list path.
+// With a cell-block list (the default) the LJ and EAM operators run k_block.hip
+// instead; these kernels serve the global list (XSMD_LIST=global, or no brick that
+// fits in LDS) and the Newton-symmetric operators.
 #include "pair_math.hpp"
 
 namespace xsmd {
