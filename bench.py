@@ -54,6 +54,8 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-symmetric", action="store_true", help="skip the full-list vs Newton-symmetric operator timing")
+    ap.add_argument("--no-lj", action="store_true", help="skip the LJ argon leg")
+    ap.add_argument("--lj-cells", type=int, default=64, help="FCC cells per edge of the LJ leg (64 -> 1.05M atoms)")
     ap.add_argument("--stream-copy-gib", type=float, default=0.0,
                     help="also time a plain HBM copy of this many GiB (the counter calibration kernel)")
     ap.add_argument("--seed", type=int, default=2024)
@@ -242,6 +244,43 @@ def adjoint_leg(name, cfg, masses, names, tab, nfd, repeats, threads):
     }
 
 
+def lj_leg(cells, steps, warmup):
+    """LJ argon (the C1 parameters: a = 5.26 A, eps = 0.0104 eV, sigma = 3.4 A, rc = 8.5 A,
+    skin 0.3 A, 100 K, dt = 2 fs) at cells^3 FCC cells: NVE atom-steps/s of both forms."""
+    from exastamp_amd import _lib, inputs
+    from exastamp_amd.operators import Simulation
+    from exastamp_amd.units import EV
+    import ctypes as C
+    eps, sig, rc, skin, dt = 0.0104 * EV, 3.4, 8.5, 0.3, 0.002
+    cfg = inputs.lattice_config((cells, cells, cells), 5.26, [39.948], T=100.0, seed=7)
+    sim = Simulation(0)
+    sim.species([("Ar", 39.948)])
+    out = {"workload": f"LJ Ar FCC {cells}^3 cells = {cfg.n} atoms, rc 8.5 A, skin 0.3 A, NVE dt 2 fs",
+           "atoms": cfg.n, "steps": steps}
+    for name, mixed in (("mixed", True), ("fp64", False)):
+        sim.set_config(cfg)
+        sim.lj_parameters(parameters={"epsilon": eps, "sigma": sig}, rcut=rc)
+        sim.md_run("lj", warmup, dt, rc, skin, mixed=mixed)
+        sim.ctx.xs_profile_enable(1)
+        sim.ctx.xs_synchronize()
+        t0 = time.perf_counter()
+        st = _lib.RunStats()
+        flags = (_lib.XS_MIXED if mixed else 0) | _lib.XS_CONTINUE
+        sim.ctx.xs_md_run(_lib.XS_POT_LJ, steps, dt, rc, skin, flags, 0, None, C.byref(st))
+        sim.ctx.xs_synchronize()
+        sec = time.perf_counter() - t0
+        ms = np.zeros(len(_lib.PROF_CLASSES))
+        cnt = np.zeros(len(_lib.PROF_CLASSES), dtype=np.int64)
+        sim.ctx.xs_profile_get(_lib.dptr(ms), _lib.ptr(cnt, C.c_int64))
+        sim.ctx.xs_profile_enable(0)
+        k = {c: i for i, c in enumerate(_lib.PROF_CLASSES)}
+        out[name] = {"value": cfg.n * steps / sec, "rebuilds": int(st.rebuilds),
+                     "lj_ms_avg": float(ms[k["lj"]] / max(cnt[k["lj"]], 1))}
+    out["list_kind"] = sim.list_info()["kind"]
+    sim.close()
+    return out
+
+
 def cpu_baseline(cfg, tab, steps, threads, dt, rc, skin):
     """The FP64 oracle (OpenMP, full list, the reference's CPU structure) on the same
     system for a bounded number of steps."""
@@ -375,6 +414,12 @@ def main():
         cpu = cpu_baseline(cfg, tab, args.cpu_steps, args.cpu_threads, dt, rc, skin)
         log(f"cpu: {cpu['value']:.4g} atom-steps/s on {cpu['cores']} threads")
 
+    lj = None
+    if not args.no_lj:
+        log(f"LJ argon leg: {args.lj_cells}^3 FCC cells, 200 steps per form")
+        lj = lj_leg(args.lj_cells, 200, 10)
+        log(f"LJ: mixed {lj['mixed']['value']:.4g}, FP64 {lj['fp64']['value']:.4g} atom-steps/s")
+
     adjoint = None
     if not args.no_adjoint:
         from exastamp_amd import inputs as inp
@@ -425,6 +470,7 @@ def main():
         "kernels_mixed": mixed["kernels"],
         "kernels_fp64": None if fp64 is None else fp64["kernels"],
         "newton_symmetric": sym,
+        "lj": lj,
         "roofline": roof,
         "stream_copy": stream,
         "cpu_baseline": cpu,
