@@ -5,7 +5,6 @@ FP64 CPU oracle (oracle/); `-m "not gpu"` tests pin the oracle itself and check
 the host-side logic and the library's exports without touching a GPU.
 """
 import functools
-import os
 import sys
 from pathlib import Path
 

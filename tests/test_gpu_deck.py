@@ -16,7 +16,6 @@ tests/golden/decks) run through the deck runner on the GPU.
   kinetic energy within 3 % of the recorded 1.7675e-3 eV/atom (the 50 m/s velocity
   noise), particle count exact.
 """
-import numpy as np
 import pytest
 
 from conftest import GOLDEN

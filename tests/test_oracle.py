@@ -6,7 +6,6 @@ reference ships; the AlCu run the reference recorded), closed forms, brute force
 and invariants (energy/force consistency, strain/virial consistency, momentum
 conservation, time reversibility).
 """
-import math
 
 import numpy as np
 import pytest
