@@ -23,6 +23,12 @@ the whole periodic box, images are resolved inside the library) are accepted as
 no-ops; any other operator name raises ``DeckError``, as exaStamp fails on an
 unknown operator.
 
+The particle-update graph of config_move_particles.msp (trigger_move_particles,
+update_particles_full/fast, parallel_update_particles, update_particle_neighbors) is
+restated node for node, with particle_displ_over (and its async form of
+config_speculative_verlet.msp) as the trigger and chunk_neighbors as the list
+build, so decks that override any of these nodes run as written.
+
 Differences that a deck cannot see in its results: the Newton-symmetric variants
 (``eam_symmetry: true``, ``lj_compute_force_symetric``) evaluate each pair once
 from a half list and add the partner's share with FP64 atomics (same forces up to
@@ -92,6 +98,44 @@ verlet_nve:
 chunk_neighbors:
   config:
     chunk_size: 1
+chunk_neighbors_impl: chunk_neighbors
+update_particle_neighbors:
+  - amr_grid_pairs
+  - chunk_neighbors_impl
+  - resize_particle_locks
+parallel_update_particles:
+  - migrate_cell_particles
+  - rebuild_amr
+  - backup_r
+  - ghost_full_update
+  - grid_post_processing
+  - update_particle_neighbors
+update_particles_full_body:
+  - move_particles
+  - trigger_load_balance
+  - load_balancing_if_triggered
+  - parallel_update_particles
+update_particles_full:
+  condition: trigger_move_particles
+  body:
+    - update_particles_full_body
+update_particles_fast_body:
+  - ghost_update_r
+update_particles_fast:
+  condition: not trigger_move_particles
+  body:
+    - update_particles_fast_body
+trigger_move_particles:
+  rebind: { threshold: max_displ , result: trigger_move_particles }
+  body:
+    - particle_displ_over
+check_and_update_particles:
+  - trigger_move_particles
+  - update_particles_full
+  - update_particles_fast
+  - loadbalance_log_helper:
+      rebind: { lb_flag: trigger_load_balance , move_flag: trigger_move_particles }
+      body: [ lb_event_counter ]
 begin_iteration:
   - trigger_thermostate_screen
   - trigger_thermostate_file
@@ -146,7 +190,16 @@ NOOPS = {
     "write_restart_if_triggered", "perform_analysis_if_triggered", "write_snapshot_if_triggered",
     "write_final_restart", "write_restart", "ghost_update_r", "ghost_update_opt", "ghost_update_all_no_fv",
     "update_force_energy_from_ghost", "ghost_config", "ghost_config_opt", "configuration", "preinit_rcut_max",
-    "nbh_dist", "xxx_simulation_epilog",
+    "xxx_simulation_epilog",
+    # particle moves on one GPU: migration, load balancing, AMR sub-cells, ghost layers and
+    # position backups have nothing to do (the list build keeps the reference positions)
+    "move_particles", "extend_domain", "load_balance", "trigger_load_balance", "load_balancing_if_triggered",
+    "migrate_cell_particles", "rebuild_amr", "backup_r", "ghost_full_update", "ghost_comm_scheme",
+    "profile_ghost_comm_scheme", "ghost_update_all_impl", "grid_memory_compact", "amr_grid_pairs",
+    "resize_particle_locks", "compute_primary_neighbors", "compute_dual_neighbors", "lb_event_counter",
+    # a consistency check of exaNBody's chunk streams; the lists here are checked by the tests
+    "verify_chunk_neighbors",
+    "ghost_update_emb", "update_opt_from_ghost", "update_rho_from_ghost",
 }
 
 
@@ -371,6 +424,7 @@ class Runner:
         self._file = None
         self.rebuilds = 0
         self.rebuilds_printed = 0
+        self.displ_pending = None
 
     # --- slots with rebind ----------------------------------------------------
     def _slot_name(self, name):
@@ -422,6 +476,8 @@ class Runner:
             return self.node(item)
         if isinstance(item, dict) and len(item) == 1:
             (k, v), = item.items()
+            if isinstance(v, dict) and "body" in v:  # a block defined in place
+                return self.block(k, v)
             return self.node(k, v if isinstance(v, dict) else ({} if v is None else {"value": v}))
         raise DeckError(f"cannot read batch item {item!r}")
 
@@ -433,17 +489,26 @@ class Runner:
                 cond = d.get("condition")
                 if cond is None:
                     raise DeckError(f"loop {name} has no condition")
-                while bool(self.get(cond, True)):
+                while self.condition(cond):
                     for item in body:
                         self.item(item)
             else:
-                if d.get("condition") is not None and not bool(self.get(d["condition"], True)):
+                if d.get("condition") is not None and not self.condition(d["condition"]):
                     return None
                 for item in body:
                     self.item(item)
         finally:
             self.rebind.pop()
         return None
+
+    def condition(self, cond) -> bool:
+        """A block condition: a slot name, or "not" and a slot name."""
+        if isinstance(cond, bool):
+            return cond
+        words = str(cond).split()
+        if len(words) == 2 and words[0] == "not":
+            return not bool(self.get(words[1], False))
+        return bool(self.get(str(cond), True))
 
     # --- helpers ----------------------------------------------------------------
     def path(self, name: str) -> Path:
@@ -588,6 +653,7 @@ def _op_init_rcut_max(r: Runner, p):
         r.node("compute_force")
     finally:
         r.dry = False
+    _op_nbh_dist(r, {})
 
 
 def _op_init_particles(r: Runner, p):
@@ -599,7 +665,7 @@ def _op_init_particles(r: Runner, p):
     r.init_atoms = copy.deepcopy(r.atoms)
     if r.tables_loaded is not None:
         r.sim.eam_alloy_init(r.tables_loaded)
-    _build_list(r)
+    r.node("update_particle_neighbors")
 
 
 def _build_list(r: Runner):
@@ -615,10 +681,29 @@ def _op_chunk_neighbors(r: Runner, p):
     _build_list(r)
 
 
-def _op_check_and_update_particles(r: Runner, p):
-    # particle_displ_over with max_displ = rcut_inc / 2 (config_move_particles.msp:19-23,116-118)
-    if r.sim.particle_displ_over(0.5 * parse_quantity(r.get("rcut_inc"))):
-        _build_list(r)
+def _op_nbh_dist(r: Runner, p):
+    # the move tolerance: max_displ = rcut_inc / 2 (config_move_particles.msp:116-118)
+    r.set("max_displ", 0.5 * parse_quantity(r.get("rcut_inc")))
+
+
+def _op_particle_displ_over(r: Runner, p):
+    """particle_displ_over: result = some atom moved more than threshold since the last
+    list build; with async: true the result is published by particle_displ_over_async_end."""
+    th = r.get("threshold", None)
+    if th is None:
+        th = 0.5 * parse_quantity(r.get("rcut_inc"))
+    over = r.sim.particle_displ_over(parse_quantity(th))
+    if p.get("async", False):
+        r.displ_pending = over
+    else:
+        r.set("result", over)
+
+
+def _op_particle_displ_over_async_end(r: Runner, p):
+    if r.displ_pending is None:
+        raise DeckError("particle_displ_over_async_end without a started particle_displ_over")
+    r.set("result", r.displ_pending)
+    r.displ_pending = None
 
 
 def _publish_rcut(r: Runner, rc):
@@ -779,7 +864,9 @@ _OPS = {
     "init_rcut_max": _op_init_rcut_max,
     "init_particles": _op_init_particles,
     "chunk_neighbors": _op_chunk_neighbors,
-    "check_and_update_particles": _op_check_and_update_particles,
+    "nbh_dist": _op_nbh_dist,
+    "particle_displ_over": _op_particle_displ_over,
+    "particle_displ_over_async_end": _op_particle_displ_over_async_end,
     "lj_compute_force": _op_lj,
     "lj_compute_force_symetric": _op_lj_symetric,
     "eam_alloy_init": _op_eam_alloy_init,

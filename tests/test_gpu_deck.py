@@ -99,3 +99,74 @@ def test_eam_alcu_deck_matches_recorded_run():
     # one NVE step conserves the total energy per atom to well below the recorded
     # run's own change (1.4e-8 eV)
     assert abs(recs[1].total_ev - first.total_ev) <= 1e-7
+
+
+def _every(path, steps, sets=()):
+    """Run a deck for `steps` steps with a thermodynamic state every step."""
+    from exastamp_amd import deck
+    cfg = deck.load_deck(path, [GOLDEN], [(["global", "max_iteration"], str(steps)),
+                                          (["global", "simulation_thermostate_screen_frequency"], "1")]
+                         + list(sets))
+    r = deck.Runner(cfg, out=open("/dev/null", "w"))
+    return r, r.run()
+
+
+def _same_states(a, b, tol):
+    assert len(a) == len(b)
+    for x, y in zip(a, b):
+        assert x.step == y.step and x.particles == y.particles
+        assert _close(x.total_ev, y.total_ev, tol) and _close(x.potential_ev, y.potential_ev, tol)
+
+
+@pytest.mark.parametrize("base", ["single_specy_nosym", "single_specy_sym"])
+def test_lj_speculative_verlet_decks_follow_the_standard_scheme(base):
+    """config_speculative_verlet.msp computes forces on the old list first and redoes
+    them after a rebuild when an atom moved too far: the same trajectory, the same
+    rebuild steps as the standard scheme (bit for bit with full lists; the symmetric
+    operator's atomics reorder the last bits)."""
+    skin = [(["global", "rcut_inc"], "0.05 ang")]   # a thin skin: several rebuilds in 30 steps
+    ra, a = _every(DECKS / "lj" / f"{base}.msp", 30, skin)
+    rb, b = _every(DECKS / "lj" / f"{base}_speculative_verlet.msp", 30, skin)
+    assert ra.rebuilds >= 3 and rb.rebuilds == ra.rebuilds
+    if base.endswith("nosym"):
+        assert [x.line() for x in a] == [x.line() for x in b]
+    else:
+        _same_states(a, b, 1e-12)
+
+
+def test_lj_graph_dot_and_verify_chunk_decks():
+    """The graph-dump deck runs like its base deck (the debug configuration has no
+    effect here); the chunk-verification deck (its own neighbour update graph, chunk
+    size 4, 200 m/s velocity noise) conserves the total energy."""
+    _, a = _every(DECKS / "lj" / "single_specy_nosym.msp", 10)
+    _, b = _every(DECKS / "lj" / "single_specy_nosym_sim_graph_dot.msp", 10)
+    assert [x.line() for x in a] == [x.line() for x in b]
+    r, c = _every(DECKS / "lj" / "single_specy_nosym_verify_chunk.msp", 20)
+    assert r.rebuilds >= 1 and c[0].particles == 2048
+    assert abs(c[-1].total_ev - c[0].total_ev) <= 1e-5 * abs(c[0].total_ev)
+
+
+@pytest.mark.parametrize("base,variants", [
+    ("multi_species_nosym_cs1", ["multi_species_nosym_cs4", "multi_species_sym_cs1", "multi_species_sym_cs4"]),
+    ("single_specy_nosym_cs1", ["single_specy_nosym_cs4", "single_specy_singlepass_cs1", "single_specy_sym_cs4"]),
+    ("multi_species_singlepass_cs1", ["multi_species_singlepass_cs4"]),
+])
+def test_eam_alloy_deck_variants_agree(base, variants):
+    """The EAM alloy regression decks that differ in chunk size, single-pass form or
+    Newton symmetry give the same states (FP64 summation order for the symmetric ones)."""
+    _, ref = _every(DECKS / "eam_alloy" / f"{base}.msp", 5)
+    for v in variants:
+        _, recs = _every(DECKS / "eam_alloy" / f"{v}.msp", 5)
+        _same_states(recs, ref, 1e-11 if "sym" in v and "nosym" not in v else 1e-12)
+
+
+def test_eam_benchmark_Al_Cu_deck_forms():
+    """benchmark_Al_Cu.msp in its single-pass form and in the two alternatives it names
+    (split passes, Newton-symmetric), on the AlCu lattice of multi_species_nosym_cs1."""
+    _, single = _every(DECKS / "eam_alloy" / "benchmark_Al_Cu.msp", 5)
+    _, split = _every(DECKS / "eam_alloy" / "benchmark_Al_Cu.msp", 5, [(["compute_force"], "compute_force_nosym")])
+    _, sym = _every(DECKS / "eam_alloy" / "benchmark_Al_Cu.msp", 5, [(["compute_force"], "compute_force_sym")])
+    _, ref = _every(DECKS / "eam_alloy" / "multi_species_nosym_cs1.msp", 5)
+    _same_states(single, ref, 1e-12)
+    _same_states(split, ref, 1e-12)
+    _same_states(sym, ref, 1e-11)
