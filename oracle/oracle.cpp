@@ -221,6 +221,9 @@ struct or_ctx {
 
   bool has_lj = false;
   double lj_eps = 0, lj_sigma = 0, lj_rcut = 0, lj_ecut = 0;
+  // lj_multi_force: per type pair (packed triangle) epsilon, sigma, rcut, ecut
+  int ljm_ntypes = 0;
+  std::vector<double> ljm_eps, ljm_sigma, ljm_rcut, ljm_ecut;
   bool has_eam = false;
   Eam eam;
 
@@ -418,6 +421,25 @@ int or_set_lj(or_ctx* c, double epsilon, double sigma, double rcut)
   return 0;
 }
 
+// lj_multi_force set-up: the per-type-pair parameter table of
+// pair_potential_impl.hxx:282-320 (each pair's energy_cutoff at its own rcut).
+int or_set_lj_multi(or_ctx* c, int ntypes, const double* epsilon, const double* sigma, const double* rcut)
+{
+  if (ntypes < 1 || ntypes > 7) return -1;
+  const int np = ntypes * (ntypes + 1) / 2;
+  c->ljm_ntypes = ntypes;
+  c->ljm_eps.assign(epsilon, epsilon + np);
+  c->ljm_sigma.assign(sigma, sigma + np);
+  c->ljm_rcut.assign(rcut, rcut + np);
+  c->ljm_ecut.assign(np, 0.0);
+  for (int k = 0; k < np; k++) {
+    double e = 0.0, de = 0.0;
+    if (rcut[k] > 0.0) lj_compute_energy(epsilon[k], sigma[k], rcut[k], e, de);
+    c->ljm_ecut[k] = e;
+  }
+  return 0;
+}
+
 // Table set-up as the tail of eam_alloy.cpp:112-201 (rdr, rdrho, rhomax, splines)
 int or_set_eam(or_ctx* c, int nel, int nrho, double drho, int nr, double dr, double rcut,
                const double* frho, const double* rhor, const double* z2r)
@@ -595,6 +617,44 @@ void lj_force(or_ctx* c, int flags)
   });
 }
 
+// lj_multi_force: PairMultiForceOp (pair_potential_force_op_multiparam.h:75-113) over
+// the full list: the pair's own parameters, r <= its rcut, e shifted by its ecut.
+void lj_multi_force(or_ctx* c, int flags)
+{
+  for_each_cell(*c, [&](int64_t i) {
+    double _ep = 0., _fx = 0., _fy = 0., _fz = 0.;
+    double _vir[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    const int ta = c->type[i];
+    for (int64_t k = c->nb_off[i]; k < c->nb_off[i + 1]; k++) {
+      const int64_t j = xsmd_nbr_index(c->nb[k]);
+      const int tb = c->type[j];
+      const int lo = ta < tb ? ta : tb, hi = ta < tb ? tb : ta;
+      const int pid = hi * (hi + 1) / 2 + lo;
+      double dr[3];
+      c->pair_dr(i, c->nb[k], dr);
+      const double d2 = xsmd_d2(dr[0], dr[1], dr[2]);
+      const double r = std::sqrt(d2);
+      if (!(r <= c->ljm_rcut[pid])) continue;
+      double e = 0.0, de = 0.0;
+      lj_compute_energy(c->ljm_eps[pid], c->ljm_sigma[pid], r, e, de);
+      e -= c->ljm_ecut[pid];
+      de /= r;
+      const double fe[3] = {de * dr[0], de * dr[1], de * dr[2]};
+      _fx += fe[0];
+      _fy += fe[1];
+      _fz += fe[2];
+      _ep += .5 * e;
+      add_tensor_scaled(_vir, fe, dr, -0.5);
+    }
+    c->ax[i] += _fx;
+    c->ay[i] += _fy;
+    c->az[i] += _fz;
+    if (flags & OR_ENERGY) c->ep[i] += _ep;
+    if (flags & OR_VIRIAL)
+      for (int q = 0; q < 9; q++) c->vir[9 * (size_t)i + q] += _vir[q];
+  });
+}
+
 // EAM pass 1: parallel_memset + SymRhoOp<false>, eam_potential_multimat.cu:186-203,
 // eam_force_op_multimat.h:114-172
 void eam_rho_pass(or_ctx* c)
@@ -692,6 +752,9 @@ int or_compute_forces(or_ctx* c, int potential, int flags)
   if (potential == OR_POT_LJ) {
     if (!c->has_lj) return -2;
     lj_force(c, flags);
+  } else if (potential == OR_POT_LJ_MULTI) {
+    if (c->ljm_ntypes < 1) return -2;
+    lj_multi_force(c, flags);
   } else {
     if (!c->has_eam) return -2;
     eam_rho_pass(c);

@@ -25,7 +25,7 @@ extern "C" {
 
 typedef struct or_ctx or_ctx;
 
-enum { OR_POT_LJ = 0, OR_POT_EAM = 1 };
+enum { OR_POT_LJ = 0, OR_POT_EAM = 1, OR_POT_LJ_MULTI = 2 };
 enum { OR_ENERGY = 1, OR_VIRIAL = 2 };
 
 /* --- spline tables (eam_alloy.cpp:29-58) --------------------------------- */
@@ -50,6 +50,9 @@ int or_set_system(or_ctx* c, int64_t n, const double* x, const double* y, const 
                   int nspecies, const double* masses);
 
 int or_set_lj(or_ctx* c, double epsilon, double sigma, double rcut);
+/* lj_multi_force parameters: one (epsilon, sigma, rcut) per unordered pair of the
+ * ntypes species, entry b*(b+1)/2 + a for types a <= b. */
+int or_set_lj_multi(or_ctx* c, int ntypes, const double* epsilon, const double* sigma, const double* rcut);
 
 /* EAM tables from knot values (setfl layout), splined by or_interpolate. */
 int or_set_eam(or_ctx* c, int nel, int nrho, double drho, int nr, double dr, double rcut,

@@ -11,7 +11,7 @@ from pathlib import Path
 import numpy as np
 
 LIB_PATH = Path(__file__).resolve().parent / "liboracle.so"
-OR_POT_LJ, OR_POT_EAM = 0, 1
+OR_POT_LJ, OR_POT_EAM, OR_POT_LJ_MULTI = 0, 1, 2
 OR_ENERGY, OR_VIRIAL = 1, 2
 
 _D = C.POINTER(C.c_double)
@@ -37,6 +37,7 @@ def load():
             "or_set_system": (C.c_int, [C.c_void_p, C.c_int64, _D, _D, _D, _D, _D, _D, C.POINTER(C.c_uint8),
                                         _D, _D, C.POINTER(C.c_int), C.c_int, _D]),
             "or_set_lj": (C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_double]),
+            "or_set_lj_multi": (C.c_int, [C.c_void_p, C.c_int, _D, _D, _D]),
             "or_set_eam": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_int, C.c_double, C.c_double,
                                      _D, _D, _D]),
             "or_build_neighbors": (C.c_int, [C.c_void_p, C.c_double, C.c_double]),
@@ -130,6 +131,11 @@ class Oracle:
     def set_lj(self, eps, sigma, rcut):
         self._ok(self.lib.or_set_lj(self.h, eps, sigma, rcut), "set_lj")
 
+    def set_lj_multi(self, ntypes, eps, sigma, rcut):
+        """Per unordered type pair (entry b*(b+1)/2 + a for a <= b) LJ parameters."""
+        e, s, r = (np.ascontiguousarray(x, dtype=np.float64) for x in (eps, sigma, rcut))
+        self._ok(self.lib.or_set_lj_multi(self.h, ntypes, _d(e), _d(s), _d(r)), "set_lj_multi")
+
     def set_eam(self, t):
         fr = np.ascontiguousarray(t.frho, dtype=np.float64)
         rr = np.ascontiguousarray(t.rhor, dtype=np.float64)
@@ -141,7 +147,7 @@ class Oracle:
         self._ok(self.lib.or_build_neighbors(self.h, rcut, skin), "build_neighbors")
 
     def compute_forces(self, potential, flags=OR_ENERGY | OR_VIRIAL):
-        pot = {"lj": OR_POT_LJ, "eam": OR_POT_EAM}.get(potential, potential)
+        pot = {"lj": OR_POT_LJ, "eam": OR_POT_EAM, "lj_multi": OR_POT_LJ_MULTI}.get(potential, potential)
         self._ok(self.lib.or_compute_forces(self.h, pot, flags), "compute_forces")
 
     def eam_pass(self, rho, rho2emb, force, flags=0):
@@ -168,14 +174,14 @@ class Oracle:
         return out
 
     def md_run(self, potential, nsteps, dt, rcut, skin, thermo=True):
-        pot = {"lj": OR_POT_LJ, "eam": OR_POT_EAM}[potential]
+        pot = {"lj": OR_POT_LJ, "eam": OR_POT_EAM, "lj_multi": OR_POT_LJ_MULTI}[potential]
         out = np.zeros((nsteps + 1) * 27) if thermo else None
         nreb = self._ok(self.lib.or_md_run(self.h, pot, nsteps, dt, rcut, skin, _d(out)), "md_run")
         return nreb, (out.reshape(nsteps + 1, 27) if thermo else None)
 
     def md_steps(self, potential, nsteps, dt, rcut, skin):
         """Continue for nsteps (no initial build/force); returns the rebuild count."""
-        pot = {"lj": OR_POT_LJ, "eam": OR_POT_EAM}[potential]
+        pot = {"lj": OR_POT_LJ, "eam": OR_POT_EAM, "lj_multi": OR_POT_LJ_MULTI}[potential]
         rc = self.lib.or_md_steps(self.h, pot, nsteps, dt, rcut, skin, None)
         if rc < 0:
             raise RuntimeError(f"oracle md_steps failed ({rc})")

@@ -376,3 +376,58 @@ def test_setfl_roundtrip_and_reference_reader(tmp_path):
     assert alcu["nel"] == 2 and alcu["nrho"] == 1000 and alcu["nr"] == 3000
     assert alcu["z2r"].shape == (3, 3000)
     assert list(alcu["masses"]) == [26.982, 63.546]
+
+
+# ---------------------------------------------------------------------------
+# lj_multi_force (pair_potential_force_op_multiparam.h, pair_potential_impl.hxx:282-320)
+# ---------------------------------------------------------------------------
+
+def test_lj_multi_with_equal_pairs_is_lj():
+    """Every type pair with the same (epsilon, sigma, rcut): the forces, energies and
+    virials of the single-parameter operator, bit for bit."""
+    import pyoracle
+    from exastamp_amd import inputs
+    cfg = inputs.lattice_config((4, 4, 4), 5.26, [39.948, 39.948], T=100.0, jitter=0.1, seed=3)
+    cfg.type[:] = inputs.rng(4).integers(0, 2, size=cfg.n)
+    eps, sig, rc = 0.0104 * 9648.533212331, 3.4, 8.0
+    a = _oracle(cfg, [39.948, 39.948], lj=(eps, sig), rcut=rc)
+    b = pyoracle.Oracle(threads=4)
+    b.set_config(cfg, [39.948, 39.948])
+    b.set_lj_multi(2, [eps] * 3, [sig] * 3, [rc] * 3)
+    b.build_neighbors(rc, 0.3)
+    b.compute_forces("lj_multi")
+    sa, sb = a.state(), b.state()
+    for k in ("a", "ep", "virial"):
+        assert np.array_equal(sa[k], sb[k])
+
+
+def test_lj_multi_pair_closed_forms():
+    """Two atoms per case, one case per type pair: the pair's own parameters, its own
+    energy shift E(rcut_ab), no force beyond its own rcut (though inside the list
+    radius), none for epsilon 0."""
+    import pyoracle
+    EV = 9648.533212331
+    eps = [0.010 * EV, 0.020 * EV, 0.0]          # pairs (0,0), (0,1), (1,1)
+    sig = [3.0, 3.4, 3.0]
+    rc = [6.0, 7.0, 6.5]
+
+    def lj(e, s, r):
+        return 4 * e * ((s / r) ** 12 - (s / r) ** 6), -24 * e * (2 * (s / r) ** 12 - (s / r) ** 6) / r
+
+    for (ta, tb, pid) in ((0, 0, 0), (0, 1, 1), (1, 1, 2)):
+        for d in (3.7, 5.9, 6.8):
+            cfg = _cfg([[5.0, 10.0, 10.0], [5.0 + d, 10.0, 10.0]], [ta, tb], 30.0)
+            o = pyoracle.Oracle(threads=1)
+            o.set_config(cfg, [40.0, 60.0])
+            o.set_lj_multi(2, eps, sig, rc)
+            o.build_neighbors(7.0, 0.3)
+            o.compute_forces("lj_multi")
+            st = o.state()
+            f = st["a"] * np.array([40.0, 60.0])[[ta, tb]][:, None]   # accelerations -> forces
+            if d <= rc[pid] and eps[pid] != 0.0:
+                e, de = lj(eps[pid], sig[pid], d)
+                ecut, _ = lj(eps[pid], sig[pid], rc[pid])
+                assert st["ep"] == pytest.approx([0.5 * (e - ecut)] * 2, rel=1e-12)
+                assert f[0, 0] == pytest.approx(de, rel=1e-12) and f[1, 0] == pytest.approx(-f[0, 0], rel=1e-15)
+            else:
+                assert np.all(f == 0.0) and np.all(st["ep"] == 0.0)
