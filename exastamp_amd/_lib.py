@@ -74,6 +74,7 @@ SIGNATURES = {
     "xs_force_to_accel": (C.c_int, [_CTX]),
     "xs_lj_compute_force": (C.c_int, [_CTX, C.c_double, C.c_double, C.c_double, C.c_uint]),
     "xs_lj_parameters": (C.c_int, [_CTX, C.c_double, C.c_double, C.c_double]),
+    "xs_lj_multi_force": (C.c_int, [_CTX, C.c_int, _D, _D, _D, C.c_uint]),
     "xs_eam_alloy_tables": (C.c_int, [_CTX, C.c_int, C.c_int, C.c_double, C.c_int, C.c_double, C.c_double,
                                       _D, _D, _D]),
     "xs_eam_alloy_load": (C.c_int, [_CTX, C.c_char_p, _D, C.POINTER(C.c_int)]),

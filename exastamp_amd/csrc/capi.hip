@@ -483,6 +483,48 @@ xs_status xs_lj_parameters(xs_ctx* h, double epsilon, double sigma, double rcut)
   return guarded(h, [&](Ctx& c) { set_lj(c, epsilon, sigma, rcut); });
 }
 
+xs_status xs_lj_multi_force(xs_ctx* h, int ntypes, const double* epsilon, const double* sigma, const double* rcut,
+                            unsigned flags)
+{
+  return guarded(h, [&](Ctx& c) {
+    require(ntypes >= 1 && ntypes <= 7 && ntypes == c.nspecies, XS_E_ARG,
+            "lj_multi_force takes one parameter set per pair of the context's species");
+    require(epsilon && sigma && rcut, XS_E_ARG, "null parameter array");
+    require(!(flags & XS_SYMMETRIC), XS_E_ARG, "lj_multi_force has no Newton-symmetric form");
+    const int np = ntypes * (ntypes + 1) / 2;
+    double rcmax = 0.0;
+    for (int k = 0; k < np; k++) {
+      require(std::isfinite(epsilon[k]) && std::isfinite(sigma[k]) && std::isfinite(rcut[k]) && rcut[k] >= 0.0,
+              XS_E_ARG, "bad lj_multi_force parameters");
+      rcmax = std::max(rcmax, rcut[k]);
+    }
+    require(rcmax > 0.0, XS_E_ARG, "lj_multi_force needs a positive cutoff");
+    require_list(c);
+    require(rcmax <= c.list_rcut + c.list_skin, XS_E_ARG, "rcut exceeds the neighbour-list radius");
+    std::vector<double> d(4 * (size_t)np);
+    std::vector<float> f(4 * (size_t)np);
+    for (int k = 0; k < np; k++) {
+      const double row[4] = {sigma[k] * sigma[k], 4.0 * epsilon[k], 24.0 * epsilon[k],
+                             lj_energy_cutoff(epsilon[k], sigma[k], rcut[k])};
+      for (int q = 0; q < 4; q++) {
+        d[4 * k + q] = row[q];
+        f[4 * k + q] = (float)row[q];
+      }
+    }
+    c.ljm_d.reserve(d.size());
+    c.ljm_f.reserve(f.size());
+    c.ljm_rcut.reserve(np);
+    to_device(c, c.ljm_d, d.data(), (int64_t)d.size());
+    to_device(c, c.ljm_f, f.data(), (int64_t)f.size());
+    to_device(c, c.ljm_rcut, rcut, np);
+    XS_HIP(hipStreamSynchronize(c.stream));  // the host rows go out of scope
+    c.has_ljm = true;
+    c.ljm_ntypes = ntypes;
+    c.ljm_rcmax = rcmax;
+    launch_lj_multi(c, flags);
+  });
+}
+
 xs_status xs_eam_alloy_tables(xs_ctx* h, int nel, int nrho, double drho, int nr, double dr, double rcut,
                               const double* frho, const double* rhor, const double* z2r)
 {

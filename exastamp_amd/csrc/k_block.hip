@@ -922,19 +922,22 @@ __global__ void __launch_bounds__(kPassThreads, kPassWaves<VIRIAL>) k_block_eam_
 // LJ forces (lj_compute_force, pair_potential_force_op_singlemat.h:45-218) on the
 // block list: the staging, list walk and epilogues of k_block_eam_force with the
 // pair math of the global-list kernel (pm::lj_pair).
-template <class Real, bool ENERGY, bool VIRIAL, bool FUSED, bool DRIFT>
+// MULTI (lj_multi_force): types staged with the records; each pair takes the
+// parameters and cutoff of its two species.
+template <class Real, bool ENERGY, bool VIRIAL, bool FUSED, bool DRIFT, bool MULTI = false>
 __global__ void __launch_bounds__(kPassThreads, kPassWaves<VIRIAL>) k_block_lj(BlockArgs a, pm::LJ<Real> lp,
-                                                                           pm::Outputs o, pm::Drift dr)
+                                                                           pm::Outputs o, pm::Drift dr,
+                                                                           pm::LJMultiDev lm = {})
 {
   static_assert(!DRIFT || (FUSED && !ENERGY && !VIRIAL), "the drift epilogue is the plain MD-loop step");
   const int64_t b = lblock();
   if (b >= a.g.nblocks) return;
-  LdsLayout l = lds_layout(a.cap, a.ccap, a.g.nh_max, false, false);
+  LdsLayout l = lds_layout(a.cap, a.ccap, a.g.nh_max, MULTI, false);
   const BlockShape s = block_shape(a.g, b);
   const int total = build_halo(a.g, s, a.cell_off, l.hc, l.total);
   if (total > a.cap) return;  // excluded by the host (cap = measured maximum)
   central_table(s, l.hc, l.central_local, a.ccap, l.total + 2);
-  stage(s, l.hc, total, a.pos, a.box[s.r].L, nullptr, l.rec, nullptr, l.slot_cell);
+  stage(s, l.hc, total, a.pos, a.box[s.r].L, MULTI ? a.type : nullptr, l.rec, l.ltype, l.slot_cell);
   const int ncentral = l.total[2];
   if (ncentral > a.ccap) return;  // excluded by the host (ccap = measured maximum)
   const int half = pass_half();
@@ -942,6 +945,7 @@ __global__ void __launch_bounds__(kPassThreads, kPassWaves<VIRIAL>) k_block_lj(B
     Central ca;
     if (!central_at(l.hc, l.slot_cell, l.central_local, ncentral, q, ca)) break;
     const Pos4 pi = l.rec.get(ca.local);
+    const int ti = MULTI ? l.ltype[ca.local] : 0;
     const int cnt = a.nnb[ca.global];
     double fx = 0.0, fy = 0.0, fz = 0.0, e = 0.0;
     double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
@@ -956,7 +960,13 @@ __global__ void __launch_bounds__(kPassThreads, kPassWaves<VIRIAL>) k_block_lj(B
 #pragma unroll
       for (int u = 0; u < kBBatch; u++) {
         double de_r, eh;
-        pm::lj_pair(lp, nb[u].d2, nb[u].in, de_r, eh);
+        if constexpr (MULTI) {
+          const int pid = pm::type_pair(ti, l.ltype[nb[u].c]);
+          pm::lj_pair(pm::lj_multi_row<Real>(lm, pid), nb[u].d2, nb[u].in && pm::lj_multi_in(lm, pid, nb[u].d2),
+                      de_r, eh);
+        } else {
+          pm::lj_pair(lp, nb[u].d2, nb[u].in, de_r, eh);
+        }
         const double fe0 = nb[u].x * de_r, fe1 = nb[u].y * de_r, fe2 = nb[u].z * de_r;
         fx += fe0;
         fy += fe1;
@@ -1296,6 +1306,43 @@ void launch_lj_block(Ctx& c, double eps, double sigma, double rcut, unsigned fla
   else run(0.0);
   XS_HIP(hipGetLastError());
   if (drift) drift_commit(c);
+}
+
+void launch_lj_multi_block(Ctx& c, unsigned flags)
+{
+  BlockArgs a;
+  a.g = c.bgeom;
+  a.cell_off = c.cell_off.p;
+  a.pos = c.pos.p;
+  a.type = c.type.p;
+  a.box = c.box_d.p;
+  a.nbr16 = c.nbr16.p;
+  a.nnb = c.nnb.p;
+  a.npad = c.npad;
+  a.cap = c.bcap;
+  a.ccap = c.bcentral;
+  a.rc2 = c.ljm_rcmax * c.ljm_rcmax;
+  const size_t lds = block_lds_bytes(c, true, false);
+  const unsigned g = block_grid(c);
+  const pm::Outputs o = make_outputs(c, ForceEpilogue{});
+  const bool energy = flags & XS_ENERGY, vir = flags & XS_VIRIAL;
+  const pm::Drift dr{};
+  ProfScope prof(c.prof, c.stream, XS_PROF_LJ);
+  auto run = [&](auto zero) {
+    using Real = decltype(zero);
+    const pm::LJ<Real> none{};
+    const pm::LJMultiDev lm = ljm_dev<Real>(c);
+#define XS_BLM(EN, VI) \
+  k_block_lj<Real, EN, VI, false, false, true><<<g, kPassThreads, lds, c.stream>>>(a, none, o, dr, lm)
+    if (energy && vir) XS_BLM(true, true);
+    else if (energy) XS_BLM(true, false);
+    else if (vir) XS_BLM(false, true);
+    else XS_BLM(false, false);
+#undef XS_BLM
+  };
+  if (flags & XS_MIXED) run(0.0f);
+  else run(0.0);
+  XS_HIP(hipGetLastError());
 }
 
 void ensure_global_list(Ctx& c)

@@ -81,13 +81,15 @@ __device__ __forceinline__ void fetch(const PairCommon& pc, int64_t i, int k0, i
 template <class Real>
 using LJParams = pm::LJ<Real>;
 
-template <class Real, bool ENERGY, bool VIRIAL, bool FUSED>
-__global__ void __launch_bounds__(256) k_lj(PairCommon pc, LJParams<Real> lp, Outputs o)
+// MULTI (lj_multi_force): each pair takes the parameters and cutoff of its two species.
+template <class Real, bool ENERGY, bool VIRIAL, bool FUSED, bool MULTI = false>
+__global__ void __launch_bounds__(256) k_lj(PairCommon pc, LJParams<Real> lp, Outputs o, pm::LJMultiDev lm = {})
 {
   const int64_t i = logical_block() * blockDim.x + threadIdx.x;
   if (i >= pc.n) return;
   const double* L = pc.box[pc.rep ? pc.rep[i] : 0].L;
   const Pos4 pi = load_pos(pc.pos, i);
+  const int ti = MULTI ? pc.type[i] : 0;
   double fx = 0.0, fy = 0.0, fz = 0.0, e = 0.0;
   double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   const int cnt = pc.nnb[i];
@@ -97,7 +99,13 @@ __global__ void __launch_bounds__(256) k_lj(PairCommon pc, LJParams<Real> lp, Ou
 #pragma unroll
     for (int u = 0; u < kBatch; u++) {
       double de_r, eh;
-      pm::lj_pair(lp, nb[u].d2, nb[u].in, de_r, eh);
+      if constexpr (MULTI) {
+        const int pid = pm::type_pair(ti, pc.type[nb[u].j]);
+        pm::lj_pair(pm::lj_multi_row<Real>(lm, pid), nb[u].d2, nb[u].in && pm::lj_multi_in(lm, pid, nb[u].d2),
+                    de_r, eh);
+      } else {
+        pm::lj_pair(lp, nb[u].d2, nb[u].in, de_r, eh);
+      }
       const double fe0 = de_r * nb[u].x, fe1 = de_r * nb[u].y, fe2 = de_r * nb[u].z;
       fx += fe0;
       fy += fe1;
@@ -459,6 +467,23 @@ __global__ void __launch_bounds__(256) k_lj_sym(PairCommon pc, LJParams<Real> lp
   add_own<ENERGY, VIRIAL>(o, i, fx, fy, fz, e, v);
 }
 
+template <class Real>
+void lj_multi_dispatch(Ctx& c, unsigned flags)
+{
+  const PairCommon pc = make_common(c, c.ljm_rcmax);
+  const Outputs o = make_outputs(c, ForceEpilogue{});
+  const pm::LJMultiDev lm = ljm_dev<Real>(c);
+  const unsigned g = pair_grid(c.n);
+  const bool energy = flags & XS_ENERGY, vir = flags & XS_VIRIAL;
+  const LJParams<Real> none{};
+  ProfScope prof(c.prof, c.stream, XS_PROF_LJ);
+  if (energy && vir) k_lj<Real, true, true, false, true><<<g, kBlock, 0, c.stream>>>(pc, none, o, lm);
+  else if (energy) k_lj<Real, true, false, false, true><<<g, kBlock, 0, c.stream>>>(pc, none, o, lm);
+  else if (vir) k_lj<Real, false, true, false, true><<<g, kBlock, 0, c.stream>>>(pc, none, o, lm);
+  else k_lj<Real, false, false, false, true><<<g, kBlock, 0, c.stream>>>(pc, none, o, lm);
+  XS_HIP(hipGetLastError());
+}
+
 PairCommon half_common(Ctx& c, double rcut)
 {
   ensure_half_list(c);
@@ -617,6 +642,18 @@ void launch_lj(Ctx& c, double eps, double sigma, double rcut, unsigned flags, co
   }
   if (flags & XS_MIXED) lj_dispatch<float>(c, eps, sigma, rcut, flags, epi);
   else lj_dispatch<double>(c, eps, sigma, rcut, flags, epi);
+}
+
+void launch_lj_multi(Ctx& c, unsigned flags)
+{
+  c.drift_done = false;
+  if (c.n == 0) return;
+  if (c.list_kind == ListKind::Block) {
+    launch_lj_multi_block(c, flags);
+    return;
+  }
+  if (flags & XS_MIXED) lj_multi_dispatch<float>(c, flags);
+  else lj_multi_dispatch<double>(c, flags);
 }
 
 void launch_eam(Ctx& c, double rcut, bool do_rho, bool do_emb, bool do_force, unsigned flags,

@@ -288,6 +288,28 @@ struct LJ {
   Real sigma2, eps4, eps24, ecut;
 };
 
+// lj_multi_force: parameters per unordered pair of species (packed triangle,
+// entry b*(b+1)/2 + a for a <= b) and each pair's cutoff radius; a pair counts
+// when r <= its rcut (pair_potential_force_op_multiparam.h:82-84).
+struct LJMultiDev {
+  const void* rows;          // LJ<Real> per type pair
+  const double* __restrict__ rcut;
+};
+__device__ __forceinline__ int type_pair(int a, int b)
+{
+  const int lo = min(a, b), hi = max(a, b);
+  return hi * (hi + 1) / 2 + lo;
+}
+template <class Real>
+__device__ __forceinline__ LJ<Real> lj_multi_row(const LJMultiDev& m, int pid)
+{
+  return reinterpret_cast<const LJ<Real>*>(m.rows)[pid];
+}
+__device__ __forceinline__ bool lj_multi_in(const LJMultiDev& m, int pid, double d2)
+{
+  return sqrt(d2) <= m.rcut[pid];
+}
+
 template <class Real>
 __device__ __forceinline__ void lj_pair(const LJ<Real>& lp, double d2, bool in, double& de_r, double& e)
 {
@@ -302,5 +324,16 @@ __device__ __forceinline__ void lj_pair(const LJ<Real>& lp, double d2, bool in, 
 }  // namespace pm
 
 pm::Outputs make_outputs(Ctx& c, const ForceEpilogue& ep);
+
+// device view of the lj_multi_force tables in the precision Real
+template <class Real>
+inline pm::LJMultiDev ljm_dev(const Ctx& c)
+{
+  pm::LJMultiDev m;
+  if constexpr (sizeof(Real) == 4) m.rows = c.ljm_f.p;
+  else m.rows = c.ljm_d.p;
+  m.rcut = c.ljm_rcut.p;
+  return m;
+}
 
 }  // namespace xsmd

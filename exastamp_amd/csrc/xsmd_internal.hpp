@@ -302,6 +302,13 @@ struct Ctx {
 
   bool has_lj = false;
   double lj_eps = 0, lj_sigma = 0, lj_rcut = 0;
+  // lj_multi_force: per type pair (packed triangle) FP32 and FP64 rows
+  // {sigma^2, 4 eps, 24 eps, E(rcut)} and the pairs' cutoffs
+  bool has_ljm = false;
+  int ljm_ntypes = 0;
+  double ljm_rcmax = 0;
+  DBuf<float> ljm_f;
+  DBuf<double> ljm_d, ljm_rcut;
   bool has_eam = false;
   EamHost eam;
 
@@ -365,6 +372,9 @@ inline double lj_energy_cutoff(double eps, double sigma, double rcut)
   return 4. * eps * (r12 - r6);
 }
 void launch_lj(Ctx& c, double eps, double sigma, double rcut, unsigned flags, const ForceEpilogue& ep);
+// lj_multi_force (operator form) on the current list; tables set by xs_lj_multi_force
+void launch_lj_multi(Ctx& c, unsigned flags);
+void launch_lj_multi_block(Ctx& c, unsigned flags);
 // LJ on the cell-block list (k_block.hip)
 void launch_lj_block(Ctx& c, double eps, double sigma, double rcut, unsigned flags, const ForceEpilogue& ep);
 void launch_eam(Ctx& c, double rcut, bool do_rho, bool do_rho2emb, bool do_force, unsigned flags,

@@ -728,6 +728,17 @@ def _op_lj_symetric(r: Runner, p):
     _op_lj(r, p, symmetric=True)
 
 
+def _op_lj_multi(r: Runner, p):
+    rc = parse_quantity(p["rcut"])
+    pairs = p.get("parameters") or []
+    _publish_rcut(r, max([rc] + [parse_quantity(e["rcut"]) for e in pairs]))
+    if r.dry:
+        return
+    r.sim.lj_multi_force(rcut=rc, parameters=pairs, common_parameters=p.get("common_parameters"),
+                         trigger_thermo_state=_trigger(r), compute_virial=bool(p.get("compute_virial", True)),
+                         mixed=r.mixed)
+
+
 def _tables_from(r: Runner, params):
     f = params.get("file") if isinstance(params, dict) else None
     if f is None:
@@ -869,6 +880,7 @@ _OPS = {
     "particle_displ_over_async_end": _op_particle_displ_over_async_end,
     "lj_compute_force": _op_lj,
     "lj_compute_force_symetric": _op_lj_symetric,
+    "lj_multi_force": _op_lj_multi,
     "eam_alloy_init": _op_eam_alloy_init,
     "eam_alloy_force": _op_eam_alloy_force,
     "zero_force_energy": _op_zero,

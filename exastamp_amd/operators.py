@@ -171,6 +171,34 @@ class Simulation:
         the same forces, each pair evaluated once with Newton's third law."""
         self.lj_compute_force(parameters, rcut, symmetric=True, **kw)
 
+    def lj_multi_force(self, rcut, parameters=(), common_parameters=None, compute_virial=True,
+                       trigger_thermo_state=True, mixed=False, **_slots):
+        """lj_multi_force{rcut, common_parameters: {epsilon, sigma}, parameters: [{type_a,
+        type_b, rcut, parameters: {epsilon, sigma}}, ...]} (pair_potential_impl.hxx:282-320,
+        pair_potential_force_op_multiparam.h:229-283): every unordered pair of species takes
+        its entry (in either order) or common_parameters with the operator's rcut."""
+        rc = parse_quantity(rcut)
+        common = common_parameters or {"epsilon": 0.0, "sigma": 0.0}
+        names = list(self.species_names)
+        nt = len(names)
+        npair = nt * (nt + 1) // 2
+        eps = np.full(npair, parse_quantity(common["epsilon"]), dtype=np.float64)
+        sig = np.full(npair, parse_quantity(common["sigma"]), dtype=np.float64)
+        rcs = np.full(npair, rc, dtype=np.float64)
+        for e in parameters or ():
+            try:
+                a, b = names.index(e["type_a"]), names.index(e["type_b"])
+            except ValueError:
+                raise ValueError(f"lj_multi_force: unknown species in {e}") from None
+            lo, hi = min(a, b), max(a, b)
+            pid = hi * (hi + 1) // 2 + lo
+            eps[pid] = parse_quantity(e["parameters"]["epsilon"])
+            sig[pid] = parse_quantity(e["parameters"]["sigma"])
+            rcs[pid] = parse_quantity(e["rcut"])
+        self.rcut_max = max(self.rcut_max, rc, float(rcs.max()))
+        self.ctx.xs_lj_multi_force(nt, _lib.dptr(eps), _lib.dptr(sig), _lib.dptr(rcs),
+                                   self._flags(trigger_thermo_state, compute_virial, mixed))
+
     def lj_parameters(self, parameters, rcut):
         """Set the LJ parameters slot without computing (for md_run)."""
         eps = parse_quantity(parameters["epsilon"])

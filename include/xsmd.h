@@ -145,6 +145,14 @@ xs_status xs_lj_compute_force(xs_ctx* ctx, double epsilon, double sigma, double 
 /* The parameters slot of lj_compute_force alone (LennardJonesParms, lennard_jones.h:33-37,
  * decoded at :60-71), for xs_md_run. */
 xs_status xs_lj_parameters(xs_ctx* ctx, double epsilon, double sigma, double rcut);
+/* lj_multi_force (pair_potential_impl_multiparam.cu with the per-type-pair table of
+ * pair_potential_force_op_multiparam.h:58-180 and pair_potential_impl.hxx:282-320):
+ * LJ with its own epsilon, sigma and cutoff for every unordered pair of the ntypes
+ * species (ntypes = the species count), entry b*(b+1)/2 + a for types a <= b; each
+ * pair's energy is shifted by its E(rcut), a pair counts when r <= its rcut, and
+ * epsilon 0 gives no interaction.  Operator form (adds forces, energies, virials). */
+xs_status xs_lj_multi_force(xs_ctx* ctx, int ntypes, const double* epsilon, const double* sigma,
+                            const double* rcut, unsigned flags);
 /* eam_alloy_init + parameters (eam_alloy.cpp:66-278): tabulated knots in setfl
  * layout.  frho: nel*nrho values, rhor: nel*nr, z2r: nel(nel+1)/2*nr (packed lower
  * triangle 11,21,22,31,...), z2r = r*phi in eV.A, F in eV. */
