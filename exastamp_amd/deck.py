@@ -728,6 +728,38 @@ def _op_lj_symetric(r: Runner, p):
     _op_lj(r, p, symmetric=True)
 
 
+def _lj_energy(eps, sigma, rr):
+    # lj_compute_energy (lennard_jones.h:40-50)
+    ratio = sigma / rr
+    r6 = ratio ** 6
+    r12 = r6 * r6
+    return 4.0 * eps * (r12 - r6), (-24.0 * eps * (2.0 * r12 - r6)) / rr
+
+
+def _op_lj_plot(r: Runner, p):
+    """lj_plot (pair_potential_plot.cpp:45-103): e(r), de/dr and their central-difference
+    check at r = i rcut / samples (0 < i < samples), one line each, in `file`
+    (default lj.csv), values in internal units, C++ stream formatting."""
+    eps = parse_quantity(p["parameters"]["epsilon"])
+    sig = parse_quantity(p["parameters"]["sigma"])
+    rc = parse_quantity(p["rcut"])
+    n = int(p.get("samples", 1000))
+    step = rc / n
+    small = step * 1.0e-2
+    with open(str(p.get("file", "lj.csv")), "w") as out:
+        for i in range(1, n):
+            rr = i * step
+            e1, _ = _lj_energy(eps, sig, rr - small)
+            e2, _ = _lj_energy(eps, sig, rr + small)
+            e, de = _lj_energy(eps, sig, rr)
+            approx = (e2 - e1) / (2.0 * small)
+            diff = approx - de
+            ampl = max(abs(approx), abs(de))
+            if ampl > 0.0:
+                diff /= ampl
+            out.write(" ".join(format(x, "g") for x in (rr, e, de, approx, diff)) + "\n")
+
+
 def _op_lj_multi(r: Runner, p):
     rc = parse_quantity(p["rcut"])
     pairs = p.get("parameters") or []
@@ -881,6 +913,7 @@ _OPS = {
     "lj_compute_force": _op_lj,
     "lj_compute_force_symetric": _op_lj_symetric,
     "lj_multi_force": _op_lj_multi,
+    "lj_plot": _op_lj_plot,
     "eam_alloy_init": _op_eam_alloy_init,
     "eam_alloy_force": _op_eam_alloy_force,
     "zero_force_energy": _op_zero,

@@ -170,3 +170,23 @@ def test_eam_benchmark_Al_Cu_deck_forms():
     _same_states(single, ref, 1e-12)
     _same_states(split, ref, 1e-12)
     _same_states(sym, ref, 1e-11)
+
+
+def test_lj_plot_deck(tmp_path, monkeypatch):
+    """plot.msp (lj_plot): 999 samples of e(r) and de/dr over (0, 8 A) in lj.csv; the
+    analytic derivative agrees with the central difference, the minimum sits at
+    2^(1/6) sigma, and rows equal the closed form."""
+    import numpy as np
+    from exastamp_amd import deck, units
+    monkeypatch.chdir(tmp_path)
+    cfg = deck.load_deck(DECKS / "lj" / "plot.msp", [GOLDEN])
+    deck.Runner(cfg, out=open("/dev/null", "w")).run()
+    rows = np.loadtxt(tmp_path / "lj.csv")
+    assert rows.shape == (999, 5)
+    eps, sig = 0.0104 * units.EV, 3.4
+    r = rows[:, 0]
+    ratio = sig / r
+    e = 4 * eps * (ratio ** 12 - ratio ** 6)
+    assert np.allclose(rows[:, 1], e, rtol=1e-5, atol=1e-9)
+    assert np.abs(rows[r > 3.0, 4]).max() < 1e-4
+    assert abs(r[np.argmin(rows[:, 1])] - 2 ** (1 / 6) * sig) < 8.0 / 1000
