@@ -95,3 +95,109 @@ def test_screen_log_modes_and_status_column():
         cols = deck.log_line(rec, "mechanical", moves).split()
         assert (len(cols) == 11) if ch is None else (cols[3] == ch)
     assert "13.672" in deck.log_line(rec, "default")
+
+
+class _StubSim:
+    """Stands in for the operator mirror (no GPU): records the operator calls and
+    answers particle_displ_over from a script."""
+    over_script = ()
+
+    def __init__(self, device=0):
+        self.calls = []
+        self.n = 0
+        self._over = list(type(self).over_script)
+
+    def _log(self, name):
+        self.calls.append(name)
+
+    def species(self, spec):
+        self._log("species")
+
+    def set_config(self, cfg):
+        self.n = cfg.n
+        self._log("set_config")
+
+    def chunk_neighbors(self, **kw):
+        self._log("chunk_neighbors")
+
+    def particle_displ_over(self, th):
+        self._log("particle_displ_over")
+        return self._over.pop(0) if self._over else False
+
+    def zero_force_energy(self):
+        self._log("zero")
+
+    def force_to_accel(self):
+        self._log("accel")
+
+    def lj_compute_force(self, **kw):
+        self._log("lj")
+
+    def push_f_v_r(self, dt, dt_scale=1.0, xform_mode=None):
+        self._log("push_f_v_r")
+
+    def push_f_v(self, dt, dt_scale=0.5, xform_mode=None):
+        self._log("push_f_v")
+
+    def simulation_thermodynamic_state(self):
+        import numpy as np
+        from exastamp_amd.operators import ThermodynamicState
+        raw = np.zeros(27)
+        raw[21:24] = 1.0
+        raw[25] = 1.0
+        raw[26] = max(self.n, 1)
+        return ThermodynamicState(raw, 1000.0)
+
+
+def _stub_run(monkeypatch, deck_path, steps, over):
+    from exastamp_amd import deck, operators
+    monkeypatch.setattr(operators, "Simulation", type("S", (_StubSim,), {"over_script": tuple(over)}))
+    cfg = deck.load_deck(deck_path, [GOLDEN], [(["global", "max_iteration"], str(steps))])
+    r = deck.Runner(cfg, out=open("/dev/null", "w"))
+    r.run()
+    return r, r.sim.calls
+
+
+def _steps(calls):
+    """the calls after the initial build and forces, split at each push_f_v_r"""
+    out, cur = [], None
+    for c in calls[calls.index("push_f_v_r"):]:
+        if c == "push_f_v_r":
+            cur = []
+            out.append(cur)
+        cur.append(c)
+    return out
+
+
+def test_deck_standard_scheme_call_order(monkeypatch):
+    """verlet_nve with check_and_update_particles as in config_numerical_schemes.msp and
+    config_move_particles.msp: the trigger before the forces, a rebuild only when it fires."""
+    _, calls = _stub_run(monkeypatch, GOLDEN / "decks" / "lj" / "single_specy_nosym.msp", 3, [False, True, False])
+    steps = _steps(calls)
+    assert len(steps) == 3
+    quiet = ["push_f_v_r", "push_f_v", "particle_displ_over", "zero", "lj", "accel", "push_f_v"]
+    assert steps[0] == quiet and steps[2] == quiet
+    assert steps[1] == ["push_f_v_r", "push_f_v", "particle_displ_over", "chunk_neighbors", "zero", "lj", "accel",
+                        "push_f_v"]
+
+
+def test_deck_speculative_scheme_call_order(monkeypatch):
+    """config_speculative_verlet.msp: forces on the old list first, the async trigger read
+    after them, and on a hit a rebuild and the forces again."""
+    _, calls = _stub_run(monkeypatch, GOLDEN / "decks" / "lj" / "single_specy_nosym_speculative_verlet.msp", 3,
+                         [False, True, False])
+    steps = _steps(calls)
+    assert len(steps) == 3
+    quiet = ["push_f_v_r", "particle_displ_over", "push_f_v", "zero", "lj", "accel", "push_f_v"]
+    assert steps[0] == quiet and steps[2] == quiet
+    assert steps[1] == ["push_f_v_r", "particle_displ_over", "push_f_v", "zero", "lj", "accel", "chunk_neighbors",
+                        "zero", "lj", "accel", "push_f_v"]
+
+
+def test_deck_block_conditions():
+    from exastamp_amd import deck
+    r = deck.Runner.__new__(deck.Runner)
+    r.slots, r.rebind = {"flag": True, "off": False}, []
+    assert r.condition("flag") and not r.condition("not flag")
+    assert not r.condition("off") and r.condition("not off")
+    assert r.condition("not missing") and r.condition(True)
