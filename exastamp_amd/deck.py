@@ -721,7 +721,7 @@ def _op_lj(r: Runner, p, symmetric=False):
         return
     r.sim.lj_compute_force(parameters=p["parameters"], rcut=rc, trigger_thermo_state=_trigger(r),
                            compute_virial=bool(p.get("compute_virial", True)), mixed=r.mixed,
-                           symmetric=symmetric)
+                           symmetric=symmetric, type=p.get("type"))
 
 
 def _op_lj_symetric(r: Runner, p):

@@ -154,10 +154,17 @@ class Simulation:
         return f
 
     def lj_compute_force(self, parameters, rcut, ghost=False, compute_virial=True,
-                         trigger_thermo_state=True, mixed=False, symmetric=False, **_slots):
+                         trigger_thermo_state=True, mixed=False, symmetric=False, type=None, **_slots):
         """lj_compute_force{parameters: {epsilon, sigma}, rcut} (pair_potential_impl.hxx:108-121).
         The reference always fills ep (and the virial when the grid has one); here
-        they are filled when trigger_thermo_state (and compute_virial) are set."""
+        they are filled when trigger_thermo_state (and compute_virial) are set.  As
+        there (pair_potential_impl.hxx:143-158), a single-material operator on a system
+        of several species needs its `type` slot; every pair then takes the parameters."""
+        if len(self.species_names) > 1 and type is None:
+            raise ValueError(f"lj_compute_force: exactly 1 atom type expected (single material) but found "
+                             f"{len(self.species_names)}; give the type slot")
+        if type is not None and type not in self.species_names:
+            raise ValueError(f"lj_compute_force: unknown type {type!r}")
         eps = parse_quantity(parameters["epsilon"])
         sig = parse_quantity(parameters["sigma"])
         rc = parse_quantity(rcut)

@@ -82,7 +82,9 @@ def test_lj_multi_with_equal_pairs_is_lj():
     sim.chunk_neighbors(rcut_max=8.0, rcut_inc=0.3)
     p = {"epsilon": 0.0104 * EV, "sigma": 3.4}
     sim.zero_force_energy()
-    sim.lj_compute_force(parameters=p, rcut=8.0, trigger_thermo_state=True, compute_virial=True)
+    with pytest.raises(ValueError, match="exactly 1 atom type"):
+        sim.lj_compute_force(parameters=p, rcut=8.0)
+    sim.lj_compute_force(parameters=p, rcut=8.0, trigger_thermo_state=True, compute_virial=True, type="Zn")
     a = (sim.accelerations(), sim.ep(), sim.virial())
     sim.zero_force_energy()
     sim.lj_multi_force(rcut=8.0, parameters=(), common_parameters=p, trigger_thermo_state=True,
