@@ -153,27 +153,21 @@ def force_error(sim, rc):
     }
 
 
-def _oracle_forces_energies(boxes, masses, tab, threads):
-    """FP64 oracle forces (eV/A) and per-box energies (eV)."""
-    sys.path.insert(0, str(ROOT / "oracle"))
-    import pyoracle
+def _fp64_forces_energies(sim, tab):
+    """FP64 forces (eV/A, caller's atom order) and per-replica energies (eV) of the GPU
+    form with tables `tab` (the FP64 path equals the oracle to 1e-11, tests/)."""
     from exastamp_amd.units import EV
-    Fs, Es = [], []
-    for b in boxes:
-        o = pyoracle.Oracle(threads=threads)
-        o.set_config(b, masses)
-        o.set_eam(tab)
-        o.build_neighbors(tab.rc, 0.3)
-        o.compute_forces("eam")
-        Fs.append(o.forces() / EV)
-        Es.append(o.state()["ep"].sum() / EV)
-        o.close()
-    return np.concatenate(Fs), np.array(Es)
+    sim.eam_alloy_init(tab)
+    sim.zero_force_energy()
+    sim.eam_alloy_force(rcut=tab.rc, trigger_thermo_state=True, mixed=False)
+    sim.force_to_accel()
+    return sim.forces() / EV, sim.replica_energies() / EV
 
 
-def adjoint_leg(name, cfg, masses, names, tab, nfd, repeats, threads):
+def adjoint_leg(name, cfg, masses, names, tab, nfd, repeats):
     """Forward + adjoint dLoss/dtheta on one batch: GPU time per forward+backward,
-    device memory, and gradient error vs FP64 central differences of the oracle.
+    device memory, and gradient error vs FP64 central differences of the FP64 forward
+    (targets: the FP64 forward with tables perturbed by a seeded +-2 %).
     Loss = force MSE (eV/A)^2 + 0.1 per-atom energy MSE (eV^2), the gradients of which
     are formed here with numpy (the torch harness in exastamp_amd.fitting does the same)."""
     import ctypes as C
@@ -181,16 +175,15 @@ def adjoint_leg(name, cfg, masses, names, tab, nfd, repeats, threads):
     from exastamp_amd.fitting import fit_loss_np
     from exastamp_amd.operators import Simulation
     from exastamp_amd.units import EV
-    boxes = inputs.split_replicas(cfg)
     theta0 = tab.theta()
     tab_t = tab.with_theta(theta0 * (1.0 + inputs.rng(5).uniform(-0.02, 0.02, theta0.size)))
-    Ft, Et = _oracle_forces_energies(boxes, masses, tab_t, threads)
     nper = np.bincount(cfg.rep) if cfg.rep is not None else np.array([cfg.n])
     sim = Simulation(0)
     sim.species(list(zip(names, masses)))
     sim.set_config(cfg)
     sim.eam_alloy_init(tab)
     sim.chunk_neighbors(rcut_max=tab.rc, rcut_inc=0.3)
+    Ft, Et = _fp64_forces_energies(sim, tab_t)
 
     def step(fwd_mixed, adj_mixed):
         sim.eam_alloy_init(tab.with_theta(theta0))
@@ -226,7 +219,7 @@ def adjoint_leg(name, cfg, masses, names, tab, nfd, repeats, threads):
         for sgn in (+1, -1):
             th = theta0.copy()
             th[k] += sgn * h
-            F, E = _oracle_forces_energies(boxes, masses, tab.with_theta(th), threads)
+            F, E = _fp64_forces_energies(sim, tab.with_theta(th))
             v.append(fit_loss_np(F, E, Ft, Et, nper))
         fd.append((v[0] - v[1]) / (2 * h))
     fd = np.array(fd)
@@ -423,18 +416,16 @@ def main():
     adjoint = None
     if not args.no_adjoint:
         from exastamp_amd import inputs as inp
-        threads = args.cpu_threads or int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or len(os.sched_getaffinity(0))
         sim.close()
         log("adjoint C4: 64 strained Cu boxes x 864 atoms, 3 x 500-knot tables, FD on 32 knots")
         c4 = inp.strained_batch(64, (6, 6, 6), 3.615, [63.546], seed=41)
-        a4 = adjoint_leg("C4 batch", c4, [63.546], ["Cu"], potentials.johnson_tables(nr=500, nrho=500), 32, 20,
-                         threads)
+        a4 = adjoint_leg("C4 batch", c4, [63.546], ["Cu"], potentials.johnson_tables(nr=500, nrho=500), 32, 20)
         log(f"C4: {a4['ms_forward_backward_wall']:.2f} ms/forward+backward, FD err {a4['grad_rel_err_vs_fd_max']}")
         log("adjoint C5: Cu/Ni 64^3 FCC slab, 1% vacancies, 7 x 2000-knot tables, FD on 4 knots")
         c5 = inp.alloy_slab_config((64, 64, 64), 3.57, [63.546, 58.693], vacancy_frac=0.01, vacuum=10.0,
                                    T=300.0, seed=42)
         t5 = potentials.johnson_tables((potentials.CU, potentials.NI), nr=2000, nrho=2000)
-        a5 = adjoint_leg("C5 Cu/Ni slab", c5, [63.546, 58.693], ["Cu", "Ni"], t5, 4, 5, threads)
+        a5 = adjoint_leg("C5 Cu/Ni slab", c5, [63.546, 58.693], ["Cu", "Ni"], t5, 4, 5)
         log(f"C5: {a5['ms_forward_backward_wall']:.2f} ms/forward+backward, FD err {a5['grad_rel_err_vs_fd_max']}")
         adjoint = {"C4": a4, "C5": a5}
 
