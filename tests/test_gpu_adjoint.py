@@ -70,6 +70,11 @@ def test_adjoint_matches_central_differences():
         loss = fit_loss(F, E, torch.from_numpy(Ft), torch.from_numpy(Et), nper)
         loss.backward()
         grads[(fwd_mixed, adj_mixed)] = theta.grad.numpy().copy()
+    theta = torch.tensor(tab.theta(), requires_grad=True)
+    F, E = EamForcesEnergies.apply(theta, sim)  # default: FP64 forward, FP32-table adjoint
+    fit_loss(F, E, torch.from_numpy(Ft), torch.from_numpy(Et), nper).backward()
+    np.testing.assert_allclose(theta.grad.numpy(), grads[(False, True)], rtol=1e-9,
+                               atol=1e-12 * np.abs(grads[(False, True)]).max())
     g64 = grads[(False, False)]
     # FP64 central differences of the oracle loss on 32 random sampled knots
     active = np.flatnonzero(np.abs(g64) > 1e-3 * np.abs(g64).max())
