@@ -7,6 +7,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdlib>
+#include <cstring>
 
 namespace xsmd {
 
@@ -555,6 +556,23 @@ xs_status xs_eam_alloy_force(xs_ctx* h, double rcut, int eam_rho, int eam_rho2em
       require(rcut <= c.list_rcut + c.list_skin, XS_E_ARG, "rcut exceeds the neighbour-list radius");
     }
     launch_eam(c, rcut, eam_rho != 0, eam_rho2emb != 0, eam_force != 0, flags, ForceEpilogue{});
+  });
+}
+
+xs_status xs_johnson_force(xs_ctx* h, const double params[19], double rcut, unsigned flags)
+{
+  return guarded(h, [&](Ctx& c) {
+    require(params != nullptr, XS_E_ARG, "null Johnson parameters");
+    require(rcut > 0, XS_E_ARG, "rcut must be positive");
+    require(!(flags & (XS_MIXED | XS_SYMMETRIC)), XS_E_ARG,
+            "johnson_force runs in FP64 on the full list (no XS_MIXED or XS_SYMMETRIC form)");
+    JohnsonParams p;
+    static_assert(sizeof(JohnsonParams) == 19 * sizeof(double), "19 packed doubles");
+    std::memcpy(&p, params, sizeof p);
+    require(p.re > 0 && p.rhoe > 0, XS_E_ARG, "re and rhoe must be positive");
+    require_list(c);
+    require(rcut <= c.list_rcut + c.list_skin, XS_E_ARG, "rcut exceeds the neighbour-list radius");
+    launch_johnson(c, p, rcut, flags);
   });
 }
 

@@ -197,6 +197,128 @@ __global__ void __launch_bounds__(256) k_eam_force(PairCommon pc, EamTabDev<Real
   write_outputs<ENERGY, VIRIAL, FUSED>(o, i, fx, fy, fz, e, v);
 }
 
+// ------------------------------------------------------------------------------
+// Analytic Johnson EAM (johnson_force): the single-material operator template
+// (eam_potential.cu:62-193; EmbOp and ForceOp, eam_force_op_singlemat.h:39-171) with
+// the functions of johnson.h:59-167.  FP64 throughout; the energy terms are always
+// accumulated, as EmbOp and ForceOp do.
+// ------------------------------------------------------------------------------
+
+// One screened exponential c exp(-a (x - 1)) / (1 + (x - k)^20) of x = r / re and its
+// r-derivative by the quotient rule: phi is the sum of two such terms (johnson.h:59-99),
+// rho(r) is one (johnson.h:105-124).
+__device__ __forceinline__ void johnson_term(double c, double a, double k, double x, double ire, double& v,
+                                             double& dv)
+{
+  const double num = c * exp(-a * (x - 1.0));
+  const double s = x - k;
+  const double s20 = pow(s, 20.0);
+  const double den = 1.0 + s20;
+  const double dnum = -a * num;
+  const double dden = 20.0 * s20 / s;
+  v = num / den;
+  dv = ire * (dnum * den - num * dden) / (den * den);
+}
+
+__device__ __forceinline__ void johnson_rho(const JohnsonParams& p, double r, double& rho, double& drho)
+{
+  johnson_term(p.fe, p.beta, p.lambda, r / p.re, 1.0 / p.re, rho, drho);
+}
+
+__device__ __forceinline__ void johnson_phi(const JohnsonParams& p, double r, double& phi, double& dphi)
+{
+  const double x = r / p.re, ire = 1.0 / p.re;
+  double v2, d2;
+  johnson_term(p.A, p.alpha, p.kappa, x, ire, phi, dphi);
+  johnson_term(-p.B, p.beta, p.lambda, x, ire, v2, d2);
+  phi += v2;
+  dphi += d2;
+}
+
+// F(rho) and F'(rho) (johnson.h:131-167): cubic in rho / (0.85 rhoe) - 1 below
+// 0.85 rhoe, cubic in rho / rhoe - 1 below 1.15 rhoe, Fo (1 - ln q^eta) q^eta above.
+__device__ __forceinline__ void johnson_embed(const JohnsonParams& p, double rho, double& f, double& df)
+{
+  const double rhon = 0.85 * p.rhoe, rho0 = 1.15 * p.rhoe;
+  const double irhoe = 1.0 / p.rhoe;
+  if (rho < rhon) {
+    const double t = rho / rhon - 1.0, t2 = t * t, t3 = t * t2;
+    f = p.Fn0 + p.Fn1 * t + p.Fn2 * t2 + p.Fn3 * t3;
+    df = (p.Fn1 + 2.0 * p.Fn2 * t + 3.0 * p.Fn3 * t2) * (1.0 / rhon);
+  } else if (rho < rho0) {
+    const double t = rho * irhoe - 1.0, t2 = t * t, t3 = t * t2;
+    f = p.F0 + p.F1 * t + p.F2 * t2 + p.F3 * t3;
+    df = (p.F1 + 2.0 * p.F2 * t + 3.0 * p.F3 * t2) * irhoe;
+  } else {
+    const double q = rho * irhoe;
+    const double qe = pow(q, p.eta);
+    const double lq = log(qe);
+    f = p.Fo * (1.0 - lq) * qe;
+    df = (-p.eta * qe + (1.0 - lq) * p.eta * qe) * (p.Fo * irhoe / q);
+  }
+}
+
+// Pass 1 (EmbOp): rho_i over the pairs inside rc, then F'(rho_i) into the position
+// record's w and F(rho_i) added to ep.
+__global__ void __launch_bounds__(256) k_johnson_rho(PairCommon pc, JohnsonParams jp, double* __restrict__ rho_out,
+                                                     double* __restrict__ pos_w, double* __restrict__ ep)
+{
+  const int64_t i = logical_block() * blockDim.x + threadIdx.x;
+  if (i >= pc.n) return;
+  const double* L = pc.box[pc.rep ? pc.rep[i] : 0].L;
+  const Pos4 pi = load_pos(pc.pos, i);
+  const int cnt = pc.nnb[i];
+  double rho = 0.0;
+  for (int k0 = 0; k0 < cnt; k0 += kBatch) {
+    Nb nb[kBatch];
+    fetch<kBatch>(pc, i, k0, cnt, pi, L, nb);
+#pragma unroll
+    for (int u = 0; u < kBatch; u++) {
+      double v, dv;
+      johnson_rho(jp, nb[u].in ? sqrt(nb[u].d2) : jp.re, v, dv);
+      rho += nb[u].in ? v : 0.0;
+    }
+  }
+  rho_out[i] = rho;
+  double F, Fp;
+  johnson_embed(jp, rho, F, Fp);
+  pos_w[4 * i + 3] = Fp;
+  ep[i] += F;
+}
+
+// Pass 2 (ForceOp): f_i += (rho'(r) (F'_i + F'_j) + phi'(r)) / r (r_j - r_i), ep += phi / 2,
+// virial -1/2 f (x) dr.
+template <bool VIRIAL>
+__global__ void __launch_bounds__(256) k_johnson_force(PairCommon pc, JohnsonParams jp, Outputs o)
+{
+  const int64_t i = logical_block() * blockDim.x + threadIdx.x;
+  if (i >= pc.n) return;
+  const double* L = pc.box[pc.rep ? pc.rep[i] : 0].L;
+  const Pos4 pi = load_pos(pc.pos, i);
+  double fx = 0.0, fy = 0.0, fz = 0.0, e = 0.0;
+  double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  const int cnt = pc.nnb[i];
+  for (int k0 = 0; k0 < cnt; k0 += kBatch) {
+    Nb nb[kBatch];
+    fetch<kBatch>(pc, i, k0, cnt, pi, L, nb);
+#pragma unroll
+    for (int u = 0; u < kBatch; u++) {
+      const double r = nb[u].in ? sqrt(nb[u].d2) : jp.re;
+      double rho, drho, phi, dphi;
+      johnson_rho(jp, r, rho, drho);
+      johnson_phi(jp, r, phi, dphi);
+      const double de = nb[u].in ? (drho * (pi.w + nb[u].w) + dphi) / r : 0.0;
+      const double fe0 = de * nb[u].x, fe1 = de * nb[u].y, fe2 = de * nb[u].z;
+      fx += fe0;
+      fy += fe1;
+      fz += fe2;
+      e += nb[u].in ? 0.5 * phi : 0.0;
+      if constexpr (VIRIAL) add_virial(v, fe0, fe1, fe2, nb[u].x, nb[u].y, nb[u].z);
+    }
+  }
+  write_outputs<true, VIRIAL, false>(o, i, fx, fy, fz, e, v);
+}
+
 PairCommon make_common(Ctx& c, double rcut)
 {
   PairCommon pc;
@@ -654,6 +776,28 @@ void launch_lj_multi(Ctx& c, unsigned flags)
   }
   if (flags & XS_MIXED) lj_multi_dispatch<float>(c, flags);
   else lj_multi_dispatch<double>(c, flags);
+}
+
+void launch_johnson(Ctx& c, const JohnsonParams& p, double rcut, unsigned flags)
+{
+  c.drift_done = false;
+  // F' of this potential replaces the position records' w: no eam_alloy state survives
+  c.rho_state = RhoState::None;
+  c.rho_ckpt = false;
+  if (c.n == 0) return;
+  ensure_global_list(c);
+  const PairCommon pc = make_common(c, rcut);
+  const unsigned g = pair_grid(c.n);
+  {
+    ProfScope prof(c.prof, c.stream, XS_PROF_EAM_RHO);
+    k_johnson_rho<<<g, kBlock, 0, c.stream>>>(pc, p, c.rho.p, reinterpret_cast<double*>(c.pos.p), c.ep.p);
+    XS_HIP(hipGetLastError());
+  }
+  const Outputs o = make_outputs(c, ForceEpilogue{});
+  ProfScope prof(c.prof, c.stream, XS_PROF_EAM_FORCE);
+  if (flags & XS_VIRIAL) k_johnson_force<true><<<g, kBlock, 0, c.stream>>>(pc, p, o);
+  else k_johnson_force<false><<<g, kBlock, 0, c.stream>>>(pc, p, o);
+  XS_HIP(hipGetLastError());
 }
 
 void launch_eam(Ctx& c, double rcut, bool do_rho, bool do_emb, bool do_force, unsigned flags,

@@ -374,6 +374,13 @@ inline double lj_energy_cutoff(double eps, double sigma, double rcut)
 void launch_lj(Ctx& c, double eps, double sigma, double rcut, unsigned flags, const ForceEpilogue& ep);
 // lj_multi_force (operator form) on the current list; tables set by xs_lj_multi_force
 void launch_lj_multi(Ctx& c, unsigned flags);
+// EamJohnsonParameters (johnson.h:30-51) in internal units, in the order of
+// xs_johnson_force's parameter array
+struct JohnsonParams {
+  double re, fe, rhoe, alpha, beta, A, B, kappa, lambda, Fn0, Fn1, Fn2, Fn3, F0, F1, F2, F3, Fo, eta;
+};
+// johnson_force: both passes of the single-material EAM template, FP64, global list
+void launch_johnson(Ctx& c, const JohnsonParams& p, double rcut, unsigned flags);
 void launch_lj_multi_block(Ctx& c, unsigned flags);
 // LJ on the cell-block list (k_block.hip)
 void launch_lj_block(Ctx& c, double eps, double sigma, double rcut, unsigned flags, const ForceEpilogue& ep);

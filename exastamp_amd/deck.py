@@ -803,6 +803,14 @@ def _op_eam_alloy_force(r: Runner, p):
                           compute_virial=bool(p.get("compute_virial", False)), mixed=r.mixed)
 
 
+def _op_johnson(r: Runner, p):
+    rc = parse_quantity(p["rcut"])
+    _publish_rcut(r, rc)
+    if r.dry:
+        return
+    r.sim.johnson_force(rcut=rc, parameters=p["parameters"], compute_virial=bool(p.get("compute_virial", False)))
+
+
 def _op_zero(r: Runner, p):
     if not r.dry:
         r.sim.zero_force_energy()
@@ -916,6 +924,7 @@ _OPS = {
     "lj_plot": _op_lj_plot,
     "eam_alloy_init": _op_eam_alloy_init,
     "eam_alloy_force": _op_eam_alloy_force,
+    "johnson_force": _op_johnson,
     "zero_force_energy": _op_zero,
     "force_to_accel": _op_accel,
     "push_f_v_r": _op_push_f_v_r,

@@ -29,6 +29,10 @@ from .inputs import Config
 from .potentials import SetflTables
 from .units import EV, KB, PRESSURE_UNIT_PA, parse_quantity
 
+# EamJohnsonParameters (johnson.h:30-51): the order of xs_johnson_force's parameter array
+JOHNSON_PARAMETERS = ("re", "fe", "rhoe", "alpha", "beta", "A", "B", "kappa", "lambda", "Fn0", "Fn1", "Fn2", "Fn3",
+                      "F0", "F1", "F2", "F3", "Fo", "eta")
+
 
 @dataclasses.dataclass
 class ThermodynamicState:
@@ -232,6 +236,23 @@ class Simulation:
         self.ctx.xs_eam_alloy_tables(t.nel, t.nrho, t.drho, t.nr, t.dr, t.rc, _lib.dptr(fr), _lib.dptr(rr),
                                      _lib.dptr(zr))
         self.eam = t
+
+    JOHNSON_PARAMETERS = JOHNSON_PARAMETERS
+
+    def johnson_force(self, rcut, parameters, compute_virial=False, **_slots):
+        """johnson_force{rcut, parameters: {re, fe, rhoe, alpha, beta, A, B, kappa, lambda,
+        Fn0..Fn3, F0..F3, Fo, eta}}: analytic Johnson EAM through the single-material
+        template (eam_potential.cu:62-193, johnson.h).  Every parameter is converted with
+        its unit, as the reference's YAML decoder does (johnson.h:185-205).  Energy is
+        always accumulated (EmbOp and ForceOp add to ep unconditionally); the virial when
+        compute_virial stands for a grid with a virial field.  FP64 only."""
+        rc = parse_quantity(rcut)
+        try:
+            p = np.array([parse_quantity(parameters[k]) for k in self.JOHNSON_PARAMETERS], dtype=np.float64)
+        except KeyError as e:
+            raise ValueError(f"johnson_force: parameter {e.args[0]} missing") from None
+        self.rcut_max = max(self.rcut_max, rc)
+        self.ctx.xs_johnson_force(_lib.dptr(p), rc, XS_ENERGY | (XS_VIRIAL if compute_virial else 0))
 
     def eam_alloy_force(self, rcut, eam_rho=True, eam_rho2emb=True, eam_ghost=True, eam_force=True,
                         eam_symmetry=False, compute_virial=False, trigger_thermo_state=False,

@@ -167,6 +167,14 @@ xs_status xs_eam_alloy_load(xs_ctx* ctx, const char* path, double* masses, int* 
  * (SymRhoOp / SymForceOp with NewtonSym, eam_force_op_multimat.h:114-343). */
 xs_status xs_eam_alloy_force(xs_ctx* ctx, double rcut, int eam_rho, int eam_rho2emb,
                              int eam_force, unsigned flags);
+/* johnson_force (single-material EAM template eam_potential.cu:62-193 with the
+ * analytic functions of johnson.h:59-167): both passes, FP64, full list.
+ * params: the 19 EamJohnsonParameters in internal units, in the order
+ *   re fe rhoe alpha beta A B kappa lambda Fn0 Fn1 Fn2 Fn3 F0 F1 F2 F3 Fo eta.
+ * Adds F(rho_i) + sum phi/2 to ep and the force to the force accumulators always
+ * (EmbOp and ForceOp accumulate energy unconditionally); the virial with XS_VIRIAL.
+ * XS_MIXED and XS_SYMMETRIC are rejected.  Leaves no eam_alloy rho/F' state. */
+xs_status xs_johnson_force(xs_ctx* ctx, const double params[19], double rcut, unsigned flags);
 /* push_f_v_r (config_numerical_schemes.msp:23-25): r += v*dt + a*dt^2/2,
  * dt = dt * dt_scale. */
 xs_status xs_push_f_v_r(xs_ctx* ctx, double dt, double dt_scale);

@@ -206,6 +206,10 @@ inline void add_tensor_scaled(double* vir, const double u[3], const double v[3],
 
 }  // namespace
 
+// Johnson parameter indices, in the EamJohnsonParameters order (johnson.h:30-51)
+enum { J_RE, J_FE, J_RHOE, J_ALPHA, J_BETA, J_A, J_B, J_KAPPA, J_LAMBDA, J_FN0, J_FN1, J_FN2, J_FN3, J_F0, J_F1,
+       J_F2, J_F3, J_FO, J_ETA };
+
 // ==========================================================================
 // Context: exaNBody grid restated as cell-sorted flat SoA arrays.
 // ==========================================================================
@@ -226,6 +230,10 @@ struct or_ctx {
   std::vector<double> ljm_eps, ljm_sigma, ljm_rcut, ljm_ecut;
   bool has_eam = false;
   Eam eam;
+  // johnson_force: EamJohnsonParameters (johnson.h:30-51), internal units
+  bool has_johnson = false;
+  double jp[19] = {};
+  double johnson_rc = 0;
 
   int nc[3] = {1, 1, 1};
   double inv_cs[3] = {0, 0, 0};
@@ -476,6 +484,16 @@ int or_set_eam(or_ctx* c, int nel, int nrho, double drho, int nr, double dr, dou
 // stores at chunk_size 1 (data/config/config_move_particles.msp:54-68,
 // iterator semantics src/debug/atom_neighbors_stats.cpp:169-227,
 // filter signature src/particle_species/type_pair_rcut_neighbors.cpp:49-62).
+// johnson_force parameters (19 doubles, EamJohnsonParameters order, internal units)
+int or_set_johnson(or_ctx* c, const double* params, double rcut)
+{
+  if (!params || !(rcut > 0) || !(params[J_RE] > 0) || !(params[J_RHOE] > 0)) return -1;
+  std::copy(params, params + 19, c->jp);
+  c->johnson_rc = rcut;
+  c->has_johnson = true;
+  return 0;
+}
+
 int or_build_neighbors(or_ctx* c, double rcut, double skin)
 {
   const int64_t n = c->n;
@@ -655,6 +673,109 @@ void lj_multi_force(or_ctx* c, int flags)
   });
 }
 
+// ---- Johnson EAM (johnson.h:59-167)
+
+// c exp(-a (x - 1)) / (1 + (x - k)^20) at x = r / re and its r-derivative
+void johnson_screened(double c, double a, double k, double x, double ire, double& v, double& dv)
+{
+  const double num = c * std::exp(-a * (x - 1.0));
+  const double s = x - k;
+  const double s20 = std::pow(s, 20);
+  const double den = 1.0 + s20;
+  v = num / den;
+  dv = ire * ((-a * num) * den - num * (20 * s20 / s)) / (den * den);
+}
+
+// eam_johnson_rho (johnson.h:105-124)
+void johnson_rho(const double* p, double r, double& rho, double& drho)
+{
+  johnson_screened(p[J_FE], p[J_BETA], p[J_LAMBDA], r / p[J_RE], 1.0 / p[J_RE], rho, drho);
+}
+
+// eam_johnson_phi (johnson.h:59-99): A-term + (-B)-term
+void johnson_phi(const double* p, double r, double& phi, double& dphi)
+{
+  const double x = r / p[J_RE], ire = 1.0 / p[J_RE];
+  double vb, db;
+  johnson_screened(p[J_A], p[J_ALPHA], p[J_KAPPA], x, ire, phi, dphi);
+  johnson_screened(-p[J_B], p[J_BETA], p[J_LAMBDA], x, ire, vb, db);
+  phi += vb;
+  dphi += db;
+}
+
+// eam_johnson_fEmbed (johnson.h:131-167)
+void johnson_embed(const double* p, double rho, double& f, double& df)
+{
+  const double rhoe = p[J_RHOE], rhon = 0.85 * rhoe, rho0 = 1.15 * rhoe;
+  if (rho < rhon) {
+    const double t = rho / rhon - 1.0;
+    f = p[J_FN0] + p[J_FN1] * t + p[J_FN2] * (t * t) + p[J_FN3] * (t * (t * t));
+    df = (p[J_FN1] + 2.0 * p[J_FN2] * t + 3.0 * p[J_FN3] * (t * t)) * (1.0 / rhon);
+  } else if (rho < rho0) {
+    const double t = rho * (1.0 / rhoe) - 1.0;
+    f = p[J_F0] + p[J_F1] * t + p[J_F2] * (t * t) + p[J_F3] * (t * (t * t));
+    df = (p[J_F1] + 2.0 * p[J_F2] * t + 3.0 * p[J_F3] * (t * t)) * (1.0 / rhoe);
+  } else {
+    const double q = rho * (1.0 / rhoe);
+    const double qe = std::pow(q, p[J_ETA]);
+    const double lq = std::log(qe);
+    f = p[J_FO] * (1.0 - lq) * qe;
+    df = (-p[J_ETA] * qe + (1.0 - lq) * p[J_ETA] * qe) * (p[J_FO] * (1.0 / rhoe) / q);
+  }
+}
+
+// johnson_force: EmbOp then ForceOp of the single-material template
+// (eam_potential.cu:126-172, eam_force_op_singlemat.h:39-171).  Energy always
+// accumulates (both ops add to ep unconditionally); the virial with OR_VIRIAL.
+void johnson_force(or_ctx* c, int flags)
+{
+  const double* p = c->jp;
+  const double rc2 = c->johnson_rc * c->johnson_rc;
+  for_each_cell(*c, [&](int64_t i) {
+    double rho = 0.0;
+    for (int64_t k = c->nb_off[i]; k < c->nb_off[i + 1]; k++) {
+      double dr[3];
+      c->pair_dr(i, c->nb[k], dr);
+      const double d2 = xsmd_d2(dr[0], dr[1], dr[2]);
+      if (d2 > rc2) continue;
+      double v, dv;
+      johnson_rho(p, std::sqrt(d2), v, dv);
+      rho += v;
+    }
+    double emb, demb;
+    johnson_embed(p, rho, emb, demb);
+    c->ep[i] += emb;
+    c->rho[i] = demb;
+  });
+  for_each_cell(*c, [&](int64_t i) {
+    double f[3] = {0., 0., 0.}, ep = 0.0;
+    double vir[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int64_t k = c->nb_off[i]; k < c->nb_off[i + 1]; k++) {
+      double dr[3];
+      c->pair_dr(i, c->nb[k], dr);
+      const double d2 = xsmd_d2(dr[0], dr[1], dr[2]);
+      if (d2 > rc2) continue;
+      const double r = std::sqrt(d2);
+      double rho, drho, phi, dphi;
+      johnson_rho(p, r, rho, drho);
+      johnson_phi(p, r, phi, dphi);
+      const double de = (drho * (c->rho[i] + c->rho[xsmd_nbr_index(c->nb[k])]) + dphi) / r;
+      const double fe[3] = {de * dr[0], de * dr[1], de * dr[2]};
+      f[0] += fe[0];
+      f[1] += fe[1];
+      f[2] += fe[2];
+      ep += .5 * phi;
+      add_tensor_scaled(vir, fe, dr, -0.5);
+    }
+    c->ax[i] += f[0];
+    c->ay[i] += f[1];
+    c->az[i] += f[2];
+    c->ep[i] += ep;
+    if (flags & OR_VIRIAL)
+      for (int q = 0; q < 9; q++) c->vir[9 * (size_t)i + q] += vir[q];
+  });
+}
+
 // EAM pass 1: parallel_memset + SymRhoOp<false>, eam_potential_multimat.cu:186-203,
 // eam_force_op_multimat.h:114-172
 void eam_rho_pass(or_ctx* c)
@@ -755,6 +876,9 @@ int or_compute_forces(or_ctx* c, int potential, int flags)
   } else if (potential == OR_POT_LJ_MULTI) {
     if (c->ljm_ntypes < 1) return -2;
     lj_multi_force(c, flags);
+  } else if (potential == OR_POT_JOHNSON) {
+    if (!c->has_johnson) return -2;
+    johnson_force(c, flags);
   } else {
     if (!c->has_eam) return -2;
     eam_rho_pass(c);
@@ -1014,4 +1138,15 @@ void or_eam_embed(const or_ctx* c, double rho, int itype, double* F, double* Fp)
 void or_lj_energy(const or_ctx* c, double r, double* e, double* de)
 {
   lj_compute_energy(c->lj_eps, c->lj_sigma, r, *e, *de);
+}
+
+void or_johnson_eval(const double* params, double r, double out[4])
+{
+  johnson_rho(params, r, out[0], out[1]);
+  johnson_phi(params, r, out[2], out[3]);
+}
+
+void or_johnson_embed(const double* params, double rho, double* F, double* Fp)
+{
+  johnson_embed(params, rho, *F, *Fp);
 }

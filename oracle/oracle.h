@@ -25,7 +25,7 @@ extern "C" {
 
 typedef struct or_ctx or_ctx;
 
-enum { OR_POT_LJ = 0, OR_POT_EAM = 1, OR_POT_LJ_MULTI = 2 };
+enum { OR_POT_LJ = 0, OR_POT_EAM = 1, OR_POT_LJ_MULTI = 2, OR_POT_JOHNSON = 3 };
 enum { OR_ENERGY = 1, OR_VIRIAL = 2 };
 
 /* --- spline tables (eam_alloy.cpp:29-58) --------------------------------- */
@@ -53,6 +53,13 @@ int or_set_lj(or_ctx* c, double epsilon, double sigma, double rcut);
 /* lj_multi_force parameters: one (epsilon, sigma, rcut) per unordered pair of the
  * ntypes species, entry b*(b+1)/2 + a for types a <= b. */
 int or_set_lj_multi(or_ctx* c, int ntypes, const double* epsilon, const double* sigma, const double* rcut);
+
+/* johnson_force: the 19 EamJohnsonParameters (johnson.h:30-51) in internal units,
+ * order re fe rhoe alpha beta A B kappa lambda Fn0 Fn1 Fn2 Fn3 F0 F1 F2 F3 Fo eta. */
+int or_set_johnson(or_ctx* c, const double* params, double rcut);
+/* point values: out = rho(r), rho'(r), phi(r), phi'(r); F(rho), F'(rho) */
+void or_johnson_eval(const double* params, double r, double out[4]);
+void or_johnson_embed(const double* params, double rho, double* F, double* Fp);
 
 /* EAM tables from knot values (setfl layout), splined by or_interpolate. */
 int or_set_eam(or_ctx* c, int nel, int nrho, double drho, int nr, double dr, double rcut,

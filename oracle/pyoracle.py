@@ -11,7 +11,8 @@ from pathlib import Path
 import numpy as np
 
 LIB_PATH = Path(__file__).resolve().parent / "liboracle.so"
-OR_POT_LJ, OR_POT_EAM, OR_POT_LJ_MULTI = 0, 1, 2
+OR_POT_LJ, OR_POT_EAM, OR_POT_LJ_MULTI, OR_POT_JOHNSON = 0, 1, 2, 3
+_POT = {"lj": OR_POT_LJ, "eam": OR_POT_EAM, "lj_multi": OR_POT_LJ_MULTI, "johnson": OR_POT_JOHNSON}
 OR_ENERGY, OR_VIRIAL = 1, 2
 
 _D = C.POINTER(C.c_double)
@@ -38,6 +39,9 @@ def load():
                                         _D, _D, C.POINTER(C.c_int), C.c_int, _D]),
             "or_set_lj": (C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_double]),
             "or_set_lj_multi": (C.c_int, [C.c_void_p, C.c_int, _D, _D, _D]),
+            "or_set_johnson": (C.c_int, [C.c_void_p, _D, C.c_double]),
+            "or_johnson_eval": (None, [_D, C.c_double, _D]),
+            "or_johnson_embed": (None, [_D, C.c_double, _D, _D]),
             "or_set_eam": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_int, C.c_double, C.c_double,
                                      _D, _D, _D]),
             "or_build_neighbors": (C.c_int, [C.c_void_p, C.c_double, C.c_double]),
@@ -136,6 +140,12 @@ class Oracle:
         e, s, r = (np.ascontiguousarray(x, dtype=np.float64) for x in (eps, sigma, rcut))
         self._ok(self.lib.or_set_lj_multi(self.h, ntypes, _d(e), _d(s), _d(r)), "set_lj_multi")
 
+    def set_johnson(self, params, rcut):
+        """params: the 19 Johnson parameters (internal units, EamJohnsonParameters order)."""
+        self._jp = np.ascontiguousarray(params, dtype=np.float64)
+        assert self._jp.size == 19
+        self._ok(self.lib.or_set_johnson(self.h, _d(self._jp), rcut), "set_johnson")
+
     def set_eam(self, t):
         fr = np.ascontiguousarray(t.frho, dtype=np.float64)
         rr = np.ascontiguousarray(t.rhor, dtype=np.float64)
@@ -147,7 +157,7 @@ class Oracle:
         self._ok(self.lib.or_build_neighbors(self.h, rcut, skin), "build_neighbors")
 
     def compute_forces(self, potential, flags=OR_ENERGY | OR_VIRIAL):
-        pot = {"lj": OR_POT_LJ, "eam": OR_POT_EAM, "lj_multi": OR_POT_LJ_MULTI}.get(potential, potential)
+        pot = _POT.get(potential, potential)
         self._ok(self.lib.or_compute_forces(self.h, pot, flags), "compute_forces")
 
     def eam_pass(self, rho, rho2emb, force, flags=0):
@@ -174,14 +184,14 @@ class Oracle:
         return out
 
     def md_run(self, potential, nsteps, dt, rcut, skin, thermo=True):
-        pot = {"lj": OR_POT_LJ, "eam": OR_POT_EAM, "lj_multi": OR_POT_LJ_MULTI}[potential]
+        pot = _POT[potential]
         out = np.zeros((nsteps + 1) * 27) if thermo else None
         nreb = self._ok(self.lib.or_md_run(self.h, pot, nsteps, dt, rcut, skin, _d(out)), "md_run")
         return nreb, (out.reshape(nsteps + 1, 27) if thermo else None)
 
     def md_steps(self, potential, nsteps, dt, rcut, skin):
         """Continue for nsteps (no initial build/force); returns the rebuild count."""
-        pot = {"lj": OR_POT_LJ, "eam": OR_POT_EAM, "lj_multi": OR_POT_LJ_MULTI}[potential]
+        pot = _POT[potential]
         rc = self.lib.or_md_steps(self.h, pot, nsteps, dt, rcut, skin, None)
         if rc < 0:
             raise RuntimeError(f"oracle md_steps failed ({rc})")
@@ -259,3 +269,19 @@ def setfl_read(path):
         raise IOError(path)
     return dict(nel=ne, nrho=nrho.value, drho=drho.value, nr=nr.value, dr=dr.value, rc=rc.value,
                 frho=fr.reshape(ne, -1), rhor=rr.reshape(ne, -1), z2r=zr.reshape(-1, nr.value), masses=m)
+
+
+def johnson_eval(params, r):
+    """rho(r), rho'(r), phi(r), phi'(r) of the Johnson functions (johnson.h:59-124)."""
+    p = np.ascontiguousarray(params, dtype=np.float64)
+    out = np.zeros(4)
+    load().or_johnson_eval(_d(p), float(r), _d(out))
+    return out
+
+
+def johnson_embed(params, rho):
+    """F(rho), F'(rho) of the Johnson embedding (johnson.h:131-167)."""
+    p = np.ascontiguousarray(params, dtype=np.float64)
+    F, Fp = C.c_double(), C.c_double()
+    load().or_johnson_embed(_d(p), float(rho), C.byref(F), C.byref(Fp))
+    return F.value, Fp.value

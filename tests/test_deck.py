@@ -133,6 +133,12 @@ class _StubSim:
     def lj_compute_force(self, **kw):
         self._log("lj")
 
+    def johnson_force(self, rcut, parameters, compute_virial=False):
+        from exastamp_amd.operators import JOHNSON_PARAMETERS
+        from exastamp_amd.units import parse_quantity
+        self.johnson_args = (rcut, [parse_quantity(parameters[k]) for k in JOHNSON_PARAMETERS])
+        self._log("johnson")
+
     def push_f_v_r(self, dt, dt_scale=1.0, xform_mode=None):
         self._log("push_f_v_r")
 
@@ -201,3 +207,17 @@ def test_deck_block_conditions():
     assert r.condition("flag") and not r.condition("not flag")
     assert not r.condition("off") and r.condition("not off")
     assert r.condition("not missing") and r.condition(True)
+
+
+def test_johnson_deck_reaches_the_operator(monkeypatch):
+    """The reference's Johnson deck: rcut 6.1 A published to rcut_max (6.1 + rcut_inc
+    1 A = neighbour distance), johnson_force called once per force evaluation, its 19
+    parameters converted with their units (fe, rhoe in eV/A; energies in eV)."""
+    from exastamp_amd import units
+    r, calls = _stub_run(monkeypatch, DECKS / "eam_johnson" / "single_specy.msp", 3, ())
+    assert calls.count("johnson") == 4                     # initial forces + 3 steps
+    rc, p = r.sim.johnson_args
+    assert rc == 6.1
+    assert p[0] == 2.860082 and p[1] == 3.08634 * units.EV and p[3] == 8.489528
+    assert p[13] == -5.14 * units.EV and p[18] == 0.848843 * units.EV
+    assert float(r.get("rcut_max")) == 6.1

@@ -1,0 +1,92 @@
+"""johnson_force (analytic Johnson EAM, single-material template) through the C ABI
+against the oracle.
+
+The parameters are those of the reference's Ta deck (tests/golden/decks/eam_johnson,
+copied from data/regression_new/potentials/eam/eam_johnson/single_specy.msp), taken
+with their units.  A jittered BCC Ta crystal: forces within 1e-11 of rms|F|, energy
+and virial 1e-12 (both FP64, full list, summation order alike); the form is
+rejected with XS_MIXED; the deck itself runs 10 NVE steps and follows the oracle's
+run (1e-9), as the other regression decks do."""
+import numpy as np
+import pytest
+import yaml
+
+from conftest import GOLDEN
+from test_gpu_parity import frel, make_oracle, make_sim
+
+pytestmark = pytest.mark.gpu
+
+DECK = GOLDEN / "decks" / "eam_johnson" / "single_specy.msp"
+TA = 180.95
+
+
+def _params():
+    with open(DECK) as f:
+        return yaml.safe_load(f)["johnson_force"]["parameters"]
+
+
+def _oracle_params():
+    from exastamp_amd.operators import JOHNSON_PARAMETERS
+    from exastamp_amd.units import parse_quantity
+    return np.array([parse_quantity(_params()[k]) for k in JOHNSON_PARAMETERS])
+
+
+def _config(seed=5):
+    from exastamp_amd import inputs
+    n, a = 8, 3.3
+    g = np.arange(n)
+    iz, iy, ix = np.meshgrid(g, g, g, indexing="ij")
+    c = np.stack([ix.ravel(), iy.ravel(), iz.ravel()], 1).astype(float)
+    r = np.concatenate([c, c + 0.5], 0) * a + inputs.rng(seed).normal(0.0, 0.05, (2 * n ** 3, 3))
+    return inputs.Config(r, np.zeros_like(r), np.zeros(len(r), np.uint8), np.zeros((1, 3)),
+                         np.full((1, 3), n * a), (1, 1, 1))
+
+
+def test_johnson_force_matches_oracle(oracle_mod):
+    from exastamp_amd._lib import XsError
+    cfg = _config()
+    sim = make_sim(cfg, [TA], ["Ta"])
+    sim.chunk_neighbors(rcut_max=6.1, rcut_inc=1.0)
+    sim.zero_force_energy()
+    sim.johnson_force(rcut="6.1 ang", parameters=_params(), compute_virial=True)
+    sim.force_to_accel()
+    o = make_oracle(oracle_mod, cfg, [TA])
+    o.set_johnson(_oracle_params(), 6.1)
+    o.build_neighbors(6.1, 1.0)
+    o.compute_forces("johnson")
+    ref = o.state()
+    print(f"johnson: {cfg.n} atoms, force err {frel(sim.forces(), o.forces()):.2e} of rms|F|")
+    assert frel(sim.forces(), o.forces()) <= 1e-11
+    assert abs(sim.ep().sum() - ref["ep"].sum()) <= 1e-12 * abs(ref["ep"].sum())
+    assert np.abs(sim.ep() - ref["ep"]).max() <= 1e-12 * np.abs(ref["ep"]).max()
+    vir, vref = sim.virial().sum(0), ref["virial"].sum(0)
+    assert np.abs(vir - vref).max() <= 1e-12 * np.abs(vref).max()
+    with pytest.raises(XsError):
+        sim.ctx.xs_johnson_force(None, 6.1, 0)
+    from exastamp_amd import _lib
+    p = _oracle_params()
+    with pytest.raises(XsError, match="FP64"):
+        sim.ctx.xs_johnson_force(_lib.dptr(p), 6.1, _lib.XS_ENERGY | _lib.XS_MIXED)
+    with pytest.raises(XsError, match="neighbour-list radius"):
+        sim.ctx.xs_johnson_force(_lib.dptr(p), 8.0, _lib.XS_ENERGY)
+    sim.close()
+
+
+def test_johnson_deck_matches_oracle(oracle_mod):
+    from exastamp_amd import deck, units
+    cfg = deck.load_deck(DECK, [GOLDEN], [(["global", "max_iteration"], "10"),
+                                          (["global", "simulation_thermostate_screen_frequency"], "10")])
+    r = deck.Runner(cfg, out=open("/dev/null", "w"))
+    recs = r.run()
+    assert [x.step for x in recs] == [0, 10]
+    assert recs[0].particles == 16000
+    o = oracle_mod.Oracle()
+    o.set_config(r.init_atoms, [m for _, m in r.species])
+    o.set_johnson(_oracle_params(), 6.1)
+    _, th = o.md_run("johnson", 10, 0.001, 6.1, 1.0)
+    for rec, row in ((recs[0], th[0]), (recs[1], th[10])):
+        pe = rec.potential_ev * units.EV * rec.particles
+        ke = rec.kinetic_ev * units.EV * rec.particles
+        print(f"step {rec.step}: pe {pe:.12e} vs {row[24]:.12e}")
+        assert abs(pe - row[24]) <= 1e-9 * abs(row[24])
+        assert abs(ke - row[21:24].sum()) <= 1e-9 * abs(row[21:24].sum())

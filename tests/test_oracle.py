@@ -431,3 +431,72 @@ def test_lj_multi_pair_closed_forms():
                 assert f[0, 0] == pytest.approx(de, rel=1e-12) and f[1, 0] == pytest.approx(-f[0, 0], rel=1e-15)
             else:
                 assert np.all(f == 0.0) and np.all(st["ep"] == 0.0)
+
+
+# ---------------------------------------------------------------------------
+# Johnson EAM (johnson.h:59-167) with the parameters of the reference's Ta deck
+# (data/regression_new/potentials/eam/eam_johnson/single_specy.msp), in eV and
+# eV/A as published (order re fe rhoe alpha beta A B kappa lambda Fn0..Fn3 F0..F3 Fo eta)
+# ---------------------------------------------------------------------------
+JOHNSON_TA = np.array([2.860082, 3.08634, 33.787168, 8.489528, 4.527748, 0.611679, 1.032101, 0.176977,
+                       0.353954, -5.103845, -0.405524, 1.112997, -3.585325, -5.14, 0.0, 1.640098, 0.221375,
+                       -5.141526, 0.848843])
+
+
+def test_johnson_derivatives_match_central_differences():
+    import pyoracle
+    p, h = JOHNSON_TA, 1e-6
+    for r in (2.3, 2.86, 3.3, 4.0, 5.5):
+        v, vp, vm = (pyoracle.johnson_eval(p, x) for x in (r, r + h, r - h))
+        assert v[1] == pytest.approx((vp[0] - vm[0]) / (2 * h), rel=1e-7, abs=1e-9)   # rho'
+        assert v[3] == pytest.approx((vp[2] - vm[2]) / (2 * h), rel=1e-7, abs=1e-9)   # phi'
+    rhoe = p[2]
+    for q in (0.5, 0.84, 0.9, 1.0, 1.1, 1.3, 2.0):   # the three branches of F
+        rho, hr = q * rhoe, 1e-6 * rhoe
+        F, Fp = pyoracle.johnson_embed(p, rho)
+        Fa, _ = pyoracle.johnson_embed(p, rho + hr)
+        Fb, _ = pyoracle.johnson_embed(p, rho - hr)
+        assert Fp == pytest.approx((Fa - Fb) / (2 * hr), rel=1e-6, abs=1e-9)
+
+
+def test_johnson_embedding_is_continuous_at_its_branch_edges():
+    """The published Ta parameters join the three pieces of F(rho) at 0.85 rhoe and
+    1.15 rhoe to 1e-6 of |F|: a wrong normalisation of either cubic's variable, or of
+    the power law's, breaks the join."""
+    import pyoracle
+    p = JOHNSON_TA
+    for edge in (0.85 * p[2], 1.15 * p[2]):
+        lo, _ = pyoracle.johnson_embed(p, edge * (1 - 1e-13))
+        hi, _ = pyoracle.johnson_embed(p, edge * (1 + 1e-13))
+        assert abs(lo - hi) <= 1e-6 * abs(hi)
+    # rho = rhoe: F = F0 (the minimum of the middle cubic when F1 = 0)
+    F, Fp = pyoracle.johnson_embed(p, p[2])
+    assert F == p[13] and Fp == 0.0
+
+
+def test_johnson_force_is_minus_energy_gradient():
+    import pyoracle
+    from exastamp_amd import inputs, units
+    r, t = _bcc(5, 3.3)
+    r = r + inputs.rng(11).normal(0.0, 0.05, r.shape)
+    p = JOHNSON_TA.copy()
+    p[[1, 2, 5, 6, 9, 10, 11, 12, 13, 14, 15, 16, 17]] *= units.EV   # eV, eV/A -> internal units
+
+    def run(rr):
+        o = pyoracle.Oracle(threads=4)
+        o.set_config(_cfg(rr, t, 16.5), [180.95])
+        o.set_johnson(p, 6.1)
+        o.build_neighbors(6.1, 0.5)
+        o.compute_forces("johnson")
+        return o
+
+    f = run(r).forces()
+    assert np.abs(f.sum(0)).max() < 1e-10 * np.abs(f).max() * len(f)
+    h = 1e-5
+    for i in (0, 77, 201):
+        for a in range(3):
+            rp, rm = r.copy(), r.copy()
+            rp[i, a] += h
+            rm[i, a] -= h
+            fd = -(run(rp).state()["ep"].sum() - run(rm).state()["ep"].sum()) / (2 * h)
+            assert f[i, a] == pytest.approx(fd, rel=1e-6, abs=1e-6 * np.abs(f).max())
