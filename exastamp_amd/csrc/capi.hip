@@ -351,7 +351,8 @@ xs_status xs_field_get(xs_ctx* h, int field, double* out)
       case XS_FIELD_EP: to_id_order(c, c.ep, out); break;
       case XS_FIELD_VIRIAL: to_id_order(c, c.vir, out, 9); break;
       case XS_FIELD_RHO_DEMB:
-        if (c.rho_state == RhoState::FpFloat || c.rho_state == RhoState::FpDouble) {
+        if (c.rho_state == RhoState::FpFloat || c.rho_state == RhoState::FpDouble ||
+            c.rho_state == RhoState::FpJohnson) {
           const std::vector<int64_t> id = from_device(c, c.id, c.n);
           const std::vector<Pos4> p = from_device(c, c.pos, c.n);
           for (int64_t k = 0; k < c.n; k++) out[id[k]] = p[k].w;
@@ -559,7 +560,8 @@ xs_status xs_eam_alloy_force(xs_ctx* h, double rcut, int eam_rho, int eam_rho2em
   });
 }
 
-xs_status xs_johnson_force(xs_ctx* h, const double params[19], double rcut, unsigned flags)
+xs_status xs_johnson_force(xs_ctx* h, const double params[19], double rcut, int johnson_emb, int johnson_force,
+                           unsigned flags)
 {
   return guarded(h, [&](Ctx& c) {
     require(params != nullptr, XS_E_ARG, "null Johnson parameters");
@@ -572,7 +574,7 @@ xs_status xs_johnson_force(xs_ctx* h, const double params[19], double rcut, unsi
     require(p.re > 0 && p.rhoe > 0, XS_E_ARG, "re and rhoe must be positive");
     require_list(c);
     require(rcut <= c.list_rcut + c.list_skin, XS_E_ARG, "rcut exceeds the neighbour-list radius");
-    launch_johnson(c, p, rcut, flags);
+    launch_johnson(c, p, rcut, johnson_emb != 0, johnson_force != 0, flags);
   });
 }
 

@@ -778,21 +778,24 @@ void launch_lj_multi(Ctx& c, unsigned flags)
   else lj_multi_dispatch<double>(c, flags);
 }
 
-void launch_johnson(Ctx& c, const JohnsonParams& p, double rcut, unsigned flags)
+void launch_johnson(Ctx& c, const JohnsonParams& p, double rcut, bool do_emb, bool do_force, unsigned flags)
 {
   c.drift_done = false;
+  if (do_force && !do_emb && c.rho_state != RhoState::FpJohnson)
+    throw Error(XS_E_STATE, "johnson_force_reuse_emb needs F'(rho) from johnson_emb at the current atom order");
   // F' of this potential replaces the position records' w: no eam_alloy state survives
-  c.rho_state = RhoState::None;
   c.rho_ckpt = false;
-  if (c.n == 0) return;
+  if (do_emb) c.rho_state = RhoState::FpJohnson;
+  if (c.n == 0 || !(do_emb || do_force)) return;
   ensure_global_list(c);
   const PairCommon pc = make_common(c, rcut);
   const unsigned g = pair_grid(c.n);
-  {
+  if (do_emb) {
     ProfScope prof(c.prof, c.stream, XS_PROF_EAM_RHO);
     k_johnson_rho<<<g, kBlock, 0, c.stream>>>(pc, p, c.rho.p, reinterpret_cast<double*>(c.pos.p), c.ep.p);
     XS_HIP(hipGetLastError());
   }
+  if (!do_force) return;
   const Outputs o = make_outputs(c, ForceEpilogue{});
   ProfScope prof(c.prof, c.stream, XS_PROF_EAM_FORCE);
   if (flags & XS_VIRIAL) k_johnson_force<true><<<g, kBlock, 0, c.stream>>>(pc, p, o);

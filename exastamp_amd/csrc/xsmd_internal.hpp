@@ -125,7 +125,7 @@ struct EamHost {
 
 // What the per-atom EAM scalar currently holds: rho (c.rho) after the density
 // pass alone, or F'(rho) (pos[i].w) after the embedding step.
-enum class RhoState { None, Rho, FpDouble, FpFloat };
+enum class RhoState { None, Rho, FpDouble, FpFloat, FpJohnson };
 
 // Which neighbour list is current: the global ELL list of (index, image)
 // entries, or the cell-block list of 16-bit halo slots (k_block.hip).
@@ -379,8 +379,9 @@ void launch_lj_multi(Ctx& c, unsigned flags);
 struct JohnsonParams {
   double re, fe, rhoe, alpha, beta, A, B, kappa, lambda, Fn0, Fn1, Fn2, Fn3, F0, F1, F2, F3, Fo, eta;
 };
-// johnson_force: both passes of the single-material EAM template, FP64, global list
-void launch_johnson(Ctx& c, const JohnsonParams& p, double rcut, unsigned flags);
+// johnson_force / johnson_emb / johnson_force_reuse_emb: the passes of the
+// single-material EAM template, FP64, global list
+void launch_johnson(Ctx& c, const JohnsonParams& p, double rcut, bool do_emb, bool do_force, unsigned flags);
 void launch_lj_multi_block(Ctx& c, unsigned flags);
 // LJ on the cell-block list (k_block.hip)
 void launch_lj_block(Ctx& c, double eps, double sigma, double rcut, unsigned flags, const ForceEpilogue& ep);

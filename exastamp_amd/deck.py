@@ -803,12 +803,13 @@ def _op_eam_alloy_force(r: Runner, p):
                           compute_virial=bool(p.get("compute_virial", False)), mixed=r.mixed)
 
 
-def _op_johnson(r: Runner, p):
+def _op_johnson(r: Runner, p, op="johnson_force"):
     rc = parse_quantity(p["rcut"])
     _publish_rcut(r, rc)
     if r.dry:
         return
-    r.sim.johnson_force(rcut=rc, parameters=p["parameters"], compute_virial=bool(p.get("compute_virial", False)))
+    kw = {} if op == "johnson_emb" else {"compute_virial": bool(p.get("compute_virial", False))}
+    getattr(r.sim, op)(rcut=rc, parameters=p["parameters"], **kw)
 
 
 def _op_zero(r: Runner, p):
@@ -925,6 +926,9 @@ _OPS = {
     "eam_alloy_init": _op_eam_alloy_init,
     "eam_alloy_force": _op_eam_alloy_force,
     "johnson_force": _op_johnson,
+    "johnson_emb": lambda r, p: _op_johnson(r, p, "johnson_emb"),
+    "johnson_force_reuse_emb": lambda r, p: _op_johnson(r, p, "johnson_force_reuse_emb"),
+    "johnson_init": lambda r, p: None,
     "zero_force_energy": _op_zero,
     "force_to_accel": _op_accel,
     "push_f_v_r": _op_push_f_v_r,

@@ -241,7 +241,21 @@ class Simulation:
 
     def johnson_force(self, rcut, parameters, compute_virial=False, **_slots):
         """johnson_force{rcut, parameters: {re, fe, rhoe, alpha, beta, A, B, kappa, lambda,
-        Fn0..Fn3, F0..F3, Fo, eta}}: analytic Johnson EAM through the single-material
+        Fn0..Fn3, F0..F3, Fo, eta}} (registered at eam_potential.cu:189): both passes."""
+        self._johnson(rcut, parameters, True, True, compute_virial)
+
+    def johnson_emb(self, rcut, parameters, **_slots):
+        """johnson_emb (eam_potential.cu:190): the embedding pass only; F(rho_i) into ep,
+        F'(rho_i) kept for johnson_force_reuse_emb."""
+        self._johnson(rcut, parameters, True, False, False)
+
+    def johnson_force_reuse_emb(self, rcut, parameters, compute_virial=False, **_slots):
+        """johnson_force_reuse_emb (eam_potential.cu:191): the force pass on the F'(rho_i)
+        of the last johnson_emb (an error if the atoms were reordered since)."""
+        self._johnson(rcut, parameters, False, True, compute_virial)
+
+    def _johnson(self, rcut, parameters, emb, force, compute_virial):
+        """The Johnson operators: analytic Johnson EAM through the single-material
         template (eam_potential.cu:62-193, johnson.h).  Every parameter is converted with
         its unit, as the reference's YAML decoder does (johnson.h:185-205).  Energy is
         always accumulated (EmbOp and ForceOp add to ep unconditionally); the virial when
@@ -252,7 +266,8 @@ class Simulation:
         except KeyError as e:
             raise ValueError(f"johnson_force: parameter {e.args[0]} missing") from None
         self.rcut_max = max(self.rcut_max, rc)
-        self.ctx.xs_johnson_force(_lib.dptr(p), rc, XS_ENERGY | (XS_VIRIAL if compute_virial else 0))
+        self.ctx.xs_johnson_force(_lib.dptr(p), rc, int(emb), int(force),
+                                  XS_ENERGY | (XS_VIRIAL if compute_virial else 0))
 
     def eam_alloy_force(self, rcut, eam_rho=True, eam_rho2emb=True, eam_ghost=True, eam_force=True,
                         eam_symmetry=False, compute_virial=False, trigger_thermo_state=False,

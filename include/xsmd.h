@@ -171,10 +171,14 @@ xs_status xs_eam_alloy_force(xs_ctx* ctx, double rcut, int eam_rho, int eam_rho2
  * analytic functions of johnson.h:59-167): both passes, FP64, full list.
  * params: the 19 EamJohnsonParameters in internal units, in the order
  *   re fe rhoe alpha beta A B kappa lambda Fn0 Fn1 Fn2 Fn3 F0 F1 F2 F3 Fo eta.
- * Adds F(rho_i) + sum phi/2 to ep and the force to the force accumulators always
- * (EmbOp and ForceOp accumulate energy unconditionally); the virial with XS_VIRIAL.
- * XS_MIXED and XS_SYMMETRIC are rejected.  Leaves no eam_alloy rho/F' state. */
-xs_status xs_johnson_force(xs_ctx* ctx, const double params[19], double rcut, unsigned flags);
+ * johnson_emb runs EmbOp (adds F(rho_i) to ep, keeps F'(rho_i)); johnson_force runs
+ * ForceOp (adds the force, sum phi/2 to ep, the virial with XS_VIRIAL): both = the
+ * johnson_force operator, emb only = johnson_emb, force only = johnson_force_reuse_emb
+ * (XS_E_STATE unless johnson_emb ran at the current atom order).  Energy is always
+ * accumulated, as EmbOp and ForceOp do.  XS_MIXED and XS_SYMMETRIC are rejected.
+ * Leaves no eam_alloy rho/F' state. */
+xs_status xs_johnson_force(xs_ctx* ctx, const double params[19], double rcut, int johnson_emb,
+                           int johnson_force, unsigned flags);
 /* push_f_v_r (config_numerical_schemes.msp:23-25): r += v*dt + a*dt^2/2,
  * dt = dt * dt_scale. */
 xs_status xs_push_f_v_r(xs_ctx* ctx, double dt, double dt_scale);

@@ -62,14 +62,43 @@ def test_johnson_force_matches_oracle(oracle_mod):
     vir, vref = sim.virial().sum(0), ref["virial"].sum(0)
     assert np.abs(vir - vref).max() <= 1e-12 * np.abs(vref).max()
     with pytest.raises(XsError):
-        sim.ctx.xs_johnson_force(None, 6.1, 0)
+        sim.ctx.xs_johnson_force(None, 6.1, 1, 1, 0)
     from exastamp_amd import _lib
     p = _oracle_params()
     with pytest.raises(XsError, match="FP64"):
-        sim.ctx.xs_johnson_force(_lib.dptr(p), 6.1, _lib.XS_ENERGY | _lib.XS_MIXED)
+        sim.ctx.xs_johnson_force(_lib.dptr(p), 6.1, 1, 1, _lib.XS_ENERGY | _lib.XS_MIXED)
     with pytest.raises(XsError, match="neighbour-list radius"):
-        sim.ctx.xs_johnson_force(_lib.dptr(p), 8.0, _lib.XS_ENERGY)
+        sim.ctx.xs_johnson_force(_lib.dptr(p), 8.0, 1, 1, _lib.XS_ENERGY)
     sim.close()
+
+
+def test_johnson_split_passes_equal_the_operator():
+    """johnson_emb then johnson_force_reuse_emb give johnson_force's results bit for bit;
+    the force-only pass is refused without a johnson_emb at the current atom order, and
+    an eam_alloy force-only pass is refused on Johnson's F'."""
+    from exastamp_amd._lib import XsError
+    cfg = _config(seed=8)
+    out = []
+    for split in (False, True):
+        sim = make_sim(cfg, [TA], ["Ta"])
+        sim.chunk_neighbors(rcut_max=6.1, rcut_inc=1.0)
+        sim.zero_force_energy()
+        if split:
+            with pytest.raises(XsError, match="johnson_emb"):
+                sim.johnson_force_reuse_emb(rcut=6.1, parameters=_params())
+            sim.johnson_emb(rcut=6.1, parameters=_params())
+            sim.johnson_force_reuse_emb(rcut=6.1, parameters=_params(), compute_virial=True)
+        else:
+            sim.johnson_force(rcut=6.1, parameters=_params(), compute_virial=True)
+        out.append((sim.forces(), sim.ep(), sim.virial()))
+        if split:
+            from exastamp_amd import potentials
+            sim.eam_alloy_init(potentials.johnson_tables(nr=500, nrho=500))
+            with pytest.raises(XsError, match="eam_force needs"):
+                sim.eam_alloy_force(rcut=4.9, eam_rho=False, eam_rho2emb=False)
+        sim.close()
+    for a, b in zip(*out):
+        assert np.array_equal(a, b)
 
 
 def test_johnson_deck_matches_oracle(oracle_mod):
