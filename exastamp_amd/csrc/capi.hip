@@ -560,6 +560,17 @@ xs_status xs_eam_alloy_force(xs_ctx* h, double rcut, int eam_rho, int eam_rho2em
   });
 }
 
+xs_status xs_eam_alloy_types(xs_ctx* h, unsigned enabled)
+{
+  return guarded(h, [&](Ctx& c) {
+    if (c.eam_types != enabled) {
+      c.eam_types = enabled;
+      c.rho_state = RhoState::None;  // rho and F' of the previous mask no longer hold
+      c.rho_ckpt = false;
+    }
+  });
+}
+
 xs_status xs_johnson_force(xs_ctx* h, const double params[19], double rcut, int johnson_emb, int johnson_force,
                            unsigned flags)
 {

@@ -54,7 +54,14 @@ struct AdjTab {
   int8_t t2rhor[8][8];
   int8_t t2z2r[8][8];
   int8_t t2frho[8];
+  uint32_t types;  // types slot of eam_alloy_force: bit t set = species t takes part
 };
+
+template <class Real>
+__device__ __forceinline__ bool adj_on(const AdjTab<Real>& t, int type)
+{
+  return (t.types >> type) & 1u;
+}
 
 struct AdjArgs {
   int64_t n, npad;
@@ -156,7 +163,8 @@ __global__ void __launch_bounds__(256) k_adj_b1(AdjArgs a, AdjTab<Real> t)
   const Pos4 pi = load_pos(a.pos, i);
   const Pos4 gi = load_pos(a.gvec, i);
   double drho = 0.0;
-  const int cnt = a.nnb[i];
+  const bool oni = !MULTI || adj_on(t, ti);
+  const int cnt = oni ? a.nnb[i] : 0;
   for (int k = 0; k < cnt; k++) {
     const uint32_t e = a.nbr[(int64_t)k * a.npad + i];
     const uint32_t j = xsmd_nbr_index(e);
@@ -171,6 +179,7 @@ __global__ void __launch_bounds__(256) k_adj_b1(AdjArgs a, AdjTab<Real> t)
     // s_ij = u_ij . (g_i - g_j), u_ij = -(dx,dy,dz)/r
     const double s = (dx * (gj.x - gi.x) + dy * (gj.y - gi.y) + dz * (gj.z - gi.z)) / r;
     const int tj = MULTI ? a.type[j] : 0;
+    if (MULTI && !adj_on(t, tj)) continue;
     int m;
     double p;
     bool cl;
@@ -193,6 +202,10 @@ __global__ void __launch_bounds__(256) k_adj_b1(AdjArgs a, AdjTab<Real> t)
   const double over = rho > t.rhomax ? rho - t.rhomax : 0.0;
   double o[4];
   coeff4(ge * conv, conv * (ge * over - drho), p, t.rdrho, o);
+  if (!oni) {  // a disabled species has no embedding term (lambda and drho are 0 already)
+    o[0] = o[1] = o[2] = o[3] = 0.0;
+    a.lam[i] = 0.0;
+  }
   store_pos(reinterpret_cast<Pos4*>(a.fc), i, Pos4{o[0], o[1], o[2], o[3]});
   a.fm[i] = m;
 }
@@ -228,7 +241,7 @@ __device__ __forceinline__ void adj_b2_atom(const AdjArgs& a, const AdjTab<Real>
   const double ge = a.gE[rep];
   const double conv = t.conv;
   const double rdr = t.rdr;
-  const int cnt = a.nnb[i];
+  const int cnt = (!MULTI || adj_on(t, ti)) ? a.nnb[i] : 0;
   for (int k = 0; k < cnt; k++) {
     const uint32_t e = a.nbr[(int64_t)k * a.npad + i];
     const uint32_t j = xsmd_nbr_index(e);
@@ -242,6 +255,7 @@ __device__ __forceinline__ void adj_b2_atom(const AdjArgs& a, const AdjTab<Real>
     const double r = sqrt(d2);
     const double s = (dx * (gj.x - gi.x) + dy * (gj.y - gi.y) + dz * (gj.z - gi.z)) / r;
     const int tj = MULTI ? a.type[j] : 0;
+    if (MULTI && !adj_on(t, tj)) continue;
     int m;
     double p;
     bool cl;
@@ -308,6 +322,7 @@ AdjTab<double> adj_tab<double>(Ctx& c)
   std::memcpy(t.t2rhor, c.eam.t2rhor, sizeof(t.t2rhor));
   std::memcpy(t.t2z2r, c.eam.t2z2r, sizeof(t.t2z2r));
   std::memcpy(t.t2frho, c.eam.t2frho, sizeof(t.t2frho));
+  t.types = c.eam_types;
   return t;
 }
 
@@ -327,6 +342,7 @@ AdjTab<float> adj_tab<float>(Ctx& c)
   std::memcpy(t.t2rhor, c.eam.t2rhor, sizeof(t.t2rhor));
   std::memcpy(t.t2z2r, c.eam.t2z2r, sizeof(t.t2z2r));
   std::memcpy(t.t2frho, c.eam.t2frho, sizeof(t.t2frho));
+  t.types = c.eam_types;
   return t;
 }
 
@@ -367,7 +383,7 @@ void adj_run(Ctx& c, const AdjArgs& a)
   const int nel = e.nel, nz = nel * (nel + 1) / 2;
   const int64_t rows_f = e.nrho + 1, rows_r = e.nr + 1;
   const unsigned g = (grid_for(c.n) + 7u) / 8u * 8u;
-  const bool multi = nel > 1;
+  const bool multi = eam_multi(c);
   if (multi) k_adj_b1<Real, true><<<g, kBlock, 0, c.stream>>>(a, t);
   else k_adj_b1<Real, false><<<g, kBlock, 0, c.stream>>>(a, t);
   XS_HIP(hipGetLastError());

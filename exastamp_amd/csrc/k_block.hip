@@ -812,7 +812,8 @@ __global__ void __launch_bounds__(kPassThreads, kPassWaves<false>) k_block_eam_r
     if (!central_at(l.hc, l.slot_cell, l.central_local, ncentral, q, ca)) break;
     const Pos4 pi = l.rec.get(ca.local);
     const int ti = MULTI ? l.ltype[ca.local] : 0;
-    const int cnt = a.nnb[ca.global];
+    const bool oni = !MULTI || pm::eam_on(t, ti);
+    const int cnt = oni ? a.nnb[ca.global] : 0;
     double rho = 0.0;
     uint2 wnext = lword(a, ca.global, half);
     for (int k0 = half; k0 < cnt; k0 += 2 * kBBatch) {
@@ -822,6 +823,10 @@ __global__ void __launch_bounds__(kPassThreads, kPassWaves<false>) k_block_eam_r
       lslots(w, ca.local, k0, cnt, c);
       LNb nb[kBBatch];
       lgather<kBBatch>(a, l.rec, c, k0, cnt, pi, nb);
+      if constexpr (MULTI) {
+#pragma unroll
+        for (int u = 0; u < kBBatch; u++) nb[u].in = nb[u].in && pm::eam_on(t, l.ltype[nb[u].c]);
+      }
       // a batch with no pair inside rc on any lane of the wave (the tails of the
       // distance-sorted lists) skips the table reads; skipped terms would add 0
       bool any = false;
@@ -841,6 +846,7 @@ __global__ void __launch_bounds__(kPassThreads, kPassWaves<false>) k_block_eam_r
       if constexpr (EMB) {
         Real F, Fp;
         pm::embed(t, ti, rho, F, Fp);
+        if (!oni) F = Fp = Real(0);  // a disabled species keeps rho_dEmb = 0 and adds no F
         pos_w[4 * ca.global + 3] = (double)Fp;
         if constexpr (ENERGY) {
           if constexpr (SET_EP) ep[ca.global] = (double)F;
@@ -871,7 +877,7 @@ __global__ void __launch_bounds__(kPassThreads, kPassWaves<VIRIAL>) k_block_eam_
     const Pos4 pi = l.rec.get(ca.local);
     const int ti = MULTI ? l.ltype[ca.local] : 0;
     const Real fpi = (Real)pi.w;
-    const int cnt = a.nnb[ca.global];
+    const int cnt = (!MULTI || pm::eam_on(t, ti)) ? a.nnb[ca.global] : 0;
     double fx = 0.0, fy = 0.0, fz = 0.0, e = 0.0;
     double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     uint2 wnext = lword(a, ca.global, half);
@@ -887,6 +893,10 @@ __global__ void __launch_bounds__(kPassThreads, kPassWaves<VIRIAL>) k_block_eam_
       for (int sub = 0; sub < kBBatch; sub += 2) {
         LNb nb[2];
         lgather<2>(a, l.rec, c + sub, k0 + 2 * sub, cnt, pi, nb);
+        if constexpr (MULTI) {
+          nb[0].in = nb[0].in && pm::eam_on(t, l.ltype[nb[0].c]);
+          nb[1].in = nb[1].in && pm::eam_on(t, l.ltype[nb[1].c]);
+        }
         if (!__any(nb[0].in || nb[1].in)) continue;  // as in the density pass
 #pragma unroll
         for (int u = 0; u < 2; u++) {
@@ -1238,7 +1248,7 @@ void launch_eam_block(Ctx& c, double rcut, bool do_rho, bool do_emb, bool do_for
                       const ForceEpilogue& epi)
 {
   const bool mixed = (flags & XS_MIXED) != 0;
-  const bool multi = c.eam.nel > 1;
+  const bool multi = eam_multi(c);
   if (do_emb && !do_rho) {
     const ListKind keep = c.list_kind;
     c.list_kind = ListKind::Global;

@@ -133,17 +133,19 @@ __global__ void __launch_bounds__(256) k_eam_rho(PairCommon pc, EamTabDev<Real> 
   const int64_t i = logical_block() * blockDim.x + threadIdx.x;
   if (i >= pc.n) return;
   const int ti = MULTI ? pc.type[i] : 0;
+  const bool oni = !MULTI || eam_on(t, ti);
   double rho = 0.0;
   if constexpr (RHO) {
     const double* L = pc.box[pc.rep ? pc.rep[i] : 0].L;
     const Pos4 pi = load_pos(pc.pos, i);
-    const int cnt = pc.nnb[i];
+    const int cnt = oni ? pc.nnb[i] : 0;
     for (int k0 = 0; k0 < cnt; k0 += kBatch) {
       Nb nb[kBatch];
       fetch<kBatch>(pc, i, k0, cnt, pi, L, nb);
 #pragma unroll
       for (int u = 0; u < kBatch; u++) {
         const int tj = MULTI ? pc.type[nb[u].j] : 0;
+        if constexpr (MULTI) nb[u].in = nb[u].in && eam_on(t, tj);
         const Real val = eam_rho<Real, MULTI>(t, pair, ti, tj, nb[u].in ? nb[u].d2 : 1.0);
         rho += nb[u].in ? (double)val : 0.0;
       }
@@ -155,6 +157,7 @@ __global__ void __launch_bounds__(256) k_eam_rho(PairCommon pc, EamTabDev<Real> 
   if constexpr (EMB) {
     Real F, Fp;
     embed(t, ti, rho, F, Fp);
+    if (!oni) F = Fp = Real(0);  // a disabled species keeps rho_dEmb = 0 and adds no F
     pos_w[4 * i + 3] = (double)Fp;
     if constexpr (ENERGY) {
       if constexpr (SET_EP) ep[i] = (double)F;
@@ -176,13 +179,14 @@ __global__ void __launch_bounds__(256) k_eam_force(PairCommon pc, EamTabDev<Real
   const Real fpi = (Real)pi.w;
   double fx = 0.0, fy = 0.0, fz = 0.0, e = 0.0;
   double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-  const int cnt = pc.nnb[i];
+  const int cnt = (!MULTI || eam_on(t, ti)) ? pc.nnb[i] : 0;
   for (int k0 = 0; k0 < cnt; k0 += kBatch) {
     Nb nb[kBatch];
     fetch<kBatch>(pc, i, k0, cnt, pi, L, nb);
 #pragma unroll
     for (int u = 0; u < kBatch; u++) {
       const int tj = MULTI ? pc.type[nb[u].j] : 0;
+      if constexpr (MULTI) nb[u].in = nb[u].in && eam_on(t, tj);
       Real fp_r, phi;
       eam_pair<Real, MULTI>(t, pair, ti, tj, fpi, (Real)nb[u].w, nb[u].in ? nb[u].d2 : 1.0, fp_r, phi);
       const double fpair = nb[u].in ? (double)fp_r : 0.0;
@@ -469,7 +473,7 @@ __global__ void __launch_bounds__(256) k_eam_rho_sym(PairCommon pc, EamTabDev<Re
   const int ti = MULTI ? pc.type[i] : 0;
   const double* L = pc.box[pc.rep ? pc.rep[i] : 0].L;
   const Pos4 pi = load_pos(pc.pos, i);
-  const int cnt = pc.nnb[i];
+  const int cnt = (!MULTI || eam_on(t, ti)) ? pc.nnb[i] : 0;
   double rho = 0.0;
   for (int k0 = 0; k0 < cnt; k0 += kBatch) {
     Nb nb[kBatch];
@@ -477,6 +481,7 @@ __global__ void __launch_bounds__(256) k_eam_rho_sym(PairCommon pc, EamTabDev<Re
 #pragma unroll
     for (int u = 0; u < kBatch; u++) {
       const int tj = MULTI ? pc.type[nb[u].j] : 0;
+      if constexpr (MULTI) nb[u].in = nb[u].in && eam_on(t, tj);
       const double d2 = nb[u].in ? nb[u].d2 : 1.0;
       const Real to_i = eam_rho<Real, MULTI>(t, pair, ti, tj, d2);
       Real to_j = to_i;
@@ -533,13 +538,14 @@ __global__ void __launch_bounds__(256) k_eam_force_sym(PairCommon pc, EamTabDev<
   const Real fpi = (Real)pi.w;
   double fx = 0.0, fy = 0.0, fz = 0.0, e = 0.0;
   double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-  const int cnt = pc.nnb[i];
+  const int cnt = (!MULTI || eam_on(t, ti)) ? pc.nnb[i] : 0;
   for (int k0 = 0; k0 < cnt; k0 += kBatch) {
     Nb nb[kBatch];
     fetch<kBatch>(pc, i, k0, cnt, pi, L, nb);
 #pragma unroll
     for (int u = 0; u < kBatch; u++) {
       const int tj = MULTI ? pc.type[nb[u].j] : 0;
+      if constexpr (MULTI) nb[u].in = nb[u].in && eam_on(t, tj);
       Real fp_r, phi;
       eam_pair<Real, MULTI>(t, pair, ti, tj, fpi, (Real)nb[u].w, nb[u].in ? nb[u].d2 : 1.0, fp_r, phi);
       if (nb[u].in) {
@@ -686,6 +692,7 @@ EamTabDev<double> make_eam_tab<double>(Ctx& c)
   std::memcpy(t.t2rhor, c.eam.t2rhor, sizeof(t.t2rhor));
   std::memcpy(t.t2z2r, c.eam.t2z2r, sizeof(t.t2z2r));
   std::memcpy(t.t2frho, c.eam.t2frho, sizeof(t.t2frho));
+  t.types = c.eam_types;
   return t;
 }
 
@@ -705,6 +712,7 @@ EamTabDev<float> make_eam_tab<float>(Ctx& c)
   std::memcpy(t.t2rhor, c.eam.t2rhor, sizeof(t.t2rhor));
   std::memcpy(t.t2z2r, c.eam.t2z2r, sizeof(t.t2z2r));
   std::memcpy(t.t2frho, c.eam.t2frho, sizeof(t.t2frho));
+  t.types = c.eam_types;
   return t;
 }
 
@@ -815,7 +823,7 @@ void launch_eam(Ctx& c, double rcut, bool do_rho, bool do_emb, bool do_force, un
     if (c.rho_state != want)
       throw Error(XS_E_STATE, "eam_force needs F'(rho) from eam_rho2emb in the same precision");
   }
-  const bool multi = c.eam.nel > 1;
+  const bool multi = eam_multi(c);
   if (flags & XS_SYMMETRIC) {
     if (epi.fused) throw Error(XS_E_ARG, "the Newton-symmetric form is an operator form");
     if (mixed) {

@@ -183,6 +183,15 @@ __device__ __forceinline__ Real dcubic(const Q4<Real>& c, Real p)
   return (Real(3) * c.a * p + Real(2) * c.b) * p + c.c;
 }
 
+// types slot of eam_alloy_force (m_pair_enabled, eam_potential_multimat.cu:154-166):
+// species `type` takes part when its bit is set; a pair counts when both species do,
+// and an atom of a disabled species keeps rho = 0, F' = 0 and gets no force.
+template <class Real>
+__device__ __forceinline__ bool eam_on(const EamTabDev<Real>& t, int type)
+{
+  return (t.types >> type) & 1u;
+}
+
 // F(rho), F'(rho) in eV-converted internal units, with the linear extrapolation
 // of F above rhomax (eam_alloy.h:289-313).
 template <class Real>

@@ -105,6 +105,8 @@ struct EamTabDev {
   int8_t t2rhor[8][8] = {};
   int8_t t2z2r[8][8] = {};
   int8_t t2frho[8] = {};
+  // types slot of eam_alloy_force: bit t set = species t takes part (alloy kernels)
+  uint32_t types = ~0u;
 };
 
 struct EamHost {
@@ -311,6 +313,9 @@ struct Ctx {
   DBuf<double> ljm_d, ljm_rcut;
   bool has_eam = false;
   EamHost eam;
+  // types slot of eam_alloy_force (eam_potential_multimat.cu:90,154-166) as enable
+  // bits per species; ~0u = every species (the empty list)
+  uint32_t eam_types = ~0u;
 
   // EAM adjoint scratch: coefficient gradients, lambda_i, dL/dF records, inputs
   DBuf<double> adj_g, adj_lam, adj_in;
@@ -322,6 +327,10 @@ struct Ctx {
 
   ~Ctx();
 };
+
+// The alloy kernels (per-element tables, type maps, the types mask) serve several
+// elements, and a single element whose only species the types slot disables.
+inline bool eam_multi(const Ctx& c) { return c.eam.nel > 1 || (c.eam_types & 1u) == 0; }
 
 // ---- launches (k_neighbors.hip) ----
 // prefer_block: build the cell-block list when its LDS footprint allows

@@ -800,16 +800,30 @@ def _op_eam_alloy_force(r: Runner, p):
     r.sim.eam_alloy_force(rcut=rc, eam_rho=flag("eam_rho"), eam_rho2emb=flag("eam_rho2emb"),
                           eam_ghost=flag("eam_ghost"), eam_force=flag("eam_force"),
                           eam_symmetry=bool(p.get("eam_symmetry", False)), trigger_thermo_state=_trigger(r),
-                          compute_virial=bool(p.get("compute_virial", False)), mixed=r.mixed)
+                          compute_virial=bool(p.get("compute_virial", False)), mixed=r.mixed,
+                          types=p.get("types") or None)
+
+
+def _op_johnson_init(r: Runner, p):
+    """johnson_init (eam_potential.cu:62-70): its `parameters` slot is an OUTPUT, read by
+    the Johnson force operators through the shared slot."""
+    if p.get("parameters") is None:
+        raise DeckError("johnson_init: parameters missing")
+    r.set("parameters", p["parameters"])
 
 
 def _op_johnson(r: Runner, p, op="johnson_force"):
     rc = parse_quantity(p["rcut"])
     _publish_rcut(r, rc)
+    params = p.get("parameters")
+    if params is None:
+        params = r.get("parameters")   # set by johnson_init (INPUT_OUTPUT slot, eam_potential.cu:92)
+    if not isinstance(params, dict):
+        raise DeckError(f"{op}: no parameters (give them to the operator or run johnson_init first)")
     if r.dry:
         return
     kw = {} if op == "johnson_emb" else {"compute_virial": bool(p.get("compute_virial", False))}
-    getattr(r.sim, op)(rcut=rc, parameters=p["parameters"], **kw)
+    getattr(r.sim, op)(rcut=rc, parameters=params, **kw)
 
 
 def _op_zero(r: Runner, p):
@@ -928,7 +942,7 @@ _OPS = {
     "johnson_force": _op_johnson,
     "johnson_emb": lambda r, p: _op_johnson(r, p, "johnson_emb"),
     "johnson_force_reuse_emb": lambda r, p: _op_johnson(r, p, "johnson_force_reuse_emb"),
-    "johnson_init": lambda r, p: None,
+    "johnson_init": _op_johnson_init,
     "zero_force_energy": _op_zero,
     "force_to_accel": _op_accel,
     "push_f_v_r": _op_push_f_v_r,

@@ -276,15 +276,23 @@ class Simulation:
         accepted and has no effect (one GPU holds every atom; images are resolved in
         the pair kernels).  eam_symmetry selects the Newton-symmetric evaluation
         (each pair once from a half list, the partner's share added atomically):
-        the same quantities up to FP64 summation order."""
-        if types:
-            unknown = [t for t in types if t not in self.species_names]
-            if unknown:
-                raise ValueError(f"eam_alloy_force: unknown types {unknown}")
+        the same quantities up to FP64 summation order.  types (eam_potential_multimat.cu:90,
+        154-166): an empty list means every species; otherwise only pairs of two listed
+        species contribute, and atoms of an unlisted species get rho = F' = 0 and no force."""
+        self.ctx.xs_eam_alloy_types(self._types_mask(types))
         rc = parse_quantity(rcut)
         self.rcut_max = max(self.rcut_max, rc)
         flags = self._flags(trigger_thermo_state, compute_virial, mixed) | (XS_SYMMETRIC if eam_symmetry else 0)
         self.ctx.xs_eam_alloy_force(rc, int(bool(eam_rho)), int(bool(eam_rho2emb)), int(bool(eam_force)), flags)
+
+    def _types_mask(self, types) -> int:
+        """types slot -> enable bits per species (bit t = species t); empty = all."""
+        if not types:
+            return 0xFFFFFFFF
+        unknown = [t for t in types if t not in self.species_names]
+        if unknown:
+            raise ValueError(f"eam_alloy_force: unknown types {unknown}")
+        return sum(1 << self.species_names.index(t) for t in set(types))
 
     def eam_adjoint(self, dL_dF, dL_dE, mixed=False, rcut=None):
         """dL/dtheta for the knots (setfl order: F, rho, z2r tables) given dL/dF

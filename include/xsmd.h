@@ -167,6 +167,14 @@ xs_status xs_eam_alloy_load(xs_ctx* ctx, const char* path, double* masses, int* 
  * (SymRhoOp / SymForceOp with NewtonSym, eam_force_op_multimat.h:114-343). */
 xs_status xs_eam_alloy_force(xs_ctx* ctx, double rcut, int eam_rho, int eam_rho2emb,
                              int eam_force, unsigned flags);
+/* The types slot of eam_alloy_force (eam_potential_multimat.cu:90, applied at
+ * :154-166 through m_pair_enabled and checked at eam_force_op_multimat.h:149-153,
+ * 180, 191, 303-308): bit t of `enabled` set = species t takes part.  A pair
+ * contributes when both of its species take part; an atom of a species that does not
+ * keeps rho = 0 and F'(rho) = 0, gets no embedding energy and no force.  ~0u (the
+ * default) = every species, the empty list.  Applies to every later
+ * xs_eam_alloy_force, xs_md_run and xs_eam_adjoint on the context. */
+xs_status xs_eam_alloy_types(xs_ctx* ctx, unsigned enabled);
 /* johnson_force (single-material EAM template eam_potential.cu:62-193 with the
  * analytic functions of johnson.h:59-167): both passes, FP64, full list.
  * params: the 19 EamJohnsonParameters in internal units, in the order

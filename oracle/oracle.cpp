@@ -230,6 +230,10 @@ struct or_ctx {
   std::vector<double> ljm_eps, ljm_sigma, ljm_rcut, ljm_ecut;
   bool has_eam = false;
   Eam eam;
+  // types slot of eam_alloy_force as per-species enable bits: a pair takes part when
+  // both species are enabled (m_pair_enabled, eam_potential_multimat.cu:154-166)
+  uint32_t eam_types = ~0u;
+  bool eam_on(int t) const { return (eam_types >> t) & 1u; }
   // johnson_force: EamJohnsonParameters (johnson.h:30-51), internal units
   bool has_johnson = false;
   double jp[19] = {};
@@ -785,12 +789,14 @@ void eam_rho_pass(or_ctx* c)
   for_each_cell(*c, [&](int64_t i) {
     double rho = 0.0;
     const int ta = c->type[i];
+    if (!c->eam_on(ta)) return;  // a disabled species keeps rho = 0 (eam_force_op_multimat.h:149)
     for (int64_t k = c->nb_off[i]; k < c->nb_off[i + 1]; k++) {
       double dr[3];
       c->pair_dr(i, c->nb[k], dr);
       const double d2 = xsmd_d2(dr[0], dr[1], dr[2]);
       if (d2 > rc2) continue;
       const int tb = c->type[xsmd_nbr_index(c->nb[k])];
+      if (!c->eam_on(tb)) continue;  // pair disabled (:151-153)
       const double r = std::sqrt(d2);
       rho += eam_rho_noderiv(c->eam, r, tb, ta);
     }
@@ -803,6 +809,7 @@ void eam_rho2emb_pass(or_ctx* c, bool energy)
 {
   for_each_cell(*c, [&](int64_t i) {
     const int ta = c->type[i];
+    if (!c->eam_on(ta)) return;  // Rho2EmbOp skips a disabled species (eam_force_op_multimat.h:180,191)
     if (energy) {
       double emb = 0., demb = 0.;
       eam_fembed(c->eam, c->rho[i], emb, demb, ta);
@@ -825,13 +832,15 @@ void eam_force_pass(or_ctx* c, int flags)
     double f[3] = {0., 0., 0.};
     double ep = 0.0;
     double vir[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    for (int64_t k = c->nb_off[i]; k < c->nb_off[i + 1]; k++) {
+    const int64_t kend = c->eam_on(ta) ? c->nb_off[i + 1] : c->nb_off[i];  // (eam_force_op_multimat.h:303)
+    for (int64_t k = c->nb_off[i]; k < kend; k++) {
       double dr[3];
       c->pair_dr(i, c->nb[k], dr);
       const double d2 = xsmd_d2(dr[0], dr[1], dr[2]);
       if (d2 > rc2) continue;
       const uint32_t j = xsmd_nbr_index(c->nb[k]);
       const int tb = c->type[j];
+      if (!c->eam_on(tb)) continue;  // (:308)
       const double r = std::sqrt(d2);
       double phi = 0.;
       double dr_fe[3] = {dr[0], dr[1], dr[2]};
@@ -854,6 +863,12 @@ void eam_force_pass(or_ctx* c, int flags)
 }
 
 }  // namespace
+
+int or_set_eam_types(or_ctx* c, unsigned mask)
+{
+  c->eam_types = mask;
+  return 0;
+}
 
 int or_eam_pass(or_ctx* c, int eam_rho, int eam_rho2emb, int eam_force, int flags)
 {

@@ -65,6 +65,12 @@ void or_johnson_embed(const double* params, double rho, double* F, double* Fp);
 int or_set_eam(or_ctx* c, int nel, int nrho, double drho, int nr, double dr, double rcut,
                const double* frho, const double* rhor, const double* z2r);
 
+/* types slot of eam_alloy_force (eam_potential_multimat.cu:90,154-166): bit t set =
+ * species t takes part; a pair counts when both of its species do, and an atom of a
+ * disabled species gets rho = 0, no embedding term and no force.  ~0u = every species
+ * (the empty list, the default). */
+int or_set_eam_types(or_ctx* c, unsigned mask);
+
 /* --- operators ----------------------------------------------------------- */
 /* wrap + sort into cells + build the full neighbour list with rlist = rcut + skin */
 int or_build_neighbors(or_ctx* c, double rcut, double skin);

@@ -44,6 +44,7 @@ def load():
             "or_johnson_embed": (None, [_D, C.c_double, _D, _D]),
             "or_set_eam": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_int, C.c_double, C.c_double,
                                      _D, _D, _D]),
+            "or_set_eam_types": (C.c_int, [C.c_void_p, C.c_uint]),
             "or_build_neighbors": (C.c_int, [C.c_void_p, C.c_double, C.c_double]),
             "or_compute_forces": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
             "or_eam_pass": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int]),
@@ -152,6 +153,10 @@ class Oracle:
         zr = np.ascontiguousarray(t.z2r, dtype=np.float64)
         self._ok(self.lib.or_set_eam(self.h, t.nel, t.nrho, t.drho, t.nr, t.dr, t.rc, _d(fr), _d(rr), _d(zr)),
                  "set_eam")
+
+    def set_eam_types(self, mask=0xFFFFFFFF):
+        """eam_alloy_force's types slot as enable bits per species (bit t = species t)."""
+        self._ok(self.lib.or_set_eam_types(self.h, mask), "set_eam_types")
 
     def build_neighbors(self, rcut, skin):
         self._ok(self.lib.or_build_neighbors(self.h, rcut, skin), "build_neighbors")

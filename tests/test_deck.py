@@ -221,3 +221,27 @@ def test_johnson_deck_reaches_the_operator(monkeypatch):
     assert p[0] == 2.860082 and p[1] == 3.08634 * units.EV and p[3] == 8.489528
     assert p[13] == -5.14 * units.EV and p[18] == 0.848843 * units.EV
     assert float(r.get("rcut_max")) == 6.1
+
+
+def test_johnson_init_supplies_the_parameters_slot(monkeypatch, tmp_path):
+    """johnson_init's parameters slot is an OUTPUT (eam_potential.cu:62-70) and the force
+    operator's an INPUT_OUTPUT (:92): a deck that sets the parameters through johnson_init,
+    rebound to a shared name as the eam_alloy decks do, and gives johnson_force only its
+    rcut, runs with those parameters; with neither source it is a deck error."""
+    import yaml
+    from exastamp_amd import deck, units
+    src = yaml.safe_load((DECKS / "eam_johnson" / "single_specy.msp").read_text())
+    params = src["johnson_force"].pop("parameters")
+    src["init_parameters"] = {"rebind": {"parameters": "johnson_parameters"},
+                              "body": [{"johnson_init": {"parameters": params}}]}
+    src["compute_force"] = {"rebind": {"parameters": "johnson_parameters"}, "body": ["johnson_force"]}
+    path = tmp_path / "johnson_init.msp"
+    path.write_text(yaml.safe_dump(src))
+    r, calls = _stub_run(monkeypatch, path, 2, ())
+    assert calls.count("johnson") == 3
+    rc, p = r.sim.johnson_args
+    assert rc == 6.1 and p[1] == 3.08634 * units.EV and p[18] == 0.848843 * units.EV
+    del src["init_parameters"]
+    path.write_text(yaml.safe_dump(src))
+    with pytest.raises(deck.DeckError, match="no parameters"):
+        _stub_run(monkeypatch, path, 2, ())

@@ -500,3 +500,175 @@ def test_johnson_force_is_minus_energy_gradient():
             rm[i, a] -= h
             fd = -(run(rp).state()["ep"].sum() - run(rm).state()["ep"].sum()) / (2 * h)
             assert f[i, a] == pytest.approx(fd, rel=1e-6, abs=1e-6 * np.abs(f).max())
+
+
+def _johnson_internal(p):
+    from exastamp_amd import units
+    q = np.array(p, dtype=float)
+    q[[1, 2, 5, 6, 9, 10, 11, 12, 13, 14, 15, 16, 17]] *= units.EV   # eV, eV/A -> internal units
+    return q
+
+
+def _bcc_shells(a, rmax):
+    """Distances of the BCC lattice sites within rmax of a site (brute force)."""
+    g = np.arange(-5, 6)
+    c = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3).astype(float)
+    pts = np.concatenate([c, c + 0.5]) * a
+    d = np.sqrt((pts ** 2).sum(1))
+    return d[(d > 0) & (d < rmax)]
+
+
+def test_johnson_special_values():
+    """Values the Zhou-Johnson-Wadley form fixes by hand, independent of the oracle's
+    arithmetic: at r = re the exponentials are 1, so rho(re) = fe / (1 + (1 - lambda)^20)
+    and phi(re) = A / (1 + (1 - kappa)^20) - B / (1 + (1 - lambda)^20); the lower cubic of
+    F(rho) at its edge 0.85 rhoe is Fn0; the power law at 2 rhoe is Fo (1 - eta ln 2) 2^eta
+    (johnson.h:59-167).  A swapped alpha/beta, kappa/lambda or fe/rhoe fails one of them."""
+    import pyoracle
+    p = JOHNSON_TA
+    re, fe, rhoe, A, B, kappa, lam = p[0], p[1], p[2], p[5], p[6], p[7], p[8]
+    Fn0, F0, Fo, eta = p[9], p[13], p[17], p[18]
+    v = pyoracle.johnson_eval(p, re)
+    assert v[0] == pytest.approx(fe / (1.0 + (1.0 - lam) ** 20), rel=1e-14)
+    assert v[2] == pytest.approx(A / (1.0 + (1.0 - kappa) ** 20) - B / (1.0 + (1.0 - lam) ** 20), rel=1e-14)
+    # far out the screening denominators dominate: rho and phi fall off as x^-20 exp(...)
+    x = 6.0 / re
+    v6 = pyoracle.johnson_eval(p, 6.0)
+    assert v6[0] == pytest.approx(fe * np.exp(-p[4] * (x - 1)) / (1 + (x - lam) ** 20), rel=1e-13)
+    F, _ = pyoracle.johnson_embed(p, 0.85 * rhoe * (1 - 1e-15))
+    assert F == pytest.approx(Fn0, rel=1e-12)
+    F, _ = pyoracle.johnson_embed(p, 2.0 * rhoe)
+    assert F == pytest.approx(Fo * (1.0 - eta * np.log(2.0)) * 2.0 ** eta, rel=1e-13)
+    F, Fp = pyoracle.johnson_embed(p, rhoe)
+    assert F == F0
+
+
+def test_johnson_ta_equilibrium_lattice_and_cohesive_energy():
+    """The published Ta parameter set (Zhou, Johnson, Wadley, PRB 69, 144113 (2004); the
+    reference's Johnson deck) is fitted so that BCC Ta at its nearest-neighbour distance
+    re, a = 2 re / sqrt(3) = 3.3025 A, has the equilibrium density rhoe, is at zero
+    pressure, and has the cohesive energy 8.09 eV.  The oracle's rho(r) summed over the
+    shells within the deck's 6.1 A cutoff gives rhoe to 1e-5, and its full johnson_force
+    run gives E = -8.090 eV/atom with dE/da = 0 there (central difference)."""
+    import pyoracle
+    from exastamp_amd import units
+    p = JOHNSON_TA
+    a0 = 2.0 * p[0] / np.sqrt(3.0)
+    d = _bcc_shells(a0, 6.1)
+    assert d.size == 58   # 8 + 6 + 12 + 24 + 8 shells within 6.1 A
+    rho = sum(pyoracle.johnson_eval(p, x)[0] for x in d)
+    assert rho == pytest.approx(p[2], rel=1e-5)
+    pi = _johnson_internal(p)
+
+    def energy(a, n=6):
+        r, t = _bcc(n, a)
+        o = pyoracle.Oracle(threads=4)
+        o.set_config(_cfg(r, t, n * a), [180.95])
+        o.set_johnson(pi, 6.1)
+        o.build_neighbors(6.1, 0.3)
+        o.compute_forces("johnson")
+        assert np.abs(o.forces()).max() < 1e-9 * units.EV   # perfect crystal
+        return o.state()["ep"].sum() / units.EV / len(t)
+
+    e0 = energy(a0)
+    assert e0 == pytest.approx(-8.090, abs=5e-4)
+    h = 1e-3
+    dEda = (energy(a0 + h) - energy(a0 - h)) / (2 * h)
+    # the curvature is ~18 eV/A^2 per atom: |dE/da| < 0.01 puts the minimum within 6e-4 A of a0
+    assert abs(dEda) < 1e-2
+    assert energy(a0 + 0.02) > e0 and energy(a0 - 0.02) > e0
+
+
+def _alloy_config(kind):
+    """Jittered two-species crystals: AlCu B2 (the reference's AlCu.eam.alloy) and a
+    random Cu/Ni FCC (the synthetic two-element tables)."""
+    from exastamp_amd import inputs, potentials
+    if kind == "alcu":
+        tab = potentials.read_setfl(GOLDEN / "AlCu.eam.alloy")
+        r, t = _bcc(5, 2.95, (0, 1))
+        r = r + inputs.rng(21).normal(scale=0.07, size=r.shape)
+        return _cfg(r, t, 5 * 2.95), [26.982, 63.546], tab
+    tab = potentials.johnson_tables((potentials.CU, potentials.NI), nr=2000, nrho=2000)
+    cfg = inputs.lattice_config((4, 4, 4), 3.57, [63.546, 58.693], jitter=0.08, seed=22)
+    t = (inputs.rng(23).random(cfg.n) < 0.5).astype(np.uint8)
+    return inputs.Config(cfg.r, cfg.v, t, cfg.lo, cfg.hi), [63.546, 58.693], tab
+
+
+@pytest.mark.parametrize("kind", ["alcu", "cuni"])
+def test_alloy_force_is_minus_energy_gradient(kind):
+    """Two species with different rho(r) tables: psi' = F'_i rho'_j(r) + F'_j rho'_i(r) + phi'
+    (eam_alloy.h:245-260) takes each density derivative from the other atom's species;
+    with the types of the two rho' terms exchanged the forces stop being -grad E."""
+    from exastamp_amd import inputs
+    cfg, masses, tab = _alloy_config(kind)
+    assert len(set(cfg.type.tolist())) == 2
+    o = _oracle(cfg, masses, tab)
+    f = o.forces()
+    h = 1e-5
+    for i in (0, 31, 140):
+        for a in range(3):
+            rp, rm = cfg.r.copy(), cfg.r.copy()
+            rp[i, a] += h
+            rm[i, a] -= h
+            cp = inputs.Config(rp, cfg.v, cfg.type, cfg.lo, cfg.hi)
+            cm = inputs.Config(rm, cfg.v, cfg.type, cfg.lo, cfg.hi)
+            fd = -(_energy(cp, masses, tab=tab) - _energy(cm, masses, tab=tab)) / (2 * h)
+            assert f[i, a] == pytest.approx(fd, rel=1e-6, abs=1e-6 * np.abs(f).max())
+    rms = np.sqrt((f ** 2).sum(1).mean())
+    assert np.abs(f.sum(0)).max() < 1e-11 * rms * len(f)
+
+
+@pytest.mark.parametrize("kind", ["alcu", "cuni"])
+def test_alloy_virial_is_minus_strain_derivative(kind):
+    from exastamp_amd import inputs
+    cfg, masses, tab = _alloy_config(kind)
+    o = _oracle(cfg, masses, tab)
+    W = o.state()["virial"].sum(0)
+    h = 1e-6
+    for a in range(3):
+        sp, sm = np.ones(3), np.ones(3)
+        sp[a] += h
+        sm[a] -= h
+        ep = _energy(inputs.Config(cfg.r * sp, cfg.v, cfg.type, cfg.lo, cfg.hi * sp), masses, tab=tab)
+        em = _energy(inputs.Config(cfg.r * sm, cfg.v, cfg.type, cfg.lo, cfg.hi * sm), masses, tab=tab)
+        assert W[a, a] == pytest.approx(-(ep - em) / (2 * h), rel=1e-6)
+    assert np.allclose(W, W.T, rtol=1e-10, atol=1e-10 * np.abs(W).max())
+
+
+def test_eam_types_slot_equals_the_listed_species_alone():
+    """eam_alloy_force with types: [Cu] on a Cu/Ni crystal (eam_potential_multimat.cu:154-166):
+    every pair with a Ni atom is disabled, so the Cu atoms see exactly what they see
+    with the Ni atoms removed, and the Ni atoms get rho_dEmb = 0, no energy, no force.
+    The full mask equals the default."""
+    import pyoracle
+    from exastamp_amd import inputs, potentials
+    tab = potentials.johnson_tables((potentials.CU, potentials.NI), nr=2000, nrho=2000)
+    cfg = inputs.lattice_config((4, 4, 4), 3.57, [63.546, 58.693], jitter=0.08, seed=31)
+    t = (inputs.rng(32).random(cfg.n) < 0.5).astype(np.uint8)
+    cfg = inputs.Config(cfg.r, cfg.v, t, cfg.lo, cfg.hi)
+    masses = [63.546, 58.693]
+
+    def run(c, mask=None):
+        o = pyoracle.Oracle(threads=4)
+        o.set_config(c, masses)
+        o.set_eam(tab)
+        if mask is not None:
+            o.set_eam_types(mask)
+        o.build_neighbors(tab.rc, 0.3)
+        o.compute_forces("eam")
+        return o.forces(), o.state()
+
+    f_all, s_all = run(cfg)
+    f_full, s_full = run(cfg, 0b11)
+    assert np.array_equal(f_all, f_full) and np.array_equal(s_all["ep"], s_full["ep"])
+    f_cu, s_cu = run(cfg, 0b01)
+    cu = cfg.type == 0
+    f_sub, s_sub = run(inputs.Config(cfg.r[cu], cfg.v[cu], cfg.type[cu], cfg.lo, cfg.hi))
+    scale = np.abs(f_sub).max()
+    assert np.abs(f_cu[cu] - f_sub).max() <= 1e-12 * scale
+    assert np.abs(s_cu["ep"][cu] - s_sub["ep"]).max() <= 1e-12 * np.abs(s_sub["ep"]).max()
+    assert np.abs(s_cu["virial"][cu] - s_sub["virial"]).max() <= 1e-12 * np.abs(s_sub["virial"]).max()
+    assert np.all(f_cu[~cu] == 0.0) and np.all(s_cu["ep"][~cu] == 0.0) and np.all(s_cu["rho_dEmb"][~cu] == 0.0)
+    assert np.abs(f_all[cu] - f_sub).max() > 1e-3 * scale   # the Ni atoms do matter when enabled
+    f_none, s_none = run(cfg, 0)
+    assert np.all(f_none == 0.0) and np.all(s_none["ep"] == 0.0)
