@@ -85,6 +85,7 @@ SIGNATURES = {
     "xs_push_f_v": (C.c_int, [_CTX, C.c_double, C.c_double]),
     "xs_thermo": (C.c_int, [_CTX, _D]),
     "xs_eam_adjoint": (C.c_int, [_CTX, C.c_double, _D, _D, _D, C.c_uint]),
+    "xs_lj_adjoint": (C.c_int, [_CTX, C.c_double, C.c_double, C.c_double, _D, _D, _D, C.c_uint]),
     "xs_spline_adjoint": (C.c_int, [C.c_int, C.c_double, _D, _D]),
     "xs_stream_copy": (C.c_int, [_CTX, C.c_size_t, C.c_int, _D]),
     "xs_profile_enable": (C.c_int, [_CTX, C.c_int]),

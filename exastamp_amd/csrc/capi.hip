@@ -603,6 +603,19 @@ xs_status xs_eam_adjoint(xs_ctx* h, double rcut, const double* dL_dF, const doub
   });
 }
 
+xs_status xs_lj_adjoint(xs_ctx* h, double epsilon, double sigma, double rcut, const double* dL_dF,
+                        const double* dL_dE, double grad[2], unsigned flags)
+{
+  return guarded(h, [&](Ctx& c) {
+    require(dL_dF && dL_dE && grad, XS_E_ARG, "null adjoint argument");
+    require(epsilon >= 0 && sigma > 0 && rcut > 0, XS_E_ARG, "lj: need epsilon >= 0, sigma > 0, rcut > 0");
+    require(flags == 0, XS_E_ARG, "xs_lj_adjoint: flags must be 0 (the sweep is FP64)");
+    require_list(c);
+    require(rcut <= c.list_rcut + c.list_skin, XS_E_ARG, "rcut exceeds the neighbour-list radius");
+    lj_adjoint(c, epsilon, sigma, rcut, dL_dF, dL_dE, grad);
+  });
+}
+
 xs_status xs_spline_adjoint(int n, double delta, const double* G, double* out)
 {
   if (n < 5 || !G || !out || !(delta > 0)) return XS_E_ARG;

@@ -303,6 +303,113 @@ __global__ void k_gather_g(int64_t n, const int64_t* __restrict__ id, const doub
   store_pos(gvec, i, Pos4{g_id[3 * q], g_id[3 * q + 1], g_id[3 * q + 2], 0.0});
 }
 
+// ---------------------------------------------------------------------------
+// Lennard-Jones adjoint with respect to (epsilon, sigma).
+// With w(r) = e'(r)/r (the force factor, F_i = sum_j w(r_ij) (x_j - x_i)) and the
+// pair energy 1/2 (e(r) - e(rc)) per list entry (pair_potential_impl.hxx:487-498,
+// lennard_jones.h:40-50), for theta in {epsilon, sigma}:
+//   dL/dtheta = sum over list entries (i, j), r_ij <= rc, of
+//               1/2 gE_rep(i) (de/dtheta(r) - de/dtheta(rc)) + dw/dtheta(r) g_i.(x_j - x_i)
+// with s6 = (sigma/r)^6, s12 = s6^2:
+//   de/deps = 4 (s12 - s6),             de/dsigma = 4 eps (12 s12 - 6 s6) / sigma,
+//   dw/deps = -24 (2 s12 - s6) / r^2,   dw/dsigma = -24 eps (24 s12 - 6 s6) / (sigma r^2).
+// One thread per atom walks the global list in FP64; each workgroup writes its two
+// sums, and one workgroup adds the partials in a fixed order (no atomics, so the
+// result does not depend on scheduling).
+// ---------------------------------------------------------------------------
+struct LjAdjArgs {
+  int64_t n, npad;
+  const Pos4* pos;
+  const Pos4* gvec;       // dL/dF_i in cell order
+  const uint16_t* rep;
+  const BoxDev* box;
+  const uint32_t* nbr;
+  const int* nnb;
+  const double* gE;       // per replica
+  double rc2, eps, sigma, sigma2;
+  double decut_eps, decut_sigma;  // de/deps and de/dsigma at r = rc
+};
+
+constexpr int kLjAdjThreads = 256;
+
+// a and b summed over the workgroup in a fixed order (shuffle tree within each wave,
+// then the waves in index order through red); every thread calls it and gets both sums
+__device__ __forceinline__ void block_sum2(double a, double b, double* red, double& ra, double& rb)
+{
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    a += __shfl_xor(a, off);
+    b += __shfl_xor(b, off);
+  }
+  const int w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    red[2 * w] = a;
+    red[2 * w + 1] = b;
+  }
+  __syncthreads();
+  ra = rb = 0.0;
+  for (int k = 0; k < nw; k++) {
+    ra += red[2 * k];
+    rb += red[2 * k + 1];
+  }
+}
+
+__global__ void __launch_bounds__(kLjAdjThreads) k_lj_adjoint(LjAdjArgs a, double* __restrict__ part)
+{
+  __shared__ double red[2 * (kLjAdjThreads / 64)];
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double ge = 0.0, gs = 0.0;
+  if (i < a.n) {
+    const int rep = a.rep ? a.rep[i] : 0;
+    const double* L = a.box[rep].L;
+    const Pos4 pi = load_pos(a.pos, i);
+    const Pos4 gi = load_pos(a.gvec, i);
+    const double hE = 0.5 * a.gE[rep];
+    const int cnt = a.nnb[i];
+    for (int k = 0; k < cnt; k++) {
+      const uint32_t e = a.nbr[(int64_t)k * a.npad + i];
+      const Pos4 pj = load_pos(a.pos, xsmd_nbr_index(e));
+      const double dx = xsmd_image(pj.x, xsmd_nbr_img_x(e), L[0]) - pi.x;
+      const double dy = xsmd_image(pj.y, xsmd_nbr_img_y(e), L[1]) - pi.y;
+      const double dz = xsmd_image(pj.z, xsmd_nbr_img_z(e), L[2]) - pi.z;
+      const double d2 = xsmd_d2(dx, dy, dz);
+      if (d2 > a.rc2) continue;
+      const double ir2 = 1.0 / d2;
+      const double s2 = a.sigma2 * ir2;
+      const double s6 = s2 * s2 * s2;
+      const double s12 = s6 * s6;
+      const double gd = gi.x * dx + gi.y * dy + gi.z * dz;
+      ge += hE * (4.0 * (s12 - s6) - a.decut_eps) - 24.0 * (2.0 * s12 - s6) * ir2 * gd;
+      gs += hE * (4.0 * a.eps * (12.0 * s12 - 6.0 * s6) / a.sigma - a.decut_sigma) -
+            24.0 * a.eps * (24.0 * s12 - 6.0 * s6) / a.sigma * ir2 * gd;
+    }
+  }
+  double ra, rb;
+  block_sum2(ge, gs, red, ra, rb);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = ra;
+    part[2 * blockIdx.x + 1] = rb;
+  }
+}
+
+// out[0..1] = sum of the nparts partials, each thread a fixed stride of them
+__global__ void __launch_bounds__(kLjAdjThreads) k_lj_adjoint_reduce(int64_t nparts, const double* __restrict__ part,
+                                                                     double* __restrict__ out)
+{
+  __shared__ double red[2 * (kLjAdjThreads / 64)];
+  double a = 0.0, b = 0.0;
+  for (int64_t k = threadIdx.x; k < nparts; k += blockDim.x) {
+    a += part[2 * k];
+    b += part[2 * k + 1];
+  }
+  double ra, rb;
+  block_sum2(a, b, red, ra, rb);
+  if (threadIdx.x == 0) {
+    out[0] = ra;
+    out[1] = rb;
+  }
+}
+
 template <class Real>
 AdjTab<Real> adj_tab(Ctx& c);
 
@@ -507,6 +614,50 @@ void eam_adjoint(Ctx& c, double rcut, const double* dLdF_id, const double* dLdE,
     spline_interpolate_adjoint(e.nr, e.dr, G.data() + nF + (size_t)t * rows_r * 4, out);
   for (int t = 0; t < nz; t++, out += e.nr)
     spline_interpolate_adjoint(e.nr, e.dr, G.data() + nF + nR + (size_t)t * rows_r * 4, out);
+}
+
+void lj_adjoint(Ctx& c, double eps, double sigma, double rcut, const double* dLdF_id, const double* dLdE,
+                double grad[2])
+{
+  ensure_global_list(c);
+  const unsigned g = grid_for(c.n, kLjAdjThreads);
+  c.adj_gvec.reserve(c.npad);
+  c.adj_in.reserve(3 * (size_t)c.n + c.nrep);
+  c.adj_part.reserve(2 * (size_t)std::max(g, 1u) + 2);
+  XS_HIP(hipMemcpyAsync(c.adj_in.p, dLdF_id, sizeof(double) * 3 * c.n, hipMemcpyHostToDevice, c.stream));
+  XS_HIP(hipMemcpyAsync(c.adj_in.p + 3 * c.n, dLdE, sizeof(double) * c.nrep, hipMemcpyHostToDevice, c.stream));
+  double* out = c.adj_part.p + 2 * (size_t)std::max(g, 1u);
+  if (c.n > 0) {
+    k_gather_g<<<grid_for(c.n), kBlock, 0, c.stream>>>(c.n, c.id.p, c.adj_in.p, c.adj_gvec.p);
+    XS_HIP(hipGetLastError());
+    LjAdjArgs a;
+    a.n = c.n;
+    a.npad = c.npad;
+    a.pos = c.pos.p;
+    a.gvec = c.adj_gvec.p;
+    a.rep = c.has_rep ? c.rep.p : nullptr;
+    a.box = c.box_d.p;
+    a.nbr = c.nbr.p;
+    a.nnb = c.nnb.p;
+    a.gE = c.adj_in.p + 3 * c.n;
+    a.rc2 = rcut * rcut;
+    a.eps = eps;
+    a.sigma = sigma;
+    a.sigma2 = sigma * sigma;
+    // the derivatives of energy_cutoff = e(rc) (pair_potential_impl.hxx:487-498)
+    const double sc2 = a.sigma2 / a.rc2, sc6 = sc2 * sc2 * sc2, sc12 = sc6 * sc6;
+    a.decut_eps = 4.0 * (sc12 - sc6);
+    a.decut_sigma = 4.0 * eps * (12.0 * sc12 - 6.0 * sc6) / sigma;
+    ProfScope prof(c.prof, c.stream, XS_PROF_ADJOINT);
+    k_lj_adjoint<<<g, kLjAdjThreads, 0, c.stream>>>(a, c.adj_part.p);
+    XS_HIP(hipGetLastError());
+    k_lj_adjoint_reduce<<<1, kLjAdjThreads, 0, c.stream>>>(g, c.adj_part.p, out);
+    XS_HIP(hipGetLastError());
+    XS_HIP(hipMemcpyAsync(grad, out, 2 * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  } else {
+    grad[0] = grad[1] = 0.0;
+  }
+  XS_HIP(hipStreamSynchronize(c.stream));
 }
 
 }  // namespace xsmd

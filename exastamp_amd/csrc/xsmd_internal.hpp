@@ -406,6 +406,9 @@ const Real* eam_pair_table(Ctx& c);
 // ---- EAM adjoint (k_adjoint.hip) ----
 void eam_adjoint(Ctx& c, double rcut, const double* dLdF_id, const double* dLdE, double* grad, bool mixed);
 void spline_interpolate_adjoint(int n, double delta, const double* G, double* out);
+// dL/d(epsilon, sigma) of lj_compute_force's forces and shifted energies (k_adjoint.hip)
+void lj_adjoint(Ctx& c, double eps, double sigma, double rcut, const double* dLdF_id, const double* dLdE,
+                double grad[2]);
 
 // ---- setfl + splines (setfl.cpp) ----
 void spline_interpolate(int n, double delta, const double* f, double* coeffs);

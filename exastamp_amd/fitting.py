@@ -55,6 +55,34 @@ class EamForcesEnergies(torch.autograd.Function):
         return torch.from_numpy(g).to(gF.device), None, None, None
 
 
+class LjForcesEnergies(torch.autograd.Function):
+    """theta = (epsilon in eV, sigma in A) -> forces (n, 3) in eV/A and per-replica
+    energies in eV of lj_compute_force at rcut; backward calls the hand-written LJ
+    adjoint (xs_lj_adjoint: one FP64 sweep of the list, the energy shift's dependence
+    on epsilon and sigma included).  The system must not move between the calls."""
+
+    @staticmethod
+    def forward(ctx, theta, sim, rcut, mixed=False):
+        eps_ev, sigma = (float(x) for x in theta.detach().to("cpu", torch.float64))
+        params = {"epsilon": eps_ev * EV, "sigma": sigma}
+        sim.zero_force_energy()
+        sim.lj_compute_force(parameters=params, rcut=rcut, compute_virial=False, trigger_thermo_state=True,
+                             mixed=mixed)
+        sim.force_to_accel()
+        F = torch.from_numpy(sim.forces() / EV).to(theta.device)
+        E = torch.from_numpy(sim.replica_energies() / EV).to(theta.device)
+        ctx.sim, ctx.params, ctx.rcut = sim, params, rcut
+        return F, E
+
+    @staticmethod
+    def backward(ctx, gF, gE):
+        gf = gF.detach().to("cpu", torch.float64).numpy() / EV
+        ge = gE.detach().to("cpu", torch.float64).numpy() / EV
+        g = ctx.sim.lj_adjoint(gf, ge, parameters=ctx.params, rcut=ctx.rcut)
+        g[0] *= EV   # d/d(epsilon in eV) = EV * d/d(epsilon in internal units)
+        return torch.from_numpy(g).to(gF.device), None, None, None
+
+
 def fit_loss(F, E, F_target, E_target, atoms_per_replica, energy_weight=0.1):
     """Force MSE (eV/A)^2 over all components plus energy_weight * MSE of the
     per-atom energy (eV) over replicas."""

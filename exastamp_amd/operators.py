@@ -98,6 +98,7 @@ class Simulation:
         self.cfg: Config | None = None
         self.rcut_max = 0.0
         self.lj_params: tuple[float, float] | None = None
+        self.lj_rcut = 0.0
         self.eam: SetflTables | None = None
 
     # --- set-up -------------------------------------------------------------
@@ -174,8 +175,29 @@ class Simulation:
         rc = parse_quantity(rcut)
         self.rcut_max = max(self.rcut_max, rc)
         self.lj_params = (eps, sig)
+        self.lj_rcut = rc
         flags = self._flags(trigger_thermo_state, compute_virial, mixed) | (XS_SYMMETRIC if symmetric else 0)
         self.ctx.xs_lj_compute_force(eps, sig, rc, flags)
+
+    def lj_adjoint(self, dL_dF, dL_dE, parameters=None, rcut=None) -> np.ndarray:
+        """(dL/depsilon, dL/dsigma) in internal units given dL/dF (n, 3; forces in internal
+        units) and dL/dE per replica, for lj_compute_force with these parameters and rcut
+        (default: those of the last lj_compute_force / lj_parameters call) at the current
+        positions."""
+        if parameters is None:
+            if self.lj_params is None:
+                raise ValueError("lj_adjoint needs LJ parameters (run lj_compute_force first or pass them)")
+            eps, sig = self.lj_params
+        else:
+            eps, sig = parse_quantity(parameters["epsilon"]), parse_quantity(parameters["sigma"])
+        rc = parse_quantity(rcut) if rcut is not None else self.lj_rcut
+        g = np.ascontiguousarray(np.asarray(dL_dF, dtype=np.float64).reshape(self.n, 3))
+        ge = np.ascontiguousarray(np.asarray(dL_dE, dtype=np.float64).reshape(-1))
+        if ge.size != self.cfg.nrep:
+            raise ValueError("dL_dE needs one value per replica")
+        out = np.zeros(2)
+        self.ctx.xs_lj_adjoint(eps, sig, rc, _lib.dptr(g), _lib.dptr(ge), _lib.dptr(out), 0)
+        return out
 
     def lj_compute_force_symetric(self, parameters, rcut, **kw):
         """lj_compute_force_symetric (pair_potential_singlemat_symetric.cpp:101-332):
@@ -217,6 +239,7 @@ class Simulation:
         rc = parse_quantity(rcut)
         self.rcut_max = max(self.rcut_max, rc)
         self.lj_params = (eps, sig)
+        self.lj_rcut = rc
         self.ctx.xs_lj_parameters(eps, sig, rc)
 
     def eam_alloy_init(self, parameters):

@@ -213,6 +213,20 @@ xs_status xs_thermo(xs_ctx* ctx, double out27[27]);
  * precision of the forward. */
 xs_status xs_eam_adjoint(xs_ctx* ctx, double rcut, const double* dL_dF, const double* dL_dE,
                          double* grad, unsigned flags);
+/* No counterpart in exaStamp: reverse mode of lj_compute_force (pair_potential_impl.hxx:
+ * 39-513 with lennard_jones.h:40-50) with respect to its parameters, for fitting
+ * epsilon and sigma to reference forces and energies.  Differentiates the forces
+ * F_i = sum_j e'(r)/r (x_j - x_i) and the shifted pair energies 1/2 (e(r) - e(rcut))
+ * per list entry, the shift's own dependence on epsilon and sigma included
+ * (energy_cutoff, pair_potential_impl.hxx:487-498).  Needs the neighbour list of the
+ * current positions (those of the forward call); one FP64 sweep of the list.
+ *   dL_dF : 3n doubles, dL/dF per atom (id order, x y z), forces in internal units
+ *   dL_dE : nrep doubles, dL/dE per replica's total potential energy
+ *   grad  : receives dL/depsilon (epsilon in internal energy units) and dL/dsigma
+ *           (sigma in A)
+ * flags must be 0. */
+xs_status xs_lj_adjoint(xs_ctx* ctx, double epsilon, double sigma, double rcut, const double* dL_dF,
+                        const double* dL_dE, double grad[2], unsigned flags);
 /* Knot-gradient of the spline construction alone (transpose of the knot -> c3..c6
  * map): G has (n+1) rows of 4 coefficient gradients, out n knots.  Host only. */
 xs_status xs_spline_adjoint(int n, double delta, const double* G, double* out);
