@@ -686,6 +686,7 @@ EamTabDev<double> make_eam_tab<double>(Ctx& c)
   t.nr = c.eam.nr;
   t.nrho = c.eam.nrho;
   t.rdr = c.eam.rdr;
+  t.rdr_d = c.eam.rdr;
   t.rdrho = c.eam.rdrho;
   t.rhomax = c.eam.rhomax;
   t.conv = kEV;
@@ -706,6 +707,7 @@ EamTabDev<float> make_eam_tab<float>(Ctx& c)
   t.nr = c.eam.nr;
   t.nrho = c.eam.nrho;
   t.rdr = (float)c.eam.rdr;
+  t.rdr_d = c.eam.rdr;
   t.rdrho = (float)c.eam.rdrho;
   t.rhomax = (float)c.eam.rhomax;
   t.conv = (float)kEV;

@@ -29,13 +29,7 @@ __device__ __forceinline__ Q4<double> load4(const double* __restrict__ p)
   return {u.x, u.y, w.x, w.y};
 }
 
-// 1/sqrt(d2) and sqrt(d2): one hardware rsq in FP32, correctly rounded sqrt
-// and division in FP64 (what plain FP64 code compiles to).
-__device__ __forceinline__ void dist(float d2, float& r, float& rinv)
-{
-  rinv = __builtin_amdgcn_rsqf(d2);
-  r = d2 * rinv;
-}
+// sqrt(d2) and 1/sqrt(d2) correctly rounded in FP64 (what plain FP64 code compiles to)
 __device__ __forceinline__ void dist(double d2, double& r, double& rinv)
 {
   r = sqrt(d2);
@@ -160,6 +154,34 @@ __device__ __forceinline__ void knot_r(const EamTabDev<Real>& t, Real r, int& m,
   p = fmin(p, Real(1));
 }
 
+// Distance, 1/r and spline knot of a pair from its FP64 d^2 (the first steps of
+// eam_alloy_rho_noderiv / eam_alloy_mm_force, eam_alloy.h:198-265: p = r rdr + 1,
+// m = min(int p, nr - 1), p -= m, p = min(p, 1)).
+// FP64 form: correctly rounded sqrt and division, as plain FP64 code.
+// Mixed form: the knot fraction of r ~ 5 A at 1000 knots per A needs more than FP32
+// (its ulp at p ~ 5000 is 5e-4 of a knot, which puts ~1e-5 of relative error on the
+// forces of atoms whose pair forces nearly cancel), so 1/sqrt(d2) is the FP32
+// hardware rsq refined by one FP64 Newton step (relative error ~1e-14), r and p are
+// formed in FP64, and only the fraction (in [0, 1]) and 1/r are rounded to FP32 for
+// the spline arithmetic.
+__device__ __forceinline__ void knot_dist(const EamTabDev<double>& t, double d2, int& m, double& p, double& rinv)
+{
+  double r;
+  dist(d2, r, rinv);
+  knot_r(t, r, m, p);
+}
+__device__ __forceinline__ void knot_dist(const EamTabDev<float>& t, double d2, int& m, float& p, float& rinv)
+{
+  const double y0 = (double)__builtin_amdgcn_rsqf((float)d2);
+  const double y = y0 * fma(-0.5 * d2 * y0, y0, 1.5);
+  double q = fma(d2 * y, t.rdr_d, 1.0);
+  m = (int)q;
+  m = max(0, min(m, t.nr - 1));
+  q -= (double)m;
+  p = (float)fmin(q, 1.0);
+  rinv = (float)y;
+}
+
 template <class Real>
 __device__ __forceinline__ const Real* row_r(const Real* tab, const EamTabDev<Real>& t, int which, int m)
 {
@@ -234,11 +256,9 @@ template <class Real, bool MULTI>
 __device__ __forceinline__ Real eam_rho(const EamTabDev<Real>& t, const Real* __restrict__ pair, int ti, int tj,
                                         double d2)
 {
-  Real r, rinv;
-  dist((Real)d2, r, rinv);
+  Real p, rinv;
   int m;
-  Real p;
-  knot_r(t, r, m, p);
+  knot_dist(t, d2, m, p, rinv);
   const Real* row;
   if constexpr (MULTI) row = row_r(t.rhor, t, t.t2rhor[tj + 1][ti + 1], m);
   else row = pair_rho_row(pair, m);
@@ -268,11 +288,9 @@ template <class Real, bool MULTI>
 __device__ __forceinline__ void eam_pair(const EamTabDev<Real>& t, const Real* __restrict__ pair, int ti, int tj,
                                          Real fpi, Real fpj, double d2, Real& fpair, Real& phi)
 {
-  Real r, rinv;
-  dist((Real)d2, r, rinv);
+  Real p, rinv;
   int m;
-  Real p;
-  knot_r(t, r, m, p);
+  knot_dist(t, d2, m, p, rinv);
   if constexpr (MULTI) {
     const Real rhoip = quad(load4(row_r(t.rhor, t, t.t2rhor[ti + 1][tj + 1], m) + 4), p);
     const Real rhojp = quad(load4(row_r(t.rhor, t, t.t2rhor[tj + 1][ti + 1], m) + 4), p);

@@ -102,6 +102,7 @@ struct EamTabDev {
   const Real* z2r = nullptr;   // nel(nel+1)/2 tables of (nr+1) rows
   int nr = 0, nrho = 0;
   Real rdr = 0, rdrho = 0, rhomax = 0, conv = 0;
+  double rdr_d = 0;  // 1/dr in FP64 (the knot of r is found in FP64 in both forms)
   int8_t t2rhor[8][8] = {};
   int8_t t2z2r[8][8] = {};
   int8_t t2frho[8] = {};
