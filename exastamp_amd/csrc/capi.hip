@@ -477,6 +477,7 @@ xs_status xs_lj_compute_force(xs_ctx* h, double epsilon, double sigma, double rc
     require_list(c);
     require(rcut <= c.list_rcut + c.list_skin, XS_E_ARG, "rcut exceeds the neighbour-list radius");
     launch_lj(c, epsilon, sigma, rcut, flags, ForceEpilogue{});
+    check_pass_overflow(c);
   });
 }
 
@@ -524,6 +525,7 @@ xs_status xs_lj_multi_force(xs_ctx* h, int ntypes, const double* epsilon, const 
     c.ljm_ntypes = ntypes;
     c.ljm_rcmax = rcmax;
     launch_lj_multi(c, flags);
+    check_pass_overflow(c);
   });
 }
 
@@ -557,6 +559,7 @@ xs_status xs_eam_alloy_force(xs_ctx* h, double rcut, int eam_rho, int eam_rho2em
       require(rcut <= c.list_rcut + c.list_skin, XS_E_ARG, "rcut exceeds the neighbour-list radius");
     }
     launch_eam(c, rcut, eam_rho != 0, eam_rho2emb != 0, eam_force != 0, flags, ForceEpilogue{});
+    check_pass_overflow(c);
   });
 }
 
@@ -701,10 +704,12 @@ xs_status xs_md_run(xs_ctx* h, int potential, int nsteps, double dt, double rcut
         std::memcpy(&d2, &c.hscalar[1], sizeof(double));
         over = d2 > thr2;
       } else {  // the force pass of the previous step moved the atoms and tested them
-        XS_HIP(hipMemcpyAsync(c.hscalar + 5, c.dscalar.p + 5, sizeof(unsigned long long), hipMemcpyDeviceToHost,
-                              c.stream));
+        // the rebuild flag and the block passes' capacity flag in one copy
+        XS_HIP(hipMemcpyAsync(c.hscalar + 5, c.dscalar.p + 5, 2 * sizeof(unsigned long long),
+                              hipMemcpyDeviceToHost, c.stream));
         XS_HIP(hipStreamSynchronize(c.stream));
         over = c.hscalar[5] != 0;
+        if (c.hscalar[6] != 0) check_pass_overflow(c);
       }
       if (c.prof.on) c.prof.flush();
       if (over) {
@@ -715,6 +720,7 @@ xs_status xs_md_run(xs_ctx* h, int potential, int nsteps, double dt, double rcut
     }
     XS_HIP(hipStreamSynchronize(c.stream));
     const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    check_pass_overflow(c);
     if (c.prof.on) c.prof.flush();
     if (stats) {
       stats->steps = nsteps;

@@ -735,7 +735,19 @@ struct BlockArgs {
   int cap;   // halo slots (LDS records)
   int ccap;  // central atoms per brick at most
   double rc2;
+  // set to 1 by a brick whose halo or central atoms exceed cap or ccap (the maxima
+  // the host measured at list build, which hold while the cells do not change); such
+  // a brick leaves its atoms unwritten and the host reports the call as failed
+  unsigned long long* overflow;
 };
+
+// a brick beyond the measured capacities: raise the flag, skip the brick
+__device__ __forceinline__ bool over_capacity(const BlockArgs& a, int count, int cap)
+{
+  if (count <= cap) return false;
+  if (threadIdx.x == 0) *a.overflow = 1ull;
+  return true;
+}
 
 constexpr int kBBatch = 4;
 // EAM passes run two lanes per central atom: the two halves of a wave take the
@@ -801,11 +813,11 @@ __global__ void __launch_bounds__(kPassThreads, kPassWaves<false>) k_block_eam_r
   LdsLayout l = lds_layout(a.cap, a.ccap, a.g.nh_max, MULTI, false);
   const BlockShape s = block_shape(a.g, b);
   const int total = build_halo(a.g, s, a.cell_off, l.hc, l.total);
-  if (total > a.cap) return;  // excluded by the host (cap = measured maximum)
+  if (over_capacity(a, total, a.cap)) return;
   central_table(s, l.hc, l.central_local, a.ccap, l.total + 2);
   stage(s, l.hc, total, a.pos, a.box[s.r].L, a.type, l.rec, l.ltype, l.slot_cell);
   const int ncentral = l.total[2];
-  if (ncentral > a.ccap) return;  // excluded by the host (ccap = measured maximum)
+  if (over_capacity(a, ncentral, a.ccap)) return;
   const int half = pass_half();
   for (int q = pass_slot();; q += kPassAtoms) {
     Central ca;
@@ -868,7 +880,7 @@ __global__ void __launch_bounds__(kPassThreads, kPassWaves<VIRIAL>) k_block_eam_
   LdsLayout l = lds_layout(a.cap, a.ccap, a.g.nh_max, MULTI, true);
   const BlockShape s = block_shape(a.g, b);
   const int total = build_halo(a.g, s, a.cell_off, l.hc, l.total);
-  if (total > a.cap) return;  // excluded by the host (cap = measured maximum)
+  if (over_capacity(a, total, a.cap)) return;
   stage(s, l.hc, total, a.pos, a.box[s.r].L, a.type, l.rec, l.ltype, l.slot_cell);
   const int half = pass_half();
   for (int q = pass_slot();; q += kPassAtoms) {
@@ -945,11 +957,11 @@ __global__ void __launch_bounds__(kPassThreads, kPassWaves<VIRIAL>) k_block_lj(B
   LdsLayout l = lds_layout(a.cap, a.ccap, a.g.nh_max, MULTI, false);
   const BlockShape s = block_shape(a.g, b);
   const int total = build_halo(a.g, s, a.cell_off, l.hc, l.total);
-  if (total > a.cap) return;  // excluded by the host (cap = measured maximum)
+  if (over_capacity(a, total, a.cap)) return;
   central_table(s, l.hc, l.central_local, a.ccap, l.total + 2);
   stage(s, l.hc, total, a.pos, a.box[s.r].L, MULTI ? a.type : nullptr, l.rec, l.ltype, l.slot_cell);
   const int ncentral = l.total[2];
-  if (ncentral > a.ccap) return;  // excluded by the host (ccap = measured maximum)
+  if (over_capacity(a, ncentral, a.ccap)) return;
   const int half = pass_half();
   for (int q = pass_slot();; q += kPassAtoms) {
     Central ca;
@@ -1044,6 +1056,7 @@ void block_eam_dispatch(Ctx& c, double rcut, bool do_rho, bool do_emb, bool do_f
   a.npad = c.npad;
   a.cap = c.bcap;
   a.ccap = c.bcentral;
+  a.overflow = c.dscalar.p + 6;
   a.rc2 = rcut * rcut;
   const size_t lds_rho = block_lds_bytes(c, MULTI, false);
   const size_t lds = block_lds_bytes(c, MULTI, true);
@@ -1285,6 +1298,7 @@ void launch_lj_block(Ctx& c, double eps, double sigma, double rcut, unsigned fla
   a.npad = c.npad;
   a.cap = c.bcap;
   a.ccap = c.bcentral;
+  a.overflow = c.dscalar.p + 6;
   a.rc2 = rcut * rcut;
   const size_t lds = block_lds_bytes(c, false, false);
   const unsigned g = block_grid(c);
@@ -1331,6 +1345,7 @@ void launch_lj_multi_block(Ctx& c, unsigned flags)
   a.npad = c.npad;
   a.cap = c.bcap;
   a.ccap = c.bcentral;
+  a.overflow = c.dscalar.p + 6;
   a.rc2 = c.ljm_rcmax * c.ljm_rcmax;
   const size_t lds = block_lds_bytes(c, true, false);
   const unsigned g = block_grid(c);
@@ -1360,6 +1375,19 @@ void ensure_global_list(Ctx& c)
   if (c.list_kind == ListKind::Global || c.global_from_block) return;
   block_to_global(c);
   c.global_from_block = true;
+}
+
+void check_pass_overflow(Ctx& c)
+{
+  XS_HIP(hipMemcpyAsync(c.hscalar + 6, c.dscalar.p + 6, sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                        c.stream));
+  XS_HIP(hipStreamSynchronize(c.stream));
+  if (c.hscalar[6] == 0) return;
+  c.hscalar[6] = 0;
+  XS_HIP(hipMemsetAsync(c.dscalar.p + 6, 0, sizeof(unsigned long long), c.stream));
+  XS_HIP(hipStreamSynchronize(c.stream));
+  throw Error(XS_E_STATE, "block pass: a brick holds more halo or central atoms than measured at the list build "
+                          "(cells changed without a rebuild); its atoms were not computed");
 }
 
 }  // namespace xsmd

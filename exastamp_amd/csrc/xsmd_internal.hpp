@@ -298,7 +298,8 @@ struct Ctx {
 
   // small device scalars + pinned host mirror
   DBuf<unsigned long long> dscalar;  // [0] max count, [1] max displacement^2 bits, [2..3] halo maxima,
-                                     // [4] atom outside its cell, [5] fused-drift rebuild flag
+                                     // [4] atom outside its cell, [5] fused-drift rebuild flag,
+                                     // [6] block-pass capacity exceeded
   unsigned long long* hscalar = nullptr;
   DBuf<double> red_part;  // thermo partial sums
   DBuf<double> red_out;
@@ -349,6 +350,9 @@ void ensure_global_list(Ctx& c);
 // the half list (nbr_half / nnb_half) of the current list
 void ensure_half_list(Ctx& c);
 double max_displacement2(Ctx& c);
+// raises XS_E_STATE (and clears the flag) when a block pass found a brick beyond the
+// capacities measured at list build (dscalar[6]); synchronises the stream
+void check_pass_overflow(Ctx& c);
 void pair_count(Ctx& c, double rcut, int64_t& entries, int64_t& within);
 
 // ---- launches (k_integrate.hip) ----

@@ -176,7 +176,8 @@ def test_lj_mixed_within_tolerance(oracle_mod):
 
 
 def test_lj_nve_c1_trajectory(oracle_mod):
-    """C1: LJ Ar 4000 atoms, 100 NVE steps, dt = 2 fs, per-step energies and virial."""
+    """C1: LJ Ar 4000 atoms, 100 NVE steps, dt = 2 fs, per-step energies and full virial tensor
+    from the fused loop."""
     eps, sig, rc = lj_params()
     cfg = lj_c1_config()
     sim = make_sim(cfg, [AR_MASS])
@@ -188,13 +189,65 @@ def test_lj_nve_c1_trajectory(oracle_mod):
     assert st.rebuilds == nreb
     pe = np.array([t.potential_energy for t in th])
     ke = np.array([t.kinetic_energy for t in th])
-    vtr = np.array([np.trace(t.virial) for t in th])
+    vir = np.array([t.virial.ravel() for t in th])
     assert np.abs(pe - th_ref[:, 24]).max() <= 1e-9 * np.abs(th_ref[:, 24]).max()
     assert np.abs(ke - th_ref[:, 21:24].sum(1)).max() <= 1e-9 * th_ref[:, 21:24].sum(1).max()
-    vref = th_ref[:, 0] + th_ref[:, 4] + th_ref[:, 8]
-    assert np.abs(vtr - vref).max() <= 1e-8 * np.abs(vref).max()
+    # every component of the 3x3 virial tensor at every step
+    vref = th_ref[:, 0:9]
+    assert np.abs(vir - vref).max() <= 1e-8 * np.abs(vref).max()
     r, v = sim.state()
     assert np.abs(r - o.state()["r"]).max() < 1e-8
+
+
+def test_lj_c1_per_step_forces_and_virial(oracle_mod):
+    """C1 step by step through the operators (push_f_v_r, push_f_v, particle_displ_over
+    -> chunk_neighbors, zero_force_energy, lj_compute_force, force_to_accel, push_f_v) on
+    both sides: every step's forces on all 4000 atoms, potential energy and full 3x3
+    virial tensor against the oracle."""
+    eps, sig, rc = lj_params()
+    skin, dt = 0.3, 0.002
+    cfg = lj_c1_config()
+    sim = make_sim(cfg, [AR_MASS])
+    o = make_oracle(oracle_mod, cfg, [AR_MASS])
+    o.set_lj(eps, sig, rc)
+    o.build_neighbors(rc, skin)
+    o.compute_forces("lj")  # zero_force_energy + lj_compute_force + force_to_accel
+    sim.chunk_neighbors(rcut_max=rc, rcut_inc=skin)
+    sim.zero_force_energy()
+    sim.lj_compute_force(parameters={"epsilon": eps, "sigma": sig}, rcut=rc)
+    sim.force_to_accel()
+    worst_f = worst_v = worst_e = 0.0
+    rebuilds = [0, 0]
+    for step in range(1, 101):
+        sim.push_f_v_r(dt)
+        sim.push_f_v(dt, 0.5)
+        o.push_f_v_r(dt)
+        o.push_f_v(dt * 0.5)
+        if sim.particle_displ_over(0.5 * skin):
+            sim.chunk_neighbors(rcut_max=rc, rcut_inc=skin)
+            rebuilds[0] += 1
+        if o.displ_over(0.5 * skin):
+            o.build_neighbors(rc, skin)
+            rebuilds[1] += 1
+        sim.zero_force_energy()
+        sim.lj_compute_force(parameters={"epsilon": eps, "sigma": sig}, rcut=rc)
+        sim.force_to_accel()
+        o.compute_forces("lj")
+        f, ref = sim.forces(), o.forces()
+        st = o.state()
+        vir, vref = sim.virial().sum(0), st["virial"].sum(0)
+        pe, pref = sim.ep().sum(), st["ep"].sum()
+        worst_f = max(worst_f, frel(f, ref))
+        worst_v = max(worst_v, float(np.abs(vir - vref).max() / np.abs(vref).max()))
+        worst_e = max(worst_e, abs(pe - pref) / abs(pref))
+        sim.push_f_v(dt, 0.5)
+        o.push_f_v(dt * 0.5)
+    print(f"C1 100 steps: max |dF|/rms|F| {worst_f:.2e}, virial {worst_v:.2e}, PE {worst_e:.2e}, "
+          f"rebuilds {rebuilds}")
+    assert rebuilds[0] == rebuilds[1]
+    assert worst_f <= 1e-9
+    assert worst_v <= 1e-9
+    assert worst_e <= 1e-9
 
 
 # ----------------------------------------------------------------------------
