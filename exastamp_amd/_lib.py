@@ -84,6 +84,7 @@ SIGNATURES = {
     "xs_push_f_v_r": (C.c_int, [_CTX, C.c_double, C.c_double]),
     "xs_push_f_v": (C.c_int, [_CTX, C.c_double, C.c_double]),
     "xs_thermo": (C.c_int, [_CTX, _D]),
+    "xs_thermo_noalloc": (C.c_int, [_CTX, C.c_void_p]),
     "xs_eam_adjoint": (C.c_int, [_CTX, C.c_double, _D, _D, _D, C.c_uint]),
     "xs_lj_adjoint": (C.c_int, [_CTX, C.c_double, C.c_double, C.c_double, _D, _D, _D, C.c_uint]),
     "xs_spline_adjoint": (C.c_int, [C.c_int, C.c_double, _D, _D]),

@@ -167,6 +167,7 @@ xs_status xs_ctx_create(int device, xs_ctx** out)
   c->device = device;
   try {
     XS_HIP(hipSetDevice(device));
+    thermo_scratch_init(device);
     XS_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     XS_HIP(hipHostMalloc((void**)&c->hscalar, 8 * sizeof(unsigned long long), hipHostMallocDefault));
     c->dscalar.reserve(8);
@@ -645,6 +646,14 @@ xs_status xs_thermo(xs_ctx* h, double out27[27])
   return guarded(h, [&](Ctx& c) {
     require(out27 != nullptr, XS_E_ARG, "null output");
     launch_thermo(c, out27);
+  });
+}
+
+xs_status xs_thermo_noalloc(xs_ctx* h, double* out27_dev)
+{
+  return guarded(h, [&](Ctx& c) {
+    require(out27_dev != nullptr, XS_E_ARG, "null output");
+    launch_thermo_noalloc(c, out27_dev);
   });
 }
 

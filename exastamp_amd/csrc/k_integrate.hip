@@ -3,6 +3,7 @@
 #include "xsmd_internal.hpp"
 
 #include <algorithm>
+#include <mutex>
 
 namespace xsmd {
 namespace {
@@ -162,6 +163,36 @@ __global__ void k_thermo_final(int nblocks, const double* __restrict__ part, dou
   out[q] = s;
 }
 
+// The same sums in the same order as k_thermo_final, finished on the device into the
+// 27-double layout of xs_thermo (kinetic tensor and kinetic energy halved, particle
+// count last), so the result never leaves device memory.
+__global__ void k_thermo_final27(int nblocks, const double* __restrict__ part, int64_t n,
+                                 double* __restrict__ out27)
+{
+  const int q = threadIdx.x;
+  if (q >= 27) return;
+  if (q == 26) {
+    out27[26] = (double)n;
+    return;
+  }
+  double s = 0.0;
+  for (int b = 0; b < nblocks; b++) s += part[(int64_t)b * kThermoQ + q];
+  const bool half = (q >= 9 && q < 18) || (q >= 21 && q < 24);
+  out27[q] = half ? 0.5 * s : s;
+}
+
+// Partial sums of xs_thermo_noalloc: kThermoBlocks rows of kThermoQ doubles
+// (208 KiB) in a fixed device array rather than context buffers, so that the call
+// allocates nothing.  Each device holds its own copy of the array for the life of
+// the process; host code must use the address thermo_scratch_init obtained for that
+// device, never the array's name (which on the host is only a stand-in symbol).
+constexpr size_t kScratchDoubles = (size_t)kThermoBlocks * kThermoQ;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
 }  // namespace
 
 void launch_zero(Ctx& c)
@@ -232,6 +263,37 @@ void launch_thermo(Ctx& c, double out[27])
   out[24] = s[24];
   out[25] = s[25];
   out[26] = (double)c.n;
+}
+
+void thermo_scratch_init(int device)
+{
+  if (device < 0 || device >= kMaxDevices) throw Error(XS_E_LIMIT, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[device]) return;
+  void* p = nullptr;
+  XS_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  g_scratch_ptr[device] = static_cast<double*>(p);
+}
+
+void launch_thermo_noalloc(Ctx& c, double* out27_dev)
+{
+  double* part = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    part = (c.device >= 0 && c.device < kMaxDevices) ? g_scratch_ptr[c.device] : nullptr;
+  }
+  if (!part) throw Error(XS_E_STATE, "thermo scratch of this device not initialised");
+  const int nb = c.n > 0 ? (int)std::min<int64_t>(kThermoBlocks, grid_for(c.n)) : 0;
+  if ((size_t)nb * kThermoQ > kScratchDoubles)
+    throw Error(XS_E_LIMIT, "thermo partial sums exceed the fixed scratch (" + std::to_string(kScratchDoubles) +
+                                " doubles)");
+  if (nb > 0) {
+    k_thermo_partial<<<nb, kBlock, 0, c.stream>>>(c.n, c.npad, c.vx.p, c.vy.p, c.vz.p, c.ep.p, c.vir.p, c.type.p,
+                                                  c.mass_d.p, part);
+    XS_HIP(hipGetLastError());
+  }
+  k_thermo_final27<<<1, 64, 0, c.stream>>>(nb, part, c.n, out27_dev);
+  XS_HIP(hipGetLastError());
 }
 
 }  // namespace xsmd

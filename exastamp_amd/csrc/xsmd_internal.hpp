@@ -363,6 +363,12 @@ void launch_push_f_v(Ctx& c, double dt);
 // fused first half of a step: r += v dt + a dt^2/2 ; v += a dt/2 ; max |r - r_ref|^2
 void launch_drift_kick(Ctx& c, double dt);
 void launch_thermo(Ctx& c, double out[27]);
+// once per device, with that device current: address of the fixed partial-sum array
+// of launch_thermo_noalloc on that device
+void thermo_scratch_init(int device);
+// xs_thermo into device memory (27 doubles at out27_dev) through the fixed array:
+// no allocation, no copy, no synchronisation
+void launch_thermo_noalloc(Ctx& c, double* out27_dev);
 double stream_copy(Ctx& c, size_t bytes, int reps);
 
 // ---- launches (k_pair.hip) ----

@@ -196,6 +196,17 @@ xs_status xs_push_f_v(xs_ctx* ctx, double dt, double dt_scale);
  * simulation_thermodynamic_state.cpp:101-163): out[27] = virial(9), kinetic tensor
  * (9), momentum(3), kinetic energy(3), potential energy, mass, particle count. */
 xs_status xs_thermo(xs_ctx* ctx, double out27[27]);
+/* No counterpart in exaStamp.  For callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops): xs_thermo with the same kernels and
+ * the same summation order, but the partial sums go to a fixed device array of
+ * 1024 x 26 doubles (208 KiB, one per device, held until the process ends) and the
+ * 27 results are written to device memory at out27_dev.  The call allocates nothing,
+ * copies nothing to the host and does not synchronise: read out27_dev after
+ * xs_synchronize (or after ordering another stream behind the context's).  Every
+ * partial sum it reads is written by the same call, so nothing needs clearing; calls
+ * that overlap on one device (several contexts on other streams or threads) race on
+ * the array and must be serialised by the caller. */
+xs_status xs_thermo_noalloc(xs_ctx* ctx, double* out27_dev);
 
 /* --- EAM adjoint --------------------------------------------------------------------- */
 /* No counterpart in exaStamp: reverse mode of eam_alloy_force with respect to the
