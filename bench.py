@@ -350,6 +350,7 @@ def main():
     cfg = inputs.lattice_config((nc, nc, nc), 3.615, [63.546], T=600.0, jitter=0.0, seed=args.seed)
     sim = Simulation(0)
     setup(sim, cfg, tab)
+    s_table = sim.eam_table_info()
 
     log(f"mixed precision: {args.warmup} warmup + {args.steps} timed steps, {cfg.n} atoms")
     mixed = run_form(sim, cfg, tab, True, args.steps, args.warmup, dt, rc, skin)
@@ -458,6 +459,7 @@ def main():
         "pairs_per_atom_within_rc": pairs_per_atom,
         "list_entries_per_atom": float(entries[0]) / cfg.n,
         "neighbor_list": list_info,
+        "fp32_s_table": s_table,
         "kernels_mixed": mixed["kernels"],
         "kernels_fp64": None if fp64 is None else fp64["kernels"],
         "newton_symmetric": sym,

@@ -80,6 +80,7 @@ SIGNATURES = {
     "xs_eam_alloy_load": (C.c_int, [_CTX, C.c_char_p, _D, C.POINTER(C.c_int)]),
     "xs_eam_alloy_force": (C.c_int, [_CTX, C.c_double, C.c_int, C.c_int, C.c_int, C.c_uint]),
     "xs_eam_alloy_types": (C.c_int, [_CTX, C.c_uint]),
+    "xs_eam_table_info": (C.c_int, [_CTX, C.POINTER(C.c_int), _D, _D]),
     "xs_johnson_force": (C.c_int, [_CTX, _D, C.c_double, C.c_int, C.c_int, C.c_uint]),
     "xs_push_f_v_r": (C.c_int, [_CTX, C.c_double, C.c_double]),
     "xs_push_f_v": (C.c_int, [_CTX, C.c_double, C.c_double]),

@@ -183,6 +183,7 @@ xs_status xs_ctx_create(int device, xs_ctx** out)
     c->lds_max = prop.sharedMemPerBlockOptin > prop.sharedMemPerBlock ? prop.sharedMemPerBlockOptin
                                                                        : prop.sharedMemPerBlock;
     if (const char* ls = std::getenv("XSMD_LIST_SORT")) c->list_sort_disabled = std::strcmp(ls, "0") == 0;
+    if (const char* st = std::getenv("XSMD_EAM_S_TABLE")) c->s_table_disabled = std::strcmp(st, "0") == 0;
     if (const char* sb = std::getenv("XSMD_SORT_BITS")) {
       const int v = std::atoi(sb);
       if (v >= 1 && v <= 8) c->sort_bits = v;
@@ -572,6 +573,17 @@ xs_status xs_eam_alloy_types(xs_ctx* h, unsigned enabled)
       c.rho_state = RhoState::None;  // rho and F' of the previous mask no longer hold
       c.rho_ckpt = false;
     }
+  });
+}
+
+xs_status xs_eam_table_info(xs_ctx* h, int* s_pieces, double* r_lo, double err[3])
+{
+  return guarded(h, [&](Ctx& c) {
+    require(c.has_eam, XS_E_STATE, "eam tables not set");
+    if (s_pieces) *s_pieces = c.eam.s_pieces;
+    if (r_lo) *r_lo = std::sqrt(c.eam.s_lo);
+    if (err)
+      for (int k = 0; k < 3; k++) err[k] = c.eam.s_err[k];
   });
 }
 

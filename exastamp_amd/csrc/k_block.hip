@@ -848,7 +848,7 @@ __global__ void __launch_bounds__(kPassThreads, kPassWaves<false>) k_block_eam_r
 #pragma unroll
       for (int u = 0; u < kBBatch; u++) {
         const int tj = MULTI ? l.ltype[nb[u].c] : 0;
-        const Real val = pm::eam_rho<Real, MULTI>(t, pair, ti, tj, nb[u].in ? nb[u].d2 : 1.0);
+        const Real val = pm::eam_rho_knot<Real, MULTI>(t, pair, ti, tj, nb[u].d2, nb[u].in);
         rho += nb[u].in ? (double)val : 0.0;
       }
     }
@@ -910,18 +910,40 @@ __global__ void __launch_bounds__(kPassThreads, kPassWaves<VIRIAL>) k_block_eam_
           nb[1].in = nb[1].in && pm::eam_on(t, l.ltype[nb[1].c]);
         }
         if (!__any(nb[0].in || nb[1].in)) continue;  // as in the density pass
-#pragma unroll
-        for (int u = 0; u < 2; u++) {
-          const int tj = MULTI ? l.ltype[nb[u].c] : 0;
-          Real fp_r, phi;
-          pm::eam_pair<Real, MULTI>(t, pair, ti, tj, fpi, (Real)nb[u].w, nb[u].in ? nb[u].d2 : 1.0, fp_r, phi);
-          const double fpair = nb[u].in ? (double)fp_r : 0.0;
-          const double fe0 = nb[u].x * fpair, fe1 = nb[u].y * fpair, fe2 = nb[u].z * fpair;
+        auto add = [&](const LNb& q, bool in, Real fp, Real ph) {
+          const double fpair = in ? (double)fp : 0.0;
+          const double fe0 = q.x * fpair, fe1 = q.y * fpair, fe2 = q.z * fpair;
           fx += fe0;
           fy += fe1;
           fz += fe2;
-          if constexpr (ENERGY) e += nb[u].in ? 0.5 * (double)phi : 0.0;
-          if constexpr (VIRIAL) pm::add_virial(v, fe0, fe1, fe2, nb[u].x, nb[u].y, nb[u].z);
+          if constexpr (ENERGY) e += in ? 0.5 * (double)ph : 0.0;
+          if constexpr (VIRIAL) pm::add_virial(v, fe0, fe1, fe2, q.x, q.y, q.z);
+        };
+        if constexpr (pm::kSTable<Real, MULTI>) {
+          // FP32 single element: the s-table; the knot rows for pairs outside its range
+          const int tj[2] = {0, 0};
+          Real fp_r[2], phi[2] = {Real(0), Real(0)};
+          bool slow[2];
+          pm::eam_pair_batch<Real, MULTI, ENERGY, 2>(t, pair, ti, tj, fpi, nb, fp_r, phi, slow);
+#pragma unroll
+          for (int u = 0; u < 2; u++) add(nb[u], nb[u].in && !slow[u], fp_r[u], phi[u]);
+          if (__builtin_expect(__any(slow[0] || slow[1]), 0)) {
+#pragma unroll
+            for (int u = 0; u < 2; u++)
+              if (slow[u]) {
+                Real fp, ph = Real(0);
+                pm::eam_pair_knot<Real, false>(t, pair, 0, 0, fpi, (Real)nb[u].w, nb[u].d2, true, fp, ph);
+                add(nb[u], true, fp, ph);
+              }
+          }
+        } else {
+#pragma unroll
+          for (int u = 0; u < 2; u++) {
+            const int tj = MULTI ? l.ltype[nb[u].c] : 0;
+            Real fp_r, phi;
+            pm::eam_pair_knot<Real, MULTI>(t, pair, ti, tj, fpi, (Real)nb[u].w, nb[u].d2, nb[u].in, fp_r, phi);
+            add(nb[u], nb[u].in, fp_r, phi);
+          }
         }
         __builtin_amdgcn_sched_barrier(0);
       }

@@ -146,7 +146,7 @@ __global__ void __launch_bounds__(256) k_eam_rho(PairCommon pc, EamTabDev<Real> 
       for (int u = 0; u < kBatch; u++) {
         const int tj = MULTI ? pc.type[nb[u].j] : 0;
         if constexpr (MULTI) nb[u].in = nb[u].in && eam_on(t, tj);
-        const Real val = eam_rho<Real, MULTI>(t, pair, ti, tj, nb[u].in ? nb[u].d2 : 1.0);
+        const Real val = eam_rho_knot<Real, MULTI>(t, pair, ti, tj, nb[u].d2, nb[u].in);
         rho += nb[u].in ? (double)val : 0.0;
       }
     }
@@ -183,18 +183,30 @@ __global__ void __launch_bounds__(256) k_eam_force(PairCommon pc, EamTabDev<Real
   for (int k0 = 0; k0 < cnt; k0 += kBatch) {
     Nb nb[kBatch];
     fetch<kBatch>(pc, i, k0, cnt, pi, L, nb);
+    int tj[kBatch];
 #pragma unroll
     for (int u = 0; u < kBatch; u++) {
-      const int tj = MULTI ? pc.type[nb[u].j] : 0;
-      if constexpr (MULTI) nb[u].in = nb[u].in && eam_on(t, tj);
-      Real fp_r, phi;
-      eam_pair<Real, MULTI>(t, pair, ti, tj, fpi, (Real)nb[u].w, nb[u].in ? nb[u].d2 : 1.0, fp_r, phi);
-      const double fpair = nb[u].in ? (double)fp_r : 0.0;
+      tj[u] = MULTI ? pc.type[nb[u].j] : 0;
+      if constexpr (MULTI) nb[u].in = nb[u].in && eam_on(t, tj[u]);
+    }
+    Real fp_r[kBatch], phi[kBatch];
+    bool slow[kBatch];
+#pragma unroll
+    for (int u = 0; u < kBatch; u++) phi[u] = Real(0);
+    eam_pair_batch<Real, MULTI, ENERGY, kBatch>(t, pair, ti, tj, fpi, nb, fp_r, phi, slow);
+    if constexpr (kSTable<Real, MULTI>) {
+#pragma unroll
+      for (int u = 0; u < kBatch; u++)
+        if (slow[u]) eam_pair_knot<Real, false>(t, pair, 0, 0, fpi, (Real)nb[u].w, nb[u].d2, true, fp_r[u], phi[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < kBatch; u++) {
+      const double fpair = nb[u].in ? (double)fp_r[u] : 0.0;
       const double fe0 = nb[u].x * fpair, fe1 = nb[u].y * fpair, fe2 = nb[u].z * fpair;
       fx += fe0;
       fy += fe1;
       fz += fe2;
-      if constexpr (ENERGY) e += nb[u].in ? 0.5 * (double)phi : 0.0;
+      if constexpr (ENERGY) e += nb[u].in ? 0.5 * (double)phi[u] : 0.0;
       if constexpr (VIRIAL) add_virial(v, fe0, fe1, fe2, nb[u].x, nb[u].y, nb[u].z);
     }
   }
@@ -478,17 +490,21 @@ __global__ void __launch_bounds__(256) k_eam_rho_sym(PairCommon pc, EamTabDev<Re
   for (int k0 = 0; k0 < cnt; k0 += kBatch) {
     Nb nb[kBatch];
     fetch<kBatch>(pc, i, k0, cnt, pi, L, nb);
+    int tj[kBatch];
 #pragma unroll
     for (int u = 0; u < kBatch; u++) {
-      const int tj = MULTI ? pc.type[nb[u].j] : 0;
-      if constexpr (MULTI) nb[u].in = nb[u].in && eam_on(t, tj);
-      const double d2 = nb[u].in ? nb[u].d2 : 1.0;
-      const Real to_i = eam_rho<Real, MULTI>(t, pair, ti, tj, d2);
-      Real to_j = to_i;
-      if constexpr (MULTI) to_j = eam_rho<Real, MULTI>(t, pair, tj, ti, d2);
+      tj[u] = MULTI ? pc.type[nb[u].j] : 0;
+      if constexpr (MULTI) nb[u].in = nb[u].in && eam_on(t, tj[u]);
+    }
+    Real to_i[kBatch], to_j[kBatch];
+#pragma unroll
+    for (int u = 0; u < kBatch; u++) {
+      to_i[u] = eam_rho_knot<Real, MULTI>(t, pair, ti, tj[u], nb[u].d2, nb[u].in);
+      to_j[u] = to_i[u];
+      if constexpr (MULTI) to_j[u] = eam_rho_knot<Real, MULTI>(t, pair, tj[u], ti, nb[u].d2, nb[u].in);
       if (nb[u].in) {
-        rho += (double)to_i;
-        atomicAdd(rho_acc + nb[u].j, (double)to_j);
+        rho += (double)to_i[u];
+        atomicAdd(rho_acc + nb[u].j, (double)to_j[u]);
       }
     }
   }
@@ -542,16 +558,28 @@ __global__ void __launch_bounds__(256) k_eam_force_sym(PairCommon pc, EamTabDev<
   for (int k0 = 0; k0 < cnt; k0 += kBatch) {
     Nb nb[kBatch];
     fetch<kBatch>(pc, i, k0, cnt, pi, L, nb);
+    int tj[kBatch];
 #pragma unroll
     for (int u = 0; u < kBatch; u++) {
-      const int tj = MULTI ? pc.type[nb[u].j] : 0;
-      if constexpr (MULTI) nb[u].in = nb[u].in && eam_on(t, tj);
-      Real fp_r, phi;
-      eam_pair<Real, MULTI>(t, pair, ti, tj, fpi, (Real)nb[u].w, nb[u].in ? nb[u].d2 : 1.0, fp_r, phi);
+      tj[u] = MULTI ? pc.type[nb[u].j] : 0;
+      if constexpr (MULTI) nb[u].in = nb[u].in && eam_on(t, tj[u]);
+    }
+    Real fp_r[kBatch], phi[kBatch];
+    bool slow[kBatch];
+#pragma unroll
+    for (int u = 0; u < kBatch; u++) phi[u] = Real(0);
+    eam_pair_batch<Real, MULTI, ENERGY, kBatch>(t, pair, ti, tj, fpi, nb, fp_r, phi, slow);
+    if constexpr (kSTable<Real, MULTI>) {
+#pragma unroll
+      for (int u = 0; u < kBatch; u++)
+        if (slow[u]) eam_pair_knot<Real, false>(t, pair, 0, 0, fpi, (Real)nb[u].w, nb[u].d2, true, fp_r[u], phi[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < kBatch; u++) {
       if (nb[u].in) {
-        const double fpair = (double)fp_r;
+        const double fpair = (double)fp_r[u];
         const double fe0 = nb[u].x * fpair, fe1 = nb[u].y * fpair, fe2 = nb[u].z * fpair;
-        const double eh = 0.5 * (double)phi;
+        const double eh = 0.5 * (double)phi[u];
         fx += fe0;
         fy += fe1;
         fz += fe2;
@@ -711,6 +739,23 @@ EamTabDev<float> make_eam_tab<float>(Ctx& c)
   t.rdrho = (float)c.eam.rdrho;
   t.rhomax = (float)c.eam.rhomax;
   t.conv = (float)kEV;
+  // s = r^2 tables after the two knot planes of the FP32 pair table (setfl.cpp)
+  const EamHost& e = c.eam;
+  if (e.s_pieces > 0) {
+    const int knot_planes = 8 * (e.nr + 1);
+    t.sprows = e.s_pieces;
+    t.s_force = (knot_planes + 7) / 8 * 8;
+    t.s_phi = t.s_force + 8 * e.s_pieces;
+    t.s_rds = e.s_pieces / (e.s_hi - e.s_lo);
+    t.s_off = -e.s_lo * t.s_rds;
+    t.s_xmax = e.s_pieces;
+  } else {  // no s-table: x = -1 for every d^2, so every pair reads the knot rows
+    t.sprows = 1;
+    t.s_force = t.s_phi = 0;
+    t.s_rds = 0.0;
+    t.s_off = -1.0;
+    t.s_xmax = 1.0;
+  }
   std::memcpy(t.t2rhor, c.eam.t2rhor, sizeof(t.t2rhor));
   std::memcpy(t.t2z2r, c.eam.t2z2r, sizeof(t.t2z2r));
   std::memcpy(t.t2frho, c.eam.t2frho, sizeof(t.t2frho));

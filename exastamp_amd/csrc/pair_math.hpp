@@ -204,6 +204,15 @@ __device__ __forceinline__ Real dcubic(const Q4<Real>& c, Real p)
 {
   return (Real(3) * c.a * p + Real(2) * c.b) * p + c.c;
 }
+// The FP32 forms spell out their fused multiply-adds, so every kernel that shares
+// them (block and global lists, full and half lists) rounds them alike whatever the
+// compiler's contraction choices around them.
+__device__ __forceinline__ float cubic(const Q4<float>& c, float p) { return fmaf(fmaf(fmaf(c.a, p, c.b), p, c.c), p, c.d); }
+__device__ __forceinline__ float quad(const Q4<float>& c, float p) { return fmaf(fmaf(c.a, p, c.b), p, c.c); }
+__device__ __forceinline__ float dcubic(const Q4<float>& c, float p)
+{
+  return fmaf(fmaf(3.0f * c.a, p, 2.0f * c.b), p, c.c);
+}
 
 // types slot of eam_alloy_force (m_pair_enabled, eam_potential_multimat.cu:154-166):
 // species `type` takes part when its bit is set; a pair counts when both species do,
@@ -220,7 +229,9 @@ template <class Real>
 __device__ __forceinline__ void embed(const EamTabDev<Real>& t, int ti, double rho, Real& F, Real& Fp)
 {
   const Real rr = (Real)rho;
-  Real p = rr * t.rdrho + Real(1);
+  Real p;
+  if constexpr (sizeof(Real) == 4) p = fmaf(rr, t.rdrho, 1.0f);
+  else p = rr * t.rdrho + Real(1);
   int m = (int)p;
   m = max(1, min(m, t.nrho - 1));
   p -= (Real)m;
@@ -230,7 +241,10 @@ __device__ __forceinline__ void embed(const EamTabDev<Real>& t, int ti, double r
   const Q4<Real> dv = load4(row + 4);
   Fp = quad(dv, p);
   F = cubic(v, p);
-  if (rr > t.rhomax) F += Fp * (rr - t.rhomax);
+  if (rr > t.rhomax) {
+    if constexpr (sizeof(Real) == 4) F = fmaf(Fp, rr - t.rhomax, F);
+    else F += Fp * (rr - t.rhomax);
+  }
   F *= t.conv;
   Fp *= t.conv;
 }
@@ -250,21 +264,6 @@ __device__ __forceinline__ const Real* pair_z2_row(const Real* __restrict__ pair
   return pair + ((int64_t)t.nr + 1 + m) * 4;
 }
 
-// rho_{type_j}(r) contribution to atom i (SymRhoOp, eam_force_op_multimat.h:142-171).
-// Single-element tables read the rho cubic from plane 0 of the pair table.
-template <class Real, bool MULTI>
-__device__ __forceinline__ Real eam_rho(const EamTabDev<Real>& t, const Real* __restrict__ pair, int ti, int tj,
-                                        double d2)
-{
-  Real p, rinv;
-  int m;
-  knot_dist(t, d2, m, p, rinv);
-  const Real* row;
-  if constexpr (MULTI) row = row_r(t.rhor, t, t.t2rhor[tj + 1][ti + 1], m);
-  else row = pair_rho_row(pair, m);
-  return cubic(load4(row), p);
-}
-
 // Single-element psi'(r)/r and phi(r) from the two rows of knot m:
 // rc = rho value cubic, zc = z2r value cubic; the derivative quadratics follow
 // from the value cubics (times 1/dr).
@@ -276,21 +275,102 @@ __device__ __forceinline__ void eam_pair_rows(const EamTabDev<Real>& t, const Q4
   const Real z2p = dcubic(zc, p) * t.rdr;
   const Real z2 = cubic(zc, p);
   phi = z2 * rinv;
-  const Real phip = (z2p * rinv - phi * rinv) * t.conv;
+  Real phip, psip;
+  if constexpr (sizeof(Real) == 4) {
+    phip = fmaf(z2p, rinv, -(phi * rinv)) * t.conv;
+    psip = fmaf(fpi + fpj, rhop, phip);
+  } else {
+    phip = (z2p * rinv - phi * rinv) * t.conv;
+    psip = (fpi + fpj) * rhop + phip;
+  }
   phi *= t.conv;
-  const Real psip = (fpi + fpj) * rhop + phip;
   fpair = psip * rinv;
 }
 
-// psi'(r)/r and phi(r) of eam_alloy_mm_force (eam_alloy.h:222-265):
-// psi' = F'_i rho'_j + F'_j rho'_i + phi', phi = z2r / r.
+// ---------------------------------------------------------------------------
+// FP32 tables in s = r^2 (single element, mixed form, force pass)
+// ---------------------------------------------------------------------------
+// The mixed force pass of one element reads, instead of the knot rows, cubic pieces
+// in s = r^2 over [s_lo, s_hi] = [(0.3 rmax)^2, rmax^2] (rmax = (nr - 1) dr), uniform
+// in s: piece k covers [s_lo + k ds, s_lo + (k+1) ds] in the local coordinate
+// p = (s - s_lo) / ds - k.  Each piece is the Hermite cubic through the reference
+// spline's FP64 values and s-slopes at its two ends (value and slope continuous from
+// piece to piece), of
+//   force plane: rho'(r)/r | phi'(r) conv / r   [A B C 0 | A B C 0]: the s-derivative
+//                quadratics times 2/ds, so (A p + B) p + C is the function itself
+//   phi plane:   phi(r) conv = z2r(r) conv / r  [a b c d]  (energy)
+// so a pair needs no square root and no division: its index comes from the FP64
+// d^2 (p formed in FP64, rounded to FP32 in [0, 1]), and
+// psi'(r)/r = (F'_i + F'_j) rho'(r)/r + phi'(r)/r is one FMA of two quadratics.
+// The host takes the fewest pieces (256 .. 8192, doubling) whose FP32 evaluation stays
+// within 8 FP32 ulps of the largest |rho'/r|, |phi'/r| and 4 of the largest |phi| of
+// the FP64 spline over [s_lo, s_hi] (the FP32 knot rows are at about 1-4 of each), or
+// no s-table (then every pair reads the knot rows).  A pair with d^2 outside
+// [s_lo, s_hi] (closer than 0.3 rmax, or past the table's end) takes the knot rows:
+// the reference evaluates r beyond rmax at the table's end, which the s-table cannot
+// express.  An FP64 cubic in s could not reproduce the spline to FP64 precision, so
+// the FP64 form keeps the knot rows; the density pass, a single cubic per pair, keeps
+// them too (its FP32 s-table form measured no faster: the pass is bound by its
+// gathers, which the s-table does not reduce).
+
+// row and local coordinate of d^2 in the s-table; `out` = d^2 outside [s_lo, s_hi]
+// (with no s-table the host sets a range no d^2 falls in, so every pair is `out`)
+__device__ __forceinline__ void s_knot(const EamTabDev<float>& t, double d2, int& m, float& p, bool& out)
+{
+  const double x0 = fma(d2, t.s_rds, t.s_off);
+  const double x = fmin(fmax(x0, 0.0), t.s_xmax);
+  out = x != x0;
+  m = min((int)x, t.sprows - 1);
+  p = (float)(x - (double)m);
+}
+
+// d^2 outside the s-table's range [s_lo, s_hi]
+__device__ __forceinline__ bool s_out(const EamTabDev<float>& t, double d2)
+{
+  const double x = fma(d2, t.s_rds, t.s_off);
+  return !(x >= 0.0 && x <= t.s_xmax);
+}
+
+// the slope quadratic [A B C] of an s-table force row (its fourth value is padding)
+struct Q3 {
+  float a, b, c;
+};
+__device__ __forceinline__ Q3 load3(const float* __restrict__ p)
+{
+  const float3 v = *reinterpret_cast<const float3*>(p);
+  return {v.x, v.y, v.z};
+}
+__device__ __forceinline__ float quad3(const Q3& c, float p) { return fmaf(fmaf(c.a, p, c.b), p, c.c); }
+
+// The FP32 single-element form reads the s-table; every other form the knot rows.
 template <class Real, bool MULTI>
-__device__ __forceinline__ void eam_pair(const EamTabDev<Real>& t, const Real* __restrict__ pair, int ti, int tj,
-                                         Real fpi, Real fpj, double d2, Real& fpair, Real& phi)
+constexpr bool kSTable = !MULTI && sizeof(Real) == 4;
+
+// rho_{type_j}(r) contribution to atom i (SymRhoOp, eam_force_op_multimat.h:142-171)
+// from the knot rows (single element: plane 0 of the pair table).  `in` = the pair
+// counts; the value of a pair that does not is arbitrary (callers mask it).
+template <class Real, bool MULTI>
+__device__ __forceinline__ Real eam_rho_knot(const EamTabDev<Real>& t, const Real* __restrict__ pair, int ti, int tj,
+                                             double d2, bool in)
 {
   Real p, rinv;
   int m;
-  knot_dist(t, d2, m, p, rinv);
+  knot_dist(t, in ? d2 : 1.0, m, p, rinv);
+  const Real* row;
+  if constexpr (MULTI) row = row_r(t.rhor, t, t.t2rhor[tj + 1][ti + 1], m);
+  else row = pair_rho_row(pair, m);
+  return cubic(load4(row), p);
+}
+
+// psi'(r)/r and phi(r) of eam_alloy_mm_force (eam_alloy.h:222-265) from the knot
+// rows: psi' = F'_i rho'_j + F'_j rho'_i + phi', phi = z2r / r.
+template <class Real, bool MULTI>
+__device__ __forceinline__ void eam_pair_knot(const EamTabDev<Real>& t, const Real* __restrict__ pair, int ti, int tj,
+                                              Real fpi, Real fpj, double d2, bool in, Real& fpair, Real& phi)
+{
+  Real p, rinv;
+  int m;
+  knot_dist(t, in ? d2 : 1.0, m, p, rinv);
   if constexpr (MULTI) {
     const Real rhoip = quad(load4(row_r(t.rhor, t, t.t2rhor[ti + 1][tj + 1], m) + 4), p);
     const Real rhojp = quad(load4(row_r(t.rhor, t, t.t2rhor[tj + 1][ti + 1], m) + 4), p);
@@ -298,12 +378,52 @@ __device__ __forceinline__ void eam_pair(const EamTabDev<Real>& t, const Real* _
     const Real z2p = quad(load4(zrow + 4), p);
     const Real z2 = cubic(load4(zrow), p);
     phi = z2 * rinv;
-    const Real phip = (z2p * rinv - phi * rinv) * t.conv;
+    Real phip, psip;
+    if constexpr (sizeof(Real) == 4) {
+      phip = fmaf(z2p, rinv, -(phi * rinv)) * t.conv;
+      psip = fmaf(fpi, rhojp, fmaf(fpj, rhoip, phip));
+    } else {
+      phip = (z2p * rinv - phi * rinv) * t.conv;
+      psip = fpi * rhojp + fpj * rhoip + phip;
+    }
     phi *= t.conv;
-    const Real psip = fpi * rhojp + fpj * rhoip + phip;
     fpair = psip * rinv;
   } else {
     eam_pair_rows(t, load4(pair_rho_row(pair, m)), load4(pair_z2_row(pair, t, m)), p, rinv, fpi, fpj, fpair, phi);
+  }
+}
+
+// Pair scalars psi'(r)/r (and, with PHI, phi(r)) of U pairs (nb[u].d2, nb[u].in,
+// nb[u].w = F'_j; neighbour species tj[u]).  The s-table form evaluates all U from
+// the s-table (their row loads in flight together: psi'/r = (F'_i + F'_j) rho'(r)/r +
+// phi'(r)/r from the two slope quadratics of one 32-byte row) and flags slow[u] = a
+// counted pair outside the s-table's range, which the caller evaluates with
+// eam_pair_knot, only in the rare wave where some slow[u] is set.  Every other form
+// reads the knot rows and leaves slow[] false.
+template <class Real, bool MULTI, bool PHI, int U, class Nb>
+__device__ __forceinline__ void eam_pair_batch(const EamTabDev<Real>& t, const Real* __restrict__ pair, int ti,
+                                               const int (&tj)[U], Real fpi, const Nb (&nb)[U], Real (&fpair)[U],
+                                               Real (&phi)[U], bool (&slow)[U])
+{
+  if constexpr (kSTable<Real, MULTI>) {
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      int m;
+      float p;
+      bool out;
+      s_knot(t, nb[u].d2, m, p, out);
+      const Real* row = pair + t.s_force + (int64_t)m * 8;
+      const Q3 gr = load3(row), gz = load3(row + 4);
+      fpair[u] = fmaf(fpi + (Real)nb[u].w, quad3(gr, p), quad3(gz, p));
+      if constexpr (PHI) phi[u] = cubic(load4(pair + t.s_phi + (int64_t)m * 4), p);
+      slow[u] = nb[u].in && out;
+    }
+  } else {
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      eam_pair_knot<Real, MULTI>(t, pair, ti, tj[u], fpi, (Real)nb[u].w, nb[u].d2, nb[u].in, fpair[u], phi[u]);
+      slow[u] = false;
+    }
   }
 }
 

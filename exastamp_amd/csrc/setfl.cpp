@@ -2,6 +2,8 @@
 // the EAM tables to the device in FP64 and FP32.
 #include "xsmd_internal.hpp"
 
+#include <algorithm>
+#include <cmath>
 #include <fstream>
 #include <limits>
 
@@ -119,6 +121,108 @@ void upload(Ctx& c, DBuf<Real>& buf, const std::vector<double>& coeffs)
   XS_HIP(hipStreamSynchronize(c.stream));
 }
 
+// The reference spline of one table (FP64 rows [c0..c6, 0], row m = knot m) at r:
+// value and slope, with eam_alloy.h's index and clamp steps.
+void spline_at(const double* rows, int nr, double rdr, double r, double& v, double& d)
+{
+  double p = r * rdr + 1.0;
+  const int m = std::max(0, std::min((int)p, nr - 1));
+  p -= (double)m;
+  p = std::min(p, 1.0);
+  const double* c = rows + 8 * (size_t)m;
+  v = ((c[3] * p + c[4]) * p + c[5]) * p + c[6];
+  d = (c[0] * p + c[1]) * p + c[2];
+}
+
+// rho(r), rho'(r)/r, phi(r) conv and phi'(r) conv / r of the single-element spline
+// (phi = z2r / r) and the s-slopes (d/ds = d/dr / (2 r)) of rho and phi conv.
+struct SFuncs {
+  double rho, grho, phi, gphi, drho_ds, dphi_ds;
+};
+SFuncs s_funcs(const double* cr, const double* cz, int nr, double rdr, double r)
+{
+  double rv, rd, zv, zd;
+  spline_at(cr, nr, rdr, r, rv, rd);
+  spline_at(cz, nr, rdr, r, zv, zd);
+  const double phi = zv / r * kEV;
+  const double phip = (zd / r - zv / (r * r)) * kEV;
+  return {rv, rd / r, phi, phip / r, rd / (2.0 * r), phip / (2.0 * r)};
+}
+
+// The FP32 s = r^2 tables (layout and the accuracy rule in pair_math.hpp) appended to
+// out: the force plane, the slope quadratics of rho and phi conv times 2/ds
+// [A B C 0 | A B C 0], then the phi plane, phi conv [a b c d].
+void s_tables(Ctx& c, const double* cr, const double* cz, std::vector<float>& out)
+{
+  EamHost& e = c.eam;
+  e.s_pieces = 0;
+  for (int k = 0; k < 3; k++) e.s_err[k] = 0.0;
+  if (c.s_table_disabled) return;
+  const int nr = e.nr;
+  const double rdr = 1.0 / e.dr, rmax = (nr - 1) * e.dr;
+  const double s_lo = (0.3 * rmax) * (0.3 * rmax), s_hi = rmax * rmax;
+  for (int npc = 256; npc <= 8192; npc *= 2) {
+    const double ds = (s_hi - s_lo) / npc, rds = npc / (s_hi - s_lo), off = -s_lo * rds, g2 = 2.0 / ds;
+    std::vector<SFuncs> node(npc + 1);
+    for (int k = 0; k <= npc; k++) node[k] = s_funcs(cr, cz, nr, rdr, std::sqrt(s_lo + k * ds));
+    std::vector<float> frc(8 * (size_t)npc), phi(4 * (size_t)npc);
+    // Hermite cubic through (v0, d0 ds) and (v1, d1 ds) on p in [0, 1]: its value
+    // coefficients and, times 2/ds, its p-slope quadratic
+    auto hermite = [&](double v0, double v1, double d0, double d1, float* val, float* slope) {
+      const double D0 = d0 * ds, D1 = d1 * ds;
+      const double A = 2.0 * (v0 - v1) + D0 + D1, B = 3.0 * (v1 - v0) - 2.0 * D0 - D1;
+      if (val) {
+        val[0] = (float)A;
+        val[1] = (float)B;
+        val[2] = (float)D0;
+        val[3] = (float)v0;
+      }
+      slope[0] = (float)(3.0 * A * g2);
+      slope[1] = (float)(2.0 * B * g2);
+      slope[2] = (float)(D0 * g2);
+      slope[3] = 0.0f;
+    };
+    for (int k = 0; k < npc; k++) {
+      const SFuncs &a = node[k], &b = node[k + 1];
+      hermite(a.rho, b.rho, a.drho_ds, b.drho_ds, nullptr, &frc[8 * (size_t)k]);
+      hermite(a.phi, b.phi, a.dphi_ds, b.dphi_ds, &phi[4 * (size_t)k], &frc[8 * (size_t)k + 4]);
+    }
+    // FP32 evaluation (the device's steps) against the FP64 spline, 8 points per piece
+    double err[3] = {0, 0, 0}, mx[3] = {0, 0, 0};
+    for (int k = 0; k < npc; k++)
+      for (int q = 0; q < 8; q++) {
+        const double sq = s_lo + (k + (q + 0.5) / 8.0) * ds;
+        const SFuncs x = s_funcs(cr, cz, nr, rdr, std::sqrt(sq));
+        const double xx = std::fma(sq, rds, off);
+        const int m = std::min((int)xx, npc - 1);
+        const float p = (float)(xx - m);
+        const float* fv = &frc[8 * (size_t)m];
+        const float* pv = &phi[4 * (size_t)m];
+        const float got[3] = {std::fma(std::fma(fv[0], p, fv[1]), p, fv[2]),
+                              std::fma(std::fma(std::fma(pv[0], p, pv[1]), p, pv[2]), p, pv[3]),
+                              std::fma(std::fma(fv[4], p, fv[5]), p, fv[6])};
+        const double want[3] = {x.grho, x.phi, x.gphi};
+        for (int f = 0; f < 3; f++) {
+          err[f] = std::max(err[f], std::abs((double)got[f] - want[f]));
+          mx[f] = std::max(mx[f], std::abs(want[f]));
+        }
+      }
+    const double ulp = std::ldexp(1.0, -24);
+    double u[3];
+    for (int f = 0; f < 3; f++) u[f] = mx[f] > 0.0 ? err[f] / (ulp * mx[f]) : 0.0;
+    if (u[0] <= 8.0 && u[1] <= 4.0 && u[2] <= 8.0) {
+      while (out.size() % 8) out.push_back(0.0f);  // 32-byte aligned planes
+      e.s_pieces = npc;
+      e.s_lo = s_lo;
+      e.s_hi = s_hi;
+      for (int f = 0; f < 3; f++) e.s_err[f] = u[f];
+      out.insert(out.end(), frc.begin(), frc.end());
+      out.insert(out.end(), phi.begin(), phi.end());
+      return;
+    }
+  }
+}
+
 std::vector<double> build_all(int ntab, int n, double delta, const std::vector<double>& knots)
 {
   std::vector<double> out((size_t)ntab * (n + 1) * 8, 0.0);
@@ -167,16 +271,17 @@ void eam_upload(Ctx& c)
       pair[4 * m + k] = cr[8 * m + 3 + k];
       pair[4 * (rows + m) + k] = cz[8 * m + 3 + k];
     }
-  auto up = [&](auto& buf, auto zero) {
-    using T = decltype(zero);
-    std::vector<T> tmp(pair.size());
-    for (size_t q = 0; q < pair.size(); q++) tmp[q] = (T)pair[q];
+  auto up = [&](auto& buf, const auto& tmp) {
     buf.reserve(tmp.size());
-    XS_HIP(hipMemcpyAsync(buf.p, tmp.data(), tmp.size() * sizeof(T), hipMemcpyHostToDevice, c.stream));
+    XS_HIP(hipMemcpyAsync(buf.p, tmp.data(), tmp.size() * sizeof(tmp[0]), hipMemcpyHostToDevice, c.stream));
     XS_HIP(hipStreamSynchronize(c.stream));
   };
-  up(e.pair_d, 0.0);
-  up(e.pair_f, 0.0f);
+  up(e.pair_d, pair);
+  // FP32: the same knot planes, then the s = r^2 tables of the mixed passes
+  std::vector<float> pf(pair.size());
+  for (size_t q = 0; q < pair.size(); q++) pf[q] = (float)pair[q];
+  s_tables(c, cr.data(), cz.data(), pf);
+  up(e.pair_f, pf);
 }
 
 }  // namespace xsmd

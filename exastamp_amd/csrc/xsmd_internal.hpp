@@ -108,6 +108,12 @@ struct EamTabDev {
   int8_t t2frho[8] = {};
   // types slot of eam_alloy_force: bit t set = species t takes part (alloy kernels)
   uint32_t types = ~0u;
+  // FP32 s = r^2 tables of the single-element pair table (pair_math.hpp): pieces per
+  // plane (0 = none), plane offsets in the pair buffer (in Reals), and the piece
+  // coordinate x = d^2 s_rds + s_off of d^2, valid in [0, s_xmax = sprows]
+  int sprows = 0;
+  int s_force = 0, s_phi = 0;
+  double s_rds = 0, s_off = 0, s_xmax = 0;
 };
 
 struct EamHost {
@@ -124,6 +130,12 @@ struct EamHost {
   // the two planes serve both EAM passes
   DBuf<double> pair_d;
   DBuf<float> pair_f;
+  // FP32 s = r^2 tables after the knot planes of pair_f (pair_math.hpp): pieces
+  // (0 = none), their range [s_lo, s_hi], and their largest FP32 errors against the
+  // FP64 spline in FP32 ulps of the largest |rho'/r|, |phi| and |phi'/r|
+  int s_pieces = 0;
+  double s_lo = 0, s_hi = 0;
+  double s_err[3] = {0, 0, 0};
 };
 
 // What the per-atom EAM scalar currently holds: rho (c.rho) after the density
@@ -294,6 +306,7 @@ struct Ctx {
   size_t lds_max = 65536;   // dynamic LDS one workgroup may use
   bool list_sort_disabled = false;  // XSMD_LIST_SORT=0: block lists stay in stencil order
   int sort_bits = 7;                // distance keys of the list sort: 2^sort_bits bins over [0, rl^2), <= 8
+  bool s_table_disabled = false;    // XSMD_EAM_S_TABLE=0: the FP32 passes read the knot rows
   bool drift_done = false;  // the last force launch applied ForceEpilogue::drift
 
   // small device scalars + pinned host mirror

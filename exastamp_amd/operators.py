@@ -260,6 +260,17 @@ class Simulation:
                                      _lib.dptr(zr))
         self.eam = t
 
+    def eam_table_info(self) -> dict:
+        """The FP32 s = r^2 tables of the mixed single-element force pass: pieces per table
+        (0 = none, knot rows), the shortest distance they cover, and their largest FP32
+        errors against the FP64 spline in FP32 ulps of the largest |value| or |slope/r|."""
+        n = C.c_int(0)
+        rlo = C.c_double(0.0)
+        err = np.zeros(3)
+        self.ctx.xs_eam_table_info(C.byref(n), C.byref(rlo), _lib.dptr(err))
+        return {"s_pieces": int(n.value), "r_lo": float(rlo.value),
+                "ulps": {"rho_slope_over_r": float(err[0]), "phi": float(err[1]), "phi_slope_over_r": float(err[2])}}
+
     JOHNSON_PARAMETERS = JOHNSON_PARAMETERS
 
     def johnson_force(self, rcut, parameters, compute_virial=False, **_slots):

@@ -175,6 +175,15 @@ xs_status xs_eam_alloy_force(xs_ctx* ctx, double rcut, int eam_rho, int eam_rho2
  * default) = every species, the empty list.  Applies to every later
  * xs_eam_alloy_force, xs_md_run and xs_eam_adjoint on the context. */
 xs_status xs_eam_alloy_types(xs_ctx* ctx, unsigned enabled);
+/* FP32 tables in s = r^2 that the mixed single-element force pass reads instead of
+ * the knot rows (no square root or division per pair): *s_pieces = cubic pieces per
+ * table (0 = none: the force pass reads the FP32 knot rows; also with the environment
+ * variable XSMD_EAM_S_TABLE=0 at context creation), *r_lo = the shortest distance
+ * they cover (closer pairs read the knot rows), err[3] = their largest FP32 errors
+ * against the FP64 spline in FP32 ulps of the largest |rho'/r|, |phi| and |phi'/r|
+ * over the range (taken when the slopes stay within 8 and phi within 4).  No
+ * counterpart in exaStamp (a layout of the mixed form); any output may be NULL. */
+xs_status xs_eam_table_info(xs_ctx* ctx, int* s_pieces, double* r_lo, double err[3]);
 /* johnson_force (single-material EAM template eam_potential.cu:62-193 with the
  * analytic functions of johnson.h:59-167): both passes, FP64, full list.
  * params: the 19 EamJohnsonParameters in internal units, in the order
