@@ -52,6 +52,9 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=12)
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--cpu-1t-steps", type=int, default=1, help="steps of the single-thread CPU sample (0: skip)")
+    ap.add_argument("--drift-steps", type=int, default=1000,
+                    help="NVE steps of the energy-drift leg of each form (thermo every 10 steps, linear fit; 0: skip)")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-symmetric", action="store_true", help="skip the full-list vs Newton-symmetric operator timing")
     ap.add_argument("--no-lj", action="store_true", help="skip the LJ argon leg")
@@ -81,20 +84,24 @@ def energy(sim, rc, mixed):
 
 
 def run_form(sim, cfg, tab, mixed, steps, warmup, dt, rc, skin):
+    """Warm-up, then `steps` timed NVE steps (no profiling inside the timed region),
+    then the same number of steps again with per-kernel HIP-event timers on."""
     from exastamp_amd import _lib
-    from exastamp_amd.units import EV
     import ctypes as C
     sim.set_config(cfg)
     st_w, _ = sim.md_run("eam", warmup, dt, rc, skin, mixed=mixed)
     th0 = energy(sim, rc, mixed)
-    sim.ctx.xs_profile_enable(1)
+    flags = (_lib.XS_MIXED if mixed else 0) | _lib.XS_CONTINUE
     sim.ctx.xs_synchronize()
     t0 = time.perf_counter()
     st = _lib.RunStats()
-    flags = (_lib.XS_MIXED if mixed else 0) | _lib.XS_CONTINUE
     sim.ctx.xs_md_run(_lib.XS_POT_EAM, steps, dt, rc, skin, flags, 0, None, C.byref(st))
     sim.ctx.xs_synchronize()
     t1 = time.perf_counter()
+    sim.ctx.xs_profile_enable(1)
+    st_p = _lib.RunStats()
+    sim.ctx.xs_md_run(_lib.XS_POT_EAM, steps, dt, rc, skin, flags, 0, None, C.byref(st_p))
+    sim.ctx.xs_synchronize()
     ms = np.zeros(len(_lib.PROF_CLASSES))
     cnt = np.zeros(len(_lib.PROF_CLASSES), dtype=np.int64)
     sim.ctx.xs_profile_get(_lib.dptr(ms), _lib.ptr(cnt, C.c_int64))
@@ -102,16 +109,39 @@ def run_form(sim, cfg, tab, mixed, steps, warmup, dt, rc, skin):
     thK = energy(sim, rc, mixed)
     n = cfg.n
     sec = t1 - t0
-    drift = (thK.total_energy - th0.total_energy) / EV / n / (steps * dt)
     kern = {name: {"ms_total": float(ms[k]), "launches": int(cnt[k]),
                    "ms_avg": float(ms[k] / cnt[k]) if cnt[k] else None}
             for k, name in enumerate(_lib.PROF_CLASSES)}
     return {
         "value": n * steps / sec, "seconds": sec, "ms_per_step": 1e3 * sec / steps,
-        "rebuilds": int(st.rebuilds), "kernels": kern,
-        "drift_ev_per_atom_per_ps": float(drift), "T0": th0.temperature, "T_end": thK.temperature,
+        "rebuilds": int(st.rebuilds), "kernels": kern, "kernels_run_rebuilds": int(st_p.rebuilds),
+        "T0": th0.temperature, "T_end": thK.temperature,
         "warmup_rebuilds": int(st_w.rebuilds),
     }
+
+
+def drift_leg(sim, cfg, tab, mixed, steps, dt, rc, skin):
+    """NVE energy drift of one form: `steps` steps from the initial state with the
+    total energy every 10 steps, slope of a least-squares line (eV/atom/ps)."""
+    from exastamp_amd.units import EV
+    sim.set_config(cfg)
+    _, th = sim.md_run("eam", steps, dt, rc, skin, mixed=mixed, thermo_every=10)
+    t = np.arange(len(th)) * 10 * dt
+    e = np.array([x.total_energy for x in th]) / EV / cfg.n
+    slope = float(np.polyfit(t, e, 1)[0])
+    return {"ev_per_atom_per_ps": slope, "steps": steps, "samples": len(th),
+            "rms_fluctuation_ev_per_atom": float(np.std(e - np.polyval(np.polyfit(t, e, 1), t)))}
+
+
+def gather_ceiling(sim, rc, kernels):
+    """Gather-ceiling kernels of both passes (their data movement without the spline
+    arithmetic) against the passes' own average times in the MD loop."""
+    out = {}
+    for name, force in (("density", False), ("force", True)):
+        ms = sim.gather_ceiling(rc, force, 10)
+        k = kernels["eam_force" if force else "eam_rho"]["ms_avg"]
+        out[name] = {"ceiling_ms": ms, "pass_ms": k, "frac": ms / k if k else None}
+    return out
 
 
 def force_kernel_name(list_info):
@@ -172,9 +202,13 @@ def adjoint_leg(name, cfg, masses, names, tab, nfd, repeats):
     are formed here with numpy (the torch harness in exastamp_amd.fitting does the same)."""
     import ctypes as C
     from exastamp_amd import _lib, inputs
-    from exastamp_amd.fitting import fit_loss_np
     from exastamp_amd.operators import Simulation
     from exastamp_amd.units import EV
+
+    def fit_loss_np(F, E, F_target, E_target, atoms_per_replica, energy_weight=0.1):
+        n = np.asarray(atoms_per_replica, dtype=np.float64)
+        return float(((F - F_target) ** 2).mean() + energy_weight * (((E - E_target) / n) ** 2).mean())
+
     theta0 = tab.theta()
     tab_t = tab.with_theta(theta0 * (1.0 + inputs.rng(5).uniform(-0.02, 0.02, theta0.size)))
     nper = np.bincount(cfg.rep) if cfg.rep is not None else np.array([cfg.n])
@@ -209,6 +243,16 @@ def adjoint_leg(name, cfg, masses, names, tab, nfd, repeats):
     sim.ctx.xs_profile_get(_lib.dptr(ms), _lib.ptr(cnt, C.c_int64))
     sim.ctx.xs_profile_enable(0)
     gpu_ms = {k: float(ms[i] / repeats) for i, k in enumerate(_lib.PROF_CLASSES) if cnt[i]}
+    # without the rho/F' checkpoint the backward would first redo the density pass and
+    # the embedding (F' is the force pass's input, so only rho's 8 bytes per atom are
+    # the checkpoint's own): time that recomputation
+    sim.ctx.xs_profile_enable(1)
+    for _ in range(repeats):
+        sim.eam_alloy_force(rcut=tab.rc, eam_rho=True, eam_rho2emb=True, eam_force=False, mixed=True)
+    sim.ctx.xs_synchronize()
+    sim.ctx.xs_profile_get(_lib.dptr(ms), _lib.ptr(cnt, C.c_int64))
+    sim.ctx.xs_profile_enable(0)
+    recompute_ms = float(ms[_lib.PROF_CLASSES.index("eam_rho")] / repeats)
     g64 = grads[("fp64", "fp64")]
     active = np.flatnonzero(np.abs(g64) > 1e-3 * np.abs(g64).max())
     pick = np.random.default_rng(3).choice(active, size=min(nfd, active.size), replace=False)
@@ -233,7 +277,14 @@ def adjoint_leg(name, cfg, masses, names, tab, nfd, repeats):
         "ms_forward_backward_wall": 1e3 * wall, "gpu_ms_per_forward_backward": gpu_ms,
         "grad_rel_err_vs_fd_max": err, "fd_knots": int(len(pick)), "fd_h": h,
         "device_used_bytes": int(total.value - free.value),
-        "checkpoint_bytes": int(16 * cfg.n),
+        "checkpoint": {
+            "with": {"peak_device_bytes": int(total.value - free.value), "checkpoint_bytes": int(16 * cfg.n),
+                     "note": "rho (8 B/atom) and F'(rho) (8 B/atom, the w of the position record) kept from the forward"},
+            "without": {"peak_device_bytes": int(total.value - free.value - 8 * cfg.n),
+                        "extra_gpu_ms_per_backward": recompute_ms,
+                        "note": "rho recomputed by a density pass before the backward; F' must exist between the "
+                                "forward's two passes either way, so 8 B/atom are saved"},
+        },
     }
 
 
@@ -362,6 +413,11 @@ def main():
                           within.ctypes.data_as(C.POINTER(C.c_int64)))
     pairs_per_atom = float(within[0]) / cfg.n
     log(f"mixed: {mixed['value']:.4g} atom-steps/s, force err {ferr['max_abs_over_rms']:.2e}")
+    gather = gather_ceiling(sim, rc, mixed["kernels"]) if list_info["kind"] == "block" else None
+    if gather:
+        log("gather ceiling: density %.3f of %.3f ms, force %.3f of %.3f ms" % (
+            gather["density"]["ceiling_ms"], gather["density"]["pass_ms"], gather["force"]["ceiling_ms"],
+            gather["force"]["pass_ms"]))
 
     sym = None
     if not args.no_symmetric:
@@ -376,6 +432,16 @@ def main():
         fp64 = run_form(sim, cfg, tab, False, k64, args.warmup, dt, rc, skin)
         log(f"fp64: {fp64['value']:.4g} atom-steps/s")
 
+    drift = None
+    if args.drift_steps > 0:
+        log(f"NVE drift legs: {args.drift_steps} steps per form, thermo every 10 steps")
+        drift = {"mixed": drift_leg(sim, cfg, tab, True, args.drift_steps, dt, rc, skin)}
+        if not args.no_fp64:
+            drift["fp64"] = drift_leg(sim, cfg, tab, False, args.drift_steps, dt, rc, skin)
+        drift["ratio_mixed_over_fp64"] = (drift["mixed"]["ev_per_atom_per_ps"] / drift["fp64"]["ev_per_atom_per_ps"]
+                                          if "fp64" in drift and drift["fp64"]["ev_per_atom_per_ps"] else None)
+        log("drift: " + ", ".join(f"{k} {v['ev_per_atom_per_ps']:.3e}" for k, v in drift.items() if isinstance(v, dict)))
+
     # roofline of the dominant kernel (EAM force pass, mixed)
     kf = mixed["kernels"]["eam_force"]
     kname = force_kernel_name(list_info)
@@ -388,11 +454,17 @@ def main():
     roof = {
         "kernel": kname + " (EAM force pass + force_to_accel + half kick"
                   + (" + next-step drift/kick)" if drift_fused else ")"),
-        "bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
+        "bound": "gather",
+        "bound_note": "the passes are bound by the L1/texture path of their table and neighbour gathers (TD busy "
+                      "~95 %); achieved/peak/frac are the compulsory HBM stream against 8 TB/s, gather.frac the "
+                      "pass's time explained by its data movement alone (gather-ceiling kernel)",
+        "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
         "traffic": measured_traffic(kname),
         "bytes_per_atom": bytes_atom, "avg_launch_ms": kf["ms_avg"],
         "valu": {"achieved_tflops": tflops, "peak_tflops": FP32_PEAK_TFLOPS, "frac": tflops / FP32_PEAK_TFLOPS,
                  "flops_per_pair": FORCE_FLOPS_PER_PAIR, "pairs_per_atom": pairs_per_atom},
+        "gather": gather,
+        "gather_frac": gather["force"]["frac"] if gather else None,
     }
 
     stream = None
@@ -407,6 +479,11 @@ def main():
         log(f"CPU oracle baseline: {args.cpu_steps} steps")
         cpu = cpu_baseline(cfg, tab, args.cpu_steps, args.cpu_threads, dt, rc, skin)
         log(f"cpu: {cpu['value']:.4g} atom-steps/s on {cpu['cores']} threads")
+        if args.cpu_1t_steps > 0:
+            log(f"CPU oracle baseline, one thread: {args.cpu_1t_steps} steps")
+            one = cpu_baseline(cfg, tab, args.cpu_1t_steps, 1, dt, rc, skin)
+            cpu["single_thread"] = {k: one[k] for k in ("value", "cores", "sample", "seconds")}
+            log(f"cpu 1 thread: {one['value']:.4g} atom-steps/s")
 
     lj = None
     if not args.no_lj:
@@ -422,11 +499,11 @@ def main():
         c4 = inp.strained_batch(64, (6, 6, 6), 3.615, [63.546], seed=41)
         a4 = adjoint_leg("C4 batch", c4, [63.546], ["Cu"], potentials.johnson_tables(nr=500, nrho=500), 32, 20)
         log(f"C4: {a4['ms_forward_backward_wall']:.2f} ms/forward+backward, FD err {a4['grad_rel_err_vs_fd_max']}")
-        log("adjoint C5: Cu/Ni 64^3 FCC slab, 1% vacancies, 7 x 2000-knot tables, FD on 4 knots")
+        log("adjoint C5: Cu/Ni 64^3 FCC slab, 1% vacancies, 7 x 2000-knot tables, FD on 16 knots")
         c5 = inp.alloy_slab_config((64, 64, 64), 3.57, [63.546, 58.693], vacancy_frac=0.01, vacuum=10.0,
                                    T=300.0, seed=42)
         t5 = potentials.johnson_tables((potentials.CU, potentials.NI), nr=2000, nrho=2000)
-        a5 = adjoint_leg("C5 Cu/Ni slab", c5, [63.546, 58.693], ["Cu", "Ni"], t5, 4, 5)
+        a5 = adjoint_leg("C5 Cu/Ni slab", c5, [63.546, 58.693], ["Cu", "Ni"], t5, 16, 5)
         log(f"C5: {a5['ms_forward_backward_wall']:.2f} ms/forward+backward, FD err {a5['grad_rel_err_vs_fd_max']}")
         adjoint = {"C4": a4, "C5": a5}
 
@@ -449,12 +526,10 @@ def main():
             "atoms": cfg.n, "dt_ps": dt, "rcut": rc, "skin": skin, "table_knots": 5000,
             "parallelism": "single GPU",
         },
-        "fp64": None if fp64 is None else {k: fp64[k] for k in ("value", "ms_per_step", "rebuilds",
-                                                                  "drift_ev_per_atom_per_ps")},
+        "fp64": None if fp64 is None else {k: fp64[k] for k in ("value", "ms_per_step", "rebuilds")},
         "mixed_over_fp64": None if fp64 is None else mixed["value"] / fp64["value"],
         "force_error": ferr,
-        "nve_drift_ev_per_atom_per_ps": {"mixed": mixed["drift_ev_per_atom_per_ps"],
-                                         "fp64": None if fp64 is None else fp64["drift_ev_per_atom_per_ps"]},
+        "nve_drift_ev_per_atom_per_ps": drift,
         "rebuilds": mixed["rebuilds"],
         "pairs_per_atom_within_rc": pairs_per_atom,
         "list_entries_per_atom": float(entries[0]) / cfg.n,

@@ -752,6 +752,15 @@ xs_status xs_md_run(xs_ctx* h, int potential, int nsteps, double dt, double rcut
   });
 }
 
+xs_status xs_gather_ceiling(xs_ctx* h, double rcut, int force_pass, int reps, double* ms)
+{
+  return guarded(h, [&](Ctx& c) {
+    require(ms != nullptr, XS_E_ARG, "null output");
+    require_list(c);
+    *ms = gather_ceiling(c, rcut, force_pass != 0, reps);
+  });
+}
+
 xs_status xs_stream_copy(xs_ctx* h, size_t bytes, int reps, double* gbs)
 {
   return guarded(h, [&](Ctx& c) {

@@ -963,6 +963,79 @@ __global__ void __launch_bounds__(kPassThreads, kPassWaves<VIRIAL>) k_block_eam_
   }
 }
 
+// Gather ceiling of the mixed single-element EAM passes (a measurement, not an
+// operator): the staging, list walk, LDS neighbour reads, FP64 displacement, cutoff
+// test and table-row index of the density pass (knot rows, FORCE = false) or of the
+// force pass (s-table rows, FORCE = true), with the same table gathers, but none of
+// the spline or pair arithmetic: every gathered row is summed into a per-atom sink,
+// so no load is dead, and the sink is the only output.  Its time against the pass's
+// own is the fraction of the pass that its data movement accounts for.
+template <bool FORCE>
+__global__ void __launch_bounds__(kPassThreads, kPassWaves<false>) k_block_gather_ceiling(BlockArgs a,
+                                                                                            EamTabDev<float> t,
+                                                                                            const float* __restrict__ pair,
+                                                                                            double* __restrict__ sink)
+{
+  const int64_t b = lblock();
+  if (b >= a.g.nblocks) return;
+  LdsLayout l = lds_layout(a.cap, a.ccap, a.g.nh_max, false, FORCE);
+  const BlockShape s = block_shape(a.g, b);
+  const int total = build_halo(a.g, s, a.cell_off, l.hc, l.total);
+  if (over_capacity(a, total, a.cap)) return;
+  if constexpr (!FORCE) central_table(s, l.hc, l.central_local, a.ccap, l.total + 2);
+  stage(s, l.hc, total, a.pos, a.box[s.r].L, nullptr, l.rec, l.ltype, l.slot_cell);
+  int ncentral = 0;
+  if constexpr (!FORCE) {
+    ncentral = l.total[2];
+    if (over_capacity(a, ncentral, a.ccap)) return;
+  }
+  const int half = pass_half();
+  for (int q = pass_slot();; q += kPassAtoms) {
+    Central ca;
+    if constexpr (FORCE) {
+      if (!central_atom(s, l.hc, q, ca)) break;
+    } else {
+      if (!central_at(l.hc, l.slot_cell, l.central_local, ncentral, q, ca)) break;
+    }
+    const Pos4 pi = l.rec.get(ca.local);
+    const int cnt = a.nnb[ca.global];
+    float acc = 0.0f;
+    uint2 wnext = lword(a, ca.global, half);
+    for (int k0 = half; k0 < cnt; k0 += 2 * kBBatch) {
+      const uint2 w = wnext;
+      wnext = lword(a, ca.global, k0 + 8 < cnt ? k0 + 8 : k0);
+      int c[kBBatch];
+      lslots(w, ca.local, k0, cnt, c);
+      LNb nb[kBBatch];
+      lgather<kBBatch>(a, l.rec, c, k0, cnt, pi, nb);
+      bool any = false;
+#pragma unroll
+      for (int u = 0; u < kBBatch; u++) any |= nb[u].in;
+      if (!__any(any)) continue;
+#pragma unroll
+      for (int u = 0; u < kBBatch; u++) {
+        int m;
+        float pp;
+        if constexpr (FORCE) {
+          bool out;
+          pm::s_knot(t, nb[u].d2, m, pp, out);
+          const pm::Q3 gr = pm::load3(pair + t.s_force + (int64_t)m * 8), gz = pm::load3(pair + t.s_force + (int64_t)m * 8 + 4);
+          const float v = ((gr.a + gr.b) + (gr.c + gz.a)) + ((gz.b + gz.c) + pp) + (float)nb[u].w;
+          acc += nb[u].in ? v : 0.0f;
+        } else {
+          float rinv;
+          pm::knot_dist(t, nb[u].in ? nb[u].d2 : 1.0, m, pp, rinv);
+          const pm::Q4<float> r = pm::load4(pm::pair_rho_row(pair, m));
+          const float v = ((r.a + r.b) + (r.c + r.d)) + (pp + rinv);
+          acc += nb[u].in ? v : 0.0f;
+        }
+      }
+    }
+    const double sum = half_sum((double)acc);
+    if (half == 0) sink[ca.global] = sum;
+  }
+}
+
 // LJ forces (lj_compute_force, pair_potential_force_op_singlemat.h:45-218) on the
 // block list: the staging, list walk and epilogues of k_block_eam_force with the
 // pair math of the global-list kernel (pm::lj_pair).
@@ -1124,6 +1197,51 @@ void block_eam_dispatch(Ctx& c, double rcut, bool do_rho, bool do_emb, bool do_f
 }
 
 }  // namespace
+
+// Average time (ms) of `reps` launches of the gather-ceiling kernel of one pass (see
+// k_block_gather_ceiling) on the current block list and FP32 single-element tables.
+double gather_ceiling(Ctx& c, double rcut, bool force, int reps)
+{
+  if (c.list_kind != ListKind::Block) throw Error(XS_E_STATE, "gather ceiling needs the cell-block list");
+  if (!c.has_eam || eam_multi(c)) throw Error(XS_E_STATE, "gather ceiling needs single-element EAM tables");
+  if (reps < 1) throw Error(XS_E_ARG, "reps must be >= 1");
+  const EamTabDev<float> t = make_eam_tab<float>(c);
+  if (force && c.eam.s_pieces == 0) throw Error(XS_E_STATE, "gather ceiling of the force pass needs the s-table");
+  BlockArgs a;
+  a.g = c.bgeom;
+  a.cell_off = c.cell_off.p;
+  a.pos = c.pos.p;
+  a.type = c.type.p;
+  a.box = c.box_d.p;
+  a.nbr16 = c.nbr16.p;
+  a.nnb = c.nnb.p;
+  a.npad = c.npad;
+  a.cap = c.bcap;
+  a.ccap = c.bcentral;
+  a.overflow = c.dscalar.p + 6;
+  a.rc2 = rcut * rcut;
+  DBuf<double> sink;
+  sink.reserve(c.npad);
+  const unsigned g = block_grid(c);
+  const size_t lds = block_lds_bytes(c, false, force);
+  hipEvent_t e0, e1;
+  XS_HIP(hipEventCreate(&e0));
+  XS_HIP(hipEventCreate(&e1));
+  XS_HIP(hipEventRecord(e0, c.stream));
+  for (int r = 0; r < reps; r++) {
+    if (force) k_block_gather_ceiling<true><<<g, kPassThreads, lds, c.stream>>>(a, t, c.eam.pair_f.p, sink.p);
+    else k_block_gather_ceiling<false><<<g, kPassThreads, lds, c.stream>>>(a, t, c.eam.pair_f.p, sink.p);
+  }
+  XS_HIP(hipGetLastError());
+  XS_HIP(hipEventRecord(e1, c.stream));
+  XS_HIP(hipEventSynchronize(e1));
+  float ms = 0.f;
+  XS_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  check_pass_overflow(c);
+  return ms / reps;
+}
 
 // ---------------------------------------------------------------------------
 // host side

@@ -366,6 +366,8 @@ double max_displacement2(Ctx& c);
 // raises XS_E_STATE (and clears the flag) when a block pass found a brick beyond the
 // capacities measured at list build (dscalar[6]); synchronises the stream
 void check_pass_overflow(Ctx& c);
+// average ms of the gather-ceiling kernel of the density (force = false) or force pass
+double gather_ceiling(Ctx& c, double rcut, bool force, int reps);
 void pair_count(Ctx& c, double rcut, int64_t& entries, int64_t& within);
 
 // ---- launches (k_integrate.hip) ----

@@ -271,6 +271,14 @@ class Simulation:
         return {"s_pieces": int(n.value), "r_lo": float(rlo.value),
                 "ulps": {"rho_slope_over_r": float(err[0]), "phi": float(err[1]), "phi_slope_over_r": float(err[2])}}
 
+    def gather_ceiling(self, rcut, force_pass, reps=10) -> float:
+        """Average ms of the gather-ceiling kernel of the mixed single-element density
+        (force_pass False) or force pass (True) on the current block list: the pass's
+        staging, list walk and table-row gathers without its spline arithmetic."""
+        ms = C.c_double(0.0)
+        self.ctx.xs_gather_ceiling(float(rcut), 1 if force_pass else 0, int(reps), C.byref(ms))
+        return float(ms.value)
+
     JOHNSON_PARAMETERS = JOHNSON_PARAMETERS
 
     def johnson_force(self, rcut, parameters, compute_virial=False, **_slots):

@@ -287,6 +287,15 @@ xs_status xs_md_run(xs_ctx* ctx, int potential, int nsteps, double dt, double rc
  * fastest repeat.  The achievable-bandwidth reference for the roofline, and a
  * known-size kernel (k_stream_copy) for calibrating HBM byte counters. */
 xs_status xs_stream_copy(xs_ctx* ctx, size_t bytes, int reps, double* gbs);
+/* Gather ceiling of the mixed single-element EAM passes on the current cell-block
+ * list and tables: average ms over `reps` launches of a kernel that does everything
+ * the density pass (force_pass = 0: knot rows) or the force pass (force_pass = 1:
+ * s-table rows) does to reach its table rows -- staging, list walk, neighbour reads,
+ * FP64 displacement, cutoff test, row index, the row gathers themselves -- and none of
+ * the spline or pair arithmetic.  Its time over the pass's own is the share of the
+ * pass that its data movement accounts for (the roofline of a gather-bound pass).
+ * Writes no simulation state. */
+xs_status xs_gather_ceiling(xs_ctx* ctx, double rcut, int force_pass, int reps, double* ms);
 /* on != 0 enables and clears the counters; on == 0 disables. */
 xs_status xs_profile_enable(xs_ctx* ctx, int on);
 /* ms_total[XS_PROF_NCLASSES] and launches[XS_PROF_NCLASSES] (either may be NULL). */

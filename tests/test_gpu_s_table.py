@@ -142,3 +142,31 @@ def test_close_pairs_read_knot_rows(oracle_mod, monkeypatch):
     assert e_rms <= 1e-5
     assert abs(e - eref) <= 1e-6 * abs(eref)
     sim.close()
+
+
+def test_gather_ceiling(oracle_mod, monkeypatch):
+    """xs_gather_ceiling times both passes' gather kernels, leaves the simulation state
+    alone, and refuses the lists and tables it does not model."""
+    from exastamp_amd._lib import XsError
+    tab = cu_tables()
+    cfg = cu_config((10, 10, 10), T=600.0, jitter=0.08, seed=23)
+    sim = _sim(cfg, tab, monkeypatch)
+    f0, e0 = _mixed(sim, tab)
+    for force in (False, True):
+        ms = sim.gather_ceiling(tab.rc, force, 3)
+        print(f"gather ceiling ({'force' if force else 'density'} pass): {ms:.4f} ms")
+        assert 0.0 < ms < 1e3
+    f1, e1 = _mixed(sim, tab)
+    assert np.array_equal(f0, f1) and e0 == e1
+    with pytest.raises(XsError):
+        sim.gather_ceiling(tab.rc, False, 0)
+    sim.close()
+    sim = _sim(cfg, tab, monkeypatch, env="0")
+    assert sim.gather_ceiling(tab.rc, False, 2) > 0.0
+    with pytest.raises(XsError):
+        sim.gather_ceiling(tab.rc, True, 2)
+    sim.close()
+    sim = _sim(cfg, tab, monkeypatch, lists="global")
+    with pytest.raises(XsError):
+        sim.gather_ceiling(tab.rc, False, 2)
+    sim.close()
