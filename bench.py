@@ -232,17 +232,41 @@ def adjoint_leg(name, cfg, masses, names, tab, nfd, repeats):
 
     grads = {("mixed", "mixed"): step(True, True), ("fp64", "fp64"): step(False, False),
              ("fp64", "mixed"): step(False, True)}
-    sim.ctx.xs_profile_enable(1)
-    t0 = time.perf_counter()
-    for _ in range(repeats):
-        step(True, True)
-    sim.ctx.xs_synchronize()
-    wall = (time.perf_counter() - t0) / repeats
-    ms = np.zeros(len(_lib.PROF_CLASSES))
-    cnt = np.zeros(len(_lib.PROF_CLASSES), dtype=np.int64)
-    sim.ctx.xs_profile_get(_lib.dptr(ms), _lib.ptr(cnt, C.c_int64))
-    sim.ctx.xs_profile_enable(0)
-    gpu_ms = {k: float(ms[i] / repeats) for i, k in enumerate(_lib.PROF_CLASSES) if cnt[i]}
+    # the fitting step as a training loop runs it: theta, targets, forces, loss and
+    # gradient on the device (torch harness, device path), the library on torch's stream
+    import torch
+    from exastamp_amd.fitting import EamForcesEnergies, fit_loss
+    dev = torch.device("cuda", 0)
+    theta_d = torch.tensor(theta0, device=dev, requires_grad=True)
+    Ft_d, Et_d = torch.tensor(Ft, device=dev), torch.tensor(Et, device=dev)
+    nper_d = torch.tensor(nper, dtype=torch.float64, device=dev)
+
+    def dstep(fwd_mixed):
+        theta_d.grad = None
+        F, E = EamForcesEnergies.apply(theta_d, sim, fwd_mixed, True)
+        fit_loss(F, E, Ft_d, Et_d, nper_d).backward()
+
+    wall, gpu_ms = {}, {}
+    for label, fwd_mixed in (("fp64_forward", False), ("mixed_forward", True)):
+        for _ in range(3):
+            dstep(fwd_mixed)
+        torch.cuda.synchronize()
+        sim.ctx.xs_profile_enable(1)
+        t0 = time.perf_counter()
+        for _ in range(repeats):
+            dstep(fwd_mixed)
+        torch.cuda.synchronize()
+        wall[label] = 1e3 * (time.perf_counter() - t0) / repeats
+        ms = np.zeros(len(_lib.PROF_CLASSES))
+        cnt = np.zeros(len(_lib.PROF_CLASSES), dtype=np.int64)
+        sim.ctx.xs_profile_get(_lib.dptr(ms), _lib.ptr(cnt, C.c_int64))
+        sim.ctx.xs_profile_enable(0)
+        gpu_ms[label] = {k: float(ms[i] / repeats) for i, k in enumerate(_lib.PROF_CLASSES) if cnt[i]}
+    dstep(False)
+    g_dev = theta_d.grad.cpu().numpy()
+    g_host = grads[("fp64", "mixed")]
+    dev_vs_host = float(np.abs(g_dev - g_host).max() / np.abs(g_host).max())
+    sim.ctx.xs_ctx_own_stream()
     # without the rho/F' checkpoint the backward would first redo the density pass and
     # the embedding (F' is the force pass's input, so only rho's 8 bytes per atom are
     # the checkpoint's own): time that recomputation
@@ -274,7 +298,10 @@ def adjoint_leg(name, cfg, masses, names, tab, nfd, repeats):
     sim.close()
     return {
         "workload": name, "atoms": int(cfg.n), "replicas": int(cfg.nrep), "knots": int(theta0.size),
-        "ms_forward_backward_wall": 1e3 * wall, "gpu_ms_per_forward_backward": gpu_ms,
+        "ms_forward_backward_wall": wall, "gpu_ms_per_forward_backward": gpu_ms,
+        "harness": "torch autograd, theta/targets/forces/loss/gradient on the device (xs_eam_theta_dev, "
+                   "xs_forces_energies_dev, xs_eam_adjoint_dev on torch's stream); FP32-table adjoint",
+        "device_vs_host_grad_max_rel": dev_vs_host,
         "grad_rel_err_vs_fd_max": err, "fd_knots": int(len(pick)), "fd_h": h,
         "device_used_bytes": int(total.value - free.value),
         "checkpoint": {
@@ -498,13 +525,13 @@ def main():
         log("adjoint C4: 64 strained Cu boxes x 864 atoms, 3 x 500-knot tables, FD on 32 knots")
         c4 = inp.strained_batch(64, (6, 6, 6), 3.615, [63.546], seed=41)
         a4 = adjoint_leg("C4 batch", c4, [63.546], ["Cu"], potentials.johnson_tables(nr=500, nrho=500), 32, 20)
-        log(f"C4: {a4['ms_forward_backward_wall']:.2f} ms/forward+backward, FD err {a4['grad_rel_err_vs_fd_max']}")
+        log(f"C4: {a4['ms_forward_backward_wall']} ms/forward+backward, FD err {a4['grad_rel_err_vs_fd_max']}")
         log("adjoint C5: Cu/Ni 64^3 FCC slab, 1% vacancies, 7 x 2000-knot tables, FD on 16 knots")
         c5 = inp.alloy_slab_config((64, 64, 64), 3.57, [63.546, 58.693], vacancy_frac=0.01, vacuum=10.0,
                                    T=300.0, seed=42)
         t5 = potentials.johnson_tables((potentials.CU, potentials.NI), nr=2000, nrho=2000)
-        a5 = adjoint_leg("C5 Cu/Ni slab", c5, [63.546, 58.693], ["Cu", "Ni"], t5, 16, 5)
-        log(f"C5: {a5['ms_forward_backward_wall']:.2f} ms/forward+backward, FD err {a5['grad_rel_err_vs_fd_max']}")
+        a5 = adjoint_leg("C5 Cu/Ni slab", c5, [63.546, 58.693], ["Cu", "Ni"], t5, 16, 10)
+        log(f"C5: {a5['ms_forward_backward_wall']} ms/forward+backward, FD err {a5['grad_rel_err_vs_fd_max']}")
         adjoint = {"C4": a4, "C5": a5}
 
     out = {

@@ -55,6 +55,8 @@ SIGNATURES = {
     "xs_ctx_destroy": (C.c_int, [_CTX]),
     "xs_last_error": (C.c_char_p, [_CTX]),
     "xs_synchronize": (C.c_int, [_CTX]),
+    "xs_ctx_set_stream": (C.c_int, [_CTX, C.c_void_p]),
+    "xs_ctx_own_stream": (C.c_int, [_CTX]),
     "xs_device_memory": (C.c_int, [_CTX, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
     "xs_species_set": (C.c_int, [_CTX, C.c_int, _D]),
     "xs_system_set": (C.c_int, [_CTX, C.c_int64, _D, _D, _D, _D, _D, _D, _U8, _U16, C.c_int, _D, _D,
@@ -87,6 +89,9 @@ SIGNATURES = {
     "xs_thermo": (C.c_int, [_CTX, _D]),
     "xs_thermo_noalloc": (C.c_int, [_CTX, C.c_void_p]),
     "xs_eam_adjoint": (C.c_int, [_CTX, C.c_double, _D, _D, _D, C.c_uint]),
+    "xs_eam_adjoint_dev": (C.c_int, [_CTX, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint]),
+    "xs_eam_theta_dev": (C.c_int, [_CTX, C.c_void_p]),
+    "xs_forces_energies_dev": (C.c_int, [_CTX, C.c_void_p, C.c_void_p]),
     "xs_lj_adjoint": (C.c_int, [_CTX, C.c_double, C.c_double, C.c_double, _D, _D, _D, C.c_uint]),
     "xs_spline_adjoint": (C.c_int, [C.c_int, C.c_double, _D, _D]),
     "xs_stream_copy": (C.c_int, [_CTX, C.c_size_t, C.c_int, _D]),

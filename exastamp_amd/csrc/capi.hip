@@ -14,7 +14,8 @@ namespace xsmd {
 Ctx::~Ctx()
 {
   if (hscalar) (void)hipHostFree(hscalar);
-  if (stream) (void)hipStreamDestroy(stream);
+  if (flag_event) (void)hipEventDestroy(flag_event);
+  if (own_stream) (void)hipStreamDestroy(own_stream);
 }
 
 namespace {
@@ -26,6 +27,7 @@ xs_status guarded(xs_ctx* h, F&& f)
   if (!c) return XS_E_ARG;
   try {
     XS_HIP(hipSetDevice(c->device));
+    poll_pass_overflow(*c, false);
     f(*c);
     c->err.clear();
     return XS_OK;
@@ -168,7 +170,9 @@ xs_status xs_ctx_create(int device, xs_ctx** out)
   try {
     XS_HIP(hipSetDevice(device));
     thermo_scratch_init(device);
-    XS_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    XS_HIP(hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking));
+    c->stream = c->own_stream;
+    XS_HIP(hipEventCreateWithFlags(&c->flag_event, hipEventDisableTiming));
     XS_HIP(hipHostMalloc((void**)&c->hscalar, 8 * sizeof(unsigned long long), hipHostMallocDefault));
     c->dscalar.reserve(8);
     XS_HIP(hipMemsetAsync(c->dscalar.p, 0, 8 * sizeof(unsigned long long), c->stream));  // flags start clear
@@ -209,6 +213,7 @@ xs_status xs_ctx_destroy(xs_ctx* h)
   if (!c) return XS_E_ARG;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->own_stream && c->own_stream != c->stream) (void)hipStreamSynchronize(c->own_stream);
   delete c;
   return XS_OK;
 }
@@ -219,9 +224,33 @@ const char* xs_last_error(const xs_ctx* h)
   return c ? c->err.c_str() : "null context";
 }
 
+namespace {
+void switch_stream(Ctx& c, hipStream_t next)
+{
+  if (next == c.stream) return;
+  // work queued so far stays ahead of what comes next
+  XS_HIP(hipStreamSynchronize(c.stream));
+  c.stream = next;
+  poll_pass_overflow(c, true);
+}
+}  // namespace
+
+xs_status xs_ctx_set_stream(xs_ctx* h, void* stream)
+{
+  return guarded(h, [&](Ctx& c) { switch_stream(c, reinterpret_cast<hipStream_t>(stream)); });
+}
+
+xs_status xs_ctx_own_stream(xs_ctx* h)
+{
+  return guarded(h, [&](Ctx& c) { switch_stream(c, c.own_stream); });
+}
+
 xs_status xs_synchronize(xs_ctx* h)
 {
-  return guarded(h, [&](Ctx& c) { XS_HIP(hipStreamSynchronize(c.stream)); });
+  return guarded(h, [&](Ctx& c) {
+    XS_HIP(hipStreamSynchronize(c.stream));
+    poll_pass_overflow(c, true);
+  });
 }
 
 xs_status xs_device_memory(xs_ctx* h, size_t* free_bytes, size_t* total_bytes)
@@ -479,7 +508,7 @@ xs_status xs_lj_compute_force(xs_ctx* h, double epsilon, double sigma, double rc
     require_list(c);
     require(rcut <= c.list_rcut + c.list_skin, XS_E_ARG, "rcut exceeds the neighbour-list radius");
     launch_lj(c, epsilon, sigma, rcut, flags, ForceEpilogue{});
-    check_pass_overflow(c);
+    check_or_defer_overflow(c);
   });
 }
 
@@ -527,7 +556,7 @@ xs_status xs_lj_multi_force(xs_ctx* h, int ntypes, const double* epsilon, const 
     c.ljm_ntypes = ntypes;
     c.ljm_rcmax = rcmax;
     launch_lj_multi(c, flags);
-    check_pass_overflow(c);
+    check_or_defer_overflow(c);
   });
 }
 
@@ -561,7 +590,7 @@ xs_status xs_eam_alloy_force(xs_ctx* h, double rcut, int eam_rho, int eam_rho2em
       require(rcut <= c.list_rcut + c.list_skin, XS_E_ARG, "rcut exceeds the neighbour-list radius");
     }
     launch_eam(c, rcut, eam_rho != 0, eam_rho2emb != 0, eam_force != 0, flags, ForceEpilogue{});
-    check_pass_overflow(c);
+    check_or_defer_overflow(c);
   });
 }
 
@@ -615,7 +644,38 @@ xs_status xs_eam_adjoint(xs_ctx* h, double rcut, const double* dL_dF, const doub
     const bool mixed = (flags & XS_MIXED) != 0;
     require((c.rho_state == RhoState::FpFloat || c.rho_state == RhoState::FpDouble) && c.rho_ckpt, XS_E_STATE,
             "adjoint needs the rho/F' checkpoint of an eam_alloy_force call at the current positions");
-    eam_adjoint(c, rcut, dL_dF, dL_dE, grad, mixed);
+    eam_adjoint(c, rcut, dL_dF, dL_dE, grad, mixed, false);
+  });
+}
+
+xs_status xs_eam_adjoint_dev(xs_ctx* h, double rcut, const double* dL_dF_dev, const double* dL_dE_dev,
+                             double* grad_dev, unsigned flags)
+{
+  return guarded(h, [&](Ctx& c) {
+    require(dL_dF_dev && dL_dE_dev && grad_dev, XS_E_ARG, "null adjoint argument");
+    require(c.has_eam, XS_E_STATE, "eam tables not set");
+    require_list(c);
+    require((c.rho_state == RhoState::FpFloat || c.rho_state == RhoState::FpDouble) && c.rho_ckpt, XS_E_STATE,
+            "adjoint needs the rho/F' checkpoint of an eam_alloy_force call at the current positions");
+    eam_adjoint(c, rcut, dL_dF_dev, dL_dE_dev, grad_dev, (flags & XS_MIXED) != 0, true);
+  });
+}
+
+xs_status xs_eam_theta_dev(xs_ctx* h, const double* theta_dev)
+{
+  return guarded(h, [&](Ctx& c) {
+    require(theta_dev != nullptr, XS_E_ARG, "null theta");
+    require(c.has_eam, XS_E_STATE, "eam tables not set (the knot grid comes from xs_eam_alloy_tables)");
+    eam_theta_dev(c, theta_dev);
+  });
+}
+
+xs_status xs_forces_energies_dev(xs_ctx* h, double* F_dev, double* E_dev)
+{
+  return guarded(h, [&](Ctx& c) {
+    require(F_dev && E_dev, XS_E_ARG, "null output");
+    require(c.n == 0 || c.list_valid, XS_E_STATE, "no neighbour list (the replica ranges come from its cell order)");
+    forces_energies_dev(c, F_dev, E_dev);
   });
 }
 

@@ -19,7 +19,7 @@
 //
 // Each weight is spread on the four value coefficients c3..c6 of the spline row
 // it falls in (the slope coefficients c0..c2 are 3c3/D, 2c4/D, c5/D), accumulated
-// in FP64; the host then applies the transpose of the knot -> coefficient map.
+// in FP64; a last kernel applies the transpose of the knot -> coefficient map.
 //
 // Accumulation.  All pairs of a shell fall on a few hundred rows, and all atoms on
 // a few F rows, so atomics on the gradient rows in HBM serialise.  Instead a
@@ -27,7 +27,9 @@
 // workgroups walk all atoms (or pairs), add the contributions that fall in a window
 // into their LDS copy (LDS FP64 atomics), and write it out as a partial; a second
 // kernel sums the partials over the workgroups in a fixed order.  Windows cover
-// every row of every table, one launch per group of windows.
+// every row of every table, one launch per group of windows.  When every row of
+// every table fits in one launch (small tables), one kernel does it all per atom:
+// B1 (lambda_i stays in a register) and then B2, a single sweep of the list.
 #include "xsmd_internal.hpp"
 
 #include <algorithm>
@@ -84,14 +86,15 @@ struct AdjArgs {
 // LDS row windows of one launch: rows [lo, lo + rows) of table `table` at LDS
 // rows off .. off + rows - 1
 constexpr int kAdjSlots = 8;
-constexpr int kAdjRows = 2560;  // LDS rows per launch (80 KB)
-constexpr int kAdjThreads = 512;
+constexpr int kAdjRowsMax = 5120;  // LDS rows per launch: 32 bytes each, 160 KB at most
+constexpr int kAdjThreads = 1024;
 struct AdjSlot {
   int table, lo, rows, off;
   int64_t g0;  // row of the gradient array that holds row lo of the table
 };
 struct AdjSlots {
   int n, total;
+  int stride;  // rows of one workgroup's partial
   AdjSlot s[kAdjSlots];
 };
 
@@ -127,11 +130,11 @@ __device__ __forceinline__ void lds_zero(double* acc, int total)
   __syncthreads();
 }
 
-__device__ __forceinline__ void lds_out(const double* acc, int total, double* __restrict__ part)
+__device__ __forceinline__ void lds_out(const double* acc, const AdjSlots& S, double* __restrict__ part)
 {
   __syncthreads();
-  double* dst = part + (int64_t)blockIdx.x * kAdjRows * 4;
-  for (int q = threadIdx.x; q < 4 * total; q += blockDim.x) dst[q] = acc[q];
+  double* dst = part + (int64_t)blockIdx.x * S.stride * 4;
+  for (int q = threadIdx.x; q < 4 * S.total; q += blockDim.x) dst[q] = acc[q];
 }
 
 __device__ __forceinline__ void knot(double x, double rscale, int nmax1, int mmin, int& m, double& p,
@@ -151,27 +154,28 @@ __device__ __forceinline__ double quad4(const Real* row, double p)
   return ((double)row[4] * p + (double)row[5]) * p + (double)row[6];
 }
 
-// B1: per atom, drho_i and lambda_i, plus the F-table gradient.
-template <class Real, bool MULTI>
-__global__ void __launch_bounds__(256) k_adj_b1(AdjArgs a, AdjTab<Real> t)
+// B1 for atom i: drho_i and lambda_i, and the atom's F-table contribution o (four
+// coefficient weights) at row m of F table T.  L lanes share the atom (consecutive
+// lanes, sub = 0 .. L - 1, entries sub, sub + L, ...); every lane gets the results.
+template <class Real, bool MULTI, int L>
+__device__ __forceinline__ void adj_b1_atom(const AdjArgs& a, const AdjTab<Real>& t, int64_t i, int sub,
+                                            double& lam, double (&o)[4], int& m, int& T)
 {
-  const int64_t i = lblock() * blockDim.x + threadIdx.x;
-  if (i >= a.n) return;
   const int ti = MULTI ? a.type[i] : 0;
   const int rep = a.rep ? a.rep[i] : 0;
-  const double* L = a.box[rep].L;
+  const double* Lb = a.box[rep].L;
   const Pos4 pi = load_pos(a.pos, i);
   const Pos4 gi = load_pos(a.gvec, i);
   double drho = 0.0;
   const bool oni = !MULTI || adj_on(t, ti);
   const int cnt = oni ? a.nnb[i] : 0;
-  for (int k = 0; k < cnt; k++) {
+  for (int k = sub; k < cnt; k += L) {
     const uint32_t e = a.nbr[(int64_t)k * a.npad + i];
     const uint32_t j = xsmd_nbr_index(e);
     const Pos4 pj = load_pos(a.pos, j);
-    const double dx = xsmd_image(pj.x, xsmd_nbr_img_x(e), L[0]) - pi.x;
-    const double dy = xsmd_image(pj.y, xsmd_nbr_img_y(e), L[1]) - pi.y;
-    const double dz = xsmd_image(pj.z, xsmd_nbr_img_z(e), L[2]) - pi.z;
+    const double dx = xsmd_image(pj.x, xsmd_nbr_img_x(e), Lb[0]) - pi.x;
+    const double dy = xsmd_image(pj.y, xsmd_nbr_img_y(e), Lb[1]) - pi.y;
+    const double dz = xsmd_image(pj.z, xsmd_nbr_img_z(e), Lb[2]) - pi.z;
     const double d2 = xsmd_d2(dx, dy, dz);
     if (d2 > a.rc2) continue;
     const Pos4 gj = load_pos(a.gvec, j);
@@ -180,32 +184,44 @@ __global__ void __launch_bounds__(256) k_adj_b1(AdjArgs a, AdjTab<Real> t)
     const double s = (dx * (gj.x - gi.x) + dy * (gj.y - gi.y) + dz * (gj.z - gi.z)) / r;
     const int tj = MULTI ? a.type[j] : 0;
     if (MULTI && !adj_on(t, tj)) continue;
-    int m;
+    int mr;
     double p;
     bool cl;
-    knot(r, t.rdr, t.nr - 1, 0, m, p, cl);
-    const Real* row = t.rhor + ((int64_t)t.t2rhor[tj + 1][ti + 1] * (t.nr + 1) + m) * 8;
+    knot(r, t.rdr, t.nr - 1, 0, mr, p, cl);
+    const Real* row = t.rhor + ((int64_t)t.t2rhor[tj + 1][ti + 1] * (t.nr + 1) + mr) * 8;
     drho += quad4(row, p) * s;
   }
+#pragma unroll
+  for (int off = 1; off < L; off <<= 1) drho += __shfl_xor(drho, off);
   const double rho = a.rho[i];
-  int m;
   double p;
   bool cl;
   knot(rho, t.rdrho, t.nrho - 1, 1, m, p, cl);
-  const int ft = t.t2frho[ti + 1];
-  const Real* row = t.frho + ((int64_t)ft * (t.nrho + 1) + m) * 8;
+  T = t.t2frho[ti + 1];
+  const Real* row = t.frho + ((int64_t)T * (t.nrho + 1) + m) * 8;
   // F'' = d/drho of (c0 p + c1) p + c2, zero where p was clamped
   const double fpp = cl ? 0.0 : (2.0 * (double)row[4] * p + (double)row[5]) * t.rdrho * t.conv;
-  a.lam[i] = fpp * drho;
+  lam = fpp * drho;
   const double ge = a.gE[rep];
   const double conv = t.conv;
   const double over = rho > t.rhomax ? rho - t.rhomax : 0.0;
-  double o[4];
   coeff4(ge * conv, conv * (ge * over - drho), p, t.rdrho, o);
   if (!oni) {  // a disabled species has no embedding term (lambda and drho are 0 already)
     o[0] = o[1] = o[2] = o[3] = 0.0;
-    a.lam[i] = 0.0;
+    lam = 0.0;
   }
+}
+
+// B1: per atom, drho_i and lambda_i, plus the F-table gradient.
+template <class Real, bool MULTI>
+__global__ void __launch_bounds__(256) k_adj_b1(AdjArgs a, AdjTab<Real> t)
+{
+  const int64_t i = lblock() * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  double lam, o[4];
+  int m, T;
+  adj_b1_atom<Real, MULTI, 1>(a, t, i, 0, lam, o, m, T);
+  a.lam[i] = lam;
   store_pos(reinterpret_cast<Pos4*>(a.fc), i, Pos4{o[0], o[1], o[2], o[3]});
   a.fm[i] = m;
 }
@@ -223,32 +239,31 @@ __global__ void __launch_bounds__(kAdjThreads) k_adj_fsum(AdjArgs a, AdjTab<Real
     const double o[4] = {c.x, c.y, c.z, c.w};
     lds_add(acc, S, T, a.fm[i], o);
   }
-  lds_out(acc, S.total, part);
+  lds_out(acc, S, part);
 }
 
 // B2: per pair, the rho- and z2r-table gradients.
-template <class Real, bool MULTI>
+template <class Real, bool MULTI, int L>
 __device__ __forceinline__ void adj_b2_atom(const AdjArgs& a, const AdjTab<Real>& t, const AdjSlots& S,
-                                            double* acc, int64_t i)
+                                            double* acc, int64_t i, int sub, double lam)
 {
   const int ti = MULTI ? a.type[i] : 0;
   const int rep = a.rep ? a.rep[i] : 0;
-  const double* L = a.box[rep].L;
+  const double* Lb = a.box[rep].L;
   const Pos4 pi = load_pos(a.pos, i);
   const Pos4 gi = load_pos(a.gvec, i);
   const double fpi = pi.w;
-  const double lam = a.lam[i];
   const double ge = a.gE[rep];
   const double conv = t.conv;
   const double rdr = t.rdr;
   const int cnt = (!MULTI || adj_on(t, ti)) ? a.nnb[i] : 0;
-  for (int k = 0; k < cnt; k++) {
+  for (int k = sub; k < cnt; k += L) {
     const uint32_t e = a.nbr[(int64_t)k * a.npad + i];
     const uint32_t j = xsmd_nbr_index(e);
     const Pos4 pj = load_pos(a.pos, j);
-    const double dx = xsmd_image(pj.x, xsmd_nbr_img_x(e), L[0]) - pi.x;
-    const double dy = xsmd_image(pj.y, xsmd_nbr_img_y(e), L[1]) - pi.y;
-    const double dz = xsmd_image(pj.z, xsmd_nbr_img_z(e), L[2]) - pi.z;
+    const double dx = xsmd_image(pj.x, xsmd_nbr_img_x(e), Lb[0]) - pi.x;
+    const double dy = xsmd_image(pj.y, xsmd_nbr_img_y(e), Lb[1]) - pi.y;
+    const double dz = xsmd_image(pj.z, xsmd_nbr_img_z(e), Lb[2]) - pi.z;
     const double d2 = xsmd_d2(dx, dy, dz);
     if (d2 > a.rc2) continue;
     const Pos4 gj = load_pos(a.gvec, j);
@@ -276,18 +291,54 @@ __global__ void __launch_bounds__(kAdjThreads) k_adj_b2(AdjArgs a, AdjTab<Real> 
   extern __shared__ double acc[];
   lds_zero(acc, S.total);
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < a.n; i += (int64_t)gridDim.x * blockDim.x)
-    adj_b2_atom<Real, MULTI>(a, t, S, acc, i);
-  lds_out(acc, S.total, part);
+    adj_b2_atom<Real, MULTI, 1>(a, t, S, acc, i, 0, a.lam[i]);
+  lds_out(acc, S, part);
 }
 
-// Sum of the partials of `groups` workgroups (fixed order) into the gradient rows
-// of this launch's windows (each row belongs to exactly one window of one launch).
-__global__ void k_adj_reduce(AdjSlots S, int groups, const double* __restrict__ part, double* __restrict__ g)
+// Everything in one launch, when the rows of every table fit in its windows (small
+// tables, small systems): kFusedLanes lanes per atom walk its list twice, B1
+// (lambda_i kept in registers, the F-table contribution into LDS) and then B2.
+constexpr int kFusedLanes = 4;
+template <class Real, bool MULTI>
+__global__ void __launch_bounds__(kAdjThreads) k_adj_fused(AdjArgs a, AdjTab<Real> t, AdjSlots S,
+                                                           double* __restrict__ part)
 {
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= 4 * S.total) return;
+  extern __shared__ double acc[];
+  lds_zero(acc, S.total);
+  const int sub = threadIdx.x % kFusedLanes;
+  const int64_t step = (int64_t)gridDim.x * (blockDim.x / kFusedLanes);
+  // the lanes of an atom share i, so they leave the loop together (their shuffles)
+  for (int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kFusedLanes; i < a.n; i += step) {
+    double lam, o[4];
+    int m, T;
+    adj_b1_atom<Real, MULTI, kFusedLanes>(a, t, i, sub, lam, o, m, T);
+    if (sub == 0) lds_add(acc, S, T, m, o);
+    adj_b2_atom<Real, MULTI, kFusedLanes>(a, t, S, acc, i, sub, lam);
+  }
+  lds_out(acc, S, part);
+}
+
+// Sum of the partials of `groups` workgroups into the gradient rows of this launch's
+// windows (each row belongs to exactly one window of one launch).  A workgroup takes
+// 64 consecutive values; its 16 waves sum the partials w = wave, wave + 16, ...
+// (coalesced), and the 16 sums are added in wave order, so the result does not
+// depend on scheduling.
+constexpr int kRedThreads = 1024;
+__global__ void __launch_bounds__(kRedThreads) k_adj_reduce(AdjSlots S, int groups, const double* __restrict__ part,
+                                                            double* __restrict__ g)
+{
+  __shared__ double red[kRedThreads];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int q = blockIdx.x * 64 + lane;
+  const bool in = q < 4 * S.total;
   double v = 0.0;
-  for (int w = 0; w < groups; w++) v += part[(int64_t)w * kAdjRows * 4 + q];
+  if (in)
+    for (int w = wave; w < groups; w += kRedThreads / 64) v += part[(int64_t)w * S.stride * 4 + q];
+  red[threadIdx.x] = v;
+  __syncthreads();
+  if (wave != 0 || !in) return;
+  v = red[lane];
+  for (int k = 1; k < kRedThreads / 64; k++) v += red[k * 64 + lane];
   const int row = q >> 2;
   int k = 0;
   while (k + 1 < S.n && row >= S.s[k + 1].off) k++;
@@ -454,27 +505,29 @@ AdjTab<float> adj_tab<float>(Ctx& c)
 }
 
 // Row windows of every table (F tables use their rows_f rows, rho and z2r tables
-// rows_r), packed into launches of at most kAdjRows LDS rows and kAdjSlots windows;
-// `pairs` selects the rho/z2r tables (B2) or the F tables (fsum).
-std::vector<AdjSlots> adj_launches(int nel, int nz, int64_t rows_f, int64_t rows_r, bool pairs)
+// rows_r), packed into launches of at most `limit` LDS rows and kAdjSlots windows;
+// `what` selects the F tables (0), the rho and z2r tables (1) or all of them (2).
+std::vector<AdjSlots> adj_launches(int nel, int nz, int64_t rows_f, int64_t rows_r, int what, int limit)
 {
   std::vector<AdjSlots> out;
   AdjSlots cur{};
+  cur.stride = limit;
   auto add = [&](int T, int64_t rows, int64_t g0) {
-    for (int64_t lo = 0; lo < rows; lo += kAdjRows) {
-      const int w = (int)std::min<int64_t>(kAdjRows, rows - lo);
-      if (cur.n == kAdjSlots || cur.total + w > kAdjRows) {
+    for (int64_t lo = 0; lo < rows; lo += limit) {
+      const int w = (int)std::min<int64_t>(limit, rows - lo);
+      if (cur.n == kAdjSlots || cur.total + w > limit) {
         out.push_back(cur);
         cur = AdjSlots{};
+        cur.stride = limit;
       }
       cur.s[cur.n++] = AdjSlot{T, (int)lo, w, cur.total, g0 + lo};
       cur.total += w;
     }
   };
-  if (!pairs) {
+  const int64_t f_end = nel * rows_f;
+  if (what != 1)
     for (int t = 0; t < nel; t++) add(t, rows_f, t * rows_f);
-  } else {
-    const int64_t f_end = nel * rows_f;
+  if (what != 0) {
     for (int t = 0; t < nel; t++) add(nel + t, rows_r, f_end + t * rows_r);
     for (int t = 0; t < nz; t++) add(2 * nel + t, rows_r, f_end + (nel + t) * rows_r);
   }
@@ -489,35 +542,41 @@ void adj_run(Ctx& c, const AdjArgs& a)
   EamHost& e = c.eam;
   const int nel = e.nel, nz = nel * (nel + 1) / 2;
   const int64_t rows_f = e.nrho + 1, rows_r = e.nr + 1;
-  const unsigned g = (grid_for(c.n) + 7u) / 8u * 8u;
   const bool multi = eam_multi(c);
+  // LDS windows as large as one workgroup may hold; persistent workgroups, as many as
+  // fit on the CUs at the launch's LDS size (1024 threads each, 256 for systems too
+  // small to give every CU 1024 atoms), but no more than the atoms fill
+  const int limit = (int)std::min<size_t>(kAdjRowsMax, c.lds_max / (4 * sizeof(double)));
+  const int cus = std::max(c.num_cu, 1);
+  auto run = [&](auto k, AdjSlots S, int lanes) {
+    const int64_t items = c.n * lanes;
+    const int threads = items >= (int64_t)cus * kAdjThreads ? kAdjThreads : 256;
+    const size_t lds = (size_t)S.total * 4 * sizeof(double);
+    const int64_t per_cu = std::max<int64_t>(1, (int64_t)(c.lds_max / lds));
+    const int groups = (int)std::max<int64_t>(1, std::min<int64_t>(cus * per_cu, (items + threads - 1) / threads));
+    S.stride = S.total;
+    c.adj_part.reserve((size_t)groups * S.stride * 4);
+    XS_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k), hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)lds));
+    k<<<groups, threads, lds, c.stream>>>(a, t, S, c.adj_part.p);
+    XS_HIP(hipGetLastError());
+    k_adj_reduce<<<(unsigned)((4 * S.total + 63) / 64), kRedThreads, 0, c.stream>>>(S, groups, c.adj_part.p,
+                                                                                      c.adj_g.p);
+    XS_HIP(hipGetLastError());
+  };
+  const std::vector<AdjSlots> all = adj_launches(nel, nz, rows_f, rows_r, 2, limit);
+  if (all.size() == 1) {  // every row fits: one sweep of the list
+    run(multi ? k_adj_fused<Real, true> : k_adj_fused<Real, false>, all[0], kFusedLanes);
+    return;
+  }
+  const unsigned g = (grid_for(c.n) + 7u) / 8u * 8u;
   if (multi) k_adj_b1<Real, true><<<g, kBlock, 0, c.stream>>>(a, t);
   else k_adj_b1<Real, false><<<g, kBlock, 0, c.stream>>>(a, t);
   XS_HIP(hipGetLastError());
-  // persistent workgroups: two per CU (80 KB of LDS each), fewer for small systems
-  const int groups = (int)std::max<int64_t>(1, std::min<int64_t>(2 * std::max(c.num_cu, 1),
-                                                                   (c.n + kAdjThreads - 1) / kAdjThreads));
-  c.adj_part.reserve((size_t)groups * kAdjRows * 4);
-  for (int pass = 0; pass < 2; pass++) {
-    for (const AdjSlots& S : adj_launches(nel, nz, rows_f, rows_r, pass == 1)) {
-      const size_t lds = (size_t)S.total * 4 * sizeof(double);
-      if (pass == 0) {
-        auto k = multi ? k_adj_fsum<Real, true> : k_adj_fsum<Real, false>;
-        XS_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (int)lds));
-        k<<<groups, kAdjThreads, lds, c.stream>>>(a, t, S, c.adj_part.p);
-      } else {
-        auto k = multi ? k_adj_b2<Real, true> : k_adj_b2<Real, false>;
-        XS_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (int)lds));
-        k<<<groups, kAdjThreads, lds, c.stream>>>(a, t, S, c.adj_part.p);
-      }
-      XS_HIP(hipGetLastError());
-      k_adj_reduce<<<(unsigned)((4 * S.total + kBlock - 1) / kBlock), kBlock, 0, c.stream>>>(S, groups, c.adj_part.p,
-                                                                                          c.adj_g.p);
-      XS_HIP(hipGetLastError());
-    }
-  }
+  for (const AdjSlots& S : adj_launches(nel, nz, rows_f, rows_r, 0, limit))
+    run(multi ? k_adj_fsum<Real, true> : k_adj_fsum<Real, false>, S, 1);
+  for (const AdjSlots& S : adj_launches(nel, nz, rows_f, rows_r, 1, limit))
+    run(multi ? k_adj_b2<Real, true> : k_adj_b2<Real, false>, S, 1);
 }
 
 }  // namespace
@@ -564,20 +623,83 @@ void spline_interpolate_adjoint(int n, double delta, const double* G, double* ou
   for (int m = 1; m <= n; m++) out[m - 1] = gf[m];
 }
 
-void eam_adjoint(Ctx& c, double rcut, const double* dLdF_id, const double* dLdE, double* grad, bool mixed)
+namespace {
+
+// Gather form of spline_interpolate_adjoint for every table at once: one thread per
+// knot k (1-based) of table t sums the rows' value-coefficient gradients G (4 per
+// row, dL/d(c3, c4, c5, c6)) that depend on knot k, directly (c6 = f, and f in
+// c3, c4 of rows k and k - 1) and through the slopes s[m] of rows m = k - 2 .. k + 2.
+__device__ __forceinline__ double adj_gs(const double* __restrict__ G, int n, int m)
+{
+  // dL/ds[m]: c5 of row m, c4 and c3 of row m (s[m]) and of row m - 1 (s[m + 1] there)
+  double v = G[4 * m + 2];
+  if (m <= n - 1) v += -2.0 * G[4 * m + 1] + G[4 * m + 0];
+  if (m >= 2) v += -G[4 * (m - 1) + 1] + G[4 * (m - 1) + 0];
+  return v;
+}
+
+__global__ void k_spline_adjoint(int nel, int nz, int nrho, int nr, const double* __restrict__ G,
+                                 double* __restrict__ grad)
+{
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t kf = (int64_t)nel * nrho, kr = (int64_t)nel * nr, kz = (int64_t)nz * nr;
+  if (q >= kf + kr + kz) return;
+  int n;
+  int64_t t, k0;
+  const double* g;
+  if (q < kf) {
+    n = nrho;
+    t = q / nrho;
+    k0 = q - t * nrho;
+    g = G + t * (nrho + 1) * 4;
+  } else {
+    const int64_t u = q - kf;
+    n = nr;
+    t = u / nr;
+    k0 = u - t * nr;
+    g = G + (kf + (int64_t)nel) * 4 + t * (nr + 1) * 4;  // the F tables' rows, then nel + nz tables of nr + 1
+  }
+  const int k = (int)k0 + 1;
+  // direct: c6 = f[k]; c4, c3 of row k (-3, +2) and of row k - 1 (+3, -2)
+  double v = g[4 * k + 3];
+  if (k <= n - 1) v += -3.0 * g[4 * k + 1] + 2.0 * g[4 * k + 0];
+  if (k >= 2) v += 3.0 * g[4 * (k - 1) + 1] - 2.0 * g[4 * (k - 1) + 0];
+  // through the slopes
+  if (k == 2) v += adj_gs(g, n, 1);
+  if (k == 1) v -= adj_gs(g, n, 1) + 0.5 * adj_gs(g, n, 2);
+  if (k == 3) v += 0.5 * adj_gs(g, n, 2);
+  if (k == n) v += 0.5 * adj_gs(g, n, n - 1) + adj_gs(g, n, n);
+  if (k == n - 2) v -= 0.5 * adj_gs(g, n, n - 1);
+  if (k == n - 1) v -= adj_gs(g, n, n);
+  // interior slopes m in [3, n - 2]: f[m - 2] +w, f[m + 2] -w, f[m + 1] +8w, f[m - 1] -8w
+  const auto interior = [&](int m) { return m >= 3 && m <= n - 2; };
+  if (interior(k + 2)) v += adj_gs(g, n, k + 2) / 12.0;
+  if (interior(k - 2)) v -= adj_gs(g, n, k - 2) / 12.0;
+  if (interior(k - 1)) v += 8.0 * (adj_gs(g, n, k - 1) / 12.0);
+  if (interior(k + 1)) v -= 8.0 * (adj_gs(g, n, k + 1) / 12.0);
+  grad[q] = v;
+}
+
+}  // namespace
+
+void eam_adjoint(Ctx& c, double rcut, const double* dLdF_id, const double* dLdE, double* grad, bool mixed,
+                 bool device)
 {
   ensure_global_list(c);
   EamHost& e = c.eam;
   const int nel = e.nel, nz = nel * (nel + 1) / 2;
   const size_t rows_r = (size_t)e.nr + 1, rows_f = (size_t)e.nrho + 1;
   const size_t nF = (size_t)nel * rows_f * 4, nR = (size_t)nel * rows_r * 4, nZ = (size_t)nz * rows_r * 4;
+  const size_t nknots = (size_t)nel * e.nrho + (size_t)(nel + nz) * e.nr;
   c.adj_g.reserve(nF + nR + nZ);
   c.adj_lam.reserve(c.npad);
   c.adj_gvec.reserve(c.npad);
   c.adj_in.reserve(3 * (size_t)c.n + c.nrep);
+  if (!device) c.adj_grad.reserve(nknots);
+  const hipMemcpyKind in_kind = device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
   XS_HIP(hipMemsetAsync(c.adj_g.p, 0, sizeof(double) * (nF + nR + nZ), c.stream));
-  XS_HIP(hipMemcpyAsync(c.adj_in.p, dLdF_id, sizeof(double) * 3 * c.n, hipMemcpyHostToDevice, c.stream));
-  XS_HIP(hipMemcpyAsync(c.adj_in.p + 3 * c.n, dLdE, sizeof(double) * c.nrep, hipMemcpyHostToDevice, c.stream));
+  XS_HIP(hipMemcpyAsync(c.adj_in.p, dLdF_id, sizeof(double) * 3 * c.n, in_kind, c.stream));
+  XS_HIP(hipMemcpyAsync(c.adj_in.p + 3 * c.n, dLdE, sizeof(double) * c.nrep, in_kind, c.stream));
   if (c.n > 0) {
     k_gather_g<<<grid_for(c.n), kBlock, 0, c.stream>>>(c.n, c.id.p, c.adj_in.p, c.adj_gvec.p);
     XS_HIP(hipGetLastError());
@@ -604,16 +726,14 @@ void eam_adjoint(Ctx& c, double rcut, const double* dLdF_id, const double* dLdE,
     if (mixed) adj_run<float>(c, a);
     else adj_run<double>(c, a);
   }
-  std::vector<double> G(nF + nR + nZ);
-  XS_HIP(hipMemcpyAsync(G.data(), c.adj_g.p, sizeof(double) * G.size(), hipMemcpyDeviceToHost, c.stream));
-  XS_HIP(hipStreamSynchronize(c.stream));
-  double* out = grad;
-  for (int t = 0; t < nel; t++, out += e.nrho)
-    spline_interpolate_adjoint(e.nrho, e.drho, G.data() + (size_t)t * rows_f * 4, out);
-  for (int t = 0; t < nel; t++, out += e.nr)
-    spline_interpolate_adjoint(e.nr, e.dr, G.data() + nF + (size_t)t * rows_r * 4, out);
-  for (int t = 0; t < nz; t++, out += e.nr)
-    spline_interpolate_adjoint(e.nr, e.dr, G.data() + nF + nR + (size_t)t * rows_r * 4, out);
+  // dL/dtheta: the transpose of the knot -> coefficient map, on the device
+  double* out = device ? grad : c.adj_grad.p;
+  k_spline_adjoint<<<grid_for((int64_t)nknots), kBlock, 0, c.stream>>>(nel, nz, e.nrho, e.nr, c.adj_g.p, out);
+  XS_HIP(hipGetLastError());
+  if (!device) {
+    XS_HIP(hipMemcpyAsync(grad, out, sizeof(double) * nknots, hipMemcpyDeviceToHost, c.stream));
+    XS_HIP(hipStreamSynchronize(c.stream));
+  }
 }
 
 void lj_adjoint(Ctx& c, double eps, double sigma, double rcut, const double* dLdF_id, const double* dLdE,

@@ -1530,4 +1530,34 @@ void check_pass_overflow(Ctx& c)
                           "(cells changed without a rebuild); its atoms were not computed");
 }
 
+void check_or_defer_overflow(Ctx& c)
+{
+  if (c.stream == c.own_stream) {
+    check_pass_overflow(c);
+    return;
+  }
+  XS_HIP(hipMemcpyAsync(c.hscalar + 6, c.dscalar.p + 6, sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                        c.stream));
+  XS_HIP(hipEventRecord(c.flag_event, c.stream));
+  c.flag_pending = true;
+}
+
+void poll_pass_overflow(Ctx& c, bool wait)
+{
+  if (!c.flag_pending) return;
+  if (!wait) {
+    const hipError_t q = hipEventQuery(c.flag_event);
+    if (q == hipErrorNotReady) return;
+    XS_HIP(q);
+  } else {
+    XS_HIP(hipEventSynchronize(c.flag_event));
+  }
+  c.flag_pending = false;
+  if (c.hscalar[6] == 0) return;
+  c.hscalar[6] = 0;
+  XS_HIP(hipMemsetAsync(c.dscalar.p + 6, 0, sizeof(unsigned long long), c.stream));
+  throw Error(XS_E_STATE, "block pass (an earlier call): a brick holds more halo or central atoms than measured at "
+                          "the list build (cells changed without a rebuild); its atoms were not computed");
+}
+
 }  // namespace xsmd

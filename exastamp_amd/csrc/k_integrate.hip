@@ -299,6 +299,77 @@ void launch_thermo_noalloc(Ctx& c, double* out27_dev)
 }  // namespace xsmd
 
 namespace xsmd {
+namespace {
+
+// F (id order) = m_type a_i, the forces behind the accelerations force_to_accel left
+__global__ void k_forces_id(int64_t n, const int64_t* __restrict__ id, const uint8_t* __restrict__ type,
+                            const double* __restrict__ mass, const double* __restrict__ ax,
+                            const double* __restrict__ ay, const double* __restrict__ az, double* __restrict__ F)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double m = mass[type[i]];
+  double* f = F + 3 * id[i];
+  f[0] = ax[i] * m;
+  f[1] = ay[i] * m;
+  f[2] = az[i] * m;
+}
+
+// Per-replica energy: the atoms of replica r are the cell-sorted range
+// [cell_off[r ncell_rep], cell_off[(r + 1) ncell_rep]); kRepChunks workgroups per
+// replica each sum a fixed slice (thread-strided, then a fixed tree), and one thread
+// per replica adds the slices in order, so the sums do not depend on scheduling.
+constexpr int kRepChunks = 32;
+constexpr int kRepThreads = 256;
+
+__global__ void __launch_bounds__(kRepThreads) k_replica_partial(const int* __restrict__ cell_off,
+                                                                 int64_t ncell_rep, const double* __restrict__ ep,
+                                                                 double* __restrict__ part)
+{
+  __shared__ double red[kRepThreads / 64];
+  const int r = blockIdx.x / kRepChunks, b = blockIdx.x % kRepChunks;
+  const int64_t lo = cell_off[(int64_t)r * ncell_rep], hi = cell_off[(int64_t)(r + 1) * ncell_rep];
+  const int64_t len = hi - lo;
+  const int64_t s0 = lo + len * b / kRepChunks, s1 = lo + len * (b + 1) / kRepChunks;
+  double v = 0.0;
+  for (int64_t i = s0 + threadIdx.x; i < s1; i += kRepThreads) v += ep[i];
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < kRepThreads / 64; w++) t += red[w];
+    part[blockIdx.x] = t;
+  }
+}
+
+__global__ void k_replica_final(int nrep, const double* __restrict__ part, double* __restrict__ E)
+{
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrep) return;
+  double t = 0.0;
+  for (int b = 0; b < kRepChunks; b++) t += part[r * kRepChunks + b];
+  E[r] = t;
+}
+
+}  // namespace
+
+void forces_energies_dev(Ctx& c, double* F, double* E)
+{
+  if (c.n == 0) {
+    XS_HIP(hipMemsetAsync(E, 0, sizeof(double) * c.nrep, c.stream));
+    return;
+  }
+  k_forces_id<<<grid_for(c.n), kBlock, 0, c.stream>>>(c.n, c.id.p, c.type.p, c.mass_d.p, c.ax.p, c.ay.p, c.az.p, F);
+  XS_HIP(hipGetLastError());
+  c.rep_part.reserve((size_t)c.nrep * kRepChunks);
+  k_replica_partial<<<(unsigned)(c.nrep * kRepChunks), kRepThreads, 0, c.stream>>>(c.cell_off.p, c.ncell_rep,
+                                                                                   c.ep.p, c.rep_part.p);
+  XS_HIP(hipGetLastError());
+  k_replica_final<<<grid_for(c.nrep), kBlock, 0, c.stream>>>(c.nrep, c.rep_part.p, E);
+  XS_HIP(hipGetLastError());
+}
 
 double stream_copy(Ctx& c, size_t bytes, int reps)
 {

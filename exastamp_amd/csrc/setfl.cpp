@@ -284,4 +284,99 @@ void eam_upload(Ctx& c)
   up(e.pair_f, pf);
 }
 
+namespace {
+
+// Slope s[m] (1-based knot m) of spline_interpolate from the knots f[0..n-1].
+__device__ __forceinline__ double spline_slope_dev(const double* __restrict__ f, int n, int m)
+{
+  if (m == 1) return f[1] - f[0];
+  if (m == 2) return 0.5 * (f[2] - f[0]);
+  if (m == n - 1) return 0.5 * (f[n - 1] - f[n - 3]);
+  if (m == n) return f[n - 1] - f[n - 2];
+  return ((f[m - 3] - f[m + 1]) + 8.0 * (f[m] - f[m - 2])) / 12.0;
+}
+
+// One coefficient row per thread (row m of table t, rows 0..n): spline_interpolate's
+// arithmetic in its order, with contraction off in this file, so every coefficient
+// has the host build's bits; written in device order [c3 c4 c5 c6 c0 c1 c2 0] in
+// FP64 and FP32.
+__global__ void k_spline_rows(int ntab, int n, double delta, const double* __restrict__ knots,
+                              double* __restrict__ rows_d, float* __restrict__ rows_f)
+{
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t per = (int64_t)n + 1;
+  if (q >= (int64_t)ntab * per) return;
+  const int t = (int)(q / per), m = (int)(q - t * per);
+  const double* f = knots + (int64_t)t * n;
+  double c[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  if (m >= 1) {
+    const double sm = spline_slope_dev(f, n, m);
+    double a = 0.0, b = 0.0;
+    if (m <= n - 1) {
+      const double sm1 = spline_slope_dev(f, n, m + 1);
+      const double dv = f[m] - f[m - 1];
+      b = 3.0 * dv - 2.0 * sm - sm1;
+      a = sm + sm1 - 2.0 * dv;
+    }
+    c[0] = a;
+    c[1] = b;
+    c[2] = sm;
+    c[3] = f[m - 1];
+    c[4] = 3.0 * a / delta;
+    c[5] = 2.0 * b / delta;
+    c[6] = sm / delta;
+  }
+  double* d = rows_d + 8 * q;
+  float* g = rows_f + 8 * q;
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    d[k] = c[k];
+    g[k] = (float)c[k];
+  }
+}
+
+// The knot planes of the single-element pair table (rho and z2r value cubics of
+// table 0, eam_upload's layout) from the device rows, both precisions.
+__global__ void k_pair_planes(int64_t rows, const double* __restrict__ rhor, const double* __restrict__ z2r,
+                              double* __restrict__ pair_d, float* __restrict__ pair_f)
+{
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= 8 * rows) return;
+  const int64_t plane = q / (4 * rows), m = (q >> 2) - plane * rows, k = q & 3;
+  const double v = (plane == 0 ? rhor : z2r)[8 * m + k];
+  pair_d[q] = v;
+  pair_f[q] = (float)v;
+}
+
+void spline_rows_dev(Ctx& c, int ntab, int n, double delta, const double* knots, DBuf<double>& rows_d,
+                     DBuf<float>& rows_f)
+{
+  const int64_t total = (int64_t)ntab * (n + 1);
+  k_spline_rows<<<grid_for(total), kBlock, 0, c.stream>>>(ntab, n, delta, knots, rows_d.p, rows_f.p);
+  XS_HIP(hipGetLastError());
+}
+
+}  // namespace
+
+void eam_theta_dev(Ctx& c, const double* theta)
+{
+  EamHost& e = c.eam;
+  const int nel = e.nel, nz = nel * (nel + 1) / 2;
+  const double* f = theta;
+  const double* r = f + (size_t)nel * e.nrho;
+  const double* z = r + (size_t)nel * e.nr;
+  spline_rows_dev(c, nel, e.nrho, e.drho, f, e.frho_d, e.frho_f);
+  spline_rows_dev(c, nel, e.nr, e.dr, r, e.rhor_d, e.rhor_f);
+  spline_rows_dev(c, nz, e.nr, e.dr, z, e.z2r_d, e.z2r_f);
+  const int64_t rows = (int64_t)e.nr + 1;
+  k_pair_planes<<<grid_for(8 * rows), kBlock, 0, c.stream>>>(rows, e.rhor_d.p, e.z2r_d.p, e.pair_d.p, e.pair_f.p);
+  XS_HIP(hipGetLastError());
+  // the s = r^2 tables were built (and checked) from the previous knots on the host
+  e.s_pieces = 0;
+  e.s_lo = e.s_hi = 0.0;
+  e.s_err[0] = e.s_err[1] = e.s_err[2] = 0.0;
+  c.rho_state = RhoState::None;
+  c.rho_ckpt = false;
+}
+
 }  // namespace xsmd

@@ -244,7 +244,8 @@ struct ProfScope {
 
 struct Ctx {
   int device = 0;
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr;      // the stream every call is ordered on
+  hipStream_t own_stream = nullptr;  // the context's own (stream unless xs_ctx_set_stream chose another)
   std::string err;
 
   int nspecies = 0;
@@ -314,6 +315,8 @@ struct Ctx {
                                      // [4] atom outside its cell, [5] fused-drift rebuild flag,
                                      // [6] block-pass capacity exceeded
   unsigned long long* hscalar = nullptr;
+  hipEvent_t flag_event = nullptr;  // the deferred copy of dscalar[6] to hscalar[6] (check_or_defer_overflow)
+  bool flag_pending = false;
   DBuf<double> red_part;  // thermo partial sums
   DBuf<double> red_out;
 
@@ -335,6 +338,8 @@ struct Ctx {
   // EAM adjoint scratch: coefficient gradients, lambda_i, dL/dF records, inputs
   DBuf<double> adj_g, adj_lam, adj_in;
   DBuf<double> adj_fc, adj_part;  // per-atom F-table contributions; per-workgroup partial rows
+  DBuf<double> adj_grad;          // dL/dtheta on the device (host-output adjoint)
+  DBuf<double> rep_part;          // per-replica partial energy sums (forces_energies_dev)
   DBuf<int> adj_fm;               // per-atom F-table row
   DBuf<Pos4> adj_gvec;
 
@@ -366,6 +371,13 @@ double max_displacement2(Ctx& c);
 // raises XS_E_STATE (and clears the flag) when a block pass found a brick beyond the
 // capacities measured at list build (dscalar[6]); synchronises the stream
 void check_pass_overflow(Ctx& c);
+// The operators' form: on the context's own stream, check_pass_overflow; on a stream
+// the caller set (xs_ctx_set_stream), the flag is copied to the host asynchronously
+// and checked by poll_pass_overflow at the next call (no synchronisation mid-step).
+void check_or_defer_overflow(Ctx& c);
+// raises XS_E_STATE when a deferred flag copy has arrived (wait: wait for it) and
+// found the flag set
+void poll_pass_overflow(Ctx& c, bool wait);
 // average ms of the gather-ceiling kernel of the density (force = false) or force pass
 double gather_ceiling(Ctx& c, double rcut, bool force, int reps);
 void pair_count(Ctx& c, double rcut, int64_t& entries, int64_t& within);
@@ -430,7 +442,11 @@ template <class Real>
 const Real* eam_pair_table(Ctx& c);
 
 // ---- EAM adjoint (k_adjoint.hip) ----
-void eam_adjoint(Ctx& c, double rcut, const double* dLdF_id, const double* dLdE, double* grad, bool mixed);
+// dL/dtheta of eam_alloy_force.  dLdF_id (3n), dLdE (nrep) and grad are device
+// pointers when `device` is set (nothing is copied to or from the host and the call
+// does not synchronise), host pointers otherwise.
+void eam_adjoint(Ctx& c, double rcut, const double* dLdF_id, const double* dLdE, double* grad, bool mixed,
+                 bool device);
 void spline_interpolate_adjoint(int n, double delta, const double* G, double* out);
 // dL/d(epsilon, sigma) of lj_compute_force's forces and shifted energies (k_adjoint.hip)
 void lj_adjoint(Ctx& c, double eps, double sigma, double rcut, const double* dLdF_id, const double* dLdE,
@@ -442,6 +458,13 @@ void read_setfl(const char* path, int& nel, int& nrho, double& drho, int& nr, do
                 double& rcut, std::vector<double>& frho, std::vector<double>& rhor,
                 std::vector<double>& z2r, std::vector<double>& masses);
 void eam_upload(Ctx& c);
+// New knots theta (setfl order, device memory) for the loaded tables' grid: the spline
+// rows of both precisions built on the device, bit for bit the host build; the FP32
+// s = r^2 tables are dropped (the mixed force pass reads the knot rows).
+void eam_theta_dev(Ctx& c, const double* theta);
+// forces m a_i in id order (3n, x y z per atom) and per-replica energies (nrep) into
+// device memory, on the context's stream
+void forces_energies_dev(Ctx& c, double* F, double* E);
 
 // ---- helpers ----
 constexpr int kBlock = 256;

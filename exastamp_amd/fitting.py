@@ -15,6 +15,13 @@ two calls.
     loss = fit_loss(F, E, F_target, E_target, atoms_per_replica)
     loss.backward()
 
+With theta on the GPU the step stays on the device: the context runs on torch's
+current stream (xs_ctx_set_stream), the spline coefficients are built from theta by a
+kernel (xs_eam_theta_dev), forces and replica energies come back as device tensors
+(xs_forces_energies_dev) and the adjoint writes dL/dtheta into a device tensor
+(xs_eam_adjoint_dev); nothing is copied through the host.  On that path the mixed
+forward reads the FP32 knot rows (the s = r^2 tables need the host's accuracy check).
+
 forward_mixed selects the forward precision (FP32 pair/spline math), adjoint_mixed
 the FP32 table copies in the adjoint sweeps.  A force-matching gradient is only as
 accurate as the residual F - F_target it is built from: with an FP32 forward the
@@ -26,15 +33,38 @@ of a fitting step (the adjoint sweeps dominate).
 """
 from __future__ import annotations
 
+import ctypes as C
+
 import numpy as np
 import torch
 
+from ._lib import XS_MIXED
 from .units import EV
+
+
+def _on_torch_stream(sim, device):
+    sim.ctx.xs_ctx_set_stream(C.c_void_p(torch.cuda.current_stream(device).cuda_stream))
 
 
 class EamForcesEnergies(torch.autograd.Function):
     @staticmethod
     def forward(ctx, theta, sim, forward_mixed=False, adjoint_mixed=True):
+        ctx.sim = sim
+        ctx.adjoint_mixed = adjoint_mixed
+        ctx.device = theta.is_cuda
+        if theta.is_cuda:
+            _on_torch_stream(sim, theta.device)
+            th = theta.detach().to(torch.float64).contiguous()
+            if th.numel() != sim.eam.theta().size:
+                raise ValueError("theta does not match the loaded tables' knot count")
+            sim.ctx.xs_eam_theta_dev(C.c_void_p(th.data_ptr()))
+            sim.zero_force_energy()
+            sim.eam_alloy_force(rcut=sim.eam.rc, trigger_thermo_state=True, mixed=forward_mixed)
+            sim.force_to_accel()
+            F = torch.empty((sim.n, 3), dtype=torch.float64, device=theta.device)
+            E = torch.empty(sim.cfg.nrep, dtype=torch.float64, device=theta.device)
+            sim.ctx.xs_forces_energies_dev(C.c_void_p(F.data_ptr()), C.c_void_p(E.data_ptr()))
+            return F / EV, E / EV
         t = sim.eam.with_theta(theta.detach().to("cpu", torch.float64).numpy())
         sim.eam_alloy_init(t)
         sim.zero_force_energy()
@@ -42,13 +72,20 @@ class EamForcesEnergies(torch.autograd.Function):
         sim.force_to_accel()
         F = torch.from_numpy(sim.forces() / EV).to(theta.device)
         E = torch.from_numpy(sim.replica_energies() / EV).to(theta.device)
-        ctx.sim = sim
-        ctx.adjoint_mixed = adjoint_mixed
         return F, E
 
     @staticmethod
     def backward(ctx, gF, gE):
         # outputs are in eV units: dL/dF_internal = dL/dF_eV / EV, likewise for E
+        sim = ctx.sim
+        if ctx.device:
+            _on_torch_stream(sim, gF.device)
+            gf = (gF.detach().to(torch.float64) / EV).contiguous()
+            ge = (gE.detach().to(torch.float64) / EV).contiguous()
+            grad = torch.empty(sim.eam.theta().size, dtype=torch.float64, device=gF.device)
+            sim.ctx.xs_eam_adjoint_dev(float(sim.eam.rc), C.c_void_p(gf.data_ptr()), C.c_void_p(ge.data_ptr()),
+                                       C.c_void_p(grad.data_ptr()), XS_MIXED if ctx.adjoint_mixed else 0)
+            return grad, None, None, None
         gf = gF.detach().to("cpu", torch.float64).numpy() / EV
         ge = gE.detach().to("cpu", torch.float64).numpy() / EV
         g = ctx.sim.eam_adjoint(gf, ge, mixed=ctx.adjoint_mixed)

@@ -75,6 +75,19 @@ xs_status xs_ctx_destroy(xs_ctx* ctx);
 const char* xs_last_error(const xs_ctx* ctx);
 /* Wait for all work queued on the context's stream. */
 xs_status xs_synchronize(xs_ctx* ctx);
+/* No counterpart in exaStamp.  Orders every later call of the context on `stream` (a
+ * hipStream_t of the context's device, e.g. the caller's framework stream; NULL is the
+ * device's null stream), so that device buffers the caller's own kernels write or read
+ * (the *_dev arguments) need no further synchronisation.  Waits for the work already
+ * queued on the current stream first.  The caller keeps `stream` alive while the
+ * context uses it.  On such a stream the force operators do not wait for their block
+ * passes' capacity flag (a brick beyond the capacity measured at the list build,
+ * XS_E_STATE): the flag travels to the host with the work, and the first later call
+ * that finds it arrived (xs_synchronize waits for it) returns XS_E_STATE. */
+xs_status xs_ctx_set_stream(xs_ctx* ctx, void* stream);
+/* Back to the context's own stream (the one xs_ctx_create made), after waiting for
+ * the work queued on the current one. */
+xs_status xs_ctx_own_stream(xs_ctx* ctx);
 /* Free and total device memory of the context's GPU (hipMemGetInfo). */
 xs_status xs_device_memory(xs_ctx* ctx, size_t* free_bytes, size_t* total_bytes);
 
@@ -233,6 +246,26 @@ xs_status xs_thermo_noalloc(xs_ctx* ctx, double* out27_dev);
  * precision of the forward. */
 xs_status xs_eam_adjoint(xs_ctx* ctx, double rcut, const double* dL_dF, const double* dL_dE,
                          double* grad, unsigned flags);
+/* xs_eam_adjoint with its three arrays in device memory: nothing is copied to or from
+ * the host and the call does not synchronise (grad_dev is complete when the context's
+ * stream reaches it).  No counterpart in exaStamp; for fitting loops that keep the
+ * knots, the forces and the loss on the device. */
+xs_status xs_eam_adjoint_dev(xs_ctx* ctx, double rcut, const double* dL_dF_dev, const double* dL_dE_dev,
+                             double* grad_dev, unsigned flags);
+/* No counterpart in exaStamp (the fitting counterpart of eam_alloy_init for new knot
+ * values): theta_dev holds knots in xs_eam_adjoint's setfl order, on the grid (nel,
+ * nrho, drho, nr, dr, rcut) of the tables loaded last, in device memory.  The spline
+ * coefficients of both precisions are built on the device (the same construction and
+ * bits as xs_eam_alloy_tables); the FP32 s = r^2 tables are dropped, so the mixed
+ * force pass reads the FP32 knot rows until the next xs_eam_alloy_tables.  No copy to
+ * or from the host, no synchronisation. */
+xs_status xs_eam_theta_dev(xs_ctx* ctx, const double* theta_dev);
+/* No counterpart in exaStamp (the device form of reading forces and energies back for
+ * a fitting loss): F_dev (3n doubles, id order, x y z per atom) = m a_i of the current
+ * accelerations (valid after force_to_accel), E_dev (nrep doubles) = each replica's
+ * total potential energy (sum of ep, needs a force call with XS_ENERGY), in internal
+ * units.  No copy to the host, no synchronisation. */
+xs_status xs_forces_energies_dev(xs_ctx* ctx, double* F_dev, double* E_dev);
 /* No counterpart in exaStamp: reverse mode of lj_compute_force (pair_potential_impl.hxx:
  * 39-513 with lennard_jones.h:40-50) with respect to its parameters, for fitting
  * epsilon and sigma to reference forces and energies.  Differentiates the forces
