@@ -140,7 +140,11 @@ def gather_ceiling(sim, rc, kernels):
     for name, force in (("density", False), ("force", True)):
         ms = sim.gather_ceiling(rc, force, 10)
         k = kernels["eam_force" if force else "eam_rho"]["ms_avg"]
-        out[name] = {"ceiling_ms": ms, "pass_ms": k, "frac": ms / k if k else None}
+        out[name] = {"ceiling_ms": ms, "pass_ms": k, "frac": ms / k if k else None,
+                     # the same kernel with parts taken away: what each part costs
+                     "without_table_rows_ms": sim.gather_ceiling(rc, force, 10, level=1),
+                     "list_words_only_ms": sim.gather_ceiling(rc, force, 10, level=2),
+                     "staging_only_ms": sim.gather_ceiling(rc, force, 10, level=3)}
     return out
 
 

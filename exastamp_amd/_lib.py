@@ -95,7 +95,7 @@ SIGNATURES = {
     "xs_lj_adjoint": (C.c_int, [_CTX, C.c_double, C.c_double, C.c_double, _D, _D, _D, C.c_uint]),
     "xs_spline_adjoint": (C.c_int, [C.c_int, C.c_double, _D, _D]),
     "xs_stream_copy": (C.c_int, [_CTX, C.c_size_t, C.c_int, _D]),
-    "xs_gather_ceiling": (C.c_int, [_CTX, C.c_double, C.c_int, C.c_int, _D]),
+    "xs_gather_ceiling": (C.c_int, [_CTX, C.c_double, C.c_int, C.c_int, C.c_int, _D]),
     "xs_profile_enable": (C.c_int, [_CTX, C.c_int]),
     "xs_profile_get": (C.c_int, [_CTX, _D, _I64]),
     "xs_md_run": (C.c_int, [_CTX, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_uint, C.c_int,

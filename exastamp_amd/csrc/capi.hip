@@ -812,12 +812,12 @@ xs_status xs_md_run(xs_ctx* h, int potential, int nsteps, double dt, double rcut
   });
 }
 
-xs_status xs_gather_ceiling(xs_ctx* h, double rcut, int force_pass, int reps, double* ms)
+xs_status xs_gather_ceiling(xs_ctx* h, double rcut, int force_pass, int level, int reps, double* ms)
 {
   return guarded(h, [&](Ctx& c) {
     require(ms != nullptr, XS_E_ARG, "null output");
     require_list(c);
-    *ms = gather_ceiling(c, rcut, force_pass != 0, reps);
+    *ms = gather_ceiling(c, rcut, force_pass != 0, level, reps);
   });
 }
 

@@ -969,8 +969,11 @@ __global__ void __launch_bounds__(kPassThreads, kPassWaves<VIRIAL>) k_block_eam_
 // force pass (s-table rows, FORCE = true), with the same table gathers, but none of
 // the spline or pair arithmetic: every gathered row is summed into a per-atom sink,
 // so no load is dead, and the sink is the only output.  Its time against the pass's
-// own is the fraction of the pass that its data movement accounts for.
-template <bool FORCE>
+// own is the fraction of the pass that its data movement accounts for.  LEVEL takes
+// parts away to split that time: 0 everything above; 1 without the table-row
+// gathers (the row index still computed and summed); 2 the list words only (no
+// neighbour records, no displacement); 3 the staging only.
+template <bool FORCE, int LEVEL>
 __global__ void __launch_bounds__(kPassThreads, kPassWaves<false>) k_block_gather_ceiling(BlockArgs a,
                                                                                             EamTabDev<float> t,
                                                                                             const float* __restrict__ pair,
@@ -1000,12 +1003,20 @@ __global__ void __launch_bounds__(kPassThreads, kPassWaves<false>) k_block_gathe
     const Pos4 pi = l.rec.get(ca.local);
     const int cnt = a.nnb[ca.global];
     float acc = 0.0f;
+    if constexpr (LEVEL == 3) {
+      if (half == 0) sink[ca.global] = pi.x + cnt;
+      continue;
+    }
     uint2 wnext = lword(a, ca.global, half);
     for (int k0 = half; k0 < cnt; k0 += 2 * kBBatch) {
       const uint2 w = wnext;
       wnext = lword(a, ca.global, k0 + 8 < cnt ? k0 + 8 : k0);
       int c[kBBatch];
       lslots(w, ca.local, k0, cnt, c);
+      if constexpr (LEVEL == 2) {
+        acc += (float)((c[0] + c[1]) + (c[2] + c[3]));
+        continue;
+      }
       LNb nb[kBBatch];
       lgather<kBBatch>(a, l.rec, c, k0, cnt, pi, nb);
       bool any = false;
@@ -1019,14 +1030,25 @@ __global__ void __launch_bounds__(kPassThreads, kPassWaves<false>) k_block_gathe
         if constexpr (FORCE) {
           bool out;
           pm::s_knot(t, nb[u].d2, m, pp, out);
-          const pm::Q3 gr = pm::load3(pair + t.s_force + (int64_t)m * 8), gz = pm::load3(pair + t.s_force + (int64_t)m * 8 + 4);
-          const float v = ((gr.a + gr.b) + (gr.c + gz.a)) + ((gz.b + gz.c) + pp) + (float)nb[u].w;
+          float v;
+          if constexpr (LEVEL == 0) {
+            const pm::Q3 gr = pm::load3(pair + t.s_force + (int64_t)m * 8);
+            const pm::Q3 gz = pm::load3(pair + t.s_force + (int64_t)m * 8 + 4);
+            v = ((gr.a + gr.b) + (gr.c + gz.a)) + ((gz.b + gz.c) + pp) + (float)nb[u].w;
+          } else {
+            v = (pp + (float)m) + (float)nb[u].w;
+          }
           acc += nb[u].in ? v : 0.0f;
         } else {
           float rinv;
           pm::knot_dist(t, nb[u].in ? nb[u].d2 : 1.0, m, pp, rinv);
-          const pm::Q4<float> r = pm::load4(pm::pair_rho_row(pair, m));
-          const float v = ((r.a + r.b) + (r.c + r.d)) + (pp + rinv);
+          float v;
+          if constexpr (LEVEL == 0) {
+            const pm::Q4<float> r = pm::load4(pm::pair_rho_row(pair, m));
+            v = ((r.a + r.b) + (r.c + r.d)) + (pp + rinv);
+          } else {
+            v = (pp + rinv) + (float)m;
+          }
           acc += nb[u].in ? v : 0.0f;
         }
       }
@@ -1198,10 +1220,12 @@ void block_eam_dispatch(Ctx& c, double rcut, bool do_rho, bool do_emb, bool do_f
 
 }  // namespace
 
-// Average time (ms) of `reps` launches of the gather-ceiling kernel of one pass (see
-// k_block_gather_ceiling) on the current block list and FP32 single-element tables.
-double gather_ceiling(Ctx& c, double rcut, bool force, int reps)
+// Average time (ms) of `reps` launches of the gather-ceiling kernel of one pass at
+// one level (see k_block_gather_ceiling) on the current block list and FP32
+// single-element tables.
+double gather_ceiling(Ctx& c, double rcut, bool force, int level, int reps)
 {
+  if (level < 0 || level > 3) throw Error(XS_E_ARG, "gather ceiling level must be 0..3");
   if (c.list_kind != ListKind::Block) throw Error(XS_E_STATE, "gather ceiling needs the cell-block list");
   if (!c.has_eam || eam_multi(c)) throw Error(XS_E_STATE, "gather ceiling needs single-element EAM tables");
   if (reps < 1) throw Error(XS_E_ARG, "reps must be >= 1");
@@ -1228,10 +1252,13 @@ double gather_ceiling(Ctx& c, double rcut, bool force, int reps)
   XS_HIP(hipEventCreate(&e0));
   XS_HIP(hipEventCreate(&e1));
   XS_HIP(hipEventRecord(e0, c.stream));
-  for (int r = 0; r < reps; r++) {
-    if (force) k_block_gather_ceiling<true><<<g, kPassThreads, lds, c.stream>>>(a, t, c.eam.pair_f.p, sink.p);
-    else k_block_gather_ceiling<false><<<g, kPassThreads, lds, c.stream>>>(a, t, c.eam.pair_f.p, sink.p);
-  }
+  using K = void (*)(BlockArgs, EamTabDev<float>, const float*, double*);
+  const K ks[2][4] = {{k_block_gather_ceiling<false, 0>, k_block_gather_ceiling<false, 1>,
+                       k_block_gather_ceiling<false, 2>, k_block_gather_ceiling<false, 3>},
+                      {k_block_gather_ceiling<true, 0>, k_block_gather_ceiling<true, 1>,
+                       k_block_gather_ceiling<true, 2>, k_block_gather_ceiling<true, 3>}};
+  const K k = ks[force ? 1 : 0][level];
+  for (int r = 0; r < reps; r++) k<<<g, kPassThreads, lds, c.stream>>>(a, t, c.eam.pair_f.p, sink.p);
   XS_HIP(hipGetLastError());
   XS_HIP(hipEventRecord(e1, c.stream));
   XS_HIP(hipEventSynchronize(e1));

@@ -379,7 +379,7 @@ void check_or_defer_overflow(Ctx& c);
 // found the flag set
 void poll_pass_overflow(Ctx& c, bool wait);
 // average ms of the gather-ceiling kernel of the density (force = false) or force pass
-double gather_ceiling(Ctx& c, double rcut, bool force, int reps);
+double gather_ceiling(Ctx& c, double rcut, bool force, int level, int reps);
 void pair_count(Ctx& c, double rcut, int64_t& entries, int64_t& within);
 
 // ---- launches (k_integrate.hip) ----

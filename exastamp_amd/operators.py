@@ -271,12 +271,14 @@ class Simulation:
         return {"s_pieces": int(n.value), "r_lo": float(rlo.value),
                 "ulps": {"rho_slope_over_r": float(err[0]), "phi": float(err[1]), "phi_slope_over_r": float(err[2])}}
 
-    def gather_ceiling(self, rcut, force_pass, reps=10) -> float:
+    def gather_ceiling(self, rcut, force_pass, reps=10, level=0) -> float:
         """Average ms of the gather-ceiling kernel of the mixed single-element density
         (force_pass False) or force pass (True) on the current block list: the pass's
-        staging, list walk and table-row gathers without its spline arithmetic."""
+        staging, list walk and table-row gathers without its spline arithmetic (level
+        0); level 1 without the table-row gathers, 2 the list words only, 3 the staging
+        only."""
         ms = C.c_double(0.0)
-        self.ctx.xs_gather_ceiling(float(rcut), 1 if force_pass else 0, int(reps), C.byref(ms))
+        self.ctx.xs_gather_ceiling(float(rcut), 1 if force_pass else 0, int(level), int(reps), C.byref(ms))
         return float(ms.value)
 
     JOHNSON_PARAMETERS = JOHNSON_PARAMETERS

@@ -325,10 +325,12 @@ xs_status xs_stream_copy(xs_ctx* ctx, size_t bytes, int reps, double* gbs);
  * the density pass (force_pass = 0: knot rows) or the force pass (force_pass = 1:
  * s-table rows) does to reach its table rows -- staging, list walk, neighbour reads,
  * FP64 displacement, cutoff test, row index, the row gathers themselves -- and none of
- * the spline or pair arithmetic.  Its time over the pass's own is the share of the
- * pass that its data movement accounts for (the roofline of a gather-bound pass).
- * Writes no simulation state. */
-xs_status xs_gather_ceiling(xs_ctx* ctx, double rcut, int force_pass, int reps, double* ms);
+ * the spline or pair arithmetic (level 0).  Its time over the pass's own is the share
+ * of the pass that its data movement accounts for (the roofline of a gather-bound
+ * pass).  Levels 1-3 take parts away to split that time: 1 without the table-row
+ * gathers, 2 the list words only (no neighbour records or displacements), 3 the
+ * staging of the bricks' halos only.  Writes no simulation state. */
+xs_status xs_gather_ceiling(xs_ctx* ctx, double rcut, int force_pass, int level, int reps, double* ms);
 /* on != 0 enables and clears the counters; on == 0 disables. */
 xs_status xs_profile_enable(xs_ctx* ctx, int on);
 /* ms_total[XS_PROF_NCLASSES] and launches[XS_PROF_NCLASSES] (either may be NULL). */

@@ -153,9 +153,12 @@ def test_gather_ceiling(oracle_mod, monkeypatch):
     sim = _sim(cfg, tab, monkeypatch)
     f0, e0 = _mixed(sim, tab)
     for force in (False, True):
-        ms = sim.gather_ceiling(tab.rc, force, 3)
-        print(f"gather ceiling ({'force' if force else 'density'} pass): {ms:.4f} ms")
-        assert 0.0 < ms < 1e3
+        for level in range(4):
+            ms = sim.gather_ceiling(tab.rc, force, 3, level=level)
+            print(f"gather ceiling ({'force' if force else 'density'} pass, level {level}): {ms:.4f} ms")
+            assert 0.0 < ms < 1e3
+    with pytest.raises(XsError):
+        sim.gather_ceiling(tab.rc, False, 2, level=4)
     f1, e1 = _mixed(sim, tab)
     assert np.array_equal(f0, f1) and e0 == e1
     with pytest.raises(XsError):
