@@ -89,9 +89,9 @@ SIGNATURES = {
     "xs_thermo": (C.c_int, [_CTX, _D]),
     "xs_thermo_noalloc": (C.c_int, [_CTX, C.c_void_p]),
     "xs_eam_adjoint": (C.c_int, [_CTX, C.c_double, _D, _D, _D, C.c_uint]),
-    "xs_eam_adjoint_dev": (C.c_int, [_CTX, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint]),
+    "xs_eam_adjoint_dev": (C.c_int, [_CTX, C.c_double, C.c_void_p, C.c_void_p, C.c_double, C.c_void_p, C.c_uint]),
     "xs_eam_theta_dev": (C.c_int, [_CTX, C.c_void_p]),
-    "xs_forces_energies_dev": (C.c_int, [_CTX, C.c_void_p, C.c_void_p]),
+    "xs_forces_energies_dev": (C.c_int, [_CTX, C.c_void_p, C.c_void_p, C.c_double]),
     "xs_lj_adjoint": (C.c_int, [_CTX, C.c_double, C.c_double, C.c_double, _D, _D, _D, C.c_uint]),
     "xs_spline_adjoint": (C.c_int, [C.c_int, C.c_double, _D, _D]),
     "xs_stream_copy": (C.c_int, [_CTX, C.c_size_t, C.c_int, _D]),

@@ -644,12 +644,12 @@ xs_status xs_eam_adjoint(xs_ctx* h, double rcut, const double* dL_dF, const doub
     const bool mixed = (flags & XS_MIXED) != 0;
     require((c.rho_state == RhoState::FpFloat || c.rho_state == RhoState::FpDouble) && c.rho_ckpt, XS_E_STATE,
             "adjoint needs the rho/F' checkpoint of an eam_alloy_force call at the current positions");
-    eam_adjoint(c, rcut, dL_dF, dL_dE, grad, mixed, false);
+    eam_adjoint(c, rcut, dL_dF, dL_dE, 1.0, grad, mixed, false);
   });
 }
 
 xs_status xs_eam_adjoint_dev(xs_ctx* h, double rcut, const double* dL_dF_dev, const double* dL_dE_dev,
-                             double* grad_dev, unsigned flags)
+                             double scale, double* grad_dev, unsigned flags)
 {
   return guarded(h, [&](Ctx& c) {
     require(dL_dF_dev && dL_dE_dev && grad_dev, XS_E_ARG, "null adjoint argument");
@@ -657,7 +657,7 @@ xs_status xs_eam_adjoint_dev(xs_ctx* h, double rcut, const double* dL_dF_dev, co
     require_list(c);
     require((c.rho_state == RhoState::FpFloat || c.rho_state == RhoState::FpDouble) && c.rho_ckpt, XS_E_STATE,
             "adjoint needs the rho/F' checkpoint of an eam_alloy_force call at the current positions");
-    eam_adjoint(c, rcut, dL_dF_dev, dL_dE_dev, grad_dev, (flags & XS_MIXED) != 0, true);
+    eam_adjoint(c, rcut, dL_dF_dev, dL_dE_dev, scale, grad_dev, (flags & XS_MIXED) != 0, true);
   });
 }
 
@@ -670,12 +670,12 @@ xs_status xs_eam_theta_dev(xs_ctx* h, const double* theta_dev)
   });
 }
 
-xs_status xs_forces_energies_dev(xs_ctx* h, double* F_dev, double* E_dev)
+xs_status xs_forces_energies_dev(xs_ctx* h, double* F_dev, double* E_dev, double scale)
 {
   return guarded(h, [&](Ctx& c) {
     require(F_dev && E_dev, XS_E_ARG, "null output");
     require(c.n == 0 || c.list_valid, XS_E_STATE, "no neighbour list (the replica ranges come from its cell order)");
-    forces_energies_dev(c, F_dev, E_dev);
+    forces_energies_dev(c, F_dev, E_dev, scale);
   });
 }
 

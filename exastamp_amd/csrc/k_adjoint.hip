@@ -345,13 +345,15 @@ __global__ void __launch_bounds__(kRedThreads) k_adj_reduce(AdjSlots S, int grou
   g[(S.s[k].g0 + row - S.s[k].off) * 4 + (q & 3)] = v;
 }
 // g (dL/dF) from id order into cell order as position-like records
+// (and times `scale`); the first nrep threads scale the per-replica dL/dE copy in place
 __global__ void k_gather_g(int64_t n, const int64_t* __restrict__ id, const double* __restrict__ g_id,
-                           Pos4* __restrict__ gvec)
+                           Pos4* __restrict__ gvec, double scale, double* __restrict__ gE, int nrep)
 {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nrep) gE[i] *= scale;
   if (i >= n) return;
   const int64_t q = id[i];
-  store_pos(gvec, i, Pos4{g_id[3 * q], g_id[3 * q + 1], g_id[3 * q + 2], 0.0});
+  store_pos(gvec, i, Pos4{g_id[3 * q] * scale, g_id[3 * q + 1] * scale, g_id[3 * q + 2] * scale, 0.0});
 }
 
 // ---------------------------------------------------------------------------
@@ -682,8 +684,8 @@ __global__ void k_spline_adjoint(int nel, int nz, int nrho, int nr, const double
 
 }  // namespace
 
-void eam_adjoint(Ctx& c, double rcut, const double* dLdF_id, const double* dLdE, double* grad, bool mixed,
-                 bool device)
+void eam_adjoint(Ctx& c, double rcut, const double* dLdF_id, const double* dLdE, double scale, double* grad,
+                 bool mixed, bool device)
 {
   ensure_global_list(c);
   EamHost& e = c.eam;
@@ -701,7 +703,8 @@ void eam_adjoint(Ctx& c, double rcut, const double* dLdF_id, const double* dLdE,
   XS_HIP(hipMemcpyAsync(c.adj_in.p, dLdF_id, sizeof(double) * 3 * c.n, in_kind, c.stream));
   XS_HIP(hipMemcpyAsync(c.adj_in.p + 3 * c.n, dLdE, sizeof(double) * c.nrep, in_kind, c.stream));
   if (c.n > 0) {
-    k_gather_g<<<grid_for(c.n), kBlock, 0, c.stream>>>(c.n, c.id.p, c.adj_in.p, c.adj_gvec.p);
+    k_gather_g<<<grid_for(std::max<int64_t>(c.n, c.nrep)), kBlock, 0, c.stream>>>(
+        c.n, c.id.p, c.adj_in.p, c.adj_gvec.p, scale, c.adj_in.p + 3 * c.n, c.nrep);
     XS_HIP(hipGetLastError());
     AdjArgs a;
     a.n = c.n;
@@ -748,7 +751,7 @@ void lj_adjoint(Ctx& c, double eps, double sigma, double rcut, const double* dLd
   XS_HIP(hipMemcpyAsync(c.adj_in.p + 3 * c.n, dLdE, sizeof(double) * c.nrep, hipMemcpyHostToDevice, c.stream));
   double* out = c.adj_part.p + 2 * (size_t)std::max(g, 1u);
   if (c.n > 0) {
-    k_gather_g<<<grid_for(c.n), kBlock, 0, c.stream>>>(c.n, c.id.p, c.adj_in.p, c.adj_gvec.p);
+    k_gather_g<<<grid_for(c.n), kBlock, 0, c.stream>>>(c.n, c.id.p, c.adj_in.p, c.adj_gvec.p, 1.0, nullptr, 0);
     XS_HIP(hipGetLastError());
     LjAdjArgs a;
     a.n = c.n;

@@ -304,15 +304,16 @@ namespace {
 // F (id order) = m_type a_i, the forces behind the accelerations force_to_accel left
 __global__ void k_forces_id(int64_t n, const int64_t* __restrict__ id, const uint8_t* __restrict__ type,
                             const double* __restrict__ mass, const double* __restrict__ ax,
-                            const double* __restrict__ ay, const double* __restrict__ az, double* __restrict__ F)
+                            const double* __restrict__ ay, const double* __restrict__ az, double scale,
+                            double* __restrict__ F)
 {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const double m = mass[type[i]];
   double* f = F + 3 * id[i];
-  f[0] = ax[i] * m;
-  f[1] = ay[i] * m;
-  f[2] = az[i] * m;
+  f[0] = ax[i] * m * scale;
+  f[1] = ay[i] * m * scale;
+  f[2] = az[i] * m * scale;
 }
 
 // Per-replica energy: the atoms of replica r are the cell-sorted range
@@ -344,30 +345,31 @@ __global__ void __launch_bounds__(kRepThreads) k_replica_partial(const int* __re
   }
 }
 
-__global__ void k_replica_final(int nrep, const double* __restrict__ part, double* __restrict__ E)
+__global__ void k_replica_final(int nrep, const double* __restrict__ part, double scale, double* __restrict__ E)
 {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= nrep) return;
   double t = 0.0;
   for (int b = 0; b < kRepChunks; b++) t += part[r * kRepChunks + b];
-  E[r] = t;
+  E[r] = t * scale;
 }
 
 }  // namespace
 
-void forces_energies_dev(Ctx& c, double* F, double* E)
+void forces_energies_dev(Ctx& c, double* F, double* E, double scale)
 {
   if (c.n == 0) {
     XS_HIP(hipMemsetAsync(E, 0, sizeof(double) * c.nrep, c.stream));
     return;
   }
-  k_forces_id<<<grid_for(c.n), kBlock, 0, c.stream>>>(c.n, c.id.p, c.type.p, c.mass_d.p, c.ax.p, c.ay.p, c.az.p, F);
+  k_forces_id<<<grid_for(c.n), kBlock, 0, c.stream>>>(c.n, c.id.p, c.type.p, c.mass_d.p, c.ax.p, c.ay.p, c.az.p,
+                                                     scale, F);
   XS_HIP(hipGetLastError());
   c.rep_part.reserve((size_t)c.nrep * kRepChunks);
   k_replica_partial<<<(unsigned)(c.nrep * kRepChunks), kRepThreads, 0, c.stream>>>(c.cell_off.p, c.ncell_rep,
                                                                                    c.ep.p, c.rep_part.p);
   XS_HIP(hipGetLastError());
-  k_replica_final<<<grid_for(c.nrep), kBlock, 0, c.stream>>>(c.nrep, c.rep_part.p, E);
+  k_replica_final<<<grid_for(c.nrep), kBlock, 0, c.stream>>>(c.nrep, c.rep_part.p, scale, E);
   XS_HIP(hipGetLastError());
 }
 

@@ -442,11 +442,12 @@ template <class Real>
 const Real* eam_pair_table(Ctx& c);
 
 // ---- EAM adjoint (k_adjoint.hip) ----
-// dL/dtheta of eam_alloy_force.  dLdF_id (3n), dLdE (nrep) and grad are device
-// pointers when `device` is set (nothing is copied to or from the host and the call
-// does not synchronise), host pointers otherwise.
-void eam_adjoint(Ctx& c, double rcut, const double* dLdF_id, const double* dLdE, double* grad, bool mixed,
-                 bool device);
+// dL/dtheta of eam_alloy_force for the incoming gradients dLdF_id (3n) and dLdE (nrep)
+// times `scale`.  They and grad are device pointers when `device` is set (nothing is
+// copied to or from the host and the call does not synchronise), host pointers
+// otherwise.
+void eam_adjoint(Ctx& c, double rcut, const double* dLdF_id, const double* dLdE, double scale, double* grad,
+                 bool mixed, bool device);
 void spline_interpolate_adjoint(int n, double delta, const double* G, double* out);
 // dL/d(epsilon, sigma) of lj_compute_force's forces and shifted energies (k_adjoint.hip)
 void lj_adjoint(Ctx& c, double eps, double sigma, double rcut, const double* dLdF_id, const double* dLdE,
@@ -462,9 +463,9 @@ void eam_upload(Ctx& c);
 // rows of both precisions built on the device, bit for bit the host build; the FP32
 // s = r^2 tables are dropped (the mixed force pass reads the knot rows).
 void eam_theta_dev(Ctx& c, const double* theta);
-// forces m a_i in id order (3n, x y z per atom) and per-replica energies (nrep) into
-// device memory, on the context's stream
-void forces_energies_dev(Ctx& c, double* F, double* E);
+// forces m a_i in id order (3n, x y z per atom) and per-replica energies (nrep), both
+// times `scale`, into device memory, on the context's stream
+void forces_energies_dev(Ctx& c, double* F, double* E, double scale);
 
 // ---- helpers ----
 constexpr int kBlock = 256;

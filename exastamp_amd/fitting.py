@@ -55,7 +55,7 @@ class EamForcesEnergies(torch.autograd.Function):
         if theta.is_cuda:
             _on_torch_stream(sim, theta.device)
             th = theta.detach().to(torch.float64).contiguous()
-            if th.numel() != sim.eam.theta().size:
+            if th.numel() != sim.eam_knots:
                 raise ValueError("theta does not match the loaded tables' knot count")
             sim.ctx.xs_eam_theta_dev(C.c_void_p(th.data_ptr()))
             sim.zero_force_energy()
@@ -63,8 +63,8 @@ class EamForcesEnergies(torch.autograd.Function):
             sim.force_to_accel()
             F = torch.empty((sim.n, 3), dtype=torch.float64, device=theta.device)
             E = torch.empty(sim.cfg.nrep, dtype=torch.float64, device=theta.device)
-            sim.ctx.xs_forces_energies_dev(C.c_void_p(F.data_ptr()), C.c_void_p(E.data_ptr()))
-            return F / EV, E / EV
+            sim.ctx.xs_forces_energies_dev(C.c_void_p(F.data_ptr()), C.c_void_p(E.data_ptr()), 1.0 / EV)
+            return F, E
         t = sim.eam.with_theta(theta.detach().to("cpu", torch.float64).numpy())
         sim.eam_alloy_init(t)
         sim.zero_force_energy()
@@ -80,11 +80,11 @@ class EamForcesEnergies(torch.autograd.Function):
         sim = ctx.sim
         if ctx.device:
             _on_torch_stream(sim, gF.device)
-            gf = (gF.detach().to(torch.float64) / EV).contiguous()
-            ge = (gE.detach().to(torch.float64) / EV).contiguous()
-            grad = torch.empty(sim.eam.theta().size, dtype=torch.float64, device=gF.device)
+            gf = gF.detach().to(torch.float64).contiguous()
+            ge = gE.detach().to(torch.float64).contiguous()
+            grad = torch.empty(sim.eam_knots, dtype=torch.float64, device=gF.device)
             sim.ctx.xs_eam_adjoint_dev(float(sim.eam.rc), C.c_void_p(gf.data_ptr()), C.c_void_p(ge.data_ptr()),
-                                       C.c_void_p(grad.data_ptr()), XS_MIXED if ctx.adjoint_mixed else 0)
+                                       1.0 / EV, C.c_void_p(grad.data_ptr()), XS_MIXED if ctx.adjoint_mixed else 0)
             return grad, None, None, None
         gf = gF.detach().to("cpu", torch.float64).numpy() / EV
         ge = gE.detach().to("cpu", torch.float64).numpy() / EV

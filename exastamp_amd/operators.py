@@ -100,6 +100,8 @@ class Simulation:
         self.lj_params: tuple[float, float] | None = None
         self.lj_rcut = 0.0
         self.eam: SetflTables | None = None
+        self.eam_knots = 0          # knots of the loaded tables (the length of theta)
+        self._types_set = None      # the species mask eam_alloy_force last sent (not re-sent unchanged)
 
     # --- set-up -------------------------------------------------------------
     def species(self, spec):
@@ -259,6 +261,7 @@ class Simulation:
         self.ctx.xs_eam_alloy_tables(t.nel, t.nrho, t.drho, t.nr, t.dr, t.rc, _lib.dptr(fr), _lib.dptr(rr),
                                      _lib.dptr(zr))
         self.eam = t
+        self.eam_knots = fr.size + rr.size + zr.size
 
     def eam_table_info(self) -> dict:
         """The FP32 s = r^2 tables of the mixed single-element force pass: pieces per table
@@ -323,7 +326,10 @@ class Simulation:
         the same quantities up to FP64 summation order.  types (eam_potential_multimat.cu:90,
         154-166): an empty list means every species; otherwise only pairs of two listed
         species contribute, and atoms of an unlisted species get rho = F' = 0 and no force."""
-        self.ctx.xs_eam_alloy_types(self._types_mask(types))
+        mask = self._types_mask(types)
+        if mask != self._types_set:
+            self.ctx.xs_eam_alloy_types(mask)
+            self._types_set = mask
         rc = parse_quantity(rcut)
         self.rcut_max = max(self.rcut_max, rc)
         flags = self._flags(trigger_thermo_state, compute_virial, mixed) | (XS_SYMMETRIC if eam_symmetry else 0)

@@ -246,12 +246,14 @@ xs_status xs_thermo_noalloc(xs_ctx* ctx, double* out27_dev);
  * precision of the forward. */
 xs_status xs_eam_adjoint(xs_ctx* ctx, double rcut, const double* dL_dF, const double* dL_dE,
                          double* grad, unsigned flags);
-/* xs_eam_adjoint with its three arrays in device memory: nothing is copied to or from
- * the host and the call does not synchronise (grad_dev is complete when the context's
- * stream reaches it).  No counterpart in exaStamp; for fitting loops that keep the
- * knots, the forces and the loss on the device. */
+/* xs_eam_adjoint with its three arrays in device memory, for the incoming gradients
+ * dL_dF_dev and dL_dE_dev times `scale` (e.g. 1/EV for gradients with respect to forces
+ * and energies in eV units): nothing is copied to or from the host and the call does not
+ * synchronise (grad_dev is complete when the context's stream reaches it).  No
+ * counterpart in exaStamp; for fitting loops that keep the knots, the forces and the
+ * loss on the device. */
 xs_status xs_eam_adjoint_dev(xs_ctx* ctx, double rcut, const double* dL_dF_dev, const double* dL_dE_dev,
-                             double* grad_dev, unsigned flags);
+                             double scale, double* grad_dev, unsigned flags);
 /* No counterpart in exaStamp (the fitting counterpart of eam_alloy_init for new knot
  * values): theta_dev holds knots in xs_eam_adjoint's setfl order, on the grid (nel,
  * nrho, drho, nr, dr, rcut) of the tables loaded last, in device memory.  The spline
@@ -263,9 +265,10 @@ xs_status xs_eam_theta_dev(xs_ctx* ctx, const double* theta_dev);
 /* No counterpart in exaStamp (the device form of reading forces and energies back for
  * a fitting loss): F_dev (3n doubles, id order, x y z per atom) = m a_i of the current
  * accelerations (valid after force_to_accel), E_dev (nrep doubles) = each replica's
- * total potential energy (sum of ep, needs a force call with XS_ENERGY), in internal
- * units.  No copy to the host, no synchronisation. */
-xs_status xs_forces_energies_dev(xs_ctx* ctx, double* F_dev, double* E_dev);
+ * total potential energy (sum of ep, needs a force call with XS_ENERGY), both in
+ * internal units times `scale` (e.g. 1/EV for eV units).  No copy to the host, no
+ * synchronisation. */
+xs_status xs_forces_energies_dev(xs_ctx* ctx, double* F_dev, double* E_dev, double scale);
 /* No counterpart in exaStamp: reverse mode of lj_compute_force (pair_potential_impl.hxx:
  * 39-513 with lennard_jones.h:40-50) with respect to its parameters, for fitting
  * epsilon and sigma to reference forces and energies.  Differentiates the forces

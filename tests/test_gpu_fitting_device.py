@@ -70,7 +70,7 @@ def test_theta_dev_forces_energies(name):
     _forward(sim, tab)
     F = torch.empty((cfg.n, 3), dtype=torch.float64, device="cuda")
     E = torch.empty(cfg.nrep, dtype=torch.float64, device="cuda")
-    sim.ctx.xs_forces_energies_dev(C.c_void_p(F.data_ptr()), C.c_void_p(E.data_ptr()))
+    sim.ctx.xs_forces_energies_dev(C.c_void_p(F.data_ptr()), C.c_void_p(E.data_ptr()), 1.0)
     F, E = F.cpu().numpy(), E.cpu().numpy()
     print(f"{name}: max |dF| {np.abs(F - F_host).max():.3e}, max |dE|/|E| {np.abs((E - E_host) / E_host).max():.3e}")
     assert np.array_equal(F, F_host)
@@ -96,7 +96,9 @@ def test_adjoint_dev_equals_host(name, mixed):
     ged = torch.tensor(ge, device="cuda")
     out = torch.empty(host.size, dtype=torch.float64, device="cuda")
     sim.ctx.xs_ctx_set_stream(C.c_void_p(torch.cuda.current_stream().cuda_stream))
-    sim.ctx.xs_eam_adjoint_dev(float(tab.rc), C.c_void_p(gd.data_ptr()), C.c_void_p(ged.data_ptr()),
+    # dL/dF and dL/dE given at twice their value, scaled back by the call
+    gd2, ged2 = 2.0 * gd, 2.0 * ged
+    sim.ctx.xs_eam_adjoint_dev(float(tab.rc), C.c_void_p(gd2.data_ptr()), C.c_void_p(ged2.data_ptr()), 0.5,
                                C.c_void_p(out.data_ptr()), _lib.XS_MIXED if mixed else 0)
     dev = out.cpu().numpy()
     sim.ctx.xs_ctx_own_stream()
