@@ -139,3 +139,70 @@ def test_torch_harness_on_device(name):
     assert np.abs(gg - gc).max() <= 1e-10 * np.abs(gc).max()
     sim.ctx.xs_ctx_own_stream()
     sim.close()
+
+
+def test_device_entry_points_refuse_out_of_order_calls():
+    """Each device entry point names what it needs: tables before xs_eam_theta_dev, a
+    neighbour list before xs_forces_energies_dev, the forward's checkpoint before
+    xs_eam_adjoint_dev; null pointers are argument errors."""
+    import torch
+    from exastamp_amd._lib import XsError
+    from exastamp_amd.operators import Simulation
+    cfg, masses, names, tab = _case("cu_batch")
+    buf = torch.zeros(max(3 * cfg.n, tab.theta().size), dtype=torch.float64, device="cuda")
+    p = C.c_void_p(buf.data_ptr())
+    sim = Simulation(0)
+    sim.species(list(zip(names, masses)))
+    sim.set_config(cfg)
+    with pytest.raises(XsError, match="eam tables"):
+        sim.ctx.xs_eam_theta_dev(p)
+    with pytest.raises(XsError, match="neighbour list"):
+        sim.ctx.xs_forces_energies_dev(p, p, 1.0)
+    sim.eam_alloy_init(tab)
+    sim.chunk_neighbors(rcut_max=tab.rc, rcut_inc=0.3)
+    with pytest.raises(XsError, match="checkpoint"):
+        sim.ctx.xs_eam_adjoint_dev(float(tab.rc), p, p, 1.0, p, 0)
+    with pytest.raises(XsError):
+        sim.ctx.xs_eam_theta_dev(None)
+    with pytest.raises(XsError):
+        sim.ctx.xs_forces_energies_dev(None, p, 1.0)
+    sim.close()
+
+
+def test_harness_on_a_side_stream():
+    """The device harness on a non-default torch stream (default priority) gives the
+    default stream's results bit for bit, and the context returns to its own stream for
+    host-path calls afterwards."""
+    import torch
+    from exastamp_amd.fitting import EamForcesEnergies, fit_loss
+    cfg, masses, names, tab = _case("cu_batch")
+    sim = _sim(cfg, masses, names, tab)
+    from exastamp_amd.units import EV
+    _forward(sim, tab)
+    Ft = torch.tensor(sim.forces() / EV, device="cuda")
+    Et = torch.tensor(sim.replica_energies() / EV, device="cuda")
+    nper = torch.tensor(np.bincount(cfg.rep), dtype=torch.float64, device="cuda")
+    th = _perturbed(tab, seed=17)
+
+    def step():
+        theta = torch.tensor(th, device="cuda", requires_grad=True)
+        F, E = EamForcesEnergies.apply(theta, sim)
+        fit_loss(F, E, Ft, Et, nper).backward()
+        return F.detach().cpu().numpy(), theta.grad.cpu().numpy()
+
+    F0, g0 = step()
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        F1, g1 = step()
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    assert np.array_equal(F0, F1)
+    scale = np.abs(g0).max()
+    assert np.abs(g1 - g0).max() <= 1e-12 * scale   # LDS accumulation order
+    sim.ctx.xs_ctx_own_stream()
+    # the host path on the context's own stream still works and agrees
+    sim.eam_alloy_init(tab.with_theta(th))
+    _forward(sim, tab)
+    assert np.abs(sim.forces() / EV - F0).max() <= 1e-12 * np.abs(F0).max()
+    sim.close()
