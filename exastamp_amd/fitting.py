@@ -122,7 +122,8 @@ class LjForcesEnergies(torch.autograd.Function):
 
 def fit_loss(F, E, F_target, E_target, atoms_per_replica, energy_weight=0.1):
     """Force MSE (eV/A)^2 over all components plus energy_weight * MSE of the
-    per-atom energy (eV) over replicas."""
+    per-atom energy (eV) over replicas.  In a device loop give atoms_per_replica as a
+    tensor on the device: a host array is copied (synchronously) at every call."""
     n = torch.as_tensor(atoms_per_replica, dtype=F.dtype, device=F.device)
     return ((F - F_target) ** 2).mean() + energy_weight * (((E - E_target) / n) ** 2).mean()
 
