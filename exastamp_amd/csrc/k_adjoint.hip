@@ -27,9 +27,11 @@
 // workgroups walk all atoms (or pairs), add the contributions that fall in a window
 // into their LDS copy (LDS FP64 atomics), and write it out as a partial; a second
 // kernel sums the partials over the workgroups in a fixed order.  Windows cover
-// every row of every table, one launch per group of windows.  When every row of
-// every table fits in one launch (small tables), one kernel does it all per atom:
-// B1 (lambda_i stays in a register) and then B2, a single sweep of the list.
+// every row of every table, one launch per group of windows.  The first launch walks
+// the list for B1 (lambda_i stays in registers) and then B2 per atom; when every row
+// of every table fits in it (small tables) that is the whole sweep, otherwise it
+// stores lambda_i and the F-table weights for the F-row launches and the later B2
+// launches.
 #include "xsmd_internal.hpp"
 
 #include <algorithm>
@@ -212,20 +214,6 @@ __device__ __forceinline__ void adj_b1_atom(const AdjArgs& a, const AdjTab<Real>
   }
 }
 
-// B1: per atom, drho_i and lambda_i, plus the F-table gradient.
-template <class Real, bool MULTI>
-__global__ void __launch_bounds__(256) k_adj_b1(AdjArgs a, AdjTab<Real> t)
-{
-  const int64_t i = lblock() * blockDim.x + threadIdx.x;
-  if (i >= a.n) return;
-  double lam, o[4];
-  int m, T;
-  adj_b1_atom<Real, MULTI, 1>(a, t, i, 0, lam, o, m, T);
-  a.lam[i] = lam;
-  store_pos(reinterpret_cast<Pos4*>(a.fc), i, Pos4{o[0], o[1], o[2], o[3]});
-  a.fm[i] = m;
-}
-
 // F-table rows: the per-atom contributions of B1 into this launch's windows
 template <class Real, bool MULTI>
 __global__ void __launch_bounds__(kAdjThreads) k_adj_fsum(AdjArgs a, AdjTab<Real> t, AdjSlots S,
@@ -295,11 +283,14 @@ __global__ void __launch_bounds__(kAdjThreads) k_adj_b2(AdjArgs a, AdjTab<Real> 
   lds_out(acc, S, part);
 }
 
-// Everything in one launch, when the rows of every table fit in its windows (small
-// tables, small systems): kFusedLanes lanes per atom walk its list twice, B1
-// (lambda_i kept in registers, the F-table contribution into LDS) and then B2.
+// B1 and B2 in one launch, kFusedLanes lanes per atom walking its list twice: B1
+// (lambda_i kept in registers), then B2 into this launch's windows.  When the rows of
+// every table fit the launch (small tables, small systems) the F-table contribution
+// goes into LDS too and this is the whole sweep; with STORE (the first of several
+// launches) lambda_i and the F-table contribution are stored for k_adj_fsum and the
+// later k_adj_b2 launches instead, so the list is walked once less.
 constexpr int kFusedLanes = 4;
-template <class Real, bool MULTI>
+template <class Real, bool MULTI, bool STORE>
 __global__ void __launch_bounds__(kAdjThreads) k_adj_fused(AdjArgs a, AdjTab<Real> t, AdjSlots S,
                                                            double* __restrict__ part)
 {
@@ -312,7 +303,15 @@ __global__ void __launch_bounds__(kAdjThreads) k_adj_fused(AdjArgs a, AdjTab<Rea
     double lam, o[4];
     int m, T;
     adj_b1_atom<Real, MULTI, kFusedLanes>(a, t, i, sub, lam, o, m, T);
-    if (sub == 0) lds_add(acc, S, T, m, o);
+    if constexpr (STORE) {
+      if (sub == 0) {
+        a.lam[i] = lam;
+        store_pos(reinterpret_cast<Pos4*>(a.fc), i, Pos4{o[0], o[1], o[2], o[3]});
+        a.fm[i] = m;
+      }
+    } else {
+      if (sub == 0) lds_add(acc, S, T, m, o);
+    }
     adj_b2_atom<Real, MULTI, kFusedLanes>(a, t, S, acc, i, sub, lam);
   }
   lds_out(acc, S, part);
@@ -568,17 +567,15 @@ void adj_run(Ctx& c, const AdjArgs& a)
   };
   const std::vector<AdjSlots> all = adj_launches(nel, nz, rows_f, rows_r, 2, limit);
   if (all.size() == 1) {  // every row fits: one sweep of the list
-    run(multi ? k_adj_fused<Real, true> : k_adj_fused<Real, false>, all[0], kFusedLanes);
+    run(multi ? k_adj_fused<Real, true, false> : k_adj_fused<Real, false, false>, all[0], kFusedLanes);
     return;
   }
-  const unsigned g = (grid_for(c.n) + 7u) / 8u * 8u;
-  if (multi) k_adj_b1<Real, true><<<g, kBlock, 0, c.stream>>>(a, t);
-  else k_adj_b1<Real, false><<<g, kBlock, 0, c.stream>>>(a, t);
-  XS_HIP(hipGetLastError());
+  // B1 with the first window set of the pair tables, then the F tables, then the rest
+  const std::vector<AdjSlots> pairs = adj_launches(nel, nz, rows_f, rows_r, 1, limit);
+  run(multi ? k_adj_fused<Real, true, true> : k_adj_fused<Real, false, true>, pairs[0], kFusedLanes);
   for (const AdjSlots& S : adj_launches(nel, nz, rows_f, rows_r, 0, limit))
     run(multi ? k_adj_fsum<Real, true> : k_adj_fsum<Real, false>, S, 1);
-  for (const AdjSlots& S : adj_launches(nel, nz, rows_f, rows_r, 1, limit))
-    run(multi ? k_adj_b2<Real, true> : k_adj_b2<Real, false>, S, 1);
+  for (size_t k = 1; k < pairs.size(); k++) run(multi ? k_adj_b2<Real, true> : k_adj_b2<Real, false>, pairs[k], 1);
 }
 
 }  // namespace
