@@ -120,12 +120,35 @@ class LjForcesEnergies(torch.autograd.Function):
         return torch.from_numpy(g).to(gF.device), None, None, None
 
 
+class _FitLoss(torch.autograd.Function):
+    """fit_loss as one autograd node: L = mean(rF^2) + w mean(rE^2) with rF = F - F_t and
+    rE = (E - E_t) / n; dL/dF = 2 rF / size(rF), dL/dE = 2 w rE / (n size(rE)).  One node
+    instead of the dozen of the elementwise form keeps a device fitting step from being
+    bound by the autograd engine's per-node work on the host."""
+
+    @staticmethod
+    def forward(ctx, F, E, F_target, E_target, n, energy_weight):
+        rF = F - F_target
+        rE = (E - E_target) / n
+        ctx.save_for_backward(rF, rE, n)
+        ctx.energy_weight = energy_weight
+        return (rF * rF).mean() + energy_weight * (rE * rE).mean()
+
+    @staticmethod
+    def backward(ctx, g):
+        rF, rE, n = ctx.saved_tensors
+        gF = rF * (g * (2.0 / rF.numel()))
+        gE = rE / n * (g * (2.0 * ctx.energy_weight / rE.numel()))
+        return gF, gE, None, None, None, None
+
+
 def fit_loss(F, E, F_target, E_target, atoms_per_replica, energy_weight=0.1):
     """Force MSE (eV/A)^2 over all components plus energy_weight * MSE of the
     per-atom energy (eV) over replicas.  In a device loop give atoms_per_replica as a
-    tensor on the device: a host array is copied (synchronously) at every call."""
+    tensor on the device: a host array is copied (synchronously) at every call.
+    Gradients flow to F and E (not to the targets)."""
     n = torch.as_tensor(atoms_per_replica, dtype=F.dtype, device=F.device)
-    return ((F - F_target) ** 2).mean() + energy_weight * (((E - E_target) / n) ** 2).mean()
+    return _FitLoss.apply(F, E, F_target.detach(), E_target.detach(), n, energy_weight)
 
 
 def fit_loss_np(F, E, F_target, E_target, atoms_per_replica, energy_weight=0.1):

@@ -88,3 +88,25 @@ def test_atomic_mass_constant_from_recorded_density():
     assert volume == 72.0 ** 3
     mass_da = 8000 * 26.982 + 8000 * 63.546
     assert abs(mass_da * units.AMU_KG * 1e3 / (volume * 1e-24) / rho - 1.0) < 2e-12
+
+
+def test_fit_loss_is_the_elementwise_loss():
+    """fit_loss (one autograd node with its analytic backward) gives the value and the
+    gradients of the elementwise form: force MSE + 0.1 per-atom energy MSE."""
+    import torch
+    from exastamp_amd.fitting import fit_loss
+    g = torch.Generator().manual_seed(3)
+    F = torch.randn(120, 3, dtype=torch.float64, generator=g, requires_grad=True)
+    E = torch.randn(5, dtype=torch.float64, generator=g, requires_grad=True)
+    Ft = torch.randn(120, 3, dtype=torch.float64, generator=g)
+    Et = torch.randn(5, dtype=torch.float64, generator=g)
+    n = [20, 30, 25, 25, 20]
+    L = fit_loss(F, E, Ft, Et, n)
+    (3.0 * L).backward()
+    gF, gE = F.grad.clone(), E.grad.clone()
+    F.grad = E.grad = None
+    nt = torch.tensor(n, dtype=torch.float64)
+    L2 = ((F - Ft) ** 2).mean() + 0.1 * (((E - Et) / nt) ** 2).mean()
+    (3.0 * L2).backward()
+    assert abs(float(L.detach()) - float(L2.detach())) <= 1e-15 * float(L2.detach())
+    assert torch.allclose(gF, F.grad, rtol=1e-14, atol=0) and torch.allclose(gE, E.grad, rtol=1e-14, atol=1e-30)
