@@ -24,6 +24,8 @@ XS_FIELD_EP = 0
 XS_FIELD_VIRIAL = 1
 XS_FIELD_RHO_DEMB = 2
 XS_FIELD_ACCEL = 3
+XS_XFORM_IDENTITY = 0
+XS_XFORM_INV = 1
 
 PROF_CLASSES = ["neighbors", "eam_rho", "eam_force", "lj", "drift", "thermo", "adjoint"]
 
@@ -84,7 +86,8 @@ SIGNATURES = {
     "xs_eam_alloy_types": (C.c_int, [_CTX, C.c_uint]),
     "xs_eam_table_info": (C.c_int, [_CTX, C.POINTER(C.c_int), _D, _D]),
     "xs_johnson_force": (C.c_int, [_CTX, _D, C.c_double, C.c_int, C.c_int, C.c_uint]),
-    "xs_push_f_v_r": (C.c_int, [_CTX, C.c_double, C.c_double]),
+    "xs_push_f_v_r": (C.c_int, [_CTX, C.c_double, C.c_double, C.c_int]),
+    "xs_domain_xform": (C.c_int, [_CTX, _D]),
     "xs_push_f_v": (C.c_int, [_CTX, C.c_double, C.c_double]),
     "xs_thermo": (C.c_int, [_CTX, _D]),
     "xs_thermo_noalloc": (C.c_int, [_CTX, C.c_void_p]),

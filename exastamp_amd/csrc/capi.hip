@@ -703,9 +703,46 @@ xs_status xs_spline_adjoint(int n, double delta, const double* G, double* out)
   return XS_OK;
 }
 
-xs_status xs_push_f_v_r(xs_ctx* h, double dt, double dt_scale)
+xs_status xs_push_f_v_r(xs_ctx* h, double dt, double dt_scale, int xform_mode)
 {
-  return guarded(h, [&](Ctx& c) { launch_push_f_v_r(c, dt * dt_scale); });
+  return guarded(h, [&](Ctx& c) {
+    if (xform_mode != XS_XFORM_IDENTITY && xform_mode != XS_XFORM_INV)
+      throw Error(XS_E_ARG, "push_f_v_r: xform_mode must be XS_XFORM_IDENTITY or XS_XFORM_INV");
+    launch_push_f_v_r(c, dt * dt_scale, xform_mode == XS_XFORM_INV);
+  });
+}
+
+xs_status xs_domain_xform(xs_ctx* h, const double m[9])
+{
+  return guarded(h, [&](Ctx& c) {
+    static const double id[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    XformDev x{};
+    const double* a = m ? m : id;
+    bool ident = true;
+    for (int q = 0; q < 9; q++) {
+      if (!std::isfinite(a[q])) throw Error(XS_E_ARG, "domain xform: non-finite entry");
+      x.m[q] = a[q];
+      ident = ident && a[q] == id[q];
+    }
+    const double det = xform_det(x);
+    if (!(det > 0.0)) throw Error(XS_E_ARG, "domain xform: det(m) must be positive");
+    // inverse by cofactors, as the oracle computes it
+    const double* M = x.m;
+    const double inv[9] = {(M[4] * M[8] - M[5] * M[7]) / det, (M[2] * M[7] - M[1] * M[8]) / det,
+                           (M[1] * M[5] - M[2] * M[4]) / det, (M[5] * M[6] - M[3] * M[8]) / det,
+                           (M[0] * M[8] - M[2] * M[6]) / det, (M[2] * M[3] - M[0] * M[5]) / det,
+                           (M[3] * M[7] - M[4] * M[6]) / det, (M[1] * M[6] - M[0] * M[7]) / det,
+                           (M[0] * M[4] - M[1] * M[3]) / det};
+    for (int q = 0; q < 9; q++) x.mi[q] = inv[q];
+    x.on = ident ? 0 : 1;
+    c.xf = x;
+    c.xf_d.reserve(1);
+    XS_HIP(hipMemcpyAsync(c.xf_d.p, &c.xf, sizeof(XformDev), hipMemcpyHostToDevice, c.stream));
+    XS_HIP(hipStreamSynchronize(c.stream));
+    c.list_valid = false;
+    c.rho_state = RhoState::None;
+    c.rho_ckpt = false;
+  });
 }
 
 xs_status xs_push_f_v(xs_ctx* h, double dt, double dt_scale)

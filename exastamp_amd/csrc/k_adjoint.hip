@@ -83,6 +83,7 @@ struct AdjArgs {
   double* fc;             // F-table contribution of atom i (4 coefficients; B1 out)
   int* fm;                // its F-table row
   int nel;                // elements: table ids F 0..nel-1, rho nel..2nel-1, z2r 2nel..
+  const XformDev* xf;     // physical displacements m * (r_j(image) - r_i) when not NULL
 };
 
 // LDS row windows of one launch: rows [lo, lo + rows) of table `table` at LDS
@@ -175,9 +176,10 @@ __device__ __forceinline__ void adj_b1_atom(const AdjArgs& a, const AdjTab<Real>
     const uint32_t e = a.nbr[(int64_t)k * a.npad + i];
     const uint32_t j = xsmd_nbr_index(e);
     const Pos4 pj = load_pos(a.pos, j);
-    const double dx = xsmd_image(pj.x, xsmd_nbr_img_x(e), Lb[0]) - pi.x;
-    const double dy = xsmd_image(pj.y, xsmd_nbr_img_y(e), Lb[1]) - pi.y;
-    const double dz = xsmd_image(pj.z, xsmd_nbr_img_z(e), Lb[2]) - pi.z;
+    double dx = xsmd_image(pj.x, xsmd_nbr_img_x(e), Lb[0]) - pi.x;
+    double dy = xsmd_image(pj.y, xsmd_nbr_img_y(e), Lb[1]) - pi.y;
+    double dz = xsmd_image(pj.z, xsmd_nbr_img_z(e), Lb[2]) - pi.z;
+    if (a.xf) xsmd_xform_apply(a.xf->m, &dx, &dy, &dz);
     const double d2 = xsmd_d2(dx, dy, dz);
     if (d2 > a.rc2) continue;
     const Pos4 gj = load_pos(a.gvec, j);
@@ -249,9 +251,10 @@ __device__ __forceinline__ void adj_b2_atom(const AdjArgs& a, const AdjTab<Real>
     const uint32_t e = a.nbr[(int64_t)k * a.npad + i];
     const uint32_t j = xsmd_nbr_index(e);
     const Pos4 pj = load_pos(a.pos, j);
-    const double dx = xsmd_image(pj.x, xsmd_nbr_img_x(e), Lb[0]) - pi.x;
-    const double dy = xsmd_image(pj.y, xsmd_nbr_img_y(e), Lb[1]) - pi.y;
-    const double dz = xsmd_image(pj.z, xsmd_nbr_img_z(e), Lb[2]) - pi.z;
+    double dx = xsmd_image(pj.x, xsmd_nbr_img_x(e), Lb[0]) - pi.x;
+    double dy = xsmd_image(pj.y, xsmd_nbr_img_y(e), Lb[1]) - pi.y;
+    double dz = xsmd_image(pj.z, xsmd_nbr_img_z(e), Lb[2]) - pi.z;
+    if (a.xf) xsmd_xform_apply(a.xf->m, &dx, &dy, &dz);
     const double d2 = xsmd_d2(dx, dy, dz);
     if (d2 > a.rc2) continue;
     const Pos4 gj = load_pos(a.gvec, j);
@@ -380,6 +383,7 @@ struct LjAdjArgs {
   const double* gE;       // per replica
   double rc2, eps, sigma, sigma2;
   double decut_eps, decut_sigma;  // de/deps and de/dsigma at r = rc
+  const XformDev* xf;
 };
 
 constexpr int kLjAdjThreads = 256;
@@ -421,9 +425,10 @@ __global__ void __launch_bounds__(kLjAdjThreads) k_lj_adjoint(LjAdjArgs a, doubl
     for (int k = 0; k < cnt; k++) {
       const uint32_t e = a.nbr[(int64_t)k * a.npad + i];
       const Pos4 pj = load_pos(a.pos, xsmd_nbr_index(e));
-      const double dx = xsmd_image(pj.x, xsmd_nbr_img_x(e), L[0]) - pi.x;
-      const double dy = xsmd_image(pj.y, xsmd_nbr_img_y(e), L[1]) - pi.y;
-      const double dz = xsmd_image(pj.z, xsmd_nbr_img_z(e), L[2]) - pi.z;
+      double dx = xsmd_image(pj.x, xsmd_nbr_img_x(e), L[0]) - pi.x;
+      double dy = xsmd_image(pj.y, xsmd_nbr_img_y(e), L[1]) - pi.y;
+      double dz = xsmd_image(pj.z, xsmd_nbr_img_z(e), L[2]) - pi.z;
+      if (a.xf) xsmd_xform_apply(a.xf->m, &dx, &dy, &dz);
       const double d2 = xsmd_d2(dx, dy, dz);
       if (d2 > a.rc2) continue;
       const double ir2 = 1.0 / d2;
@@ -722,6 +727,7 @@ void eam_adjoint(Ctx& c, double rcut, const double* dLdF_id, const double* dLdE,
     a.fc = c.adj_fc.p;
     a.fm = c.adj_fm.p;
     a.nel = nel;
+    a.xf = c.xf_dev();
     ProfScope prof(c.prof, c.stream, XS_PROF_ADJOINT);
     if (mixed) adj_run<float>(c, a);
     else adj_run<double>(c, a);
@@ -764,6 +770,7 @@ void lj_adjoint(Ctx& c, double eps, double sigma, double rcut, const double* dLd
     a.eps = eps;
     a.sigma = sigma;
     a.sigma2 = sigma * sigma;
+    a.xf = c.xf_dev();
     // the derivatives of energy_cutoff = e(rc) (pair_potential_impl.hxx:487-498)
     const double sc2 = a.sigma2 / a.rc2, sc6 = sc2 * sc2 * sc2, sc12 = sc6 * sc6;
     a.decut_eps = 4.0 * (sc12 - sc6);

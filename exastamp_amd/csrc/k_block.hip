@@ -163,16 +163,17 @@ __device__ __forceinline__ int find_cell(const HaloCell* hc, int nh, int c)
   return lo;
 }
 
-// Stage halo atoms: f(c, p, j) for slot c, position p (+ image shift) and, with
-// WITH_W, w (else 0: the 24 bytes of x, y, z are read, not the whole record),
-// global index j; ends with __syncthreads().  slot_cell (total <= its capacity)
+// Stage halo atoms: f(c, p, j) for slot c, position p (+ image shift; under a
+// non-identity xform mapped to physical space, so that the passes' displacements
+// are physical) and, with WITH_W, w (else 0: the 24 bytes of x, y, z are read, not
+// the whole record), global index j; ends with __syncthreads().  slot_cell (total <= its capacity)
 // first receives the halo cell of every slot, so that a thread finds the cell of
 // a slot with one LDS read instead of a binary search over the halo table.
 // (Issuing several slots' position loads per thread before using them measured
 // slower: the force pass lost 4 %.)
 template <bool WITH_W, class F>
 __device__ void stage_each(const BlockShape& s, const HaloCell* hc, int total, const Pos4* __restrict__ pos,
-                           const double* L, uint16_t* slot_cell, F f)
+                           const double* L, const XformDev* xf, uint16_t* slot_cell, F f)
 {
   const int nh = s.hx * s.hy * s.hz;
   for (int h = threadIdx.x; h < nh; h += blockDim.x) {
@@ -197,6 +198,7 @@ __device__ void stage_each(const BlockShape& s, const HaloCell* hc, int total, c
       p.y = xsmd_image(p.y, xsmd_nbr_img_y(code), L[1]);
       p.z = xsmd_image(p.z, xsmd_nbr_img_z(code), L[2]);
     }
+    if (xf) xsmd_xform_apply(xf->m, &p.x, &p.y, &p.z);
     f(c, p, j);
   }
   __syncthreads();
@@ -204,15 +206,15 @@ __device__ void stage_each(const BlockShape& s, const HaloCell* hc, int total, c
 
 // Stage halo atoms into the records (and types when requested).
 __device__ void stage(const BlockShape& s, const HaloCell* hc, int total, const Pos4* __restrict__ pos,
-                      const double* L, const uint8_t* __restrict__ type, const LRec& rec, uint8_t* ltype,
-                      uint16_t* slot_cell)
+                      const double* L, const XformDev* xf, const uint8_t* __restrict__ type, const LRec& rec,
+                      uint8_t* ltype, uint16_t* slot_cell)
 {
   const auto put = [&](int c, const Pos4& p, int j) {
     rec.put(c, p);
     if (ltype) ltype[c] = type[j];
   };
-  if (rec.has_w) stage_each<true>(s, hc, total, pos, L, slot_cell, put);
-  else stage_each<false>(s, hc, total, pos, L, slot_cell, put);
+  if (rec.has_w) stage_each<true>(s, hc, total, pos, L, xf, slot_cell, put);
+  else stage_each<false>(s, hc, total, pos, L, xf, slot_cell, put);
 }
 
 // Central atom q of the block: its halo cell h, local slot and global index.
@@ -455,7 +457,11 @@ __device__ __forceinline__ uint2 bins_interleave(const uint32_t (&b)[2])
 // BINS: each entry also gets its distance key for the sort, min(2^nbits - 1,
 // floor(d2 * bin_scale)) with bin_scale = 2^nbits / rl^2 (bins uniform in d^2), in
 // bins8, grouped like the list.
-template <bool BINS>
+// XF: a non-identity domain xform.  The halo stays in grid space and each candidate's
+// displacement is mapped by xf->m before the test, in the operation order of the
+// global build (so the lists stay bit-identical); grid-space cell boxes do not
+// bound physical distances, so no cell is culled.
+template <bool BINS, bool XF>
 __global__ void __launch_bounds__(kBuildThreads) k_block_build(BlockGeom g, const int* __restrict__ cell_off,
                                                                const Pos4* __restrict__ pos,
                                                                const BoxDev* __restrict__ box, double rl2,
@@ -465,7 +471,7 @@ __global__ void __launch_bounds__(kBuildThreads) k_block_build(BlockGeom g, cons
                                                                int* __restrict__ brick_rows,
                                                                unsigned long long* __restrict__ flags,
                                                                uint8_t* __restrict__ bins8, float bin_scale,
-                                                               int bin_max)
+                                                               int bin_max, const XformDev* __restrict__ xf)
 {
   const int64_t b = lblock();
   if (b >= g.nblocks) return;
@@ -490,10 +496,10 @@ __global__ void __launch_bounds__(kBuildThreads) k_block_build(BlockGeom g, cons
     return;
   }
   const BoxDev bx = box[s.r];
-  const bool cull = *nocull == 0ull;
+  const bool cull = !XF && *nocull == 0ull;
   if (threadIdx.x == 0) l.total[1] = 0;  // longest list of the brick (made visible by stage's barrier)
   central_table(s, l.hc, l.central_local, ccap, l.total + 2);
-  stage_each<false>(s, l.hc, total, pos, bx.L, l.slot_cell,
+  stage_each<false>(s, l.hc, total, pos, bx.L, nullptr, l.slot_cell,
                     [&](int c, const Pos4& p, int) { P[c] = double3{p.x, p.y, p.z}; });
   const int ncentral = l.total[2];
   if (ncentral > ccap) {  // cannot happen when ccap = measured maximum
@@ -560,8 +566,14 @@ __global__ void __launch_bounds__(kBuildThreads) k_block_build(BlockGeom g, cons
           for (int c = e.lstart; c < end; c += 2) {
             // record c + 1 <= cap exists (the next cell's first atom, or padding)
             const double3 p0 = P[c], p1 = P[c + 1];
-            const double d0 = xsmd_d2(p0.x - xi, p0.y - yi, p0.z - zi);
-            const double d1 = xsmd_d2(p1.x - xi, p1.y - yi, p1.z - zi);
+            double u0 = p0.x - xi, v0 = p0.y - yi, w0 = p0.z - zi;
+            double u1 = p1.x - xi, v1 = p1.y - yi, w1 = p1.z - zi;
+            if constexpr (XF) {
+              xsmd_xform_apply(xf->m, &u0, &v0, &w0);
+              xsmd_xform_apply(xf->m, &u1, &v1, &w1);
+            }
+            const double d0 = xsmd_d2(u0, v0, w0);
+            const double d1 = xsmd_d2(u1, v1, w1);
             if (d0 < rl2 && c != a.local) hit(c, d0);
             if (c + 1 < end && d1 < rl2 && c + 1 != a.local) hit(c + 1, d1);
           }
@@ -739,6 +751,7 @@ struct BlockArgs {
   // the host measured at list build, which hold while the cells do not change); such
   // a brick leaves its atoms unwritten and the host reports the call as failed
   unsigned long long* overflow;
+  const XformDev* xf;  // the staging maps positions to physical space when not NULL
 };
 
 // a brick beyond the measured capacities: raise the flag, skip the brick
@@ -815,7 +828,7 @@ __global__ void __launch_bounds__(kPassThreads, kPassWaves<false>) k_block_eam_r
   const int total = build_halo(a.g, s, a.cell_off, l.hc, l.total);
   if (over_capacity(a, total, a.cap)) return;
   central_table(s, l.hc, l.central_local, a.ccap, l.total + 2);
-  stage(s, l.hc, total, a.pos, a.box[s.r].L, a.type, l.rec, l.ltype, l.slot_cell);
+  stage(s, l.hc, total, a.pos, a.box[s.r].L, a.xf, a.type, l.rec, l.ltype, l.slot_cell);
   const int ncentral = l.total[2];
   if (over_capacity(a, ncentral, a.ccap)) return;
   const int half = pass_half();
@@ -881,7 +894,7 @@ __global__ void __launch_bounds__(kPassThreads, kPassWaves<VIRIAL>) k_block_eam_
   const BlockShape s = block_shape(a.g, b);
   const int total = build_halo(a.g, s, a.cell_off, l.hc, l.total);
   if (over_capacity(a, total, a.cap)) return;
-  stage(s, l.hc, total, a.pos, a.box[s.r].L, a.type, l.rec, l.ltype, l.slot_cell);
+  stage(s, l.hc, total, a.pos, a.box[s.r].L, a.xf, a.type, l.rec, l.ltype, l.slot_cell);
   const int half = pass_half();
   for (int q = pass_slot();; q += kPassAtoms) {
     Central ca;
@@ -986,7 +999,7 @@ __global__ void __launch_bounds__(kPassThreads, kPassWaves<false>) k_block_gathe
   const int total = build_halo(a.g, s, a.cell_off, l.hc, l.total);
   if (over_capacity(a, total, a.cap)) return;
   if constexpr (!FORCE) central_table(s, l.hc, l.central_local, a.ccap, l.total + 2);
-  stage(s, l.hc, total, a.pos, a.box[s.r].L, nullptr, l.rec, l.ltype, l.slot_cell);
+  stage(s, l.hc, total, a.pos, a.box[s.r].L, a.xf, nullptr, l.rec, l.ltype, l.slot_cell);
   int ncentral = 0;
   if constexpr (!FORCE) {
     ncentral = l.total[2];
@@ -1076,7 +1089,7 @@ __global__ void __launch_bounds__(kPassThreads, kPassWaves<VIRIAL>) k_block_lj(B
   const int total = build_halo(a.g, s, a.cell_off, l.hc, l.total);
   if (over_capacity(a, total, a.cap)) return;
   central_table(s, l.hc, l.central_local, a.ccap, l.total + 2);
-  stage(s, l.hc, total, a.pos, a.box[s.r].L, MULTI ? a.type : nullptr, l.rec, l.ltype, l.slot_cell);
+  stage(s, l.hc, total, a.pos, a.box[s.r].L, a.xf, MULTI ? a.type : nullptr, l.rec, l.ltype, l.slot_cell);
   const int ncentral = l.total[2];
   if (over_capacity(a, ncentral, a.ccap)) return;
   const int half = pass_half();
@@ -1133,7 +1146,9 @@ __global__ void __launch_bounds__(kPassThreads, kPassWaves<VIRIAL>) k_block_lj(B
 // with the plain MD-loop epilogue; false when it does not apply.
 bool drift_setup(Ctx& c, const ForceEpilogue& epi, unsigned flags, pm::Drift& dr)
 {
-  if (!(epi.fused && epi.drift) || (flags & (XS_ENERGY | XS_VIRIAL))) return false;
+  // under a non-identity xform the loop's own drift kernel (which maps the step by the
+  // inverse xform) runs instead, so this epilogue stays the identity form
+  if (!(epi.fused && epi.drift) || (flags & (XS_ENERGY | XS_VIRIAL)) || c.xf_dev()) return false;
   c.pos2.reserve(c.npad);
   dr.pos_out = c.pos2.p;
   dr.rrx = c.rrx.p;
@@ -1168,6 +1183,7 @@ void block_eam_dispatch(Ctx& c, double rcut, bool do_rho, bool do_emb, bool do_f
   a.pos = c.pos.p;
   a.type = c.type.p;
   a.box = c.box_d.p;
+  a.xf = c.xf_dev();
   a.nbr16 = c.nbr16.p;
   a.nnb = c.nnb.p;
   a.npad = c.npad;
@@ -1237,6 +1253,7 @@ double gather_ceiling(Ctx& c, double rcut, bool force, int level, int reps)
   a.pos = c.pos.p;
   a.type = c.type.p;
   a.box = c.box_d.p;
+  a.xf = c.xf_dev();
   a.nbr16 = c.nbr16.p;
   a.nnb = c.nnb.p;
   a.npad = c.npad;
@@ -1383,11 +1400,13 @@ void block_build(Ctx& c, double rl2)
     c.nnb.reserve(c.npad);
     c.brick_rows.reserve(c.bgeom.nblocks);
     XS_HIP(hipMemsetAsync(c.dscalar.p, 0, 2 * sizeof(unsigned long long), c.stream));
-    auto k = sorted ? k_block_build<true> : k_block_build<false>;
+    auto k = c.xf_dev() ? (sorted ? k_block_build<true, true> : k_block_build<false, true>)
+                     : (sorted ? k_block_build<true, false> : k_block_build<false, false>);
     k<<<block_grid(c), kBuildThreads, lds, c.stream>>>(c.bgeom, c.cell_off.p, c.pos.p, c.box_d.p, rl2, c.nbr16.p,
                                                        c.nnb.p, c.nmax, c.npad, c.bcap, c.bcentral, c.bgeom.nh_max,
                                                        c.dscalar.p + 4, c.brick_rows.p, c.dscalar.p,
-                                                       sorted ? c.bins8.p : nullptr, bin_scale, (1 << bits) - 1);
+                                                       sorted ? c.bins8.p : nullptr, bin_scale, (1 << bits) - 1,
+                                                       c.xf_dev());
     XS_HIP(hipGetLastError());
     XS_HIP(hipMemcpyAsync(c.hscalar, c.dscalar.p, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
                           c.stream));
@@ -1460,6 +1479,7 @@ void launch_lj_block(Ctx& c, double eps, double sigma, double rcut, unsigned fla
   a.pos = c.pos.p;
   a.type = c.type.p;
   a.box = c.box_d.p;
+  a.xf = c.xf_dev();
   a.nbr16 = c.nbr16.p;
   a.nnb = c.nnb.p;
   a.npad = c.npad;
@@ -1507,6 +1527,7 @@ void launch_lj_multi_block(Ctx& c, unsigned flags)
   a.pos = c.pos.p;
   a.type = c.type.p;
   a.box = c.box_d.p;
+  a.xf = c.xf_dev();
   a.nbr16 = c.nbr16.p;
   a.nnb = c.nnb.p;
   a.npad = c.npad;

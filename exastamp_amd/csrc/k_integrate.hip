@@ -32,18 +32,24 @@ __global__ void k_force_to_accel(int64_t n, double* ax, double* ay, double* az,
   az[i] /= m;
 }
 
-// x += v*dt + a*dt2 with dt2 = dt*dt/2, evaluated as x + fma(v, dt, a*dt2).
+// x += v*dt + a*dt2 with dt2 = dt*dt/2, evaluated as x + fma(v, dt, a*dt2); with
+// xf (xform_mode INV_XFORM under a non-identity xform) the physical step is mapped
+// to grid space first, x += mi * (v*dt + a*dt2)
+// (push_vec3_2nd_order_xform_minimization.cpp:34).
 __global__ void k_push_f_v_r(int64_t n, double dt, double dt2, Pos4* __restrict__ pos,
                              const double* __restrict__ vx, const double* __restrict__ vy,
                              const double* __restrict__ vz, const double* __restrict__ ax,
-                             const double* __restrict__ ay, const double* __restrict__ az)
+                             const double* __restrict__ ay, const double* __restrict__ az,
+                             const XformDev* __restrict__ xf)
 {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   Pos4 p = load_pos(pos, i);
-  p.x += fma(vx[i], dt, ax[i] * dt2);
-  p.y += fma(vy[i], dt, ay[i] * dt2);
-  p.z += fma(vz[i], dt, az[i] * dt2);
+  double sx = fma(vx[i], dt, ax[i] * dt2), sy = fma(vy[i], dt, ay[i] * dt2), sz = fma(vz[i], dt, az[i] * dt2);
+  if (xf) xsmd_xform_apply(xf->mi, &sx, &sy, &sz);
+  p.x += sx;
+  p.y += sy;
+  p.z += sz;
   store_pos(pos, i, p);
 }
 
@@ -60,12 +66,14 @@ __global__ void k_push_f_v(int64_t n, double dt, double* vx, double* vy, double*
 
 // push_f_v_r(dt) then push_f_v(dt/2), plus the largest squared displacement
 // since the last list build (as bits of a non-negative double, which order
-// like the doubles themselves): reduced per block, one atomic per block.
+// like the doubles themselves): reduced per block, one atomic per block.  Under a
+// non-identity xform the step is mapped by mi (INV_XFORM) and the displacement is
+// measured in physical space.
 __global__ void __launch_bounds__(256) k_drift_kick(
     int64_t n, double dt, double dt2, double hdt, Pos4* __restrict__ pos, double* vx, double* vy,
     double* vz, const double* __restrict__ ax, const double* __restrict__ ay,
     const double* __restrict__ az, const double* __restrict__ rrx, const double* __restrict__ rry,
-    const double* __restrict__ rrz, unsigned long long* __restrict__ maxd2)
+    const double* __restrict__ rrz, unsigned long long* __restrict__ maxd2, const XformDev* __restrict__ xf)
 {
   double d2 = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
@@ -73,14 +81,18 @@ __global__ void __launch_bounds__(256) k_drift_kick(
     const double a0 = ax[i], a1 = ay[i], a2 = az[i];
     const double v0 = vx[i], v1 = vy[i], v2 = vz[i];
     Pos4 p = load_pos(pos, i);
-    p.x += fma(v0, dt, a0 * dt2);
-    p.y += fma(v1, dt, a1 * dt2);
-    p.z += fma(v2, dt, a2 * dt2);
+    double sx = fma(v0, dt, a0 * dt2), sy = fma(v1, dt, a1 * dt2), sz = fma(v2, dt, a2 * dt2);
+    if (xf) xsmd_xform_apply(xf->mi, &sx, &sy, &sz);
+    p.x += sx;
+    p.y += sy;
+    p.z += sz;
     store_pos(pos, i, p);
     vx[i] = fma(a0, hdt, v0);
     vy[i] = fma(a1, hdt, v1);
     vz[i] = fma(a2, hdt, v2);
-    d2 = fmax(d2, xsmd_d2(p.x - rrx[i], p.y - rry[i], p.z - rrz[i]));
+    double ux = p.x - rrx[i], uy = p.y - rry[i], uz = p.z - rrz[i];
+    if (xf) xsmd_xform_apply(xf->m, &ux, &uy, &uz);
+    d2 = fmax(d2, xsmd_d2(ux, uy, uz));
   }
   for (int off = 32; off > 0; off >>= 1) d2 = fmax(d2, __shfl_xor(d2, off));
   __shared__ double sh[kBlock / 64];
@@ -210,13 +222,14 @@ void launch_force_to_accel(Ctx& c)
   XS_HIP(hipGetLastError());
 }
 
-void launch_push_f_v_r(Ctx& c, double dt)
+void launch_push_f_v_r(Ctx& c, double dt, bool inv_xform)
 {
   if (c.n == 0) return;
   c.rho_ckpt = false;
   const double dt2 = dt * dt * 0.5;
   k_push_f_v_r<<<grid_for(c.n), kBlock, 0, c.stream>>>(c.n, dt, dt2, c.pos.p, c.vx.p, c.vy.p, c.vz.p,
-                                                       c.ax.p, c.ay.p, c.az.p);
+                                                       c.ax.p, c.ay.p, c.az.p,
+                                                       inv_xform ? c.xf_dev() : nullptr);
   XS_HIP(hipGetLastError());
 }
 
@@ -236,7 +249,7 @@ void launch_drift_kick(Ctx& c, double dt)
   ProfScope prof(c.prof, c.stream, XS_PROF_DRIFT);
   const unsigned g = std::min<unsigned>(grid_for(c.n), 4096);
   k_drift_kick<<<g, kBlock, 0, c.stream>>>(c.n, dt, dt2, dt * 0.5, c.pos.p, c.vx.p, c.vy.p, c.vz.p, c.ax.p,
-                                           c.ay.p, c.az.p, c.rrx.p, c.rry.p, c.rrz.p, c.dscalar.p + 1);
+                                           c.ay.p, c.az.p, c.rrx.p, c.rry.p, c.rrz.p, c.dscalar.p + 1, c.xf_dev());
   XS_HIP(hipGetLastError());
 }
 

@@ -87,7 +87,7 @@ __global__ void __launch_bounds__(256) k_build(
     const BoxDev* __restrict__ box, int px, int py, int pz, int nx, int ny, int nz,
     int64_t ncell_rep, const int* __restrict__ cell_off, double rl2,
     uint32_t* __restrict__ nbr, int* __restrict__ nnb, int nmax, int64_t npad,
-    unsigned long long* __restrict__ maxcnt)
+    unsigned long long* __restrict__ maxcnt, const XformDev* __restrict__ xf)
 {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -118,9 +118,10 @@ __global__ void __launch_bounds__(256) k_build(
         for (int j = j0; j < j1; j++) {
           if (self_img && j == (int)i) continue;
           const Pos4 pj = load_pos(pos, j);
-          const double ddx = xsmd_image(pj.x, kx, b.L[0]) - pi.x;
-          const double ddy = xsmd_image(pj.y, ky, b.L[1]) - pi.y;
-          const double ddz = xsmd_image(pj.z, kz, b.L[2]) - pi.z;
+          double ddx = xsmd_image(pj.x, kx, b.L[0]) - pi.x;
+          double ddy = xsmd_image(pj.y, ky, b.L[1]) - pi.y;
+          double ddz = xsmd_image(pj.z, kz, b.L[2]) - pi.z;
+          if (xf) xsmd_xform_apply(xf->m, &ddx, &ddy, &ddz);
           if (xsmd_d2(ddx, ddy, ddz) < rl2) {
             if (cnt < nmax) nbr[(int64_t)cnt * npad + i] = xsmd_nbr_encode((uint32_t)j, kx, ky, kz);
             cnt++;
@@ -151,13 +152,16 @@ __global__ void __launch_bounds__(256) k_max_displ2(int64_t n, const Pos4* __res
                                                     const double* __restrict__ rrx,
                                                     const double* __restrict__ rry,
                                                     const double* __restrict__ rrz,
-                                                    unsigned long long* __restrict__ out)
+                                                    unsigned long long* __restrict__ out,
+                                                    const XformDev* __restrict__ xf)
 {
   double m = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
     const Pos4 p = load_pos(pos, i);
-    m = fmax(m, xsmd_d2(p.x - rrx[i], p.y - rry[i], p.z - rrz[i]));
+    double ux = p.x - rrx[i], uy = p.y - rry[i], uz = p.z - rrz[i];
+    if (xf) xsmd_xform_apply(xf->m, &ux, &uy, &uz);  // the threshold is a physical distance
+    m = fmax(m, xsmd_d2(ux, uy, uz));
   }
   block_max_to(m, out);
 }
@@ -165,7 +169,7 @@ __global__ void __launch_bounds__(256) k_max_displ2(int64_t n, const Pos4* __res
 __global__ void k_pair_count(int64_t n, int64_t npad, const Pos4* __restrict__ pos,
                              const uint16_t* __restrict__ rep, const BoxDev* __restrict__ box,
                              const uint32_t* __restrict__ nbr, const int* __restrict__ nnb, double rc2,
-                             unsigned long long* __restrict__ out)
+                             unsigned long long* __restrict__ out, const XformDev* __restrict__ xf)
 {
   unsigned long long e = 0, w = 0;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
@@ -177,9 +181,10 @@ __global__ void k_pair_count(int64_t n, int64_t npad, const Pos4* __restrict__ p
     for (int k = 0; k < cnt; k++) {
       const uint32_t en = nbr[(int64_t)k * npad + i];
       const Pos4 pj = load_pos(pos, xsmd_nbr_index(en));
-      const double dx = xsmd_image(pj.x, xsmd_nbr_img_x(en), L[0]) - pi.x;
-      const double dy = xsmd_image(pj.y, xsmd_nbr_img_y(en), L[1]) - pi.y;
-      const double dz = xsmd_image(pj.z, xsmd_nbr_img_z(en), L[2]) - pi.z;
+      double dx = xsmd_image(pj.x, xsmd_nbr_img_x(en), L[0]) - pi.x;
+      double dy = xsmd_image(pj.y, xsmd_nbr_img_y(en), L[1]) - pi.y;
+      double dz = xsmd_image(pj.z, xsmd_nbr_img_z(en), L[2]) - pi.z;
+      if (xf) xsmd_xform_apply(xf->m, &dx, &dy, &dz);
       if (xsmd_d2(dx, dy, dz) <= rc2) w++;
     }
   }
@@ -212,10 +217,12 @@ void build_neighbor_list(Ctx& c, double rcut, double skin, bool prefer_block)
   if (!(rl > 0)) throw Error(XS_E_ARG, "rcut + skin must be positive");
 
   // cell grid: common dims for all replicas (from the smallest box), each
-  // replica with its own n/L so every cell is at least rl wide
+  // replica with its own n/L so every cell is at least rl wide (in grid space,
+  // rl times the xform's reach along the axis)
   for (int a = 0; a < 3; a++) {
+    const double rla = c.xf.on ? rl * xsmd_xform_reach(c.xf.mi, a) : rl;
     int nmin = 1 << 30;
-    for (const BoxDev& b : c.box_h) nmin = std::min(nmin, xsmd_cells_along(b.L[a], rl));
+    for (const BoxDev& b : c.box_h) nmin = std::min(nmin, xsmd_cells_along(b.L[a], rla));
     c.nc[a] = nmin;
   }
   for (BoxDev& b : c.box_h)
@@ -280,7 +287,7 @@ void build_neighbor_list(Ctx& c, double rcut, double skin, bool prefer_block)
   // list capacity: estimate from the density, grow to the measured maximum on overflow
   if (c.nmax == 0) {
     double vol = 0;
-    for (const BoxDev& b : c.box_h) vol += b.L[0] * b.L[1] * b.L[2];
+    for (const BoxDev& b : c.box_h) vol += b.L[0] * b.L[1] * b.L[2] * xform_det(c.xf);
     const double dens = (double)n / vol;
     const double est = dens * 4.18879020478639 * rl * rl * rl;  // 4/3 pi rl^3
     c.nmax = std::max(16, (int)std::ceil(est * 1.25 / 8.0) * 8);
@@ -301,7 +308,7 @@ void build_neighbor_list(Ctx& c, double rcut, double skin, bool prefer_block)
     XS_HIP(hipMemsetAsync(c.dscalar.p, 0, sizeof(unsigned long long), c.stream));
     k_build<<<g, kBlock, 0, c.stream>>>(n, c.pos.p, c.keys2.p, c.box_d.p, c.periodic[0], c.periodic[1],
                                         c.periodic[2], c.nc[0], c.nc[1], c.nc[2], c.ncell_rep, c.cell_off.p, rl2,
-                                        c.nbr.p, c.nnb.p, c.nmax, c.npad, c.dscalar.p);
+                                        c.nbr.p, c.nnb.p, c.nmax, c.npad, c.dscalar.p, c.xf_dev());
     XS_HIP(hipGetLastError());
     XS_HIP(hipMemcpyAsync(c.hscalar, c.dscalar.p, sizeof(unsigned long long), hipMemcpyDeviceToHost,
                           c.stream));
@@ -324,7 +331,7 @@ void pair_count(Ctx& c, double rcut, int64_t& entries, int64_t& within)
   XS_HIP(hipMemsetAsync(c.dscalar.p + 2, 0, 2 * sizeof(unsigned long long), c.stream));
   const unsigned g = std::min<unsigned>(grid_for(c.n), 4096);
   k_pair_count<<<g, kBlock, 0, c.stream>>>(c.n, c.npad, c.pos.p, c.has_rep ? c.rep.p : nullptr, c.box_d.p,
-                                           c.nbr.p, c.nnb.p, rcut * rcut, c.dscalar.p + 2);
+                                           c.nbr.p, c.nnb.p, rcut * rcut, c.dscalar.p + 2, c.xf_dev());
   XS_HIP(hipGetLastError());
   XS_HIP(hipMemcpyAsync(c.hscalar + 2, c.dscalar.p + 2, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
                         c.stream));
@@ -338,7 +345,7 @@ double max_displacement2(Ctx& c)
   if (c.n == 0) return 0.0;
   XS_HIP(hipMemsetAsync(c.dscalar.p + 1, 0, sizeof(unsigned long long), c.stream));
   const unsigned g = std::min<unsigned>(grid_for(c.n), 2048);
-  k_max_displ2<<<g, kBlock, 0, c.stream>>>(c.n, c.pos.p, c.rrx.p, c.rry.p, c.rrz.p, c.dscalar.p + 1);
+  k_max_displ2<<<g, kBlock, 0, c.stream>>>(c.n, c.pos.p, c.rrx.p, c.rry.p, c.rrz.p, c.dscalar.p + 1, c.xf_dev());
   XS_HIP(hipGetLastError());
   XS_HIP(hipMemcpyAsync(c.hscalar + 1, c.dscalar.p + 1, sizeof(unsigned long long), hipMemcpyDeviceToHost,
                         c.stream));

@@ -31,6 +31,7 @@ struct PairCommon {
   const uint32_t* __restrict__ nbr;
   const int* __restrict__ nnb;
   double rc2;
+  const XformDev* xf;  // displacements m * (r_j(image) - r_i) when not NULL
 };
 
 constexpr int kBatch = 4;
@@ -66,6 +67,7 @@ __device__ __forceinline__ void fetch(const PairCommon& pc, int64_t i, int k0, i
     nb[u].x = xj - pi.x;
     nb[u].y = yj - pi.y;
     nb[u].z = zj - pi.z;
+    if (pc.xf) xsmd_xform_apply(pc.xf->m, &nb[u].x, &nb[u].y, &nb[u].z);
     nb[u].d2 = xsmd_d2(nb[u].x, nb[u].y, nb[u].z);
     nb[u].w = pj[u].w;
     nb[u].j = xsmd_nbr_index(e[u]);
@@ -347,6 +349,7 @@ PairCommon make_common(Ctx& c, double rcut)
   pc.nbr = c.nbr.p;
   pc.nnb = c.nnb.p;
   pc.rc2 = rcut * rcut;
+  pc.xf = c.xf_dev();
   return pc;
 }
 

@@ -91,6 +91,24 @@ struct BoxDev {
   double inv_cs[3];
 };
 
+// Domain xform of a deformed domain (exaNBody's Domain::xform): m maps grid-space
+// displacements to physical ones, mi = m^-1 maps a physical step back to grid space.
+// Positions stay in grid coordinates; forces, velocities and the virial are physical.
+// Kernels take a device pointer to it, NULL for the identity (no transform at all), so
+// the identity form keeps its argument size and code.
+struct XformDev {
+  double m[9];
+  double mi[9];
+  int on;
+};
+
+inline double xform_det(const XformDev& x)
+{
+  const double* m = x.m;
+  return m[0] * (m[4] * m[8] - m[5] * m[7]) - m[1] * (m[3] * m[8] - m[5] * m[6]) +
+         m[2] * (m[3] * m[7] - m[4] * m[6]);
+}
+
 // Spline tables on the device.  Each row of the reference's SplineCoeffs
 // (eam_alloy.h:37-42: c0..c2 derivative quadratic, c3..c6 value cubic) is
 // stored as 8 values reordered to [c3 c4 c5 c6 | c0 c1 c2 0] so the value
@@ -259,6 +277,9 @@ struct Ctx {
   int periodic[3] = {1, 1, 1};
   std::vector<BoxDev> box_h;
   DBuf<BoxDev> box_d;
+  XformDev xf = {{1, 0, 0, 0, 1, 0, 0, 0, 1}, {1, 0, 0, 0, 1, 0, 0, 0, 1}, 0};
+  DBuf<XformDev> xf_d;  // device copy of xf; kernels get xf_dev(): NULL for the identity
+  const XformDev* xf_dev() const { return xf.on ? xf_d.p : nullptr; }
 
   // per-atom state in cell-sorted order (and the permutation targets)
   DBuf<Pos4> pos, pos2;
@@ -385,7 +406,9 @@ void pair_count(Ctx& c, double rcut, int64_t& entries, int64_t& within);
 // ---- launches (k_integrate.hip) ----
 void launch_zero(Ctx& c);
 void launch_force_to_accel(Ctx& c);
-void launch_push_f_v_r(Ctx& c, double dt);
+// inv_xform: xform_mode INV_XFORM (the step is mapped to grid space by the xform's
+// inverse); false: IDENTITY
+void launch_push_f_v_r(Ctx& c, double dt, bool inv_xform);
 void launch_push_f_v(Ctx& c, double dt);
 // fused first half of a step: r += v dt + a dt^2/2 ; v += a dt/2 ; max |r - r_ref|^2
 void launch_drift_kick(Ctx& c, double dt);

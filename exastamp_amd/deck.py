@@ -43,6 +43,7 @@ from __future__ import annotations
 import argparse
 import copy
 import dataclasses
+import math
 import sys
 from pathlib import Path
 
@@ -306,12 +307,13 @@ class ThermoRecord:
     volume: float
     mass: float          # Da
     density: float       # g/cm^3
+    angles: tuple = (90.0, 90.0, 90.0)  # alpha beta gamma, degrees
 
     def line(self) -> str:
         """The row dump_thermodynamic_state writes (dump_thermodynamic_state.cpp:101-121),
-        without the newline; angles are 90 degrees (orthogonal boxes)."""
+        without the newline."""
         return CSV_ROW % (self.step, self.time_ps, self.particles, self.total_ev, self.kinetic_ev,
-                          self.potential_ev, self.temperature_k, *self.p_full, *self.lengths, 90.0, 90.0, 90.0,
+                          self.potential_ev, self.temperature_k, *self.p_full, *self.lengths, *self.angles,
                           self.volume, self.density)
 
     def log_values(self) -> dict:
@@ -322,7 +324,30 @@ class ThermoRecord:
                 "pre": self.pressure_pa, "prx": self.p_vir[0], "pry": self.p_vir[1], "prz": self.p_vir[2],
                 "pxy": self.p_full[3], "pxz": self.p_full[4], "pyz": self.p_full[5], "smi": self.vonmises_pa,
                 "vol": self.volume, "mas": self.mass, "bxa": self.lengths[0], "bxb": self.lengths[1],
-                "bxc": self.lengths[2], "baa": 90.0, "bab": 90.0, "bag": 90.0, "rho": self.density}
+                "bxc": self.lengths[2], "baa": self.angles[0], "bab": self.angles[1], "bag": self.angles[2],
+                "rho": self.density}
+
+
+def cell_geometry(xform, extent):
+    """Box edge lengths A, B, C and angles alpha, beta, gamma (degrees) of the domain
+    as print_thermodynamic_state.cpp:252-262 and dump_thermodynamic_state.cpp:90-99
+    compute them: a, b, c = xform times the grid box edges; the three angles all divide
+    by B*C there, which this keeps so the outputs agree (it only matters for a
+    non-orthogonal xform with B*C != C*A or A*B)."""
+    m = np.asarray(xform, dtype=np.float64)
+    e = np.asarray(extent, dtype=np.float64)
+    a, b, c = m[:, 0] * e[0], m[:, 1] * e[1], m[:, 2] * e[2]
+    A, B, C = (float(math.sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2])) for v in (a, b, c))
+
+    def dot(u, w):
+        return float(u[0] * w[0] + u[1] * w[1] + u[2] * w[2])
+
+    def ang(x):
+        return math.acos(x) / math.acos(-1.0) * 180.0 if -1.0 <= x <= 1.0 else float("nan")
+
+    if B * C == 0.0:
+        return (A, B, C), (90.0, 90.0, 90.0)
+    return (A, B, C), (ang(dot(b, c) / (B * C)), ang(dot(c, a) / (B * C)), ang(dot(a, b) / (B * C)))
 
 
 # the file row and header of dump_thermodynamic_state (dump_thermodynamic_state.cpp:61,101)
@@ -529,7 +554,7 @@ class Runner:
         mises = np.sqrt(0.5 * ((d[0] - d[1]) ** 2 + (d[1] - d[2]) ** 2 + (d[2] - d[0]) ** 2
                                + 6.0 * (o[0] ** 2 + o[1] ** 2 + o[2] ** 2))) / v
         a = self.atoms
-        lengths = tuple(float(x) for x in (a.hi[0] - a.lo[0])) if a is not None else (0.0, 0.0, 0.0)
+        lengths, angles = cell_geometry(self.sim.xform, a.hi[0] - a.lo[0] if a is not None else np.zeros(3))
         rec = ThermoRecord(step=int(self.get("timestep", 0)), time_ps=float(self.get("physical_time", 0.0)),
                            particles=n, total_ev=st.total_energy / EV / n, kinetic_ev=st.kinetic_energy / EV / n,
                            potential_ev=st.potential_energy / EV / n, temperature_k=st.temperature,
@@ -538,7 +563,7 @@ class Runner:
                            p_full=tuple(float(full[i, j]) for i, j in ((0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2))),
                            p_vir=tuple(float(vir[k, k] / v * PRESSURE_UNIT_PA) for k in range(3)),
                            vonmises_pa=float(mises * PRESSURE_UNIT_PA), lengths=lengths, volume=v, mass=st.mass,
-                           density=st.mass / v * G_PER_CM3_PER_DA_PER_A3)
+                           density=st.mass / v * G_PER_CM3_PER_DA_PER_A3, angles=angles)
         self.records.append(rec)
         return rec
 
@@ -588,8 +613,10 @@ def _op_domain(r: Runner, p):
     d = dict(r.domain)
     d.update(p)
     xf = d.get("xform")
-    if xf is not None and not np.allclose(np.asarray(xf, dtype=float), np.eye(3)):
-        raise DeckError("domain: only the identity xform is supported")
+    if xf is not None:
+        m = np.asarray(xf, dtype=float)
+        if m.shape != (3, 3) or not np.all(np.isfinite(m)) or not np.linalg.det(m) > 0.0:
+            raise DeckError("domain: xform must be a 3x3 matrix with a positive determinant")
     r.domain = d
 
 
@@ -662,6 +689,9 @@ def _op_init_particles(r: Runner, p):
     _op_setup_species_check(r)
     r.sim.species(r.species)
     r.sim.set_config(r.atoms)
+    xf = r.domain.get("xform")
+    if xf is not None and not np.array_equal(np.asarray(xf, dtype=float), np.eye(3)):
+        r.sim.set_xform(np.asarray(xf, dtype=float))
     r.init_atoms = copy.deepcopy(r.atoms)
     if r.tables_loaded is not None:
         r.sim.eam_alloy_init(r.tables_loaded)

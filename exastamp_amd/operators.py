@@ -102,6 +102,7 @@ class Simulation:
         self.eam: SetflTables | None = None
         self.eam_knots = 0          # knots of the loaded tables (the length of theta)
         self._types_set = None      # the species mask eam_alloy_force last sent (not re-sent unchanged)
+        self.xform = np.eye(3)      # domain xform (grid -> physical)
 
     # --- set-up -------------------------------------------------------------
     def species(self, spec):
@@ -126,6 +127,19 @@ class Simulation:
         self.n = cfg.n
         self.cfg = cfg
         self._keep = cols  # keep host buffers alive until the copy completed
+
+    def set_xform(self, m=None):
+        """Domain xform (exaNBody Domain::set_xform): the 3x3 map from grid coordinates
+        (the positions of set_config and state) to physical ones; None = identity.
+        Pair displacements, forces, velocities and the virial are physical; the
+        thermodynamic volume is det(xform) times the grid box volume.  The neighbour
+        list must be rebuilt (chunk_neighbors) before the next force operator."""
+        a = np.eye(3) if m is None else np.asarray(m, dtype=np.float64).reshape(3, 3)
+        if not np.all(np.isfinite(a)) or not np.linalg.det(a) > 0.0:
+            raise ValueError("domain xform: det must be positive")
+        flat = np.ascontiguousarray(a.ravel())
+        self.ctx.xs_domain_xform(_lib.dptr(flat))
+        self.xform = a.copy()
 
     # --- neighbour list -------------------------------------------------------
     def chunk_neighbors(self, rcut_max=None, rcut_inc=0.0, config=None):
@@ -367,9 +381,18 @@ class Simulation:
         return np.bincount(rep, weights=ep, minlength=self.cfg.nrep)
 
     def push_f_v_r(self, dt, dt_scale=1.0, xform_mode="INV_XFORM"):
-        self.ctx.xs_push_f_v_r(parse_quantity(dt), float(dt_scale))
+        """push_f_v_r{dt_scale, xform_mode}: INV_XFORM maps the physical step into grid
+        space by the inverse xform, IDENTITY adds it as is."""
+        modes = {"INV_XFORM": _lib.XS_XFORM_INV, "IDENTITY": _lib.XS_XFORM_IDENTITY}
+        if xform_mode not in modes:
+            raise ValueError(f"push_f_v_r: xform_mode {xform_mode!r} (INV_XFORM or IDENTITY)")
+        self.ctx.xs_push_f_v_r(parse_quantity(dt), float(dt_scale), modes[xform_mode])
 
     def push_f_v(self, dt, dt_scale=1.0, xform_mode="IDENTITY"):
+        """push_f_v{dt_scale, xform_mode}: v += a dt in physical space (the scheme's
+        IDENTITY mode; a mapped velocity kick is not part of this path)."""
+        if xform_mode != "IDENTITY" and not np.array_equal(self.xform, np.eye(3)):
+            raise ValueError("push_f_v: only xform_mode IDENTITY under a non-identity xform")
         self.ctx.xs_push_f_v(parse_quantity(dt), float(dt_scale))
 
     def simulation_thermodynamic_state(self) -> ThermodynamicState:
@@ -378,8 +401,11 @@ class Simulation:
         return ThermodynamicState(out, self.volume())
 
     def volume(self) -> float:
+        """det(xform) times the grid box volume (simulation_thermodynamic_state.cpp:202-211)."""
         c = self.cfg
-        return float(np.prod(c.hi - c.lo, axis=1).sum()) if c is not None else 0.0
+        if c is None:
+            return 0.0
+        return float(np.prod(c.hi - c.lo, axis=1).sum()) * float(np.linalg.det(self.xform))
 
     # --- fused NVE loop -------------------------------------------------------
     def md_run(self, potential, nsteps, dt, rcut, skin, mixed=False, thermo_every=0):

@@ -185,16 +185,19 @@ namespace exaStamp
     }
   };
 
-  // push_f_v_r / push_f_v (config_numerical_schemes.msp:23-28): dt * dt_scale
+  // push_f_v_r / push_f_v (config_numerical_schemes.msp:23-28): dt * dt_scale;
+  // push_f_v_r's xform_mode INV_XFORM maps the step by the inverse domain xform
   class XsmdPushFVR : public OperatorNode
   {
-    ADD_SLOT( double , dt       , INPUT , REQUIRED );
-    ADD_SLOT( double , dt_scale , INPUT , 1.0 );
+    ADD_SLOT( double      , dt         , INPUT , REQUIRED );
+    ADD_SLOT( double      , dt_scale   , INPUT , 1.0 );
+    ADD_SLOT( std::string , xform_mode , INPUT , std::string("INV_XFORM") );
   public:
     inline void execute() override final
     {
       XsmdState& st = XsmdState::instance();
-      if( st.on_gpu ) st.check( xs_push_f_v_r( st.ctx, *dt, *dt_scale ), "xs_push_f_v_r" );
+      const int mode = ( *xform_mode == "IDENTITY" ) ? XS_XFORM_IDENTITY : XS_XFORM_INV;
+      if( st.on_gpu ) st.check( xs_push_f_v_r( st.ctx, *dt, *dt_scale, mode ), "xs_push_f_v_r" );
     }
   };
   class XsmdPushFV : public OperatorNode

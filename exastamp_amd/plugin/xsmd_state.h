@@ -86,6 +86,10 @@ namespace exaStamp
     const int periodic[3] = { domain.periodic_boundary_x(), domain.periodic_boundary_y(), domain.periodic_boundary_z() };
     st.check( xs_system_set( st.context(), st.n, rx.data(), ry.data(), rz.data(), vx.data(), vy.data(), vz.data(),
                              type.data(), nullptr, 1, lo, hi, periodic ), "xs_system_set" );
+    // grid coordinates in, physical displacements through the domain's xform
+    const auto X = domain.xform();
+    const double m[9] = { X.m11, X.m12, X.m13, X.m21, X.m22, X.m23, X.m31, X.m32, X.m33 };
+    st.check( xs_domain_xform( st.context(), domain.xform_is_identity() ? nullptr : m ), "xs_domain_xform" );
     st.on_gpu = true;
   }
 

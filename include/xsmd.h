@@ -110,6 +110,17 @@ int64_t xs_system_size(const xs_ctx* ctx);
 xs_status xs_system_get(xs_ctx* ctx, double* rx, double* ry, double* rz, double* vx,
                         double* vy, double* vz);
 xs_status xs_field_get(xs_ctx* ctx, int field, double* out);
+/* Domain xform of a deformed domain (exaNBody's Domain::xform / set_xform; read by
+ * the pair operators at pair_potential_impl.hxx:388-413, which switch to the
+ * LinearXForm traversal when it is not the identity, by the thermodynamic state for
+ * the volume at simulation_thermodynamic_state.cpp:203-211, and by push_f_v_r's
+ * INV_XFORM mode, push_vec3_2nd_order_xform_minimization.cpp:34).  m is row-major
+ * 3x3 and maps grid coordinates (the positions given to xs_system_set and returned by
+ * xs_system_get) to physical ones; NULL or the identity restores the plain form.
+ * Pair displacements, forces, velocities and virials are physical.  det(m) must be
+ * positive (XS_E_ARG otherwise).  The list becomes invalid: call xs_chunk_neighbors
+ * before the next force operator.  Applies to every replica. */
+xs_status xs_domain_xform(xs_ctx* ctx, const double m[9]);
 /* Overwrite velocities (id order), e.g. for time-reversal tests. */
 xs_status xs_velocity_set(xs_ctx* ctx, const double* vx, const double* vy, const double* vz);
 
@@ -210,8 +221,13 @@ xs_status xs_eam_table_info(xs_ctx* ctx, int* s_pieces, double* r_lo, double err
 xs_status xs_johnson_force(xs_ctx* ctx, const double params[19], double rcut, int johnson_emb,
                            int johnson_force, unsigned flags);
 /* push_f_v_r (config_numerical_schemes.msp:23-25): r += v*dt + a*dt^2/2,
- * dt = dt * dt_scale. */
-xs_status xs_push_f_v_r(xs_ctx* ctx, double dt, double dt_scale);
+ * dt = dt * dt_scale; xform_mode XS_XFORM_INV (the scheme's INV_XFORM) maps the
+ * physical step into grid space by the inverse domain xform first
+ * (push_vec3_2nd_order_xform_minimization.cpp:34), XS_XFORM_IDENTITY adds it as is;
+ * the two agree when the xform is the identity. */
+#define XS_XFORM_IDENTITY 0
+#define XS_XFORM_INV 1
+xs_status xs_push_f_v_r(xs_ctx* ctx, double dt, double dt_scale, int xform_mode);
 /* push_f_v (config_numerical_schemes.msp:24-28): v += a*dt, dt = dt * dt_scale. */
 xs_status xs_push_f_v(xs_ctx* ctx, double dt, double dt_scale);
 /* simulation_thermodynamic_state reduction (src/thermo_state/

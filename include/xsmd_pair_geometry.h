@@ -81,6 +81,29 @@ XSMD_HD double xsmd_cell_gap(double x, double lo, double L, int n, int c, int pe
   return g;
 }
 
+/* Deformed domain.  Positions live in grid space; the domain's xform m (row-major
+ * 3x3, exaNBody's Domain::xform) maps a grid-space displacement d to the physical one,
+ * m*d, as LinearXForm does for every pair (pair_potential_impl.hxx:388-401 selects the
+ * LinearXForm traversal when the xform is not the identity).  Fixed operation order,
+ * so the GPU list build and the oracle agree bit for bit. */
+XSMD_HD void xsmd_xform_apply(const double* m, double* x, double* y, double* z)
+{
+  const double a = *x, b = *y, c = *z;
+  *x = fma(m[2], c, fma(m[1], b, m[0] * a));
+  *y = fma(m[5], c, fma(m[4], b, m[3] * a));
+  *z = fma(m[8], c, fma(m[7], b, m[6] * a));
+}
+
+/* Grid-space reach along axis a of a physical distance of 1 under xform m, given
+ * mi = m^-1: a physical displacement d has grid component (mi d)_a, at most
+ * |row a of mi| * |d| in magnitude.  Cells of grid width >= rl * reach therefore
+ * hold every physical neighbour within rl in the 27-cell stencil. */
+XSMD_HD double xsmd_xform_reach(const double* mi, int a)
+{
+  const double* r = mi + 3 * a;
+  return sqrt(fma(r[2], r[2], fma(r[1], r[1], r[0] * r[0])));
+}
+
 /* Number of cells along an axis of length L for a minimum cell size cs. */
 XSMD_HD int xsmd_cells_along(double L, double cs)
 {

@@ -222,6 +222,10 @@ struct or_ctx {
   double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0}, L[3] = {0, 0, 0};
   int periodic[3] = {1, 1, 1};
   std::vector<double> mass;
+  // Domain xform (grid -> physical, row-major) and its inverse; positions are grid
+  // coordinates, pair displacements, forces and velocities physical.
+  bool xf_on = false;
+  double xf[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1}, xfi[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
 
   bool has_lj = false;
   double lj_eps = 0, lj_sigma = 0, lj_rcut = 0, lj_ecut = 0;
@@ -256,6 +260,7 @@ struct or_ctx {
     d[0] = xsmd_image(x[j], xsmd_nbr_img_x(e), L[0]) - x[i];
     d[1] = xsmd_image(y[j], xsmd_nbr_img_y(e), L[1]) - y[i];
     d[2] = xsmd_image(z[j], xsmd_nbr_img_z(e), L[2]) - z[i];
+    if (xf_on) xsmd_xform_apply(xf, &d[0], &d[1], &d[2]);
   }
 };
 
@@ -503,7 +508,7 @@ int or_build_neighbors(or_ctx* c, double rcut, double skin)
   const int64_t n = c->n;
   const double rl = rcut + skin;
   for (int a = 0; a < 3; a++) {
-    c->nc[a] = xsmd_cells_along(c->L[a], rl);
+    c->nc[a] = xsmd_cells_along(c->L[a], c->xf_on ? rl * xsmd_xform_reach(c->xfi, a) : rl);
     c->inv_cs[a] = c->nc[a] / c->L[a];
   }
   for (int64_t i = 0; i < n; i++) {
@@ -557,9 +562,10 @@ int or_build_neighbors(or_ctx* c, double rcut, double skin)
     stencil(*c, cx, cy, cz, [&](int64_t cb, int kx, int ky, int kz) {
       for (int64_t j = c->cell_start[cb]; j < c->cell_start[cb + 1]; j++) {
         if (j == i && kx == 0 && ky == 0 && kz == 0) continue;
-        const double dx = xsmd_image(c->x[j], kx, c->L[0]) - c->x[i];
-        const double dy = xsmd_image(c->y[j], ky, c->L[1]) - c->y[i];
-        const double dz = xsmd_image(c->z[j], kz, c->L[2]) - c->z[i];
+        double dx = xsmd_image(c->x[j], kx, c->L[0]) - c->x[i];
+        double dy = xsmd_image(c->y[j], ky, c->L[1]) - c->y[i];
+        double dz = xsmd_image(c->z[j], kz, c->L[2]) - c->z[i];
+        if (c->xf_on) xsmd_xform_apply(c->xf, &dx, &dy, &dz);
         if (xsmd_d2(dx, dy, dz) < rl2) emit(xsmd_nbr_encode((uint32_t)j, kx, ky, kz));
       }
     });
@@ -904,16 +910,51 @@ int or_compute_forces(or_ctx* c, int potential, int flags)
   return 0;
 }
 
-// push_f_v_r: x += v*dt + a*dt^2/2 (src/numerical_schemes/push_vec3_2nd_order_xform_minimization.cpp:34,
-// identity xform; dt already multiplied by dt_scale).
+// push_f_v_r: x += v*dt + a*dt^2/2 (dt already multiplied by dt_scale); with
+// xform_mode INV_XFORM and a non-identity xform the step is mapped back to grid
+// space first, x += inv_xform * (v*dt + a*dt^2/2)
+// (src/numerical_schemes/push_vec3_2nd_order_xform_minimization.cpp:34).
 int or_push_f_v_r(or_ctx* c, double dt)
 {
+  return or_push_f_v_r_mode(c, dt, 1);
+}
+
+int or_push_f_v_r_mode(or_ctx* c, double dt, int inv_xform)
+{
   const double dt2 = dt * dt * 0.5;
+  const bool map = inv_xform && c->xf_on;
   for_each_cell(*c, [&](int64_t i) {
-    c->x[i] += std::fma(c->vx[i], dt, c->ax[i] * dt2);
-    c->y[i] += std::fma(c->vy[i], dt, c->ay[i] * dt2);
-    c->z[i] += std::fma(c->vz[i], dt, c->az[i] * dt2);
+    double sx = std::fma(c->vx[i], dt, c->ax[i] * dt2);
+    double sy = std::fma(c->vy[i], dt, c->ay[i] * dt2);
+    double sz = std::fma(c->vz[i], dt, c->az[i] * dt2);
+    if (map) xsmd_xform_apply(c->xfi, &sx, &sy, &sz);
+    c->x[i] += sx;
+    c->y[i] += sy;
+    c->z[i] += sz;
   });
+  return 0;
+}
+
+// Domain xform: m (row-major) maps grid to physical coordinates; NULL or the
+// identity selects the plain traversal.  Returns -1 for a singular or reflecting m.
+int or_set_xform(or_ctx* c, const double* m)
+{
+  static const double id[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+  if (!m) m = id;
+  const double det = m[0] * (m[4] * m[8] - m[5] * m[7]) - m[1] * (m[3] * m[8] - m[5] * m[6]) +
+                     m[2] * (m[3] * m[7] - m[4] * m[6]);
+  if (!(det > 0.0)) return -1;
+  const double inv[9] = {(m[4] * m[8] - m[5] * m[7]) / det, (m[2] * m[7] - m[1] * m[8]) / det,
+                         (m[1] * m[5] - m[2] * m[4]) / det, (m[5] * m[6] - m[3] * m[8]) / det,
+                         (m[0] * m[8] - m[2] * m[6]) / det, (m[2] * m[3] - m[0] * m[5]) / det,
+                         (m[3] * m[7] - m[4] * m[6]) / det, (m[1] * m[6] - m[0] * m[7]) / det,
+                         (m[0] * m[4] - m[1] * m[3]) / det};
+  bool ident = true;
+  for (int q = 0; q < 9; q++) ident = ident && m[q] == id[q];
+  c->xf_on = !ident;
+  std::copy(m, m + 9, c->xf);
+  std::copy(inv, inv + 9, c->xfi);
+  c->list_valid = false;
   return 0;
 }
 
@@ -936,8 +977,9 @@ int or_displ_over(or_ctx* c, double threshold)
   int over = 0;
 #pragma omp parallel for reduction(| : over) num_threads(c->threads())
   for (int64_t i = 0; i < c->n; i++) {
-    const double d2 = xsmd_d2(c->x[i] - c->xr[i], c->y[i] - c->yr[i], c->z[i] - c->zr[i]);
-    if (d2 > t2) over = 1;
+    double dx = c->x[i] - c->xr[i], dy = c->y[i] - c->yr[i], dz = c->z[i] - c->zr[i];
+    if (c->xf_on) xsmd_xform_apply(c->xf, &dx, &dy, &dz);  // the threshold is a physical distance
+    if (xsmd_d2(dx, dy, dz) > t2) over = 1;
   }
   return over;
 }
@@ -1084,9 +1126,10 @@ void brute(const or_ctx* c, double rlist, F&& f)
         for (int ky = -c->periodic[1]; ky <= c->periodic[1]; ky++)
           for (int kx = -c->periodic[0]; kx <= c->periodic[0]; kx++) {
             if (i == j && kx == 0 && ky == 0 && kz == 0) continue;
-            const double dx = xsmd_image(c->x[j], kx, c->L[0]) - c->x[i];
-            const double dy = xsmd_image(c->y[j], ky, c->L[1]) - c->y[i];
-            const double dz = xsmd_image(c->z[j], kz, c->L[2]) - c->z[i];
+            double dx = xsmd_image(c->x[j], kx, c->L[0]) - c->x[i];
+            double dy = xsmd_image(c->y[j], ky, c->L[1]) - c->y[i];
+            double dz = xsmd_image(c->z[j], kz, c->L[2]) - c->z[i];
+            if (c->xf_on) xsmd_xform_apply(c->xf, &dx, &dy, &dz);
             if (xsmd_d2(dx, dy, dz) < rl2) f(i, j, kx, ky, kz);
           }
 }

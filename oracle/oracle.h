@@ -81,6 +81,12 @@ int or_eam_pass(or_ctx* c, int eam_rho, int eam_rho2emb, int eam_force, int flag
 int or_zero_force_energy(or_ctx* c);
 int or_force_to_accel(or_ctx* c);
 int or_push_f_v_r(or_ctx* c, double dt);
+/* push_f_v_r with its xform_mode: inv_xform = 1 (INV_XFORM) maps the step by the
+ * inverse xform, 0 (IDENTITY) adds it as is. */
+int or_push_f_v_r_mode(or_ctx* c, double dt, int inv_xform);
+/* Domain xform (row-major grid -> physical map; NULL = identity).  Invalidates the
+ * list.  -1 when det(m) <= 0. */
+int or_set_xform(or_ctx* c, const double* m);
 int or_push_f_v(or_ctx* c, double dt);
 int or_displ_over(or_ctx* c, double threshold);
 int or_thermo(or_ctx* c, double out[27]);

@@ -51,6 +51,8 @@ def load():
             "or_zero_force_energy": (C.c_int, [C.c_void_p]),
             "or_force_to_accel": (C.c_int, [C.c_void_p]),
             "or_push_f_v_r": (C.c_int, [C.c_void_p, C.c_double]),
+            "or_push_f_v_r_mode": (C.c_int, [C.c_void_p, C.c_double, C.c_int]),
+            "or_set_xform": (C.c_int, [C.c_void_p, _D]),
             "or_push_f_v": (C.c_int, [C.c_void_p, C.c_double]),
             "or_displ_over": (C.c_int, [C.c_void_p, C.c_double]),
             "or_thermo": (C.c_int, [C.c_void_p, _D]),
@@ -174,8 +176,15 @@ class Oracle:
     def force_to_accel(self):
         self.lib.or_force_to_accel(self.h)
 
-    def push_f_v_r(self, dt):
-        self.lib.or_push_f_v_r(self.h, dt)
+    def push_f_v_r(self, dt, xform_mode="INV_XFORM"):
+        if xform_mode not in ("INV_XFORM", "IDENTITY"):
+            raise ValueError(f"push_f_v_r: xform_mode {xform_mode}")
+        self.lib.or_push_f_v_r_mode(self.h, dt, int(xform_mode == "INV_XFORM"))
+
+    def set_xform(self, m=None):
+        """Domain xform: 3x3 grid -> physical map (None = identity); invalidates the list."""
+        a = None if m is None else np.ascontiguousarray(np.asarray(m, dtype=np.float64).reshape(9))
+        self._ok(self.lib.or_set_xform(self.h, None if a is None else _d(a)), "set_xform")
 
     def push_f_v(self, dt):
         self.lib.or_push_f_v(self.h, dt)

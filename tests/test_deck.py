@@ -103,8 +103,10 @@ class _StubSim:
     over_script = ()
 
     def __init__(self, device=0):
+        import numpy as np
         self.calls = []
         self.n = 0
+        self.xform = np.eye(3)
         self._over = list(type(self).over_script)
 
     def _log(self, name):

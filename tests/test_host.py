@@ -110,3 +110,30 @@ def test_fit_loss_is_the_elementwise_loss():
     (3.0 * L2).backward()
     assert abs(float(L.detach()) - float(L2.detach())) <= 1e-15 * float(L2.detach())
     assert torch.allclose(gF, F.grad, rtol=1e-14, atol=0) and torch.allclose(gE, E.grad, rtol=1e-14, atol=1e-30)
+
+
+def test_thermo_cell_geometry():
+    """Box edges and angles of the thermo outputs: xform times the grid box edges
+    (print_thermodynamic_state.cpp:252-262); a shear in the b-c plane tilts alpha."""
+    import math
+    from exastamp_amd.deck import cell_geometry
+    lengths, angles = cell_geometry(np.eye(3), [3.0, 4.0, 5.0])
+    assert lengths == (3.0, 4.0, 5.0) and angles == (90.0, 90.0, 90.0)
+    t = 0.3
+    lengths, angles = cell_geometry([[1, 0, 0], [0, 1, t], [0, 0, 1]], [3.0, 4.0, 5.0])
+    assert lengths[2] == pytest.approx(5.0 * math.hypot(1.0, t), rel=1e-15)
+    assert angles[0] == pytest.approx(math.degrees(math.atan2(1.0, t)), rel=1e-12)
+    assert angles[1] == pytest.approx(90.0) and angles[2] == pytest.approx(90.0)
+
+
+def test_deck_domain_accepts_a_deforming_xform():
+    from exastamp_amd.deck import DeckError, _op_domain
+
+    class R:
+        domain = {}
+
+    r = R()
+    _op_domain(r, {"xform": [[1.0, 0.1, 0.0], [0.0, 1.0, 0.0], [0.0, 0.0, 1.0]]})
+    assert np.allclose(r.domain["xform"][0], [1.0, 0.1, 0.0])
+    with pytest.raises(DeckError):
+        _op_domain(r, {"xform": [[1.0, 0.0, 0.0], [0.0, -1.0, 0.0], [0.0, 0.0, 1.0]]})
