@@ -190,3 +190,37 @@ def test_lj_plot_deck(tmp_path, monkeypatch):
     assert np.allclose(rows[:, 1], e, rtol=1e-5, atol=1e-9)
     assert np.abs(rows[r > 3.0, 4]).max() < 1e-4
     assert abs(r[np.argmin(rows[:, 1])] - 2 ** (1 / 6) * sig) < 8.0 / 1000
+
+
+def test_lj_deck_with_a_sheared_domain_matches_oracle(oracle_mod):
+    """The LJ deck with its domain's xform set to a shear: the runner hands the xform to
+    the library, the 10-step run follows the oracle's sheared NVE run (1e-9), and the
+    thermo record carries the sheared box (volume det(m) V, edge b of length
+    40 A * |column 1 of m|)."""
+    import math
+
+    import numpy as np
+
+    from exastamp_amd import deck, units
+    m = [[1.0, 0.08, 0.0], [0.0, 1.0, 0.05], [0.0, 0.0, 1.02]]
+    cfg = deck.load_deck(DECKS / "lj" / "single_specy_nosym.msp", [GOLDEN],
+                         [(["global", "max_iteration"], "10"),
+                          (["global", "simulation_thermostate_screen_frequency"], "10")])
+    doms = [item["domain"] for item in cfg["setup_system"] if isinstance(item, dict) and "domain" in item]
+    assert len(doms) == 1
+    doms[0]["xform"] = m
+    r = deck.Runner(cfg, out=open("/dev/null", "w"))
+    recs = r.run()
+    assert [x.step for x in recs] == [0, 10]
+    o = oracle_mod.Oracle()
+    o.set_config(r.init_atoms, [mm for _, mm in r.species])
+    o.set_xform(m)
+    o.set_lj(0.0104 * units.EV, 3.4, 8.0)
+    _, th = o.md_run("lj", 10, 0.001, 8.0, 1.0)
+    for rec, row in ((recs[0], th[0]), (recs[1], th[10])):
+        pe = rec.potential_ev * units.EV * rec.particles
+        ke = rec.kinetic_ev * units.EV * rec.particles
+        assert _close(pe, row[24], 1e-9)
+        assert abs(ke - row[21:24].sum()) <= 1e-9 * abs(row[21:24].sum()) + 1e-9 * abs(row[24])
+    assert recs[0].volume == pytest.approx(40.0 ** 3 * np.linalg.det(m), rel=1e-12)
+    assert recs[0].lengths[1] == pytest.approx(40.0 * math.hypot(0.08, 1.0), rel=1e-12)
