@@ -77,9 +77,10 @@ __device__ __forceinline__ bool wrap_axis(int c, int n, int periodic, int& w, in
 }
 
 // Staged halo records in LDS as arrays of doubles x[], y[], z[] and, for the
-// passes that read it, w[]: a random 8-byte LDS read costs a wave ~16 CU cycles
-// against ~30 for a 16-byte read, so three or four 8-byte reads (~23, ~31) beat the
-// two 16-byte reads of {x, y}, {z, w} (~66).  Passes without w alias w to x.
+// passes that read it, w[]: random 16-byte reads conflict on the LDS banks more than
+// 8-byte ones, and the passes measured faster with three or four 8-byte reads per
+// neighbour than with the two 16-byte reads of {x, y}, {z, w} records.  Passes
+// without w alias w to x.
 struct LRec {
   double* x;
   double* y;
