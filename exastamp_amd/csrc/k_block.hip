@@ -1148,7 +1148,8 @@ __global__ void __launch_bounds__(kPassThreads, kPassWaves<VIRIAL>) k_block_lj(B
 bool drift_setup(Ctx& c, const ForceEpilogue& epi, unsigned flags, pm::Drift& dr)
 {
   // under a non-identity xform the loop's own drift kernel (which maps the step by the
-  // inverse xform) runs instead, so this epilogue stays the identity form
+  // inverse xform) runs instead: the epilogue drifts from the staged record, which is
+  // then a physical position, and it keeps the identity form's code
   if (!(epi.fused && epi.drift) || (flags & (XS_ENERGY | XS_VIRIAL)) || c.xf_dev()) return false;
   c.pos2.reserve(c.npad);
   dr.pos_out = c.pos2.p;
