@@ -214,3 +214,37 @@ def test_xform_errors_and_stale_list():
     _eam_forces(sim, tab)
     with pytest.raises(ValueError):
         sim.push_f_v_r(0.001, xform_mode="XFORM")
+
+
+def test_rotated_replica_batch_rotates_every_replica():
+    """The xform applies to every replica of a batch (strained boxes of the adjoint
+    batches): each replica's forces rotate and its energy is unchanged."""
+    from exastamp_amd import inputs
+    tab = cu_tables(500)
+    cfg = inputs.strained_batch(5, (4, 4, 4), 3.615, [CU_MASS], seed=13)
+    R = _rot([0.5, 0.5, -1.0], 0.7)
+    a, b = _eam_sim(cfg, tab, R), _eam_sim(cfg, tab, None)
+    _eam_forces(a, tab)
+    _eam_forces(b, tab)
+    assert frel(a.forces(), b.forces() @ R.T) <= 1e-11
+    ea, eb = a.replica_energies(), b.replica_energies()
+    assert np.abs(ea - eb).max() <= 1e-12 * np.abs(eb).max()
+
+
+def test_sheared_mixed_nve_conserves_energy_like_fp64():
+    """The mixed MD loop under shear (block passes on staged physical positions, the
+    drift in its own kernel): its energy drift over 200 steps is within 2x the FP64
+    run's, as without an xform."""
+    tab = cu_tables()
+    cfg = cu_config((8, 8, 8), T=600.0, jitter=0.05, seed=28)
+    drift = {}
+    for mixed in (False, True):
+        sim = make_sim(cfg, [CU_MASS])
+        sim.set_xform(SHEAR)
+        sim.eam_alloy_init(tab)
+        st, th = sim.md_run("eam", 200, 0.001, 4.95, 0.3, mixed=mixed, thermo_every=10)
+        assert st.rebuilds >= 3
+        e = np.array([t.total_energy for t in th])
+        drift[mixed] = abs(e[-1] - e[0])
+        scale = abs(th[0].potential_energy)
+    assert drift[True] <= 2.0 * drift[False] + 1e-9 * scale
