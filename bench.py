@@ -58,6 +58,8 @@ def parse():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-symmetric", action="store_true", help="skip the full-list vs Newton-symmetric operator timing")
     ap.add_argument("--no-lj", action="store_true", help="skip the LJ argon leg")
+    ap.add_argument("--no-c2", action="store_true",
+                    help="skip the 500 000-atom FP64 GPU-vs-CPU leg (EAM Cu 50^3 cells, 100 steps)")
     ap.add_argument("--lj-cells", type=int, default=64, help="FCC cells per edge of the LJ leg (64 -> 1.05M atoms)")
     ap.add_argument("--stream-copy-gib", type=float, default=0.0,
                     help="also time a plain HBM copy of this many GiB (the counter calibration kernel)")
@@ -118,6 +120,41 @@ def run_form(sim, cfg, tab, mixed, steps, warmup, dt, rc, skin):
         "T0": th0.temperature, "T_end": thK.temperature,
         "warmup_rebuilds": int(st_w.rebuilds),
     }
+
+
+C2_CELLS, C2_STEPS = 50, 100
+
+
+def c2_config(seed):
+    """EAM Cu FCC 50^3 cells = 500 000 atoms, periodic, seeded 0.05 A Gaussian jitter,
+    zero initial velocities (the plain-FP64 GPU-vs-CPU configuration)."""
+    from exastamp_amd import inputs
+    return inputs.lattice_config((C2_CELLS,) * 3, 3.615, [63.546], T=0.0, jitter=0.05, seed=seed)
+
+
+def c2_leg(tab, dt, rc, skin, seed):
+    """The 500 000-atom FP64 configuration on the GPU: list build and first forces
+    untimed (as the CPU sample's setup), then 100 NVE steps of 1 fs timed."""
+    from exastamp_amd import _lib
+    from exastamp_amd.operators import Simulation
+    import ctypes as C
+    cfg = c2_config(seed)
+    sim = Simulation(0)
+    setup(sim, cfg, tab)
+    sim.md_run("eam", 5, dt, rc, skin)  # warm-up (kernels, buffers)
+    sim.set_config(cfg)
+    sim.md_run("eam", 0, dt, rc, skin)  # first_iteration: list and forces at t = 0
+    sim.ctx.xs_synchronize()
+    st = _lib.RunStats()
+    t0 = time.perf_counter()
+    sim.ctx.xs_md_run(_lib.XS_POT_EAM, C2_STEPS, dt, rc, skin, _lib.XS_CONTINUE, 0, None, C.byref(st))
+    sim.ctx.xs_synchronize()
+    sec = time.perf_counter() - t0
+    sim.close()
+    return {"workload": f"EAM Cu FCC {C2_CELLS}^3 cells = {cfg.n} atoms, 0.05 A jitter, 0 K, FP64, "
+                        f"{C2_STEPS} NVE steps of 1 fs",
+            "gpu_fp64": {"value": cfg.n * C2_STEPS / sec, "unit": "atom-steps/s", "seconds": sec,
+                         "rebuilds": int(st.rebuilds)}}
 
 
 def drift_leg(sim, cfg, tab, mixed, steps, dt, rc, skin):
@@ -520,6 +557,18 @@ def main():
             cpu["single_thread"] = {k: one[k] for k in ("value", "cores", "sample", "seconds")}
             log(f"cpu 1 thread: {one['value']:.4g} atom-steps/s")
 
+    c2 = None
+    if not args.no_c2:
+        log(f"C2 leg: EAM Cu {C2_CELLS}^3 cells, FP64, {C2_STEPS} steps on the GPU")
+        c2 = c2_leg(tab, dt, rc, skin, args.seed)
+        log(f"C2: GPU FP64 {c2['gpu_fp64']['value']:.4g} atom-steps/s")
+        if cpu is not None:
+            log(f"C2: CPU oracle, {C2_STEPS} steps")
+            c2cpu = cpu_baseline(c2_config(args.seed), tab, C2_STEPS, args.cpu_threads, dt, rc, skin)
+            cpu["c2"] = {k: c2cpu[k] for k in ("value", "cores", "sample", "seconds")}
+            c2["gpu_over_cpu"] = c2["gpu_fp64"]["value"] / c2cpu["value"]
+            log(f"C2: CPU {c2cpu['value']:.4g} atom-steps/s on {c2cpu['cores']} threads")
+
     lj = None
     if not args.no_lj:
         log(f"LJ argon leg: {args.lj_cells}^3 FCC cells, 200 steps per form")
@@ -574,6 +623,7 @@ def main():
         "kernels_fp64": None if fp64 is None else fp64["kernels"],
         "newton_symmetric": sym,
         "lj": lj,
+        "c2_fp64_500k": c2,
         "roofline": roof,
         "stream_copy": stream,
         "cpu_baseline": cpu,
