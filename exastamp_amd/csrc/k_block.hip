@@ -353,18 +353,28 @@ __device__ __forceinline__ int central_count(const BlockShape& s, const HaloCell
   return n;
 }
 
+// Destination lane of this lane's key in one stable split pass: keys with a 0 bit
+// first (in lane order), then those with a 1.  zeros = ballot of the 0 keys; mbcnt
+// counts the 0 keys below this lane, so the 1 keys below it are lane minus that.
+// Both candidates are formed and one is selected, so the pass has no branch.
+__device__ __forceinline__ int split_dest(bool one, unsigned long long zeros, int lane)
+{
+  const int zb = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(zeros >> 32),
+                                                __builtin_amdgcn_mbcnt_lo((uint32_t)zeros, 0u));
+  const int nz = __popcll(zeros);
+  const int d1 = nz + lane - zb;
+  return one ? d1 : zb;
+}
+
 // Stable sort of one 32-bit key per lane over the 64 lanes of a wave by key bits
 // [lo, lo + nbits): least-significant-digit radix sort, one bit per pass (ballot,
-// prefix popcounts, forward permute).  Keys whose sort bits are all ones go last.
+// prefix counts, forward permute).  Keys whose sort bits are all ones go last.
 __device__ __forceinline__ uint32_t wave_radix_sort(uint32_t key, int lo, int nbits)
 {
   const int lane = threadIdx.x & 63;
-  const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   for (int bit = lo; bit < lo + nbits; bit++) {
     const bool one = (key >> bit) & 1u;
-    const unsigned long long zeros = __ballot(!one);
-    const int nz = __popcll(zeros);
-    const int dest = one ? nz + __popcll(~zeros & below) : __popcll(zeros & below);
+    const int dest = split_dest(one, __ballot(!one), lane);
     key = (uint32_t)__builtin_amdgcn_ds_permute(dest << 2, (int)key);
   }
   return key;
@@ -375,12 +385,10 @@ __device__ __forceinline__ uint32_t wave_radix_sort(uint32_t key, int lo, int nb
 __device__ __forceinline__ void wave_radix_sort2(uint32_t& k0, uint32_t& k1, int lo, int nbits)
 {
   const int lane = threadIdx.x & 63;
-  const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   for (int bit = lo; bit < lo + nbits; bit++) {
     const bool one0 = (k0 >> bit) & 1u, one1 = (k1 >> bit) & 1u;
-    const unsigned long long z0 = __ballot(!one0), z1 = __ballot(!one1);
-    const int d0 = one0 ? __popcll(z0) + __popcll(~z0 & below) : __popcll(z0 & below);
-    const int d1 = one1 ? __popcll(z1) + __popcll(~z1 & below) : __popcll(z1 & below);
+    const int d0 = split_dest(one0, __ballot(!one0), lane);
+    const int d1 = split_dest(one1, __ballot(!one1), lane);
     k0 = (uint32_t)__builtin_amdgcn_ds_permute(d0 << 2, (int)k0);
     k1 = (uint32_t)__builtin_amdgcn_ds_permute(d1 << 2, (int)k1);
   }
