@@ -790,13 +790,23 @@ struct LNb {
 };
 
 // List slots of entries k0, k0 + 2, k0 + 4, k0 + 6 of atom gi (this lane's parity of
-// the list).  k0 = half + 8 q, so one 8-byte load (lword) brings the four entries of
-// group q (in bounds for k0 < nmax); lslots turns entries past this atom's count
-// into the atom itself (a masked pair at d2 = 0).  The passes load the next group's
-// word one iteration ahead (clamped to the current group at the end of the list).
-__device__ __forceinline__ uint2 lword(const BlockArgs& a, int64_t gi, int k0)
+// the list).  k0 = half + 8 q, so one 8-byte word brings the four entries of group q
+// (in bounds for k0 < nmax); lslots turns entries past this atom's count into the
+// atom itself (a masked pair at d2 = 0).  The passes load the next group's word one
+// iteration ahead (staying on the current group at the end of the list).  In the
+// bl_index layout the word of (gi, half + 8 q) is word 2 gi + half + q * 2 npad of
+// the array, so a lane steps a 32-bit word index instead of recomputing the 64-bit
+// index each iteration (the list holds at most 2^26 atoms x 64 entries = 2^30 words).
+struct LWords {
+  const uint2* base;
+  uint32_t i;     // this lane's word of the current group
+  uint32_t step;  // words between groups
+  __device__ __forceinline__ uint2 get() const { return base[i]; }
+  __device__ __forceinline__ void advance(bool more) { i += more ? step : 0u; }
+};
+__device__ __forceinline__ LWords lwords(const BlockArgs& a, int64_t gi, int half)
 {
-  return *reinterpret_cast<const uint2*>(a.nbr16 + bl_index(a.npad, gi, k0));
+  return {reinterpret_cast<const uint2*>(a.nbr16), (uint32_t)(2 * gi + half), (uint32_t)(2 * a.npad)};
 }
 __device__ __forceinline__ void lslots(uint2 w, int self, int k0, int cnt, int (&c)[4])
 {
@@ -849,10 +859,12 @@ __global__ void __launch_bounds__(kPassThreads, kPassWaves<false>) k_block_eam_r
     const bool oni = !MULTI || pm::eam_on(t, ti);
     const int cnt = oni ? a.nnb[ca.global] : 0;
     double rho = 0.0;
-    uint2 wnext = lword(a, ca.global, half);
+    LWords lw = lwords(a, ca.global, half);
+    uint2 wnext = lw.get();
     for (int k0 = half; k0 < cnt; k0 += 2 * kBBatch) {
       const uint2 w = wnext;
-      wnext = lword(a, ca.global, k0 + 8 < cnt ? k0 + 8 : k0);
+      lw.advance(k0 + 8 < cnt);
+      wnext = lw.get();
       int c[kBBatch];
       lslots(w, ca.local, k0, cnt, c);
       LNb nb[kBBatch];
@@ -914,10 +926,12 @@ __global__ void __launch_bounds__(kPassThreads, kPassWaves<VIRIAL>) k_block_eam_
     const int cnt = (!MULTI || pm::eam_on(t, ti)) ? a.nnb[ca.global] : 0;
     double fx = 0.0, fy = 0.0, fz = 0.0, e = 0.0;
     double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    uint2 wnext = lword(a, ca.global, half);
+    LWords lw = lwords(a, ca.global, half);
+    uint2 wnext = lw.get();
     for (int k0 = half; k0 < cnt; k0 += 2 * kBBatch) {
       const uint2 w = wnext;
-      wnext = lword(a, ca.global, k0 + 8 < cnt ? k0 + 8 : k0);
+      lw.advance(k0 + 8 < cnt);
+      wnext = lw.get();
       int c[kBBatch];
       lslots(w, ca.local, k0, cnt, c);
       // two entries at a time: their four table rows stay in flight together within
@@ -1029,10 +1043,12 @@ __global__ void __launch_bounds__(kPassThreads, kPassWaves<false>) k_block_gathe
       if (half == 0) sink[ca.global] = pi.x + cnt;
       continue;
     }
-    uint2 wnext = lword(a, ca.global, half);
+    LWords lw = lwords(a, ca.global, half);
+    uint2 wnext = lw.get();
     for (int k0 = half; k0 < cnt; k0 += 2 * kBBatch) {
       const uint2 w = wnext;
-      wnext = lword(a, ca.global, k0 + 8 < cnt ? k0 + 8 : k0);
+      lw.advance(k0 + 8 < cnt);
+      wnext = lw.get();
       int c[kBBatch];
       lslots(w, ca.local, k0, cnt, c);
       if constexpr (LEVEL == 2) {
@@ -1110,10 +1126,12 @@ __global__ void __launch_bounds__(kPassThreads, kPassWaves<VIRIAL>) k_block_lj(B
     const int cnt = a.nnb[ca.global];
     double fx = 0.0, fy = 0.0, fz = 0.0, e = 0.0;
     double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    uint2 wnext = lword(a, ca.global, half);
+    LWords lw = lwords(a, ca.global, half);
+    uint2 wnext = lw.get();
     for (int k0 = half; k0 < cnt; k0 += 2 * kBBatch) {
       const uint2 w = wnext;
-      wnext = lword(a, ca.global, k0 + 8 < cnt ? k0 + 8 : k0);
+      lw.advance(k0 + 8 < cnt);
+      wnext = lw.get();
       int c[kBBatch];
       lslots(w, ca.local, k0, cnt, c);
       LNb nb[kBBatch];
