@@ -76,34 +76,37 @@ __device__ __forceinline__ bool wrap_axis(int c, int n, int periodic, int& w, in
   return true;
 }
 
-// Staged halo records in LDS as arrays of doubles x[], y[], z[] and, for the
-// passes that read it, w[]: random 16-byte reads conflict on the LDS banks more than
-// 8-byte ones, and the passes measured faster with three or four 8-byte reads per
-// neighbour than with the two 16-byte reads of {x, y}, {z, w} records.  Passes
-// without w alias w to x.
+// Staged halo records in LDS: {x, y, z} of slot c as three doubles at 24 c (one
+// address and immediate offsets per neighbour) and, for the passes that read it, w[c]
+// in an array of its own.  Random 8-byte reads at a 24-byte stride spread over the
+// same 32 bank pairs as reads of 8-byte arrays (6 c mod 64 covers the even banks),
+// while 16-byte reads conflict more: the passes measured faster with 8-byte reads
+// per component than with the two 16-byte reads of {x, y}, {z, w} records.  Passes
+// without w alias w to the x of the records (never read).
 struct LRec {
-  double* x;
-  double* y;
-  double* z;
+  double* xyz;
   double* w;
   bool has_w;
-  __device__ __forceinline__ Pos4 get(int c) const { return {x[c], y[c], z[c], w[c]}; }
+  __device__ __forceinline__ Pos4 get(int c) const
+  {
+    const double* r = xyz + 3 * c;
+    return {r[0], r[1], r[2], w[has_w ? c : 3 * c]};
+  }
   __device__ __forceinline__ void put(int c, const Pos4& p) const
   {
-    x[c] = p.x;
-    y[c] = p.y;
-    z[c] = p.z;
+    double* r = xyz + 3 * c;
+    r[0] = p.x;
+    r[1] = p.y;
+    r[2] = p.z;
     if (has_w) w[c] = p.w;
   }
 };
 
-// record arrays at p for cap slots; returns the bytes they take
+// records at p for cap slots; returns the bytes they take
 __device__ __forceinline__ size_t lrec_at(unsigned char* p, int cap, bool with_w, LRec& r)
 {
-  r.x = reinterpret_cast<double*>(p);
-  r.y = r.x + cap;
-  r.z = r.y + cap;
-  r.w = with_w ? r.z + cap : r.x;
+  r.xyz = reinterpret_cast<double*>(p);
+  r.w = with_w ? r.xyz + 3 * (size_t)cap : r.xyz;
   r.has_w = with_w;
   return (size_t)cap * sizeof(double) * (with_w ? 4 : 3);
 }
