@@ -1073,8 +1073,9 @@ __global__ void __launch_bounds__(kPassThreads, kPassWaves<false>) k_block_gathe
           pm::s_knot(t, nb[u].d2, m, pp, out);
           float v;
           if constexpr (LEVEL == 0) {
-            const pm::Q3 gr = pm::load3(pair + t.s_force + (int64_t)m * 8);
-            const pm::Q3 gz = pm::load3(pair + t.s_force + (int64_t)m * 8 + 4);
+            const float* row = pm::at32(pair + t.s_force, (uint32_t)m * 8u);
+            const pm::Q3 gr = pm::load3(row);
+            const pm::Q3 gz = pm::load3(row + 4);
             v = ((gr.a + gr.b) + (gr.c + gz.a)) + ((gz.b + gz.c) + pp) + (float)nb[u].w;
           } else {
             v = (pp + (float)m) + (float)nb[u].w;

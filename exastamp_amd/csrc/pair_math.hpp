@@ -182,10 +182,20 @@ __device__ __forceinline__ void knot_dist(const EamTabDev<float>& t, double d2, 
   rinv = (float)y;
 }
 
+// Element `i` of a table whose base is the same for every lane: the byte offset is
+// formed in 32 bits (every table here is far below 4 GiB and its indices are
+// non-negative), so the loads take the scalar-base form with one 32-bit offset per
+// lane instead of 64-bit address arithmetic.
+template <class T>
+__device__ __forceinline__ const T* at32(const T* base, uint32_t i)
+{
+  return reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + (i * (uint32_t)sizeof(T)));
+}
+
 template <class Real>
 __device__ __forceinline__ const Real* row_r(const Real* tab, const EamTabDev<Real>& t, int which, int m)
 {
-  return tab + ((int64_t)which * (t.nr + 1) + m) * 8;
+  return at32(tab, ((uint32_t)which * (uint32_t)(t.nr + 1) + (uint32_t)m) * 8u);
 }
 
 template <class Real>
@@ -256,12 +266,12 @@ __device__ __forceinline__ void embed(const EamTabDev<Real>& t, int ti, double r
 template <class Real>
 __device__ __forceinline__ const Real* pair_rho_row(const Real* __restrict__ pair, int m)
 {
-  return pair + (int64_t)m * 4;
+  return at32(pair, (uint32_t)m * 4u);
 }
 template <class Real>
 __device__ __forceinline__ const Real* pair_z2_row(const Real* __restrict__ pair, const EamTabDev<Real>& t, int m)
 {
-  return pair + ((int64_t)t.nr + 1 + m) * 4;
+  return at32(pair, (uint32_t)(t.nr + 1 + m) * 4u);
 }
 
 // Single-element psi'(r)/r and phi(r) from the two rows of knot m:
@@ -412,10 +422,10 @@ __device__ __forceinline__ void eam_pair_batch(const EamTabDev<Real>& t, const R
       float p;
       bool out;
       s_knot(t, nb[u].d2, m, p, out);
-      const Real* row = pair + t.s_force + (int64_t)m * 8;
+      const Real* row = at32(pair + t.s_force, (uint32_t)m * 8u);
       const Q3 gr = load3(row), gz = load3(row + 4);
       fpair[u] = fmaf(fpi + (Real)nb[u].w, quad3(gr, p), quad3(gz, p));
-      if constexpr (PHI) phi[u] = cubic(load4(pair + t.s_phi + (int64_t)m * 4), p);
+      if constexpr (PHI) phi[u] = cubic(load4(at32(pair + t.s_phi, (uint32_t)m * 4u)), p);
       slow[u] = nb[u].in && out;
     }
   } else {
