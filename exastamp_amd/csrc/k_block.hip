@@ -814,8 +814,11 @@ __device__ __forceinline__ LWords lwords(const BlockArgs& a, int64_t gi, int hal
 __device__ __forceinline__ void lslots(uint2 w, int self, int k0, int cnt, int (&c)[4])
 {
   const int raw[4] = {(int)(w.x & 0xffffu), (int)(w.x >> 16), (int)(w.y & 0xffffu), (int)(w.y >> 16)};
+  // slots are 16-bit: saying so of `self` too keeps the record addresses 24-bit
+  // multiplies instead of 64-bit ones
+  const int me = self & 0xffff;
 #pragma unroll
-  for (int u = 0; u < 4; u++) c[u] = (k0 + 2 * u < cnt) ? raw[u] : self;
+  for (int u = 0; u < 4; u++) c[u] = (k0 + 2 * u < cnt) ? raw[u] : me;
 }
 
 // neighbour records of slots c[0..U-1] (entries k0, k0 + 2, ...)
