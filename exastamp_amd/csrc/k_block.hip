@@ -796,20 +796,20 @@ struct LNb {
 // the list).  k0 = half + 8 q, so one 8-byte word brings the four entries of group q
 // (in bounds for k0 < nmax); lslots turns entries past this atom's count into the
 // atom itself (a masked pair at d2 = 0).  The passes load the next group's word one
-// iteration ahead (staying on the current group at the end of the list).  In the
-// bl_index layout the word of (gi, half + 8 q) is word 2 gi + half + q * 2 npad of
-// the array, so a lane steps a 32-bit word index instead of recomputing the 64-bit
-// index each iteration (the list holds at most 2^26 atoms x 64 entries = 2^30 words).
+// iteration ahead (none at the end of the list).  In the bl_index layout the word of
+// (gi, half + 8 q) is word 2 gi + half of group q, whose words start q * 2 npad words
+// into the array.  Every lane of a wave is at the same q in a given iteration, so
+// the group's start is one scalar address and each lane adds its own fixed 32-bit
+// offset (the list holds at most 2^26 atoms x 64 entries = 2^30 words).
 struct LWords {
   const uint2* base;
-  uint32_t i;     // this lane's word of the current group
-  uint32_t step;  // words between groups
-  __device__ __forceinline__ uint2 get() const { return base[i]; }
-  __device__ __forceinline__ void advance(bool more) { i += more ? step : 0u; }
+  uint32_t lane;  // this lane's word within a group
+  int64_t step;   // words per group
+  __device__ __forceinline__ uint2 get(int q) const { return (base + q * step)[lane]; }
 };
 __device__ __forceinline__ LWords lwords(const BlockArgs& a, int64_t gi, int half)
 {
-  return {reinterpret_cast<const uint2*>(a.nbr16), (uint32_t)(2 * gi + half), (uint32_t)(2 * a.npad)};
+  return {reinterpret_cast<const uint2*>(a.nbr16), (uint32_t)(2 * gi + half), 2 * a.npad};
 }
 __device__ __forceinline__ void lslots(uint2 w, int self, int k0, int cnt, int (&c)[4])
 {
@@ -866,11 +866,10 @@ __global__ void __launch_bounds__(kPassThreads, kPassWaves<false>) k_block_eam_r
     const int cnt = oni ? a.nnb[ca.global] : 0;
     double rho = 0.0;
     LWords lw = lwords(a, ca.global, half);
-    uint2 wnext = lw.get();
-    for (int k0 = half; k0 < cnt; k0 += 2 * kBBatch) {
+    uint2 wnext = lw.get(0);
+    for (int k0 = half, q = 0; k0 < cnt; k0 += 2 * kBBatch, q++) {
       const uint2 w = wnext;
-      lw.advance(k0 + 8 < cnt);
-      wnext = lw.get();
+      if (k0 + 8 < cnt) wnext = lw.get(q + 1);
       int c[kBBatch];
       lslots(w, ca.local, k0, cnt, c);
       LNb nb[kBBatch];
@@ -933,11 +932,10 @@ __global__ void __launch_bounds__(kPassThreads, kPassWaves<VIRIAL>) k_block_eam_
     double fx = 0.0, fy = 0.0, fz = 0.0, e = 0.0;
     double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     LWords lw = lwords(a, ca.global, half);
-    uint2 wnext = lw.get();
-    for (int k0 = half; k0 < cnt; k0 += 2 * kBBatch) {
+    uint2 wnext = lw.get(0);
+    for (int k0 = half, q = 0; k0 < cnt; k0 += 2 * kBBatch, q++) {
       const uint2 w = wnext;
-      lw.advance(k0 + 8 < cnt);
-      wnext = lw.get();
+      if (k0 + 8 < cnt) wnext = lw.get(q + 1);
       int c[kBBatch];
       lslots(w, ca.local, k0, cnt, c);
       // two entries at a time: their four table rows stay in flight together within
@@ -1050,11 +1048,10 @@ __global__ void __launch_bounds__(kPassThreads, kPassWaves<false>) k_block_gathe
       continue;
     }
     LWords lw = lwords(a, ca.global, half);
-    uint2 wnext = lw.get();
-    for (int k0 = half; k0 < cnt; k0 += 2 * kBBatch) {
+    uint2 wnext = lw.get(0);
+    for (int k0 = half, q = 0; k0 < cnt; k0 += 2 * kBBatch, q++) {
       const uint2 w = wnext;
-      lw.advance(k0 + 8 < cnt);
-      wnext = lw.get();
+      if (k0 + 8 < cnt) wnext = lw.get(q + 1);
       int c[kBBatch];
       lslots(w, ca.local, k0, cnt, c);
       if constexpr (LEVEL == 2) {
@@ -1134,11 +1131,10 @@ __global__ void __launch_bounds__(kPassThreads, kPassWaves<VIRIAL>) k_block_lj(B
     double fx = 0.0, fy = 0.0, fz = 0.0, e = 0.0;
     double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     LWords lw = lwords(a, ca.global, half);
-    uint2 wnext = lw.get();
-    for (int k0 = half; k0 < cnt; k0 += 2 * kBBatch) {
+    uint2 wnext = lw.get(0);
+    for (int k0 = half, q = 0; k0 < cnt; k0 += 2 * kBBatch, q++) {
       const uint2 w = wnext;
-      lw.advance(k0 + 8 < cnt);
-      wnext = lw.get();
+      if (k0 + 8 < cnt) wnext = lw.get(q + 1);
       int c[kBBatch];
       lslots(w, ca.local, k0, cnt, c);
       LNb nb[kBBatch];
