@@ -102,13 +102,14 @@ struct LRec {
   }
 };
 
-// records at p for cap slots; returns the bytes they take
+// records at p for cap slots plus the far record at slot cap (see stage); returns
+// the bytes they take
 __device__ __forceinline__ size_t lrec_at(unsigned char* p, int cap, bool with_w, LRec& r)
 {
   r.xyz = reinterpret_cast<double*>(p);
-  r.w = with_w ? r.xyz + 3 * (size_t)cap : r.xyz;
+  r.w = with_w ? r.xyz + 3 * (size_t)(cap + 1) : r.xyz;
   r.has_w = with_w;
-  return (size_t)cap * sizeof(double) * (with_w ? 4 : 3);
+  return (size_t)(cap + 1) * sizeof(double) * (with_w ? 4 : 3);
 }
 
 // Halo table of block b in LDS (hc[0..nh)), filled by all threads; returns the
@@ -208,11 +209,31 @@ __device__ void stage_each(const BlockShape& s, const HaloCell* hc, int total, c
   __syncthreads();
 }
 
-// Stage halo atoms into the records (and types when requested).
+// Stage halo atoms into the records (and types when requested), and the far record
+// at slot cap: list entries past an atom's count point there (lslots), so their
+// displacement is finite and longer than any cutoff, and the passes mask them by
+// d2 <= rc2 alone.  It lies at least 10^4 A from every staged position (which are
+// within the box and its images) and close enough that its knot index fits an int.
 __device__ void stage(const BlockShape& s, const HaloCell* hc, int total, const Pos4* __restrict__ pos,
-                      const double* L, const XformDev* xf, const uint8_t* __restrict__ type, const LRec& rec,
-                      uint8_t* ltype, uint16_t* slot_cell)
+                      const BoxDev& box, const XformDev* xf, const uint8_t* __restrict__ type, const LRec& rec,
+                      uint8_t* ltype, uint16_t* slot_cell, int cap)
 {
+  const double* L = box.L;
+  if (threadIdx.x == 0) {  // visible after stage_each's barriers
+    const double far = 4.0 * (fabs(box.lo[0]) + fabs(box.lo[1]) + fabs(box.lo[2]) + L[0] + L[1] + L[2]) + 1e4;
+    Pos4 f = {far, far, far, 0.0};
+    if (xf) {
+      xsmd_xform_apply(xf->m, &f.x, &f.y, &f.z);
+      const double g = 1e4 + 4.0 * fabs(f.x - far) + 4.0 * fabs(f.y - far) + 4.0 * fabs(f.z - far);
+      f = {f.x + g, f.y + g, f.z + g, 0.0};  // still far in physical space
+    }
+    double* r = rec.xyz + 3 * cap;
+    r[0] = f.x;
+    r[1] = f.y;
+    r[2] = f.z;
+    if (rec.has_w) rec.w[cap] = 0.0;
+    if (ltype) ltype[cap] = 0;
+  }
   const auto put = [&](int c, const Pos4& p, int j) {
     rec.put(c, p);
     if (ltype) ltype[c] = type[j];
@@ -794,8 +815,8 @@ struct LNb {
 
 // List slots of entries k0, k0 + 2, k0 + 4, k0 + 6 of atom gi (this lane's parity of
 // the list).  k0 = half + 8 q, so one 8-byte word brings the four entries of group q
-// (in bounds for k0 < nmax); lslots turns entries past this atom's count into the
-// atom itself (a masked pair at d2 = 0).  The passes load the next group's word one
+// (in bounds for k0 < nmax); lslots turns entries past this atom's count into the far
+// record (a pair longer than any cutoff).  The passes load the next group's word one
 // iteration ahead (none at the end of the list).  In the bl_index layout the word of
 // (gi, half + 8 q) is word 2 gi + half of group q, whose words start q * 2 npad words
 // into the array.  Every lane of a wave is at the same q in a given iteration, so
@@ -811,14 +832,14 @@ __device__ __forceinline__ LWords lwords(const BlockArgs& a, int64_t gi, int hal
 {
   return {reinterpret_cast<const uint2*>(a.nbr16), (uint32_t)(2 * gi + half), 2 * a.npad};
 }
-__device__ __forceinline__ void lslots(uint2 w, int self, int k0, int cnt, int (&c)[4])
+__device__ __forceinline__ void lslots(uint2 w, int far_slot, int k0, int cnt, int (&c)[4])
 {
   const int raw[4] = {(int)(w.x & 0xffffu), (int)(w.x >> 16), (int)(w.y & 0xffffu), (int)(w.y >> 16)};
-  // slots are 16-bit: saying so of `self` too keeps the record addresses 24-bit
+  // slots are 16-bit: saying so of the far slot too keeps the record addresses 24-bit
   // multiplies instead of 64-bit ones
-  const int me = self & 0xffff;
+  const int fs = far_slot & 0xffff;
 #pragma unroll
-  for (int u = 0; u < 4; u++) c[u] = (k0 + 2 * u < cnt) ? raw[u] : me;
+  for (int u = 0; u < 4; u++) c[u] = (k0 + 2 * u < cnt) ? raw[u] : fs;
 }
 
 // neighbour records of slots c[0..U-1] (entries k0, k0 + 2, ...)
@@ -835,7 +856,7 @@ __device__ __forceinline__ void lgather(const BlockArgs& a, const LRec& rec, con
     nb[u].d2 = xsmd_d2(nb[u].x, nb[u].y, nb[u].z);
     nb[u].w = pj.w;
     nb[u].c = c[u];
-    nb[u].in = (k0 + 2 * u < cnt) && nb[u].d2 <= a.rc2;
+    nb[u].in = nb[u].d2 <= a.rc2;  // entries past the count read the far record
   }
 }
 
@@ -853,7 +874,7 @@ __global__ void __launch_bounds__(kPassThreads, kPassWaves<false>) k_block_eam_r
   const int total = build_halo(a.g, s, a.cell_off, l.hc, l.total);
   if (over_capacity(a, total, a.cap)) return;
   central_table(s, l.hc, l.central_local, a.ccap, l.total + 2);
-  stage(s, l.hc, total, a.pos, a.box[s.r].L, a.xf, a.type, l.rec, l.ltype, l.slot_cell);
+  stage(s, l.hc, total, a.pos, a.box[s.r], a.xf, a.type, l.rec, l.ltype, l.slot_cell, a.cap);
   const int ncentral = l.total[2];
   if (over_capacity(a, ncentral, a.ccap)) return;
   const int half = pass_half();
@@ -871,7 +892,7 @@ __global__ void __launch_bounds__(kPassThreads, kPassWaves<false>) k_block_eam_r
       const uint2 w = wnext;
       if (k0 + 8 < cnt) wnext = lw.get(q + 1);
       int c[kBBatch];
-      lslots(w, ca.local, k0, cnt, c);
+      lslots(w, a.cap, k0, cnt, c);
       LNb nb[kBBatch];
       lgather<kBBatch>(a, l.rec, c, k0, cnt, pi, nb);
       if constexpr (MULTI) {
@@ -887,8 +908,10 @@ __global__ void __launch_bounds__(kPassThreads, kPassWaves<false>) k_block_eam_r
 #pragma unroll
       for (int u = 0; u < kBBatch; u++) {
         const int tj = MULTI ? l.ltype[nb[u].c] : 0;
-        const Real val = pm::eam_rho_knot<Real, MULTI>(t, pair, ti, tj, nb[u].d2, nb[u].in);
-        rho += nb[u].in ? (double)val : 0.0;
+        // every d2 here is finite and positive (entries past the count read the far
+        // record), so the knot index needs no guard value; the sum masks by `in`
+        const Real val = pm::eam_rho_knot<Real, MULTI>(t, pair, ti, tj, nb[u].d2, true);
+        rho += (double)(nb[u].in ? val : Real(0));
       }
     }
     rho = half_sum(rho);
@@ -920,7 +943,7 @@ __global__ void __launch_bounds__(kPassThreads, kPassWaves<VIRIAL>) k_block_eam_
   const BlockShape s = block_shape(a.g, b);
   const int total = build_halo(a.g, s, a.cell_off, l.hc, l.total);
   if (over_capacity(a, total, a.cap)) return;
-  stage(s, l.hc, total, a.pos, a.box[s.r].L, a.xf, a.type, l.rec, l.ltype, l.slot_cell);
+  stage(s, l.hc, total, a.pos, a.box[s.r], a.xf, a.type, l.rec, l.ltype, l.slot_cell, a.cap);
   const int half = pass_half();
   for (int q = pass_slot();; q += kPassAtoms) {
     Central ca;
@@ -937,7 +960,7 @@ __global__ void __launch_bounds__(kPassThreads, kPassWaves<VIRIAL>) k_block_eam_
       const uint2 w = wnext;
       if (k0 + 8 < cnt) wnext = lw.get(q + 1);
       int c[kBBatch];
-      lslots(w, ca.local, k0, cnt, c);
+      lslots(w, a.cap, k0, cnt, c);
       // two entries at a time: their four table rows stay in flight together within
       // the register budget of six waves per SIMD (the barrier keeps the compiler
       // from hoisting the second pair's gathers above the first pair's rows)
@@ -981,7 +1004,7 @@ __global__ void __launch_bounds__(kPassThreads, kPassWaves<VIRIAL>) k_block_eam_
           for (int u = 0; u < 2; u++) {
             const int tj = MULTI ? l.ltype[nb[u].c] : 0;
             Real fp_r, phi;
-            pm::eam_pair_knot<Real, MULTI>(t, pair, ti, tj, fpi, (Real)nb[u].w, nb[u].d2, nb[u].in, fp_r, phi);
+            pm::eam_pair_knot<Real, MULTI>(t, pair, ti, tj, fpi, (Real)nb[u].w, nb[u].d2, true, fp_r, phi);
             add(nb[u], nb[u].in, fp_r, phi);
           }
         }
@@ -1026,7 +1049,7 @@ __global__ void __launch_bounds__(kPassThreads, kPassWaves<false>) k_block_gathe
   const int total = build_halo(a.g, s, a.cell_off, l.hc, l.total);
   if (over_capacity(a, total, a.cap)) return;
   if constexpr (!FORCE) central_table(s, l.hc, l.central_local, a.ccap, l.total + 2);
-  stage(s, l.hc, total, a.pos, a.box[s.r].L, a.xf, nullptr, l.rec, l.ltype, l.slot_cell);
+  stage(s, l.hc, total, a.pos, a.box[s.r], a.xf, nullptr, l.rec, l.ltype, l.slot_cell, a.cap);
   int ncentral = 0;
   if constexpr (!FORCE) {
     ncentral = l.total[2];
@@ -1053,7 +1076,7 @@ __global__ void __launch_bounds__(kPassThreads, kPassWaves<false>) k_block_gathe
       const uint2 w = wnext;
       if (k0 + 8 < cnt) wnext = lw.get(q + 1);
       int c[kBBatch];
-      lslots(w, ca.local, k0, cnt, c);
+      lslots(w, a.cap, k0, cnt, c);
       if constexpr (LEVEL == 2) {
         acc += (float)((c[0] + c[1]) + (c[2] + c[3]));
         continue;
@@ -1083,7 +1106,7 @@ __global__ void __launch_bounds__(kPassThreads, kPassWaves<false>) k_block_gathe
           acc += nb[u].in ? v : 0.0f;
         } else {
           float rinv;
-          pm::knot_dist(t, nb[u].in ? nb[u].d2 : 1.0, m, pp, rinv);
+          pm::knot_dist(t, nb[u].d2, m, pp, rinv);
           float v;
           if constexpr (LEVEL == 0) {
             const pm::Q4<float> r = pm::load4(pm::pair_rho_row(pair, m));
@@ -1118,7 +1141,8 @@ __global__ void __launch_bounds__(kPassThreads, kPassWaves<VIRIAL>) k_block_lj(B
   const int total = build_halo(a.g, s, a.cell_off, l.hc, l.total);
   if (over_capacity(a, total, a.cap)) return;
   central_table(s, l.hc, l.central_local, a.ccap, l.total + 2);
-  stage(s, l.hc, total, a.pos, a.box[s.r].L, a.xf, MULTI ? a.type : nullptr, l.rec, l.ltype, l.slot_cell);
+  stage(s, l.hc, total, a.pos, a.box[s.r], a.xf, MULTI ? a.type : nullptr, l.rec, l.ltype, l.slot_cell,
+        a.cap);
   const int ncentral = l.total[2];
   if (over_capacity(a, ncentral, a.ccap)) return;
   const int half = pass_half();
@@ -1136,7 +1160,7 @@ __global__ void __launch_bounds__(kPassThreads, kPassWaves<VIRIAL>) k_block_lj(B
       const uint2 w = wnext;
       if (k0 + 8 < cnt) wnext = lw.get(q + 1);
       int c[kBBatch];
-      lslots(w, ca.local, k0, cnt, c);
+      lslots(w, a.cap, k0, cnt, c);
       LNb nb[kBBatch];
       lgather<kBBatch>(a, l.rec, c, k0, cnt, pi, nb);
 #pragma unroll
@@ -1401,8 +1425,10 @@ void block_setup(Ctx& c)
 
 size_t block_lds_bytes(const Ctx& c, bool with_type, bool with_w)
 {
-  return (size_t)c.bcap * sizeof(double) * (with_w ? 4 : 3) + (size_t)c.bgeom.nh_max * sizeof(HaloCell) +
-         4 * sizeof(int) + (size_t)(c.bcap + c.bcentral) * sizeof(uint16_t) + (with_type ? (size_t)c.bcap : 0);
+  // bcap records and the far record, halo table, counters, slot cells, central slots,
+  // per-record types (lds_layout)
+  return (size_t)(c.bcap + 1) * sizeof(double) * (with_w ? 4 : 3) + (size_t)c.bgeom.nh_max * sizeof(HaloCell) +
+         4 * sizeof(int) + (size_t)(c.bcap + c.bcentral) * sizeof(uint16_t) + (with_type ? (size_t)c.bcap + 1 : 0);
 }
 
 bool block_feasible(const Ctx& c)

@@ -357,8 +357,10 @@ template <class Real, bool MULTI>
 constexpr bool kSTable = !MULTI && sizeof(Real) == 4;
 
 // rho_{type_j}(r) contribution to atom i (SymRhoOp, eam_force_op_multimat.h:142-171)
-// from the knot rows (single element: plane 0 of the pair table).  `in` = the pair
-// counts; the value of a pair that does not is arbitrary (callers mask it).
+// from the knot rows (single element: plane 0 of the pair table).  `in` false
+// evaluates at d2 = 1 instead (a guard for list padding whose d2 may be 0); the value
+// of a pair that does not count is arbitrary and callers mask it.  Callers whose d2 is
+// always finite and positive pass true.
 template <class Real, bool MULTI>
 __device__ __forceinline__ Real eam_rho_knot(const EamTabDev<Real>& t, const Real* __restrict__ pair, int ti, int tj,
                                              double d2, bool in)
