@@ -421,19 +421,6 @@ __device__ __forceinline__ void wave_radix_sort2(uint32_t& k0, uint32_t& k1, int
 // Lists of at most 64 entries are kept sorted by distance (see k_list_sort).
 constexpr int kSortRows = 64;
 
-// Squared distance from p to the box of the halo cell at unwrapped grid coordinate
-// (gx, gy, gz), widened by a margin that covers the rounding of cell assignment.
-// Along a non-periodic axis the edge cells also hold the atoms clamped into them,
-// so their boxes extend to infinity there.
-__device__ __forceinline__ double cell_box_d2(const BlockGeom& g, const BoxDev& b, int gx, int gy, int gz,
-                                              const Pos4& p)
-{
-  const double ux = xsmd_cell_gap(p.x, b.lo[0], b.L[0], g.nx, gx, g.px);
-  const double uy = xsmd_cell_gap(p.y, b.lo[1], b.L[1], g.ny, gy, g.py);
-  const double uz = xsmd_cell_gap(p.z, b.lo[2], b.L[2], g.nz, gz, g.pz);
-  return xsmd_d2(ux, uy, uz);
-}
-
 // Block-list storage.  The entries of atom i come in groups of 8: group q holds
 // entries 8q .. 8q + 7 in the 16 bytes at nbr16 + (q * npad + i) * 8, the even
 // entries first (8q, 8q + 2, 8q + 4, 8q + 6, then 8q + 1, ..., 8q + 7).  The two
@@ -449,9 +436,11 @@ __device__ __forceinline__ int64_t bl_index(int64_t npad, int64_t i, int k)
 // global build, ascending halo slot).  One lane per central atom walks the
 // candidates itself, two at a time (their LDS reads overlap), and packs its hits
 // into the current 8-entry group in registers; a full group goes out as one
-// 16-byte store.  A neighbour cell whose box lies beyond rl is skipped (its atoms
-// would all fail the exact test), unless some atom sits outside its cell's box
-// (nocull, set by the cell-key kernel).  Entries past nmax are counted, not
+// 16-byte store.  A neighbour cell whose box (widened by a margin that covers the
+// rounding of cell assignment; unbounded on the outer side of a non-periodic edge
+// cell, which also holds the atoms clamped into it) lies beyond rl is trimmed from
+// the walk (its atoms would all fail the exact test), unless some atom sits outside
+// its cell's box (nocull, set by the cell-key kernel).  Entries past nmax are counted, not
 // stored (the host grows nmax and builds again).
 constexpr int kBuildThreads = 256;
 
@@ -588,30 +577,44 @@ __global__ void __launch_bounds__(kBuildThreads) k_block_build(BlockGeom g, cons
         ready = true;
       }
     };
+    // The three cells dx = -1, 0, 1 of one (dy, dz) row of the stencil hold
+    // consecutive halo slots, so a row is walked as one slot range: from the first
+    // to the last of its cells that the box test keeps (a culled cell between them
+    // is walked, and its atoms fail the exact test).  Nine walks of about three
+    // cells each, rather than 27 of one, even out the trip counts of the wave's
+    // lanes and halve the odd-count tails of the pairwise walk.
     for (int dz = -1; dz <= 1; dz++)
-      for (int dy = -1; dy <= 1; dy++)
-        for (int dx = -1; dx <= 1; dx++) {
-          const HaloCell e = l.hc[((iz + dz) * s.hy + (iy + dy)) * s.hx + (ix + dx)];
-          if (e.count == 0) continue;
-          if (cull && cell_box_d2(g, bx, s.cx0 - 1 + ix + dx, s.cy0 - 1 + iy + dy, s.cz0 - 1 + iz + dz, pi) >= rl2)
-            continue;
-          const int end = e.lstart + e.count;
-          for (int c = e.lstart; c < end; c += 2) {
-            // record c + 1 <= cap exists (the next cell's first atom, or padding)
-            const double3 p0 = P[c], p1 = P[c + 1];
-            double u0 = p0.x - xi, v0 = p0.y - yi, w0 = p0.z - zi;
-            double u1 = p1.x - xi, v1 = p1.y - yi, w1 = p1.z - zi;
-            if constexpr (XF) {
-              xsmd_xform_apply(xf->m, &u0, &v0, &w0);
-              xsmd_xform_apply(xf->m, &u1, &v1, &w1);
-            }
-            const double d0 = xsmd_d2(u0, v0, w0);
-            const double d1 = xsmd_d2(u1, v1, w1);
-            if (d0 < rl2 && c != a.local) hit(c, d0);
-            if (c + 1 < end && d1 < rl2 && c + 1 != a.local) hit(c + 1, d1);
-          }
-          if (ready) flush();
+      for (int dy = -1; dy <= 1; dy++) {
+        const int h0 = ((iz + dz) * s.hy + (iy + dy)) * s.hx + ix - 1;
+        int beg = l.hc[h0].lstart;
+        int end = l.hc[h0 + 2].lstart + l.hc[h0 + 2].count;
+        if (cull) {
+          const int gx = s.cx0 - 2 + ix, gy = s.cy0 - 1 + iy + dy, gz = s.cz0 - 1 + iz + dz;
+          const double uy = xsmd_cell_gap(pi.y, bx.lo[1], bx.L[1], g.ny, gy, g.py);
+          const double uz = xsmd_cell_gap(pi.z, bx.lo[2], bx.L[2], g.nz, gz, g.pz);
+          bool keep[3];
+#pragma unroll
+          for (int k = 0; k < 3; k++)
+            keep[k] = xsmd_d2(xsmd_cell_gap(pi.x, bx.lo[0], bx.L[0], g.nx, gx + k, g.px), uy, uz) < rl2;
+          if (!keep[0]) beg = keep[1] ? l.hc[h0 + 1].lstart : l.hc[h0 + 2].lstart;
+          if (!keep[2]) end = keep[1] ? l.hc[h0 + 2].lstart : l.hc[h0 + 1].lstart;
         }
+        for (int c = beg; c < end; c += 2) {
+          // record c + 1 <= cap exists (the next cell's first atom, or padding)
+          const double3 p0 = P[c], p1 = P[c + 1];
+          double u0 = p0.x - xi, v0 = p0.y - yi, w0 = p0.z - zi;
+          double u1 = p1.x - xi, v1 = p1.y - yi, w1 = p1.z - zi;
+          if constexpr (XF) {
+            xsmd_xform_apply(xf->m, &u0, &v0, &w0);
+            xsmd_xform_apply(xf->m, &u1, &v1, &w1);
+          }
+          const double d0 = xsmd_d2(u0, v0, w0);
+          const double d1 = xsmd_d2(u1, v1, w1);
+          if (d0 < rl2 && c != a.local) hit(c, d0);
+          if (c + 1 < end && d1 < rl2 && c + 1 != a.local) hit(c + 1, d1);
+        }
+        if (ready) flush();
+      }
     if ((cnt & 7) != 0) {  // the last, partial group: move it down (unused places hold slot 0)
       for (int k = cnt & 7; k < 8; k++) {
         fifo_push(w, 0u);
