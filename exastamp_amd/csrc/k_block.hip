@@ -537,8 +537,8 @@ __global__ void __launch_bounds__(kBuildThreads) k_block_build(BlockGeom g, cons
     int cnt = 0;
     uint32_t w[4] = {0u, 0u, 0u, 0u};
     // A hit at slot c goes into the group FIFO w; a full group moves to r and is
-    // stored (when it fits in nmax) after the current stencil cell, so that the
-    // wave issues at most one store per cell rather than one per candidate pair.
+    // stored (when it fits in nmax) after the current stencil row, so that the
+    // wave issues one store per row rather than one per candidate pair.
     // gp is the atom's next group (bl_index(npad, i, 8 q)); gq its index q.
     uint32_t r[4] = {0u, 0u, 0u, 0u};
     uint32_t bw[2] = {0u, 0u}, br[2] = {0u, 0u};  // distance keys of w and r (BINS)
@@ -567,7 +567,7 @@ __global__ void __launch_bounds__(kBuildThreads) k_block_build(BlockGeom g, cons
       }
       cnt++;
       if ((cnt & 7) == 0) {
-        if (ready) flush();  // a second full group within one cell
+        if (ready) flush();  // a second full group within one row
         r[0] = w[0];
         r[1] = w[1];
         r[2] = w[2];
