@@ -293,7 +293,7 @@ def adjoint_leg(name, cfg, masses, names, tab, nfd, repeats):
 
     wall, gpu_ms = {}, {}
     for label, fwd_mixed in (("fp64_forward", False), ("mixed_forward", True)):
-        for _ in range(3):
+        for _ in range(20):  # clocks, torch's allocator and the library's buffers settle
             dstep(fwd_mixed)
         torch.cuda.synchronize()
         t0 = time.perf_counter()  # wall time: no per-launch event timers in this loop
