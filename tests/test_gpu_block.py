@@ -39,10 +39,18 @@ def _case(name):
         return cfg, CUNI, ["Cu", "Ni"], cuni_tables()
     if name == "replicas":           # several boxes, different strains
         return inputs.strained_batch(5, (4, 4, 4), 3.615, [CU_MASS], seed=13), [CU_MASS], None, cu_tables(500)
+    if name == "cu_open_y":          # y not periodic: atoms on both y faces and beyond them
+        c = cu_config((6, 7, 5), jitter=0.1, seed=11)
+        r = c.r.copy()
+        by_y = np.argsort(r[:, 1])
+        r[by_y[:3], 1] -= np.array([0.2, 0.45, 0.7])          # below lo (the bottom plane is at 0)
+        r[by_y[-3:], 1] += 1.8 + np.array([0.15, 0.4, 0.65])  # above hi (the top plane is at hi - a/2)
+        assert (r[:, 1] < c.lo[0, 1]).sum() >= 3 and (r[:, 1] >= c.hi[0, 1]).sum() == 3
+        return inputs.Config(r, c.v, c.type, c.lo, c.hi, periodic=(1, 0, 1)), [CU_MASS], None, cu_tables()
     raise KeyError(name)
 
 
-CASES = ["cu_jitter", "cu_one_cell", "cu_two_cells", "cuni_slab", "replicas"]
+CASES = ["cu_jitter", "cu_one_cell", "cu_two_cells", "cuni_slab", "replicas", "cu_open_y"]
 
 
 def _sim(monkeypatch, kind, cfg, masses, names, tab):
@@ -90,9 +98,11 @@ def test_block_forces_match_global(monkeypatch, name, mixed):
         assert _close(x, y, 1e-6 if mixed else 1e-10)
 
 
-@pytest.mark.parametrize("name", ["cu_one_cell", "cu_two_cells"])
+@pytest.mark.parametrize("name", ["cu_one_cell", "cu_two_cells", "cu_open_y"])
 def test_block_small_boxes_vs_oracle(oracle_mod, monkeypatch, name):
-    """Boxes of one and two cells per axis against the oracle directly."""
+    """Boxes of one and two cells per axis, and a box open along y with atoms beyond
+    its faces (clamped into the edge cells, whose boxes are unbounded outward), against
+    the oracle directly."""
     cfg, masses, names, tab = _case(name)
     sim = _sim(monkeypatch, "block", cfg, masses, names, tab)
     _outputs(sim, tab.rc, False)
