@@ -177,10 +177,24 @@ def test_block_lj_forces_match_global(monkeypatch, mixed):
         assert _close(x, y, 1e-6 if mixed else 1e-10)
 
 
-def test_block_lj_vs_oracle(monkeypatch, oracle_mod):
-    """LJ on the block list against the oracle (FP64 1e-11 of rms|F|)."""
+def _lj_open_z_config():
+    """The jittered argon crystal, open along z, with atoms moved beyond both z faces."""
+    from exastamp_amd import inputs
+    c = _lj_jitter_config()
+    r = c.r.copy()
+    by_z = np.argsort(r[:, 2])
+    r[by_z[:2], 2] -= np.array([0.3, 0.9])
+    r[by_z[-2:], 2] += 2.63 + np.array([0.2, 0.7])  # the top plane is at hi - a/2
+    assert (r[:, 2] >= c.hi[0, 2]).sum() == 2
+    return inputs.Config(r, c.v, c.type, c.lo, c.hi, periodic=(1, 1, 0))
+
+
+@pytest.mark.parametrize("open_z", [False, True])
+def test_block_lj_vs_oracle(monkeypatch, oracle_mod, open_z):
+    """LJ on the block list against the oracle (FP64 1e-11 of rms|F|), periodic and
+    open along z with atoms beyond the faces."""
     eps, sig, rc = lj_params()
-    cfg = _lj_jitter_config()
+    cfg = _lj_open_z_config() if open_z else _lj_jitter_config()
     sim = _lj_sim(monkeypatch, "block", cfg)
     sim.chunk_neighbors(rcut_max=rc, rcut_inc=0.3)
     assert sim.list_info()["kind"] == "block"
@@ -261,7 +275,30 @@ def test_block_md_run_vs_oracle_energy(monkeypatch, oracle_mod):
     assert np.abs(pe - th_ref[:, 24]).max() <= 1e-6 * np.abs(th_ref[:, 24]).max()
 
 
-@pytest.mark.parametrize("name", ["cu_jitter", "cuni_slab", "replicas"])
+def test_block_md_run_open_box_vs_oracle(monkeypatch, oracle_mod):
+    """60 FP64 NVE steps at 600 K in the box open along y, some of whose face atoms start
+    beyond the faces (clamped into the edge cells at every rebuild; seven rebuilds):
+    equal rebuild counts, per-step potential energy within 1e-9 and positions within
+    1e-8 A of the oracle's trajectory."""
+    _env(monkeypatch)
+    cfg, masses, names, tab = _case("cu_open_y")
+    sim = make_sim(cfg, masses)
+    sim.eam_alloy_init(tab)
+    st, th = sim.md_run("eam", 60, 0.001, tab.rc, 0.3, thermo_every=1)
+    assert sim.list_info()["kind"] == "block"
+    o = make_oracle(oracle_mod, cfg, masses)
+    o.set_eam(tab)
+    nreb, th_ref = o.md_run("eam", 60, 0.001, tab.rc, 0.3)
+    assert st.rebuilds == nreb and nreb >= 1
+    pe = np.array([t.potential_energy for t in th])
+    assert np.abs(pe - th_ref[:, 24]).max() <= 1e-9 * np.abs(th_ref[:, 24]).max()
+    r, _ = sim.state()
+    ro = o.state()["r"]
+    assert np.abs(r - ro).max() < 1e-8
+    assert (ro[:, 1] < cfg.lo[0, 1]).sum() >= 3 and (ro[:, 1] >= cfg.hi[0, 1]).sum() >= 1
+
+
+@pytest.mark.parametrize("name", ["cu_jitter", "cuni_slab", "replicas", "cu_open_y"])
 def test_distance_sorted_lists(monkeypatch, name):
     """Block lists are reordered by distance after the build (XSMD_LIST_SORT=0 keeps
     the stencil order): the converted global list is the same entry by entry, and
