@@ -49,10 +49,23 @@ def _eam_forces(sim, tab, mixed=False):
     sim.force_to_accel()
 
 
+def _open_z(cfg):
+    """cfg open along z, with two atoms moved below the bottom face and two above the top."""
+    from exastamp_amd import inputs
+    r = cfg.r.copy()
+    by_z = np.argsort(r[:, 2])
+    r[by_z[:2], 2] -= np.array([0.3, 0.6])
+    r[by_z[-2:], 2] += 1.8 + np.array([0.2, 0.5])  # the top plane is at hi - a/2
+    return inputs.Config(r, cfg.v, cfg.type, cfg.lo, cfg.hi, periodic=(1, 1, 0))
+
+
+@pytest.mark.parametrize("open_z", [False, True])
 @pytest.mark.parametrize("kind", ["block", "global"])
-def test_sheared_eam_matches_oracle(oracle_mod, monkeypatch, kind):
+def test_sheared_eam_matches_oracle(oracle_mod, monkeypatch, kind, open_z):
     tab = cu_tables()
     cfg = cu_config((9, 8, 7), T=600.0, jitter=0.1, seed=21)
+    if open_z:
+        cfg = _open_z(cfg)
     _env(monkeypatch, kind)
     sim = _eam_sim(cfg, tab, SHEAR)
     assert sim.list_info()["kind"] == kind
