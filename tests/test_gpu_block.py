@@ -213,6 +213,25 @@ def test_block_lj_vs_oracle(monkeypatch, oracle_mod, open_z):
     assert np.abs(vir - vref).max() <= 1e-12 * np.abs(vref).max()
 
 
+def test_block_lj_md_run_open_z_vs_oracle(monkeypatch, oracle_mod):
+    """60 FP64 LJ NVE steps (dt 4 fs) in the argon crystal open along z with atoms
+    beyond both faces: equal rebuild counts (eight), per-step potential energy within
+    1e-9 and positions within 1e-8 A of the oracle's trajectory."""
+    eps, sig, rc = lj_params()
+    cfg = _lj_open_z_config()
+    sim = _lj_sim(monkeypatch, "block", cfg)
+    st, th = sim.md_run("lj", 60, 0.004, rc, 0.3, thermo_every=1)
+    assert sim.list_info()["kind"] == "block"
+    o = make_oracle(oracle_mod, cfg, [AR_MASS])
+    o.set_lj(eps, sig, rc)
+    nreb, th_ref = o.md_run("lj", 60, 0.004, rc, 0.3)
+    assert st.rebuilds == nreb and nreb >= 1
+    pe = np.array([t.potential_energy for t in th])
+    assert np.abs(pe - th_ref[:, 24]).max() <= 1e-9 * np.abs(th_ref[:, 24]).max()
+    r, _ = sim.state()
+    assert np.abs(r - o.state()["r"]).max() < 1e-8
+
+
 @pytest.mark.parametrize("mixed", [False, True])
 def test_block_lj_md_run_matches_global(monkeypatch, mixed):
     """40 fused LJ NVE steps (drift epilogue on the block list) against the global list."""
